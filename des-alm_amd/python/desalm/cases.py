@@ -1,0 +1,165 @@
+"""Synthetic workloads: grids, wall distance and initial fields.
+
+Arrays are numpy, flat over the fine level with the reference's node order
+(i fastest, then j, then k; src/init.F90:880-980).  Vector fields are
+(n1, m) C-ordered, i.e. the reference's q(1:m, l) layout.
+
+Workloads (constants not stated by the workload definition take the shipped
+deck's values, ind3dmg.dat:8-16):
+  cavity   33^3 lid-driven cavity, Re 1000, steady, laminar, 2 levels
+  channel  129^3 SA channel, Re_tau 395, log-law start + 5% seeded noise,
+           3 levels, BDF2 dual time
+"""
+import math
+
+import numpy as np
+
+from .params import make_params, BC_WALL, BC_INTERFACE, BC_PERIODIC
+
+
+class Case:
+    """A complete input set for one run of the inner iteration."""
+
+    def __init__(self, name, params, x, y, z, wd, q, qn, qnm1, xnut):
+        self.name = name
+        self.params = params
+        self.x, self.y, self.z = x, y, z
+        self.wd = wd
+        self.q, self.qn, self.qnm1, self.xnut = q, qn, qnm1, xnut
+
+    @property
+    def n1(self):
+        return self.params.img * self.params.jmg * self.params.kmg
+
+    def state(self):
+        return self.q.copy(), self.qn.copy(), self.qnm1.copy(), self.xnut.copy()
+
+
+def tanh_stretch(n, delta):
+    """x = 1/2 [1 + tanh(delta (2 s - 1)) / tanh(delta)], s uniform on [0,1]."""
+    s = np.linspace(0.0, 1.0, n)
+    return 0.5 * (1.0 + np.tanh(delta * (2.0 * s - 1.0)) / math.tanh(delta))
+
+
+def mesh(xi, yj, zk):
+    """Tensor-product coordinates flattened in i-fastest order."""
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+    return X.ravel().copy(), Y.ravel().copy(), Z.ravel().copy()
+
+
+def fv1(nut, ren):
+    chi = nut * ren
+    chic = chi ** 3
+    return chic / (chic + 357.911)
+
+
+def cavity(n=33, ng=2, itm=200, stretch=1.1, ren=1000.0, **kw):
+    """Plain FP64 laminar lid-driven cavity (workload 1)."""
+    s = tanh_stretch(n, stretch)
+    x, y, z = mesh(s, s, s)
+    bt = [BC_WALL] * 6
+    bt[3] = BC_INTERFACE  # lid j = jmax: nodes left untouched, u = 1 preserved
+    p = make_params(n, n, n, ng=ng, itm=itm, ren=ren, beta=1.0, unsteady=False, des=False,
+                    btype=bt, fix=(2, 2, 2), **kw)
+    n1 = n * n * n
+    q = np.zeros((n1, 5))
+    j = (np.arange(n1) // n) % n
+    q[j == n - 1, 1] = 1.0
+    zero1 = np.zeros(n1)
+    return Case("cavity", p, x, y, z, zero1.copy(), q, q.copy(), q.copy(), zero1.copy())
+
+
+def channel_y(nj, dy1_wall):
+    """Wall-normal tanh-stretched coordinate on [-1, 1] whose first spacing
+    at each wall is dy1_wall (solved for the stretching factor)."""
+    s = np.linspace(0.0, 1.0, nj)
+
+    def first(g):
+        yy = np.tanh(g * (2.0 * s - 1.0)) / math.tanh(g)
+        return yy[1] - yy[0]
+
+    lo, hi = 1e-3, 10.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first(mid) > dy1_wall:
+            lo = mid
+        else:
+            hi = mid
+    g = 0.5 * (lo + hi)
+    return np.tanh(g * (2.0 * s - 1.0)) / math.tanh(g)
+
+
+def loglaw(yplus):
+    return np.where(yplus < 11.0, yplus, np.log(np.maximum(yplus, 1e-30)) / 0.41 + 5.2)
+
+
+def channel(ni=129, nj=129, nk=129, ng=3, itm=30, ren=395.0, delti=0.005, seed=20260101,
+            noise=0.05, des=True, **kw):
+    """Turbulent channel, SA (workload 2).  Streamwise/spanwise faces use the
+    reference's btype 6; the walls are at j = 1 and j = jmax."""
+    xi = np.linspace(0.0, 2.0 * math.pi, ni)
+    yj = channel_y(nj, 2.0 / ren if nj < 65 else 1.0 / ren)
+    zk = np.linspace(0.0, math.pi, nk)
+    x, y, z = mesh(xi, yj, zk)
+    n1 = ni * nj * nk
+    wd = 1.0 - np.abs(y)
+    yplus = wd * ren
+    u = loglaw(yplus)
+    uc = float(loglaw(np.array([ren]))[0])
+    rng = np.random.default_rng(seed)
+    q = np.zeros((n1, 5))
+    q[:, 1] = u + noise * uc * rng.uniform(-1.0, 1.0, n1)
+    q[:, 2] = noise * uc * rng.uniform(-1.0, 1.0, n1)
+    q[:, 3] = noise * uc * rng.uniform(-1.0, 1.0, n1)
+    wall = (wd == 0.0)
+    q[wall, 1:4] = 0.0
+    # nu~ from a mixing-length estimate, clipped to the SA floor of 1e-7
+    nut = np.maximum(0.41 * wd * np.minimum(yplus, 200.0) / ren * (1.0 - np.exp(-yplus / 26.0)), 1.0e-7)
+    q[:, 4] = nut
+    xnut = nut * fv1(nut, ren)
+    bt = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+    p = make_params(ni, nj, nk, ng=ng, itm=itm, ren=ren, beta=1.0, delti=delti, unsteady=True,
+                    des=des, btype=bt, fix=(2, nj // 2, 2), **kw)
+    return Case("channel", p, x, y, z, wd, q, q.copy(), q.copy(), xnut)
+
+
+def wavy(ni, nj, nk, amp=0.04, ng=2, seed=7, des=False, unsteady=True, btype=None, ren=500.0,
+         itm=3, delti=0.02, stretch=0.8, **kw):
+    """Curvilinear test block: a stretched box with smooth sinusoidal
+    distortion of every coordinate (non-orthogonal metrics, all cross terms
+    non-zero) and smooth random initial fields.  Used by the parity tests."""
+    si = tanh_stretch(ni, stretch)
+    sj = tanh_stretch(nj, stretch)
+    sk = tanh_stretch(nk, stretch)
+    X, Y, Z = mesh(si, sj, sk)
+    two_pi = 2.0 * math.pi
+    x = X + amp * np.sin(two_pi * Y) * np.sin(math.pi * Z)
+    y = Y + amp * np.sin(two_pi * Z) * np.sin(math.pi * X)
+    z = Z + amp * np.sin(two_pi * X) * np.sin(math.pi * Y)
+    n1 = ni * nj * nk
+    rng = np.random.default_rng(seed)
+
+    def smooth(scale):
+        a = rng.uniform(-1, 1, 6)
+        return scale * (a[0] * np.sin(3.1 * X + a[1]) * np.cos(2.3 * Y + a[2]) +
+                        a[3] * np.sin(1.7 * Z + a[4]) + 0.3 * a[5] * X * Y)
+
+    q = np.zeros((n1, 5))
+    q[:, 0] = smooth(0.3)
+    q[:, 1] = 1.0 + smooth(0.5)
+    q[:, 2] = smooth(0.4)
+    q[:, 3] = smooth(0.4)
+    wd = np.minimum(np.minimum(Y, 1.0 - Y), np.minimum(Z, 1.0 - Z)) + 1e-3
+    q[:, 4] = (2.0 + smooth(1.0) + 1.5) / ren
+    qn = q.copy()
+    qnm1 = q.copy()
+    qn[:, :4] += 0.01 * rng.standard_normal((n1, 4))
+    qnm1[:, :4] += 0.02 * rng.standard_normal((n1, 4))
+    qn[:, 4] *= 1.0 + 0.01 * rng.standard_normal(n1)
+    qnm1[:, 4] *= 1.0 + 0.02 * rng.standard_normal(n1)
+    xnut = q[:, 4] * fv1(q[:, 4], ren)
+    if btype is None:
+        btype = (BC_WALL, 3, BC_WALL, BC_INTERFACE, 2, 3)
+    p = make_params(ni, nj, nk, ng=ng, itm=itm, ren=ren, delti=delti, unsteady=unsteady, des=des,
+                    btype=btype, fix=(3, 2, 4), **kw)
+    return Case("wavy", p, x, y, z, wd, q, qn, qnm1, xnut)

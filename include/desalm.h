@@ -1,0 +1,181 @@
+/*
+ * desalm.h -- C ABI of libdesalm.so, the MI355X (gfx950) implementation of
+ * the DES-ALM pseudo-time inner iteration: residual assembly, DAF implicit
+ * operator with i/j/k tridiagonal line sweeps, SA/DES eddy-viscosity update
+ * and the FAS multigrid V-cycle.
+ *
+ * Plain pointers and sizes only.  All floating-point arrays are double
+ * precision and use the reference's own array-of-structures layout, i.e. a
+ * field q(1:m, l) is m consecutive doubles per node and the node index l runs
+ * i fastest, then j, then k, then multigrid level (src/init.F90:880-980 of
+ * the reference, "le_idx" numbering on a single rank without ghost layers).
+ * The library converts to its device structure-of-arrays layout internally.
+ *
+ * Return codes: 0 = success, < 0 = argument or HIP error (text available from
+ * desalm_last_error), > 0 = LAPACK-INFO-like zero pivot reported by a line
+ * solve (1-based row of the first singular row).
+ *
+ * One context drives one HIP device on one stream; a context is not
+ * re-entrant (the reference runs one MPI rank per context).
+ */
+#ifndef DESALM_H
+#define DESALM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DESALM_MAX_LEVELS 6
+
+/* Arithmetic used by the DAF factor and its line solves.  The residual,
+ * metrics and multigrid restriction are always FP64. */
+#define DESALM_PREC_FP64 0   /* plain FP64 everywhere */
+#define DESALM_PREC_MIXED 1  /* FP32 DAF factor + sweeps as a defect correction */
+
+/* Boundary types, as the reference's btype (ind3dmg.dat notes; src/bcond_fm.F90). */
+#define DESALM_BC_INTERFACE 0
+#define DESALM_BC_WALL 1
+#define DESALM_BC_SYMMETRY 2
+#define DESALM_BC_FREESTREAM 3
+#define DESALM_BC_INFLOW 4
+#define DESALM_BC_OUTLET_MOC 5   /* not supported: rejected by desalm_init */
+#define DESALM_BC_PERIODIC 6
+#define DESALM_BC_FREE_SURFACE 8
+
+/*
+ * Run parameters.  Mirrors the module state the reference reads inside
+ * mg_driver (src/global.F90, src/global_param.F90, src/global_app.F90) as it
+ * is filled by init_input_file (src/init.F90:176-228) and init_daf_bc
+ * (src/init.F90:486-508) for a single zone on a single rank.
+ * Indices in this struct (iter[], btype[], ep[][]) are 0-based versions of
+ * the reference's 1-based arrays.
+ */
+typedef struct desalm_params {
+    int32_t ng;                          /* multigrid levels (global.F90:17) */
+    int32_t img, jmg, kmg;               /* fine-level node counts */
+    int32_t icrs, jcrs, kcrs;            /* 1 = full coarsening in that direction */
+    int32_t iter[DESALM_MAX_LEVELS];     /* smoother passes per level, iter(n) */
+    int32_t itm;                         /* max pseudo-iterations, itm(1) */
+    int32_t it_min;                      /* no convergence exit while itc <= it_min */
+    double er_min;                       /* exit when max drop of log10 u,v,w norms < er_min */
+    double eo_min;                       /* exit when max log10 u,v,w norm < eo_min */
+    double ren;                          /* Reynolds number */
+    double beta;                         /* artificial compressibility */
+    double delti;                        /* physical time step */
+    int32_t unsteady;                    /* 1: BDF2 term on (e_source = 1) */
+    int32_t des;                         /* 1: SA/DES update of nu~ on the fine level */
+    int32_t itk;                         /* SA sub-iterations per smoother pass */
+    int32_t precision;                   /* DESALM_PREC_* */
+    double cfl1, vnn1;                   /* DAF local time step limits */
+    double cfl2, vnn2;                   /* SA local time step limits (dtev) */
+    double pdiss_coef;                   /* pressure dissipation coefficient */
+    double ep[2][3];                     /* ep(1,d): DAF implicit smoothing, ep(2,d): prolongation smoothing */
+    int32_t btype[6];                    /* faces: i=1, i=max, j=1, j=max, k=1, k=max */
+    int32_t bdir[6];                     /* symmetry-plane normal direction (1,2,3) */
+    int32_t ifix, jfix, kfix;            /* 1-based node whose pressure is pinned to 0 */
+    int32_t reserved[8];
+} desalm_params;
+
+typedef struct desalm_ctx desalm_ctx;
+
+/* Allocate device state for all levels (replaces the module setup done by
+ * init_mg, src/init.F90:1025-1124, and init_daf_bc, src/init.F90:439-511).
+ * device: HIP device ordinal. */
+int desalm_init(const desalm_params* p, int device, desalm_ctx** ctx);
+
+/* Total node count over all levels and the offset (in nodes) of level n
+ * (1-based), i.e. le_idx_b(ng) and le_idx_a(n)-1 of the reference. */
+int64_t desalm_nodes_all(const desalm_ctx* ctx);
+int64_t desalm_level_offset(const desalm_ctx* ctx, int n);
+
+/* Fine-level grid coordinates x,y,z (img*jmg*kmg each).  Injects them to
+ * the coarse levels (src/mg_inject.F90:2-64 as called from init) and
+ * computes csi, eta, zet, aj on every level and dtev on the fine level
+ * (src/mg_metrics.F90:56-300) on the device. */
+int desalm_set_grid(desalm_ctx* ctx, const double* x, const double* y, const double* z);
+
+/* Alternative to desalm_set_grid: metrics of all levels in the reference
+ * layout csi(1:3, nall), eta(1:3, nall), zet(1:3, nall), aj(nall). */
+int desalm_set_metrics(desalm_ctx* ctx, const double* csi, const double* eta,
+                       const double* zet, const double* aj);
+
+/* Wall distance of the fine level (img*jmg*kmg), pre-scaled for DES as the
+ * reference reads it from its grid file (src/init.F90:2313).  dtev may be
+ * NULL (keep the value computed by desalm_set_grid) or a fine-level array. */
+int desalm_set_wall_distance(desalm_ctx* ctx, const double* wd, const double* dtev);
+
+/* Fine-level solution in the reference layout: q(1:5, n1), qn(1:5, n1),
+ * qnm1(1:5, n1), xnut(n1), n1 = img*jmg*kmg.  Coarse levels are produced by
+ * the V-cycle's own injections.  Any pointer may be NULL to leave that
+ * field unchanged (get) or zero (set). */
+int desalm_set_state(desalm_ctx* ctx, const double* q, const double* qn,
+                     const double* qnm1, const double* xnut);
+int desalm_get_state(desalm_ctx* ctx, double* q, double* qn, double* qnm1, double* xnut);
+
+/*
+ * One physical step's pseudo-time inner iteration: replaces
+ * `subroutine mg_driver` (src/mg_driver.F90:2-323).  Shifts qnm1 <- qn <- q,
+ * runs up to itm FAS V-cycles of DAF smoothing (SA/DES update on the fine
+ * level when des = 1), applies the fine-grid BCs and stops on the
+ * reference's convergence test (mg_ck_conver, src/mg_driver.F90:335-405).
+ * itc_out: pseudo-iterations done.  resid_out (may be NULL): itm*5 doubles,
+ * row it holds the log10 L1 update norms of mg_residual
+ * (src/mg_residual.F90:2-104) after iteration it.
+ * State stays on the device; use desalm_get_state to read it back.
+ */
+int desalm_mg_driver(desalm_ctx* ctx, int* itc_out, double* resid_out);
+
+/*
+ * One DAF smoother pass on level n (1-based) with host arrays of that level:
+ * replaces `subroutine solver_daf` (src/solver_daf.F90:2-16,80-89).
+ * q, qn, qnm1, pk, rh: (1:4, nl) where nl = nodes of level n; csi, eta, zet:
+ * (1:3, nl); aj, xnut: (nl).  q, pk and rh are updated in place exactly as
+ * the reference's intent(inout/out) dummies.  Copies in and out per call:
+ * this is the parity entry, not the production path.
+ */
+int desalm_solver_daf(desalm_ctx* ctx, int n, int decide_recalc_rh, int decide_calc_pk,
+                      double* q, const double* qn, const double* qnm1,
+                      const double* csi, const double* eta, const double* zet,
+                      const double* aj, const double* xnut, double* pk, double* rh);
+
+/*
+ * SA/DES eddy-viscosity update on the fine level with host arrays: replaces
+ * `subroutine des_eddy` (src/des_eddy.F90:2-14).  q: (1:5, n1) updated in
+ * place (component 5 = nu~); qn5, qnm1_5: nu~ of the two previous physical
+ * steps (n1); csi, eta, zet (1:3, n1); aj, wd, dtev (n1); xnut (n1) output.
+ */
+int desalm_des_eddy(desalm_ctx* ctx, double* q, const double* qn5, const double* qnm1_5,
+                    const double* csi, const double* eta, const double* zet,
+                    const double* aj, const double* wd, const double* dtev, double* xnut);
+
+/*
+ * Batched tridiagonal solves with the reference's LAPACK SGTSV semantics
+ * (src/sgtsv.f:1-263, Gaussian elimination with partial pivoting) in FP64 on
+ * the device, one system per column: nsys systems of size n stored
+ * column-major (entry r of system s at [s*n + r]); dl and du hold n-1 used
+ * entries per system at [s*n + 0 .. n-2].  All arrays are host arrays and
+ * are overwritten (b with the solution).  info_out (may be NULL): nsys ints,
+ * the SGTSV INFO per system.
+ */
+int desalm_gtsv_batch(int n, int nsys, double* dl, double* d, double* du, double* b,
+                      int* info_out);
+
+/* Read a device field of level n back in the reference layout.
+ * which: 0 q(5), 1 qn(5), 2 qnm1(5), 3 qold_mg(5), 4 rh(4), 5 pk(4),
+ * 6 xnut(1), 7 csi(3), 8 eta(3), 9 zet(3), 10 aj(1), 11 dtev(1), 12 wd(1).
+ * out must hold ncomp * nodes(level n) doubles. */
+int desalm_get_level_field(desalm_ctx* ctx, int which, int n, double* out);
+
+/* Device timing of the last desalm_mg_driver call (milliseconds, from HIP
+ * events on the context's stream). */
+double desalm_last_driver_ms(const desalm_ctx* ctx);
+
+const char* desalm_last_error(const desalm_ctx* ctx);
+int desalm_finalize(desalm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DESALM_H */

@@ -1,0 +1,1982 @@
+/*
+ * desalm_oracle.c -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * CPU FP64 restatement of the reference's pseudo-time inner iteration
+ * (jcsandov/DES-ALM, Fortran 90).  It is the checker of the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * Loops, operand order and quirks follow the reference routine cited above
+ * each function; single zone, single rank (every MPI neighbour is
+ * MPI_PROC_NULL, no ghost layers, src/init.F90:575,671-680), no blanking
+ * boxes (nblk = 0), case hooks (wf_eta, conservation, ALM forcing, hub
+ * zeroing) off, characteristic outlet (btype 5) unsupported.
+ *
+ * Pinning of each part (what checks it beyond this restatement) is listed in
+ * the tests that import it: the reference Fortran compiled from its own
+ * sources (oracle/ref, all composed routines), LAPACK dgtsv via scipy
+ * (or_gtsv), closed forms and invariants (metrics, residual terms, transfer
+ * operators).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "desalm_oracle.h"
+
+#define NGM (DESALM_MAX_LEVELS + 1)
+
+struct or_state {
+    desalm_params p;
+    int ng;
+    int img[NGM], jmg[NGM], kmg[NGM];
+    long off[NGM];
+    long nl[NGM];
+    long nall;
+    double dc[NGM], de[NGM], dz[NGM];
+    double e_source;
+    double cvin;
+    double sa[6][5], sb[6][5]; /* sa(m,b) -> sa[b-1][m-1] */
+    double *x, *y, *z;
+    double *xc, *xe, *xz, *yc, *ye, *yz, *zc, *ze, *zz;
+    double *csi, *eta, *zet, *aj;
+    double *q, *qn, *qnm1, *qold_mg, *rh, *pk;
+    double *xnut, *wd, *dtev;
+    double init_resd[5];
+    int itc;
+};
+
+/* constants of src/precision.F90 */
+static const double one_half = 0.5, pt5 = 0.5, pt25 = 0.25, one_fourth = 0.25;
+static const double onept5 = 1.5, one_pt_five = 1.5;
+
+#define LEVEL(n)                                   \
+    const int ni = o->img[n], nj = o->jmg[n], nk = o->kmg[n]; \
+    const long off = o->off[n];                    \
+    (void)off
+#define IX(i, j, k) ((long)((i)-1) + (long)ni * ((long)((j)-1) + (long)nj * (long)((k)-1)))
+
+static double dmin2(double a, double b) { return a < b ? a : b; }
+static double dmax2(double a, double b) { return a > b ? a : b; }
+
+/* ------------------------------------------------------------------ */
+/* LAPACK SGTSV (vendored LAPACK 3.0, src/sgtsv.f:89-259), FP64.        */
+/* dl(1..n-1), d(1..n), du(1..n-1), b(ldb, nrhs); 0-based storage.     */
+/* ------------------------------------------------------------------ */
+int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb) {
+    int i, jr;
+    double fact, temp;
+    if (n < 0) return -1;
+    if (nrhs < 0) return -2;
+    if (ldb < (n > 1 ? n : 1)) return -7;
+    if (n == 0) return 0;
+    /* forward elimination with row interchanges (sgtsv.f:108-192) */
+    for (i = 0; i < n - 2; i++) {
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] != 0.0) {
+                fact = dl[i] / d[i];
+                d[i + 1] = d[i + 1] - fact * du[i];
+                for (jr = 0; jr < nrhs; jr++) b[i + 1 + jr * ldb] = b[i + 1 + jr * ldb] - fact * b[i + jr * ldb];
+            } else {
+                return i + 1;
+            }
+            dl[i] = 0.0;
+        } else {
+            fact = d[i] / dl[i];
+            d[i] = dl[i];
+            temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            dl[i] = du[i + 1];
+            du[i + 1] = -fact * dl[i];
+            du[i] = temp;
+            for (jr = 0; jr < nrhs; jr++) {
+                temp = b[i + jr * ldb];
+                b[i + jr * ldb] = b[i + 1 + jr * ldb];
+                b[i + 1 + jr * ldb] = temp - fact * b[i + 1 + jr * ldb];
+            }
+        }
+    }
+    if (n > 1) {
+        i = n - 2;
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] != 0.0) {
+                fact = dl[i] / d[i];
+                d[i + 1] = d[i + 1] - fact * du[i];
+                for (jr = 0; jr < nrhs; jr++) b[i + 1 + jr * ldb] = b[i + 1 + jr * ldb] - fact * b[i + jr * ldb];
+            } else {
+                return i + 1;
+            }
+        } else {
+            fact = d[i] / dl[i];
+            d[i] = dl[i];
+            temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            du[i] = temp;
+            for (jr = 0; jr < nrhs; jr++) {
+                temp = b[i + jr * ldb];
+                b[i + jr * ldb] = b[i + 1 + jr * ldb];
+                b[i + 1 + jr * ldb] = temp - fact * b[i + 1 + jr * ldb];
+            }
+        }
+    }
+    if (d[n - 1] == 0.0) return n;
+    /* back substitution (sgtsv.f:232-257) */
+    for (jr = 0; jr < nrhs; jr++) {
+        double* bb = b + jr * ldb;
+        bb[n - 1] = bb[n - 1] / d[n - 1];
+        if (n > 1) bb[n - 2] = (bb[n - 2] - du[n - 2] * bb[n - 1]) / d[n - 2];
+        for (i = n - 3; i >= 0; i--) bb[i] = (bb[i] - du[i] * bb[i + 1] - dl[i] * bb[i + 2]) / d[i];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* setup                                                               */
+/* ------------------------------------------------------------------ */
+static double* zalloc(long n) {
+    double* p = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    return p;
+}
+
+or_state* or_create(const desalm_params* p) {
+    int n, b;
+    or_state* o;
+    if (!p || p->ng < 1 || p->ng > DESALM_MAX_LEVELS) return NULL;
+    for (b = 0; b < 6; b++)
+        if (p->btype[b] == DESALM_BC_OUTLET_MOC) return NULL;
+    o = (or_state*)calloc(1, sizeof(or_state));
+    o->p = *p;
+    o->ng = p->ng;
+    /* level sizes, init_input_file (src/init.F90:254-260) */
+    o->img[1] = p->img;
+    o->jmg[1] = p->jmg;
+    o->kmg[1] = p->kmg;
+    for (n = 2; n <= o->ng; n++) {
+        o->img[n] = (o->img[n - 1] + p->icrs) / (1 << p->icrs);
+        o->jmg[n] = (o->jmg[n - 1] + p->jcrs) / (1 << p->jcrs);
+        o->kmg[n] = (o->kmg[n - 1] + p->kcrs) / (1 << p->kcrs);
+    }
+    o->nall = 0;
+    for (n = 1; n <= o->ng; n++) {
+        o->off[n] = o->nall;
+        o->nl[n] = (long)o->img[n] * o->jmg[n] * o->kmg[n];
+        o->nall += o->nl[n];
+        /* init_mg (src/init.F90:1116-1120) */
+        o->dc[n] = 1.0 / pow(2.0, (double)((n - 1) * p->icrs));
+        o->de[n] = 1.0 / pow(2.0, (double)((n - 1) * p->jcrs));
+        o->dz[n] = 1.0 / pow(2.0, (double)((n - 1) * p->kcrs));
+    }
+    o->e_source = p->unsteady ? 1.0 : 0.0;
+    o->cvin = 0.1 / p->ren;
+    /* init_daf_bc (src/init.F90:486-508) */
+    for (b = 0; b < 6; b++) {
+        int m;
+        for (m = 0; m < 5; m++) {
+            o->sa[b][m] = 4.0 / 3.0;
+            o->sb[b][m] = -(1.0 / 3.0);
+        }
+        if (p->btype[b] == 1)
+            for (m = 1; m < 4; m++) o->sa[b][m] = o->sb[b][m] = 0.0;
+        if (p->btype[b] == 2 && p->bdir[b] >= 1 && p->bdir[b] <= 3)
+            o->sa[b][p->bdir[b]] = o->sb[b][p->bdir[b]] = 0.0;
+    }
+    o->x = zalloc(o->nall); o->y = zalloc(o->nall); o->z = zalloc(o->nall);
+    o->xc = zalloc(o->nall); o->xe = zalloc(o->nall); o->xz = zalloc(o->nall);
+    o->yc = zalloc(o->nall); o->ye = zalloc(o->nall); o->yz = zalloc(o->nall);
+    o->zc = zalloc(o->nall); o->ze = zalloc(o->nall); o->zz = zalloc(o->nall);
+    o->csi = zalloc(3 * o->nall); o->eta = zalloc(3 * o->nall); o->zet = zalloc(3 * o->nall);
+    o->aj = zalloc(o->nall);
+    o->q = zalloc(5 * o->nall); o->qn = zalloc(5 * o->nall); o->qnm1 = zalloc(5 * o->nall);
+    o->qold_mg = zalloc(5 * o->nall);
+    o->rh = zalloc(4 * o->nall); o->pk = zalloc(4 * o->nall);
+    o->xnut = zalloc(o->nall); o->wd = zalloc(o->nall); o->dtev = zalloc(o->nall);
+    return o;
+}
+
+void or_destroy(or_state* o) {
+    if (!o) return;
+    free(o->x); free(o->y); free(o->z);
+    free(o->xc); free(o->xe); free(o->xz); free(o->yc); free(o->ye); free(o->yz);
+    free(o->zc); free(o->ze); free(o->zz);
+    free(o->csi); free(o->eta); free(o->zet); free(o->aj);
+    free(o->q); free(o->qn); free(o->qnm1); free(o->qold_mg); free(o->rh); free(o->pk);
+    free(o->xnut); free(o->wd); free(o->dtev);
+    free(o);
+}
+
+long or_nall(const or_state* o) { return o->nall; }
+long or_level_offset(const or_state* o, int n) { return (n >= 1 && n <= o->ng) ? o->off[n] : -1; }
+
+/* ------------------------------------------------------------------ */
+/* mg_inject (src/mg_inject.F90:207-269): scalar field fine -> coarse  */
+/* levels, all nodes, injection at ii = 2^icrs*i - icrs.               */
+/* ------------------------------------------------------------------ */
+static void mg_inject(const or_state* o, double* dum, int stride, int comp) {
+    int n, i, j, k;
+    for (n = 2; n <= o->ng; n++) {
+        const int ni = o->img[n], nj = o->jmg[n];
+        const int fi = o->img[n - 1], fj = o->jmg[n - 1];
+        for (k = 1; k <= o->kmg[n]; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    int kk = (1 << o->p.kcrs) * k - o->p.kcrs;
+                    int jj = (1 << o->p.jcrs) * j - o->p.jcrs;
+                    int ii = (1 << o->p.icrs) * i - o->p.icrs;
+                    long ll = o->off[n - 1] + (ii - 1) + (long)fi * ((jj - 1) + (long)fj * (kk - 1));
+                    long l = o->off[n] + (i - 1) + (long)ni * ((j - 1) + (long)nj * (k - 1));
+                    dum[l * stride + comp] = dum[ll * stride + comp];
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* mg_metrics (src/mg_metrics.F90:56-300)                               */
+/* ------------------------------------------------------------------ */
+static void mg_metrics(or_state* o) {
+    int n, i, j, k;
+    for (n = 1; n <= o->ng; n++) {
+        LEVEL(n);
+        const long ic = 1, ie = ni, iz = (long)ni * nj;
+        double dum1, dum;
+        double *x = o->x, *y = o->y, *z = o->z;
+        /* csi direction */
+        dum1 = o->dc[n];
+        dum = one_half * o->dc[n];
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 2; i <= ni - 1; i++) {
+                    long l = off + IX(i, j, k);
+                    o->xc[l] = dum * (x[l + ic] - x[l - ic]);
+                    o->yc[l] = dum * (y[l + ic] - y[l - ic]);
+                    o->zc[l] = dum * (z[l + ic] - z[l - ic]);
+                }
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++) {
+                long l = off + IX(1, j, k);
+                o->xc[l] = dum1 * (x[l + ic] - x[l]);
+                o->yc[l] = dum1 * (y[l + ic] - y[l]);
+                o->zc[l] = dum1 * (z[l + ic] - z[l]);
+                l = off + IX(ni, j, k);
+                o->xc[l] = dum1 * (x[l] - x[l - ic]);
+                o->yc[l] = dum1 * (y[l] - y[l - ic]);
+                o->zc[l] = dum1 * (z[l] - z[l - ic]);
+            }
+        /* eta direction */
+        dum1 = o->de[n];
+        dum = one_half * o->de[n];
+        for (k = 1; k <= nk; k++)
+            for (j = 2; j <= nj - 1; j++)
+                for (i = 1; i <= ni; i++) {
+                    long l = off + IX(i, j, k);
+                    o->xe[l] = dum * (x[l + ie] - x[l - ie]);
+                    o->ye[l] = dum * (y[l + ie] - y[l - ie]);
+                    o->ze[l] = dum * (z[l + ie] - z[l - ie]);
+                }
+        for (k = 1; k <= nk; k++)
+            for (i = 1; i <= ni; i++) {
+                long l = off + IX(i, 1, k);
+                o->xe[l] = dum1 * (x[l + ie] - x[l]);
+                o->ye[l] = dum1 * (y[l + ie] - y[l]);
+                o->ze[l] = dum1 * (z[l + ie] - z[l]);
+                l = off + IX(i, nj, k);
+                o->xe[l] = dum1 * (x[l] - x[l - ie]);
+                o->ye[l] = dum1 * (y[l] - y[l - ie]);
+                o->ze[l] = dum1 * (z[l] - z[l - ie]);
+            }
+        /* zet direction */
+        dum1 = o->dz[n];
+        dum = one_half * o->dz[n];
+        for (k = 2; k <= nk - 1; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    long l = off + IX(i, j, k);
+                    o->xz[l] = dum * (x[l + iz] - x[l - iz]);
+                    o->yz[l] = dum * (y[l + iz] - y[l - iz]);
+                    o->zz[l] = dum * (z[l + iz] - z[l - iz]);
+                }
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                long l = off + IX(i, j, 1);
+                o->xz[l] = dum1 * (x[l + iz] - x[l]);
+                o->yz[l] = dum1 * (y[l + iz] - y[l]);
+                o->zz[l] = dum1 * (z[l + iz] - z[l]);
+                l = off + IX(i, j, nk);
+                o->xz[l] = dum1 * (x[l] - x[l - iz]);
+                o->yz[l] = dum1 * (y[l] - y[l - iz]);
+                o->zz[l] = dum1 * (z[l] - z[l - iz]);
+            }
+        /* Jacobian and inverse transformation (mg_metrics.F90:223-249) */
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    long l = off + IX(i, j, k);
+                    double xc = o->xc[l], xe = o->xe[l], xz = o->xz[l];
+                    double yc = o->yc[l], ye = o->ye[l], yz = o->yz[l];
+                    double zc = o->zc[l], ze = o->ze[l], zz = o->zz[l];
+                    double gj = xc * (ye * zz - yz * ze) + xe * (yz * zc - yc * zz) + xz * (yc * ze - ye * zc);
+                    double a = 1.0 / gj;
+                    o->aj[l] = a;
+                    o->csi[3 * l + 0] = a * (ye * zz - yz * ze);
+                    o->csi[3 * l + 1] = a * (xz * ze - xe * zz);
+                    o->csi[3 * l + 2] = a * (xe * yz - xz * ye);
+                    o->eta[3 * l + 0] = a * (yz * zc - yc * zz);
+                    o->eta[3 * l + 1] = a * (xc * zz - xz * zc);
+                    o->eta[3 * l + 2] = a * (xz * yc - xc * yz);
+                    o->zet[3 * l + 0] = a * (yc * ze - ye * zc);
+                    o->zet[3 * l + 1] = a * (xe * zc - xc * ze);
+                    o->zet[3 * l + 2] = a * (xc * ye - xe * yc);
+                }
+    }
+    /* dtev on the start level (mg_metrics.F90:262-282) */
+    {
+        LEVEL(1);
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    long l = off + IX(i, j, k);
+                    double Sc = sqrt(o->xc[l] * o->xc[l] + o->yc[l] * o->yc[l] + o->zc[l] * o->zc[l]);
+                    double Se = sqrt(o->xe[l] * o->xe[l] + o->ye[l] * o->ye[l] + o->ze[l] * o->ze[l]);
+                    double Sz = sqrt(o->xz[l] * o->xz[l] + o->yz[l] * o->yz[l] + o->zz[l] * o->zz[l]);
+                    double SL = dmin2(dmin2(Sc, Se), Sz);
+                    double d = SL * dmin2(o->p.cfl2, o->p.vnn2 * o->p.ren * SL);
+                    if (o->p.unsteady) d = dmin2(d, o->p.delti * 2.0 / 3.0);
+                    o->dtev[l] = d;
+                }
+    }
+}
+
+int or_set_grid(or_state* o, const double* x, const double* y, const double* z) {
+    long n1 = o->nl[1];
+    memcpy(o->x, x, n1 * sizeof(double));
+    memcpy(o->y, y, n1 * sizeof(double));
+    memcpy(o->z, z, n1 * sizeof(double));
+    mg_inject(o, o->x, 1, 0);
+    mg_inject(o, o->y, 1, 0);
+    mg_inject(o, o->z, 1, 0);
+    mg_metrics(o);
+    return 0;
+}
+
+int or_set_wd(or_state* o, const double* wd) {
+    memcpy(o->wd, wd, o->nl[1] * sizeof(double));
+    return 0;
+}
+
+int or_set_state(or_state* o, const double* q, const double* qn, const double* qnm1, const double* xnut) {
+    long n1 = o->nl[1];
+    if (q) memcpy(o->q, q, 5 * n1 * sizeof(double));
+    if (qn) memcpy(o->qn, qn, 5 * n1 * sizeof(double));
+    if (qnm1) memcpy(o->qnm1, qnm1, 5 * n1 * sizeof(double));
+    if (xnut) memcpy(o->xnut, xnut, n1 * sizeof(double));
+    return 0;
+}
+
+int or_get_state(const or_state* o, double* q, double* qn, double* qnm1, double* xnut) {
+    long n1 = o->nl[1];
+    if (q) memcpy(q, o->q, 5 * n1 * sizeof(double));
+    if (qn) memcpy(qn, o->qn, 5 * n1 * sizeof(double));
+    if (qnm1) memcpy(qnm1, o->qnm1, 5 * n1 * sizeof(double));
+    if (xnut) memcpy(xnut, o->xnut, n1 * sizeof(double));
+    return 0;
+}
+
+static double* or_field(const or_state* o, const char* name, int* ncomp) {
+    struct { const char* nm; double* p; int c; } t[] = {
+        {"q", o->q, 5}, {"qn", o->qn, 5}, {"qnm1", o->qnm1, 5}, {"qold_mg", o->qold_mg, 5},
+        {"rh", o->rh, 4}, {"pk", o->pk, 4}, {"csi", o->csi, 3}, {"eta", o->eta, 3},
+        {"zet", o->zet, 3}, {"aj", o->aj, 1}, {"xnut", o->xnut, 1}, {"wd", o->wd, 1},
+        {"dtev", o->dtev, 1}, {"x", o->x, 1}, {"y", o->y, 1}, {"z", o->z, 1}};
+    size_t i;
+    for (i = 0; i < sizeof(t) / sizeof(t[0]); i++)
+        if (strcmp(name, t[i].nm) == 0) {
+            *ncomp = t[i].c;
+            return t[i].p;
+        }
+    return NULL;
+}
+
+int or_get_array(const or_state* o, const char* name, double* out) {
+    int c;
+    double* p = or_field(o, name, &c);
+    if (!p) return -1;
+    memcpy(out, p, (size_t)c * o->nall * sizeof(double));
+    return 0;
+}
+
+int or_set_array(or_state* o, const char* name, const double* in) {
+    int c;
+    double* p = or_field(o, name, &c);
+    if (!p) return -1;
+    memcpy(p, in, (size_t)c * o->nall * sizeof(double));
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* bcond_fm (src/bcond_fm.F90:2-322): fine-grid BCs on q(1:4).         */
+/* Its own index range includes the boundary nodes (bcond_fm.F90:60-76) */
+/* ------------------------------------------------------------------ */
+static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
+    LEVEL(1);
+    int i, j, k, m, b;
+    double pfix;
+#define QB(m, i, j, k) qv[5 * IX(i, j, k) + (m)-1]
+    /* i = 1 */
+    b = 1;
+    switch (o->p.btype[0]) {
+        case 1: case 2: case 3:
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++)
+                    for (m = 1; m <= 4; m++)
+                        QB(m, 1, j, k) = o->sa[b - 1][m - 1] * QB(m, 2, j, k) + o->sb[b - 1][m - 1] * QB(m, 3, j, k);
+            break;
+        default: break; /* 0 interface, 6 periodic (commented out, :105-110) */
+    }
+    /* i = imax */
+    b = 2;
+    switch (o->p.btype[1]) {
+        case 1: case 2: case 3:
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++)
+                    for (m = 1; m <= 4; m++)
+                        QB(m, ni, j, k) = o->sa[b - 1][m - 1] * QB(m, ni - 1, j, k) + o->sb[b - 1][m - 1] * QB(m, ni - 2, j, k);
+            break;
+        case 8: /* free surface (:160-166) */
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++)
+                    for (m = 1; m <= 4; m++) QB(m, ni, j, k) = QB(m, ni - 1, j, k);
+            break;
+        default: break;
+    }
+    /* j = 1 */
+    b = 3;
+    switch (o->p.btype[2]) {
+        case 1: case 2: case 3:
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++)
+                        QB(m, i, 1, k) = o->sa[b - 1][m - 1] * QB(m, i, 2, k) + o->sb[b - 1][m - 1] * QB(m, i, 3, k);
+            break;
+        case 6: /* plane copy from j = jmax (:197-202) */
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, 1, k) = QB(m, i, nj, k);
+            break;
+        default: break;
+    }
+    /* j = jmax */
+    b = 4;
+    switch (o->p.btype[3]) {
+        case 1: case 2: case 3:
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++)
+                        QB(m, i, nj, k) = o->sa[b - 1][m - 1] * QB(m, i, nj - 1, k) + o->sb[b - 1][m - 1] * QB(m, i, nj - 2, k);
+            break;
+        case 6:
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, nj, k) = QB(m, i, 1, k);
+            break;
+        default: break;
+    }
+    /* k = 1 */
+    b = 5;
+    switch (o->p.btype[4]) {
+        case 1: case 2: case 3:
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++)
+                        QB(m, i, j, 1) = o->sa[b - 1][m - 1] * QB(m, i, j, 2) + o->sb[b - 1][m - 1] * QB(m, i, j, 3);
+            break;
+        case 6:
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, j, 1) = QB(m, i, j, nk);
+            break;
+        default: break;
+    }
+    /* k = kmax */
+    b = 6;
+    switch (o->p.btype[5]) {
+        case 1: case 2: case 3:
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++)
+                        QB(m, i, j, nk) = o->sa[b - 1][m - 1] * QB(m, i, j, nk - 1) + o->sb[b - 1][m - 1] * QB(m, i, j, nk - 2);
+            break;
+        case 6:
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, j, nk) = QB(m, i, j, 1);
+            break;
+        default: break;
+    }
+    /* fixed pressure reference (:310-322) */
+    pfix = QB(1, o->p.ifix, o->p.jfix, o->p.kfix);
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) QB(1, i, j, k) = QB(1, i, j, k) - pfix;
+#undef QB
+}
+
+/* ------------------------------------------------------------------ */
+/* solver_daf and its internal procedures (src/solver_daf.F90)          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const or_state* o;
+    int n, ni, nj, nk;
+    double dc, de, dz;
+    int is, ie, js, je, ks, ke, idend;
+    double* q;          /* stride 5 */
+    const double *qn, *qnm1; /* stride 5 */
+    double *rh, *pk;    /* stride 4 */
+    const double *csi, *eta, *zet, *aj, *xnut;
+    double *qold_af, *diss, *visc, *ucn, *fv, *dtau, *mai, *n1i, *n2i, *mc, *spr;
+} daf_t;
+
+#define Q(m, i, j, k) d->q[5 * IX(i, j, k) + (m)-1]
+#define QN(m, i, j, k) d->qn[5 * IX(i, j, k) + (m)-1]
+#define QNM1(m, i, j, k) d->qnm1[5 * IX(i, j, k) + (m)-1]
+#define RH(m, i, j, k) d->rh[4 * IX(i, j, k) + (m)-1]
+#define PK(m, i, j, k) d->pk[4 * IX(i, j, k) + (m)-1]
+#define CSI(c, i, j, k) d->csi[3 * IX(i, j, k) + (c)-1]
+#define ETA(c, i, j, k) d->eta[3 * IX(i, j, k) + (c)-1]
+#define ZET(c, i, j, k) d->zet[3 * IX(i, j, k) + (c)-1]
+#define AJ(i, j, k) d->aj[IX(i, j, k)]
+#define XNUT(i, j, k) d->xnut[IX(i, j, k)]
+#define UCN(c, i, j, k) d->ucn[3 * IX(i, j, k) + (c)-1]
+#define FV(c, i, j, k) d->fv[3 * IX(i, j, k) + (c)-1]
+#define VISC(c, i, j, k) d->visc[3 * IX(i, j, k) + (c)-1]
+#define DISS(m, i, j, k) d->diss[4 * IX(i, j, k) + (m)-1]
+#define DTAU(i, j, k) d->dtau[IX(i, j, k)]
+#define SPR(c, i, j, k) d->spr[3 * IX(i, j, k) + (c)-1]
+#define MAT(a, r, c, i, j, k) d->a[16 * IX(i, j, k) + 4 * ((r)-1) + (c)-1]
+#define DLV const int ni = d->ni, nj = d->nj, nk = d->nk
+
+/* rhs_contra_j (src/rhs_contra_j.F90:2-31) */
+static void rhs_contra_j(daf_t* d) {
+    DLV;
+    int i, j, k;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double tmp = 1.0 / AJ(i, j, k);
+                UCN(1, i, j, k) = tmp * (CSI(1, i, j, k) * Q(2, i, j, k) + CSI(2, i, j, k) * Q(3, i, j, k) + CSI(3, i, j, k) * Q(4, i, j, k));
+                UCN(2, i, j, k) = tmp * (ETA(1, i, j, k) * Q(2, i, j, k) + ETA(2, i, j, k) * Q(3, i, j, k) + ETA(3, i, j, k) * Q(4, i, j, k));
+                UCN(3, i, j, k) = tmp * (ZET(1, i, j, k) * Q(2, i, j, k) + ZET(2, i, j, k) * Q(3, i, j, k) + ZET(3, i, j, k) * Q(4, i, j, k));
+            }
+}
+
+/* rhs_daf_dtau (src/rhs_daf_dtau.F90:2-100), the live local time step */
+static void rhs_daf_dtau(daf_t* d) {
+    DLV;
+    int i, j, k;
+    const double cfl = d->o->p.cfl1, vnn = d->o->p.vnn1, rei = 1.0 / d->o->p.ren;
+    const double dcsq = d->dc * d->dc, desq = d->de * d->de, dzsq = d->dz * d->dz;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double g1 = CSI(1, i, j, k) * CSI(1, i, j, k) + CSI(2, i, j, k) * CSI(2, i, j, k) + CSI(3, i, j, k) * CSI(3, i, j, k);
+                double g2 = ETA(1, i, j, k) * ETA(1, i, j, k) + ETA(2, i, j, k) * ETA(2, i, j, k) + ETA(3, i, j, k) * ETA(3, i, j, k);
+                double g3 = ZET(1, i, j, k) * ZET(1, i, j, k) + ZET(2, i, j, k) * ZET(2, i, j, k) + ZET(3, i, j, k) * ZET(3, i, j, k);
+                double e1 = sqrt(g1), e2 = sqrt(g2), e3 = sqrt(g3);
+                double dti1 = cfl / e1 / d->dc, dti2 = cfl / e2 / d->de, dti3 = cfl / e3 / d->dz;
+                double ret = rei + XNUT(i, j, k);
+                double dtv1 = vnn / ret / dcsq / g1, dtv2 = vnn / ret / desq / g2, dtv3 = vnn / ret / dzsq / g3;
+                double t = dmin2(dti1, dti2);
+                t = dmin2(t, dti3);
+                t = dmin2(t, dtv1);
+                t = dmin2(t, dtv2);
+                t = dmin2(t, dtv3);
+                DTAU(i, j, k) = t;
+            }
+}
+
+/* rhs_viscous (src/rhs_viscous.F90:2-333) */
+static void rhs_viscous(daf_t* d) {
+    DLV;
+    int i, j, k, c;
+    const double rei = 1.0 / d->o->p.ren;
+    const double dc = d->dc, de = d->de, dz = d->dz;
+    const double dc4 = one_fourth * dc, de4 = one_fourth * de, dz4 = one_fourth * dz;
+    long N = (long)ni * nj * nk;
+    double* ret = zalloc(N);
+    memset(d->fv, 0, 3 * N * sizeof(double));
+    memset(d->visc, 0, 3 * N * sizeof(double));
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) ret[IX(i, j, k)] = rei + XNUT(i, j, k);
+#define RET(i, j, k) ret[IX(i, j, k)]
+    /* csi faces */
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is - 1; i <= d->ie; i++) {
+                double ucs = dc * (Q(2, i + 1, j, k) - Q(2, i, j, k));
+                double vcs = dc * (Q(3, i + 1, j, k) - Q(3, i, j, k));
+                double wcs = dc * (Q(4, i + 1, j, k) - Q(4, i, j, k));
+                double uet = de4 * (Q(2, i + 1, j + 1, k) - Q(2, i + 1, j - 1, k) + Q(2, i, j + 1, k) - Q(2, i, j - 1, k));
+                double vet = de4 * (Q(3, i + 1, j + 1, k) - Q(3, i + 1, j - 1, k) + Q(3, i, j + 1, k) - Q(3, i, j - 1, k));
+                double wet = de4 * (Q(4, i + 1, j + 1, k) - Q(4, i + 1, j - 1, k) + Q(4, i, j + 1, k) - Q(4, i, j - 1, k));
+                double uze = dz4 * (Q(2, i + 1, j, k + 1) - Q(2, i + 1, j, k - 1) + Q(2, i, j, k + 1) - Q(2, i, j, k - 1));
+                double vze = dz4 * (Q(3, i + 1, j, k + 1) - Q(3, i + 1, j, k - 1) + Q(3, i, j, k + 1) - Q(3, i, j, k - 1));
+                double wze = dz4 * (Q(4, i + 1, j, k + 1) - Q(4, i + 1, j, k - 1) + Q(4, i, j, k + 1) - Q(4, i, j, k - 1));
+                double a1 = CSI(1, i + 1, j, k) + CSI(1, i, j, k), a2 = CSI(2, i + 1, j, k) + CSI(2, i, j, k), a3 = CSI(3, i + 1, j, k) + CSI(3, i, j, k);
+                double b1 = ETA(1, i + 1, j, k) + ETA(1, i, j, k), b2 = ETA(2, i + 1, j, k) + ETA(2, i, j, k), b3 = ETA(3, i + 1, j, k) + ETA(3, i, j, k);
+                double c1 = ZET(1, i + 1, j, k) + ZET(1, i, j, k), c2 = ZET(2, i + 1, j, k) + ZET(2, i, j, k), c3 = ZET(3, i + 1, j, k) + ZET(3, i, j, k);
+                double uu1 = a1 * ucs + b1 * uet + c1 * uze, vv1 = a1 * vcs + b1 * vet + c1 * vze, ww1 = a1 * wcs + b1 * wet + c1 * wze;
+                double uu2 = a2 * ucs + b2 * uet + c2 * uze, vv2 = a2 * vcs + b2 * vet + c2 * vze, ww2 = a2 * wcs + b2 * wet + c2 * wze;
+                double uu3 = a3 * ucs + b3 * uet + c3 * uze, vv3 = a3 * vcs + b3 * vet + c3 * vze, ww3 = a3 * wcs + b3 * wet + c3 * wze;
+                double c11 = a1 * a1 + a2 * a2 + a3 * a3, c12 = a1 * b1 + a2 * b2 + a3 * b3, c13 = a1 * c1 + a2 * c2 + a3 * c3;
+                double vtj = one_fourth * (RET(i + 1, j, k) + RET(i, j, k)) / (AJ(i + 1, j, k) + AJ(i, j, k));
+                FV(1, i, j, k) = vtj * (a1 * uu1 + a2 * vv1 + a3 * ww1 + c11 * ucs + c12 * uet + c13 * uze);
+                FV(2, i, j, k) = vtj * (a1 * uu2 + a2 * vv2 + a3 * ww2 + c11 * vcs + c12 * vet + c13 * vze);
+                FV(3, i, j, k) = vtj * (a1 * uu3 + a2 * vv3 + a3 * ww3 + c11 * wcs + c12 * wet + c13 * wze);
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                for (c = 1; c <= 3; c++) VISC(c, i, j, k) = dc * (FV(c, i, j, k) - FV(c, i - 1, j, k));
+    /* eta faces */
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js - 1; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double ucs = dc4 * (Q(2, i + 1, j + 1, k) - Q(2, i - 1, j + 1, k) + Q(2, i + 1, j, k) - Q(2, i - 1, j, k));
+                double vcs = dc4 * (Q(3, i + 1, j + 1, k) - Q(3, i - 1, j + 1, k) + Q(3, i + 1, j, k) - Q(3, i - 1, j, k));
+                double wcs = dc4 * (Q(4, i + 1, j + 1, k) - Q(4, i - 1, j + 1, k) + Q(4, i + 1, j, k) - Q(4, i - 1, j, k));
+                double uet = de * (Q(2, i, j + 1, k) - Q(2, i, j, k));
+                double vet = de * (Q(3, i, j + 1, k) - Q(3, i, j, k));
+                double wet = de * (Q(4, i, j + 1, k) - Q(4, i, j, k));
+                double uze = dz4 * (Q(2, i, j + 1, k + 1) - Q(2, i, j + 1, k - 1) + Q(2, i, j, k + 1) - Q(2, i, j, k - 1));
+                double vze = dz4 * (Q(3, i, j + 1, k + 1) - Q(3, i, j + 1, k - 1) + Q(3, i, j, k + 1) - Q(3, i, j, k - 1));
+                double wze = dz4 * (Q(4, i, j + 1, k + 1) - Q(4, i, j + 1, k - 1) + Q(4, i, j, k + 1) - Q(4, i, j, k - 1));
+                double a1 = CSI(1, i, j + 1, k) + CSI(1, i, j, k), a2 = CSI(2, i, j + 1, k) + CSI(2, i, j, k), a3 = CSI(3, i, j + 1, k) + CSI(3, i, j, k);
+                double b1 = ETA(1, i, j + 1, k) + ETA(1, i, j, k), b2 = ETA(2, i, j + 1, k) + ETA(2, i, j, k), b3 = ETA(3, i, j + 1, k) + ETA(3, i, j, k);
+                double c1 = ZET(1, i, j + 1, k) + ZET(1, i, j, k), c2 = ZET(2, i, j + 1, k) + ZET(2, i, j, k), c3 = ZET(3, i, j + 1, k) + ZET(3, i, j, k);
+                double uu1 = a1 * ucs + b1 * uet + c1 * uze, vv1 = a1 * vcs + b1 * vet + c1 * vze, ww1 = a1 * wcs + b1 * wet + c1 * wze;
+                double uu2 = a2 * ucs + b2 * uet + c2 * uze, vv2 = a2 * vcs + b2 * vet + c2 * vze, ww2 = a2 * wcs + b2 * wet + c2 * wze;
+                double uu3 = a3 * ucs + b3 * uet + c3 * uze, vv3 = a3 * vcs + b3 * vet + c3 * vze, ww3 = a3 * wcs + b3 * wet + c3 * wze;
+                double c21 = b1 * a1 + b2 * a2 + b3 * a3, c22 = b1 * b1 + b2 * b2 + b3 * b3, c23 = b1 * c1 + b2 * c2 + b3 * c3;
+                double vtj = one_fourth * (RET(i, j + 1, k) + RET(i, j, k)) / (AJ(i, j + 1, k) + AJ(i, j, k));
+                FV(1, i, j, k) = vtj * (b1 * uu1 + b2 * vv1 + b3 * ww1 + c21 * ucs + c22 * uet + c23 * uze);
+                FV(2, i, j, k) = vtj * (b1 * uu2 + b2 * vv2 + b3 * ww2 + c21 * vcs + c22 * vet + c23 * vze);
+                FV(3, i, j, k) = vtj * (b1 * uu3 + b2 * vv3 + b3 * ww3 + c21 * wcs + c22 * wet + c23 * wze);
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                for (c = 1; c <= 3; c++) VISC(c, i, j, k) = VISC(c, i, j, k) + de * (FV(c, i, j, k) - FV(c, i, j - 1, k));
+    /* zet faces */
+    for (k = d->ks - 1; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double ucs = dc4 * (Q(2, i + 1, j, k + 1) - Q(2, i - 1, j, k + 1) + Q(2, i + 1, j, k) - Q(2, i - 1, j, k));
+                double vcs = dc4 * (Q(3, i + 1, j, k + 1) - Q(3, i - 1, j, k + 1) + Q(3, i + 1, j, k) - Q(3, i - 1, j, k));
+                double wcs = dc4 * (Q(4, i + 1, j, k + 1) - Q(4, i - 1, j, k + 1) + Q(4, i + 1, j, k) - Q(4, i - 1, j, k));
+                double uet = de4 * (Q(2, i, j + 1, k + 1) - Q(2, i, j - 1, k + 1) + Q(2, i, j + 1, k) - Q(2, i, j - 1, k));
+                double vet = de4 * (Q(3, i, j + 1, k + 1) - Q(3, i, j - 1, k + 1) + Q(3, i, j + 1, k) - Q(3, i, j - 1, k));
+                double wet = de4 * (Q(4, i, j + 1, k + 1) - Q(4, i, j - 1, k + 1) + Q(4, i, j + 1, k) - Q(4, i, j - 1, k));
+                double uze = dz * (Q(2, i, j, k + 1) - Q(2, i, j, k));
+                double vze = dz * (Q(3, i, j, k + 1) - Q(3, i, j, k));
+                double wze = dz * (Q(4, i, j, k + 1) - Q(4, i, j, k));
+                double a1 = CSI(1, i, j, k + 1) + CSI(1, i, j, k), a2 = CSI(2, i, j, k + 1) + CSI(2, i, j, k), a3 = CSI(3, i, j, k + 1) + CSI(3, i, j, k);
+                double b1 = ETA(1, i, j, k + 1) + ETA(1, i, j, k), b2 = ETA(2, i, j, k + 1) + ETA(2, i, j, k), b3 = ETA(3, i, j, k + 1) + ETA(3, i, j, k);
+                double c1 = ZET(1, i, j, k + 1) + ZET(1, i, j, k), c2 = ZET(2, i, j, k + 1) + ZET(2, i, j, k), c3 = ZET(3, i, j, k + 1) + ZET(3, i, j, k);
+                double uu1 = a1 * ucs + b1 * uet + c1 * uze, vv1 = a1 * vcs + b1 * vet + c1 * vze, ww1 = a1 * wcs + b1 * wet + c1 * wze;
+                double uu2 = a2 * ucs + b2 * uet + c2 * uze, vv2 = a2 * vcs + b2 * vet + c2 * vze, ww2 = a2 * wcs + b2 * wet + c2 * wze;
+                double uu3 = a3 * ucs + b3 * uet + c3 * uze, vv3 = a3 * vcs + b3 * vet + c3 * vze, ww3 = a3 * wcs + b3 * wet + c3 * wze;
+                double c31 = c1 * a1 + c2 * a2 + c3 * a3, c32 = c1 * b1 + c2 * b2 + c3 * b3, c33 = c1 * c1 + c2 * c2 + c3 * c3;
+                double vtj = one_fourth * (RET(i, j, k + 1) + RET(i, j, k)) / (AJ(i, j, k + 1) + AJ(i, j, k));
+                FV(1, i, j, k) = vtj * (c1 * uu1 + c2 * vv1 + c3 * ww1 + c31 * ucs + c32 * uet + c33 * uze);
+                FV(2, i, j, k) = vtj * (c1 * uu2 + c2 * vv2 + c3 * ww2 + c31 * vcs + c32 * vet + c33 * vze);
+                FV(3, i, j, k) = vtj * (c1 * uu3 + c2 * vv3 + c3 * ww3 + c31 * wcs + c32 * wet + c33 * wze);
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                for (c = 1; c <= 3; c++) VISC(c, i, j, k) = VISC(c, i, j, k) + dz * (FV(c, i, j, k) - FV(c, i, j, k - 1));
+#undef RET
+    free(ret);
+}
+
+/* rhs_diss_p (src/rhs_diss_p.F90:2-205) */
+static void rhs_diss_p(daf_t* d) {
+    DLV;
+    int i, j, k;
+    const double dcsq = d->dc * d->dc, desq = d->de * d->de, dzsq = d->dz * d->dz;
+    long N = (long)ni * nj * nk;
+    memset(d->fv, 0, 3 * N * sizeof(double));
+    memset(d->diss, 0, 4 * N * sizeof(double));
+    for (k = d->ks - 1; k <= d->ke + 1; k++)
+        for (j = d->js - 1; j <= d->je + 1; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double g1 = CSI(1, i, j, k) * CSI(1, i, j, k) + CSI(2, i, j, k) * CSI(2, i, j, k) + CSI(3, i, j, k) * CSI(3, i, j, k);
+                double tmp1 = g1 * DTAU(i, j, k) / AJ(i, j, k);
+                double tmp2 = dcsq * (Q(1, i + 1, j, k) - 2.0 * Q(1, i, j, k) + Q(1, i - 1, j, k));
+                FV(1, i, j, k) = tmp1 * tmp2;
+            }
+    for (k = d->ks - 1; k <= d->ke + 1; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is - 1; i <= d->ie + 1; i++) {
+                double g2 = ETA(1, i, j, k) * ETA(1, i, j, k) + ETA(2, i, j, k) * ETA(2, i, j, k) + ETA(3, i, j, k) * ETA(3, i, j, k);
+                double tmp1 = g2 * DTAU(i, j, k) / AJ(i, j, k);
+                double tmp2 = desq * (Q(1, i, j + 1, k) - 2.0 * Q(1, i, j, k) + Q(1, i, j - 1, k));
+                FV(2, i, j, k) = tmp1 * tmp2;
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js - 1; j <= d->je + 1; j++)
+            for (i = d->is - 1; i <= d->ie + 1; i++) {
+                double g3 = ZET(1, i, j, k) * ZET(1, i, j, k) + ZET(2, i, j, k) * ZET(2, i, j, k) + ZET(3, i, j, k) * ZET(3, i, j, k);
+                double tmp1 = g3 * DTAU(i, j, k) / AJ(i, j, k);
+                double tmp2 = dzsq * (Q(1, i, j, k + 1) - 2.0 * Q(1, i, j, k) + Q(1, i, j, k - 1));
+                FV(3, i, j, k) = tmp1 * tmp2;
+            }
+    /* one-sided copies at the domain boundaries (:124-129) */
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++) {
+            FV(1, d->is - 1, j, k) = FV(1, d->is, j, k);
+            FV(1, d->ie + 1, j, k) = FV(1, d->ie, j, k);
+        }
+    for (k = 1; k <= nk; k++)
+        for (i = 1; i <= ni; i++) {
+            FV(2, i, d->js - 1, k) = FV(2, i, d->js, k);
+            FV(2, i, d->je + 1, k) = FV(2, i, d->je, k);
+        }
+    for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++) {
+            FV(3, i, j, d->ks - 1) = FV(3, i, j, d->ks);
+            FV(3, i, j, d->ke + 1) = FV(3, i, j, d->ke);
+        }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                DISS(1, i, j, k) = d->o->p.pdiss_coef *
+                                   ((FV(1, i + 1, j, k) - 2.0 * FV(1, i, j, k) + FV(1, i - 1, j, k)) +
+                                    (FV(2, i, j + 1, k) - 2.0 * FV(2, i, j, k) + FV(2, i, j - 1, k)) +
+                                    (FV(3, i, j, k + 1) - 2.0 * FV(3, i, j, k) + FV(3, i, j, k - 1)));
+}
+
+/* rhs_flux_sans_convec (src/rhs_flux_sans_convec.F90:2-140) */
+static void rhs_flux_sans_convec(daf_t* d) {
+    DLV;
+    int i, j, k, c;
+    const double dc2 = one_half * d->dc, de2 = one_half * d->de, dz2 = one_half * d->dz;
+    long N = (long)ni * nj * nk;
+    double* p = zalloc(N);
+#define P(i, j, k) p[IX(i, j, k)]
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) P(i, j, k) = Q(1, i, j, k) / AJ(i, j, k);
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                for (c = 1; c <= 3; c++)
+                    RH(c + 1, i, j, k) = dc2 * (P(i + 1, j, k) * CSI(c, i + 1, j, k) - P(i - 1, j, k) * CSI(c, i - 1, j, k));
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                for (c = 1; c <= 3; c++)
+                    RH(c + 1, i, j, k) = RH(c + 1, i, j, k) + de2 * (P(i, j + 1, k) * ETA(c, i, j + 1, k) - P(i, j - 1, k) * ETA(c, i, j - 1, k));
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++)
+                for (c = 1; c <= 3; c++)
+                    RH(c + 1, i, j, k) = RH(c + 1, i, j, k) + dz2 * (P(i, j, k + 1) * ZET(c, i, j, k + 1) - P(i, j, k - 1) * ZET(c, i, j, k - 1));
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double dudx = dc2 * (UCN(1, i + 1, j, k) - UCN(1, i - 1, j, k));
+                double dvdy = de2 * (UCN(2, i, j + 1, k) - UCN(2, i, j - 1, k));
+                double dwdz = dz2 * (UCN(3, i, j, k + 1) - UCN(3, i, j, k - 1));
+                RH(1, i, j, k) = dudx + dvdy + dwdz;
+            }
+#undef P
+    free(p);
+}
+
+/* rhs_convec_quick (src/rhs_convec_quick.F90:2-438) */
+static void rhs_convec_quick(daf_t* d, int decide_upwind) {
+    DLV;
+    int i, j, k, m, dir;
+    const double coef = 0.125;
+    long N = (long)ni * nj * nk;
+    double* up = zalloc(N);
+    double* um = zalloc(N);
+    const int ista = d->is + 1, iend = d->ie - 1, jsta = d->js + 1, jend = d->je - 1, ksta = d->ks + 1, kend = d->ke - 1;
+#define UP(i, j, k) up[IX(i, j, k)]
+#define UM(i, j, k) um[IX(i, j, k)]
+    for (dir = 1; dir <= 3; dir++) {
+        const int di = dir == 1, dj = dir == 2, dk = dir == 3;
+        const double h = dir == 1 ? d->dc : dir == 2 ? d->de : d->dz;
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    double ucon = one_half * UCN(dir, i, j, k);
+                    UP(i, j, k) = ucon + fabs(ucon);
+                    UM(i, j, k) = ucon - fabs(ucon);
+                }
+        if (decide_upwind == 1) {
+            for (k = d->ks; k <= d->ke; k++)
+                for (j = d->js; j <= d->je; j++)
+                    for (i = d->is; i <= d->ie; i++)
+                        for (m = 2; m <= 4; m++)
+                            FV(m - 1, i, j, k) = h * (UP(i, j, k) * (Q(m, i, j, k) - Q(m, i - di, j - dj, k - dk)) +
+                                                      UM(i, j, k) * (Q(m, i + di, j + dj, k + dk) - Q(m, i, j, k)));
+        } else {
+            int i0 = d->is, i1 = d->ie, j0 = d->js, j1 = d->je, k0 = d->ks, k1 = d->ke;
+            if (dir == 1) { i0 = ista; i1 = iend; }
+            if (dir == 2) { j0 = jsta; j1 = jend; }
+            if (dir == 3) { k0 = ksta; k1 = kend; }
+            for (k = k0; k <= k1; k++)
+                for (j = j0; j <= j1; j++)
+                    for (i = i0; i <= i1; i++)
+                        for (m = 2; m <= 4; m++)
+                            FV(m - 1, i, j, k) =
+                                coef * h *
+                                (UP(i, j, k) * (3.0 * Q(m, i + di, j + dj, k + dk) + 3.0 * Q(m, i, j, k) -
+                                                 7.0 * Q(m, i - di, j - dj, k - dk) + Q(m, i - 2 * di, j - 2 * dj, k - 2 * dk)) +
+                                 UM(i, j, k) * (-3.0 * Q(m, i - di, j - dj, k - dk) - 3.0 * Q(m, i, j, k) +
+                                                 7.0 * Q(m, i + di, j + dj, k + dk) - Q(m, i + 2 * di, j + 2 * dj, k + 2 * dk)));
+            /* first-order upwind at the first and last interior nodes (:140-161) */
+            if (dir == 1) {
+                for (k = d->ks; k <= d->ke; k++)
+                    for (j = d->js; j <= d->je; j++) {
+                        int ib, ii;
+                        for (ib = 0; ib < 2; ib++) {
+                            ii = ib == 0 ? d->is : d->ie;
+                            for (m = 2; m <= 4; m++)
+                                FV(m - 1, ii, j, k) = h * (UP(ii, j, k) * (Q(m, ii, j, k) - Q(m, ii - 1, j, k)) +
+                                                           UM(ii, j, k) * (Q(m, ii + 1, j, k) - Q(m, ii, j, k)));
+                        }
+                    }
+            } else if (dir == 2) {
+                for (k = d->ks; k <= d->ke; k++)
+                    for (i = d->is; i <= d->ie; i++) {
+                        int jb, jj;
+                        for (jb = 0; jb < 2; jb++) {
+                            jj = jb == 0 ? d->js : d->je;
+                            for (m = 2; m <= 4; m++)
+                                FV(m - 1, i, jj, k) = h * (UP(i, jj, k) * (Q(m, i, jj, k) - Q(m, i, jj - 1, k)) +
+                                                           UM(i, jj, k) * (Q(m, i, jj + 1, k) - Q(m, i, jj, k)));
+                        }
+                    }
+            } else {
+                for (j = d->js; j <= d->je; j++)
+                    for (i = d->is; i <= d->ie; i++) {
+                        int kb, kk;
+                        for (kb = 0; kb < 2; kb++) {
+                            kk = kb == 0 ? d->ks : d->ke;
+                            for (m = 2; m <= 4; m++)
+                                FV(m - 1, i, j, kk) = h * (UP(i, j, kk) * (Q(m, i, j, kk) - Q(m, i, j, kk - 1)) +
+                                                           UM(i, j, kk) * (Q(m, i, j, kk + 1) - Q(m, i, j, kk)));
+                        }
+                    }
+            }
+        }
+        for (k = d->ks; k <= d->ke; k++)
+            for (j = d->js; j <= d->je; j++)
+                for (i = d->is; i <= d->ie; i++)
+                    for (m = 2; m <= 4; m++) RH(m, i, j, k) = RH(m, i, j, k) + FV(m - 1, i, j, k);
+    }
+    /* diss(2:4) = 0 (:433) */
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++)
+                for (m = 2; m <= 4; m++) DISS(m, i, j, k) = 0.0;
+#undef UP
+#undef UM
+    free(up);
+    free(um);
+}
+
+/* rhs_unst_visc_diss (src/rhs_unst_visc_diss.F90:2-52) */
+static void rhs_unst_visc_diss(daf_t* d) {
+    DLV;
+    int i, j, k, m;
+    const double es = d->o->e_source, delti = d->o->p.delti;
+    (void)nk;
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double dtaj = 2.0 * delti * AJ(i, j, k);
+                double dqdt[3];
+                for (m = 2; m <= 4; m++)
+                    dqdt[m - 2] = es * (3.0 * Q(m, i, j, k) - 4.0 * QN(m, i, j, k) + QNM1(m, i, j, k)) / dtaj;
+                RH(1, i, j, k) = RH(1, i, j, k) + DISS(1, i, j, k);
+                for (m = 2; m <= 4; m++)
+                    RH(m, i, j, k) = RH(m, i, j, k) + DISS(m, i, j, k) + dqdt[m - 2] - VISC(m - 1, i, j, k);
+            }
+}
+
+/* rhs_pk (src/rhs_pk.F90:2-69): FAS forcing on interior +/- 1 */
+static void rhs_pk(daf_t* d, int calc) {
+    DLV;
+    int i, j, k, m;
+    (void)nk;
+    if (calc == 1)
+        for (k = d->ks - 1; k <= d->ke + 1; k++)
+            for (j = d->js - 1; j <= d->je + 1; j++)
+                for (i = d->is - 1; i <= d->ie + 1; i++)
+                    for (m = 1; m <= 4; m++) PK(m, i, j, k) = PK(m, i, j, k) - RH(m, i, j, k);
+    for (k = d->ks - 1; k <= d->ke + 1; k++)
+        for (j = d->js - 1; j <= d->je + 1; j++)
+            for (i = d->is - 1; i <= d->ie + 1; i++)
+                for (m = 1; m <= 4; m++) RH(m, i, j, k) = RH(m, i, j, k) + PK(m, i, j, k);
+}
+
+static void zero_rh_boundaries(daf_t* d) {
+    DLV;
+    int i, j, k, m;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (m = 1; m <= 4; m++) RH(m, d->is - 1, j, k) = 0.0, RH(m, d->ie + 1, j, k) = 0.0;
+    for (k = 1; k <= nk; k++)
+        for (i = 1; i <= ni; i++)
+            for (m = 1; m <= 4; m++) RH(m, i, d->js - 1, k) = 0.0, RH(m, i, d->je + 1, k) = 0.0;
+    for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++)
+            for (m = 1; m <= 4; m++) RH(m, i, j, d->ks - 1) = 0.0, RH(m, i, j, d->ke + 1) = 0.0;
+}
+
+/* rhs_modal_matrices (src/rhs_modal_matrices.F90:1-270) */
+static void matmul4(const double a[4][4], const double b[4][4], double c[4][4]) {
+    int r, s, t;
+    for (r = 0; r < 4; r++)
+        for (s = 0; s < 4; s++) {
+            double acc = 0.0;
+            for (t = 0; t < 4; t++) acc = acc + a[r][t] * b[t][s];
+            c[r][s] = acc;
+        }
+}
+
+static void rhs_modal_matrices(daf_t* d) {
+    DLV;
+    int i, j, k, r, c;
+    const double es = d->o->e_source, delti = d->o->p.delti, beta = d->o->p.beta;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double csi1 = CSI(1, i, j, k), csi2 = CSI(2, i, j, k), csi3 = CSI(3, i, j, k);
+                double eta1 = ETA(1, i, j, k), eta2 = ETA(2, i, j, k), eta3 = ETA(3, i, j, k);
+                double zet1 = ZET(1, i, j, k), zet2 = ZET(2, i, j, k), zet3 = ZET(3, i, j, k);
+                double csi1s = csi1 * csi1, csi2s = csi2 * csi2, csi3s = csi3 * csi3;
+                double eta1s = eta1 * eta1, eta2s = eta2 * eta2, eta3s = eta3 * eta3;
+                double zet1s = zet1 * zet1, zet2s = zet2 * zet2, zet3s = zet3 * zet3;
+                double g1 = csi1s + csi2s + csi3s, g2 = eta1s + eta2s + eta3s, g3 = zet1s + zet2s + zet3s;
+                double sg1 = sqrt(g1), sg2 = sqrt(g2), sg3 = sqrt(g3);
+                double alpha = 1.0 + es * onept5 * DTAU(i, j, k) / delti;
+                double sigma = sqrt(beta * alpha), theta = sqrt(beta / alpha);
+                double scsi, seta, szet, dcsi, deta, dzet, rc1, rc2, rc3, re1, re2, re3, rz1, rz2, rz3, ssgd, sgd2;
+                double ma[4][4], mb[4][4], mbi[4][4], mci[4][4], mai[4][4], mc[4][4], rp[4][4];
+                SPR(1, i, j, k) = theta * sg1;
+                SPR(2, i, j, k) = theta * sg2;
+                SPR(3, i, j, k) = theta * sg3;
+                scsi = sqrt(2.0 * g1 + (csi2 - csi3) * (csi2 - csi3));
+                seta = sqrt(2.0 * g2 + (eta2 - eta3) * (eta2 - eta3));
+                szet = sqrt(2.0 * g3 + (zet2 - zet3) * (zet2 - zet3));
+                dcsi = g1 + 2.0 * csi2 * csi3 + csi1 * csi1;
+                deta = g2 + 2.0 * eta2 * eta3 + eta1 * eta1;
+                dzet = g3 + 2.0 * zet2 * zet3 + zet1 * zet1;
+                rc1 = csi2 + csi3; rc2 = csi2 - csi3; rc3 = csi2s - csi3s;
+                re1 = eta2 + eta3; re2 = eta2 - eta3; re3 = eta2s - eta3s;
+                rz1 = zet2 + zet3; rz2 = zet2 - zet3; rz3 = zet2s - zet3s;
+                /* Ma */
+                ma[0][0] = 0.0; ma[0][1] = 0.0; ma[0][2] = sigma; ma[0][3] = sigma;
+                ma[1][0] = -(csi2 + csi3) / scsi; ma[1][1] = csi1 * (csi3 - csi2) / (scsi * sg1); ma[1][2] = csi1 / sg1; ma[1][3] = -csi1 / sg1;
+                ma[2][0] = csi1 / scsi; ma[2][1] = (g1 + csi2 * (csi3 - csi2)) / (scsi * sg1); ma[2][2] = csi2 / sg1; ma[2][3] = -csi2 / sg1;
+                ma[3][0] = csi1 / scsi; ma[3][1] = -(g1 - csi3 * (csi3 - csi2)) / (scsi * sg1); ma[3][2] = csi3 / sg1; ma[3][3] = -csi3 / sg1;
+                /* Mai */
+                mai[0][0] = 0.0; mai[0][1] = -scsi * (csi2 + csi3) / dcsi; mai[0][2] = scsi * csi1 / dcsi; mai[0][3] = mai[0][2];
+                ssgd = scsi / sg1 / dcsi;
+                sgd2 = 1.0 / sg1 / dcsi / 2.0;
+                mai[1][0] = 0.0; mai[1][1] = csi1 * (csi3 - csi2) * ssgd; mai[1][2] = (csi2 * csi3 + csi1s + csi3s) * ssgd; mai[1][3] = -(csi2 * csi3 + csi1s + csi2s) * ssgd;
+                mai[2][0] = pt5 / sigma; mai[2][1] = csi1 * (2.0 * g1 + 2.0 * csi2 * csi3 - csi2s - csi3s) * sgd2;
+                mai[2][2] = (g1 * rc1 + csi1s * rc2 + csi3 * rc3) * sgd2; mai[2][3] = (g1 * rc1 - csi1s * rc2 - csi2 * rc3) * sgd2;
+                mai[3][0] = pt5 / sigma; mai[3][1] = -mai[2][1]; mai[3][2] = -mai[2][2]; mai[3][3] = -mai[2][3];
+                /* Mb */
+                mb[0][0] = 0.0; mb[0][1] = 0.0; mb[0][2] = sigma; mb[0][3] = sigma;
+                mb[1][0] = -(eta2 + eta3) / seta; mb[1][1] = eta1 * (eta3 - eta2) / (seta * sg2); mb[1][2] = eta1 / sg2; mb[1][3] = -eta1 / sg2;
+                mb[2][0] = eta1 / seta; mb[2][1] = (g2 + eta2 * (eta3 - eta2)) / (seta * sg2); mb[2][2] = eta2 / sg2; mb[2][3] = -eta2 / sg2;
+                mb[3][0] = eta1 / seta; mb[3][1] = -(g2 - eta3 * (eta3 - eta2)) / (seta * sg2); mb[3][2] = eta3 / sg2; mb[3][3] = -eta3 / sg2;
+                /* Mbi */
+                mbi[0][0] = 0.0; mbi[0][1] = -seta * (eta2 + eta3) / deta; mbi[0][2] = seta * eta1 / deta; mbi[0][3] = mbi[0][2];
+                ssgd = seta / sg2 / deta;
+                sgd2 = 1.0 / sg2 / deta / 2.0;
+                mbi[1][0] = 0.0; mbi[1][1] = eta1 * (eta3 - eta2) * ssgd; mbi[1][2] = (eta2 * eta3 + eta1s + eta3s) * ssgd; mbi[1][3] = -(eta2 * eta3 + eta1s + eta2s) * ssgd;
+                mbi[2][0] = pt5 / sigma; mbi[2][1] = eta1 * (2.0 * g2 + 2.0 * eta2 * eta3 - eta2s - eta3s) * sgd2;
+                mbi[2][2] = (g2 * re1 + eta1s * re2 + eta3 * re3) * sgd2; mbi[2][3] = (g2 * re1 - eta1s * re2 - eta2 * re3) * sgd2;
+                mbi[3][0] = pt5 / sigma; mbi[3][1] = -mbi[2][1]; mbi[3][2] = -mbi[2][2]; mbi[3][3] = -mbi[2][3];
+                /* Mc */
+                mc[0][0] = 0.0; mc[0][1] = 0.0; mc[0][2] = sigma; mc[0][3] = sigma;
+                mc[1][0] = -(zet2 + zet3) / szet; mc[1][1] = zet1 * (zet3 - zet2) / (szet * sg3); mc[1][2] = zet1 / sg3; mc[1][3] = -zet1 / sg3;
+                mc[2][0] = zet1 / szet; mc[2][1] = (g3 + zet2 * (zet3 - zet2)) / (szet * sg3); mc[2][2] = zet2 / sg3; mc[2][3] = -zet2 / sg3;
+                mc[3][0] = zet1 / szet; mc[3][1] = -(g3 - zet3 * (zet3 - zet2)) / (szet * sg3); mc[3][2] = zet3 / sg3; mc[3][3] = -zet3 / sg3;
+                /* Mci */
+                mci[0][0] = 0.0; mci[0][1] = -szet * (zet2 + zet3) / dzet; mci[0][2] = szet * zet1 / dzet; mci[0][3] = mci[0][2];
+                ssgd = szet / sg3 / dzet;
+                sgd2 = 1.0 / sg3 / dzet / 2.0;
+                mci[1][0] = 0.0; mci[1][1] = zet1 * (zet3 - zet2) * ssgd; mci[1][2] = (zet2 * zet3 + zet1s + zet3s) * ssgd; mci[1][3] = -(zet2 * zet3 + zet1s + zet2s) * ssgd;
+                mci[2][0] = pt5 / sigma; mci[2][1] = zet1 * (2.0 * g3 + 2.0 * zet2 * zet3 - zet2s - zet3s) * sgd2;
+                mci[2][2] = (g3 * rz1 + zet1s * rz2 + zet3 * rz3) * sgd2; mci[2][3] = (g3 * rz1 - zet1s * rz2 - zet2 * rz3) * sgd2;
+                mci[3][0] = pt5 / sigma; mci[3][1] = -mci[2][1]; mci[3][2] = -mci[2][2]; mci[3][3] = -mci[2][3];
+                for (r = 0; r < 4; r++)
+                    for (c = 0; c < 4; c++) {
+                        MAT(mai, r + 1, c + 1, i, j, k) = mai[r][c];
+                        MAT(mc, r + 1, c + 1, i, j, k) = mc[r][c];
+                    }
+                matmul4(mbi, ma, rp);
+                for (r = 0; r < 4; r++)
+                    for (c = 0; c < 4; c++) MAT(n1i, r + 1, c + 1, i, j, k) = rp[r][c];
+                matmul4(mci, mb, rp);
+                for (r = 0; r < 4; r++)
+                    for (c = 0; c < 4; c++) MAT(n2i, r + 1, c + 1, i, j, k) = rp[r][c];
+            }
+}
+
+static void matvec_node(const double* a /* 16, row-major */, double* v /* 4 */) {
+    double rp[4];
+    int r, c;
+    for (r = 0; r < 4; r++) {
+        double acc = 0.0;
+        for (c = 0; c < 4; c++) acc = acc + a[4 * r + c] * v[c];
+        rp[r] = acc;
+    }
+    for (r = 0; r < 4; r++) v[r] = rp[r];
+}
+
+/* one tridiagonal line solve of the DAF: builds the 4 scalar systems of
+ * rhs_diag_solver for a line in direction dir and solves them with SGTSV */
+static void daf_line(daf_t* d, int dir, int i0, int j0, int k0, int len, const double* brh1, const double* brh2,
+                     int use_b1, int use_b2, double* aw, double* ap, double* ae, double* aq) {
+    DLV;
+    int s, m;
+    const double h = dir == 1 ? d->dc : dir == 2 ? d->de : d->dz;
+    const double h2 = pt5 * h, hsq = h * h;
+    const double epd = d->o->p.ep[0][dir - 1];
+    const double lambda[4] = {0.0, 0.0, 1.0, -1.0};
+    const int di = dir == 1, dj = dir == 2, dk = dir == 3;
+    (void)nk;
+    for (s = 0; s < len; s++) {
+        int i = i0 + s * di, j = j0 + s * dj, k = k0 + s * dk;
+        double dtc = 1.0, tmp, dds;
+        if (DTAU(i, j, k) == 0.0) dtc = 0.0;
+        tmp = DTAU(i, j, k) * epd * SPR(dir, i, j, k) * hsq;
+        for (m = 0; m < 4; m++) ap[m * len + s] = 1.0 + 2.0 * tmp;
+        dds = dtc * DTAU(i - di, j - dj, k - dk) * h2 * SPR(dir, i - di, j - dj, k - dk);
+        aw[0 * len + s] = -tmp;
+        aw[1 * len + s] = -tmp;
+        aw[2 * len + s] = -dds * lambda[2] - tmp;
+        aw[3 * len + s] = -dds * lambda[3] - tmp;
+        dds = dtc * DTAU(i + di, j + dj, k + dk) * h2 * SPR(dir, i + di, j + dj, k + dk);
+        ae[0 * len + s] = -tmp;
+        ae[1 * len + s] = -tmp;
+        ae[2 * len + s] = dds * lambda[2] - tmp;
+        ae[3 * len + s] = dds * lambda[3] - tmp;
+        for (m = 0; m < 4; m++) aq[m * len + s] = RH(m + 1, i, j, k);
+    }
+    if (use_b1)
+        for (m = 0; m < 4; m++) aq[m * len + 0] = aq[m * len + 0] - aw[m * len + 0] * brh1[m];
+    if (use_b2)
+        for (m = 0; m < 4; m++) aq[m * len + len - 1] = aq[m * len + len - 1] - ae[m * len + len - 1] * brh2[m];
+    for (m = 0; m < 4; m++)
+        or_gtsv(len, 1, aw + m * len + 1, ap + m * len, ae + m * len, aq + m * len, len);
+    for (s = 0; s < len; s++) {
+        int i = i0 + s * di, j = j0 + s * dj, k = k0 + s * dk;
+        for (m = 0; m < 4; m++) RH(m + 1, i, j, k) = aq[m * len + s];
+    }
+}
+
+/* rhs_diag_solver (src/rhs_diag_solver.F90:2-636) */
+static void rhs_diag_solver(daf_t* d) {
+    DLV;
+    const or_state* o = d->o;
+    int i, j, k, m, c;
+    const int n = d->n;
+    const int is = d->is, ie = d->ie, js = d->js, je = d->je, ks = d->ks, ke = d->ke, idend = d->idend;
+    const double alpha_mult = o->e_source * one_pt_five / o->p.delti;
+    int maxlen = ni > nj ? ni : nj;
+    double *aw, *ap, *ae, *aq, *brh;
+    const int *bt = o->p.btype;
+    if (nk > maxlen) maxlen = nk;
+    aw = zalloc(4 * maxlen); ap = zalloc(4 * maxlen); ae = zalloc(4 * maxlen); aq = zalloc(4 * maxlen);
+    brh = zalloc(8L * (long)maxlen * maxlen); /* brh(1:4, 1:2, line) */
+#define BRH(m, side, a, b, na) brh[((long)((b)) * (na) + (a)) * 8 + ((side)-1) * 4 + (m)-1]
+    /* rh = S (-dtau J rh) (:56-72) */
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (i = is; i <= idend; i++) {
+                double alpha = 1.0 + alpha_mult * DTAU(i, j, k);
+                double r[4];
+                for (m = 1; m <= 4; m++) r[m - 1] = -DTAU(i, j, k) * RH(m, i, j, k) * AJ(i, j, k);
+                RH(1, i, j, k) = o->p.beta * r[0];
+                for (m = 2; m <= 4; m++) RH(m, i, j, k) = 1.0 / alpha * r[m - 1];
+            }
+    /* ---- csi sweep ---- */
+    memset(brh, 0, 8L * maxlen * maxlen * sizeof(double));
+    if (n == 1 && bt[0] != 5)
+        for (k = ks; k <= ke; k++)
+            for (j = js; j <= je; j++) {
+                for (m = 1; m <= 4; m++)
+                    BRH(m, 1, j, k, nj) = o->sa[0][m - 1] * RH(m, is, j, k) + o->sb[0][m - 1] * RH(m, is + 1, j, k);
+                /* sa(:,1) has 5 entries, rh(:,i,j,k) 4 (:97): the fifth
+                 * product reads rh(1) of the next i node and is stored in
+                 * brh(1,2,j,k) */
+                BRH(1, 2, j, k, nj) = o->sa[0][4] * RH(1, is + 1, j, k) + o->sb[0][4] * RH(1, is + 2, j, k);
+            }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (i = is; i <= idend; i++) matvec_node(&MAT(mai, 1, 1, i, j, k), &RH(1, i, j, k));
+    if (n == 1 && bt[0] != 5)
+        for (k = ks; k <= ke; k++)
+            for (j = js; j <= je; j++) matvec_node(&MAT(mai, 1, 1, is - 1, j, k), &BRH(1, 1, j, k, nj));
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++) {
+            /* front-side brh is built only under myfront /= MPI_PROC_NULL (:102),
+             * so on the domain boundary it stays zero when consumed (:202-205) */
+            daf_line(d, 1, is, j, k, ie - is + 1, &BRH(1, 1, j, k, nj), &BRH(1, 2, j, k, nj),
+                     n == 1 && bt[0] != 5, n == 1 && bt[1] != 5, aw, ap, ae, aq);
+        }
+    /* ---- eta sweep ---- */
+    memset(brh, 0, 8L * maxlen * maxlen * sizeof(double));
+    if (n == 1) {
+        if (bt[2] != 5)
+            for (k = ks; k <= ke; k++)
+                for (i = is; i <= idend; i++) {
+                    for (m = 1; m <= 4; m++)
+                        BRH(m, 1, i, k, ni) = o->sa[2][m - 1] * RH(m, i, js, k) + o->sb[2][m - 1] * RH(m, i, js + 1, k);
+                    /* fifth product of the 5 x 4 non-conforming expression (:286) */
+                    BRH(1, 2, i, k, ni) = o->sa[2][4] * RH(1, i + 1, js, k) + o->sb[2][4] * RH(1, i + 1, js + 1, k);
+                }
+        if (bt[3] != 5)
+            for (k = ks; k <= ke; k++)
+                for (i = is; i <= idend; i++)
+                    for (m = 1; m <= 4; m++)
+                        BRH(m, 2, i, k, ni) = o->sa[3][m - 1] * RH(m, i, je, k) + o->sb[3][m - 1] * RH(m, i, je - 1, k);
+    }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (i = is; i <= idend; i++) matvec_node(&MAT(n1i, 1, 1, i, j, k), &RH(1, i, j, k));
+    if (n == 1) {
+        if (bt[2] != 5)
+            for (k = ks; k <= ke; k++)
+                for (i = is; i <= idend; i++) matvec_node(&MAT(n1i, 1, 1, i, 1, k), &BRH(1, 1, i, k, ni));
+        if (bt[3] != 5)
+            for (k = ks; k <= ke; k++)
+                for (i = is; i <= idend; i++) matvec_node(&MAT(n1i, 1, 1, i, je + 1, k), &BRH(1, 2, i, k, ni));
+    }
+    for (k = ks; k <= ke; k++)
+        for (i = is; i <= idend; i++)
+            daf_line(d, 2, i, js, k, je - js + 1, &BRH(1, 1, i, k, ni), &BRH(1, 2, i, k, ni),
+                     n == 1 && bt[2] != 5, n == 1 && bt[3] != 5, aw, ap, ae, aq);
+    /* ---- zet sweep ---- */
+    memset(brh, 0, 8L * maxlen * maxlen * sizeof(double));
+    if (n == 1) {
+        if (bt[4] != 5)
+            for (j = js; j <= je; j++)
+                for (i = is; i <= idend; i++) {
+                    for (m = 1; m <= 4; m++)
+                        BRH(m, 1, i, j, ni) = o->sa[4][m - 1] * RH(m, i, j, ks) + o->sb[4][m - 1] * RH(m, i, j, ks + 1);
+                    /* fifth product of the 5 x 4 non-conforming expression (:462) */
+                    BRH(1, 2, i, j, ni) = o->sa[4][4] * RH(1, i + 1, j, ks) + o->sb[4][4] * RH(1, i + 1, j, ks + 1);
+                }
+        /* built under btype(6) /= 6 (:467) but consumed under /= 5 (:504,571) */
+        if (bt[5] != 6)
+            for (j = js; j <= je; j++)
+                for (i = is; i <= idend; i++)
+                    for (m = 1; m <= 4; m++)
+                        BRH(m, 2, i, j, ni) = o->sa[5][m - 1] * RH(m, i, j, ke) + o->sb[5][m - 1] * RH(m, i, j, ke - 1);
+    }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (i = is; i <= idend; i++) matvec_node(&MAT(n2i, 1, 1, i, j, k), &RH(1, i, j, k));
+    if (n == 1) {
+        if (bt[4] != 5)
+            for (j = js; j <= je; j++)
+                for (i = is; i <= idend; i++) matvec_node(&MAT(n2i, 1, 1, i, j, 1), &BRH(1, 1, i, j, ni));
+        if (bt[5] != 5)
+            for (j = js; j <= je; j++)
+                for (i = is; i <= idend; i++) matvec_node(&MAT(n2i, 1, 1, i, j, ke + 1), &BRH(1, 2, i, j, ni));
+    }
+    for (j = js; j <= je; j++)
+        for (i = is; i <= idend; i++)
+            daf_line(d, 3, i, j, ks, ke - ks + 1, &BRH(1, 1, i, j, ni), &BRH(1, 2, i, j, ni),
+                     n == 1 && bt[4] != 5, n == 1 && bt[5] != 5, aw, ap, ae, aq);
+    /* dQ = Mc dQ*** (:624-633) */
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (i = is; i <= idend; i++) matvec_node(&MAT(mc, 1, 1, i, j, k), &RH(1, i, j, k));
+    (void)c;
+#undef BRH
+    free(aw); free(ap); free(ae); free(aq); free(brh);
+}
+
+static int solver_daf(or_state* o, int n, int recalc, int calc_pk) {
+    daf_t dd, *d = &dd;
+    LEVEL(n);
+    long N = (long)ni * nj * nk;
+    int i, j, k, m;
+    if (ni < 5 || nj < 5 || nk < 5) return -1;
+    memset(d, 0, sizeof(*d));
+    d->o = o; d->n = n; d->ni = ni; d->nj = nj; d->nk = nk;
+    d->dc = o->dc[n]; d->de = o->de[n]; d->dz = o->dz[n];
+    d->is = 2; d->ie = ni - 1; d->js = 2; d->je = nj - 1; d->ks = 2; d->ke = nk - 1;
+    d->idend = d->ie;
+    d->q = o->q + 5 * off; d->qn = o->qn + 5 * off; d->qnm1 = o->qnm1 + 5 * off;
+    d->rh = o->rh + 4 * off; d->pk = o->pk + 4 * off;
+    d->csi = o->csi + 3 * off; d->eta = o->eta + 3 * off; d->zet = o->zet + 3 * off;
+    d->aj = o->aj + off; d->xnut = o->xnut + off;
+    d->qold_af = zalloc(4 * N); d->diss = zalloc(4 * N); d->visc = zalloc(3 * N);
+    d->ucn = zalloc(3 * N); d->fv = zalloc(3 * N); d->dtau = zalloc(N);
+    d->mai = zalloc(16 * N); d->n1i = zalloc(16 * N); d->n2i = zalloc(16 * N); d->mc = zalloc(16 * N);
+    d->spr = zalloc(3 * N);
+    memset(d->rh, 0, 4 * N * sizeof(double));
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++)
+                for (m = 1; m <= 4; m++) d->qold_af[4 * IX(i, j, k) + m - 1] = Q(m, i, j, k);
+    rhs_contra_j(d);
+    rhs_daf_dtau(d);
+    rhs_viscous(d);
+    rhs_diss_p(d);
+    rhs_flux_sans_convec(d);
+    rhs_convec_quick(d, n == 1 ? 2 : 1);
+    rhs_unst_visc_diss(d);
+    if (n > 1) rhs_pk(d, calc_pk == 1 ? 1 : 0);
+    zero_rh_boundaries(d);
+    rhs_modal_matrices(d);
+    rhs_diag_solver(d);
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->idend; i++)
+                for (m = 1; m <= 4; m++) Q(m, i, j, k) = d->qold_af[4 * IX(i, j, k) + m - 1] + RH(m, i, j, k);
+    if (n == 1) bcond_fm(o, d->q);
+    if (recalc == 1) {
+        /* recompute rh at the new q; dtau and diss(1) are NOT recomputed (:432-484) */
+        rhs_contra_j(d);
+        rhs_viscous(d);
+        rhs_flux_sans_convec(d);
+        rhs_convec_quick(d, n > 1 ? 1 : 2);
+        rhs_unst_visc_diss(d);
+        if (n > 1) rhs_pk(d, 0);
+        zero_rh_boundaries(d);
+    }
+    free(d->qold_af); free(d->diss); free(d->visc); free(d->ucn); free(d->fv); free(d->dtau);
+    free(d->mai); free(d->n1i); free(d->n2i); free(d->mc); free(d->spr);
+    return 0;
+}
+
+int or_solver_daf(or_state* o, int n, int decide_recalc_rh, int decide_calc_pk) {
+    if (n < 1 || n > o->ng) return -1;
+    return solver_daf(o, n, decide_recalc_rh, decide_calc_pk);
+}
+
+#undef Q
+#undef QN
+#undef QNM1
+#undef RH
+#undef PK
+#undef UCN
+#undef FV
+#undef VISC
+#undef DISS
+#undef DTAU
+#undef SPR
+#undef MAT
+
+/* ------------------------------------------------------------------ */
+/* des_eddy and the rhs_sa_* procedures (src/des_eddy.F90)              */
+/* ------------------------------------------------------------------ */
+/* Spalart-Allmaras constants (des_eddy.F90:63-73) */
+static const double cb1 = 0.1355, cb2 = 0.622, sig = 2.0 / 3.0, capa = 0.41, cw1 = 3.23906782, cw2 = 0.3,
+                    cw3h = 64.0, cv1c = 357.911, cv2 = 5.0;
+
+typedef struct {
+    const or_state* o;
+    int ni, nj, nk, is, ie, js, je, ks, ke;
+    double dc, de, dz;
+    double* q; /* stride 5, full */
+    const double *qn, *qnm1; /* stride 5, component 5 used */
+    const double *csi, *eta, *zet, *aj, *wd, *dtev;
+    double* xnut;
+    double *ucn, *qold, *visc, *dnu, *psa, *dk, *wo, *fw, *fv, *rh;
+} sa_t;
+
+#define Q(m, i, j, k) d->q[5 * IX(i, j, k) + (m)-1]
+#define Q5(i, j, k) d->q[5 * IX(i, j, k) + 4]
+#define UCN(c, i, j, k) d->ucn[3 * IX(i, j, k) + (c)-1]
+#define S1(a, i, j, k) d->a[IX(i, j, k)]
+#define SLV const int ni = d->ni, nj = d->nj, nk = d->nk
+
+static void sa_contra_j(sa_t* d) {
+    SLV;
+    int i, j, k;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double tmp = 1.0 / AJ(i, j, k);
+                UCN(1, i, j, k) = tmp * (CSI(1, i, j, k) * Q(2, i, j, k) + CSI(2, i, j, k) * Q(3, i, j, k) + CSI(3, i, j, k) * Q(4, i, j, k));
+                UCN(2, i, j, k) = tmp * (ETA(1, i, j, k) * Q(2, i, j, k) + ETA(2, i, j, k) * Q(3, i, j, k) + ETA(3, i, j, k) * Q(4, i, j, k));
+                UCN(3, i, j, k) = tmp * (ZET(1, i, j, k) * Q(2, i, j, k) + ZET(2, i, j, k) * Q(3, i, j, k) + ZET(3, i, j, k) * Q(4, i, j, k));
+            }
+}
+
+/* rhs_sa_source (src/rhs_sa_source.F90:2-95): interior only, early return */
+static void rhs_sa_source(sa_t* d) {
+    SLV;
+    int i, j, k;
+    const double dc2 = pt5 * d->dc, de2 = pt5 * d->de, dz2 = pt5 * d->dz;
+    (void)nk;
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double uc = dc2 * (Q(2, i + 1, j, k) - Q(2, i - 1, j, k));
+                double vc = dc2 * (Q(3, i + 1, j, k) - Q(3, i - 1, j, k));
+                double wc = dc2 * (Q(4, i + 1, j, k) - Q(4, i - 1, j, k));
+                double ue = de2 * (Q(2, i, j + 1, k) - Q(2, i, j - 1, k));
+                double ve = de2 * (Q(3, i, j + 1, k) - Q(3, i, j - 1, k));
+                double we = de2 * (Q(4, i, j + 1, k) - Q(4, i, j - 1, k));
+                double uz = dz2 * (Q(2, i, j, k + 1) - Q(2, i, j, k - 1));
+                double vz = dz2 * (Q(3, i, j, k + 1) - Q(3, i, j, k - 1));
+                double wz = dz2 * (Q(4, i, j, k + 1) - Q(4, i, j, k - 1));
+                double u2 = uc * CSI(2, i, j, k) + ue * ETA(2, i, j, k) + uz * ZET(2, i, j, k);
+                double u3 = uc * CSI(3, i, j, k) + ue * ETA(3, i, j, k) + uz * ZET(3, i, j, k);
+                double v1 = vc * CSI(1, i, j, k) + ve * ETA(1, i, j, k) + vz * ZET(1, i, j, k);
+                double v3 = vc * CSI(3, i, j, k) + ve * ETA(3, i, j, k) + vz * ZET(3, i, j, k);
+                double w1 = wc * CSI(1, i, j, k) + we * ETA(1, i, j, k) + wz * ZET(1, i, j, k);
+                double w2 = wc * CSI(2, i, j, k) + we * ETA(2, i, j, k) + wz * ZET(2, i, j, k);
+                double w12 = pt5 * (u2 - v1), w13 = pt5 * (u3 - w1), w23 = pt5 * (v3 - w2);
+                S1(wo, i, j, k) = 2.0 * (w12 * w12 + w13 * w13 + w23 * w23);
+            }
+}
+
+/* rhs_sa_grdnu (src/rhs_sa_grdnu.F90:1-63) */
+static void rhs_sa_grdnu(sa_t* d) {
+    SLV;
+    int i, j, k;
+    const double dc2 = pt5 * d->dc, de2 = pt5 * d->de, dz2 = pt5 * d->dz;
+    long N = (long)ni * nj * nk;
+    double* q5j = zalloc(N);
+#define Q5J(i, j, k) q5j[IX(i, j, k)]
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) Q5J(i, j, k) = Q5(i, j, k) / AJ(i, j, k);
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double dnu1 = dc2 * (Q5J(i + 1, j, k) * CSI(1, i + 1, j, k) - Q5J(i - 1, j, k) * CSI(1, i - 1, j, k));
+                double dnu2 = dc2 * (Q5J(i + 1, j, k) * CSI(2, i + 1, j, k) - Q5J(i - 1, j, k) * CSI(2, i - 1, j, k));
+                double dnu3 = dc2 * (Q5J(i + 1, j, k) * CSI(3, i + 1, j, k) - Q5J(i - 1, j, k) * CSI(3, i - 1, j, k));
+                double aj2;
+                dnu1 = dnu1 + de2 * (Q5J(i, j + 1, k) * ETA(1, i, j + 1, k) - Q5J(i, j - 1, k) * ETA(1, i, j - 1, k));
+                dnu2 = dnu2 + de2 * (Q5J(i, j + 1, k) * ETA(2, i, j + 1, k) - Q5J(i, j - 1, k) * ETA(2, i, j - 1, k));
+                dnu3 = dnu3 + de2 * (Q5J(i, j + 1, k) * ETA(3, i, j + 1, k) - Q5J(i, j - 1, k) * ETA(3, i, j - 1, k));
+                dnu1 = dnu1 + dz2 * (Q5J(i, j, k + 1) * ZET(1, i, j, k + 1) - Q5J(i, j, k - 1) * ZET(1, i, j, k - 1));
+                dnu2 = dnu2 + dz2 * (Q5J(i, j, k + 1) * ZET(2, i, j, k + 1) - Q5J(i, j, k - 1) * ZET(2, i, j, k - 1));
+                dnu3 = dnu3 + dz2 * (Q5J(i, j, k + 1) * ZET(3, i, j, k + 1) - Q5J(i, j, k - 1) * ZET(3, i, j, k - 1));
+                aj2 = AJ(i, j, k) * AJ(i, j, k);
+                S1(dnu, i, j, k) = aj2 * (dnu1 * dnu1 + dnu2 * dnu2 + dnu3 * dnu3);
+            }
+#undef Q5J
+    free(q5j);
+}
+
+/* rhs_sa_prod (src/rhs_sa_prod.F90:1-105) */
+static void rhs_sa_prod(sa_t* d) {
+    SLV;
+    int i, j, k;
+    const double rei = 1.0 / d->o->p.ren;
+    const double vtc_col = 0.00001, one_sixth = 1.0 / 6.0;
+    (void)nk;
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                if (S1(wd, i, j, k) == 0.0) {
+                    S1(psa, i, j, k) = 0.0;
+                } else {
+                    double q5 = Q5(i, j, k);
+                    double chi = q5 / rei;
+                    double chic = chi * chi * chi;
+                    double fv1 = chic / (chic + cv1c);
+                    double fv2 = pow(1.0 + chi / cv2, -3.0);
+                    double fv3 = ((1.0 + chi * fv1) * (1.0 - fv2)) / chi;
+                    double vtcm = sqrt(2.0 * S1(wo, i, j, k));
+                    double cawd = capa * capa * S1(wd, i, j, k) * S1(wd, i, j, k);
+                    double vtc_org = fv3 * vtcm + q5 * fv2 / cawd;
+                    double vtc = dmax2(vtc_org, vtc_col);
+                    double rr, rr6, grr_org, grr, grr6, vwd;
+                    S1(psa, i, j, k) = cb1 * vtc * q5;
+                    rr = q5 / (vtc * cawd);
+                    rr6 = rr * rr * rr * rr * rr * rr;
+                    grr_org = rr + cw2 * (rr6 - rr);
+                    grr = dmax2(0.0, grr_org);
+                    grr6 = grr * grr * grr * grr * grr * grr;
+                    S1(fw, i, j, k) = grr * pow((1.0 + cw3h) / (grr6 + cw3h), one_sixth);
+                    vwd = q5 / S1(wd, i, j, k);
+                    S1(psa, i, j, k) = S1(psa, i, j, k) - cw1 * S1(fw, i, j, k) * vwd * vwd;
+                    S1(psa, i, j, k) = S1(psa, i, j, k) + cb2 * S1(dnu, i, j, k) / sig;
+                }
+            }
+}
+
+/* rhs_sa_viscous (src/rhs_sa_viscous.F90:2-192) */
+static void rhs_sa_viscous(sa_t* d) {
+    SLV;
+    int i, j, k;
+    const double dc = d->dc, de = d->de, dz = d->dz;
+    const double dc4 = pt25 * dc, de4 = pt25 * de, dz4 = pt25 * dz;
+    const double rei = 1.0 / d->o->p.ren;
+    long N = (long)ni * nj * nk;
+    double* ret = zalloc(N);
+#define RET(i, j, k) ret[IX(i, j, k)]
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) RET(i, j, k) = (rei + Q5(i, j, k)) / sig;
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is - 1; i <= d->ie; i++) {
+                double vcs = dc * (Q5(i + 1, j, k) - Q5(i, j, k));
+                double vet = de4 * (Q5(i + 1, j + 1, k) - Q5(i + 1, j - 1, k) + Q5(i, j + 1, k) - Q5(i, j - 1, k));
+                double vze = dz4 * (Q5(i + 1, j, k + 1) - Q5(i + 1, j, k - 1) + Q5(i, j, k + 1) - Q5(i, j, k - 1));
+                double a1 = CSI(1, i + 1, j, k) + CSI(1, i, j, k), a2 = CSI(2, i + 1, j, k) + CSI(2, i, j, k), a3 = CSI(3, i + 1, j, k) + CSI(3, i, j, k);
+                double b1 = ETA(1, i + 1, j, k) + ETA(1, i, j, k), b2 = ETA(2, i + 1, j, k) + ETA(2, i, j, k), b3 = ETA(3, i + 1, j, k) + ETA(3, i, j, k);
+                double c1 = ZET(1, i + 1, j, k) + ZET(1, i, j, k), c2 = ZET(2, i + 1, j, k) + ZET(2, i, j, k), c3 = ZET(3, i + 1, j, k) + ZET(3, i, j, k);
+                double cc1 = a1 * a1 + a2 * a2 + a3 * a3, cc2 = a1 * b1 + a2 * b2 + a3 * b3, cc3 = a1 * c1 + a2 * c2 + a3 * c3;
+                double vtj = pt25 * (RET(i + 1, j, k) + RET(i, j, k)) / (AJ(i + 1, j, k) + AJ(i, j, k));
+                S1(fv, i, j, k) = vtj * (cc1 * vcs + cc2 * vet + cc3 * vze);
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) S1(visc, i, j, k) = S1(fv, i, j, k) - S1(fv, i - 1, j, k);
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js - 1; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double vcs = dc4 * (Q5(i + 1, j + 1, k) - Q5(i - 1, j + 1, k) + Q5(i + 1, j, k) - Q5(i - 1, j, k));
+                double vet = de * (Q5(i, j + 1, k) - Q5(i, j, k));
+                double vze = dz4 * (Q5(i, j + 1, k + 1) - Q5(i, j + 1, k - 1) + Q5(i, j, k + 1) - Q5(i, j, k - 1));
+                double a1 = CSI(1, i, j + 1, k) + CSI(1, i, j, k), a2 = CSI(2, i, j + 1, k) + CSI(2, i, j, k), a3 = CSI(3, i, j + 1, k) + CSI(3, i, j, k);
+                double b1 = ETA(1, i, j + 1, k) + ETA(1, i, j, k), b2 = ETA(2, i, j + 1, k) + ETA(2, i, j, k), b3 = ETA(3, i, j + 1, k) + ETA(3, i, j, k);
+                double c1 = ZET(1, i, j + 1, k) + ZET(1, i, j, k), c2 = ZET(2, i, j + 1, k) + ZET(2, i, j, k), c3 = ZET(3, i, j + 1, k) + ZET(3, i, j, k);
+                double cc1 = b1 * a1 + b2 * a2 + b3 * a3, cc2 = b1 * b1 + b2 * b2 + b3 * b3, cc3 = b1 * c1 + b2 * c2 + b3 * c3;
+                double vtj = pt25 * (RET(i, j + 1, k) + RET(i, j, k)) / (AJ(i, j + 1, k) + AJ(i, j, k));
+                S1(fv, i, j, k) = vtj * (cc1 * vcs + cc2 * vet + cc3 * vze);
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) S1(visc, i, j, k) = S1(visc, i, j, k) + S1(fv, i, j, k) - S1(fv, i, j - 1, k);
+    for (k = d->ks - 1; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double vcs = dc4 * (Q5(i + 1, j, k + 1) - Q5(i - 1, j, k + 1) + Q5(i + 1, j, k) - Q5(i - 1, j, k));
+                double vet = de4 * (Q5(i, j + 1, k + 1) - Q5(i, j - 1, k + 1) + Q5(i, j + 1, k) - Q5(i, j - 1, k));
+                double vze = dz * (Q5(i, j, k + 1) - Q5(i, j, k));
+                double a1 = CSI(1, i, j, k + 1) + CSI(1, i, j, k), a2 = CSI(2, i, j, k + 1) + CSI(2, i, j, k), a3 = CSI(3, i, j, k + 1) + CSI(3, i, j, k);
+                double b1 = ETA(1, i, j, k + 1) + ETA(1, i, j, k), b2 = ETA(2, i, j, k + 1) + ETA(2, i, j, k), b3 = ETA(3, i, j, k + 1) + ETA(3, i, j, k);
+                double c1 = ZET(1, i, j, k + 1) + ZET(1, i, j, k), c2 = ZET(2, i, j, k + 1) + ZET(2, i, j, k), c3 = ZET(3, i, j, k + 1) + ZET(3, i, j, k);
+                double cc1 = c1 * a1 + c2 * a2 + c3 * a3, cc2 = c1 * b1 + c2 * b2 + c3 * b3, cc3 = c1 * c1 + c2 * c2 + c3 * c3;
+                double vtj = pt25 * (RET(i, j, k + 1) + RET(i, j, k)) / (AJ(i, j, k + 1) + AJ(i, j, k));
+                S1(fv, i, j, k) = vtj * (cc1 * vcs + cc2 * vet + cc3 * vze);
+            }
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) S1(visc, i, j, k) = S1(visc, i, j, k) + S1(fv, i, j, k) - S1(fv, i, j, k - 1);
+#undef RET
+    free(ret);
+}
+
+/* rhs_sa_flux (src/rhs_sa_flux.F90:196-306): first-order upwind FDS */
+static void rhs_sa_flux(sa_t* d) {
+    SLV;
+    int i, j, k, dir;
+    (void)nk;
+    for (dir = 1; dir <= 3; dir++) {
+        const int di = dir == 1, dj = dir == 2, dk = dir == 3;
+        for (k = d->ks - dk; k <= d->ke; k++)
+            for (j = d->js - dj; j <= d->je; j++)
+                for (i = d->is - di; i <= d->ie; i++) {
+                    double u1 = UCN(dir, i + di, j + dj, k + dk), u0 = UCN(dir, i, j, k);
+                    double q1 = Q5(i + di, j + dj, k + dk), q0 = Q5(i, j, k);
+                    double eigh = pt5 * fabs(u1 + u0);
+                    S1(fv, i, j, k) = pt5 * (q1 * u1 + q0 * u0 - eigh * (q1 - q0));
+                }
+        for (k = d->ks; k <= d->ke; k++)
+            for (j = d->js; j <= d->je; j++)
+                for (i = d->is; i <= d->ie; i++) {
+                    if (dir == 1)
+                        S1(rh, i, j, k) = S1(fv, i, j, k) - S1(fv, i - 1, j, k);
+                    else
+                        S1(rh, i, j, k) = S1(rh, i, j, k) + S1(fv, i, j, k) - S1(fv, i - di, j - dj, k - dk);
+                }
+    }
+}
+
+/* rhs_sa_unst_visc (src/rhs_sa_unst_visc.F90:171-202) */
+static void rhs_sa_unst_visc(sa_t* d) {
+    SLV;
+    int i, j, k;
+    const double es = d->o->e_source, delti = d->o->p.delti;
+    (void)nk;
+    for (k = d->ks; k <= d->ke; k++)
+        for (j = d->js; j <= d->je; j++)
+            for (i = d->is; i <= d->ie; i++) {
+                double dtj = 2.0 * delti * AJ(i, j, k);
+                double dq5dt = es * (3.0 * Q5(i, j, k) - 4.0 * d->qn[5 * IX(i, j, k) + 4] + d->qnm1[5 * IX(i, j, k) + 4]) / dtj;
+                S1(rh, i, j, k) = S1(rh, i, j, k) + dq5dt - S1(visc, i, j, k) - S1(psa, i, j, k) / AJ(i, j, k);
+            }
+}
+
+/* rhs_sa_implicit_source (src/rhs_sa_implicit_source.F90:205-258): all nodes */
+static void rhs_sa_implicit_source(sa_t* d) {
+    SLV;
+    int i, j, k;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                S1(dk, i, j, k) = 1.0;
+                S1(rh, i, j, k) = -S1(dtev, i, j, k) * AJ(i, j, k) * S1(rh, i, j, k);
+            }
+}
+
+/* rhs_sa_adi_solver (src/rhs_sa_adi_solver.F90:2-248) */
+static void rhs_sa_adi_solver(sa_t* d) {
+    SLV;
+    int i, j, k, s, dir;
+    const double rei = 1.0 / d->o->p.ren;
+    int maxlen = ni > nj ? ni : nj;
+    double *aw, *ap, *ae, *aq, *av;
+    if (nk > maxlen) maxlen = nk;
+    aw = zalloc(maxlen); ap = zalloc(maxlen); ae = zalloc(maxlen); aq = zalloc(maxlen); av = zalloc(maxlen + 1);
+    for (dir = 1; dir <= 3; dir++) {
+        const int di = dir == 1, dj = dir == 2, dk = dir == 3;
+        const double* met = dir == 1 ? d->csi : dir == 2 ? d->eta : d->zet;
+        int a0, a1, b0, b1, len, s0;
+        /* lines: dir 1 over (j,k); dir 2 over (i,k); dir 3 over (i,j) */
+        if (dir == 1) { a0 = d->js; a1 = d->je; b0 = d->ks; b1 = d->ke; s0 = d->is; len = d->ie - d->is + 1; }
+        else if (dir == 2) { a0 = d->is; a1 = d->ie; b0 = d->ks; b1 = d->ke; s0 = d->js; len = d->je - d->js + 1; }
+        else { a0 = d->is; a1 = d->ie; b0 = d->js; b1 = d->je; s0 = d->ks; len = d->ke - d->ks + 1; }
+        {
+            int la, lb;
+            for (lb = b0; lb <= b1; lb++)
+                for (la = a0; la <= a1; la++) {
+                    /* av(s) for faces s+1/2, s = s0-1 .. s0+len-1 -> av[s - s0 + 1] */
+                    for (s = s0 - 1; s <= s0 + len - 1; s++) {
+                        int i0, j0, k0;
+                        double gx, gy, gz, ev, ucna;
+                        long l0, l1;
+                        if (dir == 1) { i0 = s; j0 = la; k0 = lb; }
+                        else if (dir == 2) { i0 = la; j0 = s; k0 = lb; }
+                        else { i0 = la; j0 = lb; k0 = s; }
+                        l0 = IX(i0, j0, k0);
+                        l1 = IX(i0 + di, j0 + dj, k0 + dk);
+                        gx = met[3 * l1 + 0] + met[3 * l0 + 0];
+                        gy = met[3 * l1 + 1] + met[3 * l0 + 1];
+                        gz = met[3 * l1 + 2] + met[3 * l0 + 2];
+                        ev = (rei + pt5 * (d->q[5 * l1 + 4] + d->q[5 * l0 + 4])) / sig;
+                        ucna = pt5 * fabs(d->ucn[3 * l1 + dir - 1] + d->ucn[3 * l0 + dir - 1]);
+                        av[s - s0 + 1] = ucna + ev * (gx * gx + gy * gy + gz * gz) / (d->aj[l1] + d->aj[l0]);
+                    }
+                    for (s = s0; s <= s0 + len - 1; s++) {
+                        int i0, j0, k0;
+                        long l, lm, lp;
+                        double dtj;
+                        if (dir == 1) { i0 = s; j0 = la; k0 = lb; }
+                        else if (dir == 2) { i0 = la; j0 = s; k0 = lb; }
+                        else { i0 = la; j0 = lb; k0 = s; }
+                        l = IX(i0, j0, k0);
+                        lm = IX(i0 - di, j0 - dj, k0 - dk);
+                        lp = IX(i0 + di, j0 + dj, k0 + dk);
+                        dtj = pt5 * d->dtev[l] * d->aj[l] / d->dk[l];
+                        aw[s - s0] = -dtj * (d->ucn[3 * lm + dir - 1] + av[s - s0]);
+                        ap[s - s0] = 1.0 + dtj * (+av[s - s0] + av[s - s0 + 1]);
+                        ae[s - s0] = dtj * (d->ucn[3 * lp + dir - 1] - av[s - s0 + 1]);
+                        aq[s - s0] = dir == 1 ? d->rh[l] / d->dk[l] : d->rh[l];
+                    }
+                    or_gtsv(len, 1, aw + 1, ap, ae, aq, len);
+                    for (s = s0; s <= s0 + len - 1; s++) {
+                        int i0, j0, k0;
+                        if (dir == 1) { i0 = s; j0 = la; k0 = lb; }
+                        else if (dir == 2) { i0 = la; j0 = s; k0 = lb; }
+                        else { i0 = la; j0 = lb; k0 = s; }
+                        d->rh[IX(i0, j0, k0)] = aq[s - s0];
+                    }
+                }
+        }
+    }
+    free(aw); free(ap); free(ae); free(aq); free(av);
+}
+
+/* rhs_sa_bcond (src/rhs_sa_bcond.F90:2-294) */
+static void rhs_sa_bcond(sa_t* d) {
+    SLV;
+    const or_state* o = d->o;
+    int i, j, k;
+    const double rei = 1.0 / o->p.ren;
+    const int* bt = o->p.btype;
+    /* i = 1 */
+    switch (bt[0]) {
+        case 1: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = 0.0; break;
+        case 2: case 3:
+            for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = o->sa[0][0] * Q5(2, j, k) + o->sb[0][0] * Q5(3, j, k);
+            break;
+        case 6: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = Q5(d->ie, j, k); break;
+        default: break;
+    }
+    switch (bt[1]) {
+        case 1: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = 0.0; break;
+        case 2: case 3:
+            for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = o->sa[1][0] * Q5(ni - 1, j, k) + o->sb[1][0] * Q5(ni - 2, j, k);
+            break;
+        case 6: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(d->is, j, k); break;
+        case 8: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(ni - 1, j, k); break;
+        default: break;
+    }
+    switch (bt[2]) {
+        case 1: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = 0.0; break;
+        case 2: case 3:
+            for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = o->sa[2][0] * Q5(i, 2, k) + o->sb[2][0] * Q5(i, 3, k);
+            break;
+        case 6: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = Q5(i, d->je, k); break;
+        default: break;
+    }
+    switch (bt[3]) {
+        case 1: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = 0.0; break;
+        case 2: case 3:
+            for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = o->sa[3][0] * Q5(i, nj - 1, k) + o->sb[3][0] * Q5(i, nj - 2, k);
+            break;
+        case 6: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = Q5(i, d->js, k); break;
+        default: break;
+    }
+    switch (bt[4]) {
+        case 1: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = 0.0; break;
+        case 2: case 3:
+            for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = o->sa[4][0] * Q5(i, j, 2) + o->sb[4][0] * Q5(i, j, 3);
+            break;
+        case 6: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = Q5(i, j, d->ke); break;
+        default: break;
+    }
+    switch (bt[5]) {
+        case 1: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = 0.0; break;
+        case 2: case 3:
+            for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = o->sa[5][0] * Q5(i, j, nk - 1) + o->sb[5][0] * Q5(i, j, nk - 2);
+            break;
+        case 6: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = Q5(i, j, d->ks); break;
+        default: break;
+    }
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double chi, chic, fv1;
+                Q5(i, j, k) = dmax2(1.0e-07, Q5(i, j, k));
+                chi = Q5(i, j, k) / rei;
+                chic = chi * chi * chi;
+                fv1 = chic / (chic + cv1c);
+                d->xnut[IX(i, j, k)] = Q5(i, j, k) * fv1;
+            }
+}
+
+static void des_eddy(or_state* o) {
+    sa_t dd, *d = &dd;
+    LEVEL(1);
+    long N = (long)ni * nj * nk;
+    int i, j, k, iterk;
+    memset(d, 0, sizeof(*d));
+    d->o = o; d->ni = ni; d->nj = nj; d->nk = nk;
+    d->is = 2; d->ie = ni - 1; d->js = 2; d->je = nj - 1; d->ks = 2; d->ke = nk - 1;
+    d->dc = o->dc[1]; d->de = o->de[1]; d->dz = o->dz[1];
+    d->q = o->q + 5 * off; d->qn = o->qn + 5 * off; d->qnm1 = o->qnm1 + 5 * off;
+    d->csi = o->csi + 3 * off; d->eta = o->eta + 3 * off; d->zet = o->zet + 3 * off;
+    d->aj = o->aj + off; d->wd = o->wd + off; d->dtev = o->dtev + off; d->xnut = o->xnut + off;
+    d->ucn = zalloc(3 * N); d->qold = zalloc(N); d->visc = zalloc(N); d->dnu = zalloc(N); d->psa = zalloc(N);
+    d->dk = zalloc(N); d->wo = zalloc(N); d->fw = zalloc(N); d->fv = zalloc(N); d->rh = zalloc(N);
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) S1(qold, i, j, k) = Q5(i, j, k);
+    sa_contra_j(d);
+    rhs_sa_source(d);
+    for (iterk = 1; iterk <= o->p.itk; iterk++) {
+        rhs_sa_grdnu(d);
+        rhs_sa_prod(d);
+        rhs_sa_viscous(d);
+        rhs_sa_flux(d);
+        rhs_sa_unst_visc(d);
+        rhs_sa_implicit_source(d);
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++) S1(rh, 1, j, k) = 0.0, S1(rh, ni, j, k) = 0.0;
+        for (k = 1; k <= nk; k++)
+            for (i = 1; i <= ni; i++) S1(rh, i, 1, k) = 0.0, S1(rh, i, nj, k) = 0.0;
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) S1(rh, i, j, 1) = 0.0, S1(rh, i, j, nk) = 0.0;
+        rhs_sa_adi_solver(d);
+        for (k = d->ks; k <= d->ke; k++)
+            for (j = d->js; j <= d->je; j++)
+                for (i = d->is; i <= d->ie; i++) {
+                    double qe5 = S1(qold, i, j, k) + S1(rh, i, j, k);
+                    double dum = 1.0 + onept5 * o->e_source * S1(dtev, i, j, k) / o->p.delti;
+                    Q5(i, j, k) = Q5(i, j, k) + (-Q5(i, j, k) + qe5) / dum;
+                }
+        rhs_sa_bcond(d);
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) S1(qold, i, j, k) = Q5(i, j, k);
+    }
+    free(d->ucn); free(d->qold); free(d->visc); free(d->dnu); free(d->psa);
+    free(d->dk); free(d->wo); free(d->fw); free(d->fv); free(d->rh);
+}
+
+int or_des_eddy(or_state* o) {
+    if (o->img[1] < 5 || o->jmg[1] < 5 || o->kmg[1] < 5) return -1;
+    des_eddy(o);
+    return 0;
+}
+
+#undef Q
+#undef Q5
+#undef UCN
+#undef S1
+#undef CSI
+#undef ETA
+#undef ZET
+#undef AJ
+#undef XNUT
+
+/* ------------------------------------------------------------------ */
+/* multigrid transfers                                                  */
+/* ------------------------------------------------------------------ */
+static long gidx(const or_state* o, int n, int i, int j, int k) {
+    return o->off[n] + (i - 1) + (long)o->img[n] * ((j - 1) + (long)o->jmg[n] * (k - 1));
+}
+
+/* mg_zero_pk_boundary (src/mg_driver.F90:407-466) */
+static void mg_zero_pk_boundary(or_state* o, int n) {
+    LEVEL(n);
+    int i, j, k, m;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (m = 0; m < 4; m++) o->pk[4 * gidx(o, n, 1, j, k) + m] = 0.0, o->pk[4 * gidx(o, n, ni, j, k) + m] = 0.0;
+    for (k = 1; k <= nk; k++)
+        for (i = 1; i <= ni; i++)
+            for (m = 0; m < 4; m++) o->pk[4 * gidx(o, n, i, 1, k) + m] = 0.0, o->pk[4 * gidx(o, n, i, nj, k) + m] = 0.0;
+    for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++)
+            for (m = 0; m < 4; m++) o->pk[4 * gidx(o, n, i, j, 1) + m] = 0.0, o->pk[4 * gidx(o, n, i, j, nk) + m] = 0.0;
+}
+
+/* mg_nlevel (src/mg_nlevel.F90:2-150).  The 27-point restriction is
+ * evaluated at interior coarse nodes only; the reference reads outside the
+ * fine level at boundary coarse nodes and those values are zeroed right
+ * after by mg_zero_pk_boundary (src/mg_driver.F90:134), so they are set to
+ * zero here. */
+static void mg_nlevel(or_state* o, int n) {
+    LEVEL(n);
+    int i, j, k, m;
+    const long ic = 1, ie = o->img[n - 1], iz = (long)o->img[n - 1] * o->jmg[n - 1];
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                int kk = (1 << o->p.kcrs) * k - o->p.kcrs, jj = (1 << o->p.jcrs) * j - o->p.jcrs, ii = (1 << o->p.icrs) * i - o->p.icrs;
+                long ll = gidx(o, n - 1, ii, jj, kk), l = gidx(o, n, i, j, k);
+                for (m = 0; m < 4; m++) o->q[5 * l + m] = o->q[5 * ll + m];
+                o->xnut[l] = o->xnut[ll];
+            }
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                long l = gidx(o, n, i, j, k);
+                for (m = 0; m < 4; m++) o->qold_mg[5 * l + m] = o->q[5 * l + m];
+            }
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                int kk = (1 << o->p.kcrs) * k - o->p.kcrs, jj = (1 << o->p.jcrs) * j - o->p.jcrs, ii = (1 << o->p.icrs) * i - o->p.icrs;
+                long ll = gidx(o, n - 1, ii, jj, kk), l = gidx(o, n, i, j, k);
+                if (i == 1 || i == ni || j == 1 || j == nj || k == 1 || k == nk) {
+                    for (m = 0; m < 4; m++) o->pk[4 * l + m] = 0.0;
+                    continue;
+                }
+                for (m = 0; m < 4; m++) {
+                    const double* r = o->rh + m;
+#define R(off) r[4 * (ll + (off))]
+                    o->pk[4 * l + m] =
+                        (pt25 * R(ic + ie + iz) + pt5 * R(ic + ie) + pt25 * R(ic + ie - iz) +
+                         pt5 * R(ie + iz) + R(ie) + pt5 * R(ie - iz) +
+                         pt25 * R(-ic + ie + iz) + pt5 * R(-ic + ie) + pt25 * R(-ic + ie - iz) +
+                         pt5 * R(ic + iz) + R(ic) + pt5 * R(ic - iz) +
+                         R(iz) + 2.0 * R(0) + R(-iz) +
+                         pt5 * R(-ic + iz) + R(-ic) + pt5 * R(-ic - iz) +
+                         pt25 * R(ic - ie + iz) + pt5 * R(ic - ie) + pt25 * R(ic - ie - iz) +
+                         pt5 * R(-ie + iz) + R(-ie) + pt5 * R(-ie - iz) +
+                         pt25 * R(-ic - ie + iz) + pt5 * R(-ic - ie) + pt25 * R(-ic - ie - iz)) *
+                        (1.0 / 16.0);
+#undef R
+                }
+            }
+}
+
+int or_mg_nlevel(or_state* o, int n) {
+    if (n < 2 || n > o->ng) return -1;
+    mg_nlevel(o, n);
+    mg_zero_pk_boundary(o, n);
+    return 0;
+}
+
+/* rs_sweep (src/rs_sweep.F90:111-291): constant-coefficient implicit
+ * smoothing of dq on interior nodes, with SGTSV and NRHS = 4 */
+static void rs_sweep(or_state* o, int n, double* dq /* level n, stride 4 */) {
+    LEVEL(n);
+    int i, j, k, m, s, dir;
+    int maxlen = ni > nj ? ni : nj;
+    double *ax, *bx, *cx, *dx;
+    if (nk > maxlen) maxlen = nk;
+    ax = zalloc(maxlen); bx = zalloc(maxlen); cx = zalloc(maxlen); dx = zalloc(4 * maxlen);
+    for (dir = 1; dir <= 3; dir++) {
+        const double epn = o->p.ep[1][dir - 1];
+        int len = (dir == 1 ? ni : dir == 2 ? nj : nk) - 2;
+        int a, b, na = dir == 1 ? nj : ni, nb = dir == 3 ? nj : nk;
+        for (b = 2; b <= nb - 1; b++)
+            for (a = 2; a <= na - 1; a++) {
+                /* line nodes: dir 1 (s+2, a, b), dir 2 (a, s+2, b), dir 3 (a, b, s+2) */
+                for (s = 0; s < len; s++) {
+                    int ii = dir == 1 ? s + 2 : a, jj = dir == 1 ? a : dir == 2 ? s + 2 : b, kk = dir == 3 ? s + 2 : b;
+                    for (m = 0; m < 4; m++) dx[m * len + s] = dq[4 * IX(ii, jj, kk) + m];
+                    ax[s] = -epn;
+                    bx[s] = 1.0 + 2.0 * epn;
+                    cx[s] = -epn;
+                }
+                or_gtsv(len, 4, ax + 1, bx, cx, dx, len);
+                for (s = 0; s < len; s++) {
+                    int ii = dir == 1 ? s + 2 : a, jj = dir == 1 ? a : dir == 2 ? s + 2 : b, kk = dir == 3 ? s + 2 : b;
+                    for (m = 0; m < 4; m++) dq[4 * IX(ii, jj, kk) + m] = dx[m * len + s];
+                }
+            }
+    }
+    (void)i; (void)j; (void)k;
+    free(ax); free(bx); free(cx); free(dx);
+}
+
+/* mg_prolong (src/mg_prolong.F90:1-281) */
+static void mg_prolong(or_state* o, int n) {
+    LEVEL(n);
+    long N = (long)ni * nj * nk;
+    double* dq = zalloc(4 * N);
+    int i, j, k, m;
+    const long ic = 1, ie = ni, iz = (long)ni * nj;
+    const int icrs = o->p.icrs, jcrs = o->p.jcrs, kcrs = o->p.kcrs;
+    /* coarse correction placed on the coincident fine nodes (:57-75) */
+    for (k = 1; k <= o->kmg[n + 1]; k++)
+        for (j = 1; j <= o->jmg[n + 1]; j++)
+            for (i = 1; i <= o->img[n + 1]; i++) {
+                int kk = (1 << kcrs) * k - kcrs, jj = (1 << jcrs) * j - jcrs, ii = (1 << icrs) * i - icrs;
+                long ll = IX(ii, jj, kk), l = gidx(o, n + 1, i, j, k);
+                for (m = 0; m < 4; m++) dq[4 * ll + m] = o->q[5 * l + m] - o->qold_mg[5 * l + m];
+            }
+    /* linear fill in k, then j, then i (:93-170) */
+    if (kcrs == 1)
+        for (k = 2; k <= nk - 1; k += 2)
+            for (j = 1; j <= nj; j += 1 << jcrs)
+                for (i = 1; i <= ni; i += 1 << icrs) {
+                    long l = IX(i, j, k);
+                    for (m = 0; m < 4; m++) dq[4 * l + m] = pt5 * dq[4 * (l + iz) + m] + pt5 * dq[4 * (l - iz) + m];
+                }
+    if (jcrs == 1)
+        for (k = 1; k <= nk; k++)
+            for (j = 2; j <= nj - 1; j += 2)
+                for (i = 1; i <= ni; i += 1 << icrs) {
+                    long l = IX(i, j, k);
+                    for (m = 0; m < 4; m++) dq[4 * l + m] = pt5 * dq[4 * (l + ie) + m] + pt5 * dq[4 * (l - ie) + m];
+                }
+    if (icrs == 1)
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++)
+                for (i = 2; i <= ni - 1; i += 2) {
+                    long l = IX(i, j, k);
+                    for (m = 0; m < 4; m++) dq[4 * l + m] = pt5 * dq[4 * (l + ic) + m] + pt5 * dq[4 * (l - ic) + m];
+                }
+    rs_sweep(o, n, dq);
+    /* dq = 0 on the boundary planes (:218-270) */
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (m = 0; m < 4; m++) dq[4 * IX(1, j, k) + m] = 0.0, dq[4 * IX(ni, j, k) + m] = 0.0;
+    for (k = 1; k <= nk; k++)
+        for (i = 1; i <= ni; i++)
+            for (m = 0; m < 4; m++) dq[4 * IX(i, 1, k) + m] = 0.0, dq[4 * IX(i, nj, k) + m] = 0.0;
+    for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++)
+            for (m = 0; m < 4; m++) dq[4 * IX(i, j, 1) + m] = 0.0, dq[4 * IX(i, j, nk) + m] = 0.0;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                long l = off + IX(i, j, k);
+                for (m = 0; m < 4; m++) o->q[5 * l + m] = o->q[5 * l + m] + dq[4 * IX(i, j, k) + m];
+            }
+    free(dq);
+}
+
+int or_mg_prolong(or_state* o, int n) {
+    if (n < 1 || n >= o->ng) return -1;
+    mg_prolong(o, n);
+    return 0;
+}
+
+/* mg_residual (src/mg_residual.F90:2-104): log10 of the L1 mean of the
+ * pseudo-time update over fine interior nodes, loop order k, j, i */
+int or_mg_residual(const or_state* o, double* residual) {
+    LEVEL(1);
+    double resd[5] = {0, 0, 0, 0, 0};
+    double tnu = (double)((long)(ni - 2) * (nj - 2) * (nk - 2));
+    int i, j, k, m;
+    for (k = 2; k <= nk - 1; k++)
+        for (j = 2; j <= nj - 1; j++)
+            for (i = 2; i <= ni - 1; i++) {
+                long l = off + IX(i, j, k);
+                for (m = 0; m < 5; m++) resd[m] = resd[m] + fabs(o->q[5 * l + m] - o->qold_mg[5 * l + m]);
+            }
+    for (m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
+    return 0;
+}
+
+/* mg_driver (src/mg_driver.F90:2-323) with mg_ck_conver (:335-405) */
+int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
+    const int ng = o->ng, ns = 1;
+    long n1 = o->nl[1], l;
+    int m, it, n, itr, decide_recalc_rh = 0;
+    if (o->img[ng] < 5 || o->jmg[ng] < 5 || o->kmg[ng] < 5) return -1;
+    o->itc = 0;
+    /* qnm1 <- qn, qn <- q on the fine level (:48-51) */
+    for (l = 0; l < 5 * n1; l++) {
+        o->qnm1[l] = o->qn[l];
+        o->qn[l] = o->q[l];
+    }
+    for (m = 0; m < 5; m++) {
+        mg_inject(o, o->qn, 5, m);
+        mg_inject(o, o->qnm1, 5, m);
+    }
+    for (it = 1; it <= o->p.itm; it++) {
+        double residual[5], eomax, ermax;
+        int converged = 0;
+        o->itc++;
+        memcpy(o->qold_mg, o->q, 5 * n1 * sizeof(double));
+        for (n = ns; n <= ng; n++) {
+            if (n > 1) {
+                mg_nlevel(o, n);
+                mg_zero_pk_boundary(o, n);
+            }
+            for (itr = 1; itr <= o->p.iter[n - 1]; itr++) {
+                if (n != ng) decide_recalc_rh = (itr != o->p.iter[n - 1]) ? 0 : 1;
+                if (o->p.des && n == ns) des_eddy(o);
+                solver_daf(o, n, decide_recalc_rh, itr);
+            }
+            mg_zero_pk_boundary(o, n);
+        }
+        for (n = ng - 1; n >= ns; n--) mg_prolong(o, n);
+        bcond_fm(o, o->q);
+        or_mg_residual(o, residual);
+        if (resid_hist)
+            for (m = 0; m < 5; m++) resid_hist[5 * (it - 1) + m] = residual[m];
+        if (o->itc == 1)
+            for (m = 0; m < 5; m++) o->init_resd[m] = residual[m];
+        eomax = dmax2(dmax2(residual[1], residual[2]), residual[3]);
+        ermax = dmax2(dmax2(residual[1] - o->init_resd[1], residual[2] - o->init_resd[2]), residual[3] - o->init_resd[3]);
+        if (o->itc > o->p.it_min && ermax < o->p.er_min) converged = 1;
+        if (o->itc > o->p.it_min && eomax < o->p.eo_min) converged = 1;
+        if (converged) break;
+    }
+    if (itc_out) *itc_out = o->itc;
+    return 0;
+}

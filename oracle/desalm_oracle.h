@@ -1,0 +1,62 @@
+/*
+ * desalm_oracle.h -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * CPU FP64 restatement of the reference's pseudo-time inner iteration, used
+ * only as the checker by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg.  Nothing in des-alm_amd/ may include, link or call it.
+ *
+ * The oracle keeps the reference's own module-state layout: arrays over all
+ * multigrid levels concatenated (le_idx numbering, src/init.F90:880-980),
+ * array-of-structures components (q(1:5,l), rh(1:4,l), csi(1:3,l)).
+ */
+#ifndef DESALM_ORACLE_H
+#define DESALM_ORACLE_H
+
+#include "../include/desalm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_state or_state;
+
+or_state* or_create(const desalm_params* p);
+void or_destroy(or_state* o);
+long or_nall(const or_state* o);
+long or_level_offset(const or_state* o, int n);
+
+/* mg_inject(x,y,z) + mg_metrics on all levels (src/mg_metrics.F90:56-300). */
+int or_set_grid(or_state* o, const double* x, const double* y, const double* z);
+/* fine-level wall distance; wd of the coarse levels stays zero (init.F90:2313) */
+int or_set_wd(or_state* o, const double* wd_fine);
+/* fine-level state (AoS q(1:5,n1) etc.); NULL leaves a field as it is */
+int or_set_state(or_state* o, const double* q, const double* qn, const double* qnm1,
+                 const double* xnut);
+int or_get_state(const or_state* o, double* q, double* qn, double* qnm1, double* xnut);
+
+/* whole-array access, all levels, reference layout.
+ * names: q qn qnm1 qold_mg (5) rh pk (4) csi eta zet (3) aj xnut wd dtev x y z (1) */
+int or_get_array(const or_state* o, const char* name, double* out);
+int or_set_array(or_state* o, const char* name, const double* in);
+
+/* one physical step of mg_driver (src/mg_driver.F90:2-323);
+ * resid: itm*5 log10 norms (may be NULL) */
+int or_mg_driver(or_state* o, int* itc, double* resid);
+/* one DAF pass on level n of the stored state (src/solver_daf.F90) */
+int or_solver_daf(or_state* o, int n, int decide_recalc_rh, int decide_calc_pk);
+/* one des_eddy call on the fine level of the stored state (src/des_eddy.F90) */
+int or_des_eddy(or_state* o);
+/* mg_nlevel(n) + mg_zero_pk_boundary (src/mg_nlevel.F90:2-150) */
+int or_mg_nlevel(or_state* o, int n);
+/* mg_prolong(n) (src/mg_prolong.F90:1-281) */
+int or_mg_prolong(or_state* o, int n);
+/* mg_residual(1) (src/mg_residual.F90:2-104): 5 log10 norms */
+int or_mg_residual(const or_state* o, double* residual);
+
+/* LAPACK SGTSV algorithm in FP64 (src/sgtsv.f:1-263); returns INFO */
+int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

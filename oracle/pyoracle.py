@@ -1,0 +1,240 @@
+"""TEST INFRASTRUCTURE: Python bindings of the CPU oracle (liboracle.so) and a
+runner for the reference Fortran harness (oracle/_ref/desalm_ref_harness).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, and only as the checker of the HIP path.
+"""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_EXE = os.path.join(HERE, "_ref", "desalm_ref_harness")
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so (gcc) if it is missing or stale."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.or_create.restype = vp
+        L.or_create.argtypes = [vp]
+        L.or_destroy.argtypes = [vp]
+        L.or_nall.restype = ctypes.c_long
+        L.or_nall.argtypes = [vp]
+        L.or_level_offset.restype = ctypes.c_long
+        L.or_level_offset.argtypes = [vp, ctypes.c_int]
+        for f in ("or_set_grid",):
+            getattr(L, f).argtypes = [vp, dp, dp, dp]
+        L.or_set_wd.argtypes = [vp, dp]
+        L.or_set_state.argtypes = [vp, dp, dp, dp, dp]
+        L.or_get_state.argtypes = [vp, dp, dp, dp, dp]
+        L.or_get_array.argtypes = [vp, ctypes.c_char_p, dp]
+        L.or_set_array.argtypes = [vp, ctypes.c_char_p, dp]
+        L.or_mg_driver.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
+        L.or_solver_daf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_des_eddy.argtypes = [vp]
+        L.or_mg_nlevel.argtypes = [vp, ctypes.c_int]
+        L.or_mg_prolong.argtypes = [vp, ctypes.c_int]
+        L.or_mg_residual.argtypes = [vp, dp]
+        L.or_gtsv.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
+        for f in ("or_set_grid", "or_set_wd", "or_set_state", "or_get_state", "or_get_array",
+                  "or_set_array", "or_mg_driver", "or_solver_daf", "or_des_eddy", "or_mg_nlevel",
+                  "or_mg_prolong", "or_mg_residual", "or_gtsv"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+NCOMP = {"q": 5, "qn": 5, "qnm1": 5, "qold_mg": 5, "rh": 4, "pk": 4, "csi": 3, "eta": 3,
+         "zet": 3, "aj": 1, "xnut": 1, "wd": 1, "dtev": 1, "x": 1, "y": 1, "z": 1}
+
+
+class Oracle:
+    """One oracle state: module-like arrays over all levels."""
+
+    def __init__(self, params):
+        self.params = params.copy()
+        self.h = lib().or_create(ctypes.byref(self.params))
+        if not self.h:
+            raise ValueError("or_create rejected the parameters")
+        self.nall = lib().or_nall(self.h)
+        self.n1 = params.img * params.jmg * params.kmg
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.or_destroy(h)
+            self.h = None
+
+    def level_offset(self, n):
+        return lib().or_level_offset(self.h, n)
+
+    def load_case(self, case):
+        lib().or_set_grid(self.h, _p(np.ascontiguousarray(case.x)), _p(np.ascontiguousarray(case.y)),
+                          _p(np.ascontiguousarray(case.z)))
+        lib().or_set_wd(self.h, _p(np.ascontiguousarray(case.wd)))
+        q, qn, qnm1, xnut = case.state()
+        self.set_state(q, qn, qnm1, xnut)
+        return self
+
+    def set_state(self, q=None, qn=None, qnm1=None, xnut=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        q, qn, qnm1, xnut = c(q), c(qn), c(qnm1), c(xnut)
+        lib().or_set_state(self.h, _p(q), _p(qn), _p(qnm1), _p(xnut))
+
+    def get_state(self):
+        q = np.zeros((self.n1, 5)); qn = np.zeros((self.n1, 5)); qnm1 = np.zeros((self.n1, 5))
+        xnut = np.zeros(self.n1)
+        lib().or_get_state(self.h, _p(q), _p(qn), _p(qnm1), _p(xnut))
+        return q, qn, qnm1, xnut
+
+    def get(self, name):
+        c = NCOMP[name]
+        out = np.zeros((self.nall, c)) if c > 1 else np.zeros(self.nall)
+        rc = lib().or_get_array(self.h, name.encode(), _p(out))
+        assert rc == 0, name
+        return out
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == NCOMP[name] * self.nall
+        assert lib().or_set_array(self.h, name.encode(), _p(a)) == 0
+
+    def mg_driver(self):
+        itc = ctypes.c_int(0)
+        hist = np.zeros((max(self.params.itm, 1), 5))
+        rc = lib().or_mg_driver(self.h, ctypes.byref(itc), _p(hist))
+        assert rc == 0
+        return itc.value, hist[: itc.value]
+
+    def solver_daf(self, n, recalc, calc_pk):
+        assert lib().or_solver_daf(self.h, n, recalc, calc_pk) == 0
+
+    def des_eddy(self):
+        assert lib().or_des_eddy(self.h) == 0
+
+    def mg_nlevel(self, n):
+        assert lib().or_mg_nlevel(self.h, n) == 0
+
+    def mg_prolong(self, n):
+        assert lib().or_mg_prolong(self.h, n) == 0
+
+    def residual(self):
+        r = np.zeros(5)
+        lib().or_mg_residual(self.h, _p(r))
+        return r
+
+
+def gtsv(dl, d, du, b):
+    """SGTSV algorithm in FP64 on copies; returns (x, info)."""
+    n = d.shape[0]
+    b = np.array(b, dtype=np.float64, order="F", copy=True)
+    nrhs = 1 if b.ndim == 1 else b.shape[1]
+    dl = np.ascontiguousarray(dl, dtype=np.float64).copy()
+    d = np.ascontiguousarray(d, dtype=np.float64).copy()
+    du = np.ascontiguousarray(du, dtype=np.float64).copy()
+    bf = np.asfortranarray(b)
+    info = lib().or_gtsv(n, nrhs, _p(dl) if n > 1 else _p(np.zeros(1)), _p(d),
+                         _p(du) if n > 1 else _p(np.zeros(1)),
+                         bf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), max(n, 1))
+    return bf, info
+
+
+# ---------------------------------------------------------------------------
+# reference Fortran harness
+# ---------------------------------------------------------------------------
+def ref_available():
+    return os.path.exists(REF_EXE)
+
+
+def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600):
+    """Run the reference's own Fortran (compiled from its sources by
+    oracle/ref/build_ref.sh) on `case`.  The reference is built with three
+    multigrid levels (src/global.F90:17); a two-level case runs with iter(3)=0,
+    which leaves level 2 untouched by the third level's zero correction.
+    Returns a dict of fine-level arrays and all-level metrics."""
+    p = case.params
+    if p.ng > 3:
+        raise ValueError("the reference is compiled for 3 levels")
+    hdr = np.zeros(40, dtype=np.int32)
+    hdr[0] = mode
+    hdr[2:5] = (p.img, p.jmg, p.kmg)
+    hdr[5:8] = (p.icrs, p.jcrs, p.kcrs)
+    it = list(iters) if iters is not None else [p.iter[n] if n < p.ng else 0 for n in range(3)]
+    hdr[8:11] = it
+    hdr[11] = p.itm
+    hdr[12] = p.it_min
+    hdr[13] = p.unsteady
+    hdr[14] = p.des
+    hdr[15] = p.itk
+    hdr[16:22] = list(p.btype)
+    hdr[22:28] = list(p.bdir)
+    hdr[28:31] = (p.ifix, p.jfix, p.kfix)
+    hdr[31] = nsteps
+    hdr[32] = daf_level
+    hdr[33] = recalc
+    hdr[34] = calc_pk
+    rhd = np.zeros(20)
+    rhd[:10] = (p.er_min, p.eo_min, p.ren, p.beta, p.delti, p.cfl1, p.vnn1, p.cfl2, p.vnn2, p.pdiss_coef)
+    rhd[10:13] = list(p.ep[0])
+    rhd[13:16] = list(p.ep[1])
+    n1 = p.img * p.jmg * p.kmg
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            hdr.tofile(f)
+            rhd.tofile(f)
+            for a in (case.x, case.y, case.z, case.wd, case.q, case.qn, case.qnm1, case.xnut):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        env = dict(os.environ)
+        subprocess.run([REF_EXE, fin, fout], check=True, cwd=td, timeout=timeout, env=env,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        with open(fout, "rb") as f:
+            nall, n1r = np.fromfile(f, dtype=np.int32, count=2)
+            assert n1r == n1
+            out = {}
+            out["q"] = np.fromfile(f, dtype=np.float64, count=5 * n1).reshape(n1, 5)
+            out["qn"] = np.fromfile(f, dtype=np.float64, count=5 * n1).reshape(n1, 5)
+            out["qnm1"] = np.fromfile(f, dtype=np.float64, count=5 * n1).reshape(n1, 5)
+            out["xnut"] = np.fromfile(f, dtype=np.float64, count=n1)
+            out["rh"] = np.fromfile(f, dtype=np.float64, count=4 * n1).reshape(n1, 4)
+            out["csi"] = np.fromfile(f, dtype=np.float64, count=3 * nall).reshape(nall, 3)
+            out["eta"] = np.fromfile(f, dtype=np.float64, count=3 * nall).reshape(nall, 3)
+            out["zet"] = np.fromfile(f, dtype=np.float64, count=3 * nall).reshape(nall, 3)
+            out["aj"] = np.fromfile(f, dtype=np.float64, count=nall)
+            out["dtev"] = np.fromfile(f, dtype=np.float64, count=nall)
+    return out
+
+
+def relerr(a, b):
+    """Normwise relative difference max|a-b| / max|b| (0 if both are 0)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.max(np.abs(b))
+    num = np.max(np.abs(a - b))
+    if den == 0.0:
+        return float(num)
+    return float(num / den)
