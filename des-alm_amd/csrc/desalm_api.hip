@@ -1,0 +1,538 @@
+// desalm_api.hip -- C ABI of libdesalm.so: context, device memory, the
+// drop-in entry points for mg_driver / solver_daf / des_eddy and the
+// V-cycle orchestration (src/mg_driver.F90:2-323).
+//
+// All state of a context lives on its device for the context's lifetime;
+// host arrays are only borrowed for the duration of a call.  Every kernel is
+// enqueued on one stream; the only host synchronisation inside
+// desalm_mg_driver is the 5-double read-back of the update norms per
+// pseudo-iteration that the convergence test needs.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "desalm_internal.h"
+
+using namespace desalm;
+
+struct desalm_ctx {
+    desalm_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ng = 0;
+    Level lv[kMaxLevels + 1];
+    long nall = 0, n1 = 0, maxdim = 0;
+    Consts C;
+    Fields F;
+    std::vector<void*> allocs;
+    double* rs_tab = nullptr;      // per prolongation level: 3 directions x 2*maxdim
+    double* staging = nullptr;     // 5 * nall doubles: AoS transfers
+    double* dq_buf = nullptr;      // 4 * n1 doubles: dq of the solver_daf parity entry
+    std::string err;
+    double last_ms = 0.0;
+    unsigned long long last_pivots = 0;
+    double init_resd[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;  // errors without a context (init, gtsv)
+
+int fail(desalm_ctx* c, const std::string& msg) {
+    if (c) c->err = msg;
+    g_err = msg;
+    return -1;
+}
+
+#define HIPCHK(c, x)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            return fail((c), std::string(#x " failed: ") + hipGetErrorString(e_));        \
+    } while (0)
+
+double* dalloc(desalm_ctx* c, long n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
+    hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(double));
+    c->allocs.push_back(p);
+    return (double*)p;
+}
+
+int validate(const desalm_params* p, std::string& why) {
+    if (p->ng < 1 || p->ng > kMaxLevels) { why = "ng must be in 1..6"; return -1; }
+    if ((p->icrs | p->jcrs | p->kcrs) & ~1) { why = "icrs/jcrs/kcrs must be 0 or 1"; return -1; }
+    int i = p->img, j = p->jmg, k = p->kmg;
+    for (int n = 1; n <= p->ng; n++) {
+        if (i < 5 || j < 5 || k < 5) { why = "every level needs at least 5 nodes per direction"; return -1; }
+        if (n < p->ng) {
+            if ((p->icrs && !(i & 1)) || (p->jcrs && !(j & 1)) || (p->kcrs && !(k & 1))) {
+                why = "coarsened directions need an odd node count on every level but the coarsest";
+                return -1;
+            }
+            i = (i + p->icrs) / (1 << p->icrs);
+            j = (j + p->jcrs) / (1 << p->jcrs);
+            k = (k + p->kcrs) / (1 << p->kcrs);
+        }
+    }
+    for (int b = 0; b < 6; b++)
+        if (p->btype[b] == DESALM_BC_OUTLET_MOC || p->btype[b] < 0 || p->btype[b] > 8) {
+            why = "unsupported boundary type (characteristic outlet, btype 5, is out of scope)";
+            return -1;
+        }
+    if (p->ifix < 1 || p->ifix > p->img || p->jfix < 1 || p->jfix > p->jmg || p->kfix < 1 || p->kfix > p->kmg) {
+        why = "pinned-pressure node outside the fine level";
+        return -1;
+    }
+    if (p->itk < 0 || p->itm < 0) { why = "itk and itm must be non-negative"; return -1; }
+    if (p->precision != DESALM_PREC_FP64) { why = "only DESALM_PREC_FP64 is implemented"; return -1; }
+    return 0;
+}
+
+// SGTSV factorisation of the constant smoothing matrix (-ep, 1+2ep, -ep)
+// of length len: fact[0..len-2] then d[0..len-1] (src/sgtsv.f:166-182)
+void rs_factor(double ep, int len, double* out) {
+    double* fact = out;
+    double* d = out + (len - 1);
+    for (int s = 0; s < len; s++) d[s] = 1.0 + 2.0 * ep;
+    for (int s = 0; s < len - 1; s++) {
+        fact[s] = (-ep) / d[s];
+        d[s + 1] = d[s + 1] - fact[s] * (-ep);
+    }
+}
+
+void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out) {
+    const Level& L = c->lv[n];
+    launch_dtau(c->stream, L, c->F, c->C);
+    launch_residual(c->stream, L, c->F, c->C, n, 0, calc_pk);
+    launch_daf_i(c->stream, L, c->F, c->C, n);
+    launch_daf_j(c->stream, L, c->F, c->C, n);
+    launch_daf_k(c->stream, L, c->F, c->C, n, dq_out);
+    if (n == 1) launch_bcond_fm(c->stream, L, c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix);
+    if (recalc == 1) launch_residual(c->stream, L, c->F, c->C, n, 1, 0);
+}
+
+int copy_in_level(desalm_ctx* c, double* const* soa, int ncomp, const double* host, const Level& L) {
+    if (!host) return 0;
+    HIPCHK(c, hipMemcpyAsync(c->staging, host, sizeof(double) * ncomp * L.n, hipMemcpyHostToDevice, c->stream));
+    double* dst[8];
+    for (int m = 0; m < ncomp; m++) dst[m] = soa[m] + L.off;
+    launch_aos_to_soa(c->stream, c->staging, ncomp, L.n, dst);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int copy_out_level(desalm_ctx* c, double* const* soa, int ncomp, double* host, const Level& L) {
+    if (!host) return 0;
+    double* src[8];
+    for (int m = 0; m < ncomp; m++) src[m] = soa[m] + L.off;
+    launch_soa_to_aos(c->stream, src, ncomp, L.n, c->staging);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(host, c->staging, sizeof(double) * ncomp * L.n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
+    if (!p || !out) return fail(nullptr, "null argument");
+    *out = nullptr;
+    std::string why;
+    if (validate(p, why)) return fail(nullptr, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+        return fail(nullptr, "no HIP device " + std::to_string(device));
+    desalm_ctx* c = new desalm_ctx();
+    c->p = *p;
+    c->device = device;
+    c->ng = p->ng;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return fail(nullptr, "hipSetDevice failed"); }
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(nullptr, "hipStreamCreate failed"); }
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    // levels (src/init.F90:254-260, init_mg :1116-1120)
+    int i = p->img, j = p->jmg, k = p->kmg;
+    long off = 0;
+    for (int n = 1; n <= c->ng; n++) {
+        Level& L = c->lv[n];
+        L.ni = i; L.nj = j; L.nk = k;
+        L.off = off;
+        L.n = (long)i * j * k;
+        L.dc = 1.0 / pow(2.0, (double)((n - 1) * p->icrs));
+        L.de = 1.0 / pow(2.0, (double)((n - 1) * p->jcrs));
+        L.dz = 1.0 / pow(2.0, (double)((n - 1) * p->kcrs));
+        off += L.n;
+        long md = i > j ? i : j;
+        md = md > k ? md : k;
+        if (md > c->maxdim) c->maxdim = md;
+        i = (i + p->icrs) / (1 << p->icrs);
+        j = (j + p->jcrs) / (1 << p->jcrs);
+        k = (k + p->kcrs) / (1 << p->kcrs);
+    }
+    c->nall = off;
+    c->n1 = c->lv[1].n;
+    // constants (init_input_file, init_daf_bc)
+    Consts& C = c->C;
+    memset(&C, 0, sizeof(C));
+    C.ren = p->ren;
+    C.rei = 1.0 / p->ren;
+    C.beta = p->beta;
+    C.delti = p->delti;
+    C.e_source = p->unsteady ? 1.0 : 0.0;
+    C.unsteady = p->unsteady ? 1 : 0;
+    C.cfl1 = p->cfl1; C.vnn1 = p->vnn1; C.cfl2 = p->cfl2; C.vnn2 = p->vnn2;
+    C.pdiss = p->pdiss_coef;
+    for (int a = 0; a < 2; a++)
+        for (int d = 0; d < 3; d++) C.ep[a][d] = p->ep[a][d];
+    for (int b = 0; b < 6; b++) {
+        C.btype[b] = p->btype[b];
+        for (int m = 0; m < 5; m++) {
+            C.sa[b][m] = 4.0 / 3.0;
+            C.sb[b][m] = -(1.0 / 3.0);
+        }
+        if (p->btype[b] == 1)
+            for (int m = 1; m < 4; m++) C.sa[b][m] = C.sb[b][m] = 0.0;
+        if (p->btype[b] == 2 && p->bdir[b] >= 1 && p->bdir[b] <= 3) C.sa[b][p->bdir[b]] = C.sb[b][p->bdir[b]] = 0.0;
+    }
+    // device fields
+    Fields& F = c->F;
+    memset(&F, 0, sizeof(F));
+    bool ok = true;
+    auto A = [&](long n) {
+        double* q = dalloc(c, n);
+        if (!q) ok = false;
+        return q;
+    };
+    for (int m = 0; m < 3; m++) { F.csi[m] = A(c->nall); F.eta[m] = A(c->nall); F.zet[m] = A(c->nall); F.x[m] = A(c->nall); }
+    F.aj = A(c->nall);
+    for (int m = 0; m < 5; m++) { F.q[m] = A(c->nall); F.qn[m] = A(c->nall); F.qnm1[m] = A(c->nall); F.qold[m] = A(c->nall); }
+    for (int m = 0; m < 4; m++) { F.rh[m] = A(c->nall); F.pk[m] = A(c->nall); }
+    F.xnut = A(c->nall); F.wd = A(c->nall); F.dtev = A(c->nall);
+    F.dtau = A(c->nall); F.diss1 = A(c->nall);
+    for (int m = 0; m < 3; m++) F.cp[m] = A(c->n1);
+    for (int m = 0; m < 4; m++) F.dp[m] = A(c->n1);
+    for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
+    F.wo = A(c->n1); F.rh5 = A(c->n1);
+    F.resid_part = A(5L * kResidBlocks);
+    F.resid = A(5);
+    F.pfix = A(1);
+    {
+        void* pv = nullptr;
+        if (hipMalloc(&pv, sizeof(unsigned long long)) != hipSuccess) ok = false;
+        else { hipMemset(pv, 0, sizeof(unsigned long long)); c->allocs.push_back(pv); F.pivots = (unsigned long long*)pv; }
+    }
+    c->staging = A(5 * c->nall);
+    c->dq_buf = A(4 * c->n1);
+    // prolongation smoothing factorisations per target level
+    const long per_level = 3 * 2 * c->maxdim;
+    c->rs_tab = A(per_level * (c->ng > 1 ? c->ng - 1 : 1));
+    if (!ok) {
+        desalm_finalize(c);
+        return fail(nullptr, "device allocation failed");
+    }
+    {
+        std::vector<double> tab(per_level * (c->ng > 1 ? c->ng - 1 : 1), 0.0);
+        for (int n = 1; n < c->ng; n++) {
+            const Level& L = c->lv[n];
+            const int dims[3] = {L.ni, L.nj, L.nk};
+            for (int d = 0; d < 3; d++) {
+                const long stride = 2L * std::max(L.ni, std::max(L.nj, L.nk));
+                rs_factor(p->ep[1][d], dims[d] - 2, tab.data() + (n - 1) * per_level + d * stride);
+            }
+        }
+        if (hipMemcpy(c->rs_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            desalm_finalize(c);
+            return fail(nullptr, "rs table upload failed");
+        }
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        desalm_finalize(c);
+        return fail(nullptr, "device synchronisation failed after allocation");
+    }
+    *out = c;
+    return 0;
+}
+
+int64_t desalm_nodes_all(const desalm_ctx* c) { return c ? c->nall : -1; }
+
+int64_t desalm_level_offset(const desalm_ctx* c, int n) {
+    if (!c || n < 1 || n > c->ng) return -1;
+    return c->lv[n].off;
+}
+
+int desalm_set_grid(desalm_ctx* c, const double* x, const double* y, const double* z) {
+    if (!c || !x || !y || !z) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const double* h[3] = {x, y, z};
+    for (int m = 0; m < 3; m++)
+        HIPCHK(c, hipMemcpyAsync(c->F.x[m], h[m], sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
+    launch_inject(c->stream, c->lv, c->ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.x, 3);
+    for (int n = 1; n <= c->ng; n++) launch_metrics(c->stream, c->lv[n], c->F, c->C, n == 1);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_set_metrics(desalm_ctx* c, const double* csi, const double* eta, const double* zet, const double* aj) {
+    if (!c || !csi || !eta || !zet || !aj) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    Level all = c->lv[1];
+    all.off = 0;
+    all.n = c->nall;
+    if (copy_in_level(c, c->F.csi, 3, csi, all)) return -1;
+    if (copy_in_level(c, c->F.eta, 3, eta, all)) return -1;
+    if (copy_in_level(c, c->F.zet, 3, zet, all)) return -1;
+    if (copy_in_level(c, &c->F.aj, 1, aj, all)) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_set_wall_distance(desalm_ctx* c, const double* wd, const double* dtev) {
+    if (!c || !wd) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->F.wd, wd, sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
+    if (dtev) HIPCHK(c, hipMemcpyAsync(c->F.dtev, dtev, sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_set_state(desalm_ctx* c, const double* q, const double* qn, const double* qnm1, const double* xnut) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->lv[1];
+    if (copy_in_level(c, c->F.q, 5, q, L)) return -1;
+    if (copy_in_level(c, c->F.qn, 5, qn, L)) return -1;
+    if (copy_in_level(c, c->F.qnm1, 5, qnm1, L)) return -1;
+    if (xnut) HIPCHK(c, hipMemcpyAsync(c->F.xnut, xnut, sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_get_state(desalm_ctx* c, double* q, double* qn, double* qnm1, double* xnut) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->lv[1];
+    if (copy_out_level(c, c->F.q, 5, q, L)) return -1;
+    if (copy_out_level(c, c->F.qn, 5, qn, L)) return -1;
+    if (copy_out_level(c, c->F.qnm1, 5, qnm1, L)) return -1;
+    if (xnut) {
+        HIPCHK(c, hipMemcpyAsync(xnut, c->F.xnut, sizeof(double) * c->n1, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const desalm_params& p = c->p;
+    const Level& L1 = c->lv[1];
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    // qnm1 <- qn, qn <- q on the fine level (mg_driver.F90:48-51)
+    for (int m = 0; m < 5; m++) {
+        HIPCHK(c, hipMemcpyAsync(c->F.qnm1[m], c->F.qn[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->F.qn[m], c->F.q[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
+    }
+    launch_inject(s, c->lv, c->ng, p.icrs, p.jcrs, p.kcrs, c->F.qn, 5);
+    launch_inject(s, c->lv, c->ng, p.icrs, p.jcrs, p.kcrs, c->F.qnm1, 5);
+    launch_copy_qold(s, L1, c->F);
+    HIPCHK(c, hipGetLastError());
+    const double tnu = (double)((long)(L1.ni - 2) * (L1.nj - 2) * (L1.nk - 2));
+    int itc = 0;
+    int decide_recalc_rh = 0;
+    for (int it = 1; it <= p.itm; it++) {
+        itc++;
+        for (int n = 1; n <= c->ng; n++) {
+            if (n > 1) launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, p.icrs, p.jcrs, p.kcrs);
+            for (int itr = 1; itr <= p.iter[n - 1]; itr++) {
+                if (n != c->ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
+                if (p.des && n == 1) launch_des_eddy(s, L1, c->F, c->C, p.itk);
+                solver_daf(c, n, decide_recalc_rh, itr, nullptr);
+            }
+        }
+        for (int n = c->ng - 1; n >= 1; n--) {
+            const long per_level = 3 * 2 * c->maxdim;
+            launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, p.icrs, p.jcrs, p.kcrs,
+                           c->rs_tab + (n - 1) * per_level);
+        }
+        launch_bcond_fm(s, L1, c->F, c->C, p.ifix, p.jfix, p.kfix);
+        // update norms, then qold_mg <- q for the next pseudo-iteration
+        launch_resid_and_save(s, L1, c->F, 1);
+        HIPCHK(c, hipGetLastError());
+        double resd[5];
+        HIPCHK(c, hipMemcpyAsync(resd, c->F.resid, sizeof(resd), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        double residual[5];
+        for (int m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
+        if (resid_out)
+            for (int m = 0; m < 5; m++) resid_out[5 * (it - 1) + m] = residual[m];
+        // mg_ck_conver (mg_driver.F90:372-398)
+        if (itc == 1)
+            for (int m = 0; m < 5; m++) c->init_resd[m] = residual[m];
+        const double eomax = fmax(fmax(residual[1], residual[2]), residual[3]);
+        const double ermax = fmax(fmax(residual[1] - c->init_resd[1], residual[2] - c->init_resd[2]),
+                                  residual[3] - c->init_resd[3]);
+        bool converged = false;
+        if (itc > p.it_min && ermax < p.er_min) converged = true;
+        if (itc > p.it_min && eomax < p.eo_min) converged = true;
+        if (converged) break;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    if (itc_out) *itc_out = itc;
+    return 0;
+}
+
+int desalm_solver_daf(desalm_ctx* c, int n, int decide_recalc_rh, int decide_calc_pk, double* q, const double* qn,
+                      const double* qnm1, const double* csi, const double* eta, const double* zet, const double* aj,
+                      const double* xnut, double* pk, double* rh) {
+    if (!c || !q || !qn || !qnm1 || !csi || !eta || !zet || !aj || !xnut || !pk || !rh)
+        return fail(c, "null argument");
+    if (n < 1 || n > c->ng) return fail(c, "level out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->lv[n];
+    hipStream_t s = c->stream;
+    if (copy_in_level(c, c->F.q, 4, q, L)) return -1;
+    if (copy_in_level(c, c->F.qn, 4, qn, L)) return -1;
+    if (copy_in_level(c, c->F.qnm1, 4, qnm1, L)) return -1;
+    if (copy_in_level(c, c->F.csi, 3, csi, L)) return -1;
+    if (copy_in_level(c, c->F.eta, 3, eta, L)) return -1;
+    if (copy_in_level(c, c->F.zet, 3, zet, L)) return -1;
+    HIPCHK(c, hipMemcpyAsync(c->F.aj + L.off, aj, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->F.xnut + L.off, xnut, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    if (copy_in_level(c, c->F.pk, 4, pk, L)) return -1;
+    double* dq = c->dq_buf;
+    HIPCHK(c, hipMemsetAsync(dq, 0, sizeof(double) * 4 * L.n, s));
+    HIPCHK(c, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
+    solver_daf(c, n, decide_recalc_rh, decide_calc_pk, decide_recalc_rh == 1 ? nullptr : dq);
+    HIPCHK(c, hipGetLastError());
+    if (decide_recalc_rh != 1) {
+        // rh leaves solver_daf holding dq = Mc dQ*** (rhs_diag_solver.F90:624-633)
+        for (int m = 0; m < 4; m++)
+            HIPCHK(c, hipMemcpyAsync(c->F.rh[m] + L.off, dq + m * L.n, sizeof(double) * L.n, hipMemcpyDeviceToDevice, s));
+    }
+    if (copy_out_level(c, c->F.q, 4, q, L)) return -1;
+    if (copy_out_level(c, c->F.pk, 4, pk, L)) return -1;
+    if (copy_out_level(c, c->F.rh, 4, rh, L)) return -1;
+    HIPCHK(c, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int desalm_des_eddy(desalm_ctx* c, double* q, const double* qn5, const double* qnm1_5, const double* csi,
+                    const double* eta, const double* zet, const double* aj, const double* wd, const double* dtev,
+                    double* xnut) {
+    if (!c || !q || !qn5 || !qnm1_5 || !csi || !eta || !zet || !aj || !wd || !dtev || !xnut)
+        return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->lv[1];
+    hipStream_t s = c->stream;
+    if (copy_in_level(c, c->F.q, 5, q, L)) return -1;
+    HIPCHK(c, hipMemcpyAsync(c->F.qn[4], qn5, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->F.qnm1[4], qnm1_5, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    if (copy_in_level(c, c->F.csi, 3, csi, L)) return -1;
+    if (copy_in_level(c, c->F.eta, 3, eta, L)) return -1;
+    if (copy_in_level(c, c->F.zet, 3, zet, L)) return -1;
+    HIPCHK(c, hipMemcpyAsync(c->F.aj, aj, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->F.wd, wd, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->F.dtev, dtev, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->F.xnut, xnut, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
+    launch_des_eddy(s, L, c->F, c->C, c->p.itk);
+    HIPCHK(c, hipGetLastError());
+    if (copy_out_level(c, c->F.q, 5, q, L)) return -1;
+    HIPCHK(c, hipMemcpyAsync(xnut, c->F.xnut, sizeof(double) * L.n, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int desalm_gtsv_batch(int n, int nsys, double* dl, double* d, double* du, double* b, int* info_out) {
+    if (n < 0 || nsys < 0) return fail(nullptr, "negative size");
+    if (n == 0 || nsys == 0) return 0;
+    if (!dl || !d || !du || !b) return fail(nullptr, "null argument");
+    const size_t bytes = sizeof(double) * (size_t)n * nsys;
+    double *ddl = nullptr, *dd = nullptr, *ddu = nullptr, *db = nullptr;
+    int* dinfo = nullptr;
+    int rc = 0;
+    if (hipMalloc(&ddl, bytes) != hipSuccess || hipMalloc(&dd, bytes) != hipSuccess ||
+        hipMalloc(&ddu, bytes) != hipSuccess || hipMalloc(&db, bytes) != hipSuccess ||
+        hipMalloc(&dinfo, sizeof(int) * nsys) != hipSuccess) {
+        rc = fail(nullptr, "device allocation failed");
+    } else {
+        hipMemcpy(ddl, dl, bytes, hipMemcpyHostToDevice);
+        hipMemcpy(dd, d, bytes, hipMemcpyHostToDevice);
+        hipMemcpy(ddu, du, bytes, hipMemcpyHostToDevice);
+        hipMemcpy(db, b, bytes, hipMemcpyHostToDevice);
+        launch_gtsv_batch(0, n, nsys, ddl, dd, ddu, db, dinfo);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            rc = fail(nullptr, "gtsv kernel failed");
+        } else {
+            hipMemcpy(dl, ddl, bytes, hipMemcpyDeviceToHost);
+            hipMemcpy(d, dd, bytes, hipMemcpyDeviceToHost);
+            hipMemcpy(du, ddu, bytes, hipMemcpyDeviceToHost);
+            hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost);
+            if (info_out) hipMemcpy(info_out, dinfo, sizeof(int) * nsys, hipMemcpyDeviceToHost);
+        }
+    }
+    hipFree(ddl); hipFree(dd); hipFree(ddu); hipFree(db); hipFree(dinfo);
+    return rc;
+}
+
+int desalm_get_level_field(desalm_ctx* c, int which, int n, double* out) {
+    if (!c || !out) return fail(c, "null argument");
+    if (n < 1 || n > c->ng) return fail(c, "level out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    Fields& F = c->F;
+    double* one[1];
+    switch (which) {
+        case 0: return copy_out_level(c, F.q, 5, out, c->lv[n]);
+        case 1: return copy_out_level(c, F.qn, 5, out, c->lv[n]);
+        case 2: return copy_out_level(c, F.qnm1, 5, out, c->lv[n]);
+        case 3: return copy_out_level(c, F.qold, 5, out, c->lv[n]);
+        case 4: return copy_out_level(c, F.rh, 4, out, c->lv[n]);
+        case 5: return copy_out_level(c, F.pk, 4, out, c->lv[n]);
+        case 6: one[0] = F.xnut; return copy_out_level(c, one, 1, out, c->lv[n]);
+        case 7: return copy_out_level(c, F.csi, 3, out, c->lv[n]);
+        case 8: return copy_out_level(c, F.eta, 3, out, c->lv[n]);
+        case 9: return copy_out_level(c, F.zet, 3, out, c->lv[n]);
+        case 10: one[0] = F.aj; return copy_out_level(c, one, 1, out, c->lv[n]);
+        case 11: one[0] = F.dtev; return copy_out_level(c, one, 1, out, c->lv[n]);
+        case 12: one[0] = F.wd; return copy_out_level(c, one, 1, out, c->lv[n]);
+        default: return fail(c, "unknown field");
+    }
+}
+
+double desalm_last_driver_ms(const desalm_ctx* c) { return c ? c->last_ms : -1.0; }
+
+int64_t desalm_last_pivot_events(const desalm_ctx* c) { return c ? (int64_t)c->last_pivots : -1; }
+
+const char* desalm_last_error(const desalm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int desalm_finalize(desalm_ctx* c) {
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// desalm_internal.h -- device data layout and kernel launchers of libdesalm.
+//
+// Layout in HBM: every field is structure-of-arrays, one double array per
+// component, with all multigrid levels concatenated in the reference's node
+// order (level n starts at node offset `off`, nodes run i fastest, then j,
+// then k).  A kernel on level n offsets each component pointer by `off`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/desalm.h"
+
+namespace desalm {
+
+constexpr int kMaxLevels = DESALM_MAX_LEVELS;
+
+struct Level {
+    int ni, nj, nk;
+    long off;   // node offset of the level in all-level arrays
+    long n;     // ni * nj * nk
+    double dc, de, dz;
+};
+
+// Run constants used inside kernels (a trimmed copy of desalm_params plus
+// the derived sa/sb tables of init_daf_bc).
+struct Consts {
+    double ren, rei, beta, delti, e_source;
+    double cfl1, vnn1, cfl2, vnn2, pdiss;
+    double ep[2][3];
+    double sa[6][5], sb[6][5];
+    int btype[6];
+    int unsteady;
+};
+
+// Device pointers.  Fields marked "all" span every level; "fine" span the
+// fine level only (scratch is only ever used one level at a time and the
+// fine level is the largest).
+struct Fields {
+    double* csi[3];   // all
+    double* eta[3];   // all
+    double* zet[3];   // all
+    double* aj;       // all
+    double* q[5];     // all
+    double* qn[5];    // all
+    double* qnm1[5];  // all
+    double* qold[5];  // all (qold_mg)
+    double* rh[4];    // all
+    double* pk[4];    // all
+    double* xnut;     // all
+    double* wd;       // all (zero off the fine level)
+    double* dtev;     // all (zero off the fine level)
+    double* x[3];     // all: grid coordinates
+    // scratch
+    double* dtau;     // all: local pseudo time step of the level being smoothed
+    double* diss1;    // all: pressure dissipation kept for the recalc branch
+    double* cp[3];    // fine: line-solve c' of the three distinct DAF matrices
+    double* dp[4];    // fine: line-solve d' (k sweep, prolongation smoothing)
+    double* ucn[3];   // fine: contravariant velocity / J for the SA update
+    double* wo;       // fine: SA vorticity measure
+    double* rh5;      // fine: SA right-hand side
+    double* resid_part;  // [kResidBlocks * 5] partial L1 sums
+    double* resid;       // [5] L1 sums
+    double* pfix;        // [1] pinned pressure value
+    unsigned long long* pivots;  // [1] rows where LAPACK SGTSV would pivot
+};
+
+constexpr int kResidBlocks = 1024;
+
+struct Ctx;  // host context, desalm_api.hip
+
+// ---- launchers (each enqueues on `s`) ----
+// residual assembly, k_residual.hip
+void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
+void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n,
+                     int recalc, int calc_pk);
+// DAF implicit operator, k_daf.hip
+void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n);
+void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n);
+void launch_daf_k(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n,
+                  double* dq_out /* nullable: also store Mc x here (4 SoA, level-local) */);
+// fine-grid BCs, k_mg.hip
+void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix,
+                     int jfix, int kfix);
+// SA / DES, k_sa.hip
+void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk);
+// multigrid, k_mg.hip
+void launch_inject(hipStream_t s, const Level* lv, int ng, int icrs, int jcrs, int kcrs,
+                   double* const* comps, int ncomp);
+void launch_nlevel(hipStream_t s, const Level& Lc, const Level& Lf, const Fields& F, int icrs,
+                   int jcrs, int kcrs);
+void launch_zero_pk_boundary(hipStream_t s, const Level& L, const Fields& F);
+void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F,
+                    const Consts& C, int icrs, int jcrs, int kcrs, const double* rs_tab);
+void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save_only);
+void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F);
+// metrics, k_metrics.hip
+void launch_metrics(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int fine);
+// layout conversion, k_mg.hip
+void launch_aos_to_soa(hipStream_t s, const double* aos, int ncomp, long n, double* const* soa);
+void launch_soa_to_aos(hipStream_t s, double* const* soa, int ncomp, long n, double* aos);
+// batched GEPP (LAPACK SGTSV semantics), k_daf.hip
+void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,
+                       double* b, int* info);
+
+}  // namespace desalm

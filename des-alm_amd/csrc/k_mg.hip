@@ -1,0 +1,412 @@
+// k_mg.hip -- FAS multigrid transfers, fine-grid boundary conditions,
+// update norms and layout conversion.
+//
+//   k_inject        mg_inject (src/mg_inject.F90:207-269)
+//   k_nlevel        mg_nlevel: injection of q, xnut, qold_mg and the 27-point
+//                   restriction of rh into pk (src/mg_nlevel.F90:2-150),
+//                   boundary pk zeroed (mg_zero_pk_boundary)
+//   k_prolong_fill  mg_prolong's correction and its k/j/i linear fills
+//                   (src/mg_prolong.F90:57-170), evaluated per fine node
+//   k_rs<D>         rs_sweep constant-coefficient smoothing sweeps
+//                   (src/rs_sweep.F90:163-284), k sweep also adds dq to q
+//   k_bc_fm<D>      bcond_fm faces (src/bcond_fm.F90:79-306), k_pfix_*
+//                   the pinned-pressure shift (:310-322)
+//   k_resid_*       mg_residual's L1 update norms (src/mg_residual.F90:58-67)
+#include "desalm_internal.h"
+
+namespace desalm {
+
+namespace {
+
+__device__ __forceinline__ long lidx(const Level& L, int i, int j, int k) {
+    return (long)i + (long)L.ni * ((long)j + (long)L.nj * k);
+}
+
+// coarse node (i,j,k) -> coincident fine node, 0-based (fine 1-based index
+// 2^crs * i - crs of src/mg_nlevel.F90:22-24)
+__device__ __forceinline__ int c2f(int i, int crs) { return crs ? 2 * i : i; }
+
+__global__ void k_inject(Level Lc, Level Lf, double* const* comps, int ncomp, int icrs, int jcrs, int kcrs) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Lc.n) return;
+    const int i = (int)(t % Lc.ni), j = (int)((t / Lc.ni) % Lc.nj), k = (int)(t / ((long)Lc.ni * Lc.nj));
+    const long gf = Lf.off + lidx(Lf, c2f(i, icrs), c2f(j, jcrs), c2f(k, kcrs));
+    const long gc = Lc.off + t;
+    for (int m = 0; m < ncomp; m++) comps[m][gc] = comps[m][gf];
+}
+
+struct Ptrs {
+    double* p[16];
+};
+
+__global__ void k_inject_p(Level Lc, Level Lf, Ptrs P, int ncomp, int icrs, int jcrs, int kcrs) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Lc.n) return;
+    const int i = (int)(t % Lc.ni), j = (int)((t / Lc.ni) % Lc.nj), k = (int)(t / ((long)Lc.ni * Lc.nj));
+    const long gf = Lf.off + lidx(Lf, c2f(i, icrs), c2f(j, jcrs), c2f(k, kcrs));
+    const long gc = Lc.off + t;
+    for (int m = 0; m < ncomp; m++) P.p[m][gc] = P.p[m][gf];
+}
+
+__global__ void __launch_bounds__(256) k_nlevel(Level Lc, Level Lf, Fields F, int icrs, int jcrs, int kcrs) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Lc.n) return;
+    const int ni = Lc.ni, nj = Lc.nj, nk = Lc.nk;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    const long ll = Lf.off + lidx(Lf, c2f(i, icrs), c2f(j, jcrs), c2f(k, kcrs));
+    const long l = Lc.off + t;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double v = F.q[m][ll];
+        F.q[m][l] = v;
+        F.qold[m][l] = v;
+    }
+    F.xnut[l] = F.xnut[ll];
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
+        // the reference evaluates these from out-of-level neighbours and then
+        // zeroes them (src/mg_driver.F90:134)
+#pragma unroll
+        for (int m = 0; m < 4; m++) F.pk[m][l] = 0.0;
+        return;
+    }
+    const long ic = 1, ie = Lf.ni, iz = (long)Lf.ni * Lf.nj;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double* r = F.rh[m] + ll;
+        F.pk[m][l] = (0.25 * r[ic + ie + iz] + 0.5 * r[ic + ie] + 0.25 * r[ic + ie - iz] +
+                      0.5 * r[ie + iz] + r[ie] + 0.5 * r[ie - iz] +
+                      0.25 * r[-ic + ie + iz] + 0.5 * r[-ic + ie] + 0.25 * r[-ic + ie - iz] +
+                      0.5 * r[ic + iz] + r[ic] + 0.5 * r[ic - iz] +
+                      r[iz] + 2.0 * r[0] + r[-iz] +
+                      0.5 * r[-ic + iz] + r[-ic] + 0.5 * r[-ic - iz] +
+                      0.25 * r[ic - ie + iz] + 0.5 * r[ic - ie] + 0.25 * r[ic - ie - iz] +
+                      0.5 * r[-ie + iz] + r[-ie] + 0.5 * r[-ie - iz] +
+                      0.25 * r[-ic - ie + iz] + 0.5 * r[-ic - ie] + 0.25 * r[-ic - ie - iz]) *
+                     (1.0 / 16.0);
+    }
+}
+
+__global__ void k_zero_pk_boundary(Level L, Fields F) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) F.pk[m][L.off + t] = 0.0;
+    }
+}
+
+// ---- prolongation ----
+struct ProlongCtx {
+    Level Lf, Lc;
+    int icrs, jcrs, kcrs;
+};
+
+// coarse correction at a coincident fine node (mg_prolong.F90:57-75)
+__device__ __forceinline__ double corr(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
+    const int ci = P.icrs ? i / 2 : i, cj = P.jcrs ? j / 2 : j, ck = P.kcrs ? k / 2 : k;
+    const long l = P.Lc.off + lidx(P.Lc, ci, cj, ck);
+    return F.q[m][l] - F.qold[m][l];
+}
+// after the k fill (i, j coincident)
+__device__ __forceinline__ double fill_k(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
+    if (P.kcrs && (k & 1)) return 0.5 * corr(P, F, m, i, j, k + 1) + 0.5 * corr(P, F, m, i, j, k - 1);
+    return corr(P, F, m, i, j, k);
+}
+// after the j fill (i coincident)
+__device__ __forceinline__ double fill_j(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
+    if (P.jcrs && (j & 1)) return 0.5 * fill_k(P, F, m, i, j + 1, k) + 0.5 * fill_k(P, F, m, i, j - 1, k);
+    return fill_k(P, F, m, i, j, k);
+}
+__device__ __forceinline__ double fill_i(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
+    if (P.icrs && (i & 1)) return 0.5 * fill_j(P, F, m, i + 1, j, k) + 0.5 * fill_j(P, F, m, i - 1, j, k);
+    return fill_j(P, F, m, i, j, k);
+}
+
+// dq at every fine interior node into F.dp (level-local); boundary dq is
+// zero after smoothing and never enters the interior sweeps
+__global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Level& L = P.Lf;
+    if (t >= L.n) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.dp[m][t] = fill_i(P, F, m, i, j, k);
+}
+
+// rs_sweep in direction D on the interior of the fine level; tab holds the
+// SGTSV factorization of the constant matrix (-ep, 1+2ep, -ep): fact[len-1]
+// then d[len].  D = 2 also adds dq to q.
+template <int D>
+__global__ void __launch_bounds__(128) k_rs(Level L, Fields F, const double* tab, double epn) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const long sI = 1, sJ = ni, sK = (long)ni * nj;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long base, sD;
+    int len;
+    if (D == 0) {
+        if (t >= (long)(nj - 2) * (nk - 2)) return;
+        base = sJ * (1 + t % (nj - 2)) + sK * (1 + t / (nj - 2));
+        sD = sI;
+        len = ni - 2;
+    } else if (D == 1) {
+        if (t >= (long)(ni - 2) * (nk - 2)) return;
+        base = (1 + t % (ni - 2)) + sK * (1 + t / (ni - 2));
+        sD = sJ;
+        len = nj - 2;
+    } else {
+        if (t >= (long)(ni - 2) * (nj - 2)) return;
+        base = (1 + t % (ni - 2)) + sJ * (1 + t / (ni - 2));
+        sD = sK;
+        len = nk - 2;
+    }
+    const double* fact = tab;
+    const double* dd = tab + (len - 1);
+    const double du = -epn;
+    double prev[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) prev[m] = F.dp[m][base + sD];
+    for (int s = 1; s < len; s++) {
+        const long p = base + (1 + s) * sD;
+        const double f = fact[s - 1];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const double v = F.dp[m][p] - f * prev[m];
+            F.dp[m][p] = v;
+            prev[m] = v;
+        }
+    }
+    double x[4];
+    {
+        const long p = base + (long)len * sD;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            x[m] = prev[m] / dd[len - 1];
+            if (D < 2) F.dp[m][p] = x[m];
+            else F.q[m][L.off + p] = F.q[m][L.off + p] + x[m];
+        }
+    }
+    for (int s = len - 2; s >= 0; s--) {
+        const long p = base + (1 + s) * sD;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            x[m] = (F.dp[m][p] - du * x[m]) / dd[s];
+            if (D < 2) F.dp[m][p] = x[m];
+            else F.q[m][L.off + p] = F.q[m][L.off + p] + x[m];
+        }
+    }
+}
+
+// ---- bcond_fm faces of direction D (both sides) ----
+template <int D>
+__global__ void k_bc_fm(Level L, Fields F, Consts C) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const long sI = 1, sJ = ni, sK = (long)ni * nj;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long base, sD;
+    int nd;
+    if (D == 0) {
+        if (t >= (long)nj * nk) return;
+        base = sJ * (t % nj) + sK * (t / nj);
+        sD = sI;
+        nd = ni;
+    } else if (D == 1) {
+        if (t >= (long)ni * nk) return;
+        base = (t % ni) + sK * (t / ni);
+        sD = sJ;
+        nd = nj;
+    } else {
+        if (t >= (long)ni * nj) return;
+        base = t;
+        sD = sK;
+        nd = nk;
+    }
+    const long lo = L.off + base, hi = L.off + base + (long)(nd - 1) * sD;
+    const int b0 = 2 * D, b1 = 2 * D + 1;
+    switch (C.btype[b0]) {
+        case 1: case 2: case 3:
+#pragma unroll
+            for (int m = 0; m < 4; m++) F.q[m][lo] = C.sa[b0][m] * F.q[m][lo + sD] + C.sb[b0][m] * F.q[m][lo + 2 * sD];
+            break;
+        case 6:  // plane copy from the opposite boundary plane (not done for i)
+            if (D > 0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) F.q[m][lo] = F.q[m][hi];
+            }
+            break;
+        default: break;
+    }
+    switch (C.btype[b1]) {
+        case 1: case 2: case 3:
+#pragma unroll
+            for (int m = 0; m < 4; m++) F.q[m][hi] = C.sa[b1][m] * F.q[m][hi - sD] + C.sb[b1][m] * F.q[m][hi - 2 * sD];
+            break;
+        case 6:
+            if (D > 0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][lo];
+            }
+            break;
+        case 8:
+            if (D == 0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][hi - sD];
+            }
+            break;
+        default: break;
+    }
+}
+
+__global__ void k_pfix_read(Fields F, long g) { *F.pfix = F.q[0][g]; }
+
+__global__ void k_pfix_sub(Level L, Fields F) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    const double pf = *F.pfix;
+    F.q[0][L.off + t] = F.q[0][L.off + t] - pf;
+}
+
+// ---- mg_residual: deterministic two-stage L1 sums ----
+__global__ void __launch_bounds__(256) k_resid_part(Level L, Fields F, int save) {
+    __shared__ double sh[5][256];
+    double acc[5] = {0, 0, 0, 0, 0};
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < L.n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+        const long g = L.off + t;
+        const bool in = !(i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1);
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            const double qv = F.q[m][g];
+            if (in) acc[m] += fabs(qv - F.qold[m][g]);
+            if (save) F.qold[m][g] = qv;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) sh[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) sh[m][threadIdx.x] += sh[m][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) F.resid_part[blockIdx.x * 5 + m] = sh[m][0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_resid_final(Fields F, int nblocks) {
+    __shared__ double sh[5][256];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int m = 0; m < 5; m++) acc[m] += F.resid_part[b * 5 + m];
+#pragma unroll
+    for (int m = 0; m < 5; m++) sh[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) sh[m][threadIdx.x] += sh[m][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) F.resid[m] = sh[m][0];
+    }
+}
+
+__global__ void k_copy_qold(Level L, Fields F) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+#pragma unroll
+    for (int m = 0; m < 5; m++) F.qold[m][L.off + t] = F.q[m][L.off + t];
+}
+
+__global__ void k_aos_to_soa(const double* aos, int ncomp, long n, Ptrs P) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    for (int m = 0; m < ncomp; m++) P.p[m][t] = aos[t * ncomp + m];
+}
+
+__global__ void k_soa_to_aos(Ptrs P, int ncomp, long n, double* aos) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    for (int m = 0; m < ncomp; m++) aos[t * ncomp + m] = P.p[m][t];
+}
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_inject(hipStream_t s, const Level* lv, int ng, int icrs, int jcrs, int kcrs, double* const* comps,
+                   int ncomp) {
+    Ptrs P{};
+    for (int m = 0; m < ncomp && m < 16; m++) P.p[m] = comps[m];
+    for (int n = 2; n <= ng; n++) {
+        const Level& Lc = lv[n];
+        const Level& Lf = lv[n - 1];
+        hipLaunchKernelGGL(k_inject_p, dim3(nblk(Lc.n, 256)), dim3(256), 0, s, Lc, Lf, P, ncomp, icrs, jcrs, kcrs);
+    }
+}
+
+void launch_nlevel(hipStream_t s, const Level& Lc, const Level& Lf, const Fields& F, int icrs, int jcrs, int kcrs) {
+    hipLaunchKernelGGL(k_nlevel, dim3(nblk(Lc.n, 256)), dim3(256), 0, s, Lc, Lf, F, icrs, jcrs, kcrs);
+}
+
+void launch_zero_pk_boundary(hipStream_t s, const Level& L, const Fields& F) {
+    hipLaunchKernelGGL(k_zero_pk_boundary, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
+}
+
+void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F, const Consts& C, int icrs,
+                    int jcrs, int kcrs, const double* rs_tab) {
+    ProlongCtx P{Lf, Lc, icrs, jcrs, kcrs};
+    hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(Lf.n, 256)), dim3(256), 0, s, P, F);
+    // rs_tab: three direction tables of 2 * max(ni,nj,nk) doubles
+    const long stride = 2L * (Lf.ni > Lf.nj ? (Lf.ni > Lf.nk ? Lf.ni : Lf.nk) : (Lf.nj > Lf.nk ? Lf.nj : Lf.nk));
+    hipLaunchKernelGGL(k_rs<0>, dim3(nblk((long)(Lf.nj - 2) * (Lf.nk - 2), 128)), dim3(128), 0, s, Lf, F, rs_tab,
+                       C.ep[1][0]);
+    hipLaunchKernelGGL(k_rs<1>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nk - 2), 128)), dim3(128), 0, s, Lf, F,
+                       rs_tab + stride, C.ep[1][1]);
+    hipLaunchKernelGGL(k_rs<2>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nj - 2), 128)), dim3(128), 0, s, Lf, F,
+                       rs_tab + 2 * stride, C.ep[1][2]);
+}
+
+void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix, int jfix, int kfix) {
+    hipLaunchKernelGGL(k_bc_fm<0>, dim3(nblk((long)L.nj * L.nk, 256)), dim3(256), 0, s, L, F, C);
+    hipLaunchKernelGGL(k_bc_fm<1>, dim3(nblk((long)L.ni * L.nk, 256)), dim3(256), 0, s, L, F, C);
+    hipLaunchKernelGGL(k_bc_fm<2>, dim3(nblk((long)L.ni * L.nj, 256)), dim3(256), 0, s, L, F, C);
+    const long g = L.off + (ifix - 1) + (long)L.ni * ((jfix - 1) + (long)L.nj * (kfix - 1));
+    hipLaunchKernelGGL(k_pfix_read, dim3(1), dim3(1), 0, s, F, g);
+    hipLaunchKernelGGL(k_pfix_sub, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
+}
+
+void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save) {
+    hipLaunchKernelGGL(k_resid_part, dim3(kResidBlocks), dim3(256), 0, s, L, F, save);
+    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, s, F, kResidBlocks);
+}
+
+void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F) {
+    hipLaunchKernelGGL(k_copy_qold, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
+}
+
+void launch_aos_to_soa(hipStream_t s, const double* aos, int ncomp, long n, double* const* soa) {
+    Ptrs P{};
+    for (int m = 0; m < ncomp && m < 16; m++) P.p[m] = soa[m];
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(n, 256)), dim3(256), 0, s, aos, ncomp, n, P);
+}
+
+void launch_soa_to_aos(hipStream_t s, double* const* soa, int ncomp, long n, double* aos) {
+    Ptrs P{};
+    for (int m = 0; m < ncomp && m < 16; m++) P.p[m] = soa[m];
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(n, 256)), dim3(256), 0, s, P, ncomp, n, aos);
+}
+
+}  // namespace desalm

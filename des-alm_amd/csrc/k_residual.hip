@@ -1,0 +1,283 @@
+// k_residual.hip -- residual assembly of one DAF smoother pass on one level.
+//
+// One thread per node of the level gathers its stencil straight from the
+// SoA fields and produces the whole right-hand side of solver_daf steps
+// 2-10 (src/solver_daf.F90:199-315): contravariant velocity, QUICK or
+// first-order convection, central pressure gradient and divergence, 2nd-order
+// viscous fluxes, 4th-difference pressure dissipation, BDF2 term, FAS forcing
+// and zeroing of the boundary planes.  Face fluxes are evaluated by the two
+// nodes that share them, so no intermediate face arrays touch HBM.
+// Operand order inside every expression follows the reference routine cited.
+#include "desalm_internal.h"
+
+namespace desalm {
+
+namespace {
+
+struct Nb {
+    long sI, sJ, sK;
+};
+
+__device__ __forceinline__ double sqr(double a) { return a * a; }
+
+// rhs_daf_dtau (src/rhs_daf_dtau.F90:85-117), every node of the level
+__global__ void k_dtau(Level L, Fields F, Consts C) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    const long g = L.off + t;
+    const double c1 = F.csi[0][g], c2 = F.csi[1][g], c3 = F.csi[2][g];
+    const double e1 = F.eta[0][g], e2 = F.eta[1][g], e3 = F.eta[2][g];
+    const double z1 = F.zet[0][g], z2 = F.zet[1][g], z3 = F.zet[2][g];
+    const double g1 = c1 * c1 + c2 * c2 + c3 * c3;
+    const double g2 = e1 * e1 + e2 * e2 + e3 * e3;
+    const double g3 = z1 * z1 + z2 * z2 + z3 * z3;
+    const double dti1 = C.cfl1 / sqrt(g1) / L.dc;
+    const double dti2 = C.cfl1 / sqrt(g2) / L.de;
+    const double dti3 = C.cfl1 / sqrt(g3) / L.dz;
+    const double ret = C.rei + F.xnut[g];
+    const double dtv1 = C.vnn1 / ret / (L.dc * L.dc) / g1;
+    const double dtv2 = C.vnn1 / ret / (L.de * L.de) / g2;
+    const double dtv3 = C.vnn1 / ret / (L.dz * L.dz) / g3;
+    double d = fmin(dti1, dti2);
+    d = fmin(d, dti3);
+    d = fmin(d, dtv1);
+    d = fmin(d, dtv2);
+    d = fmin(d, dtv3);
+    F.dtau[g] = d;
+}
+
+// One viscous face flux (rhs_viscous.F90:112-163 for D=0, :189-243 for D=1,
+// :263-316 for D=2): the face between node t and node t + s_D.
+template <int D>
+__device__ __forceinline__ void visc_face(const Level& L, const Fields& F, const Consts& C, long t,
+                                          const Nb& nb, double fv[3]) {
+    const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
+    const long o = L.off;
+    const long t0 = o + t, t1 = o + t + sD;
+    const double hc = L.dc, he = L.de, hz = L.dz;
+    const double hc4 = 0.25 * hc, he4 = 0.25 * he, hz4 = 0.25 * hz;
+    double dcs[3], det[3], dze[3];  // d(u,v,w)/d(csi,eta,zet) at the face
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const double* u = F.q[m + 1];
+        if (D == 0) {
+            dcs[m] = hc * (u[t1] - u[t0]);
+            det[m] = he4 * (u[t1 + nb.sJ] - u[t1 - nb.sJ] + u[t0 + nb.sJ] - u[t0 - nb.sJ]);
+            dze[m] = hz4 * (u[t1 + nb.sK] - u[t1 - nb.sK] + u[t0 + nb.sK] - u[t0 - nb.sK]);
+        } else if (D == 1) {
+            dcs[m] = hc4 * (u[t1 + nb.sI] - u[t1 - nb.sI] + u[t0 + nb.sI] - u[t0 - nb.sI]);
+            det[m] = he * (u[t1] - u[t0]);
+            dze[m] = hz4 * (u[t1 + nb.sK] - u[t1 - nb.sK] + u[t0 + nb.sK] - u[t0 - nb.sK]);
+        } else {
+            dcs[m] = hc4 * (u[t1 + nb.sI] - u[t1 - nb.sI] + u[t0 + nb.sI] - u[t0 - nb.sI]);
+            det[m] = he4 * (u[t1 + nb.sJ] - u[t1 - nb.sJ] + u[t0 + nb.sJ] - u[t0 - nb.sJ]);
+            dze[m] = hz * (u[t1] - u[t0]);
+        }
+    }
+    const double a1 = F.csi[0][t1] + F.csi[0][t0], a2 = F.csi[1][t1] + F.csi[1][t0], a3 = F.csi[2][t1] + F.csi[2][t0];
+    const double b1 = F.eta[0][t1] + F.eta[0][t0], b2 = F.eta[1][t1] + F.eta[1][t0], b3 = F.eta[2][t1] + F.eta[2][t0];
+    const double c1 = F.zet[0][t1] + F.zet[0][t0], c2 = F.zet[1][t1] + F.zet[1][t0], c3 = F.zet[2][t1] + F.zet[2][t0];
+    // uu_r = r-th metric row applied to the gradient of each velocity component
+    double uu[3][3];  // uu[r][m]: r = 1..3 (x,y,z), m = u,v,w
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        uu[0][m] = a1 * dcs[m] + b1 * det[m] + c1 * dze[m];
+        uu[1][m] = a2 * dcs[m] + b2 * det[m] + c2 * dze[m];
+        uu[2][m] = a3 * dcs[m] + b3 * det[m] + c3 * dze[m];
+    }
+    const double f1 = D == 0 ? a1 : D == 1 ? b1 : c1;
+    const double f2 = D == 0 ? a2 : D == 1 ? b2 : c2;
+    const double f3 = D == 0 ? a3 : D == 1 ? b3 : c3;
+    const double cca = f1 * a1 + f2 * a2 + f3 * a3;
+    const double ccb = f1 * b1 + f2 * b2 + f3 * b3;
+    const double ccc = f1 * c1 + f2 * c2 + f3 * c3;
+    const double ret1 = C.rei + F.xnut[t1], ret0 = C.rei + F.xnut[t0];
+    const double vtj = 0.25 * (ret1 + ret0) / (F.aj[t1] + F.aj[t0]);
+    // component m of the flux: face metric . (uu[.][m]) + cc . grad(u_m)
+    // (rhs_viscous.F90:158-163: fv(1) uses uu1,vv1,ww1 = uu[0][0],uu[0][1],uu[0][2])
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        fv[r] = vtj * (f1 * uu[r][0] + f2 * uu[r][1] + f3 * uu[r][2] + cca * dcs[r] + ccb * det[r] + ccc * dze[r]);
+    }
+}
+
+// fv of rhs_diss_p for direction D at node t (rhs_diss_p.F90:58-117)
+template <int D>
+__device__ __forceinline__ double diss_fv(const Level& L, const Fields& F, long t, const Nb& nb) {
+    const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
+    const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
+    const long g = L.off + t;
+    const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
+    const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
+    const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
+    const double gg = m0[g] * m0[g] + m1[g] * m1[g] + m2[g] * m2[g];
+    const double tmp1 = gg * F.dtau[g] / F.aj[g];
+    const double* p = F.q[0];
+    const double tmp2 = (h * h) * (p[g + sD] - 2.0 * p[g] + p[g - sD]);
+    return tmp1 * tmp2;
+}
+
+// contravariant velocity / J, component D, at global node g (rhs_contra_j.F90:14-29)
+template <int D>
+__device__ __forceinline__ double ucn(const Fields& F, long g) {
+    const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
+    const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
+    const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
+    const double tmp = 1.0 / F.aj[g];
+    return tmp * (m0[g] * F.q[1][g] + m1[g] * F.q[2][g] + m2[g] * F.q[3][g]);
+}
+
+// QUICK / first-order upwind contribution of direction D to rh(2:4)
+// (rhs_convec_quick.F90:89-201 and its eta/zet copies)
+template <int D>
+__device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, const Nb& nb, int pos,
+                                       int len, int quick, double out[3]) {
+    const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
+    const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
+    const double ucon = 0.5 * ucn<D>(F, g);
+    const double up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+    // first interior node is pos 1, last is len - 2: first-order there
+    const bool first_order = !quick || pos == 1 || pos == len - 2;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const double* u = F.q[m + 1];
+        if (first_order) {
+            out[m] = h * (up * (u[g] - u[g - sD]) + um * (u[g + sD] - u[g]));
+        } else {
+            out[m] = 0.125 * h *
+                     (up * (3.0 * u[g + sD] + 3.0 * u[g] - 7.0 * u[g - sD] + u[g - 2 * sD]) +
+                      um * (-3.0 * u[g - sD] - 3.0 * u[g] + 7.0 * u[g + sD] - u[g + 2 * sD]));
+        }
+    }
+}
+
+// Residual of solver_daf on one level.  recalc = 0: first evaluation of the
+// pass (dtau and diss(1) fresh, diss(1) stored); recalc = 1: the
+// decide_recalc_rh branch (src/solver_daf.F90:432-484), which reuses the
+// stored diss(1) of the pre-update q.
+__global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc, int calc_pk) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int i = (int)(t % ni);
+    const int j = (int)((t / ni) % nj);
+    const int k = (int)(t / ((long)ni * nj));
+    const long g = L.off + t;
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
+        // rh = 0 on the boundary planes; pk there is left as it is
+        // (rhs_pk adds rh = 0 to it, then the plane is zeroed, :301-306)
+#pragma unroll
+        for (int m = 0; m < 4; m++) F.rh[m][g] = 0.0;
+        return;
+    }
+    const Nb nb{1, (long)ni, (long)ni * nj};
+    const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
+
+    // central pressure gradient (rhs_flux_sans_convec.F90:265-326)
+    double r[4];
+    {
+        const double* p = F.q[0];
+        const double pim = p[g - nb.sI] / F.aj[g - nb.sI], pip = p[g + nb.sI] / F.aj[g + nb.sI];
+        const double pjm = p[g - nb.sJ] / F.aj[g - nb.sJ], pjp = p[g + nb.sJ] / F.aj[g + nb.sJ];
+        const double pkm = p[g - nb.sK] / F.aj[g - nb.sK], pkp = p[g + nb.sK] / F.aj[g + nb.sK];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double v = dc2 * (pip * F.csi[c][g + nb.sI] - pim * F.csi[c][g - nb.sI]);
+            v = v + de2 * (pjp * F.eta[c][g + nb.sJ] - pjm * F.eta[c][g - nb.sJ]);
+            v = v + dz2 * (pkp * F.zet[c][g + nb.sK] - pkm * F.zet[c][g - nb.sK]);
+            r[c + 1] = v;
+        }
+        // divergence of U/J (:333-344)
+        const double dudx = dc2 * (ucn<0>(F, g + nb.sI) - ucn<0>(F, g - nb.sI));
+        const double dvdy = de2 * (ucn<1>(F, g + nb.sJ) - ucn<1>(F, g - nb.sJ));
+        const double dwdz = dz2 * (ucn<2>(F, g + nb.sK) - ucn<2>(F, g - nb.sK));
+        r[0] = dudx + dvdy + dwdz;
+    }
+    // convection: QUICK on the fine level, first-order upwind on coarse levels
+    {
+        const int quick = (n == 1);
+        double fv[3];
+        convec<0>(L, F, g, nb, i, ni, quick, fv);
+        r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
+        convec<1>(L, F, g, nb, j, nj, quick, fv);
+        r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
+        convec<2>(L, F, g, nb, k, nk, quick, fv);
+        r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
+    }
+    // viscous terms (rhs_viscous.F90:112-329)
+    double visc[3];
+    {
+        double fp[3], fm[3];
+        visc_face<0>(L, F, C, t, nb, fp);
+        visc_face<0>(L, F, C, t - nb.sI, nb, fm);
+#pragma unroll
+        for (int c = 0; c < 3; c++) visc[c] = L.dc * (fp[c] - fm[c]);
+        visc_face<1>(L, F, C, t, nb, fp);
+        visc_face<1>(L, F, C, t - nb.sJ, nb, fm);
+#pragma unroll
+        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.de * (fp[c] - fm[c]);
+        visc_face<2>(L, F, C, t, nb, fp);
+        visc_face<2>(L, F, C, t - nb.sK, nb, fm);
+#pragma unroll
+        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.dz * (fp[c] - fm[c]);
+    }
+    // pressure dissipation (rhs_diss_p.F90:58-203); the boundary copies of
+    // fv (:124-129) replace the out-of-domain neighbour by the node itself
+    double diss1;
+    if (!recalc) {
+        const double f0i = diss_fv<0>(L, F, t, nb);
+        const double fmi = (i == 1) ? f0i : diss_fv<0>(L, F, t - nb.sI, nb);
+        const double fpi = (i == ni - 2) ? f0i : diss_fv<0>(L, F, t + nb.sI, nb);
+        const double f0j = diss_fv<1>(L, F, t, nb);
+        const double fmj = (j == 1) ? f0j : diss_fv<1>(L, F, t - nb.sJ, nb);
+        const double fpj = (j == nj - 2) ? f0j : diss_fv<1>(L, F, t + nb.sJ, nb);
+        const double f0k = diss_fv<2>(L, F, t, nb);
+        const double fmk = (k == 1) ? f0k : diss_fv<2>(L, F, t - nb.sK, nb);
+        const double fpk = (k == nk - 2) ? f0k : diss_fv<2>(L, F, t + nb.sK, nb);
+        diss1 = C.pdiss * ((fpi - 2.0 * f0i + fmi) + (fpj - 2.0 * f0j + fmj) + (fpk - 2.0 * f0k + fmk));
+        F.diss1[g] = diss1;
+    } else {
+        diss1 = F.diss1[g];
+    }
+    // unsteady, viscous and dissipation terms (rhs_unst_visc_diss.F90:171-186);
+    // diss(2:4) is zero (rhs_convec_quick.F90:433)
+    {
+        const double dtaj = 2.0 * C.delti * F.aj[g];
+        r[0] = r[0] + diss1;
+#pragma unroll
+        for (int m = 1; m < 4; m++) {
+            const double dqdt = C.e_source * (3.0 * F.q[m][g] - 4.0 * F.qn[m][g] + F.qnm1[m][g]) / dtaj;
+            r[m] = r[m] + 0.0 + dqdt - visc[m - 1];
+        }
+    }
+    // FAS forcing on coarse levels (rhs_pk.F90:232-264)
+    if (n > 1) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            double pk = F.pk[m][g];
+            if (!recalc && calc_pk == 1) {
+                pk = pk - r[m];
+                F.pk[m][g] = pk;
+            }
+            r[m] = r[m] + pk;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
+}
+
+}  // namespace
+
+void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_dtau, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C);
+}
+
+void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int recalc,
+                     int calc_pk) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C, n, recalc,
+                       calc_pk);
+}
+
+}  // namespace desalm

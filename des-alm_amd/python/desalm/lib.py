@@ -1,0 +1,180 @@
+"""ctypes binding of libdesalm.so (include/desalm.h).
+
+This is the Python-side caller of the C ABI used by the tests and bench.py;
+it adds nothing to the computation.  It raises if the HIP library has not been
+built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import Params
+
+PKG_ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+LIB_PATH = os.path.join(PKG_ROOT, "libdesalm.so")
+
+FIELD = {"q": (0, 5), "qn": (1, 5), "qnm1": (2, 5), "qold_mg": (3, 5), "rh": (4, 4), "pk": (5, 4),
+         "xnut": (6, 1), "csi": (7, 3), "eta": (8, 3), "zet": (9, 3), "aj": (10, 1), "dtev": (11, 1),
+         "wd": (12, 1)}
+
+_lib = None
+
+
+class DesalmError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DesalmError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.desalm_init.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(vp)]
+        L.desalm_nodes_all.argtypes = [vp]
+        L.desalm_nodes_all.restype = ctypes.c_int64
+        L.desalm_level_offset.argtypes = [vp, ctypes.c_int]
+        L.desalm_level_offset.restype = ctypes.c_int64
+        L.desalm_set_grid.argtypes = [vp, dp, dp, dp]
+        L.desalm_set_metrics.argtypes = [vp, dp, dp, dp, dp]
+        L.desalm_set_wall_distance.argtypes = [vp, dp, dp]
+        L.desalm_set_state.argtypes = [vp, dp, dp, dp, dp]
+        L.desalm_get_state.argtypes = [vp, dp, dp, dp, dp]
+        L.desalm_mg_driver.argtypes = [vp, ip, dp]
+        L.desalm_solver_daf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [dp] * 11
+        L.desalm_des_eddy.argtypes = [vp] + [dp] * 10
+        L.desalm_gtsv_batch.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ip]
+        L.desalm_get_level_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.desalm_last_driver_ms.argtypes = [vp]
+        L.desalm_last_driver_ms.restype = ctypes.c_double
+        L.desalm_last_pivot_events.argtypes = [vp]
+        L.desalm_last_pivot_events.restype = ctypes.c_int64
+        L.desalm_last_error.argtypes = [vp]
+        L.desalm_last_error.restype = ctypes.c_char_p
+        L.desalm_finalize.argtypes = [vp]
+        for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
+                  "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
+                  "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solver:
+    """One desalm context on one HIP device."""
+
+    def __init__(self, params, device=0):
+        self.params = params.copy()
+        h = ctypes.c_void_p()
+        rc = lib().desalm_init(ctypes.byref(self.params), device, ctypes.byref(h))
+        if rc != 0:
+            raise DesalmError("desalm_init: " + lib().desalm_last_error(None).decode())
+        self.h = h
+        self.nall = lib().desalm_nodes_all(h)
+        self.n1 = params.img * params.jmg * params.kmg
+        self.dims = params.level_dims()
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().desalm_finalize(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise DesalmError(f"{what}: rc={rc}: " + lib().desalm_last_error(self.h).decode())
+
+    def load_case(self, case):
+        self.set_grid(case.x, case.y, case.z)
+        self.set_wall_distance(case.wd)
+        self.set_state(*case.state())
+        return self
+
+    def set_grid(self, x, y, z):
+        x, y, z = _c(x), _c(y), _c(z)
+        self._chk(lib().desalm_set_grid(self.h, _p(x), _p(y), _p(z)), "desalm_set_grid")
+
+    def set_metrics(self, csi, eta, zet, aj):
+        a = [_c(v) for v in (csi, eta, zet, aj)]
+        self._chk(lib().desalm_set_metrics(self.h, *[_p(v) for v in a]), "desalm_set_metrics")
+
+    def set_wall_distance(self, wd, dtev=None):
+        wd, dtev = _c(wd), _c(dtev)
+        self._chk(lib().desalm_set_wall_distance(self.h, _p(wd), _p(dtev)), "desalm_set_wall_distance")
+
+    def set_state(self, q=None, qn=None, qnm1=None, xnut=None):
+        a = [_c(v) for v in (q, qn, qnm1, xnut)]
+        self._chk(lib().desalm_set_state(self.h, *[_p(v) for v in a]), "desalm_set_state")
+
+    def get_state(self):
+        q = np.zeros((self.n1, 5)); qn = np.zeros((self.n1, 5)); qnm1 = np.zeros((self.n1, 5))
+        xnut = np.zeros(self.n1)
+        self._chk(lib().desalm_get_state(self.h, _p(q), _p(qn), _p(qnm1), _p(xnut)), "desalm_get_state")
+        return q, qn, qnm1, xnut
+
+    def mg_driver(self):
+        itc = ctypes.c_int(0)
+        hist = np.zeros((max(self.params.itm, 1), 5))
+        self._chk(lib().desalm_mg_driver(self.h, ctypes.byref(itc), _p(hist)), "desalm_mg_driver")
+        return itc.value, hist[: itc.value]
+
+    def solver_daf(self, n, recalc, calc_pk, q, qn, qnm1, csi, eta, zet, aj, xnut, pk, rh):
+        """In-place on q, pk, rh (float64 C-contiguous arrays of level n)."""
+        args = [q, _c(qn), _c(qnm1), _c(csi), _c(eta), _c(zet), _c(aj), _c(xnut), pk, rh]
+        self._chk(lib().desalm_solver_daf(self.h, n, recalc, calc_pk, *[_p(a) for a in args]), "desalm_solver_daf")
+
+    def des_eddy(self, q, qn5, qnm1_5, csi, eta, zet, aj, wd, dtev, xnut):
+        """In-place on q (n1,5) and xnut (n1)."""
+        args = [q, _c(qn5), _c(qnm1_5), _c(csi), _c(eta), _c(zet), _c(aj), _c(wd), _c(dtev), xnut]
+        self._chk(lib().desalm_des_eddy(self.h, *[_p(a) for a in args]), "desalm_des_eddy")
+
+    def level_field(self, name, n=1):
+        which, nc = FIELD[name]
+        i, j, k = self.dims[n - 1]
+        out = np.zeros((i * j * k, nc)) if nc > 1 else np.zeros(i * j * k)
+        self._chk(lib().desalm_get_level_field(self.h, which, n, _p(out)), "desalm_get_level_field")
+        return out
+
+    @property
+    def last_ms(self):
+        return lib().desalm_last_driver_ms(self.h)
+
+    @property
+    def pivot_events(self):
+        return lib().desalm_last_pivot_events(self.h)
+
+
+def gtsv_batch(dl, d, du, b):
+    """Batched SGTSV on the device.  Arrays (nsys, n); returns (x, info)."""
+    d = np.array(d, dtype=np.float64, copy=True, order="C")
+    nsys, n = d.shape
+    dl = np.array(dl, dtype=np.float64, copy=True, order="C").reshape(nsys, n)
+    du = np.array(du, dtype=np.float64, copy=True, order="C").reshape(nsys, n)
+    b = np.array(b, dtype=np.float64, copy=True, order="C").reshape(nsys, n)
+    info = np.zeros(nsys, dtype=np.int32)
+    rc = lib().desalm_gtsv_batch(n, nsys, _p(dl), _p(d), _p(du), _p(b),
+                                 info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if rc != 0:
+        raise DesalmError("desalm_gtsv_batch: " + lib().desalm_last_error(None).decode())
+    return b, info

@@ -172,6 +172,13 @@ int desalm_get_level_field(desalm_ctx* ctx, int which, int n, double* out);
  * events on the context's stream). */
 double desalm_last_driver_ms(const desalm_ctx* ctx);
 
+/* Rows, summed over every line solve of the last desalm_mg_driver /
+ * desalm_solver_daf / desalm_des_eddy call, at which LAPACK SGTSV would have
+ * interchanged rows (|d_i| < |dl_i|).  The line solves eliminate without
+ * interchanges; a non-zero count says their row order departed from the
+ * reference's SGTSV on that call. */
+int64_t desalm_last_pivot_events(const desalm_ctx* ctx);
+
 const char* desalm_last_error(const desalm_ctx* ctx);
 int desalm_finalize(desalm_ctx* ctx);
 

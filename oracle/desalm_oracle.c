@@ -62,6 +62,14 @@ static double dmax2(double a, double b) { return a > b ? a : b; }
 /* LAPACK SGTSV (vendored LAPACK 3.0, src/sgtsv.f:89-259), FP64.        */
 /* dl(1..n-1), d(1..n), du(1..n-1), b(ldb, nrhs); 0-based storage.     */
 /* ------------------------------------------------------------------ */
+static long g_pivots = 0; /* row interchanges done by or_gtsv since the last reset */
+
+long or_gtsv_pivots(int reset) {
+    long v = g_pivots;
+    if (reset) g_pivots = 0;
+    return v;
+}
+
 int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb) {
     int i, jr;
     double fact, temp;
@@ -81,6 +89,7 @@ int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int l
             }
             dl[i] = 0.0;
         } else {
+            g_pivots++;
             fact = d[i] / dl[i];
             d[i] = dl[i];
             temp = d[i + 1];
@@ -106,6 +115,7 @@ int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int l
                 return i + 1;
             }
         } else {
+            g_pivots++;
             fact = d[i] / dl[i];
             d[i] = dl[i];
             temp = d[i + 1];

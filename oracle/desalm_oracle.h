@@ -55,6 +55,8 @@ int or_mg_residual(const or_state* o, double* residual);
 
 /* LAPACK SGTSV algorithm in FP64 (src/sgtsv.f:1-263); returns INFO */
 int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb);
+/* row interchanges performed by or_gtsv so far (reset to 0 if reset != 0) */
+long or_gtsv_pivots(int reset);
 
 #ifdef __cplusplus
 }
