@@ -19,6 +19,27 @@
 
 using namespace desalm;
 
+// Per-kernel device timing with HIP events on the context stream.
+enum KernelId {
+    KID_DTAU, KID_RESIDUAL, KID_RESIDUAL_RECALC, KID_DAF_I, KID_DAF_J, KID_DAF_K, KID_BCOND,
+    KID_DES_EDDY, KID_NLEVEL, KID_PROLONG, KID_RESID, KID_COUNT
+};
+static const char* kKernelNames[KID_COUNT] = {
+    "dtau", "residual", "residual_recalc", "daf_i", "daf_j", "daf_k", "bcond_fm",
+    "des_eddy", "nlevel", "prolong", "resid_norm"};
+
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int id, level;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[KID_COUNT][kMaxLevels + 1] = {};
+    long long count[KID_COUNT][kMaxLevels + 1] = {};
+};
+
 struct desalm_ctx {
     desalm_params p;
     int device = 0;
@@ -37,6 +58,7 @@ struct desalm_ctx {
     unsigned long long last_pivots = 0;
     double init_resd[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Profiler prof;
 };
 
 namespace {
@@ -106,15 +128,59 @@ void rs_factor(double ep, int len, double* out) {
     }
 }
 
+hipEvent_t prof_event(desalm_ctx* c) {
+    if (!c->prof.pool.empty()) {
+        hipEvent_t e = c->prof.pool.back();
+        c->prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs `launch` between two events when profiling is on.
+template <class F>
+void timed(desalm_ctx* c, int id, int level, F&& launch) {
+    if (!c->prof.on) {
+        launch();
+        return;
+    }
+    hipEvent_t a = prof_event(c), b = prof_event(c);
+    if (!a || !b) {
+        launch();
+        return;
+    }
+    (void)hipEventRecord(a, c->stream);
+    launch();
+    (void)hipEventRecord(b, c->stream);
+    c->prof.pending.push_back({id, level, a, b});
+}
+
+// Accumulates finished records; call after the stream has been synchronised.
+void prof_flush(desalm_ctx* c) {
+    for (auto& r : c->prof.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->prof.ms[r.id][r.level] += ms;
+            c->prof.count[r.id][r.level] += 1;
+        }
+        c->prof.pool.push_back(r.a);
+        c->prof.pool.push_back(r.b);
+    }
+    c->prof.pending.clear();
+}
+
 void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out) {
     const Level& L = c->lv[n];
-    launch_dtau(c->stream, L, c->F, c->C);
-    launch_residual(c->stream, L, c->F, c->C, n, 0, calc_pk);
-    launch_daf_i(c->stream, L, c->F, c->C, n);
-    launch_daf_j(c->stream, L, c->F, c->C, n);
-    launch_daf_k(c->stream, L, c->F, c->C, n, dq_out);
-    if (n == 1) launch_bcond_fm(c->stream, L, c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix);
-    if (recalc == 1) launch_residual(c->stream, L, c->F, c->C, n, 1, 0);
+    hipStream_t s = c->stream;
+    timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C); });
+    timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk); });
+    timed(c, KID_DAF_I, n, [&] { launch_daf_i(s, L, c->F, c->C, n); });
+    timed(c, KID_DAF_J, n, [&] { launch_daf_j(s, L, c->F, c->C, n); });
+    timed(c, KID_DAF_K, n, [&] { launch_daf_k(s, L, c->F, c->C, n, dq_out); });
+    if (n == 1) timed(c, KID_BCOND, n, [&] { launch_bcond_fm(s, L, c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix); });
+    if (recalc == 1) timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0); });
 }
 
 int copy_in_level(desalm_ctx* c, double* const* soa, int ncomp, const double* host, const Level& L) {
@@ -353,25 +419,29 @@ int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
     for (int it = 1; it <= p.itm; it++) {
         itc++;
         for (int n = 1; n <= c->ng; n++) {
-            if (n > 1) launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, p.icrs, p.jcrs, p.kcrs);
+            if (n > 1)
+                timed(c, KID_NLEVEL, n, [&] { launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, p.icrs, p.jcrs, p.kcrs); });
             for (int itr = 1; itr <= p.iter[n - 1]; itr++) {
                 if (n != c->ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
-                if (p.des && n == 1) launch_des_eddy(s, L1, c->F, c->C, p.itk);
+                if (p.des && n == 1) timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, L1, c->F, c->C, p.itk); });
                 solver_daf(c, n, decide_recalc_rh, itr, nullptr);
             }
         }
         for (int n = c->ng - 1; n >= 1; n--) {
             const long per_level = 3 * 2 * c->maxdim;
-            launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, p.icrs, p.jcrs, p.kcrs,
-                           c->rs_tab + (n - 1) * per_level);
+            timed(c, KID_PROLONG, n, [&] {
+                launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, p.icrs, p.jcrs, p.kcrs,
+                               c->rs_tab + (n - 1) * per_level);
+            });
         }
-        launch_bcond_fm(s, L1, c->F, c->C, p.ifix, p.jfix, p.kfix);
+        timed(c, KID_BCOND, 1, [&] { launch_bcond_fm(s, L1, c->F, c->C, p.ifix, p.jfix, p.kfix); });
         // update norms, then qold_mg <- q for the next pseudo-iteration
-        launch_resid_and_save(s, L1, c->F, 1);
+        timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, L1, c->F, 1); });
         HIPCHK(c, hipGetLastError());
         double resd[5];
         HIPCHK(c, hipMemcpyAsync(resd, c->F.resid, sizeof(resd), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
+        prof_flush(c);
         double residual[5];
         for (int m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
         if (resid_out)
@@ -521,13 +591,36 @@ double desalm_last_driver_ms(const desalm_ctx* c) { return c ? c->last_ms : -1.0
 
 int64_t desalm_last_pivot_events(const desalm_ctx* c) { return c ? (int64_t)c->last_pivots : -1; }
 
+int desalm_profile(desalm_ctx* c, int enable) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    c->prof.on = enable != 0;
+    memset(c->prof.ms, 0, sizeof(c->prof.ms));
+    memset(c->prof.count, 0, sizeof(c->prof.count));
+    return 0;
+}
+
+int desalm_profile_get(const desalm_ctx* c, int kernel, int level, int64_t* count, double* total_ms) {
+    if (!c || kernel < 0 || kernel >= KID_COUNT || level < 1 || level > kMaxLevels) return -1;
+    if (count) *count = c->prof.count[kernel][level];
+    if (total_ms) *total_ms = c->prof.ms[kernel][level];
+    return 0;
+}
+
+const char* desalm_profile_name(int kernel) {
+    return (kernel >= 0 && kernel < KID_COUNT) ? kKernelNames[kernel] : nullptr;
+}
+
 const char* desalm_last_error(const desalm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
 int desalm_finalize(desalm_ctx* c) {
     if (!c) return 0;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* p : c->allocs) hipFree(p);
+    prof_flush(c);
+    for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
+    for (void* p : c->allocs) (void)hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -45,7 +45,7 @@ def lib():
         L.desalm_set_state.argtypes = [vp, dp, dp, dp, dp]
         L.desalm_get_state.argtypes = [vp, dp, dp, dp, dp]
         L.desalm_mg_driver.argtypes = [vp, ip, dp]
-        L.desalm_solver_daf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [dp] * 11
+        L.desalm_solver_daf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [dp] * 10
         L.desalm_des_eddy.argtypes = [vp] + [dp] * 10
         L.desalm_gtsv_batch.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ip]
         L.desalm_get_level_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
@@ -53,6 +53,13 @@ def lib():
         L.desalm_last_driver_ms.restype = ctypes.c_double
         L.desalm_last_pivot_events.argtypes = [vp]
         L.desalm_last_pivot_events.restype = ctypes.c_int64
+        L.desalm_profile.argtypes = [vp, ctypes.c_int]
+        L.desalm_profile.restype = ctypes.c_int
+        L.desalm_profile_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_double)]
+        L.desalm_profile_get.restype = ctypes.c_int
+        L.desalm_profile_name.argtypes = [ctypes.c_int]
+        L.desalm_profile_name.restype = ctypes.c_char_p
         L.desalm_last_error.argtypes = [vp]
         L.desalm_last_error.restype = ctypes.c_char_p
         L.desalm_finalize.argtypes = [vp]
@@ -154,6 +161,26 @@ class Solver:
         i, j, k = self.dims[n - 1]
         out = np.zeros((i * j * k, nc)) if nc > 1 else np.zeros(i * j * k)
         self._chk(lib().desalm_get_level_field(self.h, which, n, _p(out)), "desalm_get_level_field")
+        return out
+
+    def profile(self, enable=True):
+        self._chk(lib().desalm_profile(self.h, 1 if enable else 0), "desalm_profile")
+
+    def profile_stats(self):
+        """{(kernel_name, level): (launches, total_ms)} of the recorded launches."""
+        out = {}
+        kid = 0
+        while True:
+            nm = lib().desalm_profile_name(kid)
+            if nm is None:
+                break
+            for n in range(1, self.params.ng + 1):
+                cnt = ctypes.c_int64(0)
+                ms = ctypes.c_double(0.0)
+                lib().desalm_profile_get(self.h, kid, n, ctypes.byref(cnt), ctypes.byref(ms))
+                if cnt.value:
+                    out[(nm.decode(), n)] = (cnt.value, ms.value)
+            kid += 1
         return out
 
     @property

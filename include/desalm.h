@@ -179,6 +179,14 @@ double desalm_last_driver_ms(const desalm_ctx* ctx);
  * reference's SGTSV on that call. */
 int64_t desalm_last_pivot_events(const desalm_ctx* ctx);
 
+/* Device time per kernel family and level, from HIP events recorded on the
+ * context's stream around each launch inside desalm_mg_driver.
+ * desalm_profile(ctx, 1) clears the counters and starts recording, 0 stops.
+ * kernel: 0 .. desalm_profile_name() returns NULL; level 1-based. */
+int desalm_profile(desalm_ctx* ctx, int enable);
+int desalm_profile_get(const desalm_ctx* ctx, int kernel, int level, int64_t* count, double* total_ms);
+const char* desalm_profile_name(int kernel);
+
 const char* desalm_last_error(const desalm_ctx* ctx);
 int desalm_finalize(desalm_ctx* ctx);
 
