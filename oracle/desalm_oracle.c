@@ -1273,6 +1273,41 @@ static int solver_daf(or_state* o, int n, int recalc, int calc_pk) {
     return 0;
 }
 
+/* residual assembly only (solver_daf.F90:195-315): rh of level n from the
+ * stored state, nothing else changed (used by the closed-form checks) */
+int or_residual(or_state* o, int n, int calc_pk) {
+    daf_t dd, *d = &dd;
+    if (n < 1 || n > o->ng) return -1;
+    {
+        LEVEL(n);
+        long N = (long)ni * nj * nk;
+        if (ni < 5 || nj < 5 || nk < 5) return -1;
+        memset(d, 0, sizeof(*d));
+        d->o = o; d->n = n; d->ni = ni; d->nj = nj; d->nk = nk;
+        d->dc = o->dc[n]; d->de = o->de[n]; d->dz = o->dz[n];
+        d->is = 2; d->ie = ni - 1; d->js = 2; d->je = nj - 1; d->ks = 2; d->ke = nk - 1;
+        d->idend = d->ie;
+        d->q = o->q + 5 * off; d->qn = o->qn + 5 * off; d->qnm1 = o->qnm1 + 5 * off;
+        d->rh = o->rh + 4 * off; d->pk = o->pk + 4 * off;
+        d->csi = o->csi + 3 * off; d->eta = o->eta + 3 * off; d->zet = o->zet + 3 * off;
+        d->aj = o->aj + off; d->xnut = o->xnut + off;
+        d->diss = zalloc(4 * N); d->visc = zalloc(3 * N); d->ucn = zalloc(3 * N); d->fv = zalloc(3 * N);
+        d->dtau = zalloc(N);
+        memset(d->rh, 0, 4 * N * sizeof(double));
+        rhs_contra_j(d);
+        rhs_daf_dtau(d);
+        rhs_viscous(d);
+        rhs_diss_p(d);
+        rhs_flux_sans_convec(d);
+        rhs_convec_quick(d, n == 1 ? 2 : 1);
+        rhs_unst_visc_diss(d);
+        if (n > 1) rhs_pk(d, calc_pk == 1 ? 1 : 0);
+        zero_rh_boundaries(d);
+        free(d->diss); free(d->visc); free(d->ucn); free(d->fv); free(d->dtau);
+    }
+    return 0;
+}
+
 int or_solver_daf(or_state* o, int n, int decide_recalc_rh, int decide_calc_pk) {
     if (n < 1 || n > o->ng) return -1;
     return solver_daf(o, n, decide_recalc_rh, decide_calc_pk);

@@ -48,6 +48,10 @@ def lib():
         L.or_mg_driver.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
         L.or_solver_daf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_des_eddy.argtypes = [vp]
+        L.or_residual.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_residual.restype = ctypes.c_int
+        L.or_gtsv_pivots.argtypes = [ctypes.c_int]
+        L.or_gtsv_pivots.restype = ctypes.c_long
         L.or_mg_nlevel.argtypes = [vp, ctypes.c_int]
         L.or_mg_prolong.argtypes = [vp, ctypes.c_int]
         L.or_mg_residual.argtypes = [vp, dp]
@@ -132,6 +136,10 @@ class Oracle:
     def solver_daf(self, n, recalc, calc_pk):
         assert lib().or_solver_daf(self.h, n, recalc, calc_pk) == 0
 
+    def residual_only(self, n=1, calc_pk=0):
+        assert lib().or_residual(self.h, n, calc_pk) == 0
+        return self.get("rh")
+
     def des_eddy(self):
         assert lib().or_des_eddy(self.h) == 0
 
@@ -145,6 +153,11 @@ class Oracle:
         r = np.zeros(5)
         lib().or_mg_residual(self.h, _p(r))
         return r
+
+
+def gtsv_pivots(reset=True):
+    """Row interchanges done by the oracle's SGTSV since the last reset."""
+    return lib().or_gtsv_pivots(1 if reset else 0)
 
 
 def gtsv(dl, d, du, b):

@@ -1,0 +1,69 @@
+"""Writes the golden fixtures of tests/golden/ from the reference Fortran.
+
+The reference's own sources are compiled in FP64 by oracle/ref/build_ref.sh
+(this container only; the sources never leave /root/reference) and run on
+small seeded inputs built by desalm.cases.  Each fixture stores the case
+recipe, a fingerprint of its inputs and the reference outputs.  Run:
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, os.path.join(ROOT, "des-alm_amd", "python"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from desalm import cases  # noqa: E402
+import pyoracle as po  # noqa: E402
+
+# name -> (case builder, builder kwargs, harness mode and options)
+FIXTURES = {
+    "wavy_solver_daf": ("wavy", dict(ni=13, nj=11, nk=9, ng=2, itm=2), dict(mode=2, recalc=0, calc_pk=1)),
+    "wavy_solver_daf_recalc": ("wavy", dict(ni=13, nj=11, nk=9, ng=2, itm=2), dict(mode=2, recalc=1, calc_pk=1)),
+    "wavy_des_eddy": ("wavy", dict(ni=13, nj=11, nk=9, ng=2, itm=2, des=True, itk=2), dict(mode=3)),
+    "wavy_metrics": ("wavy", dict(ni=13, nj=11, nk=9, ng=3, itm=1), dict(mode=4)),
+    "wavy_mg_driver_ng3_sa": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=3, des=True, iters=(1, 2, 3)),
+                              dict(mode=1, nsteps=1)),
+    "channelbc_mg_driver_ng3_sa": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=3, des=True, iters=(1, 2, 3),
+                                                btype=(6, 6, 1, 1, 6, 6)), dict(mode=1, nsteps=2)),
+    "cavity17_mg_driver_ng2": ("cavity", dict(n=17, ng=2, itm=8), dict(mode=1, nsteps=1)),
+}
+
+
+def build(kind, kw):
+    return getattr(cases, kind)(**kw)
+
+
+def fingerprint(case):
+    h = hashlib.sha256()
+    for a in (case.x, case.y, case.z, case.wd, case.q, case.qn, case.qnm1, case.xnut):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()
+
+
+def main():
+    if not po.ref_available():
+        sys.exit("oracle/_ref/desalm_ref_harness is not built (needs /root/reference)")
+    for name, (kind, kw, opts) in FIXTURES.items():
+        case = build(kind, kw)
+        out = po.run_reference(case, **opts)
+        nall_c = case.params.nodes_all()
+        keep = {"q": out["q"], "xnut": out["xnut"]}
+        if opts["mode"] == 2:
+            keep["rh"] = out["rh"]
+        if opts["mode"] == 4:
+            for nm in ("csi", "eta", "zet", "aj", "dtev"):
+                keep[nm] = out[nm][:nall_c]
+        meta = {"kind": kind, "kwargs": {k: list(v) if isinstance(v, tuple) else v for k, v in kw.items()},
+                "options": opts, "inputs_sha256": fingerprint(case)}
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=json.dumps(meta), **keep)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

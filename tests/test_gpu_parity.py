@@ -149,4 +149,5 @@ def test_gtsv_batch_matches_lapack_algorithm(gpu):
         xo, io = po.gtsv(dl[s, :n - 1], d[s], du[s, :n - 1], b[s])
         assert info[s] == io
         if io == 0:
-            assert np.max(np.abs(x[s] - xo[:, 0])) <= 1e-12 * max(1.0, np.max(np.abs(xo)))
+            xo = np.ravel(xo)
+            assert np.max(np.abs(x[s] - xo)) <= 1e-12 * max(1.0, np.max(np.abs(xo)))
