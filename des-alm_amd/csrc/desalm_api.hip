@@ -283,8 +283,16 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     for (int m = 0; m < 4; m++) { F.rh[m] = A(c->nall); F.pk[m] = A(c->nall); }
     F.xnut = A(c->nall); F.wd = A(c->nall); F.dtev = A(c->nall);
     F.dtau = A(c->nall); F.diss1 = A(c->nall);
-    for (int m = 0; m < 3; m++) F.cp[m] = A(c->n1);
+    for (int m = 0; m < 4; m++) F.cp[m] = A(c->n1);
     for (int m = 0; m < 4; m++) F.dp[m] = A(c->n1);
+    F.sd = A(c->n1);
+    {
+        const Level& L1 = c->lv[1];
+        long ml = std::max((long)(L1.nj - 2) * (L1.nk - 2), std::max((long)(L1.ni - 2) * (L1.nk - 2), (long)(L1.ni - 2) * (L1.nj - 2)));
+        F.brh = A(8 * ml);
+    }
+    for (int m = 0; m < 3; m++) F.av[m] = A(c->n1);
+    F.dtj = A(c->n1);
     for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
     F.wo = A(c->n1); F.rh5 = A(c->n1);
     F.resid_part = A(5L * kResidBlocks);

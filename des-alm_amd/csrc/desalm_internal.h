@@ -54,8 +54,12 @@ struct Fields {
     // scratch
     double* dtau;     // all: local pseudo time step of the level being smoothed
     double* diss1;    // all: pressure dissipation kept for the recalc branch
-    double* cp[3];    // fine: line-solve c' of the three distinct DAF matrices
-    double* dp[4];    // fine: line-solve d' (k sweep, prolongation smoothing)
+    double* cp[4];    // fine: line-solve c' per scalar system
+    double* dp[4];    // fine: sweep right-hand sides / d', prolongation dq
+    double* sd;       // fine: dtau * spectral radius of the sweep direction
+    double* brh;      // 8 per line: semi-explicit boundary vectors of a sweep
+    double* av[3];    // fine: SA ADI face coefficients per direction
+    double* dtj;      // fine: SA ADI 0.5 dtev J
     double* ucn[3];   // fine: contravariant velocity / J for the SA update
     double* wo;       // fine: SA vorticity measure
     double* rh5;      // fine: SA right-hand side

@@ -1,18 +1,17 @@
 // k_daf.hip -- diagonalized approximate-factorization implicit operator.
 //
 // rhs_modal_matrices + rhs_diag_solver (src/rhs_modal_matrices.F90,
-// src/rhs_diag_solver.F90) as three line-sweep kernels:
-//   i sweep: S scaling, Ma^-1, i-direction tridiagonal solves
-//   j sweep: N1^-1 = Mb^-1 Ma, j-direction solves
-//   k sweep: N2^-1 = Mc^-1 Mb, k-direction solves, Mc, and q = q + dq
-// The eigenvector matrices are rebuilt in registers from the metrics and
-// dtau of each node instead of being stored (the reference keeps 67 words
-// per node).  One thread owns one grid line and runs a Thomas elimination
-// on its four scalar systems (three distinct matrices: eigenvalue
-// multipliers 0, 0, +1, -1).  LAPACK SGTSV, which the reference calls, would
-// pick the same pivots as Thomas unless |d_i| < |dl_i|; rows where that
-// happens are counted in F.pivots so a caller can see when the pivot-free
-// elimination departed from SGTSV's row order.
+// src/rhs_diag_solver.F90).  Each direction D is two kernels:
+//   k_daf_pre<D>  pointwise: the modal transform that precedes the sweep
+//                 (S and Ma^-1 for i, N1^-1 = Mb^-1 Ma for j, N2^-1 = Mc^-1 Mb
+//                 for k), dtau*spr_D per node, and the boundary vectors brh
+//   k_daf_tri<D>  one thread per (line, scalar system): Thomas elimination
+// and k_daf_post applies Mc and adds dq to q.  The eigenvector matrices are
+// rebuilt in registers from the metrics and dtau of each node instead of
+// being stored (the reference keeps 67 words per node).  LAPACK SGTSV, which
+// the reference calls, picks the same pivots as Thomas unless |d_i| <
+// |dl_i|; rows where that happens are counted in F.pivots so a caller can see
+// when the pivot-free elimination departed from SGTSV's row order.
 #include "desalm_internal.h"
 
 namespace desalm {
@@ -125,316 +124,244 @@ __device__ __forceinline__ double spr_at(const Fields& F, const Consts& C, long 
     return modal_theta(C, F.dtau[g]) * sqrt(gg);
 }
 
-// Line-sweep state for the three distinct DAF matrices
-struct Thomas3 {
-    double cp[3];   // c' of matrices 0 (lambda 0), 1 (lambda +1), 2 (lambda -1)
-    double dp[4];   // d' of the four systems
-    double den[3];  // modified diagonal of the previous row
-};
-
-// one forward row: coefficients (a_X, b, c_X), right-hand sides aq
-__device__ __forceinline__ void thomas_row(Thomas3& T, bool first, const double a[3], double b, const double c[3],
-                                           const double aq[4], unsigned& piv) {
-    double inv[3];
-#pragma unroll
-    for (int x = 0; x < 3; x++) {
-        double den;
-        if (first) {
-            den = b;
-        } else {
-            // SGTSV would interchange rows when |d_prev| < |dl|
-            if (fabs(T.den[x]) < fabs(a[x])) piv++;
-            den = b - a[x] * T.cp[x];
-        }
-        T.den[x] = den;
-        inv[x] = 1.0 / den;
-        T.cp[x] = c[x] * inv[x];
-    }
-    const int mat[4] = {0, 0, 1, 2};
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-        const int x = mat[m];
-        T.dp[m] = first ? aq[m] * inv[x] : (aq[m] - a[x] * T.dp[m]) * inv[x];
-    }
-}
-
-// coefficients of one DAF row in direction D (rhs_diag_solver.F90:152-187)
-__device__ __forceinline__ void daf_coefs(double dtau0, double spr0, double dtaum, double sprm, double dtaup,
-                                          double sprp, double epd, double h, double a[3], double& b,
-                                          double c[3]) {
-    const double h2 = 0.5 * h, hsq = h * h;
-    const double dtc = (dtau0 == 0.0) ? 0.0 : 1.0;
-    const double tmp = dtau0 * epd * spr0 * hsq;
-    b = 1.0 + 2.0 * tmp;
-    const double ddw = dtc * dtaum * h2 * sprm;
-    const double dde = dtc * dtaup * h2 * sprp;
-    a[0] = -tmp;
-    a[1] = -ddw * 1.0 - tmp;
-    a[2] = -ddw * -1.0 - tmp;
-    c[0] = -tmp;
-    c[1] = dde * 1.0 - tmp;
-    c[2] = dde * -1.0 - tmp;
-}
-
-// ---- i sweep (rhs_diag_solver.F90:51-258) ----
-__global__ void __launch_bounds__(128) k_daf_i(Level L, Fields F, Consts C, int use_b1, int use_b2) {
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long nlines = (long)(nj - 2) * (nk - 2);
+// ---- pointwise transform before a sweep (rhs_diag_solver.F90:56-122,
+// :300-311, :476-487) ----
+// For every node of the level: sd = dtau * spr_D, the only per-node input
+// the tridiagonal rows need.  For interior nodes: the right-hand side of the
+// sweep, written to F.dp:
+//   D = 0: Ma^-1 S (-dtau J rh),  D = 1: N1^-1 rh,  D = 2: N2^-1 rh.
+// The thread at the first (last) interior node of a line also builds the
+// semi-explicit boundary vector brh of that line side (fine level only)
+// from the untransformed rh, into F.brh[line * 8 + side * 4 + m].
+template <int D>
+__global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, int use_b1, int use_b2,
+                                                 int b2_full) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlines) return;
-    const int j = 1 + (int)(t % (nj - 2));
-    const int k = 1 + (int)(t / (nj - 2));
-    const long base = L.off + (long)ni * (j + (long)nj * k);  // global index of (0, j, k)
-    const long lbase = (long)ni * (j + (long)nj * k);          // level-local index of (0, j, k)
-    const int len = ni - 2;
+    if (t >= L.n) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    const long g = L.off + t;
+    F.sd[t] = F.dtau[g] * spr_at<D>(F, C, g);
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+    const long sI = 1, sJ = ni, sK = (long)ni * nj;
     const double alpha_mult = C.e_source * 1.5 / C.delti;
-    const double epd = C.ep[0][0];
-    const double h = L.dc;
-
     // S (-dtau J rh) at a node (:56-72)
-    auto scaled = [&](long g, double v[4]) {
-        const double dt = F.dtau[g];
+    auto scaled = [&](long gg, double v[4]) {
+        const double dt = F.dtau[gg];
         const double alpha = 1.0 + alpha_mult * dt;
         double r[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) r[m] = -dt * F.rh[m][g] * F.aj[g];
+        for (int m = 0; m < 4; m++) r[m] = -dt * F.rh[m][gg] * F.aj[gg];
         v[0] = C.beta * r[0];
 #pragma unroll
         for (int m = 1; m < 4; m++) v[m] = 1.0 / alpha * r[m];
     };
-
-    double brh1[4] = {0, 0, 0, 0};
-    double ovf = 0.0;
-    double s1[4], s2[4], s3[4];
-    scaled(base + 1, s1);
-    scaled(base + 2, s2);
-    if (use_b1) {
-        scaled(base + 3, s3);
+    double v[4];
+    if (D == 0) {
+        scaled(g, v);
+        apply_mai(F, C, g, v);
+    } else {
 #pragma unroll
-        for (int m = 0; m < 4; m++) brh1[m] = C.sa[0][m] * s1[m] + C.sb[0][m] * s2[m];
-        // fifth product of sa(1:5,1)*rh(1:4,..) (:97) lands in brh(1,2,j,k)
-        ovf = C.sa[0][4] * s2[0] + C.sb[0][4] * s3[0];
-        apply_mai(F, C, base, brh1);
+        for (int m = 0; m < 4; m++) v[m] = F.rh[m][g];
+        if (D == 1) apply_n1i(F, C, g, v);
+        else apply_n2i(F, C, g, v);
     }
-
-    Thomas3 T;
-    unsigned piv = 0;
-    double dtm = F.dtau[base], sprm = spr_at<0>(F, C, base);
-    double dt0 = F.dtau[base + 1], spr0 = spr_at<0>(F, C, base + 1);
-    for (int s = 0; s < len; s++) {
-        const long g = base + 1 + s;
-        const double dtp = F.dtau[g + 1], sprp = spr_at<0>(F, C, g + 1);
-        double aq[4];
-        if (s == 0) {
 #pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = s1[m];
-        } else if (s == 1) {
-#pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = s2[m];
+    for (int m = 0; m < 4; m++) F.dp[m][t] = v[m];
+    if (!use_b1 && !use_b2) return;
+    const int p = D == 0 ? i : D == 1 ? j : k;
+    const int nd = D == 0 ? ni : D == 1 ? nj : nk;
+    const long sD = D == 0 ? sI : D == 1 ? sJ : sK;
+    const long line = D == 0 ? (long)(j - 1) + (long)(nj - 2) * (k - 1)
+                    : D == 1 ? (long)(i - 1) + (long)(ni - 2) * (k - 1)
+                             : (long)(i - 1) + (long)(ni - 2) * (j - 1);
+    double* brh = F.brh + line * 8;
+    const int b0 = 2 * D, b1 = 2 * D + 1;
+    if (p == 1 && use_b1) {
+        double r1[4], r2[4], w[4];
+        if (D == 0) {
+            double r3[4];
+            scaled(g, r1);
+            scaled(g + sD, r2);
+            scaled(g + 2 * sD, r3);
+            // fifth product of sa(1:5,1)*rh(1:4,..) (:97): lands in brh(1,2,j,k)
+            const double ovf = C.sa[0][4] * r2[0] + C.sb[0][4] * r3[0];
+            brh[4] = ovf; brh[5] = 0.0; brh[6] = 0.0; brh[7] = 0.0;
         } else {
-            scaled(g, aq);
-        }
-        apply_mai(F, C, g, aq);
-        double a[3], b, c[3];
-        daf_coefs(dt0, spr0, dtm, sprm, dtp, sprp, epd, h, a, b, c);
-        if (s == 0 && use_b1) {
-            const double aw4[4] = {a[0], a[0], a[1], a[2]};
 #pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = aq[m] - aw4[m] * brh1[m];
+            for (int m = 0; m < 4; m++) { r1[m] = F.rh[m][g]; r2[m] = F.rh[m][g + sD]; }
         }
-        if (s == len - 1 && use_b2) {
-            // the i = imax correction is built only on interior ranks (:102)
-            // and stays zero except for the overflowed component
-            aq[0] = aq[0] - c[0] * ovf;
+#pragma unroll
+        for (int m = 0; m < 4; m++) w[m] = C.sa[b0][m] * r1[m] + C.sb[b0][m] * r2[m];
+        if (D == 0) apply_mai(F, C, g - sD, w);
+        else if (D == 1) apply_n1i(F, C, g - sD, w);
+        else apply_n2i(F, C, g - sD, w);
+#pragma unroll
+        for (int m = 0; m < 4; m++) brh[m] = w[m];
+        if (D == 2 && !b2_full && use_b2) {
+            // brh(:,2) is not rebuilt when btype(6) == 6 (:467) and keeps the
+            // fifth product of the k = 1 build (:462), which reads rh(1) of node i+1
+            double o[4] = {C.sa[4][4] * F.rh[0][g + sI] + C.sb[4][4] * F.rh[0][g + sI + sD], 0.0, 0.0, 0.0};
+            apply_n2i(F, C, g + (long)(nd - 2) * sD, o);
+#pragma unroll
+            for (int m = 0; m < 4; m++) brh[4 + m] = o[m];
         }
-        thomas_row(T, s == 0, a, b, c, aq, piv);
-        const long lt = lbase + 1 + s;
-#pragma unroll
-        for (int x = 0; x < 3; x++) F.cp[x][lt] = T.cp[x];
-#pragma unroll
-        for (int m = 0; m < 4; m++) F.rh[m][g] = T.dp[m];
-        dtm = dt0; sprm = spr0; dt0 = dtp; spr0 = sprp;
     }
-    // back substitution
-    double x[4];
+    if (p == nd - 2 && use_b2 && (D == 1 || (D == 2 && b2_full))) {
+        double w[4];
 #pragma unroll
-    for (int m = 0; m < 4; m++) x[m] = T.dp[m];
-    const int mat[4] = {0, 0, 1, 2};
-    for (int s = len - 2; s >= 0; s--) {
-        const long g = base + 1 + s;
-        const long lt = lbase + 1 + s;
+        for (int m = 0; m < 4; m++) w[m] = C.sa[b1][m] * F.rh[m][g] + C.sb[b1][m] * F.rh[m][g - sD];
+        if (D == 1) apply_n1i(F, C, g + sD, w);
+        else apply_n2i(F, C, g + sD, w);
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            x[m] = F.rh[m][g] - F.cp[mat[m]][lt] * x[m];
-            F.rh[m][g] = x[m];
+        for (int m = 0; m < 4; m++) brh[4 + m] = w[m];
+    }
+}
+
+// ---- one scalar tridiagonal system per thread (rhs_diag_solver.F90:149-249
+// and its j/k copies, the four SGTSV calls of a line) ----
+// blockIdx.y = system m (eigenvalue multiplier 0, 0, +1, -1).  Rows are
+// rebuilt from sd = dtau spr of the node and its two neighbours:
+//   tmp = sd_s ep h^2, dds = dtc sd_(s-+1) h/2,
+//   a = -lambda dds_w - tmp, b = 1 + 2 tmp, c = lambda dds_e - tmp.
+// Thomas elimination: c' in F.cp[m], d' over the input in F.dp[m], the
+// solution into rh.  Loads run one group of four rows ahead of the
+// elimination.
+constexpr int kPF = 4;
+
+template <int D>
+__global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, int use_b1, int use_b2) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int m = blockIdx.y;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nlines, base, sD;
+    int len;
+    if (D == 0) {
+        nlines = (long)(nj - 2) * (nk - 2);
+        if (t >= nlines) return;
+        base = (long)ni * ((1 + t % (nj - 2)) + (long)nj * (1 + t / (nj - 2)));
+        sD = 1;
+        len = ni - 2;
+    } else if (D == 1) {
+        nlines = (long)(ni - 2) * (nk - 2);
+        if (t >= nlines) return;
+        base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
+        sD = ni;
+        len = nj - 2;
+    } else {
+        nlines = (long)(ni - 2) * (nj - 2);
+        if (t >= nlines) return;
+        base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
+        sD = (long)ni * nj;
+        len = nk - 2;
+    }
+    const double lam = m == 2 ? 1.0 : m == 3 ? -1.0 : 0.0;
+    const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
+    const double h2 = 0.5 * h, hsq = h * h, epd = C.ep[0][D];
+    const double* __restrict__ sd = F.sd;
+    double* __restrict__ dpm = F.dp[m];
+    double* __restrict__ cpm = F.cp[m];
+    double* __restrict__ rhm = F.rh[m] + L.off;
+    const double brh1 = use_b1 ? F.brh[t * 8 + m] : 0.0;
+    const double brh2 = use_b2 ? F.brh[t * 8 + 4 + m] : 0.0;
+
+    double sdm = sd[base], sd0 = sd[base + sD];
+    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
+    unsigned piv = 0;
+    double dcur[kPF], scur[kPF];
+#pragma unroll
+    for (int u = 0; u < kPF; u++) {
+        const long p = base + (1 + u) * sD;
+        dcur[u] = u < len ? dpm[p] : 0.0;
+        scur[u] = u < len ? sd[p + sD] : 0.0;
+    }
+    for (int s0 = 0; s0 < len; s0 += kPF) {
+        double dnx[kPF], snx[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 + kPF + u;
+            const long p = base + (1 + s) * sD;
+            dnx[u] = s < len ? dpm[p] : 0.0;
+            snx[u] = s < len ? sd[p + sD] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 + u;
+            if (s < len) {
+                const long p = base + (1 + s) * sD;
+                const double sdp = scur[u];
+                const double tmp = sd0 * epd * hsq;
+                const double dtc = (sd0 == 0.0) ? 0.0 : 1.0;
+                const double ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
+                const double a = -ddw * lam - tmp;
+                const double c = dde * lam - tmp;
+                const double b = 1.0 + 2.0 * tmp;
+                double d = dcur[u];
+                if (s == 0 && use_b1) d = d - a * brh1;
+                if (s == len - 1 && use_b2) d = d - c * brh2;
+                double den;
+                if (s == 0) {
+                    den = b;
+                } else {
+                    if (fabs(denprev) < fabs(a)) piv++;  // SGTSV would interchange rows here
+                    den = b - a * cprev;
+                }
+                const double inv = 1.0 / den;
+                cprev = c * inv;
+                dprev = (s == 0) ? d * inv : (d - a * dprev) * inv;
+                denprev = den;
+                cpm[p] = cprev;
+                dpm[p] = dprev;
+                sdm = sd0;
+                sd0 = sdp;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) { dcur[u] = dnx[u]; scur[u] = snx[u]; }
+    }
+    // back substitution, loads one group ahead
+    double x = dprev;
+    rhm[base + (long)len * sD] = x;
+    for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
+        double dd[kPF], cc[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 - u;
+            const long p = base + (1 + s) * sD;
+            dd[u] = s >= 0 ? dpm[p] : 0.0;
+            cc[u] = s >= 0 ? cpm[p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 - u;
+            if (s >= 0) {
+                x = dd[u] - cc[u] * x;
+                rhm[base + (1 + s) * sD] = x;
+            }
         }
     }
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
-// ---- j sweep (rhs_diag_solver.F90:265-434) ----
-__global__ void __launch_bounds__(128) k_daf_j(Level L, Fields F, Consts C, int use_b1, int use_b2) {
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long nlines = (long)(ni - 2) * (nk - 2);
+// dQ = Mc dQ*** and q = q + dQ on interior nodes (rhs_diag_solver.F90:624-633,
+// solver_daf.F90:409-418)
+__global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, double* dq_out) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlines) return;
-    const int i = 1 + (int)(t % (ni - 2));
-    const int k = 1 + (int)(t / (ni - 2));
-    const long sJ = ni;
-    const long base = L.off + i + (long)ni * nj * k;  // (i, 0, k)
-    const long lbase = i + (long)ni * nj * k;
-    const int len = nj - 2;
-    const double epd = C.ep[0][1];
-    const double h = L.de;
-
-    double brh1[4] = {0, 0, 0, 0}, brh2[4] = {0, 0, 0, 0};
-    if (use_b1) {
-#pragma unroll
-        for (int m = 0; m < 4; m++) brh1[m] = C.sa[2][m] * F.rh[m][base + sJ] + C.sb[2][m] * F.rh[m][base + 2 * sJ];
-        apply_n1i(F, C, base, brh1);
-    }
-    if (use_b2) {
-        const long ge = base + (long)(nj - 2) * sJ;
-#pragma unroll
-        for (int m = 0; m < 4; m++) brh2[m] = C.sa[3][m] * F.rh[m][ge] + C.sb[3][m] * F.rh[m][ge - sJ];
-        apply_n1i(F, C, base + (long)(nj - 1) * sJ, brh2);
-    }
-
-    Thomas3 T;
-    unsigned piv = 0;
-    double dtm = F.dtau[base], sprm = spr_at<1>(F, C, base);
-    double dt0 = F.dtau[base + sJ], spr0 = spr_at<1>(F, C, base + sJ);
-    for (int s = 0; s < len; s++) {
-        const long g = base + (1 + s) * sJ;
-        const double dtp = F.dtau[g + sJ], sprp = spr_at<1>(F, C, g + sJ);
-        double aq[4];
-#pragma unroll
-        for (int m = 0; m < 4; m++) aq[m] = F.rh[m][g];
-        apply_n1i(F, C, g, aq);
-        double a[3], b, c[3];
-        daf_coefs(dt0, spr0, dtm, sprm, dtp, sprp, epd, h, a, b, c);
-        if (s == 0 && use_b1) {
-            const double aw4[4] = {a[0], a[0], a[1], a[2]};
-#pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = aq[m] - aw4[m] * brh1[m];
-        }
-        if (s == len - 1 && use_b2) {
-            const double ae4[4] = {c[0], c[0], c[1], c[2]};
-#pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = aq[m] - ae4[m] * brh2[m];
-        }
-        thomas_row(T, s == 0, a, b, c, aq, piv);
-        const long lt = lbase + (1 + s) * sJ;
-#pragma unroll
-        for (int x = 0; x < 3; x++) F.cp[x][lt] = T.cp[x];
-#pragma unroll
-        for (int m = 0; m < 4; m++) F.rh[m][g] = T.dp[m];
-        dtm = dt0; sprm = spr0; dt0 = dtp; spr0 = sprp;
-    }
-    double x[4];
-#pragma unroll
-    for (int m = 0; m < 4; m++) x[m] = T.dp[m];
-    const int mat[4] = {0, 0, 1, 2};
-    for (int s = len - 2; s >= 0; s--) {
-        const long g = base + (1 + s) * sJ;
-        const long lt = lbase + (1 + s) * sJ;
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            x[m] = F.rh[m][g] - F.cp[mat[m]][lt] * x[m];
-            F.rh[m][g] = x[m];
-        }
-    }
-    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
-}
-
-// ---- k sweep, Mc and the q update (rhs_diag_solver.F90:441-633,
-// solver_daf.F90:409-418) ----
-// rh is only read here (the k = 1 correction of a line also reads rh of
-// the next i line), so d' goes to F.dp and the result straight into q.
-__global__ void __launch_bounds__(128) k_daf_k(Level L, Fields F, Consts C, int use_b1, int build_b2_full,
-                                               int use_b2, double* dq_out) {
+    if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long nlines = (long)(ni - 2) * (nj - 2);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlines) return;
-    const int i = 1 + (int)(t % (ni - 2));
-    const int j = 1 + (int)(t / (ni - 2));
-    const long sK = (long)ni * nj;
-    const long base = L.off + i + (long)ni * j;  // (i, j, 0)
-    const long lbase = i + (long)ni * j;
-    const int len = nk - 2;
-    const double epd = C.ep[0][2];
-    const double h = L.dz;
-
-    double brh1[4] = {0, 0, 0, 0}, brh2[4] = {0, 0, 0, 0};
-    if (use_b1) {
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+    const long g = L.off + t;
+    double v[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) brh1[m] = C.sa[4][m] * F.rh[m][base + sK] + C.sb[4][m] * F.rh[m][base + 2 * sK];
-        // fifth product of sa(1:5,5)*rh(1:4,..) (:462) lands in brh(1,2,i,j);
-        // it reads rh(1) of node i+1
-        brh2[0] = C.sa[4][4] * F.rh[0][base + 1 + sK] + C.sb[4][4] * F.rh[0][base + 1 + 2 * sK];
-        apply_n2i(F, C, base, brh1);
+    for (int m = 0; m < 4; m++) v[m] = F.rh[m][g];
+    apply_mc(F, C, g, v);
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.q[m][g] = F.q[m][g] + v[m];
+    if (dq_out) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) dq_out[m * L.n + t] = v[m];
     }
-    if (build_b2_full) {  // built under btype(6) /= 6 (:467)
-        const long ge = base + (long)(nk - 2) * sK;
-#pragma unroll
-        for (int m = 0; m < 4; m++) brh2[m] = C.sa[5][m] * F.rh[m][ge] + C.sb[5][m] * F.rh[m][ge - sK];
-    }
-    if (use_b2) apply_n2i(F, C, base + (long)(nk - 1) * sK, brh2);  // transformed under /= 5 (:504)
-
-    Thomas3 T;
-    unsigned piv = 0;
-    double dtm = F.dtau[base], sprm = spr_at<2>(F, C, base);
-    double dt0 = F.dtau[base + sK], spr0 = spr_at<2>(F, C, base + sK);
-    for (int s = 0; s < len; s++) {
-        const long g = base + (1 + s) * sK;
-        const double dtp = F.dtau[g + sK], sprp = spr_at<2>(F, C, g + sK);
-        double aq[4];
-#pragma unroll
-        for (int m = 0; m < 4; m++) aq[m] = F.rh[m][g];
-        apply_n2i(F, C, g, aq);
-        double a[3], b, c[3];
-        daf_coefs(dt0, spr0, dtm, sprm, dtp, sprp, epd, h, a, b, c);
-        if (s == 0 && use_b1) {
-            const double aw4[4] = {a[0], a[0], a[1], a[2]};
-#pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = aq[m] - aw4[m] * brh1[m];
-        }
-        if (s == len - 1 && use_b2) {
-            const double ae4[4] = {c[0], c[0], c[1], c[2]};
-#pragma unroll
-            for (int m = 0; m < 4; m++) aq[m] = aq[m] - ae4[m] * brh2[m];
-        }
-        thomas_row(T, s == 0, a, b, c, aq, piv);
-        const long lt = lbase + (1 + s) * sK;
-#pragma unroll
-        for (int x = 0; x < 3; x++) F.cp[x][lt] = T.cp[x];
-#pragma unroll
-        for (int m = 0; m < 4; m++) F.dp[m][lt] = T.dp[m];
-        dtm = dt0; sprm = spr0; dt0 = dtp; spr0 = sprp;
-    }
-    const long nl = L.n;
-    double x[4];
-#pragma unroll
-    for (int m = 0; m < 4; m++) x[m] = T.dp[m];
-    const int mat[4] = {0, 0, 1, 2};
-    for (int s = len - 1; s >= 0; s--) {
-        const long g = base + (1 + s) * sK;
-        const long lt = lbase + (1 + s) * sK;
-        if (s < len - 1) {
-#pragma unroll
-            for (int m = 0; m < 4; m++) x[m] = F.dp[m][lt] - F.cp[mat[m]][lt] * x[m];
-        }
-        double dq[4] = {x[0], x[1], x[2], x[3]};
-        apply_mc(F, C, g, dq);
-#pragma unroll
-        for (int m = 0; m < 4; m++) F.q[m][g] = F.q[m][g] + dq[m];
-        if (dq_out) {
-#pragma unroll
-            for (int m = 0; m < 4; m++) dq_out[m * nl + lt] = dq[m];
-        }
-    }
-    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
 // ---- batched LAPACK SGTSV (src/sgtsv.f:89-259) in FP64, one system per thread ----
@@ -502,30 +429,33 @@ __global__ void k_gtsv_batch(int n, int nsys, double* dl, double* d, double* du,
 
 }  // namespace
 
-void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) {
-    const long nl = (long)(L.nj - 2) * (L.nk - 2);
-    const int b = 128;
-    const int use_b1 = (n == 1 && C.btype[0] != 5);
-    const int use_b2 = (n == 1 && C.btype[1] != 5);
-    hipLaunchKernelGGL(k_daf_i, dim3((unsigned)((nl + b - 1) / b)), dim3(b), 0, s, L, F, C, use_b1, use_b2);
+static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int D) {
+    const int use_b1 = (n == 1 && C.btype[2 * D] != 5);
+    const int use_b2 = (n == 1 && C.btype[2 * D + 1] != 5);
+    const int b2_full = (n == 1 && C.btype[5] != 6);
+    const unsigned gp = (unsigned)((L.n + 255) / 256);
+    const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
+                                                                    : (long)(L.ni - 2) * (L.nj - 2);
+    const dim3 gt((unsigned)((nl + 127) / 128), 4);
+    if (D == 0) {
+        hipLaunchKernelGGL(k_daf_pre<0>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL(k_daf_tri<0>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);
+    } else if (D == 1) {
+        hipLaunchKernelGGL(k_daf_pre<1>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL(k_daf_tri<1>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);
+    } else {
+        hipLaunchKernelGGL(k_daf_pre<2>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL(k_daf_tri<2>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);
+    }
 }
 
-void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) {
-    const long nl = (long)(L.ni - 2) * (L.nk - 2);
-    const int b = 128;
-    const int use_b1 = (n == 1 && C.btype[2] != 5);
-    const int use_b2 = (n == 1 && C.btype[3] != 5);
-    hipLaunchKernelGGL(k_daf_j, dim3((unsigned)((nl + b - 1) / b)), dim3(b), 0, s, L, F, C, use_b1, use_b2);
-}
+void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) { daf_sweep(s, L, F, C, n, 0); }
+
+void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) { daf_sweep(s, L, F, C, n, 1); }
 
 void launch_daf_k(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, double* dq_out) {
-    const long nl = (long)(L.ni - 2) * (L.nj - 2);
-    const int b = 128;
-    const int use_b1 = (n == 1 && C.btype[4] != 5);
-    const int build_b2 = (n == 1 && C.btype[5] != 6);
-    const int use_b2 = (n == 1 && C.btype[5] != 5);
-    hipLaunchKernelGGL(k_daf_k, dim3((unsigned)((nl + b - 1) / b)), dim3(b), 0, s, L, F, C, use_b1, build_b2,
-                       use_b2, dq_out);
+    daf_sweep(s, L, F, C, n, 2);
+    hipLaunchKernelGGL(k_daf_post, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L, F, C, dq_out);
 }
 
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du, double* b, int* info) {

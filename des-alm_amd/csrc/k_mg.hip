@@ -137,13 +137,17 @@ __global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
     for (int m = 0; m < 4; m++) F.dp[m][t] = fill_i(P, F, m, i, j, k);
 }
 
-// rs_sweep in direction D on the interior of the fine level; tab holds the
-// SGTSV factorization of the constant matrix (-ep, 1+2ep, -ep): fact[len-1]
-// then d[len].  D = 2 also adds dq to q.
+// rs_sweep in direction D on the interior of the fine level, one thread per
+// (line, component m = blockIdx.y); tab holds the SGTSV factorization of the
+// constant matrix (-ep, 1+2ep, -ep): fact[len-1] then d[len].  D = 2 also
+// adds dq to q.
+constexpr int kPF = 4;
+
 template <int D>
 __global__ void __launch_bounds__(128) k_rs(Level L, Fields F, const double* tab, double epn) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const long sI = 1, sJ = ni, sK = (long)ni * nj;
+    const int m = blockIdx.y;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long base, sD;
     int len;
@@ -166,36 +170,55 @@ __global__ void __launch_bounds__(128) k_rs(Level L, Fields F, const double* tab
     const double* fact = tab;
     const double* dd = tab + (len - 1);
     const double du = -epn;
-    double prev[4];
+    double* __restrict__ v = F.dp[m];
+    double prev = v[base + sD];
+    double bcur[kPF];
 #pragma unroll
-    for (int m = 0; m < 4; m++) prev[m] = F.dp[m][base + sD];
-    for (int s = 1; s < len; s++) {
-        const long p = base + (1 + s) * sD;
-        const double f = fact[s - 1];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const double v = F.dp[m][p] - f * prev[m];
-            F.dp[m][p] = v;
-            prev[m] = v;
-        }
+    for (int u = 0; u < kPF; u++) {
+        const int s = 1 + u;
+        bcur[u] = s < len ? v[base + (1 + s) * sD] : 0.0;
     }
-    double x[4];
+    for (int s0 = 1; s0 < len; s0 += kPF) {
+        double bnx[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 + kPF + u;
+            bnx[u] = s < len ? v[base + (1 + s) * sD] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 + u;
+            if (s < len) {
+                prev = bcur[u] - fact[s - 1] * prev;
+                v[base + (1 + s) * sD] = prev;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) bcur[u] = bnx[u];
+    }
+    double* qm = F.q[m] + L.off;
+    double x = prev / dd[len - 1];
     {
         const long p = base + (long)len * sD;
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            x[m] = prev[m] / dd[len - 1];
-            if (D < 2) F.dp[m][p] = x[m];
-            else F.q[m][L.off + p] = F.q[m][L.off + p] + x[m];
-        }
+        if (D < 2) v[p] = x;
+        else qm[p] = qm[p] + x;
     }
-    for (int s = len - 2; s >= 0; s--) {
-        const long p = base + (1 + s) * sD;
+    for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
+        double b[kPF];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            x[m] = (F.dp[m][p] - du * x[m]) / dd[s];
-            if (D < 2) F.dp[m][p] = x[m];
-            else F.q[m][L.off + p] = F.q[m][L.off + p] + x[m];
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 - u;
+            b[u] = s >= 0 ? v[base + (1 + s) * sD] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 - u;
+            if (s >= 0) {
+                x = (b[u] - du * x) / dd[s];
+                const long p = base + (1 + s) * sD;
+                if (D < 2) v[p] = x;
+                else qm[p] = qm[p] + x;
+            }
         }
     }
 }
@@ -371,11 +394,11 @@ void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Field
     hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(Lf.n, 256)), dim3(256), 0, s, P, F);
     // rs_tab: three direction tables of 2 * max(ni,nj,nk) doubles
     const long stride = 2L * (Lf.ni > Lf.nj ? (Lf.ni > Lf.nk ? Lf.ni : Lf.nk) : (Lf.nj > Lf.nk ? Lf.nj : Lf.nk));
-    hipLaunchKernelGGL(k_rs<0>, dim3(nblk((long)(Lf.nj - 2) * (Lf.nk - 2), 128)), dim3(128), 0, s, Lf, F, rs_tab,
+    hipLaunchKernelGGL(k_rs<0>, dim3(nblk((long)(Lf.nj - 2) * (Lf.nk - 2), 128), 4), dim3(128), 0, s, Lf, F, rs_tab,
                        C.ep[1][0]);
-    hipLaunchKernelGGL(k_rs<1>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nk - 2), 128)), dim3(128), 0, s, Lf, F,
+    hipLaunchKernelGGL(k_rs<1>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nk - 2), 128), 4), dim3(128), 0, s, Lf, F,
                        rs_tab + stride, C.ep[1][1]);
-    hipLaunchKernelGGL(k_rs<2>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nj - 2), 128)), dim3(128), 0, s, Lf, F,
+    hipLaunchKernelGGL(k_rs<2>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nj - 2), 128), 4), dim3(128), 0, s, Lf, F,
                        rs_tab + 2 * stride, C.ep[1][2]);
 }
 

@@ -7,6 +7,7 @@
 //   k_sa_prep   U/J (stored, rhs_contra_j) and wo (rhs_sa_source)
 //   k_sa_rhs    rhs_sa_grdnu, rhs_sa_prod, rhs_sa_viscous, rhs_sa_flux,
 //               rhs_sa_unst_visc, rhs_sa_implicit_source, boundary zeroing
+//   k_sa_adi_pre the ADI operators' face coefficients and dtev J / 2
 //   k_sa_adi<D> rhs_sa_adi_solver sweep in direction D (D = 2 also applies
 //               the update of des_eddy.F90:260-272)
 //   k_sa_bc<D>  rhs_sa_bcond faces, k_sa_clamp its nu~ floor and xnut
@@ -173,89 +174,156 @@ __global__ void __launch_bounds__(256) k_sa_rhs(Level L, Fields F, Consts C) {
     F.rh5[t] = -F.dtev[g] * F.aj[g] * r;
 }
 
-// one ADI sweep (rhs_sa_adi_solver.F90:44-244); D = 2 also does the
-// relaxed update q5 += (-q5 + qold + rh) / (1 + 1.5 e_source dtev / delti)
+// Face and node coefficients of the three ADI operators
+// (rhs_sa_adi_solver.F90:53-75 and copies), for every fine node:
+//   av_D = |U/J face average| + (1/Re + nu~ face average)/sig * |grad csi_D|^2_face / (J + J')
+// on the face between the node and its +D neighbour, and dtj = dtev J / 2.
+// nu~ does not change between the three sweeps of one sub-iteration.
+__global__ void __launch_bounds__(256) k_sa_adi_pre(Level L, Fields F, Consts C) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    const long o = L.off;
+    const double* q5 = F.q[4];
+    F.dtj[t] = 0.5 * F.dtev[o + t] * F.aj[o + t];
+    const int pos[3] = {i, j, k};
+    const int nd[3] = {ni, nj, nk};
+    const long st[3] = {1, (long)ni, (long)ni * nj};
+#pragma unroll
+    for (int D = 0; D < 3; D++) {
+        if (pos[D] >= nd[D] - 1) continue;
+        const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
+        const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
+        const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
+        const long p = t, p1 = t + st[D];
+        const double gx = m0[o + p1] + m0[o + p], gy = m1[o + p1] + m1[o + p], gz = m2[o + p1] + m2[o + p];
+        const double ev = (C.rei + 0.5 * (q5[o + p1] + q5[o + p])) / sig;
+        const double ucna = 0.5 * fabs(F.ucn[D][p1] + F.ucn[D][p]);
+        F.av[D][t] = ucna + ev * (gx * gx + gy * gy + gz * gz) / (F.aj[o + p1] + F.aj[o + p]);
+    }
+}
+
+// one ADI sweep (rhs_sa_adi_solver.F90:44-244), one thread per line:
+//   aw = -dtj (U_(s-1) + av_(s-1/2)), ap = 1 + dtj (av_(s-1/2) + av_(s+1/2)),
+//   ae = dtj (U_(s+1) - av_(s+1/2)).
+// D = 2 also does the relaxed update of des_eddy.F90:260-272:
+//   q5 += (-q5 + qold + rh) / (1 + 1.5 e_source dtev / delti)
+constexpr int kPF = 4;
+
 template <int D>
 __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long sI = 1, sJ = ni, sK = (long)ni * nj;
-    long nlines, base;
-    int len;
-    long sD;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long base, sD;
+    int len;
     if (D == 0) {
-        nlines = (long)(nj - 2) * (nk - 2);
-        if (t >= nlines) return;
-        const int j = 1 + (int)(t % (nj - 2)), k = 1 + (int)(t / (nj - 2));
-        base = sJ * j + sK * k;
+        if (t >= (long)(nj - 2) * (nk - 2)) return;
+        base = (long)ni * ((1 + t % (nj - 2)) + (long)nj * (1 + t / (nj - 2)));
+        sD = 1;
         len = ni - 2;
-        sD = sI;
     } else if (D == 1) {
-        nlines = (long)(ni - 2) * (nk - 2);
-        if (t >= nlines) return;
-        const int i = 1 + (int)(t % (ni - 2)), k = 1 + (int)(t / (ni - 2));
-        base = i + sK * k;
+        if (t >= (long)(ni - 2) * (nk - 2)) return;
+        base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
+        sD = ni;
         len = nj - 2;
-        sD = sJ;
     } else {
-        nlines = (long)(ni - 2) * (nj - 2);
-        if (t >= nlines) return;
-        const int i = 1 + (int)(t % (ni - 2)), j = 1 + (int)(t / (ni - 2));
-        base = i + sJ * j;
+        if (t >= (long)(ni - 2) * (nj - 2)) return;
+        base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
+        sD = (long)ni * nj;
         len = nk - 2;
-        sD = sK;
     }
-    const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
-    const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
-    const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
-    const double* U = F.ucn[D];
-    const double* q5 = F.q[4];
-    const long o = L.off;
-    // face coefficient av between level-local nodes p and p + sD
-    auto av_face = [&](long p) {
-        const long p1 = p + sD;
-        const double gx = m0[o + p1] + m0[o + p], gy = m1[o + p1] + m1[o + p], gz = m2[o + p1] + m2[o + p];
-        const double ev = (C.rei + 0.5 * (q5[o + p1] + q5[o + p])) / sig;
-        const double ucna = 0.5 * fabs(U[p1] + U[p]);
-        return ucna + ev * (gx * gx + gy * gy + gz * gz) / (F.aj[o + p1] + F.aj[o + p]);
-    };
-    double avm = av_face(base);  // face 1/2
+    const double* __restrict__ U = F.ucn[D];
+    const double* __restrict__ av = F.av[D];
+    const double* __restrict__ dtj = F.dtj;
+    double* __restrict__ rh = F.rh5;
+    double* __restrict__ cp = F.cp[0];
+    double avm = av[base];
+    double um = U[base], u0 = U[base + sD];
     double cprev = 0.0, dprev = 0.0, denprev = 1.0;
     unsigned piv = 0;
-    for (int s = 0; s < len; s++) {
-        const long p = base + (1 + s) * sD;
-        const double avp = av_face(p);
-        const double dtj = 0.5 * F.dtev[o + p] * F.aj[o + p];
-        const double aw = -dtj * (U[p - sD] + avm);
-        const double ap = 1.0 + dtj * (avm + avp);
-        const double ae = dtj * (U[p + sD] - avp);
-        const double aq = F.rh5[p];
-        double den;
-        if (s == 0) {
-            den = ap;
-        } else {
-            if (fabs(denprev) < fabs(aw)) piv++;
-            den = ap - aw * cprev;
-        }
-        const double inv = 1.0 / den;
-        cprev = ae * inv;
-        dprev = (s == 0) ? aq * inv : (aq - aw * dprev) * inv;
-        denprev = den;
-        F.cp[0][p] = cprev;
-        F.rh5[p] = dprev;
-        avm = avp;
+    double b_av[kPF], b_dt[kPF], b_up[kPF], b_rh[kPF];
+#pragma unroll
+    for (int u = 0; u < kPF; u++) {
+        const long p = base + (1 + u) * sD;
+        const bool ok = u < len;
+        b_av[u] = ok ? av[p] : 0.0;
+        b_dt[u] = ok ? dtj[p] : 0.0;
+        b_up[u] = ok ? U[p + sD] : 0.0;
+        b_rh[u] = ok ? rh[p] : 0.0;
     }
-    double x = dprev;
-    for (int s = len - 1; s >= 0; s--) {
-        const long p = base + (1 + s) * sD;
-        if (s < len - 1) x = F.rh5[p] - F.cp[0][p] * x;
+    for (int s0 = 0; s0 < len; s0 += kPF) {
+        double n_av[kPF], n_dt[kPF], n_up[kPF], n_rh[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 + kPF + u;
+            const long p = base + (1 + s) * sD;
+            const bool ok = s < len;
+            n_av[u] = ok ? av[p] : 0.0;
+            n_dt[u] = ok ? dtj[p] : 0.0;
+            n_up[u] = ok ? U[p + sD] : 0.0;
+            n_rh[u] = ok ? rh[p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 + u;
+            if (s < len) {
+                const long p = base + (1 + s) * sD;
+                const double avp = b_av[u], dt = b_dt[u], up = b_up[u];
+                const double aw = -dt * (um + avm);
+                const double ap = 1.0 + dt * (avm + avp);
+                const double ae = dt * (up - avp);
+                const double aq = b_rh[u];
+                double den;
+                if (s == 0) {
+                    den = ap;
+                } else {
+                    if (fabs(denprev) < fabs(aw)) piv++;
+                    den = ap - aw * cprev;
+                }
+                const double inv = 1.0 / den;
+                cprev = ae * inv;
+                dprev = (s == 0) ? aq * inv : (aq - aw * dprev) * inv;
+                denprev = den;
+                cp[p] = cprev;
+                rh[p] = dprev;
+                avm = avp;
+                um = u0;
+                u0 = up;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) { b_av[u] = n_av[u]; b_dt[u] = n_dt[u]; b_up[u] = n_up[u]; b_rh[u] = n_rh[u]; }
+    }
+    auto finish = [&](long p, double x) {
         if (D < 2) {
-            F.rh5[p] = x;
+            rh[p] = x;
         } else {
-            const double qv = q5[o + p];
+            const long g = L.off + p;
+            const double qv = F.q[4][g];
             const double qe5 = qv + x;  // qold == q5 at this point of the sub-iteration
-            const double dum = 1.0 + 1.5 * C.e_source * F.dtev[o + p] / C.delti;
-            F.q[4][o + p] = qv + (-qv + qe5) / dum;
+            const double dum = 1.0 + 1.5 * C.e_source * F.dtev[g] / C.delti;
+            F.q[4][g] = qv + (-qv + qe5) / dum;
+        }
+    };
+    double x = dprev;
+    finish(base + (long)len * sD, x);
+    for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
+        double dd[kPF], cc[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 - u;
+            const long p = base + (1 + s) * sD;
+            dd[u] = s >= 0 ? rh[p] : 0.0;
+            cc[u] = s >= 0 ? cp[p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int s = s0 - u;
+            if (s >= 0) {
+                x = dd[u] - cc[u] * x;
+                finish(base + (1 + s) * sD, x);
+            }
         }
     }
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
@@ -336,6 +404,7 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
     for (int it = 0; it < itk; it++) {
         hipLaunchKernelGGL(k_sa_rhs, dim3(gn), dim3(b), 0, s, L, F, C);
+        hipLaunchKernelGGL(k_sa_adi_pre, dim3(gn), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<0>, dim3(gi), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
