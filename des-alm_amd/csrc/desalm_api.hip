@@ -294,7 +294,7 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     for (int m = 0; m < 3; m++) F.av[m] = A(c->n1);
     F.dtj = A(c->n1);
     for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
-    F.wo = A(c->n1); F.rh5 = A(c->n1);
+    F.wo = A(c->n1); F.rh5 = A(c->n1); F.pj = A(c->n1);
     F.resid_part = A(5L * kResidBlocks);
     F.resid = A(5);
     F.pfix = A(1);

@@ -60,7 +60,8 @@ struct Fields {
     double* brh;      // 8 per line: semi-explicit boundary vectors of a sweep
     double* av[3];    // fine: SA ADI face coefficients per direction
     double* dtj;      // fine: SA ADI 0.5 dtev J
-    double* ucn[3];   // fine: contravariant velocity / J for the SA update
+    double* ucn[3];   // fine: contravariant velocity / J (residual, SA update)
+    double* pj;       // fine: p / J
     double* wo;       // fine: SA vorticity measure
     double* rh5;      // fine: SA right-hand side
     double* resid_part;  // [kResidBlocks * 5] partial L1 sums

@@ -1,12 +1,15 @@
 // k_residual.hip -- residual assembly of one DAF smoother pass on one level.
 //
-// One thread per node of the level gathers its stencil straight from the
-// SoA fields and produces the whole right-hand side of solver_daf steps
-// 2-10 (src/solver_daf.F90:199-315): contravariant velocity, QUICK or
-// first-order convection, central pressure gradient and divergence, 2nd-order
-// viscous fluxes, 4th-difference pressure dissipation, BDF2 term, FAS forcing
-// and zeroing of the boundary planes.  Face fluxes are evaluated by the two
-// nodes that share them, so no intermediate face arrays touch HBM.
+// Three kernels produce the right-hand side of solver_daf steps 2-10
+// (src/solver_daf.F90:199-315):
+//   k_prep     dtau, U/J and p/J per node
+//   k_residual central pressure gradient and divergence, QUICK or first-order
+//              convection, 4th-difference pressure dissipation, BDF2 term,
+//              zeroing of the boundary planes
+//   k_visc_pk  2nd-order viscous fluxes and the FAS forcing
+// One thread per node gathers its stencil from the SoA fields; face fluxes
+// are evaluated by the two nodes that share them, so no face arrays touch
+// HBM.
 // Operand order inside every expression follows the reference routine cited.
 #include "desalm_internal.h"
 
@@ -20,14 +23,25 @@ struct Nb {
 
 __device__ __forceinline__ double sqr(double a) { return a * a; }
 
-// rhs_daf_dtau (src/rhs_daf_dtau.F90:85-117), every node of the level
-__global__ void k_dtau(Level L, Fields F, Consts C) {
+// Per-node inputs of the residual, every node of the level:
+// rhs_daf_dtau (src/rhs_daf_dtau.F90:85-117) when with_dtau, and always
+// U/J (rhs_contra_j.F90:14-29) into F.ucn and p/J
+// (rhs_flux_sans_convec.F90:265-271) into F.pj (level-local).
+__global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
     const double c1 = F.csi[0][g], c2 = F.csi[1][g], c3 = F.csi[2][g];
     const double e1 = F.eta[0][g], e2 = F.eta[1][g], e3 = F.eta[2][g];
     const double z1 = F.zet[0][g], z2 = F.zet[1][g], z3 = F.zet[2][g];
+    const double aj = F.aj[g];
+    const double u = F.q[1][g], v = F.q[2][g], w = F.q[3][g];
+    const double tmp = 1.0 / aj;
+    F.ucn[0][t] = tmp * (c1 * u + c2 * v + c3 * w);
+    F.ucn[1][t] = tmp * (e1 * u + e2 * v + e3 * w);
+    F.ucn[2][t] = tmp * (z1 * u + z2 * v + z3 * w);
+    F.pj[t] = F.q[0][g] / aj;
+    if (!with_dtau) return;
     const double g1 = c1 * c1 + c2 * c2 + c3 * c3;
     const double g2 = e1 * e1 + e2 * e2 + e3 * e3;
     const double g3 = z1 * z1 + z2 * z2 + z3 * z3;
@@ -117,24 +131,20 @@ __device__ __forceinline__ double diss_fv(const Level& L, const Fields& F, long 
     return tmp1 * tmp2;
 }
 
-// contravariant velocity / J, component D, at global node g (rhs_contra_j.F90:14-29)
+// contravariant velocity / J, component D, at level-local node t (k_prep)
 template <int D>
-__device__ __forceinline__ double ucn(const Fields& F, long g) {
-    const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
-    const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
-    const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
-    const double tmp = 1.0 / F.aj[g];
-    return tmp * (m0[g] * F.q[1][g] + m1[g] * F.q[2][g] + m2[g] * F.q[3][g]);
+__device__ __forceinline__ double ucn(const Fields& F, long t) {
+    return F.ucn[D][t];
 }
 
 // QUICK / first-order upwind contribution of direction D to rh(2:4)
 // (rhs_convec_quick.F90:89-201 and its eta/zet copies)
 template <int D>
-__device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, const Nb& nb, int pos,
+__device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, long t, const Nb& nb, int pos,
                                        int len, int quick, double out[3]) {
     const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
     const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
-    const double ucon = 0.5 * ucn<D>(F, g);
+    const double ucon = 0.5 * ucn<D>(F, t);
     const double up = ucon + fabs(ucon), um = ucon - fabs(ucon);
     // first interior node is pos 1, last is len - 2: first-order there
     const bool first_order = !quick || pos == 1 || pos == len - 2;
@@ -151,11 +161,12 @@ __device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, 
     }
 }
 
-// Residual of solver_daf on one level.  recalc = 0: first evaluation of the
-// pass (dtau and diss(1) fresh, diss(1) stored); recalc = 1: the
-// decide_recalc_rh branch (src/solver_daf.F90:432-484), which reuses the
-// stored diss(1) of the pre-update q.
-__global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc, int calc_pk) {
+// Residual of solver_daf on one level without its viscous term and FAS
+// forcing (k_visc_pk adds those).  recalc = 0: first evaluation of the pass
+// (diss(1) fresh and stored); recalc = 1: the decide_recalc_rh branch
+// (src/solver_daf.F90:432-484), which reuses the stored diss(1) of the
+// pre-update q and dtau.
+__global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
@@ -173,13 +184,13 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
     const Nb nb{1, (long)ni, (long)ni * nj};
     const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
 
-    // central pressure gradient (rhs_flux_sans_convec.F90:265-326)
+    // central pressure gradient (rhs_flux_sans_convec.F90:280-326)
     double r[4];
     {
-        const double* p = F.q[0];
-        const double pim = p[g - nb.sI] / F.aj[g - nb.sI], pip = p[g + nb.sI] / F.aj[g + nb.sI];
-        const double pjm = p[g - nb.sJ] / F.aj[g - nb.sJ], pjp = p[g + nb.sJ] / F.aj[g + nb.sJ];
-        const double pkm = p[g - nb.sK] / F.aj[g - nb.sK], pkp = p[g + nb.sK] / F.aj[g + nb.sK];
+        const double* pj = F.pj;
+        const double pim = pj[t - nb.sI], pip = pj[t + nb.sI];
+        const double pjm = pj[t - nb.sJ], pjp = pj[t + nb.sJ];
+        const double pkm = pj[t - nb.sK], pkp = pj[t + nb.sK];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             double v = dc2 * (pip * F.csi[c][g + nb.sI] - pim * F.csi[c][g - nb.sI]);
@@ -188,38 +199,21 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
             r[c + 1] = v;
         }
         // divergence of U/J (:333-344)
-        const double dudx = dc2 * (ucn<0>(F, g + nb.sI) - ucn<0>(F, g - nb.sI));
-        const double dvdy = de2 * (ucn<1>(F, g + nb.sJ) - ucn<1>(F, g - nb.sJ));
-        const double dwdz = dz2 * (ucn<2>(F, g + nb.sK) - ucn<2>(F, g - nb.sK));
+        const double dudx = dc2 * (ucn<0>(F, t + nb.sI) - ucn<0>(F, t - nb.sI));
+        const double dvdy = de2 * (ucn<1>(F, t + nb.sJ) - ucn<1>(F, t - nb.sJ));
+        const double dwdz = dz2 * (ucn<2>(F, t + nb.sK) - ucn<2>(F, t - nb.sK));
         r[0] = dudx + dvdy + dwdz;
     }
     // convection: QUICK on the fine level, first-order upwind on coarse levels
     {
         const int quick = (n == 1);
         double fv[3];
-        convec<0>(L, F, g, nb, i, ni, quick, fv);
+        convec<0>(L, F, g, t, nb, i, ni, quick, fv);
         r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
-        convec<1>(L, F, g, nb, j, nj, quick, fv);
+        convec<1>(L, F, g, t, nb, j, nj, quick, fv);
         r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
-        convec<2>(L, F, g, nb, k, nk, quick, fv);
+        convec<2>(L, F, g, t, nb, k, nk, quick, fv);
         r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
-    }
-    // viscous terms (rhs_viscous.F90:112-329)
-    double visc[3];
-    {
-        double fp[3], fm[3];
-        visc_face<0>(L, F, C, t, nb, fp);
-        visc_face<0>(L, F, C, t - nb.sI, nb, fm);
-#pragma unroll
-        for (int c = 0; c < 3; c++) visc[c] = L.dc * (fp[c] - fm[c]);
-        visc_face<1>(L, F, C, t, nb, fp);
-        visc_face<1>(L, F, C, t - nb.sJ, nb, fm);
-#pragma unroll
-        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.de * (fp[c] - fm[c]);
-        visc_face<2>(L, F, C, t, nb, fp);
-        visc_face<2>(L, F, C, t - nb.sK, nb, fm);
-#pragma unroll
-        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.dz * (fp[c] - fm[c]);
     }
     // pressure dissipation (rhs_diss_p.F90:58-203); the boundary copies of
     // fv (:124-129) replace the out-of-domain neighbour by the node itself
@@ -239,7 +233,7 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
     } else {
         diss1 = F.diss1[g];
     }
-    // unsteady, viscous and dissipation terms (rhs_unst_visc_diss.F90:171-186);
+    // unsteady and dissipation terms (rhs_unst_visc_diss.F90:171-186);
     // diss(2:4) is zero (rhs_convec_quick.F90:433)
     {
         const double dtaj = 2.0 * C.delti * F.aj[g];
@@ -247,15 +241,51 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
 #pragma unroll
         for (int m = 1; m < 4; m++) {
             const double dqdt = C.e_source * (3.0 * F.q[m][g] - 4.0 * F.qn[m][g] + F.qnm1[m][g]) / dtaj;
-            r[m] = r[m] + 0.0 + dqdt - visc[m - 1];
+            r[m] = r[m] + 0.0 + dqdt;
         }
     }
-    // FAS forcing on coarse levels (rhs_pk.F90:232-264)
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
+}
+
+// Viscous term subtracted from rh(2:4) (rhs_viscous.F90:112-329,
+// rhs_unst_visc_diss.F90:181-183) and the FAS forcing of coarse levels
+// (rhs_pk.F90:232-264): pk = pk - rh when calc_pk, then rh = rh + pk.
+__global__ void __launch_bounds__(256) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int i = (int)(t % ni);
+    const int j = (int)((t / ni) % nj);
+    const int k = (int)(t / ((long)ni * nj));
+    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+    const long g = L.off + t;
+    const Nb nb{1, (long)ni, (long)ni * nj};
+    double visc[3];
+    {
+        double fp[3], fm[3];
+        visc_face<0>(L, F, C, t, nb, fp);
+        visc_face<0>(L, F, C, t - nb.sI, nb, fm);
+#pragma unroll
+        for (int c = 0; c < 3; c++) visc[c] = L.dc * (fp[c] - fm[c]);
+        visc_face<1>(L, F, C, t, nb, fp);
+        visc_face<1>(L, F, C, t - nb.sJ, nb, fm);
+#pragma unroll
+        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.de * (fp[c] - fm[c]);
+        visc_face<2>(L, F, C, t, nb, fp);
+        visc_face<2>(L, F, C, t - nb.sK, nb, fm);
+#pragma unroll
+        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.dz * (fp[c] - fm[c]);
+    }
+    double r[4];
+    r[0] = F.rh[0][g];
+#pragma unroll
+    for (int m = 1; m < 4; m++) r[m] = F.rh[m][g] - visc[m - 1];
     if (n > 1) {
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             double pk = F.pk[m][g];
-            if (!recalc && calc_pk == 1) {
+            if (calc_pk == 1) {
                 pk = pk - r[m];
                 F.pk[m][g] = pk;
             }
@@ -270,14 +300,17 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
 
 void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C) {
     const int b = 256;
-    hipLaunchKernelGGL(k_dtau, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C);
+    hipLaunchKernelGGL(k_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C, 1);
 }
 
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int recalc,
                      int calc_pk) {
     const int b = 256;
-    hipLaunchKernelGGL(k_residual, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C, n, recalc,
-                       calc_pk);
+    const dim3 grid((unsigned)((L.n + b - 1) / b));
+    // the recalc branch needs U/J and p/J of the updated q (dtau is kept)
+    if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, 0);
+    hipLaunchKernelGGL(k_residual, grid, dim3(b), 0, s, L, F, C, n, recalc);
+    hipLaunchKernelGGL(k_visc_pk, grid, dim3(b), 0, s, L, F, C, n, recalc ? 0 : calc_pk);
 }
 
 }  // namespace desalm
