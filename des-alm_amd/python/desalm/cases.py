@@ -17,15 +17,50 @@ import numpy as np
 from .params import make_params, BC_WALL, BC_INTERFACE, BC_PERIODIC
 
 
+class Overset:
+    """Overset donor table of the fine level (src/global_osg.F90:18-28).
+
+    Donor d interpolates trilinearly from the cell whose lower corner is
+    hosts[d] (1-based i, j, k) at weights coef[d] into the fringe node
+    nodes[d] (1-based i, j, k).  nu~ is interpolated after every des_eddy
+    (mg_driver.F90:182); q(1:4) after every fine smoother pass when
+    apply_mom is set, which the reference does when nzone > 1
+    (mg_driver.F90:241-244)."""
+
+    def __init__(self, hosts, coef, nodes, apply_mom=True):
+        self.hosts = np.ascontiguousarray(hosts, dtype=np.int32).reshape(-1, 3)
+        self.coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1, 3)
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.int32).reshape(-1, 3)
+        self.apply_mom = bool(apply_mom)
+
+
+def overset_table(ni, nj, nk, nd, seed=5, reach=2, apply_mom=True):
+    """nd distinct interior fringe nodes, each fed by a host cell within
+    `reach` cells of it, with uniform random trilinear weights."""
+    rng = np.random.default_rng(seed)
+    n_int = (ni - 2) * (nj - 2) * (nk - 2)
+    pick = rng.choice(n_int, size=nd, replace=False)
+    i = 2 + pick % (ni - 2)
+    j = 2 + (pick // (ni - 2)) % (nj - 2)
+    k = 2 + pick // ((ni - 2) * (nj - 2))
+    nodes = np.stack([i, j, k], axis=1)
+    off = rng.integers(-reach, reach + 1, size=(nd, 3))
+    lim = np.array([ni - 1, nj - 1, nk - 1])
+    hosts = np.clip(nodes + off, 1, lim)
+    coef = rng.uniform(0.0, 1.0, size=(nd, 3))
+    return Overset(hosts, coef, nodes, apply_mom)
+
+
 class Case:
     """A complete input set for one run of the inner iteration."""
 
-    def __init__(self, name, params, x, y, z, wd, q, qn, qnm1, xnut):
+    def __init__(self, name, params, x, y, z, wd, q, qn, qnm1, xnut, overset=None):
         self.name = name
         self.params = params
         self.x, self.y, self.z = x, y, z
         self.wd = wd
         self.q, self.qn, self.qnm1, self.xnut = q, qn, qnm1, xnut
+        self.overset = overset
 
     @property
     def n1(self):
@@ -124,10 +159,11 @@ def channel(ni=129, nj=129, nk=129, ng=3, itm=30, ren=395.0, delti=0.005, seed=2
 
 
 def wavy(ni, nj, nk, amp=0.04, ng=2, seed=7, des=False, unsteady=True, btype=None, ren=500.0,
-         itm=3, delti=0.02, stretch=0.8, **kw):
+         itm=3, delti=0.02, stretch=0.8, overset=0, apply_mom=True, **kw):
     """Curvilinear test block: a stretched box with smooth sinusoidal
     distortion of every coordinate (non-orthogonal metrics, all cross terms
-    non-zero) and smooth random initial fields.  Used by the parity tests."""
+    non-zero) and smooth random initial fields.  Used by the parity tests.
+    overset > 0 adds a donor table of that many fringe nodes."""
     si = tanh_stretch(ni, stretch)
     sj = tanh_stretch(nj, stretch)
     sk = tanh_stretch(nk, stretch)
@@ -162,4 +198,5 @@ def wavy(ni, nj, nk, amp=0.04, ng=2, seed=7, des=False, unsteady=True, btype=Non
         btype = (BC_WALL, 3, BC_WALL, BC_INTERFACE, 2, 3)
     p = make_params(ni, nj, nk, ng=ng, itm=itm, ren=ren, delti=delti, unsteady=unsteady, des=des,
                     btype=btype, fix=(3, 2, 4), **kw)
-    return Case("wavy", p, x, y, z, wd, q, qn, qnm1, xnut)
+    ov = overset_table(ni, nj, nk, overset, apply_mom=apply_mom) if overset else None
+    return Case("wavy", p, x, y, z, wd, q, qn, qnm1, xnut, overset=ov)

@@ -43,6 +43,14 @@ struct or_state {
     double *xnut, *wd, *dtev;
     double init_resd[5];
     int itc;
+    /* overset donor table of one zone on one rank (global_osg.F90:18-28):
+     * donor d interpolates from the cell with lower corner hosts(1:3,d) at
+     * trilinear weights coef(1:3,d) into the fine node nodes(1:3,d) */
+    long nover;
+    int* ohosts;    /* 3 per donor, 1-based i,j,k */
+    double* ocoef;  /* 3 per donor */
+    int* onodes;    /* 3 per donor, 1-based i,j,k */
+    int apply_mom;  /* mg_bintp_mom runs (the reference's nzone > 1) */
 };
 
 /* constants of src/precision.F90 */
@@ -210,6 +218,7 @@ void or_destroy(or_state* o) {
     free(o->csi); free(o->eta); free(o->zet); free(o->aj);
     free(o->q); free(o->qn); free(o->qnm1); free(o->qold_mg); free(o->rh); free(o->pk);
     free(o->xnut); free(o->wd); free(o->dtev);
+    free(o->ohosts); free(o->ocoef); free(o->onodes);
     free(o);
 }
 
@@ -1977,6 +1986,101 @@ int or_mg_residual(const or_state* o, double* residual) {
 }
 
 /* mg_driver (src/mg_driver.F90:2-323) with mg_ck_conver (:335-405) */
+/* ------------------------------------------------------------------ */
+/* Overset boundary interpolation (src/mg_bintp_mom.F90:2-172,          */
+/* src/mg_bintp_tur.F90:2-182) for one zone on one rank: every donor     */
+/* value is formed from the pre-update q (the send buffer of the         */
+/* alltoallv) before any fringe node is written.                        */
+/* ------------------------------------------------------------------ */
+int or_set_overset(or_state* o, long n, const int* hosts, const double* coef, const int* nodes, int apply_mom) {
+    free(o->ohosts); free(o->ocoef); free(o->onodes);
+    o->ohosts = NULL; o->ocoef = NULL; o->onodes = NULL;
+    o->nover = 0;
+    o->apply_mom = apply_mom;
+    if (n <= 0) return 0;
+    LEVEL(1);
+    for (long d = 0; d < n; d++) {
+        const int* h = hosts + 3 * d;
+        const int* t = nodes + 3 * d;
+        if (h[0] < 1 || h[0] + 1 > ni || h[1] < 1 || h[1] + 1 > nj || h[2] < 1 || h[2] + 1 > nk) return -1;
+        if (t[0] < 1 || t[0] > ni || t[1] < 1 || t[1] > nj || t[2] < 1 || t[2] > nk) return -1;
+    }
+    o->ohosts = (int*)malloc(sizeof(int) * 3 * n);
+    o->onodes = (int*)malloc(sizeof(int) * 3 * n);
+    o->ocoef = (double*)malloc(sizeof(double) * 3 * n);
+    memcpy(o->ohosts, hosts, sizeof(int) * 3 * n);
+    memcpy(o->onodes, nodes, sizeof(int) * 3 * n);
+    memcpy(o->ocoef, coef, sizeof(double) * 3 * n);
+    o->nover = n;
+    return 0;
+}
+
+/* trilinear value of component m of q at donor d (mg_bintp_mom.F90:85-131) */
+static double bintp_value(const or_state* o, long d, int m) {
+    LEVEL(1);
+    const int ih = o->ohosts[3 * d], jh = o->ohosts[3 * d + 1], kh = o->ohosts[3 * d + 2];
+    const double xco = o->ocoef[3 * d], yco = o->ocoef[3 * d + 1], zco = o->ocoef[3 * d + 2];
+    const long n1 = IX(ih, jh, kh), n2 = IX(ih + 1, jh, kh), n3 = IX(ih, jh + 1, kh);
+    const long n4 = IX(ih, jh, kh + 1), n5 = IX(ih + 1, jh, kh + 1), n6 = IX(ih, jh + 1, kh + 1);
+    const long n7 = IX(ih + 1, jh + 1, kh), n8 = IX(ih + 1, jh + 1, kh + 1);
+    const double one = 1.0;
+    const double c000 = (one - xco) * (one - yco) * (one - zco);
+    const double c100 = xco * (one - yco) * (one - zco);
+    const double c010 = (one - xco) * yco * (one - zco);
+    const double c001 = (one - xco) * (one - yco) * zco;
+    const double c101 = xco * (one - yco) * zco;
+    const double c011 = (one - xco) * yco * zco;
+    const double c110 = xco * yco * (one - zco);
+    const double c111 = xco * yco * zco;
+    const double* q = o->q;
+    return q[5 * n1 + m] * c000 + q[5 * n2 + m] * c100 + q[5 * n3 + m] * c010 + q[5 * n4 + m] * c001 +
+           q[5 * n5 + m] * c101 + q[5 * n6 + m] * c011 + q[5 * n7 + m] * c110 + q[5 * n8 + m] * c111;
+}
+
+/* mg_bintp_mom: q(1:4) at the fringe nodes (mg_bintp_mom.F90:59-165) */
+static void bintp_mom(or_state* o) {
+    if (o->nover == 0) return;
+    LEVEL(1);
+    double* buf = (double*)malloc(sizeof(double) * 4 * o->nover);
+    for (long d = 0; d < o->nover; d++)
+        for (int m = 0; m < 4; m++) buf[4 * d + m] = bintp_value(o, d, m);
+    for (long d = 0; d < o->nover; d++) {
+        const int* t = o->onodes + 3 * d;
+        const long l = IX(t[0], t[1], t[2]);
+        for (int m = 0; m < 4; m++) o->q[5 * l + m] = buf[4 * d + m];
+    }
+    free(buf);
+}
+
+/* mg_bintp_tur: nu~ at the fringe nodes, then the nu~ floor and xnut of
+ * the SA model (mg_bintp_tur.F90:36-179, cv1^3 = 357.911 at :23) */
+static void bintp_tur(or_state* o) {
+    if (o->nover == 0) return;
+    LEVEL(1);
+    const double rei = 1.0 / o->p.ren, cv1c = 357.911;
+    double* buf = (double*)malloc(sizeof(double) * o->nover);
+    for (long d = 0; d < o->nover; d++) buf[d] = bintp_value(o, d, 4);
+    for (long d = 0; d < o->nover; d++) {
+        const int* t = o->onodes + 3 * d;
+        const long l = IX(t[0], t[1], t[2]);
+        o->q[5 * l + 4] = buf[d];
+        if (o->p.des) {
+            o->q[5 * l + 4] = dmax2(o->cvin, o->q[5 * l + 4]);
+            const double chi = o->q[5 * l + 4] / rei;
+            const double chic = chi * chi * chi;
+            const double fv1 = chic / (chic + cv1c);
+            o->xnut[l] = o->q[5 * l + 4] * fv1;
+        }
+    }
+    free(buf);
+}
+
+int or_bintp(or_state* o, int which) {
+    if (which == 0) bintp_mom(o);
+    else bintp_tur(o);
+    return 0;
+}
+
 int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
     const int ng = o->ng, ns = 1;
     long n1 = o->nl[1], l;
@@ -2004,8 +2108,12 @@ int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
             }
             for (itr = 1; itr <= o->p.iter[n - 1]; itr++) {
                 if (n != ng) decide_recalc_rh = (itr != o->p.iter[n - 1]) ? 0 : 1;
-                if (o->p.des && n == ns) des_eddy(o);
+                if (o->p.des && n == ns) {
+                    des_eddy(o);
+                    bintp_tur(o);
+                }
                 solver_daf(o, n, decide_recalc_rh, itr);
+                if (n == ns && o->apply_mom) bintp_mom(o);
             }
             mg_zero_pk_boundary(o, n);
         }

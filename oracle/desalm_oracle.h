@@ -55,6 +55,14 @@ int or_mg_prolong(or_state* o, int n);
 /* mg_residual(1) (src/mg_residual.F90:2-104): 5 log10 norms */
 int or_mg_residual(const or_state* o, double* residual);
 
+/* overset donor table of the fine level (src/global_osg.F90:18-28): hosts,
+ * nodes 3 ints (1-based i,j,k) and coef 3 doubles per donor; apply_mom = 1
+ * interpolates q(1:4) after every fine smoother pass (the reference does
+ * when nzone > 1), nu~ is interpolated after every des_eddy.  n = 0 clears. */
+int or_set_overset(or_state* o, long n, const int* hosts, const double* coef, const int* nodes, int apply_mom);
+/* one mg_bintp_mom (which = 0) or mg_bintp_tur (which = 1) on the stored state */
+int or_bintp(or_state* o, int which);
+
 /* LAPACK SGTSV algorithm in FP64 (src/sgtsv.f:1-263); returns INFO */
 int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb);
 /* row interchanges performed by or_gtsv so far (reset to 0 if reset != 0) */

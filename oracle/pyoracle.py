@@ -14,6 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_EXE = os.path.join(HERE, "_ref", "desalm_ref_harness")
+SHIM_MGD_EXE = os.path.join(HERE, "_ref", "desalm_shim_mgd")
+SHIM_DAF_EXE = os.path.join(HERE, "_ref", "desalm_shim_daf")
 
 _lib = None
 
@@ -56,9 +58,12 @@ def lib():
         L.or_mg_prolong.argtypes = [vp, ctypes.c_int]
         L.or_mg_residual.argtypes = [vp, dp]
         L.or_gtsv.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_set_overset.argtypes = [vp, ctypes.c_long, ip, dp, ip, ctypes.c_int]
+        L.or_bintp.argtypes = [vp, ctypes.c_int]
         for f in ("or_set_grid", "or_set_wd", "or_set_state", "or_get_state", "or_get_array",
                   "or_set_array", "or_mg_driver", "or_solver_daf", "or_des_eddy", "or_mg_nlevel",
-                  "or_mg_prolong", "or_mg_residual", "or_gtsv"):
+                  "or_mg_prolong", "or_mg_residual", "or_gtsv", "or_set_overset", "or_bintp"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -101,7 +106,23 @@ class Oracle:
         lib().or_set_wd(self.h, _p(np.ascontiguousarray(case.wd)))
         q, qn, qnm1, xnut = case.state()
         self.set_state(q, qn, qnm1, xnut)
+        if getattr(case, "overset", None) is not None:
+            self.set_overset(case.overset)
         return self
+
+    def set_overset(self, ov):
+        """Donor table (desalm.cases.Overset) of the fine level."""
+        ip = ctypes.POINTER(ctypes.c_int)
+        h = np.ascontiguousarray(ov.hosts, dtype=np.int32)
+        c = np.ascontiguousarray(ov.coef, dtype=np.float64)
+        t = np.ascontiguousarray(ov.nodes, dtype=np.int32)
+        rc = lib().or_set_overset(self.h, len(c), h.ctypes.data_as(ip), _p(c), t.ctypes.data_as(ip),
+                                  int(ov.apply_mom))
+        assert rc == 0, "or_set_overset rejected the table"
+
+    def bintp(self, which):
+        """One mg_bintp_mom (which = "mom") or mg_bintp_tur ("tur")."""
+        lib().or_bintp(self.h, 0 if which == "mom" else 1)
 
     def set_state(self, q=None, qn=None, qnm1=None, xnut=None):
         c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
@@ -182,12 +203,15 @@ def ref_available():
     return os.path.exists(REF_EXE)
 
 
-def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600):
+def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600, exe=None):
     """Run the reference's own Fortran (compiled from its sources by
     oracle/ref/build_ref.sh) on `case`.  The reference is built with three
     multigrid levels (src/global.F90:17); a two-level case runs with iter(3)=0,
     which leaves level 2 untouched by the third level's zero correction.
-    Returns a dict of fine-level arrays and all-level metrics."""
+    exe: another build of the same harness (SHIM_MGD_EXE, SHIM_DAF_EXE: the
+    reference with the Fortran drop-in linked in place of one routine).
+    Returns a dict of fine-level arrays and all-level metrics, plus the text
+    of the 'conver' file under "conver" when the run wrote one."""
     p = case.params
     if p.ng > 3:
         raise ValueError("the reference is compiled for 3 levels")
@@ -214,6 +238,8 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
     rhd[10:13] = list(p.ep[0])
     rhd[13:16] = list(p.ep[1])
     n1 = p.img * p.jmg * p.kmg
+    ov = getattr(case, "overset", None)
+    hdr[35] = 0 if ov is None else len(ov.coef)
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
@@ -222,9 +248,15 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
             rhd.tofile(f)
             for a in (case.x, case.y, case.z, case.wd, case.q, case.qn, case.qnm1, case.xnut):
                 np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+            if ov is not None:
+                np.ascontiguousarray(ov.hosts, dtype=np.int32).tofile(f)
+                np.ascontiguousarray(ov.coef, dtype=np.float64).tofile(f)
+                np.ascontiguousarray(ov.nodes, dtype=np.int32).tofile(f)
         env = dict(os.environ)
-        subprocess.run([REF_EXE, fin, fout], check=True, cwd=td, timeout=timeout, env=env,
+        subprocess.run([exe or REF_EXE, fin, fout], check=True, cwd=td, timeout=timeout, env=env,
                        stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        conver = os.path.join(td, "conver")
+        text = open(conver).read() if os.path.exists(conver) else None
         with open(fout, "rb") as f:
             nall, n1r = np.fromfile(f, dtype=np.int32, count=2)
             assert n1r == n1
@@ -239,6 +271,7 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
             out["zet"] = np.fromfile(f, dtype=np.float64, count=3 * nall).reshape(nall, 3)
             out["aj"] = np.fromfile(f, dtype=np.float64, count=nall)
             out["dtev"] = np.fromfile(f, dtype=np.float64, count=nall)
+    out["conver"] = text
     return out
 
 

@@ -32,6 +32,57 @@ for u in mg_driver solver_daf des_eddy mg_nlevel mg_inject mg_metrics bcond_fm r
 done
 cc "$REF/sgtsv.f" sgtsv
 cc "$REF/xerbla.f" xerbla
+# mg_bintp_mom/tur are internal procedures of mg_driver (mg_driver.F90:332-333);
+# a generated wrapper INCLUDEs them from the reference directory so the
+# harness can call each one on its own.
+cat > "$OUT/obj/harness_bintp.F90" <<EOF2
+subroutine harness_bintp(which)
+  use global
+  use global_param
+  use global_app
+  use global_mpi
+  use global_osg
+  use global_turb
+  implicit none
+  integer, intent(in) :: which
+  integer :: m, i, j, k, l
+  if (which == 0) then
+     call mg_bintp_mom
+  else
+     call mg_bintp_tur
+  end if
+contains
+  include '$REF/mg_bintp_tur.F90'
+  include '$REF/mg_bintp_mom.F90'
+end subroutine harness_bintp
+EOF2
+cc "$OUT/obj/harness_bintp.F90" harness_bintp
+# MPI_REAL arguments moved as 8-byte reals in this FP64 variant
+gcc -O2 -c "$HERE/mpi_real8.c" -I"$MPI_INC" -o "$OUT/obj/mpi_real8.o"
 cc "$HERE/desalm_ref_harness.F90" harness
-"$FC" -o "$OUT/desalm_ref_harness" "$OUT"/obj/*.o -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB"
+REFOBJ=""
+for o in precision global global_param global_app global_mpi global_osg global_turb checksum hooks_off \
+         mg_driver solver_daf des_eddy mg_nlevel mg_inject mg_metrics bcond_fm rsmooth mg_brhs nonreflect_ibc \
+         sgtsv xerbla harness_bintp mpi_real8 harness; do
+    REFOBJ="$REFOBJ $OUT/obj/$o.o"
+done
+"$FC" -o "$OUT/desalm_ref_harness" $REFOBJ -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB"
 echo "built $OUT/desalm_ref_harness"
+
+# The same harness with the Fortran drop-in of des-alm_amd/fortran linked in
+# place of one reference routine: desalm_shim_mgd swaps mg_driver,
+# desalm_shim_daf swaps solver_daf (the reference mg_driver then calls the
+# GPU solver_daf).  Needs des-alm_amd/libdesalm.so (found through $ORIGIN).
+SHIM="$HERE/../../des-alm_amd/fortran"
+LIBDIR="$HERE/../../des-alm_amd"
+if [ -f "$LIBDIR/libdesalm.so" ]; then
+    cc "$SHIM/desalm_f.F90" shim_desalm_f
+    cc "$SHIM/mg_driver.F90" shim_mg_driver
+    cc "$SHIM/solver_daf.F90" shim_solver_daf
+    LIBS=(-L"$LIBDIR" -ldesalm -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB" -Wl,-rpath,'$ORIGIN/../../des-alm_amd')
+    "$FC" -o "$OUT/desalm_shim_mgd" ${REFOBJ/ $OUT\/obj\/mg_driver.o/} "$OUT/obj/shim_desalm_f.o" \
+        "$OUT/obj/shim_mg_driver.o" "${LIBS[@]}"
+    "$FC" -o "$OUT/desalm_shim_daf" ${REFOBJ/ $OUT\/obj\/solver_daf.o/} "$OUT/obj/shim_desalm_f.o" \
+        "$OUT/obj/shim_solver_daf.o" "${LIBS[@]}"
+    echo "built $OUT/desalm_shim_mgd $OUT/desalm_shim_daf"
+fi

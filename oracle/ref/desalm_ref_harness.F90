@@ -17,8 +17,12 @@
 !   real64 rhd(20): er_min, eo_min, ren, beta, delti, cfl1, vnn1, cfl2, vnn2,
 !                   pdiss, ep(1,1:3), ep(2,1:3), (pad)
 !   real64 x(n1), y(n1), z(n1), wd(n1), q(5,n1), qn(5,n1), qnm1(5,n1), xnut(n1)
+!   then, when hdr(36) = nd > 0, an overset donor table of the fine level
+!   (global_osg.F90:18-28, zone 1 to zone 1 on rank 0):
+!   int32 hosts(3,nd), real64 coef(3,nd), int32 nodes(3,nd)
 ! modes: 1 = nsteps calls of mg_driver, 2 = one solver_daf on the fine level,
-!        3 = one des_eddy, 4 = metrics only
+!        3 = one des_eddy, 4 = metrics only, 5 = one mg_bintp_mom,
+!        6 = one mg_bintp_tur
 ! output (stream): int32 nall, n1; real64 q(5,n1), qn(5,n1), qnm1(5,n1),
 !   xnut(n1), rh(4,n1), csi(3,nall), eta(3,nall), zet(3,nall), aj(nall),
 !   dtev(nall)
@@ -34,7 +38,8 @@ program desalm_ref_harness
   integer :: hdr(40)
   real (kind = rdf) :: rhd(20)
   character (len=512) :: fin, fout
-  integer :: mode, nsteps, n, i, j, k, l, m, ib, nall, n1, istep
+  integer :: mode, nsteps, n, i, j, k, l, m, ib, nall, n1, istep, nd
+  integer, allocatable :: ibuf(:,:)
   integer :: la, lb, u
   real (kind = rdf), allocatable :: buf(:)
 
@@ -125,9 +130,12 @@ program desalm_ref_harness
      end if
   end do
 
-  ! no overset donors on one zone
+  ! overset donor table: donors and fringe nodes both on rank 0, zone 1
+  nd = hdr(36)
   allocate (donor(0:0), donee(0:0))
-  donor = 0; donee = 0
+  donor = nd; donee = nd
+  allocate (hosts(4, max(nd,1), 0:0), coef(3, max(nd,1), 0:0), nodes(4, max(nd,1), 0:0))
+  hosts = 1; coef = zero; nodes = 1
 
   n1 = img(1,1) * jmg(1,1) * kmg(1,1)
   nall = le_idx_b(ng)
@@ -143,6 +151,15 @@ program desalm_ref_harness
   read (u) buf(1:5*n1)
   qnm1(1:5,1:n1) = reshape(buf(1:5*n1), (/5, n1/))
   read (u) xnut(1:n1)
+  if (nd > 0) then
+     allocate (ibuf(3, nd))
+     read (u) ibuf
+     hosts(1:3, 1:nd, 0) = ibuf
+     read (u) coef(1:3, 1:nd, 0)
+     read (u) ibuf
+     nodes(1:3, 1:nd, 0) = ibuf
+     deallocate (ibuf)
+  end if
   close (u)
 
   ! grid on all levels and metrics, as init does (init.F90:88-110)
@@ -172,6 +189,12 @@ program desalm_ref_harness
           igp(n), jgp(n), kgp(n), dc(n), de(n), dz(n), dtev(la:lb), &
           q(1:me,la:lb), qn(me,la:lb), qnm1(me,la:lb), &
           csi(1:3,la:lb), eta(1:3,la:lb), zet(1:3,la:lb), aj(la:lb), wd(la:lb), xnut(la:lb))
+  case (5)
+     ns = 1
+     call harness_bintp(0)
+  case (6)
+     ns = 1
+     call harness_bintp(1)
   case default
   end select
 

@@ -32,6 +32,14 @@ FIXTURES = {
     "channelbc_mg_driver_ng3_sa": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=3, des=True, iters=(1, 2, 3),
                                                 btype=(6, 6, 1, 1, 6, 6)), dict(mode=1, nsteps=2)),
     "cavity17_mg_driver_ng2": ("cavity", dict(n=17, ng=2, itm=8), dict(mode=1, nsteps=1)),
+    # overset interpolation (a donor table inside one zone)
+    "wavy_bintp_mom": ("wavy", dict(ni=13, nj=11, nk=9, ng=2, des=True, overset=40), dict(mode=5)),
+    "wavy_bintp_tur": ("wavy", dict(ni=13, nj=11, nk=9, ng=2, des=True, overset=40), dict(mode=6)),
+    "wavy_mg_driver_overset_sa": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=3, des=True, iters=(1, 2, 3),
+                                               overset=60, apply_mom=False), dict(mode=1, nsteps=2)),
+    # early exit on the reference's convergence test (mg_ck_conver)
+    "wavy_mg_driver_converge": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=12, it_min=2, er_min=-0.15,
+                                             iters=(1, 2, 3)), dict(mode=1, nsteps=1)),
 }
 
 
@@ -40,10 +48,23 @@ def build(kind, kw):
 
 
 def fingerprint(case):
+    """sha256 of every input the harness receives."""
     h = hashlib.sha256()
     for a in (case.x, case.y, case.z, case.wd, case.q, case.qn, case.qnm1, case.xnut):
         h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    ov = getattr(case, "overset", None)
+    if ov is not None:
+        h.update(ov.hosts.tobytes())
+        h.update(ov.coef.tobytes())
+        h.update(ov.nodes.tobytes())
+        h.update(bytes([ov.apply_mom]))
     return h.hexdigest()
+
+
+def conver_table(text):
+    """Rows of the 'conver' file (mg_driver.F90:363): iteration, 5 log10 norms."""
+    rows = [ln.split() for ln in text.strip().splitlines()]
+    return np.array([[float(v) for v in r] for r in rows])
 
 
 def main():
@@ -59,6 +80,8 @@ def main():
         if opts["mode"] == 4:
             for nm in ("csi", "eta", "zet", "aj", "dtev"):
                 keep[nm] = out[nm][:nall_c]
+        if opts["mode"] == 1 and out["conver"]:
+            keep["conver"] = conver_table(out["conver"])
         meta = {"kind": kind, "kwargs": {k: list(v) if isinstance(v, tuple) else v for k, v in kw.items()},
                 "options": opts, "inputs_sha256": fingerprint(case)}
         np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=json.dumps(meta), **keep)

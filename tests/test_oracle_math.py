@@ -210,3 +210,35 @@ def test_no_row_interchanges_in_workloads():
     c = cases.cavity(17, ng=2, itm=5)
     po.Oracle(c.params).load_case(c).mg_driver()
     assert po.gtsv_pivots(reset=True) == 0
+
+
+def test_bintp_exact_for_trilinear_fields():
+    """mg_bintp_mom/tur reproduce a field that is trilinear in the node
+    indices: the value at a fringe node is the field at host + coef."""
+    c = cases.wavy(13, 11, 9, ng=2, des=True, overset=50, seed=17)
+    ni, nj, nk = 13, 11, 9
+    k, j, i = np.meshgrid(np.arange(1, nk + 1), np.arange(1, nj + 1), np.arange(1, ni + 1), indexing="ij")
+    i, j, k = i.ravel().astype(float), j.ravel().astype(float), k.ravel().astype(float)
+
+    def f(a, b, d, s):
+        return s * (0.3 + 0.2 * a - 0.1 * b + 0.05 * d + 0.01 * a * b - 0.02 * b * d + 0.015 * a * d
+                    + 0.001 * a * b * d)
+
+    q = c.q.copy()
+    for m in range(5):
+        q[:, m] = f(i, j, k, 1.0 + m)
+    q[:, 4] += 10.0  # nu~ > 0 keeps the floor of mg_bintp_tur inactive
+    ov = c.overset
+    pts = ov.hosts + ov.coef
+    expect = np.stack([f(pts[:, 0], pts[:, 1], pts[:, 2], 1.0 + m) for m in range(5)], axis=1)
+    expect[:, 4] += 10.0
+    fringe = (ov.nodes[:, 0] - 1) + ni * ((ov.nodes[:, 1] - 1) + nj * (ov.nodes[:, 2] - 1))
+    for which, comps in (("mom", range(4)), ("tur", [4])):
+        o = po.Oracle(c.params).load_case(c)
+        o.set_state(q=q)
+        o.bintp(which)
+        out = o.get_state()[0]
+        for m in comps:
+            assert np.allclose(out[fringe, m], expect[:, m], rtol=1e-13, atol=1e-13), (which, m)
+        untouched = np.setdiff1d(np.arange(c.n1), fringe)
+        assert np.array_equal(out[untouched], q[untouched])

@@ -16,6 +16,7 @@ import pytest
 
 from desalm import cases
 import pyoracle as po
+from golden.make_golden import conver_table, fingerprint
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
@@ -27,32 +28,46 @@ def _case(meta):
 
 
 def _run_oracle(case, opts):
+    """Oracle run of a harness mode; returns the oracle and, for mg_driver,
+    the per-iteration rows (iteration, 5 log10 norms) of every step."""
     o = po.Oracle(case.params).load_case(case)
     mode = opts["mode"]
+    rows = []
     if mode == 1:
         for _ in range(opts.get("nsteps", 1)):
-            o.mg_driver()
+            itc, hist = o.mg_driver()
+            rows += [[it + 1] + list(hist[it]) for it in range(itc)]
     elif mode == 2:
         o.solver_daf(1, opts["recalc"], opts["calc_pk"])
     elif mode == 3:
         o.des_eddy()
-    return o
+    elif mode == 5:
+        o.bintp("mom")
+    elif mode == 6:
+        o.bintp("tur")
+    return o, np.array(rows)
+
+
+def check_conver(rows, conver):
+    """Oracle norms against the reference's 'conver' lines, printed g13.6
+    (six significant digits); -Inf where a component does not move."""
+    assert rows.shape == conver.shape
+    assert np.array_equal(rows[:, 0], conver[:, 0])
+    a, b = rows[:, 1:], conver[:, 1:]
+    fin = np.isfinite(b)
+    assert np.array_equal(np.isfinite(a), fin)
+    assert np.array_equal(a[~fin], b[~fin])
+    assert np.all(np.abs(a[fin] - b[fin]) <= 1e-5 * np.abs(b[fin]) + 1e-9)
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
 def test_oracle_matches_reference_golden(path):
     z = np.load(path, allow_pickle=False)
     meta = json.loads(str(z["meta"]))
-    import importlib
-    mk = importlib.import_module("golden.make_golden") if False else None  # noqa: F841
     case = _case(meta)
     # the fixture was produced from exactly these inputs
-    import hashlib
-    h = hashlib.sha256()
-    for a in (case.x, case.y, case.z, case.wd, case.q, case.qn, case.qnm1, case.xnut):
-        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
-    assert h.hexdigest() == meta["inputs_sha256"], "case builder changed; regenerate the fixtures"
-    o = _run_oracle(case, meta["options"])
+    assert fingerprint(case) == meta["inputs_sha256"], "case builder changed; regenerate the fixtures"
+    o, rows = _run_oracle(case, meta["options"])
     n1 = case.n1
     if meta["options"]["mode"] == 4:
         for nm in ("csi", "eta", "zet", "aj", "dtev"):
@@ -63,6 +78,8 @@ def test_oracle_matches_reference_golden(path):
     assert np.array_equal(o.get("xnut")[:n1], z["xnut"])
     if "rh" in z.files:
         assert np.array_equal(o.get("rh")[:n1], z["rh"])
+    if "conver" in z.files:
+        check_conver(rows, z["conver"])
 
 
 @pytest.mark.skipif(not po.ref_available(), reason="reference harness not built (needs /root/reference)")
@@ -76,3 +93,24 @@ def test_oracle_matches_reference_live(ng, des, bt):
     q = o.get_state()[0]
     assert np.array_equal(q, r["q"])
     assert np.array_equal(o.get_state()[3], r["xnut"])
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="reference harness not built (needs /root/reference)")
+@pytest.mark.parametrize("mode", [5, 6])
+def test_oracle_bintp_matches_reference_live(mode):
+    c = cases.wavy(15, 13, 11, ng=2, des=True, overset=70, seed=31)
+    o, _ = _run_oracle(c, dict(mode=mode))
+    r = po.run_reference(c, mode=mode)
+    assert np.array_equal(o.get_state()[0], r["q"])
+    assert np.array_equal(o.get_state()[3], r["xnut"])
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="reference harness not built (needs /root/reference)")
+def test_oracle_overset_mg_driver_and_norms_live():
+    c = cases.wavy(17, 17, 21, ng=3, itm=6, it_min=1, er_min=-0.08, des=True, iters=(1, 2, 3), seed=5,
+                   overset=90, apply_mom=False)
+    o, rows = _run_oracle(c, dict(mode=1, nsteps=2))
+    r = po.run_reference(c, mode=1, nsteps=2)
+    assert np.array_equal(o.get_state()[0], r["q"])
+    assert np.array_equal(o.get_state()[3], r["xnut"])
+    check_conver(rows, conver_table(r["conver"]))
