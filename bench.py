@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--iters", type=int, default=10, help="pseudo-iterations per physical step")
     ap.add_argument("--n", type=int, default=129, help="nodes per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--cpu-iters", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

@@ -22,11 +22,11 @@ using namespace desalm;
 // Per-kernel device timing with HIP events on the context stream.
 enum KernelId {
     KID_DTAU, KID_RESIDUAL, KID_RESIDUAL_RECALC, KID_DAF_I, KID_DAF_J, KID_DAF_K, KID_BCOND,
-    KID_DES_EDDY, KID_NLEVEL, KID_PROLONG, KID_RESID, KID_COUNT
+    KID_DES_EDDY, KID_NLEVEL, KID_PROLONG, KID_RESID, KID_OVERSET, KID_COUNT
 };
 static const char* kKernelNames[KID_COUNT] = {
     "dtau", "residual", "residual_recalc", "daf_i", "daf_j", "daf_k", "bcond_fm",
-    "des_eddy", "nlevel", "prolong", "resid_norm"};
+    "des_eddy", "nlevel", "prolong", "resid_norm", "overset"};
 
 struct Profiler {
     bool on = false;
@@ -59,6 +59,11 @@ struct desalm_ctx {
     double init_resd[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Profiler prof;
+    // overset donor table (desalm_set_overset); its buffers are freed on
+    // replacement and by desalm_finalize
+    Overset ov{0, nullptr, nullptr, nullptr, nullptr};
+    int ov_mom = 0;
+    double cvin = 0.0;
 };
 
 namespace {
@@ -81,8 +86,8 @@ int fail(desalm_ctx* c, const std::string& msg) {
 double* dalloc(desalm_ctx* c, long n) {
     void* p = nullptr;
     if (hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
-    hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(double));
     c->allocs.push_back(p);
+    if (hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
     return (double*)p;
 }
 
@@ -222,8 +227,10 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     c->ng = p->ng;
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail(nullptr, "hipSetDevice failed"); }
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(nullptr, "hipStreamCreate failed"); }
-    hipEventCreate(&c->ev0);
-    hipEventCreate(&c->ev1);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        desalm_finalize(c);
+        return fail(nullptr, "hipEventCreate failed");
+    }
     // levels (src/init.F90:254-260, init_mg :1116-1120)
     int i = p->img, j = p->jmg, k = p->kmg;
     long off = 0;
@@ -250,6 +257,7 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     memset(&C, 0, sizeof(C));
     C.ren = p->ren;
     C.rei = 1.0 / p->ren;
+    c->cvin = 0.1 / p->ren;  // cvin = pt1 / ren (src/init.F90:250)
     C.beta = p->beta;
     C.delti = p->delti;
     C.e_source = p->unsteady ? 1.0 : 0.0;
@@ -301,7 +309,11 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     {
         void* pv = nullptr;
         if (hipMalloc(&pv, sizeof(unsigned long long)) != hipSuccess) ok = false;
-        else { hipMemset(pv, 0, sizeof(unsigned long long)); c->allocs.push_back(pv); F.pivots = (unsigned long long*)pv; }
+        else {
+            c->allocs.push_back(pv);
+            F.pivots = (unsigned long long*)pv;
+            if (hipMemset(pv, 0, sizeof(unsigned long long)) != hipSuccess) ok = false;
+        }
     }
     c->staging = A(5 * c->nall);
     c->dq_buf = A(4 * c->n1);
@@ -431,8 +443,16 @@ int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
                 timed(c, KID_NLEVEL, n, [&] { launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, p.icrs, p.jcrs, p.kcrs); });
             for (int itr = 1; itr <= p.iter[n - 1]; itr++) {
                 if (n != c->ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
-                if (p.des && n == 1) timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, L1, c->F, c->C, p.itk); });
+                if (p.des && n == 1) {
+                    timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, L1, c->F, c->C, p.itk); });
+                    // mg_bintp_tur after des_eddy (mg_driver.F90:182)
+                    if (c->ov.n > 0)
+                        timed(c, KID_OVERSET, 1, [&] { launch_bintp(s, L1, c->F, c->C, c->ov, 1, c->cvin, p.des); });
+                }
                 solver_daf(c, n, decide_recalc_rh, itr, nullptr);
+                // mg_bintp_mom after the fine smoother pass (mg_driver.F90:241-244)
+                if (n == 1 && c->ov.n > 0 && c->ov_mom)
+                    timed(c, KID_OVERSET, 1, [&] { launch_bintp(s, L1, c->F, c->C, c->ov, 0, c->cvin, p.des); });
             }
         }
         for (int n = c->ng - 1; n >= 1; n--) {
@@ -469,7 +489,7 @@ int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
     HIPCHK(c, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     if (itc_out) *itc_out = itc;
     return 0;
@@ -551,23 +571,25 @@ int desalm_gtsv_batch(int n, int nsys, double* dl, double* d, double* du, double
         hipMalloc(&ddu, bytes) != hipSuccess || hipMalloc(&db, bytes) != hipSuccess ||
         hipMalloc(&dinfo, sizeof(int) * nsys) != hipSuccess) {
         rc = fail(nullptr, "device allocation failed");
+    } else if (hipMemcpy(ddl, dl, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(dd, d, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(ddu, du, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(nullptr, "gtsv: host-to-device copy failed");
     } else {
-        hipMemcpy(ddl, dl, bytes, hipMemcpyHostToDevice);
-        hipMemcpy(dd, d, bytes, hipMemcpyHostToDevice);
-        hipMemcpy(ddu, du, bytes, hipMemcpyHostToDevice);
-        hipMemcpy(db, b, bytes, hipMemcpyHostToDevice);
         launch_gtsv_batch(0, n, nsys, ddl, dd, ddu, db, dinfo);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
             rc = fail(nullptr, "gtsv kernel failed");
-        } else {
-            hipMemcpy(dl, ddl, bytes, hipMemcpyDeviceToHost);
-            hipMemcpy(d, dd, bytes, hipMemcpyDeviceToHost);
-            hipMemcpy(du, ddu, bytes, hipMemcpyDeviceToHost);
-            hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost);
-            if (info_out) hipMemcpy(info_out, dinfo, sizeof(int) * nsys, hipMemcpyDeviceToHost);
+        } else if (hipMemcpy(dl, ddl, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                   hipMemcpy(d, dd, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                   hipMemcpy(du, ddu, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                   hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                   (info_out && hipMemcpy(info_out, dinfo, sizeof(int) * nsys, hipMemcpyDeviceToHost) != hipSuccess)) {
+            rc = fail(nullptr, "gtsv: device-to-host copy failed");
         }
     }
-    hipFree(ddl); hipFree(dd); hipFree(ddu); hipFree(db); hipFree(dinfo);
+    // teardown: a failing free leaves nothing to recover
+    (void)hipFree(ddl); (void)hipFree(dd); (void)hipFree(ddu); (void)hipFree(db); (void)hipFree(dinfo);
     return rc;
 }
 
@@ -622,16 +644,77 @@ const char* desalm_profile_name(int kernel) {
 
 const char* desalm_last_error(const desalm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
+static void free_overset(desalm_ctx* c) {
+    (void)hipFree((void*)c->ov.host);
+    (void)hipFree((void*)c->ov.coef);
+    (void)hipFree((void*)c->ov.node);
+    (void)hipFree(c->ov.buf);
+    c->ov = Overset{0, nullptr, nullptr, nullptr, nullptr};
+}
+
+int desalm_set_overset(desalm_ctx* c, int64_t n, const int32_t* hosts, const double* coef, const int32_t* nodes,
+                       int apply_mom) {
+    if (!c) return fail(c, "null argument");
+    if (n < 0 || (n > 0 && (!hosts || !coef || !nodes))) return fail(c, "desalm_set_overset: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_overset(c);
+    c->ov_mom = apply_mom != 0;
+    if (n == 0) return 0;
+    const Level& L = c->lv[1];
+    std::vector<long> host(n), node(n);
+    for (int64_t d = 0; d < n; d++) {
+        const int32_t* h = hosts + 3 * d;
+        const int32_t* t = nodes + 3 * d;
+        if (h[0] < 1 || h[0] + 1 > L.ni || h[1] < 1 || h[1] + 1 > L.nj || h[2] < 1 || h[2] + 1 > L.nk)
+            return fail(c, "desalm_set_overset: host cell " + std::to_string(d) + " outside the fine level");
+        if (t[0] < 1 || t[0] > L.ni || t[1] < 1 || t[1] > L.nj || t[2] < 1 || t[2] > L.nk)
+            return fail(c, "desalm_set_overset: fringe node " + std::to_string(d) + " outside the fine level");
+        host[d] = (h[0] - 1) + (long)L.ni * ((h[1] - 1) + (long)L.nj * (h[2] - 1));
+        node[d] = (t[0] - 1) + (long)L.ni * ((t[1] - 1) + (long)L.nj * (t[2] - 1));
+    }
+    {
+        std::vector<long> sorted(node);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(c, "desalm_set_overset: a fringe node is listed twice");
+    }
+    long *dh = nullptr, *dn = nullptr;
+    double *dc = nullptr, *db = nullptr;
+    if (hipMalloc((void**)&dh, sizeof(long) * n) != hipSuccess || hipMalloc((void**)&dn, sizeof(long) * n) != hipSuccess ||
+        hipMalloc((void**)&dc, sizeof(double) * 3 * n) != hipSuccess ||
+        hipMalloc((void**)&db, sizeof(double) * 4 * n) != hipSuccess) {
+        (void)hipFree(dh); (void)hipFree(dn); (void)hipFree(dc); (void)hipFree(db);
+        return fail(c, "desalm_set_overset: out of device memory");
+    }
+    c->ov = Overset{(long)n, dh, dc, dn, db};
+    HIPCHK(c, hipMemcpy(dh, host.data(), sizeof(long) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dn, node.data(), sizeof(long) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dc, coef, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int desalm_bintp(desalm_ctx* c, int which) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    launch_bintp(c->stream, c->lv[1], c->F, c->C, c->ov, which != 0, c->cvin, c->p.des);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int desalm_finalize(desalm_ctx* c) {
     if (!c) return 0;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    // teardown: errors here leave nothing to recover, so they are ignored
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_overset(c);
     prof_flush(c);
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }

@@ -72,6 +72,17 @@ struct Fields {
 
 constexpr int kResidBlocks = 1024;
 
+// Overset donor table of the fine level (device pointers, level-local node
+// indices): donor d reads the cell whose lower corner is host[d] with
+// trilinear weights coef[3d..3d+2] and writes fringe node node[d].
+struct Overset {
+    long n;
+    const long* host;
+    const double* coef;
+    const long* node;
+    double* buf;  // 4 * n: gathered donor values
+};
+
 struct Ctx;  // host context, desalm_api.hip
 
 // ---- launchers (each enqueues on `s`) ----
@@ -104,6 +115,9 @@ void launch_metrics(hipStream_t s, const Level& L, const Fields& F, const Consts
 // layout conversion, k_mg.hip
 void launch_aos_to_soa(hipStream_t s, const double* aos, int ncomp, long n, double* const* soa);
 void launch_soa_to_aos(hipStream_t s, double* const* soa, int ncomp, long n, double* aos);
+// overset interpolation (tur = 0: q(1:4), 1: nu~ and xnut), k_overset.hip
+void launch_bintp(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
+                  double cvin, int des);
 // batched GEPP (LAPACK SGTSV semantics), k_daf.hip
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,
                        double* b, int* info);

@@ -13,6 +13,7 @@
 // |dl_i|; rows where that happens are counted in F.pivots so a caller can see
 // when the pivot-free elimination departed from SGTSV's row order.
 #include "desalm_internal.h"
+#include "line_tile.h"
 
 namespace desalm {
 
@@ -343,6 +344,130 @@ __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, in
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
+// The i-direction sweep of k_daf_tri with its rows staged through LDS
+// (line_tile.h): a workgroup owns kTL i-lines, thread = (system m, line l).
+// Chunks of kTR rows of sd and d are loaded one chunk ahead of the
+// elimination; c' and d' go back to HBM per chunk except for the last chunk,
+// which the back substitution starts from in LDS.  Same arithmetic as
+// k_daf_tri<0>.
+__global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts C, int use_b1, int use_b2) {
+    constexpr int NT = 4 * kTL;
+    constexpr int kSS = kTR + 3;  // LDS stride of the sd tile (kTR + 2 rows, odd)
+    using IOr = TileIO<NT, kTL, kTR, kTS>;
+    using IOs = TileIO<NT, kTL, kTR + 2, kSS>;
+    __shared__ long s_base[kTL];
+    __shared__ double s_sd[kTL * kSS];
+    __shared__ double s_d[4][kTL * kTS];
+    __shared__ double s_c[4][kTL * kTS];
+
+    const long nlines = (long)(L.nj - 2) * (L.nk - 2);
+    const long t0 = (long)blockIdx.x * kTL;
+    const int nl = (int)min((long)kTL, nlines - t0);
+    const int m = (int)threadIdx.x / kTL, l = (int)threadIdx.x % kTL;
+    const bool act = l < nl;
+    const long t = t0 + l;
+    if (threadIdx.x < kTL) s_base[threadIdx.x] = act ? iline_base(L, t) : 0;
+    __syncthreads();
+
+    const int len = L.ni - 2;
+    const int nch = (len + kTR - 1) / kTR;
+    const double lam = m == 2 ? 1.0 : m == 3 ? -1.0 : 0.0;
+    const double h = L.dc;
+    const double h2 = 0.5 * h, hsq = h * h, epd = C.ep[0][0];
+    const double* __restrict__ sd = F.sd;
+    const double brh1 = (use_b1 && act) ? F.brh[t * 8 + m] : 0.0;
+    const double brh2 = (use_b2 && act) ? F.brh[t * 8 + 4 + m] : 0.0;
+    double* myd = s_d[m] + l * kTS;
+    double* myc = s_c[m] + l * kTS;
+    const double* mys = s_sd + l * kSS;
+
+    // row s of a line is node base + 1 + s; sd of rows s-1..s+1 are nodes base + s .. base + s + 2
+    double vd[4][IOr::kPer], vs[IOs::kPer];
+    for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], F.dp[mm] + 1, s_base, nl, 0, len);
+    IOs::load(vs, sd, s_base, nl, 0, len + 2);
+
+    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
+    unsigned piv = 0;
+    for (int ch = 0; ch < nch; ch++) {
+        const int s0 = ch * kTR;
+        __syncthreads();
+        for (int mm = 0; mm < 4; mm++) IOr::to_lds(vd[mm], s_d[mm]);
+        IOs::to_lds(vs, s_sd);
+        __syncthreads();
+        if (ch + 1 < nch) {
+            for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], F.dp[mm] + 1, s_base, nl, s0 + kTR, len);
+            IOs::load(vs, sd, s_base, nl, s0 + kTR, len + 2);
+        }
+        if (act) {
+            for (int u = 0; u < kTR; u++) {
+                const int s = s0 + u;
+                if (s >= len) break;
+                const double sdm = mys[u], sd0 = mys[u + 1], sdp = mys[u + 2];
+                const double tmp = sd0 * epd * hsq;
+                const double dtc = (sd0 == 0.0) ? 0.0 : 1.0;
+                const double ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
+                const double a = -ddw * lam - tmp;
+                const double c = dde * lam - tmp;
+                const double b = 1.0 + 2.0 * tmp;
+                double d = myd[u];
+                if (s == 0 && use_b1) d = d - a * brh1;
+                if (s == len - 1 && use_b2) d = d - c * brh2;
+                double den;
+                if (s == 0) {
+                    den = b;
+                } else {
+                    if (fabs(denprev) < fabs(a)) piv++;  // SGTSV would interchange rows here
+                    den = b - a * cprev;
+                }
+                const double inv = 1.0 / den;
+                cprev = c * inv;
+                dprev = (s == 0) ? d * inv : (d - a * dprev) * inv;
+                denprev = den;
+                myc[u] = cprev;
+                myd[u] = dprev;
+            }
+        }
+        __syncthreads();
+        if (ch + 1 < nch) {
+            for (int mm = 0; mm < 4; mm++) {
+                IOr::store(s_c[mm], F.cp[mm] + 1, s_base, nl, s0, len);
+                IOr::store(s_d[mm], F.dp[mm] + 1, s_base, nl, s0, len);
+            }
+        }
+    }
+    // back substitution, last chunk first (already in LDS)
+    double vc[4][IOr::kPer];
+    double x = 0.0;
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int s0 = ch * kTR;
+        if (ch < nch - 1) {
+            __syncthreads();
+            for (int mm = 0; mm < 4; mm++) {
+                IOr::to_lds(vc[mm], s_c[mm]);
+                IOr::to_lds(vd[mm], s_d[mm]);
+            }
+            __syncthreads();
+        }
+        if (ch > 0) {
+            for (int mm = 0; mm < 4; mm++) {
+                IOr::load(vc[mm], F.cp[mm] + 1, s_base, nl, s0 - kTR, len);
+                IOr::load(vd[mm], F.dp[mm] + 1, s_base, nl, s0 - kTR, len);
+            }
+        }
+        if (act) {
+            for (int u = kTR - 1; u >= 0; u--) {
+                const int s = s0 + u;
+                if (s >= len) continue;
+                x = (s == len - 1) ? myd[u] : myd[u] - myc[u] * x;
+                myd[u] = x;
+            }
+        }
+        __syncthreads();
+        for (int mm = 0; mm < 4; mm++) IOr::store(s_d[mm], F.rh[mm] + L.off + 1, s_base, nl, s0, len);
+    }
+    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
+}
+
 // dQ = Mc dQ*** and q = q + dQ on interior nodes (rhs_diag_solver.F90:624-633,
 // solver_daf.F90:409-418)
 __global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, double* dq_out) {
@@ -439,7 +564,8 @@ static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Cons
     const dim3 gt((unsigned)((nl + 127) / 128), 4);
     if (D == 0) {
         hipLaunchKernelGGL(k_daf_pre<0>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
-        hipLaunchKernelGGL(k_daf_tri<0>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);
+        const unsigned gi = (unsigned)((nl + kTL - 1) / kTL);
+        hipLaunchKernelGGL(k_daf_tri_i, dim3(gi), dim3(4 * kTL), 0, s, L, F, C, use_b1, use_b2);
     } else if (D == 1) {
         hipLaunchKernelGGL(k_daf_pre<1>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
         hipLaunchKernelGGL(k_daf_tri<1>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);

@@ -13,6 +13,7 @@
 //                   the pinned-pressure shift (:310-322)
 //   k_resid_*       mg_residual's L1 update norms (src/mg_residual.F90:58-67)
 #include "desalm_internal.h"
+#include "line_tile.h"
 
 namespace desalm {
 
@@ -223,6 +224,75 @@ __global__ void __launch_bounds__(128) k_rs(Level L, Fields F, const double* tab
     }
 }
 
+// The i-direction rs_sweep of k_rs with its rows staged through LDS
+// (line_tile.h): thread = (component m, line l), kTL lines per workgroup.
+// Same arithmetic as k_rs<0>.
+__global__ void __launch_bounds__(4 * kTL) k_rs_i(Level L, Fields F, const double* tab, double epn) {
+    constexpr int NT = 4 * kTL;
+    using IO = TileIO<NT, kTL, kTR, kTS>;
+    __shared__ long s_base[kTL];
+    __shared__ double s_v[4][kTL * kTS];
+
+    const long nlines = (long)(L.nj - 2) * (L.nk - 2);
+    const long t0 = (long)blockIdx.x * kTL;
+    const int nl = (int)min((long)kTL, nlines - t0);
+    const int m = (int)threadIdx.x / kTL, l = (int)threadIdx.x % kTL;
+    const bool act = l < nl;
+    if (threadIdx.x < kTL) s_base[threadIdx.x] = act ? iline_base(L, t0 + l) : 0;
+    __syncthreads();
+
+    const int len = L.ni - 2;
+    const int nch = (len + kTR - 1) / kTR;
+    const double* fact = tab;
+    const double* dd = tab + (len - 1);
+    const double du = -epn;
+    double* myv = s_v[m] + l * kTS;
+
+    double v[4][IO::kPer];
+    for (int mm = 0; mm < 4; mm++) IO::load(v[mm], F.dp[mm] + 1, s_base, nl, 0, len);
+    double prev = 0.0;
+    for (int ch = 0; ch < nch; ch++) {
+        const int s0 = ch * kTR;
+        __syncthreads();
+        for (int mm = 0; mm < 4; mm++) IO::to_lds(v[mm], s_v[mm]);
+        __syncthreads();
+        if (ch + 1 < nch)
+            for (int mm = 0; mm < 4; mm++) IO::load(v[mm], F.dp[mm] + 1, s_base, nl, s0 + kTR, len);
+        if (act) {
+            for (int u = 0; u < kTR; u++) {
+                const int s = s0 + u;
+                if (s >= len) break;
+                prev = (s == 0) ? myv[u] : myv[u] - fact[s - 1] * prev;
+                myv[u] = prev;
+            }
+        }
+        __syncthreads();
+        if (ch + 1 < nch)
+            for (int mm = 0; mm < 4; mm++) IO::store(s_v[mm], F.dp[mm] + 1, s_base, nl, s0, len);
+    }
+    double x = 0.0;
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int s0 = ch * kTR;
+        if (ch < nch - 1) {
+            __syncthreads();
+            for (int mm = 0; mm < 4; mm++) IO::to_lds(v[mm], s_v[mm]);
+            __syncthreads();
+        }
+        if (ch > 0)
+            for (int mm = 0; mm < 4; mm++) IO::load(v[mm], F.dp[mm] + 1, s_base, nl, s0 - kTR, len);
+        if (act) {
+            for (int u = kTR - 1; u >= 0; u--) {
+                const int s = s0 + u;
+                if (s >= len) continue;
+                x = (s == len - 1) ? myv[u] / dd[len - 1] : (myv[u] - du * x) / dd[s];
+                myv[u] = x;
+            }
+        }
+        __syncthreads();
+        for (int mm = 0; mm < 4; mm++) IO::store(s_v[mm], F.dp[mm] + 1, s_base, nl, s0, len);
+    }
+}
+
 // ---- bcond_fm faces of direction D (both sides) ----
 template <int D>
 __global__ void k_bc_fm(Level L, Fields F, Consts C) {
@@ -394,7 +464,7 @@ void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Field
     hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(Lf.n, 256)), dim3(256), 0, s, P, F);
     // rs_tab: three direction tables of 2 * max(ni,nj,nk) doubles
     const long stride = 2L * (Lf.ni > Lf.nj ? (Lf.ni > Lf.nk ? Lf.ni : Lf.nk) : (Lf.nj > Lf.nk ? Lf.nj : Lf.nk));
-    hipLaunchKernelGGL(k_rs<0>, dim3(nblk((long)(Lf.nj - 2) * (Lf.nk - 2), 128), 4), dim3(128), 0, s, Lf, F, rs_tab,
+    hipLaunchKernelGGL(k_rs_i, dim3(nblk((long)(Lf.nj - 2) * (Lf.nk - 2), kTL)), dim3(4 * kTL), 0, s, Lf, F, rs_tab,
                        C.ep[1][0]);
     hipLaunchKernelGGL(k_rs<1>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nk - 2), 128), 4), dim3(128), 0, s, Lf, F,
                        rs_tab + stride, C.ep[1][1]);

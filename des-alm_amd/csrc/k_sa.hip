@@ -12,6 +12,7 @@
 //               the update of des_eddy.F90:260-272)
 //   k_sa_bc<D>  rhs_sa_bcond faces, k_sa_clamp its nu~ floor and xnut
 #include "desalm_internal.h"
+#include "line_tile.h"
 
 namespace desalm {
 
@@ -329,6 +330,128 @@ __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
+// The i-direction sweep of k_sa_adi with its rows staged through LDS
+// (line_tile.h); one thread per line, kSL lines per workgroup, and a second
+// wave that only helps to move the tiles.  Chunks of av,
+// U/J, dtj and rh are loaded one chunk ahead of the elimination; c' and d'
+// go back to HBM per chunk except for the last one.  Same arithmetic as
+// k_sa_adi<0>.
+constexpr int kSL = 64;
+
+__global__ void __launch_bounds__(2 * kSL) k_sa_adi_i(Level L, Fields F) {
+    constexpr int NT = 2 * kSL;
+    using IOr = TileIO<NT, kSL, kTR, kTS>;          // rows s0.. at node base + 1 + s
+    using IOa = TileIO<NT, kSL, kTR + 1, kTR + 1>;  // av at nodes base + s0 .. base + s0 + kTR
+    using IOu = TileIO<NT, kSL, kTR + 2, kTR + 3>;  // U/J at nodes base + s0 .. base + s0 + kTR + 1
+    __shared__ long s_base[kSL];
+    __shared__ double s_av[kSL * (kTR + 1)];
+    __shared__ double s_u[kSL * (kTR + 3)];
+    __shared__ double s_dt[kSL * kTS];
+    __shared__ double s_r[kSL * kTS];
+    __shared__ double s_c[kSL * kTS];
+
+    const long nlines = (long)(L.nj - 2) * (L.nk - 2);
+    const long t0 = (long)blockIdx.x * kSL;
+    const int nl = (int)min((long)kSL, nlines - t0);
+    const int l = (int)threadIdx.x;
+    const bool act = l < nl;  // threads kSL.. only move tiles
+    if (l < kSL) s_base[l] = act ? iline_base(L, t0 + l) : 0;
+    __syncthreads();
+
+    const int len = L.ni - 2;
+    const int nch = (len + kTR - 1) / kTR;
+    const double* __restrict__ U = F.ucn[0];
+    const double* __restrict__ av = F.av[0];
+    double* __restrict__ rh = F.rh5;
+    double* __restrict__ cp = F.cp[0];
+    const int lc = l < kSL ? l : 0;
+    double* myr = s_r + lc * kTS;
+    double* myc = s_c + lc * kTS;
+    const double* mydt = s_dt + lc * kTS;
+    const double* mya = s_av + lc * (kTR + 1);
+    const double* myu = s_u + lc * (kTR + 3);
+
+    double va[IOa::kPer], vu[IOu::kPer], vt[IOr::kPer], vr[IOr::kPer];
+    IOa::load(va, av, s_base, nl, 0, len + 1);
+    IOu::load(vu, U, s_base, nl, 0, len + 2);
+    IOr::load(vt, F.dtj + 1, s_base, nl, 0, len);
+    IOr::load(vr, rh + 1, s_base, nl, 0, len);
+
+    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
+    unsigned piv = 0;
+    for (int ch = 0; ch < nch; ch++) {
+        const int s0 = ch * kTR;
+        __syncthreads();
+        IOa::to_lds(va, s_av);
+        IOu::to_lds(vu, s_u);
+        IOr::to_lds(vt, s_dt);
+        IOr::to_lds(vr, s_r);
+        __syncthreads();
+        if (ch + 1 < nch) {
+            const int s1 = s0 + kTR;
+            IOa::load(va, av, s_base, nl, s1, len + 1);
+            IOu::load(vu, U, s_base, nl, s1, len + 2);
+            IOr::load(vt, F.dtj + 1, s_base, nl, s1, len);
+            IOr::load(vr, rh + 1, s_base, nl, s1, len);
+        }
+        if (act) {
+            for (int u = 0; u < kTR; u++) {
+                const int s = s0 + u;
+                if (s >= len) break;
+                const double um = myu[u], up = myu[u + 2];
+                const double avm = mya[u], avp = mya[u + 1], dt = mydt[u];
+                const double aw = -dt * (um + avm);
+                const double ap = 1.0 + dt * (avm + avp);
+                const double ae = dt * (up - avp);
+                const double aq = myr[u];
+                double den;
+                if (s == 0) {
+                    den = ap;
+                } else {
+                    if (fabs(denprev) < fabs(aw)) piv++;
+                    den = ap - aw * cprev;
+                }
+                const double inv = 1.0 / den;
+                cprev = ae * inv;
+                dprev = (s == 0) ? aq * inv : (aq - aw * dprev) * inv;
+                denprev = den;
+                myc[u] = cprev;
+                myr[u] = dprev;
+            }
+        }
+        __syncthreads();
+        if (ch + 1 < nch) {
+            IOr::store(s_c, cp + 1, s_base, nl, s0, len);
+            IOr::store(s_r, rh + 1, s_base, nl, s0, len);
+        }
+    }
+    double x = 0.0;
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int s0 = ch * kTR;
+        if (ch < nch - 1) {
+            __syncthreads();
+            IOr::to_lds(vt, s_c);
+            IOr::to_lds(vr, s_r);
+            __syncthreads();
+        }
+        if (ch > 0) {
+            IOr::load(vt, cp + 1, s_base, nl, s0 - kTR, len);
+            IOr::load(vr, rh + 1, s_base, nl, s0 - kTR, len);
+        }
+        if (act) {
+            for (int u = kTR - 1; u >= 0; u--) {
+                const int s = s0 + u;
+                if (s >= len) continue;
+                x = (s == len - 1) ? myr[u] : myr[u] - myc[u] * x;
+                myr[u] = x;
+            }
+        }
+        __syncthreads();
+        IOr::store(s_r, rh + 1, s_base, nl, s0, len);
+    }
+    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
+}
+
 // rhs_sa_bcond faces (rhs_sa_bcond.F90:40-269); faces of direction D, both sides
 template <int D>
 __global__ void k_sa_bc(Level L, Fields F, Consts C) {
@@ -396,7 +519,7 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     const unsigned gn = (unsigned)((L.n + b - 1) / b);
     hipLaunchKernelGGL(k_sa_prep, dim3(gn), dim3(b), 0, s, L, F);
     const int bl = 128;
-    const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + bl - 1) / bl);
+    const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + kSL - 1) / kSL);
     const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + bl - 1) / bl);
     const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + bl - 1) / bl);
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
@@ -405,7 +528,7 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     for (int it = 0; it < itk; it++) {
         hipLaunchKernelGGL(k_sa_rhs, dim3(gn), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi_pre, dim3(gn), dim3(b), 0, s, L, F, C);
-        hipLaunchKernelGGL(k_sa_adi<0>, dim3(gi), dim3(bl), 0, s, L, F, C);
+        hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(2 * kSL), 0, s, L, F);
         hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<0>, dim3(fi), dim3(b), 0, s, L, F, C);

@@ -63,9 +63,13 @@ def lib():
         L.desalm_last_error.argtypes = [vp]
         L.desalm_last_error.restype = ctypes.c_char_p
         L.desalm_finalize.argtypes = [vp]
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.desalm_set_overset.argtypes = [vp, ctypes.c_int64, i32p, dp, i32p, ctypes.c_int]
+        L.desalm_bintp.argtypes = [vp, ctypes.c_int]
         for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
-                  "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize"):
+                  "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize",
+                  "desalm_set_overset", "desalm_bintp"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -116,7 +120,25 @@ class Solver:
         self.set_grid(case.x, case.y, case.z)
         self.set_wall_distance(case.wd)
         self.set_state(*case.state())
+        if getattr(case, "overset", None) is not None:
+            self.set_overset(case.overset)
         return self
+
+    def set_overset(self, ov):
+        """Donor table (desalm.cases.Overset) of the fine level; None removes it."""
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        if ov is None:
+            self._chk(lib().desalm_set_overset(self.h, 0, None, None, None, 0), "desalm_set_overset")
+            return
+        h = np.ascontiguousarray(ov.hosts, dtype=np.int32)
+        c = np.ascontiguousarray(ov.coef, dtype=np.float64)
+        t = np.ascontiguousarray(ov.nodes, dtype=np.int32)
+        self._chk(lib().desalm_set_overset(self.h, len(c), h.ctypes.data_as(i32p), _p(c), t.ctypes.data_as(i32p),
+                                           int(ov.apply_mom)), "desalm_set_overset")
+
+    def bintp(self, which):
+        """One mg_bintp_mom (which = "mom") or mg_bintp_tur ("tur") on the device state."""
+        self._chk(lib().desalm_bintp(self.h, 0 if which == "mom" else 1), "desalm_bintp")
 
     def set_grid(self, x, y, z):
         x, y, z = _c(x), _c(y), _c(z)

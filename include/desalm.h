@@ -162,6 +162,27 @@ int desalm_des_eddy(desalm_ctx* ctx, double* q, const double* qn5, const double*
 int desalm_gtsv_batch(int n, int nsys, double* dl, double* d, double* du, double* b,
                       int* info_out);
 
+/*
+ * Overset donor table of the fine level, in place of the tables that
+ * init_osg_interface reads (src/global_osg.F90:18-28, src/init.F90:1979-2000),
+ * for donors and fringe nodes of this one zone: donor d interpolates
+ * trilinearly from the cell whose lower corner is hosts[3d..3d+2] (1-based
+ * i, j, k) with weights coef[3d..3d+2] into the fine node nodes[3d..3d+2].
+ * desalm_mg_driver then interpolates nu~ (and sets xnut) after every des_eddy
+ * (mg_bintp_tur, src/mg_driver.F90:182), and q(1:4) after every fine smoother
+ * pass when apply_mom != 0 (mg_bintp_mom, src/mg_driver.F90:241-244; the
+ * reference runs it when nzone > 1).  All donor values are formed before any
+ * fringe node is written.  Fringe nodes must be distinct.  n = 0 removes the
+ * table.
+ */
+int desalm_set_overset(desalm_ctx* ctx, int64_t n, const int32_t* hosts, const double* coef,
+                       const int32_t* nodes, int apply_mom);
+
+/* One interpolation with the stored table on the device state: which = 0
+ * mg_bintp_mom (src/mg_bintp_mom.F90:2-172), 1 mg_bintp_tur
+ * (src/mg_bintp_tur.F90:2-182). */
+int desalm_bintp(desalm_ctx* ctx, int which);
+
 /* Read a device field of level n back in the reference layout.
  * which: 0 q(5), 1 qn(5), 2 qnm1(5), 3 qold_mg(5), 4 rh(4), 5 pk(4),
  * 6 xnut(1), 7 csi(3), 8 eta(3), 9 zet(3), 10 aj(1), 11 dtev(1), 12 wd(1).

@@ -1,0 +1,62 @@
+"""The Fortran drop-in (des-alm_amd/fortran) linked into the reference's own
+code in place of one routine, against the unmodified reference on the same
+inputs.  desalm_shim_mgd replaces mg_driver (device-resident V-cycles),
+desalm_shim_daf replaces solver_daf (the reference's mg_driver calls the GPU
+smoother pass level by level).  Both are built by oracle/ref/build_ref.sh
+next to the reference harness; the tests skip when they are absent.
+
+Bar: fine-level q, qn, qnm1 and xnut within 1e-10 normwise relative of the
+reference (FP64 build) after the given number of physical steps; the
+'conver' file of the drop-in mg_driver holds the reference's lines.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from desalm import cases
+import pyoracle as po
+
+TOL = 1e-10
+pytestmark = pytest.mark.gpu
+
+need_shims = pytest.mark.skipif(
+    not (po.ref_available() and os.path.exists(po.SHIM_MGD_EXE) and os.path.exists(po.SHIM_DAF_EXE)),
+    reason="reference harness and Fortran drop-in executables not built (oracle/ref/build_ref.sh)")
+
+
+def conver_rows(text):
+    rows = [ln.split() for ln in text.strip().splitlines()]
+    return np.array([[float(v) for v in r[1:]] for r in rows]), [int(r[0]) for r in rows]
+
+
+def check_state(a, b):
+    for nm in ("q", "qn", "qnm1"):
+        for m in range(5):
+            assert po.relerr(a[nm][:, m], b[nm][:, m]) < TOL, (nm, m)
+    assert po.relerr(a["xnut"], b["xnut"]) < TOL
+
+
+@need_shims
+@pytest.mark.parametrize("des,bt", [(False, None), (True, (6, 6, 1, 1, 6, 6))])
+def test_shim_mg_driver(gpu, des, bt):
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=des, btype=bt, iters=(1, 2, 3))
+    ref = po.run_reference(c, mode=1, nsteps=2)
+    shim = po.run_reference(c, mode=1, nsteps=2, exe=po.SHIM_MGD_EXE)
+    check_state(shim, ref)
+    rr, ir = conver_rows(ref["conver"])
+    rs, is_ = conver_rows(shim["conver"])
+    assert ir == is_
+    # g13.6 text: six significant digits
+    assert np.allclose(rs, rr, rtol=2e-6, atol=1e-12)
+
+
+@need_shims
+@pytest.mark.parametrize("des", [False, True])
+def test_shim_solver_daf(gpu, des):
+    # the reference is compiled for three levels (src/global.F90:17), so the
+    # drop-in drives three levels too: 17 -> 9 -> 5 nodes
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=des, iters=(1, 2, 3))
+    ref = po.run_reference(c, mode=1, nsteps=1)
+    shim = po.run_reference(c, mode=1, nsteps=1, exe=po.SHIM_DAF_EXE)
+    check_state(shim, ref)

@@ -37,10 +37,25 @@ def test_metrics(gpu):
     assert rel(s.level_field("dtev", 1), o.get("dtev")[level_slice(o, 1, c.params.level_dims())]) < 1e-13
 
 
+# i-lines longer than one LDS chunk (16 rows) of the tiled i-direction line
+# solvers: 33 interior rows leave a one-row last chunk, 43 a partial one, and
+# the line counts are not multiples of a workgroup's line count.
+LONG_I = [(35, 13, 11), (45, 11, 13)]
+
+
 @pytest.mark.parametrize("recalc", [0, 1])
 @pytest.mark.parametrize("bt", [None, (6, 6, 1, 1, 6, 6), (0, 0, 2, 3, 1, 1)])
 def test_solver_daf_fine(gpu, recalc, bt):
-    c = cases.wavy(17, 13, 11, ng=2, btype=bt)
+    check_solver_daf_fine(cases.wavy(17, 13, 11, ng=2, btype=bt), recalc)
+
+
+@pytest.mark.parametrize("recalc", [0, 1])
+@pytest.mark.parametrize("dims", LONG_I)
+def test_solver_daf_fine_long_i(gpu, recalc, dims):
+    check_solver_daf_fine(cases.wavy(*dims, ng=2), recalc)
+
+
+def check_solver_daf_fine(c, recalc):
     o = po.Oracle(c.params).load_case(c)
     s = Solver(c.params).load_case(c)
     n1 = c.n1
@@ -88,7 +103,15 @@ def test_solver_daf_coarse(gpu, calc_pk):
 
 @pytest.mark.parametrize("bt", [None, (6, 6, 1, 1, 6, 6)])
 def test_des_eddy(gpu, bt):
-    c = cases.wavy(17, 13, 11, ng=2, des=True, itk=2, btype=bt)
+    check_des_eddy(cases.wavy(17, 13, 11, ng=2, des=True, itk=2, btype=bt))
+
+
+@pytest.mark.parametrize("dims", LONG_I)
+def test_des_eddy_long_i(gpu, dims):
+    check_des_eddy(cases.wavy(*dims, ng=2, des=True, itk=2))
+
+
+def check_des_eddy(c):
     o = po.Oracle(c.params).load_case(c)
     s = Solver(c.params).load_case(c)
     n1 = c.n1
@@ -124,6 +147,21 @@ def test_mg_driver(gpu, ng, des, bt, itm):
     assert s.pivot_events == 0
 
 
+@pytest.mark.parametrize("dims", LONG_I)
+def test_mg_driver_long_i(gpu, dims):
+    # the prolongation's i-direction smoothing runs on the long lines too
+    c = cases.wavy(*dims, ng=2, itm=2, des=True, iters=(1, 2, 3))
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL
+    assert s.pivot_events == 0
+
+
 def test_cavity_two_steps(gpu):
     c = cases.cavity(17, ng=2, itm=6)
     o = po.Oracle(c.params).load_case(c)
@@ -151,3 +189,32 @@ def test_gtsv_batch_matches_lapack_algorithm(gpu):
         if io == 0:
             xo = np.ravel(xo)
             assert np.max(np.abs(x[s] - xo)) <= 1e-12 * max(1.0, np.max(np.abs(xo)))
+
+
+@pytest.mark.parametrize("which", ["mom", "tur"])
+def test_bintp(gpu, which):
+    c = cases.wavy(15, 13, 11, ng=2, des=True, overset=70, seed=31)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    o.bintp(which)
+    s.bintp(which)
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL
+
+
+@pytest.mark.parametrize("apply_mom", [False, True])
+def test_mg_driver_overset(gpu, apply_mom):
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=True, iters=(1, 2, 3), overset=60, apply_mom=apply_mom)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    for _ in range(2):
+        assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL
+    assert s.pivot_events == 0
