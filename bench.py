@@ -28,7 +28,8 @@ for p in (os.path.join(ROOT, "des-alm_amd", "python"), os.path.join(ROOT, "oracl
 METRIC = "Mcell-iters/s + wall s/physical step to 1e-8 resid, FP64 plain vs mixed+learned"
 HBM_PEAK_GBS = 8000.0
 
-# compulsory FP64 words per node of the level, per launch (reads + writes)
+# compulsory FP64 words per node of the level, per launch (reads + writes),
+# after the per-kernel byte table of the hot-path survey
 WORDS = {
     "dtau": 9 + 1 + 1,                        # metrics, xnut -> dtau
     "residual": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4 + 1,   # q,metrics,aj,xnut,qn,qnm1,dtau -> rh,diss1 (fine)
@@ -36,10 +37,17 @@ WORDS = {
     "daf_i": 4 + 1 + 1 + 3 + 4,               # rh,dtau,aj,csi -> rh
     "daf_j": 4 + 1 + 3 + 3 + 4,               # rh,dtau,csi,eta -> rh
     "daf_k": 4 + 1 + 3 + 3 + 4 + 4,           # rh,dtau,eta,zet,q -> q
+    # des_eddy, per call: itk x (explicit SA rhs + update 22 words, three ADI
+    # sweeps 11 words each) + vorticity source once (13 words); itk applied below
+    "des_eddy": None,
+    "prolong": 33,                            # coarse q, qold, q, dq of the three smoothing sweeps
+    "resid_norm": 20,                         # q, qold -> qold (L1 norms)
 }
 
 
-def level_words(name, n):
+def level_words(name, n, itk=2):
+    if name == "des_eddy":
+        return itk * (22 + 3 * 11) + 13
     w = WORDS[name]
     if name == "residual" and n > 1:
         w += 4 + 4  # pk read and (first pass) written
@@ -57,6 +65,8 @@ def main():
     ap.add_argument("--n", type=int, default=129, help="nodes per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--conv-itm", type=int, default=600,
+                    help="pseudo-iteration cap of the convergence leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -66,25 +76,36 @@ def main():
     from desalm import cases
     from desalm.lib import Solver
 
-    case = cases.channel(args.n, args.n, args.n, ng=3, itm=args.iters, iters=(1, 2, 3))
-    p = case.params
-    n1 = case.n1
-    s = Solver(p, device=0).load_case(case)
-    for _ in range(args.warmup):
-        s.mg_driver()
-    s.profile(True)
-    iters_done = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        itc, hist = s.mg_driver()
-        iters_done += itc
-    t1 = time.perf_counter()
-    stats = s.profile_stats()
-    s.profile(False)
-    wall = t1 - t0
-    value = n1 * iters_done / wall / 1e6
-    q = s.get_state()[0]
-    finite = bool(np.all(np.isfinite(q)))
+    from desalm.params import PREC_FP64, PREC_MIXED
+
+    def timed_run(precision, profile):
+        """warmup + timed physical steps; returns (Mcell-iters/s, wall s, iterations, profile stats,
+        last residual history, state finite, pivot events)"""
+        cs = cases.channel(args.n, args.n, args.n, ng=3, itm=args.iters, iters=(1, 2, 3), precision=precision)
+        sv = Solver(cs.params, device=0).load_case(cs)
+        for _ in range(args.warmup):
+            sv.mg_driver()
+        if profile:
+            sv.profile(True)
+        done = 0
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            itc, hist = sv.mg_driver()
+            done += itc
+        tb = time.perf_counter()
+        st = sv.profile_stats() if profile else None
+        if profile:
+            sv.profile(False)
+        fin = bool(np.all(np.isfinite(sv.get_state()[0])))
+        piv = sv.pivot_events
+        sv.close()
+        return cs.n1 * done / (tb - ta) / 1e6, tb - ta, done, st, hist, fin, piv, cs.params
+
+    value, wall, iters_done, stats, hist, finite, pivots, p = timed_run(PREC_FP64, True)
+    n1 = p.img * p.jmg * p.kmg
+    v_mixed, wall_mixed, _, _, _, fin_mixed, _, _ = timed_run(PREC_MIXED, False)
+    mixed = {"value": round(v_mixed, 3), "unit": "Mcell-iters/s", "ms_per_step": round(wall_mixed / args.steps * 1e3, 3),
+             "state_finite": fin_mixed, "dtype": "f64 residual, f32 DAF sweeps"}
 
     # dominant kernel family by total device time
     fam = {}
@@ -92,15 +113,39 @@ def main():
         fam.setdefault(nm, [0, 0.0])
         fam[nm][0] += cnt
         fam[nm][1] += ms
-    dominant = max((k for k in fam if k in WORDS), key=lambda k: fam[k][1])
+    dominant = max((k for k in fam if k in WORDS and (k, 1) in stats), key=lambda k: fam[k][1])
     dims = p.level_dims()
     cnt1, ms1 = stats[(dominant, 1)]
-    bytes_per_launch = level_words(dominant, 1) * 8 * dims[0][0] * dims[0][1] * dims[0][2]
+    bytes_per_launch = level_words(dominant, 1, p.itk) * 8 * dims[0][0] * dims[0][1] * dims[0][2]
     achieved = bytes_per_launch / (ms1 / cnt1 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": f"{dominant} (fine level)", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None, "bytes_per_launch": bytes_per_launch,
                 "mean_launch_ms": round(ms1 / cnt1, 4)}
+
+    # convergence leg: one physical step from the same initial field with the
+    # reference's test at er_min = -8 (mg_ck_conver), capped at --conv-itm
+    def converge(precision):
+        cc = cases.channel(args.n, args.n, args.n, ng=3, itm=args.conv_itm, iters=(1, 2, 3), it_min=1,
+                           er_min=-8.0, precision=precision)
+        sc = Solver(cc.params, device=0).load_case(cc)
+        tc0 = time.perf_counter()
+        itc_v, hist_v = sc.mg_driver()
+        tc1 = time.perf_counter()
+        sc.close()
+        dmax = (hist_v[:, 1:4] - hist_v[0, 1:4]).max(axis=1)
+        half = itc_v // 2
+        rate = float((dmax[-1] - dmax[half]) / max(itc_v - 1 - half, 1))  # decades per pseudo-iteration
+        reached = bool(dmax[-1] < -8.0 and itc_v < args.conv_itm)
+        return {"reached": reached, "pseudo_iterations": int(itc_v), "wall_s": round(tc1 - tc0, 3),
+                "log10_drop": round(float(dmax[-1]), 3), "rate_decades_per_iter": round(rate, 5),
+                "projected_iters_to_target": (None if reached or rate >= 0 else
+                                              int(itc_v + (-8.0 - dmax[-1]) / rate))}
+
+    conv = None
+    if args.conv_itm > 0:
+        conv = {"target_log10_drop": -8.0, "cap": args.conv_itm, "fp64": converge(PREC_FP64),
+                "mixed": converge(PREC_MIXED)}
 
     cpu = None
     if not args.no_cpu:
@@ -122,9 +167,9 @@ def main():
            "config": {"workload": f"channel {args.n}^3 SA/DES, 3-level FAS V-cycle (iter 1,2,3), "
                                   f"BDF2 dual time, FP64 plain, {args.iters} pseudo-iterations per step",
                       "fine_nodes": n1, "pseudo_iterations": iters_done, "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu,
+           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "converge": conv,
            "kernels_ms": {f"{nm}@L{n}": [cnt, round(ms, 3)] for (nm, n), (cnt, ms) in sorted(stats.items())},
-           "state_finite": finite, "pivot_events": s.pivot_events,
+           "state_finite": finite, "pivot_events": pivots,
            "final_log10_update_norms": [round(float(v), 4) for v in hist[-1]]}
     print(json.dumps(out))
 

@@ -117,7 +117,10 @@ int validate(const desalm_params* p, std::string& why) {
         return -1;
     }
     if (p->itk < 0 || p->itm < 0) { why = "itk and itm must be non-negative"; return -1; }
-    if (p->precision != DESALM_PREC_FP64) { why = "only DESALM_PREC_FP64 is implemented"; return -1; }
+    if (p->precision != DESALM_PREC_FP64 && p->precision != DESALM_PREC_MIXED) {
+        why = "precision must be DESALM_PREC_FP64 or DESALM_PREC_MIXED";
+        return -1;
+    }
     return 0;
 }
 
@@ -262,6 +265,7 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     C.delti = p->delti;
     C.e_source = p->unsteady ? 1.0 : 0.0;
     C.unsteady = p->unsteady ? 1 : 0;
+    C.mixed = p->precision == DESALM_PREC_MIXED ? 1 : 0;
     C.cfl1 = p->cfl1; C.vnn1 = p->vnn1; C.cfl2 = p->cfl2; C.vnn2 = p->vnn2;
     C.pdiss = p->pdiss_coef;
     for (int a = 0; a < 2; a++)
@@ -303,6 +307,13 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     F.dtj = A(c->n1);
     for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
     F.wo = A(c->n1); F.rh5 = A(c->n1); F.pj = A(c->n1);
+    for (int m = 0; m < 3; m++) F.fv[m] = A(c->n1);
+    if (c->C.mixed) {
+        // FP32 DAF scratch: 13 floats per fine node, carved from doubles
+        auto Af = [&](long n) { return (float*)A((n + 1) / 2); };
+        F.sdf = Af(c->n1);
+        for (int m = 0; m < 4; m++) { F.dpf[m] = Af(c->n1); F.cpf[m] = Af(c->n1); F.xf[m] = Af(c->n1); }
+    }
     F.resid_part = A(5L * kResidBlocks);
     F.resid = A(5);
     F.pfix = A(1);

@@ -31,6 +31,7 @@ struct Consts {
     double sa[6][5], sb[6][5];
     int btype[6];
     int unsteady;
+    int mixed;  // DESALM_PREC_MIXED: DAF sweeps in FP32 on the *f scratch
 };
 
 // Device pointers.  Fields marked "all" span every level; "fine" span the
@@ -62,6 +63,12 @@ struct Fields {
     double* dtj;      // fine: SA ADI 0.5 dtev J
     double* ucn[3];   // fine: contravariant velocity / J (residual, SA update)
     double* pj;       // fine: p / J
+    double* fv[3];    // fine: pressure-dissipation fv per direction
+    // FP32 DAF scratch of the mixed-precision form (fine size; null in FP64)
+    float* sdf;
+    float* dpf[4];
+    float* cpf[4];
+    float* xf[4];
     double* wo;       // fine: SA vorticity measure
     double* rh5;      // fine: SA right-hand side
     double* resid_part;  // [kResidBlocks * 5] partial L1 sums

@@ -125,24 +125,36 @@ __device__ __forceinline__ double spr_at(const Fields& F, const Consts& C, long 
     return modal_theta(C, F.dtau[g]) * sqrt(gg);
 }
 
+// Scratch of one DAF pass in storage type S (level-local indices): S =
+// double uses the FP64 scratch and leaves each sweep's solution in the rh
+// field of the level; S = float is the mixed-precision form, whose sweeps
+// run in FP32 on FP32 scratch.
+template <typename S>
+struct DafBuf {
+    S* sd;    // dtau * spr of the sweep direction
+    S* d[4];  // sweep right-hand sides, then d'
+    S* c[4];  // c'
+    S* x[4];  // sweep solution
+};
+
 // ---- pointwise transform before a sweep (rhs_diag_solver.F90:56-122,
 // :300-311, :476-487) ----
 // For every node of the level: sd = dtau * spr_D, the only per-node input
 // the tridiagonal rows need.  For interior nodes: the right-hand side of the
-// sweep, written to F.dp:
+// sweep, written to B.d:
 //   D = 0: Ma^-1 S (-dtau J rh),  D = 1: N1^-1 rh,  D = 2: N2^-1 rh.
 // The thread at the first (last) interior node of a line also builds the
 // semi-explicit boundary vector brh of that line side (fine level only)
 // from the untransformed rh, into F.brh[line * 8 + side * 4 + m].
-template <int D>
-__global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, int use_b1, int use_b2,
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2,
                                                  int b2_full) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
     const long g = L.off + t;
-    F.sd[t] = F.dtau[g] * spr_at<D>(F, C, g);
+    B.sd[t] = (S)(F.dtau[g] * spr_at<D>(F, C, g));
     if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
     const long sI = 1, sJ = ni, sK = (long)ni * nj;
     const double alpha_mult = C.e_source * 1.5 / C.delti;
@@ -157,18 +169,20 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, in
 #pragma unroll
         for (int m = 1; m < 4; m++) v[m] = 1.0 / alpha * r[m];
     };
+    // input of sweeps j and k: the previous sweep's solution
+    auto xin = [&](int m, long tt) { return (double)B.x[m][tt]; };
     double v[4];
     if (D == 0) {
         scaled(g, v);
         apply_mai(F, C, g, v);
     } else {
 #pragma unroll
-        for (int m = 0; m < 4; m++) v[m] = F.rh[m][g];
+        for (int m = 0; m < 4; m++) v[m] = xin(m, t);
         if (D == 1) apply_n1i(F, C, g, v);
         else apply_n2i(F, C, g, v);
     }
 #pragma unroll
-    for (int m = 0; m < 4; m++) F.dp[m][t] = v[m];
+    for (int m = 0; m < 4; m++) B.d[m][t] = (S)v[m];
     if (!use_b1 && !use_b2) return;
     const int p = D == 0 ? i : D == 1 ? j : k;
     const int nd = D == 0 ? ni : D == 1 ? nj : nk;
@@ -190,7 +204,7 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, in
             brh[4] = ovf; brh[5] = 0.0; brh[6] = 0.0; brh[7] = 0.0;
         } else {
 #pragma unroll
-            for (int m = 0; m < 4; m++) { r1[m] = F.rh[m][g]; r2[m] = F.rh[m][g + sD]; }
+            for (int m = 0; m < 4; m++) { r1[m] = xin(m, t); r2[m] = xin(m, t + sD); }
         }
 #pragma unroll
         for (int m = 0; m < 4; m++) w[m] = C.sa[b0][m] * r1[m] + C.sb[b0][m] * r2[m];
@@ -202,7 +216,7 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, in
         if (D == 2 && !b2_full && use_b2) {
             // brh(:,2) is not rebuilt when btype(6) == 6 (:467) and keeps the
             // fifth product of the k = 1 build (:462), which reads rh(1) of node i+1
-            double o[4] = {C.sa[4][4] * F.rh[0][g + sI] + C.sb[4][4] * F.rh[0][g + sI + sD], 0.0, 0.0, 0.0};
+            double o[4] = {C.sa[4][4] * xin(0, t + sI) + C.sb[4][4] * xin(0, t + sI + sD), 0.0, 0.0, 0.0};
             apply_n2i(F, C, g + (long)(nd - 2) * sD, o);
 #pragma unroll
             for (int m = 0; m < 4; m++) brh[4 + m] = o[m];
@@ -211,7 +225,7 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, in
     if (p == nd - 2 && use_b2 && (D == 1 || (D == 2 && b2_full))) {
         double w[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) w[m] = C.sa[b1][m] * F.rh[m][g] + C.sb[b1][m] * F.rh[m][g - sD];
+        for (int m = 0; m < 4; m++) w[m] = C.sa[b1][m] * xin(m, t) + C.sb[b1][m] * xin(m, t - sD);
         if (D == 1) apply_n1i(F, C, g + sD, w);
         else apply_n2i(F, C, g + sD, w);
 #pragma unroll
@@ -225,16 +239,24 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, in
 // rebuilt from sd = dtau spr of the node and its two neighbours:
 //   tmp = sd_s ep h^2, dds = dtc sd_(s-+1) h/2,
 //   a = -lambda dds_w - tmp, b = 1 + 2 tmp, c = lambda dds_e - tmp.
-// Thomas elimination: c' in F.cp[m], d' over the input in F.dp[m], the
-// solution into rh.  Loads run one group of four rows ahead of the
+// Thomas elimination in type S: c' in B.c[m], d' over the input in B.d[m],
+// the solution into B.x[m].  Loads run one group of four rows ahead of the
 // elimination.
 constexpr int kPF = 4;
 
-template <int D>
-__global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, int use_b1, int use_b2) {
+// Lines per 64-wide wave of the j and k sweeps (see k_sa_adi).
+#ifndef DESALM_DAF_LPW
+#define DESALM_DAF_LPW 64
+#endif
+constexpr int kDafLpw = DESALM_DAF_LPW;
+
+template <int D, typename S>
+__global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int m = blockIdx.y;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)threadIdx.x % 64;
+    if (lane >= kDafLpw) return;
+    const long t = ((long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * kDafLpw + lane;
     long nlines, base, sD;
     int len;
     if (D == 0) {
@@ -256,58 +278,58 @@ __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, in
         sD = (long)ni * nj;
         len = nk - 2;
     }
-    const double lam = m == 2 ? 1.0 : m == 3 ? -1.0 : 0.0;
-    const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
-    const double h2 = 0.5 * h, hsq = h * h, epd = C.ep[0][D];
-    const double* __restrict__ sd = F.sd;
-    double* __restrict__ dpm = F.dp[m];
-    double* __restrict__ cpm = F.cp[m];
-    double* __restrict__ rhm = F.rh[m] + L.off;
-    const double brh1 = use_b1 ? F.brh[t * 8 + m] : 0.0;
-    const double brh2 = use_b2 ? F.brh[t * 8 + 4 + m] : 0.0;
+    const S lam = m == 2 ? S(1) : m == 3 ? S(-1) : S(0);
+    const S h = (S)(D == 0 ? L.dc : D == 1 ? L.de : L.dz);
+    const S h2 = S(0.5) * h, hsq = h * h, epd = (S)C.ep[0][D];
+    const S* __restrict__ sd = B.sd;
+    S* __restrict__ dpm = B.d[m];
+    S* __restrict__ cpm = B.c[m];
+    S* __restrict__ rhm = B.x[m];
+    const S brh1 = use_b1 ? (S)F.brh[t * 8 + m] : S(0);
+    const S brh2 = use_b2 ? (S)F.brh[t * 8 + 4 + m] : S(0);
 
-    double sdm = sd[base], sd0 = sd[base + sD];
-    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
+    S sdm = sd[base], sd0 = sd[base + sD];
+    S cprev = 0, dprev = 0, denprev = 1;
     unsigned piv = 0;
-    double dcur[kPF], scur[kPF];
+    S dcur[kPF], scur[kPF];
 #pragma unroll
     for (int u = 0; u < kPF; u++) {
         const long p = base + (1 + u) * sD;
-        dcur[u] = u < len ? dpm[p] : 0.0;
-        scur[u] = u < len ? sd[p + sD] : 0.0;
+        dcur[u] = u < len ? dpm[p] : S(0);
+        scur[u] = u < len ? sd[p + sD] : S(0);
     }
     for (int s0 = 0; s0 < len; s0 += kPF) {
-        double dnx[kPF], snx[kPF];
+        S dnx[kPF], snx[kPF];
 #pragma unroll
         for (int u = 0; u < kPF; u++) {
             const int s = s0 + kPF + u;
             const long p = base + (1 + s) * sD;
-            dnx[u] = s < len ? dpm[p] : 0.0;
-            snx[u] = s < len ? sd[p + sD] : 0.0;
+            dnx[u] = s < len ? dpm[p] : S(0);
+            snx[u] = s < len ? sd[p + sD] : S(0);
         }
 #pragma unroll
         for (int u = 0; u < kPF; u++) {
             const int s = s0 + u;
             if (s < len) {
                 const long p = base + (1 + s) * sD;
-                const double sdp = scur[u];
-                const double tmp = sd0 * epd * hsq;
-                const double dtc = (sd0 == 0.0) ? 0.0 : 1.0;
-                const double ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
-                const double a = -ddw * lam - tmp;
-                const double c = dde * lam - tmp;
-                const double b = 1.0 + 2.0 * tmp;
-                double d = dcur[u];
+                const S sdp = scur[u];
+                const S tmp = sd0 * epd * hsq;
+                const S dtc = (sd0 == S(0)) ? S(0) : S(1);
+                const S ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
+                const S a = -ddw * lam - tmp;
+                const S c = dde * lam - tmp;
+                const S b = S(1) + S(2) * tmp;
+                S d = dcur[u];
                 if (s == 0 && use_b1) d = d - a * brh1;
                 if (s == len - 1 && use_b2) d = d - c * brh2;
-                double den;
+                S den;
                 if (s == 0) {
                     den = b;
                 } else {
                     if (fabs(denprev) < fabs(a)) piv++;  // SGTSV would interchange rows here
                     den = b - a * cprev;
                 }
-                const double inv = 1.0 / den;
+                const S inv = S(1) / den;
                 cprev = c * inv;
                 dprev = (s == 0) ? d * inv : (d - a * dprev) * inv;
                 denprev = den;
@@ -321,16 +343,16 @@ __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, in
         for (int u = 0; u < kPF; u++) { dcur[u] = dnx[u]; scur[u] = snx[u]; }
     }
     // back substitution, loads one group ahead
-    double x = dprev;
+    S x = dprev;
     rhm[base + (long)len * sD] = x;
     for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
-        double dd[kPF], cc[kPF];
+        S dd[kPF], cc[kPF];
 #pragma unroll
         for (int u = 0; u < kPF; u++) {
             const int s = s0 - u;
             const long p = base + (1 + s) * sD;
-            dd[u] = s >= 0 ? dpm[p] : 0.0;
-            cc[u] = s >= 0 ? cpm[p] : 0.0;
+            dd[u] = s >= 0 ? dpm[p] : S(0);
+            cc[u] = s >= 0 ? cpm[p] : S(0);
         }
 #pragma unroll
         for (int u = 0; u < kPF; u++) {
@@ -350,15 +372,17 @@ __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, in
 // elimination; c' and d' go back to HBM per chunk except for the last chunk,
 // which the back substitution starts from in LDS.  Same arithmetic as
 // k_daf_tri<0>.
-__global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts C, int use_b1, int use_b2) {
+template <typename S>
+__global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1,
+                                                       int use_b2) {
     constexpr int NT = 4 * kTL;
     constexpr int kSS = kTR + 3;  // LDS stride of the sd tile (kTR + 2 rows, odd)
-    using IOr = TileIO<NT, kTL, kTR, kTS>;
-    using IOs = TileIO<NT, kTL, kTR + 2, kSS>;
+    using IOr = TileIO<NT, kTL, kTR, kTS, S>;
+    using IOs = TileIO<NT, kTL, kTR + 2, kSS, S>;
     __shared__ long s_base[kTL];
-    __shared__ double s_sd[kTL * kSS];
-    __shared__ double s_d[4][kTL * kTS];
-    __shared__ double s_c[4][kTL * kTS];
+    __shared__ S s_sd[kTL * kSS];
+    __shared__ S s_d[4][kTL * kTS];
+    __shared__ S s_c[4][kTL * kTS];
 
     const long nlines = (long)(L.nj - 2) * (L.nk - 2);
     const long t0 = (long)blockIdx.x * kTL;
@@ -371,22 +395,22 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
 
     const int len = L.ni - 2;
     const int nch = (len + kTR - 1) / kTR;
-    const double lam = m == 2 ? 1.0 : m == 3 ? -1.0 : 0.0;
-    const double h = L.dc;
-    const double h2 = 0.5 * h, hsq = h * h, epd = C.ep[0][0];
-    const double* __restrict__ sd = F.sd;
-    const double brh1 = (use_b1 && act) ? F.brh[t * 8 + m] : 0.0;
-    const double brh2 = (use_b2 && act) ? F.brh[t * 8 + 4 + m] : 0.0;
-    double* myd = s_d[m] + l * kTS;
-    double* myc = s_c[m] + l * kTS;
-    const double* mys = s_sd + l * kSS;
+    const S lam = m == 2 ? S(1) : m == 3 ? S(-1) : S(0);
+    const S h = (S)L.dc;
+    const S h2 = S(0.5) * h, hsq = h * h, epd = (S)C.ep[0][0];
+    const S* __restrict__ sd = B.sd;
+    const S brh1 = (use_b1 && act) ? (S)F.brh[t * 8 + m] : S(0);
+    const S brh2 = (use_b2 && act) ? (S)F.brh[t * 8 + 4 + m] : S(0);
+    S* myd = s_d[m] + l * kTS;
+    S* myc = s_c[m] + l * kTS;
+    const S* mys = s_sd + l * kSS;
 
     // row s of a line is node base + 1 + s; sd of rows s-1..s+1 are nodes base + s .. base + s + 2
-    double vd[4][IOr::kPer], vs[IOs::kPer];
-    for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], F.dp[mm] + 1, s_base, nl, 0, len);
+    S vd[4][IOr::kPer], vs[IOs::kPer];
+    for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], B.d[mm] + 1, s_base, nl, 0, len);
     IOs::load(vs, sd, s_base, nl, 0, len + 2);
 
-    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
+    S cprev = 0, dprev = 0, denprev = 1;
     unsigned piv = 0;
     for (int ch = 0; ch < nch; ch++) {
         const int s0 = ch * kTR;
@@ -395,31 +419,31 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
         IOs::to_lds(vs, s_sd);
         __syncthreads();
         if (ch + 1 < nch) {
-            for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], F.dp[mm] + 1, s_base, nl, s0 + kTR, len);
+            for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], B.d[mm] + 1, s_base, nl, s0 + kTR, len);
             IOs::load(vs, sd, s_base, nl, s0 + kTR, len + 2);
         }
         if (act) {
             for (int u = 0; u < kTR; u++) {
                 const int s = s0 + u;
                 if (s >= len) break;
-                const double sdm = mys[u], sd0 = mys[u + 1], sdp = mys[u + 2];
-                const double tmp = sd0 * epd * hsq;
-                const double dtc = (sd0 == 0.0) ? 0.0 : 1.0;
-                const double ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
-                const double a = -ddw * lam - tmp;
-                const double c = dde * lam - tmp;
-                const double b = 1.0 + 2.0 * tmp;
-                double d = myd[u];
+                const S sdm = mys[u], sd0 = mys[u + 1], sdp = mys[u + 2];
+                const S tmp = sd0 * epd * hsq;
+                const S dtc = (sd0 == S(0)) ? S(0) : S(1);
+                const S ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
+                const S a = -ddw * lam - tmp;
+                const S c = dde * lam - tmp;
+                const S b = S(1) + S(2) * tmp;
+                S d = myd[u];
                 if (s == 0 && use_b1) d = d - a * brh1;
                 if (s == len - 1 && use_b2) d = d - c * brh2;
-                double den;
+                S den;
                 if (s == 0) {
                     den = b;
                 } else {
                     if (fabs(denprev) < fabs(a)) piv++;  // SGTSV would interchange rows here
                     den = b - a * cprev;
                 }
-                const double inv = 1.0 / den;
+                const S inv = S(1) / den;
                 cprev = c * inv;
                 dprev = (s == 0) ? d * inv : (d - a * dprev) * inv;
                 denprev = den;
@@ -430,14 +454,14 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
         __syncthreads();
         if (ch + 1 < nch) {
             for (int mm = 0; mm < 4; mm++) {
-                IOr::store(s_c[mm], F.cp[mm] + 1, s_base, nl, s0, len);
-                IOr::store(s_d[mm], F.dp[mm] + 1, s_base, nl, s0, len);
+                IOr::store(s_c[mm], B.c[mm] + 1, s_base, nl, s0, len);
+                IOr::store(s_d[mm], B.d[mm] + 1, s_base, nl, s0, len);
             }
         }
     }
     // back substitution, last chunk first (already in LDS)
-    double vc[4][IOr::kPer];
-    double x = 0.0;
+    S vc[4][IOr::kPer];
+    S x = 0;
     for (int ch = nch - 1; ch >= 0; ch--) {
         const int s0 = ch * kTR;
         if (ch < nch - 1) {
@@ -450,8 +474,8 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
         }
         if (ch > 0) {
             for (int mm = 0; mm < 4; mm++) {
-                IOr::load(vc[mm], F.cp[mm] + 1, s_base, nl, s0 - kTR, len);
-                IOr::load(vd[mm], F.dp[mm] + 1, s_base, nl, s0 - kTR, len);
+                IOr::load(vc[mm], B.c[mm] + 1, s_base, nl, s0 - kTR, len);
+                IOr::load(vd[mm], B.d[mm] + 1, s_base, nl, s0 - kTR, len);
             }
         }
         if (act) {
@@ -463,14 +487,15 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
             }
         }
         __syncthreads();
-        for (int mm = 0; mm < 4; mm++) IOr::store(s_d[mm], F.rh[mm] + L.off + 1, s_base, nl, s0, len);
+        for (int mm = 0; mm < 4; mm++) IOr::store(s_d[mm], B.x[mm] + 1, s_base, nl, s0, len);
     }
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
 // dQ = Mc dQ*** and q = q + dQ on interior nodes (rhs_diag_solver.F90:624-633,
 // solver_daf.F90:409-418)
-__global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, double* dq_out) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, DafBuf<S> B, double* dq_out) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
@@ -479,7 +504,7 @@ __global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, d
     const long g = L.off + t;
     double v[4];
 #pragma unroll
-    for (int m = 0; m < 4; m++) v[m] = F.rh[m][g];
+    for (int m = 0; m < 4; m++) v[m] = (double)B.x[m][t];
     apply_mc(F, C, g, v);
 #pragma unroll
     for (int m = 0; m < 4; m++) F.q[m][g] = F.q[m][g] + v[m];
@@ -554,34 +579,66 @@ __global__ void k_gtsv_batch(int n, int nsys, double* dl, double* d, double* du,
 
 }  // namespace
 
-static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int D) {
+template <typename S>
+static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const DafBuf<S>& B, int n,
+                      int D) {
     const int use_b1 = (n == 1 && C.btype[2 * D] != 5);
     const int use_b2 = (n == 1 && C.btype[2 * D + 1] != 5);
     const int b2_full = (n == 1 && C.btype[5] != 6);
     const unsigned gp = (unsigned)((L.n + 255) / 256);
     const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
                                                                     : (long)(L.ni - 2) * (L.nj - 2);
-    const dim3 gt((unsigned)((nl + 127) / 128), 4);
+    const long lpb = (long)kDafLpw * 2;  // lines per 128-thread workgroup
+    const dim3 gt((unsigned)((nl + lpb - 1) / lpb), 4);
     if (D == 0) {
-        hipLaunchKernelGGL(k_daf_pre<0>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
         const unsigned gi = (unsigned)((nl + kTL - 1) / kTL);
-        hipLaunchKernelGGL(k_daf_tri_i, dim3(gi), dim3(4 * kTL), 0, s, L, F, C, use_b1, use_b2);
+        hipLaunchKernelGGL(k_daf_tri_i<S>, dim3(gi), dim3(4 * kTL), 0, s, L, F, C, B, use_b1, use_b2);
     } else if (D == 1) {
-        hipLaunchKernelGGL(k_daf_pre<1>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
-        hipLaunchKernelGGL(k_daf_tri<1>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);
+        hipLaunchKernelGGL((k_daf_pre<1, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL((k_daf_tri<1, S>), gt, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
     } else {
-        hipLaunchKernelGGL(k_daf_pre<2>, dim3(gp), dim3(256), 0, s, L, F, C, use_b1, use_b2, b2_full);
-        hipLaunchKernelGGL(k_daf_tri<2>, gt, dim3(128), 0, s, L, F, C, use_b1, use_b2);
+        hipLaunchKernelGGL((k_daf_pre<2, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL((k_daf_tri<2, S>), gt, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
     }
 }
 
-void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) { daf_sweep(s, L, F, C, n, 0); }
+static DafBuf<double> buf64(const Level& L, const Fields& F) {
+    DafBuf<double> B;
+    B.sd = F.sd;
+    for (int m = 0; m < 4; m++) {
+        B.d[m] = F.dp[m];
+        B.c[m] = F.cp[m];
+        B.x[m] = F.rh[m] + L.off;
+    }
+    return B;
+}
 
-void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) { daf_sweep(s, L, F, C, n, 1); }
+static DafBuf<float> buf32(const Fields& F) {
+    DafBuf<float> B;
+    B.sd = F.sdf;
+    for (int m = 0; m < 4; m++) {
+        B.d[m] = F.dpf[m];
+        B.c[m] = F.cpf[m];
+        B.x[m] = F.xf[m];
+    }
+    return B;
+}
+
+static void daf_dir(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int D) {
+    if (C.mixed) daf_sweep(s, L, F, C, buf32(F), n, D);
+    else daf_sweep(s, L, F, C, buf64(L, F), n, D);
+}
+
+void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) { daf_dir(s, L, F, C, n, 0); }
+
+void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) { daf_dir(s, L, F, C, n, 1); }
 
 void launch_daf_k(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, double* dq_out) {
-    daf_sweep(s, L, F, C, n, 2);
-    hipLaunchKernelGGL(k_daf_post, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L, F, C, dq_out);
+    daf_dir(s, L, F, C, n, 2);
+    const dim3 g((unsigned)((L.n + 255) / 256));
+    if (C.mixed) hipLaunchKernelGGL(k_daf_post<float>, g, dim3(256), 0, s, L, F, C, buf32(F), dq_out);
+    else hipLaunchKernelGGL(k_daf_post<double>, g, dim3(256), 0, s, L, F, C, buf64(L, F), dq_out);
 }
 
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du, double* b, int* info) {

@@ -26,7 +26,10 @@ __device__ __forceinline__ double sqr(double a) { return a * a; }
 // Per-node inputs of the residual, every node of the level:
 // rhs_daf_dtau (src/rhs_daf_dtau.F90:85-117) when with_dtau, and always
 // U/J (rhs_contra_j.F90:14-29) into F.ucn and p/J
-// (rhs_flux_sans_convec.F90:265-271) into F.pj (level-local).
+// (rhs_flux_sans_convec.F90:265-271) into F.pj (level-local).  With dtau
+// also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
+// nodes interior in that direction, into F.fv (level-local): the residual
+// then reads it at a node and its two neighbours.
 __global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
@@ -58,6 +61,21 @@ __global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
     d = fmin(d, dtv2);
     d = fmin(d, dtv3);
     F.dtau[g] = d;
+    // fv = |grad xi_D|^2 dtau / J * h^2 (p(+1) - 2 p + p(-1))
+    const int ni = L.ni, nj = L.nj;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    const double* p = F.q[0];
+    const double gg[3] = {g1, g2, g3};
+    const double hh[3] = {L.dc, L.de, L.dz};
+    const int pos[3] = {i, j, k}, nd[3] = {ni, nj, L.nk};
+    const long st[3] = {1, (long)ni, (long)ni * nj};
+#pragma unroll
+    for (int D = 0; D < 3; D++) {
+        if (pos[D] < 1 || pos[D] > nd[D] - 2) continue;
+        const double tmp1 = gg[D] * d / aj;
+        const double tmp2 = (hh[D] * hh[D]) * (p[g + st[D]] - 2.0 * p[g] + p[g - st[D]]);
+        F.fv[D][t] = tmp1 * tmp2;
+    }
 }
 
 // One viscous face flux (rhs_viscous.F90:112-163 for D=0, :189-243 for D=1,
@@ -113,22 +131,6 @@ __device__ __forceinline__ void visc_face(const Level& L, const Fields& F, const
     for (int r = 0; r < 3; r++) {
         fv[r] = vtj * (f1 * uu[r][0] + f2 * uu[r][1] + f3 * uu[r][2] + cca * dcs[r] + ccb * det[r] + ccc * dze[r]);
     }
-}
-
-// fv of rhs_diss_p for direction D at node t (rhs_diss_p.F90:58-117)
-template <int D>
-__device__ __forceinline__ double diss_fv(const Level& L, const Fields& F, long t, const Nb& nb) {
-    const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
-    const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
-    const long g = L.off + t;
-    const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
-    const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
-    const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
-    const double gg = m0[g] * m0[g] + m1[g] * m1[g] + m2[g] * m2[g];
-    const double tmp1 = gg * F.dtau[g] / F.aj[g];
-    const double* p = F.q[0];
-    const double tmp2 = (h * h) * (p[g + sD] - 2.0 * p[g] + p[g - sD]);
-    return tmp1 * tmp2;
 }
 
 // contravariant velocity / J, component D, at level-local node t (k_prep)
@@ -219,15 +221,18 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
     // fv (:124-129) replace the out-of-domain neighbour by the node itself
     double diss1;
     if (!recalc) {
-        const double f0i = diss_fv<0>(L, F, t, nb);
-        const double fmi = (i == 1) ? f0i : diss_fv<0>(L, F, t - nb.sI, nb);
-        const double fpi = (i == ni - 2) ? f0i : diss_fv<0>(L, F, t + nb.sI, nb);
-        const double f0j = diss_fv<1>(L, F, t, nb);
-        const double fmj = (j == 1) ? f0j : diss_fv<1>(L, F, t - nb.sJ, nb);
-        const double fpj = (j == nj - 2) ? f0j : diss_fv<1>(L, F, t + nb.sJ, nb);
-        const double f0k = diss_fv<2>(L, F, t, nb);
-        const double fmk = (k == 1) ? f0k : diss_fv<2>(L, F, t - nb.sK, nb);
-        const double fpk = (k == nk - 2) ? f0k : diss_fv<2>(L, F, t + nb.sK, nb);
+        const double* fv0 = F.fv[0];
+        const double* fv1 = F.fv[1];
+        const double* fv2 = F.fv[2];
+        const double f0i = fv0[t];
+        const double fmi = (i == 1) ? f0i : fv0[t - nb.sI];
+        const double fpi = (i == ni - 2) ? f0i : fv0[t + nb.sI];
+        const double f0j = fv1[t];
+        const double fmj = (j == 1) ? f0j : fv1[t - nb.sJ];
+        const double fpj = (j == nj - 2) ? f0j : fv1[t + nb.sJ];
+        const double f0k = fv2[t];
+        const double fmk = (k == 1) ? f0k : fv2[t - nb.sK];
+        const double fpk = (k == nk - 2) ? f0k : fv2[t + nb.sK];
         diss1 = C.pdiss * ((fpi - 2.0 * f0i + fmi) + (fpj - 2.0 * f0j + fmj) + (fpk - 2.0 * f0k + fmk));
         F.diss1[g] = diss1;
     } else {

@@ -8,8 +8,9 @@
 //   k_sa_rhs    rhs_sa_grdnu, rhs_sa_prod, rhs_sa_viscous, rhs_sa_flux,
 //               rhs_sa_unst_visc, rhs_sa_implicit_source, boundary zeroing
 //   k_sa_adi_pre the ADI operators' face coefficients and dtev J / 2
-//   k_sa_adi<D> rhs_sa_adi_solver sweep in direction D (D = 2 also applies
-//               the update of des_eddy.F90:260-272)
+//   k_sa_adi_i  rhs_sa_adi_solver sweep in i (rows staged through LDS)
+//   k_sa_adi<D> the sweeps in j and k (D = 2 also applies the update of
+//               des_eddy.F90:260-272)
 //   k_sa_bc<D>  rhs_sa_bcond faces, k_sa_clamp its nu~ floor and xnut
 #include "desalm_internal.h"
 #include "line_tile.h"
@@ -212,10 +213,20 @@ __global__ void __launch_bounds__(256) k_sa_adi_pre(Level L, Fields F, Consts C)
 //   q5 += (-q5 + qold + rh) / (1 + 1.5 e_source dtev / delti)
 constexpr int kPF = 4;
 
+// Lines per 64-wide wave of the j and k sweeps: the elimination is a chain
+// of dependent rows, so a wave takes as long with 16 lines as with 64, and
+// spreading the lines over more waves keeps more SIMDs busy.
+#ifndef DESALM_SA_LPW
+#define DESALM_SA_LPW 64
+#endif
+constexpr int kSaLpw = DESALM_SA_LPW;
+
 template <int D>
 __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)threadIdx.x % 64;
+    if (lane >= kSaLpw) return;
+    const long t = ((long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * kSaLpw + lane;
     long base, sD;
     int len;
     if (D == 0) {
@@ -336,10 +347,14 @@ __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
 // U/J, dtj and rh are loaded one chunk ahead of the elimination; c' and d'
 // go back to HBM per chunk except for the last one.  Same arithmetic as
 // k_sa_adi<0>.
-constexpr int kSL = 64;
+#ifndef DESALM_SA_SL
+#define DESALM_SA_SL 64
+#endif
+constexpr int kSL = DESALM_SA_SL;
+constexpr int kSNT = 2 * kSL > 64 ? 2 * kSL : 64;  // threads per workgroup
 
-__global__ void __launch_bounds__(2 * kSL) k_sa_adi_i(Level L, Fields F) {
-    constexpr int NT = 2 * kSL;
+__global__ void __launch_bounds__(kSNT) k_sa_adi_i(Level L, Fields F) {
+    constexpr int NT = kSNT;
     using IOr = TileIO<NT, kSL, kTR, kTS>;          // rows s0.. at node base + 1 + s
     using IOa = TileIO<NT, kSL, kTR + 1, kTR + 1>;  // av at nodes base + s0 .. base + s0 + kTR
     using IOu = TileIO<NT, kSL, kTR + 2, kTR + 3>;  // U/J at nodes base + s0 .. base + s0 + kTR + 1
@@ -520,15 +535,16 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     hipLaunchKernelGGL(k_sa_prep, dim3(gn), dim3(b), 0, s, L, F);
     const int bl = 128;
     const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + kSL - 1) / kSL);
-    const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + bl - 1) / bl);
-    const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + bl - 1) / bl);
+    const long lpb = (long)kSaLpw * (bl / 64);  // lines per workgroup
+    const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + lpb - 1) / lpb);
+    const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + lpb - 1) / lpb);
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
     const unsigned fj = (unsigned)(((long)L.ni * L.nk + b - 1) / b);
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
     for (int it = 0; it < itk; it++) {
         hipLaunchKernelGGL(k_sa_rhs, dim3(gn), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi_pre, dim3(gn), dim3(b), 0, s, L, F, C);
-        hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(2 * kSL), 0, s, L, F);
+        hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(kSNT), 0, s, L, F);
         hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<0>, dim3(fi), dim3(b), 0, s, L, F, C);

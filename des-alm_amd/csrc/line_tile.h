@@ -12,7 +12,10 @@
 
 namespace desalm {
 
-constexpr int kTL = 32;       // i-lines per workgroup
+#ifndef DESALM_TL
+#define DESALM_TL 32
+#endif
+constexpr int kTL = DESALM_TL;  // i-lines per workgroup
 constexpr int kTR = 16;       // rows per chunk
 constexpr int kTS = kTR + 1;  // LDS stride of one line's chunk (odd: no bank conflicts)
 
@@ -23,33 +26,31 @@ __device__ __forceinline__ long iline_base(const Level& L, long t) {
 }
 
 // Copies between HBM and an LDS tile of LINES lines x NRW rows (LDS stride
-// STRIDE).  Row r of line l lives at g[lb[l] + r0 + r]; rows with
-// r0 + r >= rmax and lines l >= nl are skipped (loads give 0).  NT threads
-// take the elements row-fastest.
-template <int NT, int LINES, int NRW, int STRIDE>
+// STRIDE) of element type T.  Row r of line l lives at g[lb[l] + r0 + r];
+// rows with r0 + r >= rmax and lines l >= nl are skipped (loads give 0).  NT
+// threads take the elements row-fastest.
+template <int NT, int LINES, int NRW, int STRIDE, typename T = double>
 struct TileIO {
     static constexpr int kN = LINES * NRW;
     static constexpr int kPer = (kN + NT - 1) / NT;
 
-    __device__ static void load(double (&v)[kPer], const double* __restrict__ g, const long* lb, int nl, int r0,
-                                int rmax) {
+    __device__ static void load(T (&v)[kPer], const T* __restrict__ g, const long* lb, int nl, int r0, int rmax) {
 #pragma unroll
         for (int it = 0; it < kPer; it++) {
             const int e = (int)threadIdx.x + NT * it;
             const int l = e / NRW, r = e % NRW;
             const bool ok = e < kN && l < nl && r0 + r < rmax;
-            v[it] = ok ? g[lb[l] + r0 + r] : 0.0;
+            v[it] = ok ? g[lb[l] + r0 + r] : T(0);
         }
     }
-    __device__ static void to_lds(const double (&v)[kPer], double* lds) {
+    __device__ static void to_lds(const T (&v)[kPer], T* lds) {
 #pragma unroll
         for (int it = 0; it < kPer; it++) {
             const int e = (int)threadIdx.x + NT * it;
             if (e < kN) lds[(e / NRW) * STRIDE + e % NRW] = v[it];
         }
     }
-    __device__ static void store(const double* lds, double* __restrict__ g, const long* lb, int nl, int r0,
-                                 int rmax) {
+    __device__ static void store(const T* lds, T* __restrict__ g, const long* lb, int nl, int r0, int rmax) {
 #pragma unroll
         for (int it = 0; it < kPer; it++) {
             const int e = (int)threadIdx.x + NT * it;

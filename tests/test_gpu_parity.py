@@ -10,6 +10,7 @@ import pytest
 
 from desalm import cases
 from desalm.lib import Solver, gtsv_batch
+from desalm.params import PREC_MIXED
 import pyoracle as po
 
 TOL = 1e-10
@@ -218,3 +219,44 @@ def test_mg_driver_overset(gpu, apply_mom):
         assert rel(qg[:, m], qo[:, m]) < TOL, m
     assert rel(xg, xo) < TOL
     assert s.pivot_events == 0
+
+
+# Mixed-precision form: FP32 DAF sweeps as a defect correction of the FP64
+# residual.  The update differs from the FP64 path only by the FP32 rounding
+# of the sweeps, so after a few pseudo-iterations q stays within 1e-6 of the
+# FP64 oracle (normwise relative); the differences must also be non-zero,
+# which shows that the FP32 sweeps ran.
+TOL_MIXED = 1e-6
+
+
+@pytest.mark.parametrize("des", [False, True])
+def test_mg_driver_mixed(gpu, des):
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=des, iters=(1, 2, 3), precision=PREC_MIXED)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    for _ in range(2):
+        assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    errs = [rel(qg[:, m], qo[:, m]) for m in range(4)]
+    assert max(errs) < TOL_MIXED, errs
+    assert max(errs) > 1e-14, errs
+    assert rel(xg, xo) < TOL_MIXED
+
+
+@pytest.mark.parametrize("dims", LONG_I)
+def test_solver_daf_mixed(gpu, dims):
+    c = cases.wavy(*dims, ng=2, precision=PREC_MIXED)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    n1 = c.n1
+    q = o.get("q")[:n1, :4].copy()
+    args = [o.get(nm)[:n1, :4].copy() for nm in ("qn", "qnm1")]
+    args += [o.get(nm)[:n1].copy() for nm in ("csi", "eta", "zet", "aj", "xnut")]
+    pk = np.zeros((n1, 4))
+    rh = np.zeros((n1, 4))
+    s.solver_daf(1, 0, 1, q, *args, pk, rh)
+    o.solver_daf(1, 0, 1)
+    assert rel(q, o.get("q")[:n1, :4]) < TOL_MIXED
+    # rh leaves the pass holding dq, computed through the FP32 sweeps
+    assert rel(rh, o.get("rh")[:n1]) < 1e-5
