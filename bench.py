@@ -118,9 +118,20 @@ def main():
     cnt1, ms1 = stats[(dominant, 1)]
     bytes_per_launch = level_words(dominant, 1, p.itk) * 8 * dims[0][0] * dims[0][1] * dims[0][2]
     achieved = bytes_per_launch / (ms1 / cnt1 * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE,
+    # WRITE_SIZE, scaled by a plain copy of known size), if recorded for
+    # this family and workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        ent = tr.get("families", {}).get(f"{dominant}@1")
+        if ent and tr.get("workload_n") == args.n:
+            traffic = ent["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "kernel": f"{dominant} (fine level)", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": bytes_per_launch,
+                "traffic": traffic, "bytes_per_launch": bytes_per_launch,
                 "mean_launch_ms": round(ms1 / cnt1, 4)}
 
     # convergence leg: one physical step from the same initial field with the

@@ -61,7 +61,7 @@ struct desalm_ctx {
     Profiler prof;
     // overset donor table (desalm_set_overset); its buffers are freed on
     // replacement and by desalm_finalize
-    Overset ov{0, nullptr, nullptr, nullptr, nullptr};
+    Overset ov{0, nullptr, nullptr, nullptr, nullptr, nullptr};
     int ov_mom = 0;
     double cvin = 0.0;
 };
@@ -427,82 +427,115 @@ int desalm_get_state(desalm_ctx* c, double* q, double* qn, double* qnm1, double*
     return 0;
 }
 
-int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
-    if (!c) return fail(c, "null argument");
-    HIPCHK(c, hipSetDevice(c->device));
-    const desalm_params& p = c->p;
-    const Level& L1 = c->lv[1];
-    hipStream_t s = c->stream;
-    HIPCHK(c, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
-    HIPCHK(c, hipEventRecord(c->ev0, s));
-    // qnm1 <- qn, qn <- q on the fine level (mg_driver.F90:48-51)
-    for (int m = 0; m < 5; m++) {
-        HIPCHK(c, hipMemcpyAsync(c->F.qnm1[m], c->F.qn[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
-        HIPCHK(c, hipMemcpyAsync(c->F.qn[m], c->F.q[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
+// One physical step of FAS V-cycles (mg_driver.F90:2-323) over one or more
+// zones that share a stream: zones[0]'s stream carries every launch.  The
+// zones meet only where the reference's ranks do: the overset exchanges
+// (exchange(1) = mg_bintp_tur after des_eddy, exchange(0) = mg_bintp_mom after
+// the fine pass when apply_mom) and the norm reduction of mg_residual (sums
+// over all zones, divided by zone 1's interior count, mg_residual.F90:36,71-82).
+// Between two exchanges the zones' work is independent, so it runs zone by
+// zone.  The host synchronises once per pseudo-iteration for the norms.
+extern "C++" {
+template <typename Exchange>
+static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom, Exchange exchange,
+                   double* init_resd, int* itc_out, double* resid_out) {
+    desalm_ctx* c0 = zs[0];
+    const desalm_params& p = c0->p;
+    hipStream_t s = c0->stream;
+    const int ng = c0->ng;
+    for (desalm_ctx* c : zs) {
+        const Level& L1 = c->lv[1];
+        HIPCHK(c0, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
+        // qnm1 <- qn, qn <- q on the fine level (mg_driver.F90:48-51)
+        for (int m = 0; m < 5; m++) {
+            HIPCHK(c0, hipMemcpyAsync(c->F.qnm1[m], c->F.qn[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
+            HIPCHK(c0, hipMemcpyAsync(c->F.qn[m], c->F.q[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
+        }
+        launch_inject(s, c->lv, ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.qn, 5);
+        launch_inject(s, c->lv, ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.qnm1, 5);
+        launch_copy_qold(s, L1, c->F);
     }
-    launch_inject(s, c->lv, c->ng, p.icrs, p.jcrs, p.kcrs, c->F.qn, 5);
-    launch_inject(s, c->lv, c->ng, p.icrs, p.jcrs, p.kcrs, c->F.qnm1, 5);
-    launch_copy_qold(s, L1, c->F);
-    HIPCHK(c, hipGetLastError());
-    const double tnu = (double)((long)(L1.ni - 2) * (L1.nj - 2) * (L1.nk - 2));
+    HIPCHK(c0, hipGetLastError());
+    const Level& L0 = c0->lv[1];
+    const double tnu = (double)((long)(L0.ni - 2) * (L0.nj - 2) * (L0.nk - 2));
+    std::vector<double> parts(5 * zs.size());
     int itc = 0;
     int decide_recalc_rh = 0;
     for (int it = 1; it <= p.itm; it++) {
         itc++;
-        for (int n = 1; n <= c->ng; n++) {
+        for (int n = 1; n <= ng; n++) {
             if (n > 1)
-                timed(c, KID_NLEVEL, n, [&] { launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, p.icrs, p.jcrs, p.kcrs); });
+                for (desalm_ctx* c : zs)
+                    timed(c, KID_NLEVEL, n, [&] {
+                        launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, c->p.icrs, c->p.jcrs, c->p.kcrs);
+                    });
             for (int itr = 1; itr <= p.iter[n - 1]; itr++) {
-                if (n != c->ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
+                if (n != ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
                 if (p.des && n == 1) {
-                    timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, L1, c->F, c->C, p.itk); });
-                    // mg_bintp_tur after des_eddy (mg_driver.F90:182)
-                    if (c->ov.n > 0)
-                        timed(c, KID_OVERSET, 1, [&] { launch_bintp(s, L1, c->F, c->C, c->ov, 1, c->cvin, p.des); });
+                    for (desalm_ctx* c : zs)
+                        timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, c->lv[1], c->F, c->C, c->p.itk); });
+                    if (have_table) timed(c0, KID_OVERSET, 1, [&] { exchange(1); });
                 }
-                solver_daf(c, n, decide_recalc_rh, itr, nullptr);
-                // mg_bintp_mom after the fine smoother pass (mg_driver.F90:241-244)
-                if (n == 1 && c->ov.n > 0 && c->ov_mom)
-                    timed(c, KID_OVERSET, 1, [&] { launch_bintp(s, L1, c->F, c->C, c->ov, 0, c->cvin, p.des); });
+                for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr);
+                if (n == 1 && have_table && apply_mom) timed(c0, KID_OVERSET, 1, [&] { exchange(0); });
             }
         }
-        for (int n = c->ng - 1; n >= 1; n--) {
-            const long per_level = 3 * 2 * c->maxdim;
-            timed(c, KID_PROLONG, n, [&] {
-                launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, p.icrs, p.jcrs, p.kcrs,
-                               c->rs_tab + (n - 1) * per_level);
-            });
+        for (desalm_ctx* c : zs) {
+            for (int n = ng - 1; n >= 1; n--) {
+                const long per_level = 3 * 2 * c->maxdim;
+                timed(c, KID_PROLONG, n, [&] {
+                    launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, c->p.icrs, c->p.jcrs, c->p.kcrs,
+                                   c->rs_tab + (n - 1) * per_level);
+                });
+            }
+            timed(c, KID_BCOND, 1, [&] { launch_bcond_fm(s, c->lv[1], c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix); });
+            // update norms, then qold_mg <- q for the next pseudo-iteration
+            timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1); });
         }
-        timed(c, KID_BCOND, 1, [&] { launch_bcond_fm(s, L1, c->F, c->C, p.ifix, p.jfix, p.kfix); });
-        // update norms, then qold_mg <- q for the next pseudo-iteration
-        timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, L1, c->F, 1); });
-        HIPCHK(c, hipGetLastError());
-        double resd[5];
-        HIPCHK(c, hipMemcpyAsync(resd, c->F.resid, sizeof(resd), hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipStreamSynchronize(s));
-        prof_flush(c);
+        HIPCHK(c0, hipGetLastError());
+        for (size_t z = 0; z < zs.size(); z++)
+            HIPCHK(c0, hipMemcpyAsync(&parts[5 * z], zs[z]->F.resid, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(c0, hipStreamSynchronize(s));
+        for (desalm_ctx* c : zs) prof_flush(c);
+        double resd[5] = {0, 0, 0, 0, 0};
+        for (size_t z = 0; z < zs.size(); z++)
+            for (int m = 0; m < 5; m++) resd[m] = resd[m] + parts[5 * z + m];
         double residual[5];
         for (int m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
         if (resid_out)
             for (int m = 0; m < 5; m++) resid_out[5 * (it - 1) + m] = residual[m];
         // mg_ck_conver (mg_driver.F90:372-398)
         if (itc == 1)
-            for (int m = 0; m < 5; m++) c->init_resd[m] = residual[m];
+            for (int m = 0; m < 5; m++) init_resd[m] = residual[m];
         const double eomax = fmax(fmax(residual[1], residual[2]), residual[3]);
-        const double ermax = fmax(fmax(residual[1] - c->init_resd[1], residual[2] - c->init_resd[2]),
-                                  residual[3] - c->init_resd[3]);
+        const double ermax = fmax(fmax(residual[1] - init_resd[1], residual[2] - init_resd[2]),
+                                  residual[3] - init_resd[3]);
         bool converged = false;
         if (itc > p.it_min && ermax < p.er_min) converged = true;
         if (itc > p.it_min && eomax < p.eo_min) converged = true;
         if (converged) break;
     }
+    for (desalm_ctx* c : zs)
+        HIPCHK(c0, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    if (itc_out) *itc_out = itc;
+    return 0;
+}
+}  // extern "C++"
+
+int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    std::vector<desalm_ctx*> zs{c};
+    const Level& L1 = c->lv[1];
+    auto exchange = [&](int tur) { launch_bintp(s, L1, c->F, c->C, c->ov, tur, c->cvin, c->p.des); };
+    if (vcycles(zs, c->ov.n > 0, c->ov_mom, exchange, c->init_resd, itc_out, resid_out)) return -1;
     HIPCHK(c, hipEventRecord(c->ev1, s));
-    HIPCHK(c, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (itc_out) *itc_out = itc;
     return 0;
 }
 
@@ -655,12 +688,197 @@ const char* desalm_profile_name(int kernel) {
 
 const char* desalm_last_error(const desalm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
+// ---- several zones coupled by overset interpolation ----
+struct desalm_group {
+    std::vector<desalm_ctx*> z;  // zones, borrowed; zone 1 first
+    // donor table split as the reference's alltoallv splits it: gather runs
+    // per source zone (send side), scatter per destination zone (receive side)
+    std::vector<Overset> gather, scatter;  // one entry per zone (n = 0 if none)
+    std::vector<void*> dev;
+    double* buf = nullptr;
+    long nd = 0;
+    int mom = 0;
+    double init_resd[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+static void group_free_table(desalm_group* g) {
+    for (void* p : g->dev) (void)hipFree(p);
+    g->dev.clear();
+    g->gather.assign(g->z.size(), Overset{0, nullptr, nullptr, nullptr, nullptr, nullptr});
+    g->scatter.assign(g->z.size(), Overset{0, nullptr, nullptr, nullptr, nullptr, nullptr});
+    g->buf = nullptr;
+    g->nd = 0;
+}
+
+int desalm_group_create(desalm_ctx* const* zones, int nz, desalm_group** out) {
+    if (!zones || nz < 1 || !out) return fail(nullptr, "desalm_group_create: bad arguments");
+    const desalm_ctx* a = zones[0];
+    for (int z = 0; z < nz; z++) {
+        const desalm_ctx* b = zones[z];
+        if (!b) return fail(nullptr, "desalm_group_create: null zone");
+        for (int y = 0; y < z; y++)
+            if (zones[y] == b) return fail(nullptr, "desalm_group_create: a zone is listed twice");
+        bool same = b->device == a->device && b->ng == a->ng && b->p.itm == a->p.itm &&
+                    b->p.it_min == a->p.it_min && b->p.er_min == a->p.er_min && b->p.eo_min == a->p.eo_min &&
+                    b->p.des == a->p.des && b->p.itk == a->p.itk;
+        for (int n = 0; n < a->ng; n++) same = same && b->p.iter[n] == a->p.iter[n];
+        if (!same)
+            return fail(nullptr, "desalm_group_create: zones must share the device, the multigrid levels, iter, "
+                                 "itm, it_min, er_min, eo_min, des and itk (global in the reference)");
+    }
+    desalm_group* g = new desalm_group();
+    g->z.assign(zones, zones + nz);
+    group_free_table(g);
+    if (hipSetDevice(a->device) != hipSuccess || hipEventCreate(&g->ev0) != hipSuccess ||
+        hipEventCreate(&g->ev1) != hipSuccess) {
+        delete g;
+        return fail(nullptr, "desalm_group_create: HIP event creation failed");
+    }
+    *out = g;
+    return 0;
+}
+
+int desalm_group_set_overset(desalm_group* g, int64_t n, const int32_t* hosts, const double* coef,
+                             const int32_t* nodes, int apply_mom) {
+    if (!g) return fail(nullptr, "null argument");
+    desalm_ctx* c0 = g->z[0];
+    if (n < 0 || (n > 0 && (!hosts || !coef || !nodes))) return fail(c0, "desalm_group_set_overset: bad arguments");
+    HIPCHK(c0, hipSetDevice(c0->device));
+    for (desalm_ctx* c : g->z) HIPCHK(c0, hipStreamSynchronize(c->stream));
+    group_free_table(g);
+    g->mom = apply_mom != 0;
+    if (n == 0) return 0;
+    const int nz = (int)g->z.size();
+    std::vector<std::vector<long>> ghost(nz), gcoef_idx(nz), snode(nz), sslot(nz);
+    std::vector<long> slot_of(n);
+    for (int64_t d = 0; d < n; d++) {
+        const int32_t* h = hosts + 4 * d;
+        const int32_t* t = nodes + 4 * d;
+        if (h[3] < 1 || h[3] > nz || t[3] < 1 || t[3] > nz)
+            return fail(c0, "desalm_group_set_overset: zone of donor " + std::to_string(d) + " out of range");
+        const Level& A = g->z[h[3] - 1]->lv[1];
+        const Level& B = g->z[t[3] - 1]->lv[1];
+        if (h[0] < 1 || h[0] + 1 > A.ni || h[1] < 1 || h[1] + 1 > A.nj || h[2] < 1 || h[2] + 1 > A.nk)
+            return fail(c0, "desalm_group_set_overset: host cell " + std::to_string(d) + " outside its zone");
+        if (t[0] < 1 || t[0] > B.ni || t[1] < 1 || t[1] > B.nj || t[2] < 1 || t[2] > B.nk)
+            return fail(c0, "desalm_group_set_overset: fringe node " + std::to_string(d) + " outside its zone");
+        // gather slots are numbered in the order of the source zones
+        ghost[h[3] - 1].push_back((h[0] - 1) + (long)A.ni * ((h[1] - 1) + (long)A.nj * (h[2] - 1)));
+        gcoef_idx[h[3] - 1].push_back(d);
+        snode[t[3] - 1].push_back((t[0] - 1) + (long)B.ni * ((t[1] - 1) + (long)B.nj * (t[2] - 1)));
+    }
+    long slot = 0;
+    for (int z = 0; z < nz; z++)
+        for (long d : gcoef_idx[z]) slot_of[d] = slot++;
+    for (int64_t d = 0; d < n; d++) sslot[nodes[4 * d + 3] - 1].push_back(slot_of[d]);
+    for (int z = 0; z < nz; z++) {
+        std::vector<long> sorted(snode[z]);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(c0, "desalm_group_set_overset: a fringe node of zone " + std::to_string(z + 1) +
+                                " is listed twice");
+    }
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return nullptr;
+        g->dev.push_back(p);
+        if (bytes && hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return p;
+    };
+    double* buf = nullptr;
+    if (hipMalloc((void**)&buf, sizeof(double) * 4 * n) != hipSuccess) {
+        group_free_table(g);
+        return fail(c0, "desalm_group_set_overset: out of device memory");
+    }
+    g->dev.push_back(buf);
+    g->buf = buf;
+    long goff = 0;
+    for (int z = 0; z < nz; z++) {
+        const long nzd = (long)ghost[z].size();
+        if (nzd > 0) {
+            std::vector<double> cf(3 * nzd);
+            for (long e = 0; e < nzd; e++)
+                for (int r = 0; r < 3; r++) cf[3 * e + r] = coef[3 * gcoef_idx[z][e] + r];
+            const long* dh = (const long*)up(ghost[z].data(), sizeof(long) * nzd);
+            const double* dc = (const double*)up(cf.data(), sizeof(double) * 3 * nzd);
+            if (!dh || !dc) { group_free_table(g); return fail(c0, "desalm_group_set_overset: upload failed"); }
+            // the gather of zone z fills slots goff .. goff + nzd - 1 (4 doubles per slot)
+            g->gather[z] = Overset{nzd, dh, dc, nullptr, buf + 4 * goff, nullptr};
+            goff += nzd;
+        }
+        const long nsd = (long)snode[z].size();
+        if (nsd > 0) {
+            const long* dn = (const long*)up(snode[z].data(), sizeof(long) * nsd);
+            const long* ds = (const long*)up(sslot[z].data(), sizeof(long) * nsd);
+            if (!dn || !ds) { group_free_table(g); return fail(c0, "desalm_group_set_overset: upload failed"); }
+            g->scatter[z] = Overset{nsd, nullptr, nullptr, dn, buf, ds};
+        }
+    }
+    g->nd = n;
+    return 0;
+}
+
+int desalm_group_mg_driver(desalm_group* g, int* itc_out, double* resid_out) {
+    if (!g) return fail(nullptr, "null argument");
+    desalm_ctx* c0 = g->z[0];
+    HIPCHK(c0, hipSetDevice(c0->device));
+    hipStream_t s = c0->stream;
+    // every zone's work goes on zone 1's stream for the duration of the call
+    std::vector<hipStream_t> own;
+    for (desalm_ctx* c : g->z) {
+        HIPCHK(c0, hipStreamSynchronize(c->stream));
+        own.push_back(c->stream);
+        c->stream = s;
+    }
+    auto exchange = [&](int tur) {
+        // send side: every source zone's donor values into the shared buffer;
+        // the tur pass packs one value per slot, the mom pass four
+        for (size_t z = 0; z < g->z.size(); z++) {
+            Overset o = g->gather[z];
+            if (o.n == 0) continue;
+            if (tur) o.buf = g->buf + (o.buf - g->buf) / 4;
+            launch_bintp_gather(s, g->z[z]->lv[1], g->z[z]->F, o, tur);
+        }
+        // receive side: every destination zone's fringe nodes
+        for (size_t z = 0; z < g->z.size(); z++) {
+            const Overset& o = g->scatter[z];
+            if (o.n == 0) continue;
+            desalm_ctx* c = g->z[z];
+            launch_bintp_scatter(s, c->lv[1], c->F, c->C, o, tur, c->cvin, c->p.des);
+        }
+    };
+    int rc = hipEventRecord(g->ev0, s) == hipSuccess ? 0 : fail(c0, "hipEventRecord failed");
+    if (!rc) rc = vcycles(g->z, g->nd > 0, g->mom, exchange, g->init_resd, itc_out, resid_out);
+    if (!rc && hipEventRecord(g->ev1, s) != hipSuccess) rc = fail(c0, "hipEventRecord failed");
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(c0, "hipStreamSynchronize failed");
+    for (size_t z = 0; z < g->z.size(); z++) g->z[z]->stream = own[z];
+    if (rc) return rc;
+    float ms = 0.0f;
+    HIPCHK(c0, hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    g->last_ms = ms;
+    for (desalm_ctx* c : g->z) c->last_ms = ms;
+    return 0;
+}
+
+int desalm_group_destroy(desalm_group* g) {
+    if (!g) return 0;
+    (void)hipSetDevice(g->z[0]->device);
+    for (desalm_ctx* c : g->z) (void)hipStreamSynchronize(c->stream);
+    group_free_table(g);
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    delete g;
+    return 0;
+}
+
 static void free_overset(desalm_ctx* c) {
     (void)hipFree((void*)c->ov.host);
     (void)hipFree((void*)c->ov.coef);
     (void)hipFree((void*)c->ov.node);
     (void)hipFree(c->ov.buf);
-    c->ov = Overset{0, nullptr, nullptr, nullptr, nullptr};
+    c->ov = Overset{0, nullptr, nullptr, nullptr, nullptr, nullptr};
 }
 
 int desalm_set_overset(desalm_ctx* c, int64_t n, const int32_t* hosts, const double* coef, const int32_t* nodes,
@@ -698,7 +916,7 @@ int desalm_set_overset(desalm_ctx* c, int64_t n, const int32_t* hosts, const dou
         (void)hipFree(dh); (void)hipFree(dn); (void)hipFree(dc); (void)hipFree(db);
         return fail(c, "desalm_set_overset: out of device memory");
     }
-    c->ov = Overset{(long)n, dh, dc, dn, db};
+    c->ov = Overset{(long)n, dh, dc, dn, db, nullptr};
     HIPCHK(c, hipMemcpy(dh, host.data(), sizeof(long) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(dn, node.data(), sizeof(long) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(dc, coef, sizeof(double) * 3 * n, hipMemcpyHostToDevice));

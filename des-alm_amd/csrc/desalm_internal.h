@@ -81,13 +81,15 @@ constexpr int kResidBlocks = 1024;
 
 // Overset donor table of the fine level (device pointers, level-local node
 // indices): donor d reads the cell whose lower corner is host[d] with
-// trilinear weights coef[3d..3d+2] and writes fringe node node[d].
+// trilinear weights coef[3d..3d+2] into buf; fringe node node[d] takes the
+// value in buffer slot slot[d] (slot d when slot is null).
 struct Overset {
     long n;
     const long* host;
     const double* coef;
     const long* node;
-    double* buf;  // 4 * n: gathered donor values
+    double* buf;        // 4 values per slot: gathered donor values
+    const long* slot;
 };
 
 struct Ctx;  // host context, desalm_api.hip
@@ -125,6 +127,9 @@ void launch_soa_to_aos(hipStream_t s, double* const* soa, int ncomp, long n, dou
 // overset interpolation (tur = 0: q(1:4), 1: nu~ and xnut), k_overset.hip
 void launch_bintp(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
                   double cvin, int des);
+void launch_bintp_gather(hipStream_t s, const Level& L, const Fields& F, const Overset& O, int tur);
+void launch_bintp_scatter(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
+                          double cvin, int des);
 // batched GEPP (LAPACK SGTSV semantics), k_daf.hip
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,
                        double* b, int* info);

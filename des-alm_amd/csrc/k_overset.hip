@@ -38,13 +38,15 @@ __global__ void __launch_bounds__(256) k_bintp_gather(Level L, Fields F, Overset
     }
 }
 
-// q(1:4) at the fringe nodes (mg_bintp_mom.F90:141-163)
+// q(1:4) at the fringe nodes (mg_bintp_mom.F90:141-163); the value of
+// fringe node d sits in buffer slot O.slot[d] (slot d without a slot table)
 __global__ void __launch_bounds__(256) k_bintp_scatter_mom(Level L, Fields F, Overset O) {
     const long d = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= O.n) return;
     const long l = L.off + O.node[d];
+    const long b = O.slot ? O.slot[d] : d;
 #pragma unroll
-    for (int m = 0; m < 4; m++) F.q[m][l] = O.buf[4 * d + m];
+    for (int m = 0; m < 4; m++) F.q[m][l] = O.buf[4 * b + m];
 }
 
 // nu~ at the fringe nodes, then its floor and xnut = nu~ fv1
@@ -54,7 +56,7 @@ __global__ void __launch_bounds__(256) k_bintp_scatter_tur(Level L, Fields F, Co
     const long d = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= O.n) return;
     const long l = L.off + O.node[d];
-    double q5 = O.buf[d];
+    double q5 = O.buf[O.slot ? O.slot[d] : d];
     if (des) {
         q5 = fmax(cvin, q5);
         const double chi = q5 / C.rei;
@@ -67,17 +69,24 @@ __global__ void __launch_bounds__(256) k_bintp_scatter_tur(Level L, Fields F, Co
 
 }  // namespace
 
-void launch_bintp(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
-                  double cvin, int des) {
+void launch_bintp_gather(hipStream_t s, const Level& L, const Fields& F, const Overset& O, int tur) {
     if (O.n <= 0) return;
     const unsigned g = (unsigned)((O.n + 255) / 256);
-    if (!tur) {
-        hipLaunchKernelGGL(k_bintp_gather, dim3(g), dim3(256), 0, s, L, F, O, 0, 4);
-        hipLaunchKernelGGL(k_bintp_scatter_mom, dim3(g), dim3(256), 0, s, L, F, O);
-    } else {
-        hipLaunchKernelGGL(k_bintp_gather, dim3(g), dim3(256), 0, s, L, F, O, 4, 1);
-        hipLaunchKernelGGL(k_bintp_scatter_tur, dim3(g), dim3(256), 0, s, L, F, C, O, cvin, des);
-    }
+    hipLaunchKernelGGL(k_bintp_gather, dim3(g), dim3(256), 0, s, L, F, O, tur ? 4 : 0, tur ? 1 : 4);
+}
+
+void launch_bintp_scatter(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
+                          double cvin, int des) {
+    if (O.n <= 0) return;
+    const unsigned g = (unsigned)((O.n + 255) / 256);
+    if (!tur) hipLaunchKernelGGL(k_bintp_scatter_mom, dim3(g), dim3(256), 0, s, L, F, O);
+    else hipLaunchKernelGGL(k_bintp_scatter_tur, dim3(g), dim3(256), 0, s, L, F, C, O, cvin, des);
+}
+
+void launch_bintp(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
+                  double cvin, int des) {
+    launch_bintp_gather(s, L, F, O, tur);
+    launch_bintp_scatter(s, L, F, C, O, tur, cvin, des);
 }
 
 }  // namespace desalm

@@ -66,10 +66,15 @@ def lib():
         i32p = ctypes.POINTER(ctypes.c_int32)
         L.desalm_set_overset.argtypes = [vp, ctypes.c_int64, i32p, dp, i32p, ctypes.c_int]
         L.desalm_bintp.argtypes = [vp, ctypes.c_int]
+        L.desalm_group_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.desalm_group_set_overset.argtypes = [vp, ctypes.c_int64, i32p, dp, i32p, ctypes.c_int]
+        L.desalm_group_mg_driver.argtypes = [vp, ip, dp]
+        L.desalm_group_destroy.argtypes = [vp]
         for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
                   "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize",
-                  "desalm_set_overset", "desalm_bintp"):
+                  "desalm_set_overset", "desalm_bintp", "desalm_group_create", "desalm_group_set_overset",
+                  "desalm_group_mg_driver", "desalm_group_destroy"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -212,6 +217,52 @@ class Solver:
     @property
     def pivot_events(self):
         return lib().desalm_last_pivot_events(self.h)
+
+
+class Group:
+    """Zones (Solver contexts, zone 1 first) coupled by a cross-zone donor
+    table; desalm_group_* of include/desalm.h."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[sv.h.value for sv in self.solvers])
+        self.h = ctypes.c_void_p()
+        rc = lib().desalm_group_create(arr, len(self.solvers), ctypes.byref(self.h))
+        if rc != 0:
+            raise DesalmError(f"desalm_group_create: rc={rc}: " + lib().desalm_last_error(None).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().desalm_group_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_overset(self, table):
+        """table: desalm.overset.ZoneTable (hosts/nodes (n, 4) i, j, k, zone; coef (n, 3))."""
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        h = np.ascontiguousarray(table.hosts, dtype=np.int32)
+        c = np.ascontiguousarray(table.coef, dtype=np.float64)
+        t = np.ascontiguousarray(table.nodes, dtype=np.int32)
+        rc = lib().desalm_group_set_overset(self.h, len(c), h.ctypes.data_as(i32p), _p(c), t.ctypes.data_as(i32p),
+                                            int(table.apply_mom))
+        if rc != 0:
+            raise DesalmError(f"desalm_group_set_overset: rc={rc}: " +
+                              lib().desalm_last_error(self.solvers[0].h).decode())
+
+    def mg_driver(self):
+        itm = self.solvers[0].params.itm
+        itc = ctypes.c_int(0)
+        hist = np.zeros((max(itm, 1), 5))
+        rc = lib().desalm_group_mg_driver(self.h, ctypes.byref(itc), _p(hist))
+        if rc != 0:
+            raise DesalmError(f"desalm_group_mg_driver: rc={rc}: " +
+                              lib().desalm_last_error(self.solvers[0].h).decode())
+        return itc.value, hist[: itc.value]
 
 
 def gtsv_batch(dl, d, du, b):

@@ -183,6 +183,37 @@ int desalm_set_overset(desalm_ctx* ctx, int64_t n, const int32_t* hosts, const d
  * (src/mg_bintp_tur.F90:2-182). */
 int desalm_bintp(desalm_ctx* ctx, int which);
 
+/*
+ * Several zones coupled by overset interpolation: the reference's nzone > 1
+ * runs, in which every zone has its own ranks and the zones meet only in
+ * mg_bintp_tur / mg_bintp_mom (src/mg_driver.F90:182,241-244) and in the norm
+ * reduction of mg_residual (src/mg_residual.F90:71-82).  A group borrows nz
+ * contexts (one per zone, zone 1 first, all on one device, with the same
+ * multigrid levels, iter, itm, it_min, er_min, eo_min, des and itk, which the
+ * reference holds globally) and runs their V-cycles in lockstep on zone 1's
+ * stream.  The contexts stay owned by the caller and keep their state; a
+ * zone's own single-zone table (desalm_set_overset) is not used by the group.
+ */
+typedef struct desalm_group desalm_group;
+int desalm_group_create(desalm_ctx* const* zones, int nz, desalm_group** group);
+
+/* Cross-zone donor table (src/global_osg.F90:18-28): hosts[4d..4d+3] = lower
+ * corner i, j, k and zone (1-based) of donor d's host cell, coef[3d..3d+2]
+ * its trilinear weights, nodes[4d..4d+3] = i, j, k and zone of the fringe
+ * node it fills.  nu~ is interpolated after every des_eddy, q(1:4) after
+ * every fine smoother pass when apply_mom != 0.  All donor values are formed
+ * before any fringe node is written; fringe nodes must be distinct within a
+ * zone.  n = 0 removes the table. */
+int desalm_group_set_overset(desalm_group* group, int64_t n, const int32_t* hosts, const double* coef,
+                             const int32_t* nodes, int apply_mom);
+
+/* One physical step of every zone (src/mg_driver.F90:2-323 on all ranks):
+ * itc_out and resid_out as for desalm_mg_driver, with the norms summed over
+ * all zones and divided by zone 1's interior node count, as the reference's
+ * root rank does (src/mg_residual.F90:36,71-82). */
+int desalm_group_mg_driver(desalm_group* group, int* itc_out, double* resid_out);
+int desalm_group_destroy(desalm_group* group);
+
 /* Read a device field of level n back in the reference layout.
  * which: 0 q(5), 1 qn(5), 2 qnm1(5), 3 qold_mg(5), 4 rh(4), 5 pk(4),
  * 6 xnut(1), 7 csi(3), 8 eta(3), 9 zet(3), 10 aj(1), 11 dtev(1), 12 wd(1).

@@ -1970,22 +1970,29 @@ int or_mg_prolong(or_state* o, int n) {
 
 /* mg_residual (src/mg_residual.F90:2-104): log10 of the L1 mean of the
  * pseudo-time update over fine interior nodes, loop order k, j, i */
-int or_mg_residual(const or_state* o, double* residual) {
+/* the local sums resd_tmp of mg_residual (src/mg_residual.F90:57-67) */
+static void mg_residual_sums(const or_state* o, double* resd) {
     LEVEL(1);
-    double resd[5] = {0, 0, 0, 0, 0};
-    double tnu = (double)((long)(ni - 2) * (nj - 2) * (nk - 2));
     int i, j, k, m;
+    for (m = 0; m < 5; m++) resd[m] = 0.0;
     for (k = 2; k <= nk - 1; k++)
         for (j = 2; j <= nj - 1; j++)
             for (i = 2; i <= ni - 1; i++) {
                 long l = off + IX(i, j, k);
                 for (m = 0; m < 5; m++) resd[m] = resd[m] + fabs(o->q[5 * l + m] - o->qold_mg[5 * l + m]);
             }
+}
+
+int or_mg_residual(const or_state* o, double* residual) {
+    LEVEL(1);
+    double resd[5];
+    double tnu = (double)((long)(ni - 2) * (nj - 2) * (nk - 2));
+    int m;
+    mg_residual_sums(o, resd);
     for (m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
     return 0;
 }
 
-/* mg_driver (src/mg_driver.F90:2-323) with mg_ck_conver (:335-405) */
 /* ------------------------------------------------------------------ */
 /* Overset boundary interpolation (src/mg_bintp_mom.F90:2-172,          */
 /* src/mg_bintp_tur.F90:2-182) for one zone on one rank: every donor     */
@@ -2015,11 +2022,11 @@ int or_set_overset(or_state* o, long n, const int* hosts, const double* coef, co
     return 0;
 }
 
-/* trilinear value of component m of q at donor d (mg_bintp_mom.F90:85-131) */
-static double bintp_value(const or_state* o, long d, int m) {
+/* trilinear value of component m of q in the fine-level cell with lower
+ * corner (ih, jh, kh) at weights (xco, yco, zco) (mg_bintp_mom.F90:85-131) */
+static double trilin(const or_state* o, int ih, int jh, int kh, const double* w, int m) {
     LEVEL(1);
-    const int ih = o->ohosts[3 * d], jh = o->ohosts[3 * d + 1], kh = o->ohosts[3 * d + 2];
-    const double xco = o->ocoef[3 * d], yco = o->ocoef[3 * d + 1], zco = o->ocoef[3 * d + 2];
+    const double xco = w[0], yco = w[1], zco = w[2];
     const long n1 = IX(ih, jh, kh), n2 = IX(ih + 1, jh, kh), n3 = IX(ih, jh + 1, kh);
     const long n4 = IX(ih, jh, kh + 1), n5 = IX(ih + 1, jh, kh + 1), n6 = IX(ih, jh + 1, kh + 1);
     const long n7 = IX(ih + 1, jh + 1, kh), n8 = IX(ih + 1, jh + 1, kh + 1);
@@ -2035,6 +2042,23 @@ static double bintp_value(const or_state* o, long d, int m) {
     const double* q = o->q;
     return q[5 * n1 + m] * c000 + q[5 * n2 + m] * c100 + q[5 * n3 + m] * c010 + q[5 * n4 + m] * c001 +
            q[5 * n5 + m] * c101 + q[5 * n6 + m] * c011 + q[5 * n7 + m] * c110 + q[5 * n8 + m] * c111;
+}
+
+static double bintp_value(const or_state* o, long d, int m) {
+    return trilin(o, o->ohosts[3 * d], o->ohosts[3 * d + 1], o->ohosts[3 * d + 2], o->ocoef + 3 * d, m);
+}
+
+/* nu~ written into a fringe node, then its floor and xnut (mg_bintp_tur.F90:160-176) */
+static void tur_fringe(or_state* o, long l, double v) {
+    const double rei = 1.0 / o->p.ren, cv1c = 357.911;
+    o->q[5 * l + 4] = v;
+    if (o->p.des) {
+        o->q[5 * l + 4] = dmax2(o->cvin, o->q[5 * l + 4]);
+        const double chi = o->q[5 * l + 4] / rei;
+        const double chic = chi * chi * chi;
+        const double fv1 = chic / (chic + cv1c);
+        o->xnut[l] = o->q[5 * l + 4] * fv1;
+    }
 }
 
 /* mg_bintp_mom: q(1:4) at the fringe nodes (mg_bintp_mom.F90:59-165) */
@@ -2057,20 +2081,11 @@ static void bintp_mom(or_state* o) {
 static void bintp_tur(or_state* o) {
     if (o->nover == 0) return;
     LEVEL(1);
-    const double rei = 1.0 / o->p.ren, cv1c = 357.911;
     double* buf = (double*)malloc(sizeof(double) * o->nover);
     for (long d = 0; d < o->nover; d++) buf[d] = bintp_value(o, d, 4);
     for (long d = 0; d < o->nover; d++) {
         const int* t = o->onodes + 3 * d;
-        const long l = IX(t[0], t[1], t[2]);
-        o->q[5 * l + 4] = buf[d];
-        if (o->p.des) {
-            o->q[5 * l + 4] = dmax2(o->cvin, o->q[5 * l + 4]);
-            const double chi = o->q[5 * l + 4] / rei;
-            const double chic = chi * chi * chi;
-            const double fv1 = chic / (chic + cv1c);
-            o->xnut[l] = o->q[5 * l + 4] * fv1;
-        }
+        tur_fringe(o, IX(t[0], t[1], t[2]), buf[d]);
     }
     free(buf);
 }
@@ -2081,6 +2096,7 @@ int or_bintp(or_state* o, int which) {
     return 0;
 }
 
+/* mg_driver (src/mg_driver.F90:2-323) with mg_ck_conver (:335-405) */
 int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
     const int ng = o->ng, ns = 1;
     long n1 = o->nl[1], l;
@@ -2131,5 +2147,136 @@ int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
         if (converged) break;
     }
     if (itc_out) *itc_out = o->itc;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Several zones, one rank each in the reference (nzone > 1), coupled by */
+/* a cross-zone donor table.  Each rank runs mg_driver; the ranks meet  */
+/* only in the overset exchanges (mg_bintp_tur after des_eddy,          */
+/* mg_bintp_mom after the fine pass, mg_driver.F90:182,241-244) and in  */
+/* the norm reduction of mg_residual (mpi_reduce of the local sums to   */
+/* the root, divided by the root zone's interior count,                  */
+/* mg_residual.F90:36,71-82).  Running every zone's work between two    */
+/* exchanges in zone order is therefore the same computation.           */
+/* hosts, nodes: 4 ints per donor (i, j, k, zone), 1-based, as          */
+/* hosts(1:4,.)/nodes(1:4,.) of global_osg.F90:18-24.                    */
+/* ------------------------------------------------------------------ */
+static int group_check(or_state* const* zs, int nz, long nd, const int* hosts, const int* nodes) {
+    for (long d = 0; d < nd; d++) {
+        const int* h = hosts + 4 * d;
+        const int* t = nodes + 4 * d;
+        if (h[3] < 1 || h[3] > nz || t[3] < 1 || t[3] > nz) return -1;
+        const or_state* a = zs[h[3] - 1];
+        const or_state* b = zs[t[3] - 1];
+        if (h[0] < 1 || h[0] + 1 > a->img[1] || h[1] < 1 || h[1] + 1 > a->jmg[1] || h[2] < 1 || h[2] + 1 > a->kmg[1])
+            return -1;
+        if (t[0] < 1 || t[0] > b->img[1] || t[1] < 1 || t[1] > b->jmg[1] || t[2] < 1 || t[2] > b->kmg[1]) return -1;
+    }
+    return 0;
+}
+
+static void group_bintp(or_state* const* zs, long nd, const int* hosts, const double* coef, const int* nodes,
+                        int tur) {
+    const int nc = tur ? 1 : 4, m0 = tur ? 4 : 0;
+    double* buf = (double*)malloc(sizeof(double) * (nd > 0 ? nd : 1) * nc);
+    for (long d = 0; d < nd; d++) {
+        const int* h = hosts + 4 * d;
+        for (int c = 0; c < nc; c++) buf[nc * d + c] = trilin(zs[h[3] - 1], h[0], h[1], h[2], coef + 3 * d, m0 + c);
+    }
+    for (long d = 0; d < nd; d++) {
+        const int* t = nodes + 4 * d;
+        or_state* o = zs[t[3] - 1];
+        const int ni = o->img[1], nj = o->jmg[1];
+        const long l = IX(t[0], t[1], t[2]);
+        if (tur) tur_fringe(o, l, buf[d]);
+        else
+            for (int m = 0; m < 4; m++) o->q[5 * l + m] = buf[4 * d + m];
+    }
+    free(buf);
+}
+
+int or_group_bintp(or_state* const* zs, int nz, long nd, const int* hosts, const double* coef, const int* nodes,
+                   int which) {
+    if (group_check(zs, nz, nd, hosts, nodes)) return -1;
+    group_bintp(zs, nd, hosts, coef, nodes, which != 0);
+    return 0;
+}
+
+int or_group_mg_driver(or_state* const* zs, int nz, long nd, const int* hosts, const double* coef, const int* nodes,
+                       int apply_mom, int* itc_out, double* resid_hist) {
+    if (nz < 1) return -1;
+    const or_state* z0 = zs[0];
+    const int ng = z0->ng, ns = 1;
+    int z, m, it, n, itr, decide_recalc_rh = 0, itc = 0;
+    double init_resd[5] = {0, 0, 0, 0, 0};
+    for (z = 0; z < nz; z++) {
+        const or_state* o = zs[z];
+        if (o->ng != ng || o->img[ng] < 5 || o->jmg[ng] < 5 || o->kmg[ng] < 5) return -1;
+        for (n = 0; n < ng; n++)
+            if (o->p.iter[n] != z0->p.iter[n]) return -1;
+        if (o->p.des != z0->p.des || o->p.itm != z0->p.itm) return -1;
+    }
+    if (group_check(zs, nz, nd, hosts, nodes)) return -1;
+    for (z = 0; z < nz; z++) {
+        or_state* o = zs[z];
+        const long n1 = o->nl[1];
+        o->itc = 0;
+        for (long l = 0; l < 5 * n1; l++) {
+            o->qnm1[l] = o->qn[l];
+            o->qn[l] = o->q[l];
+        }
+        for (m = 0; m < 5; m++) {
+            mg_inject(o, o->qn, 5, m);
+            mg_inject(o, o->qnm1, 5, m);
+        }
+    }
+    const double tnu = (double)((long)(z0->img[1] - 2) * (z0->jmg[1] - 2) * (z0->kmg[1] - 2));
+    for (it = 1; it <= z0->p.itm; it++) {
+        double resd[5] = {0, 0, 0, 0, 0}, residual[5], eomax, ermax;
+        int converged = 0;
+        itc++;
+        for (z = 0; z < nz; z++) {
+            zs[z]->itc = itc;
+            memcpy(zs[z]->qold_mg, zs[z]->q, 5 * zs[z]->nl[1] * sizeof(double));
+        }
+        for (n = ns; n <= ng; n++) {
+            if (n > 1)
+                for (z = 0; z < nz; z++) {
+                    mg_nlevel(zs[z], n);
+                    mg_zero_pk_boundary(zs[z], n);
+                }
+            for (itr = 1; itr <= z0->p.iter[n - 1]; itr++) {
+                if (n != ng) decide_recalc_rh = (itr != z0->p.iter[n - 1]) ? 0 : 1;
+                if (z0->p.des && n == ns) {
+                    for (z = 0; z < nz; z++) des_eddy(zs[z]);
+                    group_bintp(zs, nd, hosts, coef, nodes, 1);
+                }
+                for (z = 0; z < nz; z++) solver_daf(zs[z], n, decide_recalc_rh, itr);
+                if (n == ns && apply_mom) group_bintp(zs, nd, hosts, coef, nodes, 0);
+            }
+            for (z = 0; z < nz; z++) mg_zero_pk_boundary(zs[z], n);
+        }
+        for (z = 0; z < nz; z++) {
+            for (n = ng - 1; n >= ns; n--) mg_prolong(zs[z], n);
+            bcond_fm(zs[z], zs[z]->q);
+        }
+        for (z = 0; z < nz; z++) {
+            double part[5];
+            mg_residual_sums(zs[z], part);
+            for (m = 0; m < 5; m++) resd[m] = resd[m] + part[m];
+        }
+        for (m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
+        if (resid_hist)
+            for (m = 0; m < 5; m++) resid_hist[5 * (it - 1) + m] = residual[m];
+        if (itc == 1)
+            for (m = 0; m < 5; m++) init_resd[m] = residual[m];
+        eomax = dmax2(dmax2(residual[1], residual[2]), residual[3]);
+        ermax = dmax2(dmax2(residual[1] - init_resd[1], residual[2] - init_resd[2]), residual[3] - init_resd[3]);
+        if (itc > z0->p.it_min && ermax < z0->p.er_min) converged = 1;
+        if (itc > z0->p.it_min && eomax < z0->p.eo_min) converged = 1;
+        if (converged) break;
+    }
+    if (itc_out) *itc_out = itc;
     return 0;
 }

@@ -63,6 +63,17 @@ int or_set_overset(or_state* o, long n, const int* hosts, const double* coef, co
 /* one mg_bintp_mom (which = 0) or mg_bintp_tur (which = 1) on the stored state */
 int or_bintp(or_state* o, int which);
 
+/* one physical step of nz zones (one rank each in the reference) coupled by a
+ * cross-zone donor table: hosts, nodes 4 ints per donor (i, j, k, zone;
+ * 1-based), coef 3 doubles; nu~ is interpolated after every des_eddy, q(1:4)
+ * after every fine pass when apply_mom.  Norms: sums over all zones divided
+ * by zone 1's interior count (src/mg_residual.F90:36,71-82). */
+int or_group_mg_driver(or_state* const* zs, int nz, long nd, const int* hosts, const double* coef,
+                       const int* nodes, int apply_mom, int* itc, double* resid);
+/* one cross-zone interpolation (which = 0 q(1:4), 1 nu~) with such a table */
+int or_group_bintp(or_state* const* zs, int nz, long nd, const int* hosts, const double* coef,
+                   const int* nodes, int which);
+
 /* LAPACK SGTSV algorithm in FP64 (src/sgtsv.f:1-263); returns INFO */
 int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb);
 /* row interchanges performed by or_gtsv so far (reset to 0 if reset != 0) */

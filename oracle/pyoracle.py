@@ -60,10 +60,13 @@ def lib():
         L.or_gtsv.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
         ip = ctypes.POINTER(ctypes.c_int)
         L.or_set_overset.argtypes = [vp, ctypes.c_long, ip, dp, ip, ctypes.c_int]
+        L.or_group_mg_driver.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_long, ip, dp, ip, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), dp]
+        L.or_group_bintp.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_long, ip, dp, ip, ctypes.c_int]
         L.or_bintp.argtypes = [vp, ctypes.c_int]
         for f in ("or_set_grid", "or_set_wd", "or_set_state", "or_get_state", "or_get_array",
                   "or_set_array", "or_mg_driver", "or_solver_daf", "or_des_eddy", "or_mg_nlevel",
-                  "or_mg_prolong", "or_mg_residual", "or_gtsv", "or_set_overset", "or_bintp"):
+                  "or_mg_prolong", "or_mg_residual", "or_gtsv", "or_set_overset", "or_bintp", "or_group_mg_driver", "or_group_bintp"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -201,6 +204,39 @@ def gtsv(dl, d, du, b):
 # ---------------------------------------------------------------------------
 def ref_available():
     return os.path.exists(REF_EXE)
+
+
+def group_mg_driver(oracles, table):
+    """One physical step of several zones (Oracle objects, zone 1 first)
+    coupled by a cross-zone donor table (desalm.overset.ZoneTable); returns
+    (itc, norms history)."""
+    ip = ctypes.POINTER(ctypes.c_int)
+    arr = (ctypes.c_void_p * len(oracles))(*[o.h for o in oracles])
+    if table is None:
+        n, h, c, t, mom = 0, np.zeros((1, 4), np.int32), np.zeros((1, 3)), np.zeros((1, 4), np.int32), 0
+    else:
+        n, mom = len(table), int(table.apply_mom)
+        h = np.ascontiguousarray(table.hosts, dtype=np.int32)
+        c = np.ascontiguousarray(table.coef, dtype=np.float64)
+        t = np.ascontiguousarray(table.nodes, dtype=np.int32)
+    itc = ctypes.c_int(0)
+    hist = np.zeros((max(oracles[0].params.itm, 1), 5))
+    rc = lib().or_group_mg_driver(arr, len(oracles), n, h.ctypes.data_as(ip), _p(c), t.ctypes.data_as(ip), mom,
+                                  ctypes.byref(itc), _p(hist))
+    assert rc == 0, "or_group_mg_driver rejected the zones or the table"
+    return itc.value, hist[: itc.value]
+
+
+def group_bintp(oracles, table, which):
+    """One cross-zone interpolation (which = "mom" or "tur") with a ZoneTable."""
+    ip = ctypes.POINTER(ctypes.c_int)
+    arr = (ctypes.c_void_p * len(oracles))(*[o.h for o in oracles])
+    h = np.ascontiguousarray(table.hosts, dtype=np.int32)
+    c = np.ascontiguousarray(table.coef, dtype=np.float64)
+    t = np.ascontiguousarray(table.nodes, dtype=np.int32)
+    rc = lib().or_group_bintp(arr, len(oracles), len(table), h.ctypes.data_as(ip), _p(c), t.ctypes.data_as(ip),
+                              0 if which == "mom" else 1)
+    assert rc == 0, "or_group_bintp rejected the table"
 
 
 def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600, exe=None):
