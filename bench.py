@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--n", type=int, default=129, help="nodes per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--conv-itm", type=int, default=600,
                     help="pseudo-iteration cap of the convergence leg (0 skips it)")
     args = ap.parse_args()
@@ -103,9 +104,12 @@ def main():
 
     value, wall, iters_done, stats, hist, finite, pivots, p = timed_run(PREC_FP64, True)
     n1 = p.img * p.jmg * p.kmg
-    v_mixed, wall_mixed, _, _, _, fin_mixed, _, _ = timed_run(PREC_MIXED, False)
-    mixed = {"value": round(v_mixed, 3), "unit": "Mcell-iters/s", "ms_per_step": round(wall_mixed / args.steps * 1e3, 3),
-             "state_finite": fin_mixed, "dtype": "f64 residual, f32 DAF sweeps"}
+    mixed = None
+    if not args.no_mixed:
+        v_mixed, wall_mixed, _, _, _, fin_mixed, _, _ = timed_run(PREC_MIXED, False)
+        mixed = {"value": round(v_mixed, 3), "unit": "Mcell-iters/s",
+                 "ms_per_step": round(wall_mixed / args.steps * 1e3, 3), "state_finite": fin_mixed,
+                 "dtype": "f64 residual, f32 DAF sweeps"}
 
     # dominant kernel family by total device time
     fam = {}

@@ -460,9 +460,11 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     const double tnu = (double)((long)(L0.ni - 2) * (L0.nj - 2) * (L0.nk - 2));
     std::vector<double> parts(5 * zs.size());
     int itc = 0;
-    int decide_recalc_rh = 0;
-    for (int it = 1; it <= p.itm; it++) {
-        itc++;
+    // decide_recalc_rh is carried from level to level and from one
+    // pseudo-iteration into the next (mg_driver.F90:144-150); the body of an
+    // iteration is fixed once its starting value is
+    auto body = [&](int recalc_in) {
+        int decide_recalc_rh = recalc_in;
         for (int n = 1; n <= ng; n++) {
             if (n > 1)
                 for (desalm_ctx* c : zs)
@@ -492,6 +494,58 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
             // update norms, then qold_mg <- q for the next pseudo-iteration
             timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1); });
         }
+        return decide_recalc_rh;
+    };
+    // the value decide_recalc_rh leaves an iteration with, from a given start
+    auto recalc_out = [&](int in) {
+        int v = in;
+        for (int n = 1; n <= ng; n++)
+            for (int itr = 1; itr <= p.iter[n - 1]; itr++)
+                if (n != ng) v = (itr != p.iter[n - 1]) ? 0 : 1;
+        return v;
+    };
+    auto schedule = [&](int in) {
+        std::vector<int> v;
+        int r = in;
+        for (int n = 1; n <= ng; n++)
+            for (int itr = 1; itr <= p.iter[n - 1]; itr++) {
+                if (n != ng) r = (itr != p.iter[n - 1]) ? 0 : 1;
+                v.push_back(r);
+            }
+        return v;
+    };
+    // Without the per-launch profiler the iteration's launches are captured
+    // once into a HIP graph and replayed; this needs every iteration to run the
+    // same launches, i.e. the same decide_recalc_rh schedule from the start
+    // value 0 of the first iteration as from the carried value of the others.
+    bool profiling = false;
+    for (desalm_ctx* c : zs) profiling = profiling || c->prof.on;
+    const int carried = recalc_out(0);
+    const bool use_graph = !profiling && p.itm > 1 && schedule(0) == schedule(carried);
+    hipGraphExec_t exec = nullptr;
+    int decide_recalc_rh = 0;
+    for (int it = 1; it <= p.itm; it++) {
+        itc++;
+        if (use_graph) {
+            if (!exec) {
+                hipGraph_t graph = nullptr;
+                HIPCHK(c0, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                body(decide_recalc_rh);
+                const hipError_t ce = hipStreamEndCapture(s, &graph);
+                if (ce != hipSuccess) return fail(c0, std::string("graph capture failed: ") + hipGetErrorString(ce));
+                const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) return fail(c0, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+            }
+            const hipError_t le = hipGraphLaunch(exec, s);
+            if (le != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                return fail(c0, std::string("hipGraphLaunch failed: ") + hipGetErrorString(le));
+            }
+            decide_recalc_rh = carried;
+        } else {
+            decide_recalc_rh = body(decide_recalc_rh);
+        }
         HIPCHK(c0, hipGetLastError());
         for (size_t z = 0; z < zs.size(); z++)
             HIPCHK(c0, hipMemcpyAsync(&parts[5 * z], zs[z]->F.resid, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -515,6 +569,7 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
         if (itc > p.it_min && eomax < p.eo_min) converged = true;
         if (converged) break;
     }
+    if (exec) (void)hipGraphExecDestroy(exec);
     for (desalm_ctx* c : zs)
         HIPCHK(c0, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     if (itc_out) *itc_out = itc;
