@@ -242,21 +242,15 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, Da
 // Thomas elimination in type S: c' in B.c[m], d' over the input in B.d[m],
 // the solution into B.x[m].  Loads run one group of four rows ahead of the
 // elimination.
-constexpr int kPF = 4;
-
-// Lines per 64-wide wave of the j and k sweeps (see k_sa_adi).
-#ifndef DESALM_DAF_LPW
-#define DESALM_DAF_LPW 64
-#endif
-constexpr int kDafLpw = DESALM_DAF_LPW;
+// rows loaded per group, one group ahead of the elimination (16 measured
+// best among 4, 8 and 16 at 129^3)
+constexpr int kPF = 16;
 
 template <int D, typename S>
 __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int m = blockIdx.y;
-    const int lane = (int)threadIdx.x % 64;
-    if (lane >= kDafLpw) return;
-    const long t = ((long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * kDafLpw + lane;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long nlines, base, sD;
     int len;
     if (D == 0) {
@@ -588,8 +582,7 @@ static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Cons
     const unsigned gp = (unsigned)((L.n + 255) / 256);
     const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
                                                                     : (long)(L.ni - 2) * (L.nj - 2);
-    const long lpb = (long)kDafLpw * 2;  // lines per 128-thread workgroup
-    const dim3 gt((unsigned)((nl + lpb - 1) / lpb), 4);
+    const dim3 gt((unsigned)((nl + 127) / 128), 4);
     if (D == 0) {
         hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
         const unsigned gi = (unsigned)((nl + kTL - 1) / kTL);

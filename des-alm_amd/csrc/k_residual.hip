@@ -7,9 +7,9 @@
 //              convection, 4th-difference pressure dissipation, BDF2 term,
 //              zeroing of the boundary planes
 //   k_visc_pk  2nd-order viscous fluxes and the FAS forcing
-// One thread per node gathers its stencil from the SoA fields; face fluxes
-// are evaluated by the two nodes that share them, so no face arrays touch
-// HBM.
+// One thread per node gathers its stencil from the SoA fields and no face
+// arrays touch HBM; k_residual's face terms are evaluated by both nodes that
+// share them, k_visc_pk shares its face fluxes through LDS.
 // Operand order inside every expression follows the reference routine cited.
 #include "desalm_internal.h"
 
@@ -256,49 +256,83 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
 // Viscous term subtracted from rh(2:4) (rhs_viscous.F90:112-329,
 // rhs_unst_visc_diss.F90:181-183) and the FAS forcing of coarse levels
 // (rhs_pk.F90:232-264): pk = pk - rh when calc_pk, then rh = rh + pk.
-__global__ void __launch_bounds__(256) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
+//
+// A workgroup of kVX x kVY threads owns an (i, j) tile of (kVX-1) x (kVY-1)
+// interior nodes of one k plane (kVK = 1) plus one halo column and row on the
+// low side.  Every thread evaluates the +i face of its node (rows with an
+// output) and the +j face (columns with an output) once and shares them
+// through LDS, so the i and j faces are evaluated about once instead of by
+// both of their nodes; an output node evaluates both of its k faces.
+// Marching a workgroup through several planes (carrying the k face in
+// registers) was measured slower: each plane then waits at the barriers with
+// too few waves resident to hide the gathers.
+constexpr int kVX = 32, kVY = 8, kVK = 1;
+
+__global__ void __launch_bounds__(kVX * kVY) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk) {
+    __shared__ double s_fi[3][kVY][kVX];
+    __shared__ double s_fj[3][kVY][kVX];
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int i = (int)(t % ni);
-    const int j = (int)((t / ni) % nj);
-    const int k = (int)(t / ((long)ni * nj));
-    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
-    const long g = L.off + t;
+    const int tx = (int)threadIdx.x % kVX, ty = (int)threadIdx.x / kVX;
+    const int i = (int)blockIdx.x * (kVX - 1) + tx;  // node 0-based; tile outputs start at i = 1
+    const int j = (int)blockIdx.y * (kVY - 1) + ty;
+    const int k0 = 1 + (int)blockIdx.z * kVK;
+    const int k1 = min(k0 + kVK, nk - 1);
+    const bool in_i = i <= ni - 2, in_j = j <= nj - 2;
+    const bool out = tx >= 1 && ty >= 1 && in_i && in_j;
+    const bool face_i = ty >= 1 && in_i && in_j;  // +i face of a node on an output row
+    const bool face_j = tx >= 1 && in_i && in_j;  // +j face of a node on an output column
     const Nb nb{1, (long)ni, (long)ni * nj};
-    double visc[3];
-    {
-        double fp[3], fm[3];
-        visc_face<0>(L, F, C, t, nb, fp);
-        visc_face<0>(L, F, C, t - nb.sI, nb, fm);
+    const long col = (long)i + (long)ni * j;
+    double fk[3] = {0.0, 0.0, 0.0};
+    if (out) visc_face<2>(L, F, C, col + nb.sK * (k0 - 1), nb, fk);
+    for (int k = k0; k < k1; k++) {
+        const long t = col + nb.sK * k;
+        if (face_i) {
+            double f[3];
+            visc_face<0>(L, F, C, t, nb, f);
 #pragma unroll
-        for (int c = 0; c < 3; c++) visc[c] = L.dc * (fp[c] - fm[c]);
-        visc_face<1>(L, F, C, t, nb, fp);
-        visc_face<1>(L, F, C, t - nb.sJ, nb, fm);
-#pragma unroll
-        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.de * (fp[c] - fm[c]);
-        visc_face<2>(L, F, C, t, nb, fp);
-        visc_face<2>(L, F, C, t - nb.sK, nb, fm);
-#pragma unroll
-        for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.dz * (fp[c] - fm[c]);
-    }
-    double r[4];
-    r[0] = F.rh[0][g];
-#pragma unroll
-    for (int m = 1; m < 4; m++) r[m] = F.rh[m][g] - visc[m - 1];
-    if (n > 1) {
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            double pk = F.pk[m][g];
-            if (calc_pk == 1) {
-                pk = pk - r[m];
-                F.pk[m][g] = pk;
-            }
-            r[m] = r[m] + pk;
+            for (int c = 0; c < 3; c++) s_fi[c][ty][tx] = f[c];
         }
-    }
+        if (face_j) {
+            double f[3];
+            visc_face<1>(L, F, C, t, nb, f);
 #pragma unroll
-    for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
+            for (int c = 0; c < 3; c++) s_fj[c][ty][tx] = f[c];
+        }
+        double fkp[3] = {0.0, 0.0, 0.0};
+        if (out) visc_face<2>(L, F, C, t, nb, fkp);
+        __syncthreads();
+        if (out) {
+            const long g = L.off + t;
+            double visc[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) visc[c] = L.dc * (s_fi[c][ty][tx] - s_fi[c][ty][tx - 1]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.de * (s_fj[c][ty][tx] - s_fj[c][ty - 1][tx]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.dz * (fkp[c] - fk[c]);
+            double r[4];
+            r[0] = F.rh[0][g];
+#pragma unroll
+            for (int m = 1; m < 4; m++) r[m] = F.rh[m][g] - visc[m - 1];
+            if (n > 1) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    double pk = F.pk[m][g];
+                    if (calc_pk == 1) {
+                        pk = pk - r[m];
+                        F.pk[m][g] = pk;
+                    }
+                    r[m] = r[m] + pk;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) fk[c] = fkp[c];
+        __syncthreads();
+    }
 }
 
 }  // namespace
@@ -315,7 +349,9 @@ void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Const
     // the recalc branch needs U/J and p/J of the updated q (dtau is kept)
     if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, 0);
     hipLaunchKernelGGL(k_residual, grid, dim3(b), 0, s, L, F, C, n, recalc);
-    hipLaunchKernelGGL(k_visc_pk, grid, dim3(b), 0, s, L, F, C, n, recalc ? 0 : calc_pk);
+    const dim3 gv((unsigned)((L.ni - 2 + kVX - 2) / (kVX - 1)), (unsigned)((L.nj - 2 + kVY - 2) / (kVY - 1)),
+                  (unsigned)((L.nk - 2 + kVK - 1) / kVK));
+    hipLaunchKernelGGL(k_visc_pk, gv, dim3(kVX * kVY), 0, s, L, F, C, n, recalc ? 0 : calc_pk);
 }
 
 }  // namespace desalm

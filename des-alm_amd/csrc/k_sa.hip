@@ -211,22 +211,14 @@ __global__ void __launch_bounds__(256) k_sa_adi_pre(Level L, Fields F, Consts C)
 //   ae = dtj (U_(s+1) - av_(s+1/2)).
 // D = 2 also does the relaxed update of des_eddy.F90:260-272:
 //   q5 += (-q5 + qold + rh) / (1 + 1.5 e_source dtev / delti)
-constexpr int kPF = 4;
-
-// Lines per 64-wide wave of the j and k sweeps: the elimination is a chain
-// of dependent rows, so a wave takes as long with 16 lines as with 64, and
-// spreading the lines over more waves keeps more SIMDs busy.
-#ifndef DESALM_SA_LPW
-#define DESALM_SA_LPW 64
-#endif
-constexpr int kSaLpw = DESALM_SA_LPW;
+// rows loaded per group, one group ahead of the elimination (16 measured
+// best among 4, 8 and 16 at 129^3)
+constexpr int kPF = 16;
 
 template <int D>
 __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int lane = (int)threadIdx.x % 64;
-    if (lane >= kSaLpw) return;
-    const long t = ((long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * kSaLpw + lane;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long base, sD;
     int len;
     if (D == 0) {
@@ -321,20 +313,33 @@ __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
     double x = dprev;
     finish(base + (long)len * sD, x);
     for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
-        double dd[kPF], cc[kPF];
+        double dd[kPF], cc[kPF], qq[kPF], dv[kPF];
 #pragma unroll
         for (int u = 0; u < kPF; u++) {
             const int s = s0 - u;
             const long p = base + (1 + s) * sD;
             dd[u] = s >= 0 ? rh[p] : 0.0;
             cc[u] = s >= 0 ? cp[p] : 0.0;
+            if (D == 2) {
+                qq[u] = s >= 0 ? F.q[4][L.off + p] : 0.0;
+                dv[u] = s >= 0 ? F.dtev[L.off + p] : 0.0;
+            }
         }
 #pragma unroll
         for (int u = 0; u < kPF; u++) {
             const int s = s0 - u;
             if (s >= 0) {
                 x = dd[u] - cc[u] * x;
-                finish(base + (1 + s) * sD, x);
+                const long p = base + (1 + s) * sD;
+                if (D < 2) {
+                    rh[p] = x;
+                } else {
+                    // the update of finish() with q5 and dtev already loaded
+                    const double qv = qq[u];
+                    const double qe5 = qv + x;
+                    const double dum = 1.0 + 1.5 * C.e_source * dv[u] / C.delti;
+                    F.q[4][L.off + p] = qv + (-qv + qe5) / dum;
+                }
             }
         }
     }
@@ -347,11 +352,8 @@ __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
 // U/J, dtj and rh are loaded one chunk ahead of the elimination; c' and d'
 // go back to HBM per chunk except for the last one.  Same arithmetic as
 // k_sa_adi<0>.
-#ifndef DESALM_SA_SL
-#define DESALM_SA_SL 64
-#endif
-constexpr int kSL = DESALM_SA_SL;
-constexpr int kSNT = 2 * kSL > 64 ? 2 * kSL : 64;  // threads per workgroup
+constexpr int kSL = 64;
+constexpr int kSNT = 2 * kSL;  // threads per workgroup
 
 __global__ void __launch_bounds__(kSNT) k_sa_adi_i(Level L, Fields F) {
     constexpr int NT = kSNT;
@@ -535,9 +537,8 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     hipLaunchKernelGGL(k_sa_prep, dim3(gn), dim3(b), 0, s, L, F);
     const int bl = 128;
     const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + kSL - 1) / kSL);
-    const long lpb = (long)kSaLpw * (bl / 64);  // lines per workgroup
-    const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + lpb - 1) / lpb);
-    const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + lpb - 1) / lpb);
+    const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + bl - 1) / bl);
+    const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + bl - 1) / bl);
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
     const unsigned fj = (unsigned)(((long)L.ni * L.nk + b - 1) / b);
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);

@@ -12,10 +12,7 @@
 
 namespace desalm {
 
-#ifndef DESALM_TL
-#define DESALM_TL 32
-#endif
-constexpr int kTL = DESALM_TL;  // i-lines per workgroup
+constexpr int kTL = 32;       // i-lines per workgroup
 constexpr int kTR = 16;       // rows per chunk
 constexpr int kTS = kTR + 1;  // LDS stride of one line's chunk (odd: no bank conflicts)
 
