@@ -22,11 +22,11 @@ using namespace desalm;
 // Per-kernel device timing with HIP events on the context stream.
 enum KernelId {
     KID_DTAU, KID_RESIDUAL, KID_RESIDUAL_RECALC, KID_DAF_I, KID_DAF_J, KID_DAF_K, KID_BCOND,
-    KID_DES_EDDY, KID_NLEVEL, KID_PROLONG, KID_RESID, KID_OVERSET, KID_COUNT
+    KID_DES_EDDY, KID_NLEVEL, KID_PROLONG, KID_RESID, KID_OVERSET, KID_ALM, KID_COUNT
 };
 static const char* kKernelNames[KID_COUNT] = {
     "dtau", "residual", "residual_recalc", "daf_i", "daf_j", "daf_k", "bcond_fm",
-    "des_eddy", "nlevel", "prolong", "resid_norm", "overset"};
+    "des_eddy", "nlevel", "prolong", "resid_norm", "overset", "alm"};
 
 struct Profiler {
     bool on = false;
@@ -64,6 +64,7 @@ struct desalm_ctx {
     Overset ov{0, nullptr, nullptr, nullptr, nullptr, nullptr};
     int ov_mom = 0;
     double cvin = 0.0;
+    Alm alm;  // actuator-line rotors (desalm_set_turbines)
 };
 
 namespace {
@@ -179,15 +180,24 @@ void prof_flush(desalm_ctx* c) {
     c->prof.pending.clear();
 }
 
+// bcond_fm on the fine level (src/bcond_fm.F90:2-362), with the
+// actuator-line hub zeroing at its end (:328-355)
+void bcond(desalm_ctx* c, hipStream_t s) {
+    launch_bcond_fm(s, c->lv[1], c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix);
+    if (!c->alm.t.empty()) launch_alm_hub(s, c->lv[1], c->F, c->alm);
+}
+
 void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out) {
     const Level& L = c->lv[n];
     hipStream_t s = c->stream;
     timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C); });
     timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk); });
+    // actuator-line forcing on every level (solver_daf.F90:378-381)
+    if (!c->alm.t.empty()) timed(c, KID_ALM, n, [&] { launch_alm_rhs(s, L, c->F, c->alm, c->C.ren); });
     timed(c, KID_DAF_I, n, [&] { launch_daf_i(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_J, n, [&] { launch_daf_j(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_K, n, [&] { launch_daf_k(s, L, c->F, c->C, n, dq_out); });
-    if (n == 1) timed(c, KID_BCOND, n, [&] { launch_bcond_fm(s, L, c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix); });
+    if (n == 1) timed(c, KID_BCOND, n, [&] { bcond(c, s); });
     if (recalc == 1) timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0); });
 }
 
@@ -490,7 +500,7 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
                                    c->rs_tab + (n - 1) * per_level);
                 });
             }
-            timed(c, KID_BCOND, 1, [&] { launch_bcond_fm(s, c->lv[1], c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix); });
+            timed(c, KID_BCOND, 1, [&] { bcond(c, s); });
             // update norms, then qold_mg <- q for the next pseudo-iteration
             timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1); });
         }
@@ -743,6 +753,111 @@ const char* desalm_profile_name(int kernel) {
 
 const char* desalm_last_error(const desalm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
+// ---- actuator-line model ----
+static void free_alm(desalm_ctx* c) {
+    for (void* p : c->alm.dev) (void)hipFree(p);
+    c->alm = Alm();
+}
+
+int desalm_set_turbines(desalm_ctx* c, int32_t n_tur, const desalm_turbine* tur, int32_t n_polar,
+                        const desalm_polar* pol, double length_scale, double vel_infty) {
+    if (!c) return fail(c, "null argument");
+    if (n_tur < 0 || n_polar < 0 || (n_tur > 0 && (!tur || n_polar < 1 || !pol)))
+        return fail(c, "desalm_set_turbines: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_alm(c);
+    if (n_tur == 0) return 0;
+    const Level& L1 = c->lv[1];
+    const long jk = (long)L1.nj * L1.nk;
+    // validate on the host: indices inside the fine level, polar indices in
+    // range, distinct spreading nodes per rotor
+    for (int t = 0; t < n_tur; t++) {
+        const desalm_turbine& T = tur[t];
+        if (T.nblades < 1 || T.nelems < 1 || T.nangles < 1 || T.n_disk < 0 || T.n_ker < 0 || T.dtheta_interp <= 0.0)
+            return fail(c, "desalm_set_turbines: bad sizes of rotor " + std::to_string(t));
+        const long na = (long)T.nelems * T.nangles;
+        for (long e = 0; e < na; e++) {
+            const int i = T.node_indexes[e], j = T.node_indexes[e + na], k = T.node_indexes[e + 2 * na];
+            if (i < 1 || i + 1 > L1.ni || j < 1 || j + 1 > L1.nj || k < 1 || k + 1 > L1.nk)
+                return fail(c, "desalm_set_turbines: element cell outside the fine level");
+        }
+        for (long e = 0; e < jk; e++)
+            if (T.polar_before[e] < 0 || T.polar_before[e] >= n_polar || T.polar_after[e] < 0 ||
+                T.polar_after[e] >= n_polar)
+                return fail(c, "desalm_set_turbines: polar index out of range");
+        std::vector<long> kn;
+        for (int e = 0; e < T.n_ker; e++) {
+            const int* p = T.ker_nodes + 3 * e;
+            if (p[0] < 1 || p[0] > L1.ni || p[1] < 1 || p[1] > L1.nj || p[2] < 1 || p[2] > L1.nk)
+                return fail(c, "desalm_set_turbines: spreading node outside the fine level");
+            kn.push_back((p[0] - 1) + (long)L1.ni * ((p[1] - 1) + (long)L1.nj * (p[2] - 1)));
+        }
+        std::sort(kn.begin(), kn.end());
+        if (std::adjacent_find(kn.begin(), kn.end()) != kn.end())
+            return fail(c, "desalm_set_turbines: a spreading node is listed twice");
+        for (int e = 0; e < T.n_disk; e++)
+            if (T.disk_nodes[3 * e + 1] < 1 || T.disk_nodes[3 * e + 1] > L1.nj || T.disk_nodes[3 * e + 2] < 1 ||
+                T.disk_nodes[3 * e + 2] > L1.nk)
+                return fail(c, "desalm_set_turbines: disk node outside the fine level");
+        if (T.itur1 < 1 || T.itur2 > L1.ni) return fail(c, "desalm_set_turbines: itur1..itur2 outside the fine level");
+    }
+    for (int p = 0; p < n_polar; p++)
+        if (pol[p].n < 1 || !pol[p].angle || !pol[p].cl || !pol[p].cd)
+            return fail(c, "desalm_set_turbines: empty polar " + std::to_string(p));
+    bool ok = true;
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        void* d = nullptr;
+        if (hipMalloc(&d, bytes ? bytes : 8) != hipSuccess) { ok = false; return nullptr; }
+        c->alm.dev.push_back(d);
+        if (bytes && src && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
+        return d;
+    };
+    Alm& A = c->alm;
+    A.length_scale = length_scale;
+    A.vel_infty = vel_infty;
+    A.jmg = L1.nj;
+    A.kmg = L1.nk;
+    for (int t = 0; t < n_tur && ok; t++) {
+        const desalm_turbine& T = tur[t];
+        const long na = (long)T.nelems * T.nangles;
+        AlmTurbine D;
+        D.nblades = T.nblades; D.nelems = T.nelems; D.nangles = T.nangles;
+        D.itur1 = T.itur1; D.itur2 = T.itur2; D.n_disk = T.n_disk; D.n_ker = T.n_ker;
+        D.tur_rad = T.tur_rad; D.hub_rad = T.hub_rad; D.tsr = T.tsr; D.angle = T.angle; D.dtheta = T.dtheta_interp;
+        D.node_idx = (const int*)up(T.node_indexes, sizeof(int) * 3 * na);
+        D.node_pos = (const double*)up(T.node_indexes_pos, sizeof(double) * 27 * na);
+        D.disk_dat = (const double*)up(T.disk_dat, sizeof(double) * 8 * jk);
+        D.pol_b = (const int*)up(T.polar_before, sizeof(int) * jk);
+        D.pol_a = (const int*)up(T.polar_after, sizeof(int) * jk);
+        D.disk_nodes = (const int*)up(T.disk_nodes, sizeof(int) * 3 * T.n_disk);
+        D.ker_nodes = (const int*)up(T.ker_nodes, sizeof(int) * 3 * T.n_ker);
+        D.ker_pos = (const double*)up(T.ker_pos, sizeof(double) * 3 * T.n_ker);
+        D.ker_dat = (const double*)up(T.ker_dat, sizeof(double) * 3 * T.n_ker);
+        D.forces = (double*)up(nullptr, sizeof(double) * 6 * T.nblades * T.nelems);
+        A.t.push_back(D);
+    }
+    std::vector<int> off(n_polar), cnt(n_polar);
+    std::vector<double> pa, pl, pd;
+    for (int p = 0; p < n_polar; p++) {
+        off[p] = (int)pa.size();
+        cnt[p] = pol[p].n;
+        pa.insert(pa.end(), pol[p].angle, pol[p].angle + pol[p].n);
+        pl.insert(pl.end(), pol[p].cl, pol[p].cl + pol[p].n);
+        pd.insert(pd.end(), pol[p].cd, pol[p].cd + pol[p].n);
+    }
+    A.pol.off = (const int*)up(off.data(), sizeof(int) * n_polar);
+    A.pol.cnt = (const int*)up(cnt.data(), sizeof(int) * n_polar);
+    A.pol.angle = (const double*)up(pa.data(), sizeof(double) * pa.size());
+    A.pol.cl = (const double*)up(pl.data(), sizeof(double) * pl.size());
+    A.pol.cd = (const double*)up(pd.data(), sizeof(double) * pd.size());
+    if (!ok) {
+        free_alm(c);
+        return fail(c, "desalm_set_turbines: device allocation or copy failed");
+    }
+    return 0;
+}
+
 // ---- several zones coupled by overset interpolation ----
 struct desalm_group {
     std::vector<desalm_ctx*> z;  // zones, borrowed; zone 1 first
@@ -993,6 +1108,7 @@ int desalm_finalize(desalm_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_overset(c);
+    free_alm(c);
     prof_flush(c);
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/desalm.h"
 
 namespace desalm {
@@ -92,6 +94,39 @@ struct Overset {
     const long* slot;
 };
 
+// Actuator-line rotor on the device (desalm_turbine of desalm.h)
+struct AlmTurbine {
+    int nblades, nelems, nangles, itur1, itur2, n_disk, n_ker;
+    double tur_rad, hub_rad, tsr, angle, dtheta;
+    const int* node_idx;     // [nelems * nangles * 3]
+    const double* node_pos;  // [nelems * nangles * 27]
+    const double* disk_dat;  // [jmg * kmg * 8]
+    const int* pol_b;        // [jmg * kmg]
+    const int* pol_a;
+    const int* disk_nodes;   // [n_disk * 3]
+    const int* ker_nodes;    // [n_ker * 3]
+    const double* ker_pos;
+    const double* ker_dat;
+    double* forces;          // [nblades * nelems * 6] position and force per element
+};
+
+// all polars, concatenated: polar p is points off[p] .. off[p] + cnt[p] - 1
+struct AlmPolars {
+    const int* off;
+    const int* cnt;
+    const double* angle;
+    const double* cl;
+    const double* cd;
+};
+
+struct Alm {
+    std::vector<AlmTurbine> t;
+    AlmPolars pol{nullptr, nullptr, nullptr, nullptr, nullptr};
+    double length_scale = 0.0, vel_infty = 0.0;
+    int jmg = 0, kmg = 0;
+    std::vector<void*> dev;  // device allocations of the tables
+};
+
 struct Ctx;  // host context, desalm_api.hip
 
 // ---- launchers (each enqueues on `s`) ----
@@ -130,6 +165,10 @@ void launch_bintp(hipStream_t s, const Level& L, const Fields& F, const Consts& 
 void launch_bintp_gather(hipStream_t s, const Level& L, const Fields& F, const Overset& O, int tur);
 void launch_bintp_scatter(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const Overset& O, int tur,
                           double cvin, int des);
+// actuator-line model, k_alm.hip: forcing added to rh of level L, and
+// the hub zeroing of bcond_fm on the fine level
+void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A, double ren);
+void launch_alm_hub(hipStream_t s, const Level& L, const Fields& F, const Alm& A);
 // batched GEPP (LAPACK SGTSV semantics), k_daf.hip
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,
                        double* b, int* info);

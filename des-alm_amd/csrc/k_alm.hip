@@ -1,0 +1,175 @@
+// k_alm.hip -- actuator-line model: rhs_lines (src/rhs_lines.F90:47-483)
+// and the hub zeroing of bcond_fm (src/bcond_fm.F90:328-355).
+//
+//   k_alm_elements  one thread per (blade, element): velocity at the
+//                   element, angle of attack, polar lookup, lift and drag ->
+//                   the element's position and force (forces_lines)
+//   k_alm_spread    one thread per spreading node: Gaussian sum over all
+//                   elements of its rotor, added to rh(2:4) / J
+//   k_alm_hub       one thread per disk node: u = 0 on hub nodes,
+//                   i = itur1..itur2
+// The tables hold fine-level node indices.  rhs_lines runs on every level
+// and tests them against the current level's interior range and arrays,
+// as the reference does; the kernels take the current level.  Operand
+// order follows the reference routine.
+#include "desalm_internal.h"
+
+namespace desalm {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kRho = 1178.62;  // rhs_lines.F90:86
+
+// Cl, Cd from a polar: first consecutive pair with a1 < aoa <= a2, linear
+// in between (rhs_lines.F90:232-265); outside the polar the end values
+__device__ void polar_lookup(const AlmPolars& P, int p, double aoa, double& cl, double& cd) {
+    const int o = P.off[p], n = P.cnt[p];
+    const double* a = P.angle + o;
+    const double* l = P.cl + o;
+    const double* d = P.cd + o;
+    if (aoa <= a[0]) {
+        cl = l[0];
+        cd = d[0];
+        return;
+    }
+    for (int k = 0; k + 1 < n; k++) {
+        const double a1 = a[k], a2 = a[k + 1];
+        if (aoa <= a1) break;
+        if (aoa <= a2 && aoa >= a1) {
+            double m = (l[k + 1] - l[k]) / (a2 - a1);
+            cl = m * (aoa - a1) + l[k];
+            m = (d[k + 1] - d[k]) / (a2 - a1);
+            cd = m * (aoa - a1) + d[k];
+            return;
+        }
+    }
+    cl = l[n - 1];
+    cd = d[n - 1];
+}
+
+__global__ void k_alm_elements(Level L, Fields F, AlmTurbine T, AlmPolars P, double length_scale, double vel_infty,
+                               double ren, int jmg, int kmg) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= T.nblades * T.nelems) return;
+    const int bld = 1 + e / T.nelems, elem = 1 + e % T.nelems;
+    double* f = T.forces + 6L * e;
+#pragma unroll
+    for (int c = 0; c < 6; c++) f[c] = 0.0;  // forces_lines = 0 (:95)
+    const double vel_scale = ren * 10E-7 / length_scale;
+    const double omega = (vel_infty * T.tsr) / T.tur_rad;
+    double angle_aux = T.angle + (bld - 1) * 2 * kPi / T.nblades;
+    if (angle_aux > 2 * kPi) angle_aux = angle_aux - 2 * kPi;
+    const int nang = (int)lround(angle_aux / T.dtheta) + 1;
+    const long ne = T.nelems, na = T.nangles;
+    const long ix = (elem - 1) + ne * (nang - 1);
+    const int i = T.node_idx[ix], j = T.node_idx[ix + ne * na], k = T.node_idx[ix + 2 * ne * na];
+    if (!(i >= 2 && i <= L.ni - 1 && j >= 2 && j <= L.nj - 1 && k >= 2 && k <= L.nk - 1)) return;
+    auto np = [&](int c) { return T.node_pos[ix + ne * na * (c - 1)]; };
+    const double xf = np(1), yf = np(2), zf = np(3);
+    const double x000 = np(4), y000 = np(5), z000 = np(6), x100 = np(7), y100 = np(8);
+    const double x110 = np(10), y110 = np(11), x001 = np(16), z001 = np(18);
+    const double x101 = np(19), y101 = np(20), y111 = np(23);
+    (void)y000;
+    (void)x110;
+    const long jk = (j - 1) + (long)jmg * (k - 1), ck = (long)jmg * kmg;
+    const double radius = T.disk_dat[jk] * length_scale;
+    const double chord = T.disk_dat[jk + 4 * ck], twist = T.disk_dat[jk + 5 * ck], inter = T.disk_dat[jk + 6 * ck];
+    const long sJ = L.ni, sK = (long)L.ni * L.nj;
+    const long g = L.off + (i - 1) + sJ * (j - 1) + sK * (k - 1);
+    double uvw[3];
+#pragma unroll
+    for (int m = 1; m <= 3; m++) {
+        const double* q = F.q[m];
+        const double down = 1 / ((x100 - x000) * (y110 - y100)) *
+                            ((x100 - xf) * ((y110 - yf) * (q[g] + q[g + sJ])) +
+                             (xf - x000) * ((yf - y100) * (q[g + 1] + q[g + 1 + sJ])));
+        const double up = 1 / ((x101 - x001) * (y111 - y101)) *
+                          ((x101 - xf) * ((y111 - yf) * (q[g + sK] + q[g + sJ + sK])) +
+                           (xf - x001) * ((yf - y101) * (q[g + 1 + sK] + q[g + 1 + sJ + sK])));
+        uvw[m - 1] = down + (up - down) / (z001 - z000) * (zf - z000);
+    }
+    const double u_x = uvw[0] * vel_scale, u_y = uvw[1] * vel_scale, u_z = uvw[2] * vel_scale;
+    const double u_theta = omega * radius + u_y * sin(angle_aux) - u_z * cos(angle_aux);
+    const double phi = atan2(u_x, u_theta);
+    const double aoa = phi * 180.0 / kPi - twist;
+    double cl_b, cd_b, cl_a, cd_a;
+    polar_lookup(P, T.pol_b[jk], aoa, cl_b, cd_b);
+    polar_lookup(P, T.pol_a[jk], aoa, cl_a, cd_a);
+    const double cl = (cl_a - cl_b) * inter + cl_b;
+    const double cd = (cd_a - cd_b) * inter + cd_b;
+    const double urel = sqrt(u_theta * u_theta + u_x * u_x);
+    const double fL = 0.5 * cl * kRho * (urel * urel) * chord;
+    const double fD = 0.5 * cd * kRho * (urel * urel) * chord;
+    f[0] = xf;
+    f[1] = yf;
+    f[2] = zf;
+    f[3] = fL * cos(phi) + fD * sin(phi);
+    f[4] = fL * sin(angle_aux) * sin(phi) - fD * sin(angle_aux) * sin(phi);
+    f[5] = fL * cos(angle_aux) * sin(phi) - fD * cos(angle_aux) * cos(phi);
+}
+
+__global__ void k_alm_spread(Level L, Fields F, AlmTurbine T, double length_scale, double ren) {
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= T.n_ker) return;
+    const int ik = T.ker_nodes[3 * c], jk = T.ker_nodes[3 * c + 1], kk = T.ker_nodes[3 * c + 2];
+    if (!(ik >= 2 && ik <= L.ni - 1 && jk >= 2 && jk <= L.nj - 1 && kk >= 2 && kk <= L.nk - 1)) return;
+    const double Ls = length_scale;
+    const double radius = T.ker_dat[3 * c] * Ls;
+    if (!(radius > T.hub_rad)) return;
+    const double dV = T.ker_dat[3 * c + 2] * (Ls * Ls * Ls);
+    const double xk = T.ker_pos[3 * c], yk = T.ker_pos[3 * c + 1], zk = T.ker_pos[3 * c + 2];
+    const double deltar = fabs(T.tur_rad - T.hub_rad) / T.nelems;
+    const double eps = 2 * pow(dV, 1.0 / 3.0);
+    const double kscale = 1 / ((eps * eps * eps) * pow(kPi, 3.0 / 2.0));
+    double fxa = 0.0, fya = 0.0, fza = 0.0;
+    const int ne = T.nblades * T.nelems;
+    for (int e = 0; e < ne; e++) {
+        const double* f = T.forces + 6L * e;
+        const double rdis = sqrt((xk - f[0]) * (xk - f[0]) + (yk - f[1]) * (yk - f[1]) + (zk - f[2]) * (zk - f[2])) * Ls;
+        const double ker = kscale * exp(-(rdis * rdis) / (eps * eps));
+        fxa = fxa + ker * f[3] * deltar;
+        fya = fya + ker * f[4] * deltar;
+        fza = fza + ker * f[5] * deltar;
+    }
+    const double vel_scale = ren * 10E-7 / Ls;
+    const double s = Ls / (kRho * (vel_scale * vel_scale));
+    const long g = L.off + (ik - 1) + (long)L.ni * ((jk - 1) + (long)L.nj * (kk - 1));
+    const double rj = 1.0 / F.aj[g];
+    F.rh[1][g] = F.rh[1][g] + fxa * s * rj;
+    F.rh[2][g] = F.rh[2][g] + fya * s * rj;
+    F.rh[3][g] = F.rh[3][g] + fza * s * rj;
+}
+
+__global__ void k_alm_hub(Level L, Fields F, AlmTurbine T, double length_scale, int jmg) {
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= T.n_disk) return;
+    const int di = T.disk_nodes[3 * c], dj = T.disk_nodes[3 * c + 1], dk = T.disk_nodes[3 * c + 2];
+    if (!(di >= 1 && di <= L.ni && dj >= 1 && dj <= L.nj && dk >= 1 && dk <= L.nk)) return;
+    const double radius = T.disk_dat[(dj - 1) + (long)jmg * (dk - 1)] * length_scale;
+    if (!(radius <= T.hub_rad)) return;
+    for (int it = T.itur1; it <= T.itur2; it++)
+        F.q[1][L.off + (it - 1) + (long)L.ni * ((dj - 1) + (long)L.nj * (dk - 1))] = 0.0;
+}
+
+}  // namespace
+
+void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A, double ren) {
+    for (const AlmTurbine& T : A.t) {
+        const int ne = T.nblades * T.nelems;
+        hipLaunchKernelGGL(k_alm_elements, dim3((ne + 63) / 64), dim3(64), 0, s, L, F, T, A.pol, A.length_scale,
+                           A.vel_infty, ren, A.jmg, A.kmg);
+        if (T.n_ker > 0)
+            hipLaunchKernelGGL(k_alm_spread, dim3((T.n_ker + 127) / 128), dim3(128), 0, s, L, F, T, A.length_scale,
+                               ren);
+    }
+}
+
+void launch_alm_hub(hipStream_t s, const Level& L, const Fields& F, const Alm& A) {
+    for (const AlmTurbine& T : A.t)
+        if (T.n_disk > 0)
+            hipLaunchKernelGGL(k_alm_hub, dim3((T.n_disk + 127) / 128), dim3(128), 0, s, L, F, T, A.length_scale,
+                               A.jmg);
+}
+
+}  // namespace desalm

@@ -61,6 +61,7 @@ class Case:
         self.wd = wd
         self.q, self.qn, self.qnm1, self.xnut = q, qn, qnm1, xnut
         self.overset = overset
+        self.alm = None  # desalm.alm.AlmSetup: actuator-line rotors
 
     @property
     def n1(self):

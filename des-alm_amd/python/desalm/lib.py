@@ -70,11 +70,13 @@ def lib():
         L.desalm_group_set_overset.argtypes = [vp, ctypes.c_int64, i32p, dp, i32p, ctypes.c_int]
         L.desalm_group_mg_driver.argtypes = [vp, ip, dp]
         L.desalm_group_destroy.argtypes = [vp]
+        L.desalm_set_turbines.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, ctypes.c_double,
+                                          ctypes.c_double]
         for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
                   "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize",
                   "desalm_set_overset", "desalm_bintp", "desalm_group_create", "desalm_group_set_overset",
-                  "desalm_group_mg_driver", "desalm_group_destroy"):
+                  "desalm_group_mg_driver", "desalm_group_destroy", "desalm_set_turbines"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -127,7 +129,20 @@ class Solver:
         self.set_state(*case.state())
         if getattr(case, "overset", None) is not None:
             self.set_overset(case.overset)
+        if getattr(case, "alm", None) is not None:
+            self.set_turbines(case.alm)
         return self
+
+    def set_turbines(self, alm):
+        """Actuator-line rotors (desalm.alm.AlmSetup); None turns the forcing off."""
+        if alm is None:
+            self._chk(lib().desalm_set_turbines(self.h, 0, None, 0, None, 0.0, 0.0), "desalm_set_turbines")
+            return
+        ts, ps, keep = alm.as_ctypes()
+        self._chk(lib().desalm_set_turbines(self.h, len(alm.turbines), ctypes.cast(ts, ctypes.c_void_p),
+                                            len(alm.polars), ctypes.cast(ps, ctypes.c_void_p), alm.length_scale,
+                                            alm.vel_infty), "desalm_set_turbines")
+        del keep
 
     def set_overset(self, ov):
         """Donor table (desalm.cases.Overset) of the fine level; None removes it."""

@@ -214,6 +214,57 @@ int desalm_group_set_overset(desalm_group* group, int64_t n, const int32_t* host
 int desalm_group_mg_driver(desalm_group* group, int* itc_out, double* resid_out);
 int desalm_group_destroy(desalm_group* group);
 
+/*
+ * Actuator-line model (rhs_lines, src/rhs_lines.F90:47-483, called from
+ * solver_daf at src/solver_daf.F90:378-381; hub zeroing in bcond_fm,
+ * src/bcond_fm.F90:328-355): one rotor as the tables that init_lines builds
+ * (src/init.F90:1156-1710, module global_turb).  Node indices are 1-based
+ * on the fine level.  Arrays follow the Fortran order of the reference's
+ * arrays for one turbine:
+ *   node_indexes[(elem-1) + nelems*((nang-1) + nangles*(c-1))], c = 1..3
+ *   node_indexes_pos[same with c = 1..27]: element x, y, z, then the corners
+ *     000, 100, 110, 010, 001, 101, 111, 011 (x, y, z each)
+ *   disk_dat[(j-1) + jmg*((k-1) + kmg*(c-1))], c = 1..8
+ * polar_before / polar_after[(j-1) + jmg*(k-1)]: 0-based index into the
+ * polars passed with the turbines, for the element at node (., j, k).
+ */
+typedef struct desalm_turbine {
+    int32_t nblades, nelems, nangles;
+    int32_t itur1, itur2;                 /* i range zeroed in u at hub nodes */
+    double tur_rad, hub_rad, tsr;         /* radii in length_scale units; tip speed ratio */
+    double angle;                         /* angle of the reference blade (rad) */
+    double dtheta_interp;                 /* angle bin of node_indexes (rad) */
+    const int32_t* node_indexes;
+    const double* node_indexes_pos;
+    const double* disk_dat;
+    const int32_t* polar_before;
+    const int32_t* polar_after;
+    int32_t n_disk;                       /* hub/disk nodes: disk_node_coordinates */
+    const int32_t* disk_nodes;            /* [n_disk*3] i, j, k */
+    int32_t n_ker;                        /* spreading nodes: disk_node_coordinates_ker */
+    const int32_t* ker_nodes;             /* [n_ker*3] i, j, k */
+    const double* ker_pos;                /* [n_ker*3] x, y, z */
+    const double* ker_dat;                /* [n_ker*3] disk_dat_ker components 1, 2 and 4 at those nodes */
+} desalm_turbine;
+
+/* An airfoil polar: n points of (angle of attack in degrees, ascending, Cl, Cd),
+ * the content of one airfoil file that rhs_lines reads (src/rhs_lines.F90:232-300). */
+typedef struct desalm_polar {
+    int32_t n;
+    const double* angle;
+    const double* cl;
+    const double* cd;
+} desalm_polar;
+
+/* Turn the actuator-line forcing on with n_tur rotors, their polars and the
+ * scales of global_turb (length_scale, vel_infty); n_tur = 0 turns it off.
+ * The forcing is added to rh(2:4) in every smoother pass on every level
+ * (not in the recalc branch, src/solver_daf.F90:432-484), and bcond_fm zeroes
+ * u on the hub nodes.  An angle of attack outside a polar takes the polar's
+ * end value (the reference leaves the previous element's value there). */
+int desalm_set_turbines(desalm_ctx* ctx, int32_t n_tur, const desalm_turbine* turbines, int32_t n_polar,
+                        const desalm_polar* polars, double length_scale, double vel_infty);
+
 /* Read a device field of level n back in the reference layout.
  * which: 0 q(5), 1 qn(5), 2 qnm1(5), 3 qold_mg(5), 4 rh(4), 5 pk(4),
  * 6 xnut(1), 7 csi(3), 8 eta(3), 9 zet(3), 10 aj(1), 11 dtev(1), 12 wd(1).

@@ -51,6 +51,11 @@ struct or_state {
     double* ocoef;  /* 3 per donor */
     int* onodes;    /* 3 per donor, 1-based i,j,k */
     int apply_mom;  /* mg_bintp_mom runs (the reference's nzone > 1) */
+    /* actuator-line model (module global_turb): deep copies of the tables */
+    int n_tur, n_polar;
+    desalm_turbine* tur;
+    desalm_polar* pol;
+    double length_scale, vel_infty;
 };
 
 /* constants of src/precision.F90 */
@@ -219,6 +224,7 @@ void or_destroy(or_state* o) {
     free(o->q); free(o->qn); free(o->qnm1); free(o->qold_mg); free(o->rh); free(o->pk);
     free(o->xnut); free(o->wd); free(o->dtev);
     free(o->ohosts); free(o->ocoef); free(o->onodes);
+    or_set_turbines(o, 0, NULL, 0, NULL, 0.0, 0.0);
     free(o);
 }
 
@@ -534,6 +540,17 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
     for (k = 1; k <= nk; k++)
         for (j = 1; j <= nj; j++)
             for (i = 1; i <= ni; i++) QB(1, i, j, k) = QB(1, i, j, k) - pfix;
+    /* zero u on the hub's disk nodes, i = itur1..itur2 (:328-355); the
+     * range test covers the whole level, boundaries included */
+    for (int t = 0; t < o->n_tur; t++) {
+        const desalm_turbine* T = &o->tur[t];
+        for (int c = 0; c < T->n_disk; c++) {
+            const int di = T->disk_nodes[3 * c], dj = T->disk_nodes[3 * c + 1], dk = T->disk_nodes[3 * c + 2];
+            const double radius = T->disk_dat[(dj - 1) + (long)nj * ((dk - 1) + (long)nk * 0)] * o->length_scale;
+            if (di >= 1 && di <= ni && dj >= 1 && dj <= nj && dk >= 1 && dk <= nk && radius <= T->hub_rad)
+                for (int it = T->itur1; it <= T->itur2; it++) QB(2, it, dj, dk) = 0.0;
+        }
+    }
 #undef QB
 }
 
@@ -1227,6 +1244,138 @@ static void rhs_diag_solver(daf_t* d) {
     free(aw); free(ap); free(ae); free(aq); free(brh);
 }
 
+/* ------------------------------------------------------------------ */
+/* Actuator-line model: rhs_lines (src/rhs_lines.F90:47-483), an        */
+/* internal procedure of solver_daf run on every level after the rh     */
+/* boundary zeroing (solver_daf.F90:378-381).  Node indices of the      */
+/* tables are fine-level indices; on level n they are used as they are, */
+/* against level n's interior range and arrays, as the reference does.  */
+/* ------------------------------------------------------------------ */
+/* Cl, Cd of polar P at angle of attack aoa: the first pair of consecutive
+ * points with angle1 < aoa <= angle2, linear in between
+ * (rhs_lines.F90:232-265).  Outside the polar the reference keeps the
+ * previous element's values; here the end point's values are taken. */
+static void polar_lookup(const desalm_polar* P, double aoa, double* cl, double* cd) {
+    int idx;
+    if (aoa <= P->angle[0]) {
+        *cl = P->cl[0];
+        *cd = P->cd[0];
+        return;
+    }
+    for (idx = 0; idx + 1 < P->n; idx++) {
+        const double a1 = P->angle[idx], a2 = P->angle[idx + 1];
+        if (aoa <= a1) break;
+        if (aoa <= a2 && aoa >= a1) {
+            double m = (P->cl[idx + 1] - P->cl[idx]) / (a2 - a1);
+            *cl = m * (aoa - a1) + P->cl[idx];
+            m = (P->cd[idx + 1] - P->cd[idx]) / (a2 - a1);
+            *cd = m * (aoa - a1) + P->cd[idx];
+            return;
+        }
+    }
+    *cl = P->cl[P->n - 1];
+    *cd = P->cd[P->n - 1];
+}
+
+static void rhs_lines(daf_t* d) {
+    const or_state* o = d->o;
+    if (o->n_tur == 0) return;
+    const int ni = d->ni, nj = d->nj;
+    const int jmg = o->jmg[1], kmg = o->kmg[1];
+    const double pi = 3.141592653589793;
+    const double rho = 1178.62;
+    const double L = o->length_scale;
+    const double vel_scale = o->p.ren * 10E-7 / L;
+#define QL(m, i, j, k) d->q[5 * IX(i, j, k) + (m)-1]
+    for (int t = 0; t < o->n_tur; t++) {
+        const desalm_turbine* T = &o->tur[t];
+        const int ne = T->nelems, na = T->nangles;
+        double* fl = (double*)calloc((size_t)T->nblades * ne * 6, sizeof(double)); /* forces_lines = 0 */
+        const double omega = (o->vel_infty * T->tsr) / T->tur_rad;
+        for (int bld = 1; bld <= T->nblades; bld++) {
+            double angle_aux = T->angle + (bld - 1) * 2 * pi / T->nblades;
+            if (angle_aux > 2 * pi) angle_aux = angle_aux - 2 * pi;
+            const int nang = (int)lround(angle_aux / T->dtheta_interp) + 1;
+            for (int elem = 1; elem <= ne; elem++) {
+                const long e = (elem - 1) + (long)ne * (nang - 1);
+                const int i = T->node_indexes[e], j = T->node_indexes[e + (long)ne * na],
+                          k = T->node_indexes[e + 2L * ne * na];
+                const double* ps = T->node_indexes_pos;
+#define NP(c) ps[e + (long)ne * na * ((c)-1)]
+                const double xf = NP(1), yf = NP(2), zf = NP(3);
+                const double x000 = NP(4), y000 = NP(5), z000 = NP(6), x100 = NP(7), y100 = NP(8);
+                const double x110 = NP(10), y110 = NP(11), z001 = NP(18), x001 = NP(16);
+                const double x101 = NP(19), y101 = NP(20), y111 = NP(23);
+#undef NP
+                (void)z000;
+                if (!(i >= d->is && i <= d->ie && j >= d->js && j <= d->je && k >= d->ks && k <= d->ke)) continue;
+                const double* dd = T->disk_dat;
+                const long jk = (j - 1) + (long)jmg * (k - 1), ck = (long)jmg * kmg;
+                const double radius = dd[jk] * L;
+                const double chord = dd[jk + 4 * ck], twist = dd[jk + 5 * ck], inter = dd[jk + 6 * ck];
+                double uvw[3];
+                for (int m = 2; m <= 4; m++) {
+                    const double down = 1 / ((x100 - x000) * (y110 - y100)) *
+                                        ((x100 - xf) * ((y110 - yf) * (QL(m, i, j, k) + QL(m, i, j + 1, k))) +
+                                         (xf - x000) * ((yf - y100) * (QL(m, i + 1, j, k) + QL(m, i + 1, j + 1, k))));
+                    const double up = 1 / ((x101 - x001) * (y111 - y101)) *
+                                      ((x101 - xf) * ((y111 - yf) * (QL(m, i, j, k + 1) + QL(m, i, j + 1, k + 1))) +
+                                       (xf - x001) * ((yf - y101) * (QL(m, i + 1, j, k + 1) + QL(m, i + 1, j + 1, k + 1))));
+                    uvw[m - 2] = down + (up - down) / (z001 - z000) * (zf - z000);
+                }
+                const double u_x = uvw[0] * vel_scale, u_y = uvw[1] * vel_scale, u_z = uvw[2] * vel_scale;
+                const double u_theta = omega * radius + u_y * sin(angle_aux) - u_z * cos(angle_aux);
+                const double phi = atan2(u_x, u_theta);
+                const double aoa = phi * 180.0 / pi - twist;
+                double cl_b, cd_b, cl_a, cd_a;
+                polar_lookup(&o->pol[T->polar_before[jk]], aoa, &cl_b, &cd_b);
+                polar_lookup(&o->pol[T->polar_after[jk]], aoa, &cl_a, &cd_a);
+                const double cl = (cl_a - cl_b) * inter + cl_b;
+                const double cd = (cd_a - cd_b) * inter + cd_b;
+                const double urel = sqrt(u_theta * u_theta + u_x * u_x);
+                const double fL = 0.5 * cl * rho * (urel * urel) * chord;
+                const double fD = 0.5 * cd * rho * (urel * urel) * chord;
+                double* f = fl + 6 * ((long)(bld - 1) * ne + (elem - 1));
+                f[0] = xf;
+                f[1] = yf;
+                f[2] = zf;
+                f[3] = fL * cos(phi) + fD * sin(phi);
+                f[4] = fL * sin(angle_aux) * sin(phi) - fD * sin(angle_aux) * sin(phi);
+                f[5] = fL * cos(angle_aux) * sin(phi) - fD * cos(angle_aux) * cos(phi);
+            }
+        }
+        /* Gaussian spreading onto the kernel nodes (:355-470) */
+        const double deltar = fabs(T->tur_rad - T->hub_rad) / T->nelems;
+        for (int c = 0; c < T->n_ker; c++) {
+            const int ik = T->ker_nodes[3 * c], jk2 = T->ker_nodes[3 * c + 1], kk = T->ker_nodes[3 * c + 2];
+            const double xk = T->ker_pos[3 * c], yk = T->ker_pos[3 * c + 1], zk = T->ker_pos[3 * c + 2];
+            const double radius = T->ker_dat[3 * c] * L;
+            const double dV = T->ker_dat[3 * c + 2] * (L * L * L);
+            if (!(ik >= d->is && ik <= d->ie && jk2 >= d->js && jk2 <= d->je && kk >= d->ks && kk <= d->ke)) continue;
+            if (!(radius > T->hub_rad)) continue;
+            double fxa = 0.0, fya = 0.0, fza = 0.0;
+            for (int bld = 1; bld <= T->nblades; bld++)
+                for (int elem = 1; elem <= ne; elem++) {
+                    const double* f = fl + 6 * ((long)(bld - 1) * ne + (elem - 1));
+                    const double rdis = sqrt((xk - f[0]) * (xk - f[0]) + (yk - f[1]) * (yk - f[1]) +
+                                             (zk - f[2]) * (zk - f[2])) * L;
+                    const double eps = 2 * pow(dV, 1.0 / 3.0);
+                    const double ker = 1 / ((eps * eps * eps) * pow(pi, 3.0 / 2.0)) * exp(-(rdis * rdis) / (eps * eps));
+                    fxa = fxa + ker * f[3] * deltar;
+                    fya = fya + ker * f[4] * deltar;
+                    fza = fza + ker * f[5] * deltar;
+                }
+            const double s = L / (rho * (vel_scale * vel_scale));
+            const double ajv = d->aj[IX(ik, jk2, kk)];
+            d->rh[4 * IX(ik, jk2, kk) + 1] = d->rh[4 * IX(ik, jk2, kk) + 1] + fxa * s * (1.0 / ajv);
+            d->rh[4 * IX(ik, jk2, kk) + 2] = d->rh[4 * IX(ik, jk2, kk) + 2] + fya * s * (1.0 / ajv);
+            d->rh[4 * IX(ik, jk2, kk) + 3] = d->rh[4 * IX(ik, jk2, kk) + 3] + fza * s * (1.0 / ajv);
+        }
+        free(fl);
+    }
+#undef QL
+}
+
 static int solver_daf(or_state* o, int n, int recalc, int calc_pk) {
     daf_t dd, *d = &dd;
     LEVEL(n);
@@ -1260,6 +1409,7 @@ static int solver_daf(or_state* o, int n, int recalc, int calc_pk) {
     rhs_unst_visc_diss(d);
     if (n > 1) rhs_pk(d, calc_pk == 1 ? 1 : 0);
     zero_rh_boundaries(d);
+    rhs_lines(d);
     rhs_modal_matrices(d);
     rhs_diag_solver(d);
     for (k = d->ks; k <= d->ke; k++)
@@ -2278,5 +2428,58 @@ int or_group_mg_driver(or_state* const* zs, int nz, long nd, const int* hosts, c
         if (converged) break;
     }
     if (itc_out) *itc_out = itc;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* actuator-line tables (deep copies)                                   */
+/* ------------------------------------------------------------------ */
+static void* dup(const void* p, size_t bytes) {
+    void* q = malloc(bytes ? bytes : 1);
+    if (bytes) memcpy(q, p, bytes);
+    return q;
+}
+
+int or_set_turbines(or_state* o, int n_tur, const desalm_turbine* tur, int n_polar, const desalm_polar* pol,
+                    double length_scale, double vel_infty) {
+    for (int t = 0; t < o->n_tur; t++) {
+        desalm_turbine* T = &o->tur[t];
+        free((void*)T->node_indexes); free((void*)T->node_indexes_pos); free((void*)T->disk_dat);
+        free((void*)T->polar_before); free((void*)T->polar_after); free((void*)T->disk_nodes);
+        free((void*)T->ker_nodes); free((void*)T->ker_pos); free((void*)T->ker_dat);
+    }
+    for (int p = 0; p < o->n_polar; p++) {
+        free((void*)o->pol[p].angle); free((void*)o->pol[p].cl); free((void*)o->pol[p].cd);
+    }
+    free(o->tur); free(o->pol);
+    o->tur = NULL; o->pol = NULL; o->n_tur = 0; o->n_polar = 0;
+    if (n_tur <= 0) return 0;
+    const long jk = (long)o->jmg[1] * o->kmg[1];
+    o->tur = (desalm_turbine*)calloc(n_tur, sizeof(desalm_turbine));
+    o->pol = (desalm_polar*)calloc(n_polar > 0 ? n_polar : 1, sizeof(desalm_polar));
+    for (int t = 0; t < n_tur; t++) {
+        desalm_turbine* T = &o->tur[t];
+        *T = tur[t];
+        const long na = (long)T->nelems * T->nangles;
+        T->node_indexes = (const int32_t*)dup(tur[t].node_indexes, sizeof(int32_t) * 3 * na);
+        T->node_indexes_pos = (const double*)dup(tur[t].node_indexes_pos, sizeof(double) * 27 * na);
+        T->disk_dat = (const double*)dup(tur[t].disk_dat, sizeof(double) * 8 * jk);
+        T->polar_before = (const int32_t*)dup(tur[t].polar_before, sizeof(int32_t) * jk);
+        T->polar_after = (const int32_t*)dup(tur[t].polar_after, sizeof(int32_t) * jk);
+        T->disk_nodes = (const int32_t*)dup(tur[t].disk_nodes, sizeof(int32_t) * 3 * T->n_disk);
+        T->ker_nodes = (const int32_t*)dup(tur[t].ker_nodes, sizeof(int32_t) * 3 * T->n_ker);
+        T->ker_pos = (const double*)dup(tur[t].ker_pos, sizeof(double) * 3 * T->n_ker);
+        T->ker_dat = (const double*)dup(tur[t].ker_dat, sizeof(double) * 3 * T->n_ker);
+    }
+    for (int p = 0; p < n_polar; p++) {
+        o->pol[p].n = pol[p].n;
+        o->pol[p].angle = (const double*)dup(pol[p].angle, sizeof(double) * pol[p].n);
+        o->pol[p].cl = (const double*)dup(pol[p].cl, sizeof(double) * pol[p].n);
+        o->pol[p].cd = (const double*)dup(pol[p].cd, sizeof(double) * pol[p].n);
+    }
+    o->n_tur = n_tur;
+    o->n_polar = n_polar;
+    o->length_scale = length_scale;
+    o->vel_infty = vel_infty;
     return 0;
 }

@@ -74,6 +74,11 @@ int or_group_mg_driver(or_state* const* zs, int nz, long nd, const int* hosts, c
 int or_group_bintp(or_state* const* zs, int nz, long nd, const int* hosts, const double* coef,
                    const int* nodes, int which);
 
+/* actuator-line model tables (desalm_turbine / desalm_polar of desalm.h;
+ * copied); n_tur = 0 removes them */
+int or_set_turbines(or_state* o, int n_tur, const desalm_turbine* tur, int n_polar, const desalm_polar* pol,
+                    double length_scale, double vel_infty);
+
 /* LAPACK SGTSV algorithm in FP64 (src/sgtsv.f:1-263); returns INFO */
 int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb);
 /* row interchanges performed by or_gtsv so far (reset to 0 if reset != 0) */

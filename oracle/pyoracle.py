@@ -63,10 +63,11 @@ def lib():
         L.or_group_mg_driver.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_long, ip, dp, ip, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), dp]
         L.or_group_bintp.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_long, ip, dp, ip, ctypes.c_int]
+        L.or_set_turbines.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_double]
         L.or_bintp.argtypes = [vp, ctypes.c_int]
         for f in ("or_set_grid", "or_set_wd", "or_set_state", "or_get_state", "or_get_array",
                   "or_set_array", "or_mg_driver", "or_solver_daf", "or_des_eddy", "or_mg_nlevel",
-                  "or_mg_prolong", "or_mg_residual", "or_gtsv", "or_set_overset", "or_bintp", "or_group_mg_driver", "or_group_bintp"):
+                  "or_mg_prolong", "or_mg_residual", "or_gtsv", "or_set_overset", "or_bintp", "or_group_mg_driver", "or_group_bintp", "or_set_turbines"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -111,7 +112,20 @@ class Oracle:
         self.set_state(q, qn, qnm1, xnut)
         if getattr(case, "overset", None) is not None:
             self.set_overset(case.overset)
+        if getattr(case, "alm", None) is not None:
+            self.set_turbines(case.alm)
         return self
+
+    def set_turbines(self, alm):
+        """Actuator-line tables (desalm.alm.AlmSetup); None removes them."""
+        if alm is None:
+            assert lib().or_set_turbines(self.h, 0, None, 0, None, 0.0, 0.0) == 0
+            return
+        ts, ps, keep = alm.as_ctypes()
+        rc = lib().or_set_turbines(self.h, len(alm.turbines), ctypes.cast(ts, ctypes.c_void_p), len(alm.polars),
+                                   ctypes.cast(ps, ctypes.c_void_p), alm.length_scale, alm.vel_infty)
+        assert rc == 0
+        del keep
 
     def set_overset(self, ov):
         """Donor table (desalm.cases.Overset) of the fine level."""
@@ -239,6 +253,36 @@ def group_bintp(oracles, table, which):
     assert rc == 0, "or_group_bintp rejected the table"
 
 
+def _write_alm(f, td, alm):
+    """turbine section of the harness input, and the polar files 'pNNN.txt'
+    (angle Cl Cd per line, closed by the -999 sentinel, src/rhs_lines.F90:239)"""
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).tofile(f)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    th = np.zeros((len(alm.turbines), 8), dtype=np.int32)
+    rr = [alm.length_scale, alm.vel_infty]
+    for t, T in enumerate(alm.turbines):
+        th[t, :7] = (T.nblades, T.nelems, T.nangles, T.itur1, T.itur2, len(T.disk_nodes), len(T.ker_nodes))
+    for T in alm.turbines:
+        rr += [T.tur_rad, T.hub_rad, T.tsr, T.angle, T.dtheta_interp]
+    i32(th)
+    f64(rr)
+    for T in alm.turbines:
+        i32(T.node_indexes)
+        f64(T.node_indexes_pos)
+        f64(T.disk_dat)
+        i32(T.polar_before)
+        i32(T.polar_after)
+        i32(T.disk_nodes)
+        i32(T.ker_nodes)
+        f64(T.ker_pos)
+        f64(T.ker_dat)
+    for k, (a, cl, cd) in enumerate(alm.polars):
+        with open(os.path.join(td, "p%03d.txt" % (k + 1)), "w") as pf:
+            for x, y, z in zip(a, cl, cd):
+                pf.write("%.17g %.17g %.17g\n" % (x, y, z))
+            pf.write("-999 0 0\n")
+
+
 def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600, exe=None):
     """Run the reference's own Fortran (compiled from its sources by
     oracle/ref/build_ref.sh) on `case`.  The reference is built with three
@@ -276,6 +320,8 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
     n1 = p.img * p.jmg * p.kmg
     ov = getattr(case, "overset", None)
     hdr[35] = 0 if ov is None else len(ov.coef)
+    alm = getattr(case, "alm", None)
+    hdr[36] = 0 if alm is None else len(alm.turbines)
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
@@ -288,6 +334,8 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
                 np.ascontiguousarray(ov.hosts, dtype=np.int32).tofile(f)
                 np.ascontiguousarray(ov.coef, dtype=np.float64).tofile(f)
                 np.ascontiguousarray(ov.nodes, dtype=np.int32).tofile(f)
+            if alm is not None:
+                _write_alm(f, td, alm)
         env = dict(os.environ)
         subprocess.run([exe or REF_EXE, fin, fout], check=True, cwd=td, timeout=timeout, env=env,
                        stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)

@@ -20,6 +20,17 @@
 !   then, when hdr(36) = nd > 0, an overset donor table of the fine level
 !   (global_osg.F90:18-28, zone 1 to zone 1 on rank 0):
 !   int32 hosts(3,nd), real64 coef(3,nd), int32 nodes(3,nd)
+!   then, when hdr(37) = nt > 0, the actuator-line tables of module
+!   global_turb (as init_lines builds them, src/init.F90:1156-1710):
+!   int32 th(8,nt): nblades, nelems, nangles, itur1, itur2, n_disk, n_ker, 0
+!   real64 length_scale, vel_infty, (tur_rad, hub_rad, tsr, angle,
+!          dtheta_interp) per turbine
+!   per turbine: int32 node_indexes(nelems,nangles,3),
+!   real64 node_indexes_pos(nelems,nangles,27), real64 disk_dat(jmg,kmg,8),
+!   int32 polar_before(jmg,kmg), polar_after(jmg,kmg) (airfoil files
+!   'pNNN.txt', NNN = index + 1, in the working directory),
+!   int32 disk_nodes(3,n_disk), ker_nodes(3,n_ker), real64 ker_pos(3,n_ker),
+!   ker_dat(3,n_ker) (disk_dat_ker components 1, 2, 4)
 ! modes: 1 = nsteps calls of mg_driver, 2 = one solver_daf on the fine level,
 !        3 = one des_eddy, 4 = metrics only, 5 = one mg_bintp_mom,
 !        6 = one mg_bintp_tur
@@ -38,8 +49,10 @@ program desalm_ref_harness
   integer :: hdr(40)
   real (kind = rdf) :: rhd(20)
   character (len=512) :: fin, fout
-  integer :: mode, nsteps, n, i, j, k, l, m, ib, nall, n1, istep, nd
-  integer, allocatable :: ibuf(:,:)
+  integer :: mode, nsteps, n, i, j, k, l, m, ib, nall, n1, istep, nd, nt, t, c
+  integer, allocatable :: ibuf(:,:), th(:,:), i3(:,:,:), i2(:,:)
+  real (kind = rdf), allocatable :: r2(:,:), r3(:,:,:)
+  real (kind = rdf) :: rt(2)
   integer :: la, lb, u
   real (kind = rdf), allocatable :: buf(:)
 
@@ -160,6 +173,9 @@ program desalm_ref_harness
      nodes(1:3, 1:nd, 0) = ibuf
      deallocate (ibuf)
   end if
+  nt = hdr(37)
+  turbines = nt > 0
+  if (nt > 0) call read_turbines()
   close (u)
 
   ! grid on all levels and metrics, as init does (init.F90:88-110)
@@ -215,6 +231,88 @@ program desalm_ref_harness
   call mpi_finalize(ierr)
 
 contains
+
+  ! the actuator-line tables of module global_turb, with the shapes init
+  ! allocates (src/init.F90:56-86, 204-267)
+  subroutine read_turbines()
+    integer :: mne, mna, mnb, mdk, mkr, jm, km, im
+    character (len = 10) :: nm
+    n_tur = nt
+    allocate (th(8, nt))
+    read (u) th
+    read (u) rt
+    length_scale = rt(1); vel_infty = rt(2)
+    allocate (itur1(nt), itur2(nt), jtur1(nt), jtur2(nt), ktur1(nt), ktur2(nt), tur_rad(nt), nelems(nt), &
+              nblades(nt), nangles(nt), hub_rad(nt), TSR(nt), angle(nt), dtheta(nt), dtheta_interp(nt), &
+              C_T(nt), C_P(nt))
+    allocate (r2(5, nt))
+    read (u) r2
+    do t = 1, nt
+       nblades(t) = th(1,t); nelems(t) = th(2,t); nangles(t) = th(3,t)
+       itur1(t) = th(4,t); itur2(t) = th(5,t)
+       tur_rad(t) = r2(1,t); hub_rad(t) = r2(2,t); TSR(t) = r2(3,t); angle(t) = r2(4,t)
+       dtheta_interp(t) = r2(5,t)
+    end do
+    deallocate (r2)
+    mnb = maxval(nblades); mne = maxval(nelems); mna = maxval(nangles)
+    mdk = maxval(th(6,:)); mkr = maxval(th(7,:))
+    im = img(1,1); jm = jmg(1,1); km = kmg(1,1)
+    allocate (node_indexes(nt, mne, mna, 3), node_indexes_pos(nt, mne, mna, 27), &
+              forces_lines(nt, mnb, mne, 6), disk_dat(jm, km, 8, nt), disk_dat_ker(im, jm, km, 4, nt), &
+              airfoil_before(jm, km, nt), airfoil_after(jm, km, nt), &
+              disk_node_coordinates(mdk + 1, 3, nt), disk_node_coordinates_ker(mkr + 1, 3, nt), &
+              disk_node_coordinates_pos_ker(mkr + 1, 3, nt))
+    node_indexes = 0; node_indexes_pos = zero; forces_lines = zero; disk_dat = zero; disk_dat_ker = zero
+    disk_node_coordinates = 0; disk_node_coordinates_ker = 0; disk_node_coordinates_pos_ker = zero
+    do t = 1, nt
+       allocate (i3(nelems(t), nangles(t), 3), r3(nelems(t), nangles(t), 27))
+       read (u) i3
+       read (u) r3
+       node_indexes(t, 1:nelems(t), 1:nangles(t), :) = i3
+       node_indexes_pos(t, 1:nelems(t), 1:nangles(t), :) = r3
+       deallocate (i3, r3)
+       allocate (r3(jm, km, 8))
+       read (u) r3
+       disk_dat(:, :, :, t) = r3
+       deallocate (r3)
+       allocate (i2(jm, km))
+       read (u) i2
+       do k = 1, km
+          do j = 1, jm
+             write (nm, '(a,i3.3)') 'p', i2(j,k) + 1
+             airfoil_before(j, k, t) = nm
+          end do
+       end do
+       read (u) i2
+       do k = 1, km
+          do j = 1, jm
+             write (nm, '(a,i3.3)') 'p', i2(j,k) + 1
+             airfoil_after(j, k, t) = nm
+          end do
+       end do
+       deallocate (i2)
+       allocate (i2(3, th(6,t)))
+       read (u) i2
+       disk_node_coordinates(1, 1, t) = th(6,t)
+       do c = 1, th(6,t)
+          disk_node_coordinates(c + 1, 1:3, t) = i2(1:3, c)
+       end do
+       deallocate (i2)
+       allocate (i2(3, th(7,t)), r2(3, th(7,t)), r3(3, th(7,t), 1))
+       read (u) i2
+       read (u) r2
+       read (u) r3
+       disk_node_coordinates_ker(1, 1, t) = th(7,t)
+       do c = 1, th(7,t)
+          disk_node_coordinates_ker(c + 1, 1:3, t) = i2(1:3, c)
+          disk_node_coordinates_pos_ker(c + 1, 1:3, t) = r2(1:3, c)
+          disk_dat_ker(i2(1,c), i2(2,c), i2(3,c), 1, t) = r3(1, c, 1)
+          disk_dat_ker(i2(1,c), i2(2,c), i2(3,c), 2, t) = r3(2, c, 1)
+          disk_dat_ker(i2(1,c), i2(2,c), i2(3,c), 4, t) = r3(3, c, 1)
+       end do
+       deallocate (i2, r2, r3)
+    end do
+  end subroutine read_turbines
 
   ! index maps of init_decomp_3d for one rank (init.F90:820-980)
   subroutine setup_index_maps()

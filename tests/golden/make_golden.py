@@ -37,6 +37,13 @@ FIXTURES = {
     "wavy_bintp_tur": ("wavy", dict(ni=13, nj=11, nk=9, ng=2, des=True, overset=40), dict(mode=6)),
     "wavy_mg_driver_overset_sa": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=3, des=True, iters=(1, 2, 3),
                                                overset=60, apply_mom=False), dict(mode=1, nsteps=2)),
+    # actuator-line forcing (rhs_lines in solver_daf, hub zeroing in bcond_fm)
+    "wavy_solver_daf_alm": ("wavy", dict(ni=21, nj=17, nk=17, ng=2, itm=2, amp=0.02,
+                                         alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)),
+                            dict(mode=2, recalc=0, calc_pk=1)),
+    "wavy_mg_driver_alm_sa": ("wavy", dict(ni=21, nj=17, nk=17, ng=2, itm=4, amp=0.02, des=True, iters=(1, 2, 3),
+                                           alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)),
+                              dict(mode=1, nsteps=2)),
     # early exit on the reference's convergence test (mg_ck_conver)
     "wavy_mg_driver_converge": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=12, it_min=2, er_min=-0.15,
                                              iters=(1, 2, 3)), dict(mode=1, nsteps=1)),
@@ -44,7 +51,16 @@ FIXTURES = {
 
 
 def build(kind, kw):
-    return getattr(cases, kind)(**kw)
+    """the case of a fixture recipe; an "alm" entry adds desalm.alm.rotor tables"""
+    kw = dict(kw)
+    rot = kw.pop("alm", None)
+    case = getattr(cases, kind)(**kw)
+    if rot is not None:
+        from desalm import alm
+        r = dict(rot)
+        r["center"] = tuple(r["center"])
+        case.alm = alm.rotor(case, **r)
+    return case
 
 
 def fingerprint(case):
@@ -58,6 +74,16 @@ def fingerprint(case):
         h.update(ov.coef.tobytes())
         h.update(ov.nodes.tobytes())
         h.update(bytes([ov.apply_mom]))
+    al = getattr(case, "alm", None)
+    if al is not None:
+        for T in al.turbines:
+            for a in (T.node_indexes, T.node_indexes_pos, T.disk_dat, T.polar_before, T.polar_after, T.disk_nodes,
+                      T.ker_nodes, T.ker_pos, T.ker_dat):
+                h.update(np.ascontiguousarray(a).tobytes())
+            h.update(np.array([T.tur_rad, T.hub_rad, T.tsr, T.angle, T.dtheta_interp]).tobytes())
+        for a, cl, cd in al.polars:
+            h.update(a.tobytes() + cl.tobytes() + cd.tobytes())
+        h.update(np.array([al.length_scale, al.vel_infty]).tobytes())
     return h.hexdigest()
 
 

@@ -278,3 +278,32 @@ def test_graph_replay_matches_eager(gpu):
         assert np.array_equal(he, hg)
     for a, b in zip(eager.get_state(), graph.get_state()):
         assert np.array_equal(a, b)
+
+
+def alm_case(**kw):
+    from desalm import alm
+    c = cases.wavy(25, 17, 21, ng=2, amp=0.03, seed=13, iters=(1, 2, 3), **kw)
+    c.alm = alm.rotor(c, i_rot=11, center=(0.45, 0.55), radius=0.28, hub=0.05, nblades=2, nelems=6,
+                      nangles=37, angle=1.1, tsr=5.0)
+    return c
+
+
+@pytest.mark.parametrize("recalc", [0, 1])
+def test_solver_daf_alm(gpu, recalc):
+    check_solver_daf_fine(alm_case(itm=2), recalc)
+
+
+@pytest.mark.parametrize("des", [False, True])
+def test_mg_driver_alm(gpu, des):
+    # the forcing runs on every level with the fine-level tables, and
+    # bcond_fm zeroes u on the hub nodes
+    c = alm_case(itm=3, des=des)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    for _ in range(2):
+        assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL

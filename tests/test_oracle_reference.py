@@ -16,7 +16,7 @@ import pytest
 
 from desalm import cases
 import pyoracle as po
-from golden.make_golden import conver_table, fingerprint
+from golden.make_golden import build, conver_table, fingerprint
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
@@ -24,7 +24,7 @@ GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
 
 def _case(meta):
     kw = {k: tuple(v) if isinstance(v, list) else v for k, v in meta["kwargs"].items()}
-    return getattr(cases, meta["kind"])(**kw)
+    return build(meta["kind"], kw)
 
 
 def _run_oracle(case, opts):
@@ -93,6 +93,21 @@ def test_oracle_matches_reference_live(ng, des, bt):
     q = o.get_state()[0]
     assert np.array_equal(q, r["q"])
     assert np.array_equal(o.get_state()[3], r["xnut"])
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="reference harness not built (needs /root/reference)")
+@pytest.mark.parametrize("opts", [dict(mode=2, recalc=0, calc_pk=1), dict(mode=2, recalc=1, calc_pk=1),
+                                  dict(mode=1, nsteps=2)])
+def test_oracle_alm_matches_reference_live(opts):
+    from desalm import alm
+    c = cases.wavy(25, 17, 21, ng=2, itm=3, amp=0.03, des=True, iters=(1, 2, 3), seed=13)
+    c.alm = alm.rotor(c, i_rot=11, center=(0.45, 0.55), radius=0.28, hub=0.05, nblades=2, nelems=6,
+                      nangles=37, angle=1.1, tsr=5.0)
+    o, _ = _run_oracle(c, opts)
+    r = po.run_reference(c, **opts)
+    assert np.array_equal(o.get_state()[0], r["q"])
+    if opts["mode"] == 2:
+        assert np.array_equal(o.get("rh")[:c.n1], r["rh"])
 
 
 @pytest.mark.skipif(not po.ref_available(), reason="reference harness not built (needs /root/reference)")
