@@ -42,7 +42,7 @@ WORDS = {
     # sweeps 11 words each) + vorticity source once (13 words); itk applied below
     "des_eddy": None,
     "prolong": 33,                            # coarse q, qold, q, dq of the three smoothing sweeps
-    "resid_norm": 20,                         # q, qold -> qold (L1 norms)
+    "resid_norm": 15,                         # q, qold -> qold (L1 norms)
 }
 
 
@@ -57,6 +57,109 @@ def level_words(name, n, itk=2):
     return w
 
 
+# kernel-name prefix -> family, for the fine-level dispatches of a counter run
+# (k_prep, k_residual and k_visc_pk are told apart by their place in the pass)
+FAMILY_OF = [("k_sa_", "des_eddy"), ("k_daf_pre<0", "daf_i"), ("k_daf_tri_i", "daf_i"), ("k_daf_pre<1", "daf_j"),
+             ("k_daf_tri<1", "daf_j"), ("k_daf_pre<2", "daf_k"), ("k_daf_tri<2", "daf_k"), ("k_daf_post", "daf_k"),
+             ("k_prolong_fill", "prolong"), ("k_rs", "prolong"), ("k_resid_", "resid_norm"),
+             ("k_bc_fm", "bcond_fm"), ("k_pfix_", "bcond_fm")]
+# one dispatch per family call
+FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_tri_i", "daf_j": "k_daf_tri<1", "daf_k": "k_daf_post",
+               "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_pfix_sub",
+               "dtau": "k_prep", "residual": "k_residual", "residual_recalc": "k_residual"}
+
+
+def pmc_summary(fetch_dir, write_dir, n):
+    """HBM bytes per fine-level launch of every kernel family from two
+    rocprofv3 counter runs of the same command (FETCH_SIZE, WRITE_SIZE, KiB
+    per dispatch).  Each counter is scaled by what it reads for k_copy_qold,
+    which moves exactly 5 n^3 doubles each way.  Writes profiles/traffic.json
+    and prints a markdown table.  Runs on the CPU."""
+    import csv
+    import glob
+
+    def short(name):
+        return name.replace("desalm::(anonymous namespace)::", "").replace("void ", "").split("(")[0].strip()
+
+    def load(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    if r["Counter_Name"] == counter:
+                        rows.append((int(r["Dispatch_Id"]), short(r["Kernel_Name"]), int(r["Grid_Size"]),
+                                     float(r["Counter_Value"])))
+        if not rows:
+            raise SystemExit(f"no {counter} rows under {d}")
+        return sorted(rows)
+
+    fe, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    if [(a[1], a[2]) for a in fe] != [(b[1], b[2]) for b in wr]:
+        raise SystemExit("the two counter runs launched different kernel sequences")
+    n1 = n ** 3
+    copy_kib = 5 * n1 * 8 / 1024.0
+    cf = [v for _, nm, _, v in fe if nm == "k_copy_qold"]
+    cw = [v for _, nm, _, v in wr if nm == "k_copy_qold"]
+    if not cf:
+        raise SystemExit("no k_copy_qold dispatch to calibrate on")
+    sf, sw = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
+    fine_grid = {}
+    for _, nm, g, _ in fe:
+        fine_grid[nm] = max(fine_grid.get(nm, 0), g)
+    fam = {}
+    kern = {}
+    after_daf = False
+    for (_, nm, g, vf), (_, _, _, vw) in zip(fe, wr):
+        rd, wt = vf * sf * 1024.0, vw * sw * 1024.0
+        fine = g == fine_grid[nm]
+        f = next((fm for pre, fm in FAMILY_OF if nm.startswith(pre)), None)
+        if nm in ("k_prep", "k_residual", "k_visc_pk"):
+            f = ("residual_recalc" if after_daf else ("dtau" if nm == "k_prep" else "residual")) if fine else None
+        elif f not in ("bcond_fm",) and not nm.startswith("__amd"):
+            after_daf = False
+        if nm.startswith("k_daf_post") and fine:
+            after_daf = True
+        if f is None or not (fine or nm == "k_resid_final"):
+            continue
+        e = fam.setdefault(f, {"calls": 0, "read": 0.0, "write": 0.0})
+        if nm.startswith(FAMILY_MARK[f]):
+            e["calls"] += 1
+        e["read"] += rd
+        e["write"] += wt
+        k = kern.setdefault(f"{nm}@{f}", [0, 0.0, 0.0])
+        k[0] += 1
+        k[1] += rd
+        k[2] += wt
+    fams = {}
+    for f, e in fam.items():
+        c = max(e["calls"], 1)
+        fams[f"{f}@1"] = {"hbm_bytes_per_launch": int(round((e["read"] + e["write"]) / c)),
+                          "read_bytes": int(round(e["read"] / c)), "write_bytes": int(round(e["write"] / c)),
+                          "calls": e["calls"]}
+    out = {"workload_n": n, "calibration": {"kernel": "k_copy_qold", "bytes_each_way": 5 * n1 * 8,
+                                            "fetch_scale": round(sf, 4), "write_scale": round(sw, 4)},
+           "families": fams,
+           "kernels": {k: {"dispatches": c, "read_bytes": int(r / c), "write_bytes": int(w / c)}
+                       for k, (c, r, w) in kern.items()}}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"calibration on k_copy_qold ({5 * n1 * 8 / 1e6:.1f} MB each way): FETCH_SIZE x {sf:.4f}, "
+          f"WRITE_SIZE x {sw:.4f}\n")
+    print("| family (fine level) | calls | read MB | write MB | HBM MB / launch | algorithmic MB | ratio |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for key, e in sorted(fams.items(), key=lambda kv: -kv[1]["hbm_bytes_per_launch"]):
+        f = key.split("@")[0]
+        hbm = e["hbm_bytes_per_launch"]
+        alg = f"{level_words(f, 1) * 8 * n1 / 1e6:.1f}" if f in WORDS else ""
+        ratio = f"{hbm / (level_words(f, 1) * 8 * n1):.2f}" if f in WORDS else ""
+        print(f"| {f} | {e['calls']} | {e['read_bytes'] / 1e6:.1f} | {e['write_bytes'] / 1e6:.1f} | "
+              f"{hbm / 1e6:.1f} | {alg} | {ratio} |")
+    print("\n| kernel @ family (fine level) | dispatches | read MB | write MB |")
+    print("|---|---:|---:|---:|")
+    for k, v in sorted(out["kernels"].items(), key=lambda kv: -(kv[1]["read_bytes"] + kv[1]["write_bytes"])):
+        print(f"| `{k}` | {v['dispatches']} | {v['read_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +172,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--conv-itm", type=int, default=600,
                     help="pseudo-iteration cap of the convergence leg (0 skips it)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no run: summarise two rocprofv3 counter runs into profiles/traffic.json")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.n)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)

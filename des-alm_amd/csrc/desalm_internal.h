@@ -17,6 +17,17 @@ namespace desalm {
 
 constexpr int kMaxLevels = DESALM_MAX_LEVELS;
 
+// Workgroups are handed to the 8 XCDs round robin (workgroup b runs on XCD
+// b % 8), and every XCD has its own L2.  xcd_block(b, nb) renumbers the nb
+// workgroups of a launch so that XCD x runs the x-th contiguous eighth of the
+// logical grid, in order: a stencil kernel then finds the neighbouring rows
+// and planes of its nodes in its own L2 instead of fetching them on every
+// XCD.  A bijection of [0, nb).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned x = b % 8u, l = b / 8u, q = nb / 8u, r = nb % 8u;
+    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + l;
+}
+
 struct Level {
     int ni, nj, nk;
     long off;   // node offset of the level in all-level arrays

@@ -31,7 +31,7 @@ __device__ __forceinline__ double sqr(double a) { return a * a; }
 // nodes interior in that direction, into F.fv (level-local): the residual
 // then reads it at a node and its two neighbours.
 __global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
     const double c1 = F.csi[0][g], c2 = F.csi[1][g], c3 = F.csi[2][g];
@@ -169,7 +169,7 @@ __device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, 
 // (src/solver_daf.F90:432-484), which reuses the stored diss(1) of the
 // pre-update q and dtau.
 __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni);
@@ -273,9 +273,12 @@ __global__ void __launch_bounds__(kVX * kVY) k_visc_pk(Level L, Fields F, Consts
     __shared__ double s_fj[3][kVY][kVX];
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int tx = (int)threadIdx.x % kVX, ty = (int)threadIdx.x / kVX;
-    const int i = (int)blockIdx.x * (kVX - 1) + tx;  // node 0-based; tile outputs start at i = 1
-    const int j = (int)blockIdx.y * (kVY - 1) + ty;
-    const int k0 = 1 + (int)blockIdx.z * kVK;
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+    const int bx = (int)(lb % gx), by = (int)((lb / gx) % gy), bz = (int)(lb / (gx * gy));
+    const int i = bx * (kVX - 1) + tx;  // node 0-based; tile outputs start at i = 1
+    const int j = by * (kVY - 1) + ty;
+    const int k0 = 1 + bz * kVK;
     const int k1 = min(k0 + kVK, nk - 1);
     const bool in_i = i <= ni - 2, in_j = j <= nj - 2;
     const bool out = tx >= 1 && ty >= 1 && in_i && in_j;

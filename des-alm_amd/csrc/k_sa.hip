@@ -6,8 +6,8 @@
 // Kernels:
 //   k_sa_prep   U/J (stored, rhs_contra_j) and wo (rhs_sa_source)
 //   k_sa_rhs    rhs_sa_grdnu, rhs_sa_prod, rhs_sa_viscous, rhs_sa_flux,
-//               rhs_sa_unst_visc, rhs_sa_implicit_source, boundary zeroing
-//   k_sa_adi_pre the ADI operators' face coefficients and dtev J / 2
+//               rhs_sa_unst_visc, rhs_sa_implicit_source, boundary zeroing,
+//               and the ADI operators' face coefficients and dtev J / 2
 //   k_sa_adi_i  rhs_sa_adi_solver sweep in i (rows staged through LDS)
 //   k_sa_adi<D> the sweeps in j and k (D = 2 also applies the update of
 //               des_eddy.F90:260-272)
@@ -103,11 +103,41 @@ __device__ __forceinline__ double sa_conv_face(const Level& L, const Fields& F, 
     return 0.5 * (q1 * u1 + q0 * u0 - eigh * (q1 - q0));
 }
 
+// Face and node coefficients of the three ADI operators
+// (rhs_sa_adi_solver.F90:53-75 and copies), for every fine node:
+//   av_D = |U/J face average| + (1/Re + nu~ face average)/sig * |grad csi_D|^2_face / (J + J')
+// on the face between the node and its +D neighbour, and dtj = dtev J / 2.
+// nu~ does not change between the three sweeps of one sub-iteration, so
+// k_sa_rhs computes them while it holds the node's metrics.
+__device__ __forceinline__ void sa_adi_coef(const Level& L, const Fields& F, const Consts& C, long t, int i, int j,
+                                            int k) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const long o = L.off;
+    const double* q5 = F.q[4];
+    F.dtj[t] = 0.5 * F.dtev[o + t] * F.aj[o + t];
+    const int pos[3] = {i, j, k};
+    const int nd[3] = {ni, nj, nk};
+    const long st[3] = {1, (long)ni, (long)ni * nj};
+#pragma unroll
+    for (int D = 0; D < 3; D++) {
+        if (pos[D] >= nd[D] - 1) continue;
+        const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
+        const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
+        const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
+        const long p = t, p1 = t + st[D];
+        const double gx = m0[o + p1] + m0[o + p], gy = m1[o + p1] + m1[o + p], gz = m2[o + p1] + m2[o + p];
+        const double ev = (C.rei + 0.5 * (q5[o + p1] + q5[o + p])) / sig;
+        const double ucna = 0.5 * fabs(F.ucn[D][p1] + F.ucn[D][p]);
+        F.av[D][t] = ucna + ev * (gx * gx + gy * gy + gz * gz) / (F.aj[o + p1] + F.aj[o + p]);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_sa_rhs(Level L, Fields F, Consts C) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    sa_adi_coef(L, F, C, t, i, j, k);
     if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
         F.rh5[t] = 0.0;  // des_eddy.F90:231-236
         return;
@@ -174,36 +204,6 @@ __global__ void __launch_bounds__(256) k_sa_rhs(Level L, Fields F, Consts C) {
     r = r + dq5dt - visc - psa / F.aj[g];
     // explicit source form (rhs_sa_implicit_source.F90:244-256, dk = 1)
     F.rh5[t] = -F.dtev[g] * F.aj[g] * r;
-}
-
-// Face and node coefficients of the three ADI operators
-// (rhs_sa_adi_solver.F90:53-75 and copies), for every fine node:
-//   av_D = |U/J face average| + (1/Re + nu~ face average)/sig * |grad csi_D|^2_face / (J + J')
-// on the face between the node and its +D neighbour, and dtj = dtev J / 2.
-// nu~ does not change between the three sweeps of one sub-iteration.
-__global__ void __launch_bounds__(256) k_sa_adi_pre(Level L, Fields F, Consts C) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
-    const long o = L.off;
-    const double* q5 = F.q[4];
-    F.dtj[t] = 0.5 * F.dtev[o + t] * F.aj[o + t];
-    const int pos[3] = {i, j, k};
-    const int nd[3] = {ni, nj, nk};
-    const long st[3] = {1, (long)ni, (long)ni * nj};
-#pragma unroll
-    for (int D = 0; D < 3; D++) {
-        if (pos[D] >= nd[D] - 1) continue;
-        const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
-        const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
-        const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
-        const long p = t, p1 = t + st[D];
-        const double gx = m0[o + p1] + m0[o + p], gy = m1[o + p1] + m1[o + p], gz = m2[o + p1] + m2[o + p];
-        const double ev = (C.rei + 0.5 * (q5[o + p1] + q5[o + p])) / sig;
-        const double ucna = 0.5 * fabs(F.ucn[D][p1] + F.ucn[D][p]);
-        F.av[D][t] = ucna + ev * (gx * gx + gy * gy + gz * gz) / (F.aj[o + p1] + F.aj[o + p]);
-    }
 }
 
 // one ADI sweep (rhs_sa_adi_solver.F90:44-244), one thread per line:
@@ -544,7 +544,6 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
     for (int it = 0; it < itk; it++) {
         hipLaunchKernelGGL(k_sa_rhs, dim3(gn), dim3(b), 0, s, L, F, C);
-        hipLaunchKernelGGL(k_sa_adi_pre, dim3(gn), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(kSNT), 0, s, L, F);
         hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
