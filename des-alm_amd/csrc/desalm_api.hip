@@ -40,6 +40,35 @@ struct Profiler {
     long long count[KID_COUNT][kMaxLevels + 1] = {};
 };
 
+// Learned warm start state (desalm_ws_*): the sample buffer, the network's
+// packed parameters with their Adam moments, the normalisation statistics
+// and the scratch matrices of training and inference.  Parameter layout:
+// W1 (kWsIn + 1) x nh, W2 (nh + 1) x nh, W3 (nh + 1) x kWsOut, row-major,
+// the last row of each the bias.
+struct WarmStart {
+    int mode = DESALM_WS_NONE;
+    int nh = 0;
+    long per_step = 0, cap = 0, rows = 0, records = 0;
+    unsigned long long seed = 0;
+    WsQ q2{{nullptr, nullptr, nullptr, nullptr, nullptr}};  // q^(n-2), fine level
+    float* X = nullptr;   // cap x (kWsIn + 1) raw features
+    float* T = nullptr;   // cap x kWsTgt raw targets (r, d1, d2)
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    long adam_t = 0;
+    float* stats = nullptr;  // kWsStats: feature mu, 1/sd (kWsIn each), target scale (kWsComp), unused
+    bool have_stats = false;
+    int bcap = 0;  // rows of the scratch matrices
+    float *A0 = nullptr, *A1 = nullptr, *A2 = nullptr, *Y = nullptr, *Tb = nullptr;
+    float *dZ3 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *P = nullptr;
+    double* lpart = nullptr;  // loss partial sums, kWsLossParts per training iteration
+    long lcap = 0;
+    std::vector<void*> dev;
+    int np() const { return (kWsIn + 1) * nh + (nh + 1) * nh + (nh + 1) * kWsOut; }
+    int o2() const { return (kWsIn + 1) * nh; }
+    int o3() const { return (kWsIn + 1) * nh + (nh + 1) * nh; }
+};
+constexpr int kWsLossParts = 64;
+
 struct desalm_ctx {
     desalm_params p;
     int device = 0;
@@ -65,6 +94,7 @@ struct desalm_ctx {
     int ov_mom = 0;
     double cvin = 0.0;
     Alm alm;  // actuator-line rotors (desalm_set_turbines)
+    WarmStart ws;
 };
 
 namespace {
@@ -419,6 +449,10 @@ int desalm_set_state(desalm_ctx* c, const double* q, const double* qn, const dou
     if (copy_in_level(c, c->F.qn, 5, qn, L)) return -1;
     if (copy_in_level(c, c->F.qnm1, 5, qnm1, L)) return -1;
     if (xnut) HIPCHK(c, hipMemcpyAsync(c->F.xnut, xnut, sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
+    for (int m = 0; m < 5; m++)
+        if (c->ws.q2.q[m])
+            HIPCHK(c, hipMemcpyAsync(c->ws.q2.q[m], c->F.qnm1[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice,
+                                     c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -436,6 +470,94 @@ int desalm_get_state(desalm_ctx* c, double* q, double* qn, double* qnm1, double*
     }
     return 0;
 }
+
+// ---- learned warm start (host side; kernels in k_mlp.hip) ----
+extern "C++" {
+namespace {
+
+void ws_free(WarmStart& w) {
+    for (void* p : w.dev) (void)hipFree(p);
+    w = WarmStart();
+}
+
+template <typename T>
+T* ws_alloc(desalm_ctx* c, long n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(T)) != hipSuccess) {
+        fail(c, "warm start: out of device memory");
+        return nullptr;
+    }
+    c->ws.dev.push_back(p);
+    if (hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(T)) != hipSuccess) {
+        fail(c, "warm start: hipMemset failed");
+        return nullptr;
+    }
+    return (T*)p;
+}
+
+// scratch matrices for m rows (grown, never shrunk)
+int ws_scratch(desalm_ctx* c, int m) {
+    WarmStart& w = c->ws;
+    if (m <= w.bcap) return 0;
+    const int nh = w.nh;
+    w.A0 = ws_alloc<float>(c, (long)m * (kWsIn + 1));
+    w.A1 = ws_alloc<float>(c, (long)m * (nh + 1));
+    w.A2 = ws_alloc<float>(c, (long)m * (nh + 1));
+    w.Y = ws_alloc<float>(c, (long)m * kWsOut);
+    w.Tb = ws_alloc<float>(c, (long)m * kWsTgt);
+    w.dZ3 = ws_alloc<float>(c, (long)m * kWsOut);
+    w.dZ2 = ws_alloc<float>(c, (long)m * nh);
+    w.dZ1 = ws_alloc<float>(c, (long)m * nh);
+    w.P = ws_alloc<float>(c, (long)wgrad_parts(m) * (kWsIn + 1) * nh);
+    if (!w.A0 || !w.A1 || !w.A2 || !w.Y || !w.Tb || !w.dZ3 || !w.dZ2 || !w.dZ1 || !w.P) return -1;
+    w.bcap = m;
+    return 0;
+}
+
+// forward pass of m rows already in A0
+void ws_forward(desalm_ctx* c, hipStream_t s, int m) {
+    WarmStart& w = c->ws;
+    const int nh = w.nh;
+    launch_dense(s, w.A0, kWsIn + 1, kWsIn + 1, w.w, nh, w.A1, nh + 1, m, 1, 1);
+    launch_dense(s, w.A1, nh + 1, nh + 1, w.w + w.o2(), nh, w.A2, nh + 1, m, 1, 1);
+    launch_dense(s, w.A2, nh + 1, nh + 1, w.w + w.o3(), kWsOut, w.Y, kWsOut, m, 0, 0);
+}
+
+// loss partials into lp, gradient into w.g, of m rows (A0 and Tb filled)
+void ws_backward(desalm_ctx* c, hipStream_t s, int m, double* lp) {
+    WarmStart& w = c->ws;
+    const int nh = w.nh;
+    launch_ws_loss(s, w.Y, w.Tb, m, w.dZ3, lp, kWsLossParts);
+    launch_wgrad(s, w.A2, nh + 1, nh + 1, w.dZ3, kWsOut, kWsOut, m, w.P, w.g + w.o3());
+    launch_dense_bwd(s, w.dZ3, kWsOut, kWsOut, w.w + w.o3(), nh, w.A2, nh + 1, w.dZ2, nh, m);
+    launch_wgrad(s, w.A1, nh + 1, nh + 1, w.dZ2, nh, nh, m, w.P, w.g + w.o2());
+    launch_dense_bwd(s, w.dZ2, nh, nh, w.w + w.o2(), nh, w.A1, nh + 1, w.dZ1, nh, m);
+    launch_wgrad(s, w.A0, kWsIn + 1, kWsIn + 1, w.dZ1, nh, nh, m, w.P, w.g);
+}
+
+// the warm start of one zone, after the time-level shift
+int ws_start(desalm_ctx* c, hipStream_t s) {
+    WarmStart& w = c->ws;
+    const Level& L = c->lv[1];
+    const long nint = (long)(L.ni - 2) * (L.nj - 2) * (L.nk - 2);
+    if (w.mode == DESALM_WS_EXTRAP) {
+        launch_ws_apply(s, L, c->F, w.q2, nullptr, 0, nint);
+        return 0;
+    }
+    if (!w.have_stats) return fail(c, "learned warm start: the network has not been trained (desalm_ws_train)");
+    const int chunk = 1 << 17;
+    if (ws_scratch(c, chunk)) return -1;
+    for (long r0 = 0; r0 < nint; r0 += chunk) {
+        const int m = (int)std::min((long)chunk, nint - r0);
+        launch_ws_features(s, L, c->F, w.q2, w.A0, r0, m, w.stats);
+        ws_forward(c, s, m);
+        launch_ws_apply(s, L, c->F, w.q2, w.Y, r0, m);
+    }
+    return 0;
+}
+
+}  // namespace
+}  // extern "C++"
 
 // One physical step of FAS V-cycles (mg_driver.F90:2-323) over one or more
 // zones that share a stream: zones[0]'s stream carries every launch.  The
@@ -456,13 +578,21 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     for (desalm_ctx* c : zs) {
         const Level& L1 = c->lv[1];
         HIPCHK(c0, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
-        // qnm1 <- qn, qn <- q on the fine level (mg_driver.F90:48-51)
+        // qnm1 <- qn, qn <- q on the fine level (mg_driver.F90:48-51), and
+        // q^(n-2) <- qnm1 first when a warm start keeps it
         for (int m = 0; m < 5; m++) {
+            if (c->ws.q2.q[m])
+                HIPCHK(c0, hipMemcpyAsync(c->ws.q2.q[m], c->F.qnm1[m], sizeof(double) * c->n1,
+                                          hipMemcpyDeviceToDevice, s));
             HIPCHK(c0, hipMemcpyAsync(c->F.qnm1[m], c->F.qn[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
             HIPCHK(c0, hipMemcpyAsync(c->F.qn[m], c->F.q[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, s));
         }
         launch_inject(s, c->lv, ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.qn, 5);
         launch_inject(s, c->lv, ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.qnm1, 5);
+        if (c->ws.mode != DESALM_WS_NONE && ws_start(c, s)) {
+            if (c != c0) c0->err = c->err;
+            return -1;
+        }
         launch_copy_qold(s, L1, c->F);
     }
     HIPCHK(c0, hipGetLastError());
@@ -1102,6 +1232,313 @@ int desalm_bintp(desalm_ctx* c, int which) {
     return 0;
 }
 
+int desalm_set_convergence(desalm_ctx* c, int itm, int it_min, double er_min, double eo_min) {
+    if (!c) return fail(c, "null argument");
+    if (itm < 0 || it_min < 0) return fail(c, "itm and it_min must be non-negative");
+    c->p.itm = itm;
+    c->p.it_min = it_min;
+    c->p.er_min = er_min;
+    c->p.eo_min = eo_min;
+    return 0;
+}
+
+int desalm_step(desalm_ctx* c, int* itc_out, double* resid_out) {
+    if (!c) return fail(c, "null argument");
+    // blade angles (main.F90:42-49); dtheta = TSR delti length_scale / tur_rad (init.F90:1447)
+    const double two_pi = 2.0 * 3.141592653589793;
+    for (AlmTurbine& T : c->alm.t) {
+        if (T.angle > two_pi) T.angle = T.angle - two_pi;
+        T.angle = T.angle + T.tsr * c->p.delti * c->alm.length_scale / T.tur_rad;
+    }
+    return desalm_mg_driver(c, itc_out, resid_out);
+}
+
+int desalm_ws_configure(desalm_ctx* c, int hidden, int64_t per_record, int64_t capacity, uint64_t seed) {
+    if (!c) return fail(c, "null argument");
+    if (hidden < 1 || hidden > kWsMaxHidden) return fail(c, "warm start: hidden must be in 1..64");
+    if (per_record < 1 || capacity < per_record) return fail(c, "warm start: need 1 <= samples_per_record <= capacity");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    ws_free(c->ws);
+    WarmStart& w = c->ws;
+    w.nh = hidden;
+    w.per_step = per_record;
+    w.cap = capacity;
+    w.seed = seed;
+    for (int m = 0; m < 5; m++) {
+        w.q2.q[m] = ws_alloc<double>(c, c->n1);
+        if (!w.q2.q[m]) return -1;
+        HIPCHK(c, hipMemcpyAsync(w.q2.q[m], c->F.qnm1[m], sizeof(double) * c->n1, hipMemcpyDeviceToDevice, c->stream));
+    }
+    w.X = ws_alloc<float>(c, capacity * (kWsIn + 1));
+    w.T = ws_alloc<float>(c, capacity * kWsTgt);
+    const int np = w.np();
+    w.w = ws_alloc<float>(c, np);
+    w.g = ws_alloc<float>(c, np);
+    w.m = ws_alloc<float>(c, np);
+    w.v = ws_alloc<float>(c, np);
+    w.stats = ws_alloc<float>(c, kWsStats);
+    if (!w.X || !w.T || !w.w || !w.g || !w.m || !w.v || !w.stats) return -1;
+    // Xavier-uniform hidden layers from a splitmix64 stream, zero output layer
+    std::vector<float> h(np, 0.0f);
+    unsigned long long st = seed;
+    auto uni = [&]() {
+        unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int nh = hidden;
+    const double a1 = sqrt(6.0 / (kWsIn + nh)), a2 = sqrt(6.0 / (nh + nh));
+    for (int k = 0; k < kWsIn; k++)
+        for (int n = 0; n < nh; n++) h[(size_t)k * nh + n] = (float)((2.0 * uni() - 1.0) * a1);
+    for (int k = 0; k < nh; k++)
+        for (int n = 0; n < nh; n++) h[(size_t)w.o2() + (size_t)k * nh + n] = (float)((2.0 * uni() - 1.0) * a2);
+    HIPCHK(c, hipMemcpy(w.w, h.data(), sizeof(float) * np, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_ws_mode(desalm_ctx* c, int mode) {
+    if (!c) return fail(c, "null argument");
+    if (mode < DESALM_WS_NONE || mode > DESALM_WS_LEARNED) return fail(c, "warm start: unknown mode");
+    if (mode != DESALM_WS_NONE && !c->ws.q2.q[0]) return fail(c, "warm start: call desalm_ws_configure first");
+    c->ws.mode = mode;
+    return 0;
+}
+
+int desalm_ws_record(desalm_ctx* c) {
+    if (!c) return fail(c, "null argument");
+    WarmStart& w = c->ws;
+    if (!w.q2.q[0]) return fail(c, "warm start: call desalm_ws_configure first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const long at = (w.records * w.per_step) % w.cap;
+    const long m = std::min(w.per_step, w.cap - at);
+    launch_ws_sample(c->stream, c->lv[1], c->F, w.q2, w.X + at * (kWsIn + 1), w.T + at * kWsTgt, m, w.seed,
+                     w.records);
+    HIPCHK(c, hipGetLastError());
+    w.records++;
+    w.rows = std::min(w.cap, w.rows + m);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_ws_train(desalm_ctx* c, int iters, int batch, double lr, double* loss_out) {
+    if (!c) return fail(c, "null argument");
+    WarmStart& w = c->ws;
+    if (!w.q2.q[0]) return fail(c, "warm start: call desalm_ws_configure first");
+    if (w.rows < 1) return fail(c, "warm start: no samples recorded (desalm_ws_record)");
+    if (iters < 0 || batch < 1) return fail(c, "warm start: iters >= 0 and batch >= 1 required");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (ws_scratch(c, batch)) return -1;
+    if (!w.have_stats) {
+        // per-column mean and 1/sd of the buffer's features (a column whose
+        // sd is below 1e-30 of its scale is dropped), and per component the
+        // root mean square of d1 as the target scale (1 when it is zero)
+        const int nparts = 64;
+        double* part = ws_alloc<double>(c, (long)nparts * kWsIn * 2);
+        if (!part) return -1;
+        std::vector<double> hp((size_t)nparts * kWsIn * 2);
+        std::vector<float> st(kWsStats, 0.0f);
+        for (int pass = 0; pass < 2; pass++) {
+            const int nc = pass == 0 ? kWsIn : kWsTgt;
+            launch_colstats(s, pass == 0 ? w.X : w.T, pass == 0 ? kWsIn + 1 : kWsTgt, nc, w.rows, part, nparts);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(hp.data(), part, sizeof(double) * nparts * nc * 2, hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            for (int col = 0; col < nc; col++) {
+                double s1 = 0.0, s2 = 0.0;
+                for (int b = 0; b < nparts; b++) {
+                    s1 += hp[((size_t)b * nc + col) * 2];
+                    s2 += hp[((size_t)b * nc + col) * 2 + 1];
+                }
+                const double mu = s1 / (double)w.rows;
+                if (pass == 0) {
+                    const double sd = sqrt(std::max(0.0, s2 / (double)w.rows - mu * mu));
+                    st[col] = (float)mu;
+                    st[kWsIn + col] = sd > 1e-30 * std::max(1.0, fabs(mu)) ? (float)(1.0 / sd) : 0.0f;
+                } else if (col >= kWsComp && col < 2 * kWsComp) {
+                    const double rms = sqrt(s2 / (double)w.rows);
+                    st[2 * kWsIn + col - kWsComp] = rms > 0.0 ? (float)rms : 1.0f;
+                }
+            }
+        }
+        HIPCHK(c, hipMemcpy(w.stats, st.data(), sizeof(float) * st.size(), hipMemcpyHostToDevice));
+        w.have_stats = true;
+    }
+    if (w.lcap < (long)iters * kWsLossParts) {
+        w.lpart = ws_alloc<double>(c, (long)iters * kWsLossParts);
+        if (!w.lpart) return -1;
+        w.lcap = (long)iters * kWsLossParts;
+    }
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int it = 0; it < iters; it++) {
+        launch_ws_batch(s, w.X, w.T, w.rows, w.A0, w.Tb, batch, w.seed, w.adam_t, w.stats);
+        ws_forward(c, s, batch);
+        ws_backward(c, s, batch, w.lpart + (long)it * kWsLossParts);
+        w.adam_t++;
+        const float c1 = 1.0f - powf(b1, (float)w.adam_t), c2 = 1.0f - powf(b2, (float)w.adam_t);
+        launch_adam(s, w.w, w.g, w.m, w.v, w.np(), (float)lr, b1, b2, 1e-8f, c1, c2);
+    }
+    HIPCHK(c, hipGetLastError());
+    if (loss_out && iters > 0) {
+        std::vector<double> lp((size_t)iters * kWsLossParts);
+        HIPCHK(c, hipMemcpyAsync(lp.data(), w.lpart, sizeof(double) * lp.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int it = 0; it < iters; it++) {
+            double t = 0.0;
+            for (int b = 0; b < kWsLossParts; b++) t += lp[(size_t)it * kWsLossParts + b];
+            loss_out[it] = t / ((double)batch * kWsComp);
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int desalm_ws_history(desalm_ctx* c, double* get, const double* set) {
+    if (!c) return fail(c, "null argument");
+    if (!c->ws.q2.q[0]) return fail(c, "warm start: call desalm_ws_configure first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->lv[1];
+    if (copy_out_level(c, c->ws.q2.q, 5, get, L)) return -1;
+    if (copy_in_level(c, c->ws.q2.q, 5, set, L)) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_turbine_angle(desalm_ctx* c, int t, double* get, const double* set) {
+    if (!c) return fail(c, "null argument");
+    if (t < 0 || t >= (int)c->alm.t.size()) return fail(c, "no such turbine");
+    if (get) *get = c->alm.t[t].angle;
+    if (set) c->alm.t[t].angle = *set;
+    return 0;
+}
+
+int desalm_ws_param_count(const desalm_ctx* c) { return c && c->ws.nh > 0 ? c->ws.np() : 0; }
+
+int desalm_ws_params(desalm_ctx* c, float* get, const float* set) {
+    if (!c) return fail(c, "null argument");
+    if (!c->ws.w) return fail(c, "warm start: call desalm_ws_configure first");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (get) HIPCHK(c, hipMemcpy(get, c->ws.w, sizeof(float) * c->ws.np(), hipMemcpyDeviceToHost));
+    if (set) HIPCHK(c, hipMemcpy(c->ws.w, set, sizeof(float) * c->ws.np(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int desalm_ws_stats(desalm_ctx* c, float* get, const float* set) {
+    if (!c) return fail(c, "null argument");
+    if (!c->ws.stats) return fail(c, "warm start: call desalm_ws_configure first");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = kWsStats;
+    if (get) HIPCHK(c, hipMemcpy(get, c->ws.stats, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (set) {
+        HIPCHK(c, hipMemcpy(c->ws.stats, set, sizeof(float) * n, hipMemcpyHostToDevice));
+        c->ws.have_stats = true;
+    }
+    return 0;
+}
+
+int desalm_ws_features(desalm_ctx* c, int64_t m, float* x, float* t) {
+    if (!c || !x) return fail(c, "null argument");
+    WarmStart& w = c->ws;
+    if (!w.q2.q[0]) return fail(c, "warm start: call desalm_ws_configure first");
+    const Level& L = c->lv[1];
+    if (m < 0 || m > (long)(L.ni - 2) * (L.nj - 2) * (L.nk - 2)) return fail(c, "warm start: m out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    float* X = nullptr;
+    HIPCHK(c, hipMalloc(&X, sizeof(float) * (size_t)std::max<int64_t>(m, 1) * (kWsIn + 1)));
+    // raw rows: the inference features kernel without statistics
+    launch_ws_features(s, L, c->F, w.q2, X, 0, m, nullptr);
+    std::vector<float> hx((size_t)m * (kWsIn + 1));
+    const hipError_t e1 = hipMemcpyAsync(hx.data(), X, sizeof(float) * hx.size(), hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(X);
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(c, "warm start: feature read-back failed");
+    for (long r = 0; r < m; r++)
+        for (int f = 0; f < kWsIn; f++) x[r * kWsIn + f] = hx[(size_t)r * (kWsIn + 1) + f];
+    if (t) {
+        // targets of the same ordinals: r = q - 2 q^n + q^(n-1), d1, d2
+        std::vector<double> q((size_t)c->n1), qn((size_t)c->n1), qm((size_t)c->n1), q2((size_t)c->n1);
+        for (int comp = 0; comp < kWsComp; comp++) {
+            HIPCHK(c, hipMemcpy(q.data(), c->F.q[comp], sizeof(double) * c->n1, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(qn.data(), c->F.qn[comp], sizeof(double) * c->n1, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(qm.data(), c->F.qnm1[comp], sizeof(double) * c->n1, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(q2.data(), w.q2.q[comp], sizeof(double) * c->n1, hipMemcpyDeviceToHost));
+            const long ni = L.ni - 2, nj = L.nj - 2;
+            for (long r = 0; r < m; r++) {
+                const long i = 1 + r % ni, j = 1 + (r / ni) % nj, k = 1 + r / (ni * nj);
+                const long g = i + (long)L.ni * (j + (long)L.nj * k);
+                t[r * kWsTgt + comp] = (float)(q[g] - 2.0 * qn[g] + qm[g]);
+                t[r * kWsTgt + kWsComp + comp] = (float)(qn[g] - qm[g]);
+                t[r * kWsTgt + 2 * kWsComp + comp] = (float)(qm[g] - q2[g]);
+            }
+        }
+    }
+    return 0;
+}
+
+int desalm_mlp_forward(desalm_ctx* c, const float* x, int64_t m, float* y) {
+    if (!c || !x || !y) return fail(c, "null argument");
+    WarmStart& w = c->ws;
+    if (!w.w) return fail(c, "warm start: call desalm_ws_configure first");
+    if (m < 1 || m > (1 << 24)) return fail(c, "mlp: m must be in 1..2^24");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (ws_scratch(c, (int)m)) return -1;
+    std::vector<float> a((size_t)m * (kWsIn + 1));
+    for (long r = 0; r < m; r++) {
+        for (int f = 0; f < kWsIn; f++) a[(size_t)r * (kWsIn + 1) + f] = x[r * kWsIn + f];
+        a[(size_t)r * (kWsIn + 1) + kWsIn] = 1.0f;
+    }
+    HIPCHK(c, hipMemcpyAsync(w.A0, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, s));
+    ws_forward(c, s, (int)m);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(y, w.Y, sizeof(float) * m * kWsOut, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int desalm_mlp_loss_grad(desalm_ctx* c, const float* x, const float* t, int64_t m, float* grad, double* loss) {
+    if (!c || !x || !t || !grad) return fail(c, "null argument");
+    WarmStart& w = c->ws;
+    if (!w.w) return fail(c, "warm start: call desalm_ws_configure first");
+    if (m < 1 || m > (1 << 24)) return fail(c, "mlp: m must be in 1..2^24");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (ws_scratch(c, (int)m)) return -1;
+    if (w.lcap < kWsLossParts) {
+        w.lpart = ws_alloc<double>(c, kWsLossParts);
+        if (!w.lpart) return -1;
+        w.lcap = kWsLossParts;
+    }
+    double* lp = w.lpart;
+    std::vector<float> a((size_t)m * (kWsIn + 1));
+    for (long r = 0; r < m; r++) {
+        for (int f = 0; f < kWsIn; f++) a[(size_t)r * (kWsIn + 1) + f] = x[r * kWsIn + f];
+        a[(size_t)r * (kWsIn + 1) + kWsIn] = 1.0f;
+    }
+    HIPCHK(c, hipMemcpyAsync(w.A0, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(w.Tb, t, sizeof(float) * m * kWsTgt, hipMemcpyHostToDevice, s));
+    ws_forward(c, s, (int)m);
+    ws_backward(c, s, (int)m, lp);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(grad, w.g, sizeof(float) * w.np(), hipMemcpyDeviceToHost, s));
+    double hl[kWsLossParts];
+    HIPCHK(c, hipMemcpyAsync(hl, lp, sizeof(hl), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (loss) {
+        double tsum = 0.0;
+        for (int b = 0; b < kWsLossParts; b++) tsum += hl[b];
+        *loss = tsum / ((double)m * kWsComp);
+    }
+    return 0;
+}
+
 int desalm_finalize(desalm_ctx* c) {
     if (!c) return 0;
     // teardown: errors here leave nothing to recover, so they are ignored
@@ -1109,6 +1546,7 @@ int desalm_finalize(desalm_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_overset(c);
     free_alm(c);
+    ws_free(c->ws);
     prof_flush(c);
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);

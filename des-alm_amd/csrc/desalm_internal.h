@@ -138,6 +138,23 @@ struct Alm {
     std::vector<void*> dev;  // device allocations of the tables
 };
 
+// Learned warm start (k_mlp.hip): per-node features are the last two time
+// differences of the five variables at the node and its six face
+// neighbours (each variable's 14 divided by their root mean square), plus
+// q^n at the node; the network's 10 outputs are, per variable, the
+// coefficients' departures (a - 1, b) of the start
+// q^n + a (q^n - q^(n-1)) + b (q^(n-1) - q^(n-2)).
+constexpr int kWsComp = 5;
+constexpr int kWsPts = 7;
+constexpr int kWsIn = kWsComp * 2 * kWsPts + kWsComp;  // 75
+constexpr int kWsOut = 2 * kWsComp;                     // network outputs
+constexpr int kWsTgt = 3 * kWsComp;                     // per sample: r, d1, d2
+constexpr int kWsStats = 2 * kWsIn + 2 * kWsComp;       // mu, 1/sd, scale, unused
+constexpr int kWsMaxHidden = 64;
+struct WsQ {
+    double* q[5];  // q^(n-2), fine level
+};
+
 struct Ctx;  // host context, desalm_api.hip
 
 // ---- launchers (each enqueues on `s`) ----
@@ -183,5 +200,24 @@ void launch_alm_hub(hipStream_t s, const Level& L, const Fields& F, const Alm& A
 // batched GEPP (LAPACK SGTSV semantics), k_daf.hip
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,
                        double* b, int* info);
+
+void launch_ws_sample(hipStream_t s, const Level& L, const Fields& F, const WsQ& Q, float* X, float* T, long m,
+                      unsigned long long seed, long step);
+void launch_ws_batch(hipStream_t s, const float* X, const float* T, long n, float* Xb, float* Tb, int m,
+                     unsigned long long seed, long it, const float* stats);
+void launch_ws_features(hipStream_t s, const Level& L, const Fields& F, const WsQ& Q, float* X, long r0, long m,
+                        const float* stats);
+void launch_ws_apply(hipStream_t s, const Level& L, const Fields& F, const WsQ& Q, const float* Y, long r0, long m);
+void launch_dense(hipStream_t s, const float* A, int lda, int K1, const float* W, int N, float* Y, int ldy, int M,
+                  int act, int ones);
+void launch_dense_bwd(hipStream_t s, const float* dZ, int ldz, int N, const float* W, int K, const float* Aprev,
+                      int lda, float* dA, int ldd, int M);
+int wgrad_parts(int M);
+void launch_wgrad(hipStream_t s, const float* A, int lda, int K1, const float* dZ, int ldz, int N, int M, float* P,
+                  float* G);
+void launch_ws_loss(hipStream_t s, const float* Y, const float* T, int M, float* dZ, double* part, int nparts);
+void launch_adam(hipStream_t s, float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
+                 float eps, float c1, float c2);
+void launch_colstats(hipStream_t s, const float* X, int ld, int nc, long n, double* part, int nparts);
 
 }  // namespace desalm

@@ -9,12 +9,14 @@ deck's values, ind3dmg.dat:8-16):
   cavity   33^3 lid-driven cavity, Re 1000, steady, laminar, 2 levels
   channel  129^3 SA channel, Re_tau 395, log-law start + 5% seeded noise,
            3 levels, BDF2 dual time
+  alm_box  193x129x129 SA-DES box, uniform inflow U = 1 past one actuator
+           line rotor (3 blades, R = 0.5, TSR 6), dt = 0.01
 """
 import math
 
 import numpy as np
 
-from .params import make_params, BC_WALL, BC_INTERFACE, BC_PERIODIC
+from .params import make_params, BC_WALL, BC_INTERFACE, BC_PERIODIC, BC_SYMMETRY, BC_FREESTREAM, BC_INFLOW
 
 
 class Overset:
@@ -201,3 +203,60 @@ def wavy(ni, nj, nk, amp=0.04, ng=2, seed=7, des=False, unsteady=True, btype=Non
                     btype=btype, fix=(3, 2, 4), **kw)
     ov = overset_table(ni, nj, nk, overset, apply_mom=apply_mom) if overset else None
     return Case("wavy", p, x, y, z, wd, q, qn, qnm1, xnut, overset=ov)
+
+
+def cluster(n, a, b, c, alpha):
+    """n points on [a, b] clustered around c: x(s) = s - alpha/(2 pi)
+    sin(2 pi (s - s_c)), monotonic for alpha < 1, spacing (1 - alpha) h at c
+    and (1 + alpha) h far from it."""
+    s = np.linspace(0.0, 1.0, n)
+    sc = (c - a) / (b - a)
+    x = s - alpha / (2.0 * math.pi) * np.sin(2.0 * math.pi * (s - sc))
+    x = (x - x[0]) / (x[-1] - x[0])
+    return a + (b - a) * x
+
+
+def alm_box(ni=193, nj=129, nk=129, ng=3, itm=30, delti=0.01, ren=1.0e6, radius=0.5, tsr=6.0, hub=0.05,
+            length=6.0, half=1.5, x_rot=1.5, nblades=3, nelems=16, nangles=None, angle=0.0, seed=20260404,
+            noise=0.0, des=True, iters=(1, 2, 3), rotor=True, **kw):
+    """Uniform inflow U = 1 past one actuator-line rotor (workload 4): an
+    x = [0, length] by y, z = [-half, half] box clustered around the rotor
+    plane x = x_rot and the rotor axis, inflow at i = 1 (held), zero-gradient
+    outflow at i = imax, slip side faces.  SA-DES: no walls, so the SA
+    length scale wd is the DES one, 0.65 times the largest local spacing.
+    The rotor's tables are built on the fine level (desalm.alm.rotor); the
+    Gaussian width is 2 dV^(1/3), i.e. two cells (rhs_lines.F90:434)."""
+    xi = cluster(ni, 0.0, length, x_rot, 0.6)
+    yj = cluster(nj, -half, half, 0.0, 0.6)
+    zk = cluster(nk, -half, half, 0.0, 0.6)
+    x, y, z = mesh(xi, yj, zk)
+    n1 = ni * nj * nk
+    dx = np.gradient(xi)
+    dy = np.gradient(yj)
+    dz = np.gradient(zk)
+    dmax = np.maximum(np.maximum(dx[None, None, :], dy[None, :, None]), dz[:, None, None]).ravel()
+    wd = 0.65 * dmax
+    q = np.zeros((n1, 5))
+    q[:, 1] = 1.0
+    if noise > 0.0:
+        rng = np.random.default_rng(seed)
+        inner = np.ones(n1, dtype=bool)
+        ii = np.arange(n1) % ni
+        inner &= ii > 0
+        q[:, 1] += noise * rng.uniform(-1.0, 1.0, n1) * inner
+        q[:, 2] += noise * rng.uniform(-1.0, 1.0, n1) * inner
+        q[:, 3] += noise * rng.uniform(-1.0, 1.0, n1) * inner
+    q[:, 4] = 3.0 / ren
+    xnut = q[:, 4] * fv1(q[:, 4], ren)
+    bt = (BC_INFLOW, BC_FREESTREAM, BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY)
+    p = make_params(ni, nj, nk, ng=ng, itm=itm, ren=ren, beta=1.0, delti=delti, unsteady=True, des=des,
+                    btype=bt, fix=(ni - 1, nj // 2, nk // 2), iters=iters, **kw)
+    c = Case("alm_box", p, x, y, z, wd, q, q.copy(), q.copy(), xnut)
+    if rotor:
+        from . import alm
+        i_rot = int(np.argmin(np.abs(xi - x_rot))) + 1
+        if nangles is None:
+            nangles = 361
+        c.alm = alm.rotor(c, i_rot=i_rot, center=(0.0, 0.0), radius=radius, hub=hub, nblades=nblades,
+                          nelems=nelems, nangles=nangles, tsr=tsr, angle=angle, length_scale=1.0)
+    return c

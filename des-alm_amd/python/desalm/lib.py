@@ -72,6 +72,27 @@ def lib():
         L.desalm_group_destroy.argtypes = [vp]
         L.desalm_set_turbines.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, ctypes.c_double,
                                           ctypes.c_double]
+        fp = ctypes.POINTER(ctypes.c_float)
+        i64 = ctypes.c_int64
+        L.desalm_set_convergence.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.desalm_step.argtypes = [vp, ip, dp]
+        L.desalm_ws_configure.argtypes = [vp, ctypes.c_int, i64, i64, ctypes.c_uint64]
+        L.desalm_ws_mode.argtypes = [vp, ctypes.c_int]
+        L.desalm_ws_record.argtypes = [vp]
+        L.desalm_ws_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+        L.desalm_ws_param_count.argtypes = [vp]
+        L.desalm_ws_params.argtypes = [vp, fp, fp]
+        L.desalm_ws_stats.argtypes = [vp, fp, fp]
+        L.desalm_ws_features.argtypes = [vp, i64, fp, fp]
+        L.desalm_mlp_forward.argtypes = [vp, fp, i64, fp]
+        L.desalm_mlp_loss_grad.argtypes = [vp, fp, fp, i64, fp, dp]
+        L.desalm_ws_history.argtypes = [vp, dp, dp]
+        L.desalm_turbine_angle.argtypes = [vp, ctypes.c_int, dp, dp]
+        for f in ("desalm_set_convergence", "desalm_step", "desalm_ws_configure", "desalm_ws_mode",
+                  "desalm_ws_record", "desalm_ws_train", "desalm_ws_param_count", "desalm_ws_params",
+                  "desalm_ws_stats", "desalm_ws_features", "desalm_mlp_forward", "desalm_mlp_loss_grad",
+                  "desalm_ws_history", "desalm_turbine_angle"):
+            getattr(L, f).restype = ctypes.c_int
         for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
                   "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize",
@@ -88,6 +109,14 @@ def _p(a):
     if a.dtype != np.float64 or not a.flags.c_contiguous:
         raise TypeError("arrays must be C-contiguous float64")
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f(a):
+    if a is None:
+        return None
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float32")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _c(a):
@@ -137,7 +166,9 @@ class Solver:
         """Actuator-line rotors (desalm.alm.AlmSetup); None turns the forcing off."""
         if alm is None:
             self._chk(lib().desalm_set_turbines(self.h, 0, None, 0, None, 0.0, 0.0), "desalm_set_turbines")
+            self._nturb = 0
             return
+        self._nturb = len(alm.turbines)
         ts, ps, keep = alm.as_ctypes()
         self._chk(lib().desalm_set_turbines(self.h, len(alm.turbines), ctypes.cast(ts, ctypes.c_void_p),
                                             len(alm.polars), ctypes.cast(ps, ctypes.c_void_p), alm.length_scale,
@@ -187,6 +218,100 @@ class Solver:
         hist = np.zeros((max(self.params.itm, 1), 5))
         self._chk(lib().desalm_mg_driver(self.h, ctypes.byref(itc), _p(hist)), "desalm_mg_driver")
         return itc.value, hist[: itc.value]
+
+    def set_convergence(self, itm, it_min, er_min, eo_min):
+        self._chk(lib().desalm_set_convergence(self.h, itm, it_min, er_min, eo_min), "desalm_set_convergence")
+        self.params.itm, self.params.it_min = itm, it_min
+        self.params.er_min, self.params.eo_min = er_min, eo_min
+
+    def step(self):
+        """One physical time step: blade angles advance, then mg_driver from
+        the selected warm start.  Returns (itc, log10 norm history)."""
+        itc = ctypes.c_int(0)
+        hist = np.zeros((max(self.params.itm, 1), 5))
+        self._chk(lib().desalm_step(self.h, ctypes.byref(itc), _p(hist)), "desalm_step")
+        return itc.value, hist[: itc.value]
+
+    # ---- learned warm start ----
+    WS_NONE, WS_EXTRAP, WS_LEARNED = 0, 1, 2
+
+    def ws_configure(self, hidden=32, samples_per_record=65536, capacity=1 << 22, seed=20260505):
+        self._chk(lib().desalm_ws_configure(self.h, hidden, samples_per_record, capacity, seed),
+                  "desalm_ws_configure")
+        self._ws = True
+
+    def ws_mode(self, mode):
+        self._chk(lib().desalm_ws_mode(self.h, mode), "desalm_ws_mode")
+
+    def ws_record(self):
+        self._chk(lib().desalm_ws_record(self.h), "desalm_ws_record")
+
+    def ws_train(self, iters, batch=16384, lr=1e-3):
+        loss = np.zeros(max(iters, 1))
+        self._chk(lib().desalm_ws_train(self.h, iters, batch, lr, _p(loss)), "desalm_ws_train")
+        return loss[:iters]
+
+    def ws_params(self, set_to=None):
+        n = lib().desalm_ws_param_count(self.h)
+        out = np.zeros(n, dtype=np.float32)
+        src = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.float32)
+        self._chk(lib().desalm_ws_params(self.h, _f(out), _f(src)), "desalm_ws_params")
+        return out
+
+    def ws_stats(self, set_to=None):
+        out = np.zeros(2 * 75 + 2 * 5, dtype=np.float32)
+        src = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.float32)
+        self._chk(lib().desalm_ws_stats(self.h, _f(out), _f(src)), "desalm_ws_stats")
+        return out
+
+    def ws_history(self, set_to=None):
+        """q^(n-2) (n1, 5); set_to restores it."""
+        out = np.zeros((self.n1, 5))
+        self._chk(lib().desalm_ws_history(self.h, _p(out), _p(_c(set_to))), "desalm_ws_history")
+        return out
+
+    def turbine_angle(self, t=0, set_to=None):
+        a = ctypes.c_double(0.0)
+        b = None if set_to is None else ctypes.byref(ctypes.c_double(set_to))
+        self._chk(lib().desalm_turbine_angle(self.h, t, ctypes.byref(a), b), "desalm_turbine_angle")
+        return a.value
+
+    def snapshot(self):
+        """Everything a physical step reads: state, q^(n-2) (with a warm
+        start) and blade angles."""
+        st = self.get_state()
+        hist = self.ws_history() if getattr(self, "_ws", False) else None
+        ang = [self.turbine_angle(t) for t in range(getattr(self, "_nturb", 0))]
+        return st, hist, ang
+
+    def restore(self, snap):
+        (q, qn, qnm1, xnut), hist, ang = snap
+        self.set_state(q, qn, qnm1, xnut)
+        if hist is not None:
+            self.ws_history(hist)
+        for t, a in enumerate(ang):
+            self.turbine_angle(t, a)
+
+    def ws_features(self, m):
+        x = np.zeros((m, 75), dtype=np.float32)
+        t = np.zeros((m, 15), dtype=np.float32)
+        self._chk(lib().desalm_ws_features(self.h, m, _f(x), _f(t)), "desalm_ws_features")
+        return x, t
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], 10), dtype=np.float32)
+        self._chk(lib().desalm_mlp_forward(self.h, _f(x), x.shape[0], _f(y)), "desalm_mlp_forward")
+        return y
+
+    def mlp_loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(lib().desalm_ws_param_count(self.h), dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        self._chk(lib().desalm_mlp_loss_grad(self.h, _f(x), _f(t), x.shape[0], _f(g), ctypes.byref(loss)),
+                  "desalm_mlp_loss_grad")
+        return loss.value, g
 
     def solver_daf(self, n, recalc, calc_pk, q, qn, qnm1, csi, eta, zet, aj, xnut, pk, rh):
         """In-place on q, pk, rh (float64 C-contiguous arrays of level n)."""

@@ -290,6 +290,86 @@ int desalm_profile(desalm_ctx* ctx, int enable);
 int desalm_profile_get(const desalm_ctx* ctx, int kernel, int level, int64_t* count, double* total_ms);
 const char* desalm_profile_name(int kernel);
 
+/* Convergence controls of the following desalm_mg_driver calls: at most
+ * itm pseudo-iterations, stop after it_min when the largest u, v, w log10
+ * norm has dropped below er_min relative to the first iteration or below
+ * eo_min absolutely (mg_ck_conver, mg_driver.F90:372-398; the reference reads
+ * them from its input deck, init.F90:176-228).  log_resid_out of a later
+ * call must hold 5 * itm doubles. */
+int desalm_set_convergence(desalm_ctx* ctx, int itm, int it_min, double er_min, double eo_min);
+
+/* ---- Physical time step (main.F90:38-62) ----
+ * One step of the physical-time loop around mg_driver: every rotor's blade
+ * angle advances (angle > 2 pi wraps by -2 pi first, then angle += dtheta with
+ * dtheta = TSR delti length_scale / tur_rad, main.F90:42-49 and
+ * init.F90:1447), then desalm_mg_driver runs, from the warm start selected
+ * by desalm_ws_mode. */
+int desalm_step(desalm_ctx* ctx, int* itc_out, double* log_resid_out);
+
+/* ---- Learned warm start ----
+ * No counterpart in the reference, which starts every physical step's
+ * pseudo-iterations from the previous solution (mg_driver.F90:48-51).  With
+ * a warm start, desalm_mg_driver replaces that start on the fine level's
+ * interior, after the qn / qnm1 shift, by
+ *   DESALM_WS_EXTRAP   2 q^n - q^(n-1)
+ *   DESALM_WS_LEARNED  q^n + a d1 + b d2, d1 = q^n - q^(n-1),
+ *                      d2 = q^(n-1) - q^(n-2), with (a - 1, b) per node and
+ *                      per variable the 10 outputs of a multilayer
+ *                      perceptron (75 inputs, two tanh layers of `hidden`
+ *                      units, FP32).  Inputs: for each of p, u, v, w, nu~ the
+ *                      14 differences d1, d2 at the node and its six face
+ *                      neighbours divided by their root mean square, then
+ *                      q^n at the node; standardised by the statistics.
+ * nu~ keeps its floor of 1e-7.  The pseudo-iterations then run to the same
+ * tolerance as from the reference's start. */
+#define DESALM_WS_NONE 0
+#define DESALM_WS_EXTRAP 1
+#define DESALM_WS_LEARNED 2
+
+/* Allocate the warm start: network of `hidden` (1..64) units per layer with
+ * seeded Xavier-uniform weights and a zero output layer (so it starts as
+ * linear extrapolation), a sample buffer of `capacity` rows, and q^(n-2)
+ * (set to qnm1 here and by desalm_set_state, shifted by every
+ * desalm_mg_driver).  Resets any earlier warm start. */
+int desalm_ws_configure(desalm_ctx* ctx, int hidden, int64_t samples_per_record, int64_t capacity,
+                        uint64_t seed);
+int desalm_ws_mode(desalm_ctx* ctx, int mode);
+/* After a physical step: samples_per_record seeded interior nodes of the
+ * current state give (features from q^n, q^(n-1), q^(n-2); targets r = q -
+ * 2 q^n + q^(n-1), d1, d2) rows, appended to the buffer (the oldest are
+ * overwritten when it is full). */
+int desalm_ws_record(desalm_ctx* ctx);
+/* iters Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * rows drawn from the buffer, minimising the mean over rows and variables of
+ * ((a - 1) d1 + b d2 - r)^2 / scale^2.  The statistics (feature means and
+ * 1/sd, and per variable the root mean square of d1 as scale) are fixed by
+ * the first call.  loss_out, if not NULL, receives the loss of every step
+ * (iters doubles). */
+int desalm_ws_train(desalm_ctx* ctx, int iters, int batch, double lr, double* loss_out);
+/* Packed parameters (count desalm_ws_param_count: W1 76 x hidden, W2
+ * (hidden + 1) x hidden, W3 (hidden + 1) x 10, row-major, last row of each
+ * the bias) and statistics (160 floats: 75 feature means, 75 feature 1/sd,
+ * 5 target scales, 5 unused); get when `get` is not NULL, then set when
+ * `set` is not NULL. */
+int desalm_ws_param_count(const desalm_ctx* ctx);
+int desalm_ws_params(desalm_ctx* ctx, float* get, const float* set);
+int desalm_ws_stats(desalm_ctx* ctx, float* get, const float* set);
+/* q^(n-2) of the fine level, (nodes, 5) as desalm_get_state: get when not
+ * NULL, then set when not NULL (a saved state is restored with
+ * desalm_set_state followed by this). */
+int desalm_ws_history(desalm_ctx* ctx, double* get, const double* set);
+/* Blade angle (rad) of rotor t, get and/or set (global_turb's angle(n_t)). */
+int desalm_turbine_angle(desalm_ctx* ctx, int t, double* get, const double* set);
+/* Unstandardised features (m x 75) and targets (m x 15: r, d1, d2 per
+ * variable) of the current state at interior ordinals 0 .. m-1 (i fastest),
+ * as desalm_ws_record builds them. */
+int desalm_ws_features(desalm_ctx* ctx, int64_t m, float* x, float* t);
+/* The network on host rows: y (m x 10) from standardised features x
+ * (m x 75); and the loss against targets t (m x 15, already divided by the
+ * scales) with its gradient with respect to the packed parameters. */
+int desalm_mlp_forward(desalm_ctx* ctx, const float* x, int64_t m, float* y);
+int desalm_mlp_loss_grad(desalm_ctx* ctx, const float* x, const float* t, int64_t m, float* grad, double* loss);
+
 const char* desalm_last_error(const desalm_ctx* ctx);
 int desalm_finalize(desalm_ctx* ctx);
 
