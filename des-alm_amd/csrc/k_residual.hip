@@ -7,10 +7,13 @@
 //              convection, 4th-difference pressure dissipation, BDF2 term,
 //              zeroing of the boundary planes
 //   k_visc_pk  2nd-order viscous fluxes and the FAS forcing
-// One thread per node gathers its stencil from the SoA fields and no face
-// arrays touch HBM; k_residual's face terms are evaluated by both nodes that
-// share them, k_visc_pk shares its face fluxes through LDS.
+// No face arrays touch HBM: k_residual's face terms are evaluated by both
+// nodes that share them (one thread per node gathers its stencil from the
+// SoA fields); k_visc_pk stages its tiles in LDS and shares its face fluxes
+// there.
 // Operand order inside every expression follows the reference routine cited.
+#include <algorithm>
+
 #include "desalm_internal.h"
 
 namespace desalm {
@@ -75,61 +78,6 @@ __global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
         const double tmp1 = gg[D] * d / aj;
         const double tmp2 = (hh[D] * hh[D]) * (p[g + st[D]] - 2.0 * p[g] + p[g - st[D]]);
         F.fv[D][t] = tmp1 * tmp2;
-    }
-}
-
-// One viscous face flux (rhs_viscous.F90:112-163 for D=0, :189-243 for D=1,
-// :263-316 for D=2): the face between node t and node t + s_D.
-template <int D>
-__device__ __forceinline__ void visc_face(const Level& L, const Fields& F, const Consts& C, long t,
-                                          const Nb& nb, double fv[3]) {
-    const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
-    const long o = L.off;
-    const long t0 = o + t, t1 = o + t + sD;
-    const double hc = L.dc, he = L.de, hz = L.dz;
-    const double hc4 = 0.25 * hc, he4 = 0.25 * he, hz4 = 0.25 * hz;
-    double dcs[3], det[3], dze[3];  // d(u,v,w)/d(csi,eta,zet) at the face
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        const double* u = F.q[m + 1];
-        if (D == 0) {
-            dcs[m] = hc * (u[t1] - u[t0]);
-            det[m] = he4 * (u[t1 + nb.sJ] - u[t1 - nb.sJ] + u[t0 + nb.sJ] - u[t0 - nb.sJ]);
-            dze[m] = hz4 * (u[t1 + nb.sK] - u[t1 - nb.sK] + u[t0 + nb.sK] - u[t0 - nb.sK]);
-        } else if (D == 1) {
-            dcs[m] = hc4 * (u[t1 + nb.sI] - u[t1 - nb.sI] + u[t0 + nb.sI] - u[t0 - nb.sI]);
-            det[m] = he * (u[t1] - u[t0]);
-            dze[m] = hz4 * (u[t1 + nb.sK] - u[t1 - nb.sK] + u[t0 + nb.sK] - u[t0 - nb.sK]);
-        } else {
-            dcs[m] = hc4 * (u[t1 + nb.sI] - u[t1 - nb.sI] + u[t0 + nb.sI] - u[t0 - nb.sI]);
-            det[m] = he4 * (u[t1 + nb.sJ] - u[t1 - nb.sJ] + u[t0 + nb.sJ] - u[t0 - nb.sJ]);
-            dze[m] = hz * (u[t1] - u[t0]);
-        }
-    }
-    const double a1 = F.csi[0][t1] + F.csi[0][t0], a2 = F.csi[1][t1] + F.csi[1][t0], a3 = F.csi[2][t1] + F.csi[2][t0];
-    const double b1 = F.eta[0][t1] + F.eta[0][t0], b2 = F.eta[1][t1] + F.eta[1][t0], b3 = F.eta[2][t1] + F.eta[2][t0];
-    const double c1 = F.zet[0][t1] + F.zet[0][t0], c2 = F.zet[1][t1] + F.zet[1][t0], c3 = F.zet[2][t1] + F.zet[2][t0];
-    // uu_r = r-th metric row applied to the gradient of each velocity component
-    double uu[3][3];  // uu[r][m]: r = 1..3 (x,y,z), m = u,v,w
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        uu[0][m] = a1 * dcs[m] + b1 * det[m] + c1 * dze[m];
-        uu[1][m] = a2 * dcs[m] + b2 * det[m] + c2 * dze[m];
-        uu[2][m] = a3 * dcs[m] + b3 * det[m] + c3 * dze[m];
-    }
-    const double f1 = D == 0 ? a1 : D == 1 ? b1 : c1;
-    const double f2 = D == 0 ? a2 : D == 1 ? b2 : c2;
-    const double f3 = D == 0 ? a3 : D == 1 ? b3 : c3;
-    const double cca = f1 * a1 + f2 * a2 + f3 * a3;
-    const double ccb = f1 * b1 + f2 * b2 + f3 * b3;
-    const double ccc = f1 * c1 + f2 * c2 + f3 * c3;
-    const double ret1 = C.rei + F.xnut[t1], ret0 = C.rei + F.xnut[t0];
-    const double vtj = 0.25 * (ret1 + ret0) / (F.aj[t1] + F.aj[t0]);
-    // component m of the flux: face metric . (uu[.][m]) + cc . grad(u_m)
-    // (rhs_viscous.F90:158-163: fv(1) uses uu1,vv1,ww1 = uu[0][0],uu[0][1],uu[0][2])
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        fv[r] = vtj * (f1 * uu[r][0] + f2 * uu[r][1] + f3 * uu[r][2] + cca * dcs[r] + ccb * det[r] + ccc * dze[r]);
     }
 }
 
@@ -257,56 +205,202 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
 // rhs_unst_visc_diss.F90:181-183) and the FAS forcing of coarse levels
 // (rhs_pk.F90:232-264): pk = pk - rh when calc_pk, then rh = rh + pk.
 //
-// A workgroup of kVX x kVY threads owns an (i, j) tile of (kVX-1) x (kVY-1)
-// interior nodes of one k plane (kVK = 1) plus one halo column and row on the
-// low side.  Every thread evaluates the +i face of its node (rows with an
-// output) and the +j face (columns with an output) once and shares them
-// through LDS, so the i and j faces are evaluated about once instead of by
-// both of their nodes; an output node evaluates both of its k faces.
-// Marching a workgroup through several planes (carrying the k face in
-// registers) was measured slower: each plane then waits at the barriers with
-// too few waves resident to hide the gathers.
-constexpr int kVX = 32, kVY = 8, kVK = 1;
+// A workgroup of kTX x kTY threads owns an (i, j) tile of (kTX-1) x (kTY-1)
+// interior nodes plus one halo column and row on the low side, and marches
+// through kc consecutive k planes.  Every thread evaluates the +i face of its
+// node (rows with an output) and the +j face (columns with an output) once
+// and shares them through LDS.  u, v, w of three planes (k-1, k, k+1) and
+// the metrics, J and nu_t of plane k sit in LDS for the tile plus one high
+// halo column and row (kLX x kLY nodes), so every face reads its 19-point
+// velocity footprint and both nodes' metrics from LDS instead of issuing
+// global loads.  The k face above a node is carried into the next plane (each
+// k face is evaluated once).  The next planes are loaded into registers while
+// the current one is computed.
+constexpr int kTX = 32, kTY = 8;
+constexpr int kLX = kTX + 1, kLY = kTY + 1, kLN = kLX * kLY;
+constexpr int kHalo = kLN - kTX * kTY;  // 41 halo nodes: column kTX, then row kTY
+constexpr int kMet = 11;                // csi 3, eta 3, zet 3, aj, xnut
 
-__global__ void __launch_bounds__(kVX * kVY) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk) {
-    __shared__ double s_fi[3][kVY][kVX];
-    __shared__ double s_fj[3][kVY][kVX];
+struct Met {
+    double v[kMet];
+};
+
+__device__ __forceinline__ void load_met(const Fields& F, long g, bool ok, Met& m) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        m.v[c] = ok ? F.csi[c][g] : 0.0;
+        m.v[3 + c] = ok ? F.eta[c][g] : 0.0;
+        m.v[6 + c] = ok ? F.zet[c][g] : 0.0;
+    }
+    m.v[9] = ok ? F.aj[g] : 0.0;
+    m.v[10] = ok ? F.xnut[g] : 0.0;
+}
+
+// One viscous face flux (rhs_viscous.F90:112-163 for D=0, :189-243 for D=1,
+// :263-316 for D=2) between node 0 and node 1 = node 0 + e_D; U(m, a, b, c)
+// returns velocity component m at node 0 + (a, b, c), m0 / m1 are the two
+// nodes' metrics, J and nu_t.  uu[r][m] is the r-th metric row applied to the
+// gradient of velocity component m; flux component m is the face metric
+// applied to uu[.][m] plus cc . grad(u_m) (rhs_viscous.F90:158-163).
+template <int D, typename U>
+__device__ __forceinline__ void visc_face_t(const Level& L, const Consts& C, U u, const Met& m0, const Met& m1,
+                                            double fv[3]) {
+    constexpr int X = D == 0, Y = D == 1, Z = D == 2;
+    const double hc = L.dc, he = L.de, hz = L.dz;
+    const double hc4 = 0.25 * hc, he4 = 0.25 * he, hz4 = 0.25 * hz;
+    double dcs[3], det[3], dze[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        if (D == 0) {
+            dcs[m] = hc * (u(m, 1, 0, 0) - u(m, 0, 0, 0));
+            det[m] = he4 * (u(m, 1, 1, 0) - u(m, 1, -1, 0) + u(m, 0, 1, 0) - u(m, 0, -1, 0));
+            dze[m] = hz4 * (u(m, 1, 0, 1) - u(m, 1, 0, -1) + u(m, 0, 0, 1) - u(m, 0, 0, -1));
+        } else if (D == 1) {
+            dcs[m] = hc4 * (u(m, 1, 1, 0) - u(m, -1, 1, 0) + u(m, 1, 0, 0) - u(m, -1, 0, 0));
+            det[m] = he * (u(m, 0, 1, 0) - u(m, 0, 0, 0));
+            dze[m] = hz4 * (u(m, 0, 1, 1) - u(m, 0, 1, -1) + u(m, 0, 0, 1) - u(m, 0, 0, -1));
+        } else {
+            dcs[m] = hc4 * (u(m, 1, 0, 1) - u(m, -1, 0, 1) + u(m, 1, 0, 0) - u(m, -1, 0, 0));
+            det[m] = he4 * (u(m, 0, 1, 1) - u(m, 0, -1, 1) + u(m, 0, 1, 0) - u(m, 0, -1, 0));
+            dze[m] = hz * (u(m, 0, 0, 1) - u(m, 0, 0, 0));
+        }
+    }
+    (void)X; (void)Y; (void)Z;
+    const double a1 = m1.v[0] + m0.v[0], a2 = m1.v[1] + m0.v[1], a3 = m1.v[2] + m0.v[2];
+    const double b1 = m1.v[3] + m0.v[3], b2 = m1.v[4] + m0.v[4], b3 = m1.v[5] + m0.v[5];
+    const double c1 = m1.v[6] + m0.v[6], c2 = m1.v[7] + m0.v[7], c3 = m1.v[8] + m0.v[8];
+    double uu[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        uu[0][m] = a1 * dcs[m] + b1 * det[m] + c1 * dze[m];
+        uu[1][m] = a2 * dcs[m] + b2 * det[m] + c2 * dze[m];
+        uu[2][m] = a3 * dcs[m] + b3 * det[m] + c3 * dze[m];
+    }
+    const double f1 = D == 0 ? a1 : D == 1 ? b1 : c1;
+    const double f2 = D == 0 ? a2 : D == 1 ? b2 : c2;
+    const double f3 = D == 0 ? a3 : D == 1 ? b3 : c3;
+    const double cca = f1 * a1 + f2 * a2 + f3 * a3;
+    const double ccb = f1 * b1 + f2 * b2 + f3 * b3;
+    const double ccc = f1 * c1 + f2 * c2 + f3 * c3;
+    const double ret1 = C.rei + m1.v[10], ret0 = C.rei + m0.v[10];
+    const double vtj = 0.25 * (ret1 + ret0) / (m1.v[9] + m0.v[9]);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        fv[r] = vtj * (f1 * uu[r][0] + f2 * uu[r][1] + f3 * uu[r][2] + cca * dcs[r] + ccb * det[r] + ccc * dze[r]);
+}
+
+__global__ void __launch_bounds__(kTX * kTY) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk, int kc) {
+    __shared__ double s_u[3][3][kLN];  // velocity component, plane slot (k mod 3), tile node
+    __shared__ double s_m[kMet][kLN];  // metrics, J, nu_t of the current plane
+    __shared__ double s_fi[3][kTY][kTX];
+    __shared__ double s_fj[3][kTY][kTX];
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int tx = (int)threadIdx.x % kVX, ty = (int)threadIdx.x / kVX;
     const unsigned gx = gridDim.x, gy = gridDim.y;
     const unsigned lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
     const int bx = (int)(lb % gx), by = (int)((lb / gx) % gy), bz = (int)(lb / (gx * gy));
-    const int i = bx * (kVX - 1) + tx;  // node 0-based; tile outputs start at i = 1
-    const int j = by * (kVY - 1) + ty;
-    const int k0 = 1 + bz * kVK;
-    const int k1 = min(k0 + kVK, nk - 1);
+    const int tid = (int)threadIdx.x;
+    const int tx = tid % kTX, ty = tid / kTX;
+    const int i0 = bx * (kTX - 1), j0 = by * (kTY - 1);
+    const int kb = 1 + bz * kc, ke = min(kb + kc, nk - 1);  // output planes kb .. ke-1
+    const long sJ = ni, sK = (long)ni * nj;
+    // the thread's own node, and (tid < kHalo) one halo node of the tile
+    const int own = ty * kLX + tx;
+    const int hr = tid < kLY ? tid : kTY, hc = tid < kLY ? kTX : tid - kLY;
+    const int halo = hr * kLX + hc;
+    const bool has_halo = tid < kHalo;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own_ok = i < ni && j < nj;
+    const bool halo_ok = has_halo && i0 + hc < ni && j0 + hr < nj;
+    const long col = L.off + (long)i + sJ * j;
+    const long hcol = L.off + (long)(i0 + hc) + sJ * (j0 + hr);
     const bool in_i = i <= ni - 2, in_j = j <= nj - 2;
     const bool out = tx >= 1 && ty >= 1 && in_i && in_j;
-    const bool face_i = ty >= 1 && in_i && in_j;  // +i face of a node on an output row
-    const bool face_j = tx >= 1 && in_i && in_j;  // +j face of a node on an output column
-    const Nb nb{1, (long)ni, (long)ni * nj};
-    const long col = (long)i + (long)ni * j;
+    const bool face_i = ty >= 1 && in_i && in_j;
+    const bool face_j = tx >= 1 && in_i && in_j;
+
+    auto load_uvw = [&](int k, double ro[3], double rh_[3]) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            ro[m] = own_ok ? F.q[m + 1][col + sK * k] : 0.0;
+            rh_[m] = halo_ok ? F.q[m + 1][hcol + sK * k] : 0.0;
+        }
+    };
+    auto store_uvw = [&](int k, const double ro[3], const double rh_[3]) {
+        const int sl = k % 3;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            s_u[m][sl][own] = ro[m];
+            if (has_halo) s_u[m][sl][halo] = rh_[m];
+        }
+    };
+    auto store_met = [&](const Met& mo, const Met& mh) {
+#pragma unroll
+        for (int c = 0; c < kMet; c++) {
+            s_m[c][own] = mo.v[c];
+            if (has_halo) s_m[c][halo] = mh.v[c];
+        }
+    };
+    auto met_at = [&](int node) {
+        Met m;
+#pragma unroll
+        for (int c = 0; c < kMet; c++) m.v[c] = s_m[c][node];
+        return m;
+    };
+
+    // prologue: planes kb-1, kb, kb+1 of u, v, w; metrics of plane kb
+    {
+        double ro[3], rh_[3];
+        for (int k = kb - 1; k <= kb + 1; k++) {
+            load_uvw(k, ro, rh_);
+            store_uvw(k, ro, rh_);
+        }
+    }
+    Met mcur, mh;
+    load_met(F, col + sK * kb, own_ok, mcur);
+    load_met(F, hcol + sK * kb, halo_ok, mh);
+    store_met(mcur, mh);
     double fk[3] = {0.0, 0.0, 0.0};
-    if (out) visc_face<2>(L, F, C, col + nb.sK * (k0 - 1), nb, fk);
-    for (int k = k0; k < k1; k++) {
-        const long t = col + nb.sK * k;
+    {
+        Met mlow;
+        load_met(F, col + sK * (kb - 1), own_ok, mlow);
+        __syncthreads();
+        if (out) {
+            const int sl0 = (kb - 1) % 3, sl1 = kb % 3;
+            auto u = [&](int m, int a, int b, int c) {
+                return s_u[m][c == 0 ? sl0 : sl1][own + b * kLX + a];
+            };
+            visc_face_t<2>(L, C, u, mlow, mcur, fk);
+        }
+    }
+    for (int k = kb; k < ke; k++) {
+        // next planes into registers: metrics of k+1, velocities of k+2
+        Met mnext, mhn;
+        load_met(F, col + sK * (k + 1), own_ok, mnext);
+        load_met(F, hcol + sK * (k + 1), halo_ok, mhn);
+        double ro[3] = {0.0, 0.0, 0.0}, rh_[3] = {0.0, 0.0, 0.0};
+        const bool more = k + 1 < ke;
+        if (more) load_uvw(k + 2, ro, rh_);
+        const int slm = (k + 2) % 3, sl0 = k % 3, slp = (k + 1) % 3;
+        auto uk = [&](int m, int a, int b, int c) {
+            return s_u[m][c < 0 ? slm : c == 0 ? sl0 : slp][own + b * kLX + a];
+        };
         if (face_i) {
             double f[3];
-            visc_face<0>(L, F, C, t, nb, f);
+            visc_face_t<0>(L, C, uk, mcur, met_at(own + 1), f);
 #pragma unroll
             for (int c = 0; c < 3; c++) s_fi[c][ty][tx] = f[c];
         }
         if (face_j) {
             double f[3];
-            visc_face<1>(L, F, C, t, nb, f);
+            visc_face_t<1>(L, C, uk, mcur, met_at(own + kLX), f);
 #pragma unroll
             for (int c = 0; c < 3; c++) s_fj[c][ty][tx] = f[c];
         }
         double fkp[3] = {0.0, 0.0, 0.0};
-        if (out) visc_face<2>(L, F, C, t, nb, fkp);
+        if (out) visc_face_t<2>(L, C, uk, mcur, mnext, fkp);
         __syncthreads();
         if (out) {
-            const long g = L.off + t;
+            const long g = col + sK * k;
             double visc[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) visc[c] = L.dc * (s_fi[c][ty][tx] - s_fi[c][ty][tx - 1]);
@@ -332,6 +426,12 @@ __global__ void __launch_bounds__(kVX * kVY) k_visc_pk(Level L, Fields F, Consts
 #pragma unroll
             for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
         }
+        __syncthreads();
+        // plane k+1 becomes current: its metrics into LDS, plane k+2's
+        // velocities into the slot of plane k-1
+        store_met(mnext, mhn);
+        if (more) store_uvw(k + 2, ro, rh_);
+        mcur = mnext;
 #pragma unroll
         for (int c = 0; c < 3; c++) fk[c] = fkp[c];
         __syncthreads();
@@ -352,9 +452,12 @@ void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Const
     // the recalc branch needs U/J and p/J of the updated q (dtau is kept)
     if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, 0);
     hipLaunchKernelGGL(k_residual, grid, dim3(b), 0, s, L, F, C, n, recalc);
-    const dim3 gv((unsigned)((L.ni - 2 + kVX - 2) / (kVX - 1)), (unsigned)((L.nj - 2 + kVY - 2) / (kVY - 1)),
-                  (unsigned)((L.nk - 2 + kVK - 1) / kVK));
-    hipLaunchKernelGGL(k_visc_pk, gv, dim3(kVX * kVY), 0, s, L, F, C, n, recalc ? 0 : calc_pk);
+    // planes per workgroup: enough workgroups (about 2048) to fill the chip
+    const long tiles = (long)((L.ni - 2 + kTX - 2) / (kTX - 1)) * ((L.nj - 2 + kTY - 2) / (kTY - 1));
+    const int kc = (int)std::max(1L, std::min(16L, tiles * (L.nk - 2) / 2048));
+    const dim3 gv((unsigned)((L.ni - 2 + kTX - 2) / (kTX - 1)), (unsigned)((L.nj - 2 + kTY - 2) / (kTY - 1)),
+                  (unsigned)((L.nk - 2 + kc - 1) / kc));
+    hipLaunchKernelGGL(k_visc_pk, gv, dim3(kTX * kTY), 0, s, L, F, C, n, recalc ? 0 : calc_pk, kc);
 }
 
 }  // namespace desalm
