@@ -6,12 +6,15 @@
 // Kernels:
 //   k_sa_prep   U/J (stored, rhs_contra_j) and wo (rhs_sa_source)
 //   k_sa_rhs    rhs_sa_grdnu, rhs_sa_prod, rhs_sa_viscous, rhs_sa_flux,
-//               rhs_sa_unst_visc, rhs_sa_implicit_source, boundary zeroing,
+//               rhs_sa_unst_visc, rhs_sa_implicit_source on the interior,
 //               and the ADI operators' face coefficients and dtev J / 2
+//               (LDS-staged tiles marching in k)
 //   k_sa_adi_i  rhs_sa_adi_solver sweep in i (rows staged through LDS)
 //   k_sa_adi<D> the sweeps in j and k (D = 2 also applies the update of
 //               des_eddy.F90:260-272)
 //   k_sa_bc<D>  rhs_sa_bcond faces, k_sa_clamp its nu~ floor and xnut
+#include <algorithm>
+
 #include "desalm_internal.h"
 #include "line_tile.h"
 
@@ -57,153 +60,294 @@ __global__ void k_sa_prep(Level L, Fields F) {
     F.wo[t] = 2.0 * (w12 * w12 + w13 * w13 + w23 * w23);
 }
 
-// rhs_sa_viscous face flux between level-local node t and t + s_D
-template <int D>
-__device__ __forceinline__ double sa_visc_face(const Level& L, const Fields& F, const Consts& C, long t, long sI,
-                                               long sJ, long sK) {
-    const long sD = D == 0 ? sI : D == 1 ? sJ : sK;
-    const long g0 = L.off + t, g1 = g0 + sD;
-    const double* q5 = F.q[4];
+// ---- k_sa_rhs: rhs_sa_grdnu, rhs_sa_prod, rhs_sa_viscous, rhs_sa_flux,
+// rhs_sa_unst_visc and rhs_sa_implicit_source on LDS-staged tiles ----
+//
+// Organised as k_visc_pk is: a workgroup of kQX x kQY threads owns a (kQX-1) x (kQY-1) tile of
+// interior nodes plus a low halo column and row, and marches through kc
+// planes.  nu~ of planes k-1, k, k+1, and the metrics, J and U/J, V/J of
+// plane k, sit in LDS for the tile plus a high halo column and row.  Every
+// +i and +j face (viscous flux, convective flux and the ADI coefficient
+// av_D) is evaluated once and shared through LDS; the +k face is carried
+// into the next plane.  The ADI coefficients are written on the faces the
+// sweeps read (positions 0 .. n_D - 2 along D, interior across), dtj on the
+// interior; rh5 on the interior (the sweeps never read its boundary).
+// Operand order follows the reference routines, except that the two powers
+// of rhs_sa_prod.F90:47,84 are taken as 1 / x^3 and cbrt(sqrt(y)) instead of
+// pow(x, -3) and pow(y, 1/6): a rounding-level difference that saves about
+// a quarter of the kernel's time (FP64 pow is long on the device).  The ADI
+// face coefficients are
+//   av_D = |U/J face average| + (1/Re + nu~ face average)/sig * |grad csi_D|^2_face / (J + J')
+// (rhs_sa_adi_solver.F90:53-75); nu~ does not change between the three
+// sweeps of one sub-iteration.
+constexpr int kQX = 32, kQY = 8;
+constexpr int kQLX = kQX + 1, kQLY = kQY + 1, kQLN = kQLX * kQLY;
+constexpr int kQHalo = kQLN - kQX * kQY;
+constexpr int kQM = 10;  // csi 3, eta 3, zet 3, J
+
+struct QMet {
+    double v[kQM];
+};
+
+__device__ __forceinline__ void qload_met(const Fields& F, long g, bool ok, QMet& m) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        m.v[c] = ok ? F.csi[c][g] : 0.0;
+        m.v[3 + c] = ok ? F.eta[c][g] : 0.0;
+        m.v[6 + c] = ok ? F.zet[c][g] : 0.0;
+    }
+    m.v[9] = ok ? F.aj[g] : 0.0;
+}
+
+// viscous flux of nu~ across the face node 0 -> node 0 + e_D
+// (rhs_sa_viscous.F90:58-188); Q(a, b, c) is nu~ at node 0 + (a, b, c)
+template <int D, typename Qf>
+__device__ __forceinline__ double sa_visc_face_t(const Level& L, const Consts& C, Qf q, const QMet& m0,
+                                                 const QMet& m1) {
     const double hc = L.dc, he = L.de, hz = L.dz;
     const double hc4 = 0.25 * hc, he4 = 0.25 * he, hz4 = 0.25 * hz;
     double vcs, vet, vze;
     if (D == 0) {
-        vcs = hc * (q5[g1] - q5[g0]);
-        vet = he4 * (q5[g1 + sJ] - q5[g1 - sJ] + q5[g0 + sJ] - q5[g0 - sJ]);
-        vze = hz4 * (q5[g1 + sK] - q5[g1 - sK] + q5[g0 + sK] - q5[g0 - sK]);
+        vcs = hc * (q(1, 0, 0) - q(0, 0, 0));
+        vet = he4 * (q(1, 1, 0) - q(1, -1, 0) + q(0, 1, 0) - q(0, -1, 0));
+        vze = hz4 * (q(1, 0, 1) - q(1, 0, -1) + q(0, 0, 1) - q(0, 0, -1));
     } else if (D == 1) {
-        vcs = hc4 * (q5[g1 + sI] - q5[g1 - sI] + q5[g0 + sI] - q5[g0 - sI]);
-        vet = he * (q5[g1] - q5[g0]);
-        vze = hz4 * (q5[g1 + sK] - q5[g1 - sK] + q5[g0 + sK] - q5[g0 - sK]);
+        vcs = hc4 * (q(1, 1, 0) - q(-1, 1, 0) + q(1, 0, 0) - q(-1, 0, 0));
+        vet = he * (q(0, 1, 0) - q(0, 0, 0));
+        vze = hz4 * (q(0, 1, 1) - q(0, 1, -1) + q(0, 0, 1) - q(0, 0, -1));
     } else {
-        vcs = hc4 * (q5[g1 + sI] - q5[g1 - sI] + q5[g0 + sI] - q5[g0 - sI]);
-        vet = he4 * (q5[g1 + sJ] - q5[g1 - sJ] + q5[g0 + sJ] - q5[g0 - sJ]);
-        vze = hz * (q5[g1] - q5[g0]);
+        vcs = hc4 * (q(1, 0, 1) - q(-1, 0, 1) + q(1, 0, 0) - q(-1, 0, 0));
+        vet = he4 * (q(0, 1, 1) - q(0, -1, 1) + q(0, 1, 0) - q(0, -1, 0));
+        vze = hz * (q(0, 0, 1) - q(0, 0, 0));
     }
-    const double a1 = F.csi[0][g1] + F.csi[0][g0], a2 = F.csi[1][g1] + F.csi[1][g0], a3 = F.csi[2][g1] + F.csi[2][g0];
-    const double b1 = F.eta[0][g1] + F.eta[0][g0], b2 = F.eta[1][g1] + F.eta[1][g0], b3 = F.eta[2][g1] + F.eta[2][g0];
-    const double c1 = F.zet[0][g1] + F.zet[0][g0], c2 = F.zet[1][g1] + F.zet[1][g0], c3 = F.zet[2][g1] + F.zet[2][g0];
+    const double a1 = m1.v[0] + m0.v[0], a2 = m1.v[1] + m0.v[1], a3 = m1.v[2] + m0.v[2];
+    const double b1 = m1.v[3] + m0.v[3], b2 = m1.v[4] + m0.v[4], b3 = m1.v[5] + m0.v[5];
+    const double c1 = m1.v[6] + m0.v[6], c2 = m1.v[7] + m0.v[7], c3 = m1.v[8] + m0.v[8];
     const double f1 = D == 0 ? a1 : D == 1 ? b1 : c1;
     const double f2 = D == 0 ? a2 : D == 1 ? b2 : c2;
     const double f3 = D == 0 ? a3 : D == 1 ? b3 : c3;
     const double cc1 = f1 * a1 + f2 * a2 + f3 * a3;
     const double cc2 = f1 * b1 + f2 * b2 + f3 * b3;
     const double cc3 = f1 * c1 + f2 * c2 + f3 * c3;
-    const double ret1 = (C.rei + q5[g1]) / sig, ret0 = (C.rei + q5[g0]) / sig;
-    const double vtj = 0.25 * (ret1 + ret0) / (F.aj[g1] + F.aj[g0]);
+    const double q1 = D == 0 ? q(1, 0, 0) : D == 1 ? q(0, 1, 0) : q(0, 0, 1);
+    const double q0 = q(0, 0, 0);
+    const double ret1 = (C.rei + q1) / sig, ret0 = (C.rei + q0) / sig;
+    const double vtj = 0.25 * (ret1 + ret0) / (m1.v[9] + m0.v[9]);
     return vtj * (cc1 * vcs + cc2 * vet + cc3 * vze);
 }
 
-// rhs_sa_flux face flux (first-order upwind FDS, rhs_sa_flux.F90:226-294)
+// convective flux (rhs_sa_flux.F90:226-294) and ADI coefficient
+// (rhs_sa_adi_solver.F90:53-75) of the face node 0 -> node 1 along D
 template <int D>
-__device__ __forceinline__ double sa_conv_face(const Level& L, const Fields& F, long t, long sD) {
-    const double u1 = F.ucn[D][t + sD], u0 = F.ucn[D][t];
-    const double q1 = F.q[4][L.off + t + sD], q0 = F.q[4][L.off + t];
+__device__ __forceinline__ void sa_conv_av(const Consts& C, double q0, double q1, double u0, double u1,
+                                           const QMet& m0, const QMet& m1, double& conv, double& av) {
     const double eigh = 0.5 * fabs(u1 + u0);
-    return 0.5 * (q1 * u1 + q0 * u0 - eigh * (q1 - q0));
+    conv = 0.5 * (q1 * u1 + q0 * u0 - eigh * (q1 - q0));
+    const double gx = m1.v[3 * D] + m0.v[3 * D], gy = m1.v[3 * D + 1] + m0.v[3 * D + 1];
+    const double gz = m1.v[3 * D + 2] + m0.v[3 * D + 2];
+    const double ev = (C.rei + 0.5 * (q1 + q0)) / sig;
+    const double ucna = 0.5 * fabs(u1 + u0);
+    av = ucna + ev * (gx * gx + gy * gy + gz * gz) / (m1.v[9] + m0.v[9]);
 }
 
-// Face and node coefficients of the three ADI operators
-// (rhs_sa_adi_solver.F90:53-75 and copies), for every fine node:
-//   av_D = |U/J face average| + (1/Re + nu~ face average)/sig * |grad csi_D|^2_face / (J + J')
-// on the face between the node and its +D neighbour, and dtj = dtev J / 2.
-// nu~ does not change between the three sweeps of one sub-iteration, so
-// k_sa_rhs computes them while it holds the node's metrics.
-__device__ __forceinline__ void sa_adi_coef(const Level& L, const Fields& F, const Consts& C, long t, int i, int j,
-                                            int k) {
+__global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts C, int kc) {
+    __shared__ double s_q[3][kQLN];      // nu~ of planes k-1, k, k+1 (slot k mod 3)
+    __shared__ double s_m[kQM][kQLN];    // metrics, J of plane k
+    __shared__ double s_u[2][kQLN];      // U/J, V/J of plane k
+    __shared__ double s_fv[2][kQY][kQX];  // viscous flux of the +i, +j faces
+    __shared__ double s_fc[2][kQY][kQX];  // convective flux of the +i, +j faces
     const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+    const int bx = (int)(lb % gx), by = (int)((lb / gx) % gy), bz = (int)(lb / (gx * gy));
+    const int tid = (int)threadIdx.x;
+    const int tx = tid % kQX, ty = tid / kQX;
+    const int i0 = bx * (kQX - 1), j0 = by * (kQY - 1);
+    const int kb = 1 + bz * kc, ke = min(kb + kc, nk - 1);
+    const long sJ = ni, sK = (long)ni * nj;
+    const int own = ty * kQLX + tx;
+    const int hr = tid < kQLY ? tid : kQY, hcol = tid < kQLY ? kQX : tid - kQLY;
+    const int halo = hr * kQLX + hcol;
+    const bool has_halo = tid < kQHalo;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own_ok = i < ni && j < nj;
+    const bool halo_ok = has_halo && i0 + hcol < ni && j0 + hr < nj;
+    const long col = (long)i + sJ * j;                 // level-local
+    const long hc = (long)(i0 + hcol) + sJ * (j0 + hr);
     const long o = L.off;
+    const bool in_i = i <= ni - 2, in_j = j <= nj - 2;
+    const bool out = tx >= 1 && ty >= 1 && in_i && in_j;
+    const bool face_i = ty >= 1 && in_i && in_j;
+    const bool face_j = tx >= 1 && in_i && in_j;
     const double* q5 = F.q[4];
-    F.dtj[t] = 0.5 * F.dtev[o + t] * F.aj[o + t];
-    const int pos[3] = {i, j, k};
-    const int nd[3] = {ni, nj, nk};
-    const long st[3] = {1, (long)ni, (long)ni * nj};
-#pragma unroll
-    for (int D = 0; D < 3; D++) {
-        if (pos[D] >= nd[D] - 1) continue;
-        const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
-        const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
-        const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
-        const long p = t, p1 = t + st[D];
-        const double gx = m0[o + p1] + m0[o + p], gy = m1[o + p1] + m1[o + p], gz = m2[o + p1] + m2[o + p];
-        const double ev = (C.rei + 0.5 * (q5[o + p1] + q5[o + p])) / sig;
-        const double ucna = 0.5 * fabs(F.ucn[D][p1] + F.ucn[D][p]);
-        F.av[D][t] = ucna + ev * (gx * gx + gy * gy + gz * gz) / (F.aj[o + p1] + F.aj[o + p]);
-    }
-}
 
-__global__ void __launch_bounds__(256) k_sa_rhs(Level L, Fields F, Consts C) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
-    sa_adi_coef(L, F, C, t, i, j, k);
-    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
-        F.rh5[t] = 0.0;  // des_eddy.F90:231-236
-        return;
-    }
-    const long sI = 1, sJ = ni, sK = (long)ni * nj;
-    const long g = L.off + t;
-    const double* q5 = F.q[4];
-    // gradient of nu~ (rhs_sa_grdnu.F90:22-60)
-    double dnu;
-    {
-        const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
-        const double qip = q5[g + sI] / F.aj[g + sI], qim = q5[g - sI] / F.aj[g - sI];
-        const double qjp = q5[g + sJ] / F.aj[g + sJ], qjm = q5[g - sJ] / F.aj[g - sJ];
-        const double qkp = q5[g + sK] / F.aj[g + sK], qkm = q5[g - sK] / F.aj[g - sK];
-        double d[3];
+    auto qmet_at = [&](int node) {
+        QMet m;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            d[c] = dc2 * (qip * F.csi[c][g + sI] - qim * F.csi[c][g - sI]);
-            d[c] = d[c] + de2 * (qjp * F.eta[c][g + sJ] - qjm * F.eta[c][g - sJ]);
-            d[c] = d[c] + dz2 * (qkp * F.zet[c][g + sK] - qkm * F.zet[c][g - sK]);
-        }
-        const double aj2 = F.aj[g] * F.aj[g];
-        dnu = aj2 * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int c = 0; c < kQM; c++) m.v[c] = s_m[c][node];
+        return m;
+    };
+
+    // prologue: nu~ of planes kb-1 .. kb+1; metrics, J, U/J, V/J of plane kb
+    for (int k = kb - 1; k <= kb + 1; k++) {
+        const int sl = k % 3;
+        s_q[sl][own] = own_ok ? q5[o + col + sK * k] : 0.0;
+        if (has_halo) s_q[sl][halo] = halo_ok ? q5[o + hc + sK * k] : 0.0;
     }
-    // production, destruction, cb2 diffusion (rhs_sa_prod.F90:97-165)
-    double psa;
-    {
-        const double wdv = F.wd[g];
-        if (wdv == 0.0) {
-            psa = 0.0;
-        } else {
-            const double q = q5[g];
-            const double chi = q / C.rei;
-            const double chic = chi * chi * chi;
-            const double fv1 = chic / (chic + cv1c);
-            const double fv2 = pow(1.0 + chi / cv2, -3.0);
-            const double fv3 = ((1.0 + chi * fv1) * (1.0 - fv2)) / chi;
-            const double vtcm = sqrt(2.0 * F.wo[t]);
-            const double cawd = capa * capa * wdv * wdv;
-            const double vtc_org = fv3 * vtcm + q * fv2 / cawd;
-            const double vtc = fmax(vtc_org, 0.00001);
-            psa = cb1 * vtc * q;
-            const double rr = q / (vtc * cawd);
-            const double rr6 = rr * rr * rr * rr * rr * rr;
-            const double grr = fmax(0.0, rr + cw2 * (rr6 - rr));
-            const double grr6 = grr * grr * grr * grr * grr * grr;
-            const double fw = grr * pow((1.0 + cw3h) / (grr6 + cw3h), 1.0 / 6.0);
-            const double vwd = q / wdv;
-            psa = psa - cw1 * fw * vwd * vwd;
-            psa = psa + cb2 * dnu / sig;
-        }
+    QMet mcur, mh;
+    qload_met(F, o + col + sK * kb, own_ok, mcur);
+    qload_met(F, o + hc + sK * kb, halo_ok, mh);
+#pragma unroll
+    for (int c = 0; c < kQM; c++) {
+        s_m[c][own] = mcur.v[c];
+        if (has_halo) s_m[c][halo] = mh.v[c];
     }
-    // diffusion (rhs_sa_viscous.F90:58-188)
-    double visc = sa_visc_face<0>(L, F, C, t, sI, sJ, sK) - sa_visc_face<0>(L, F, C, t - sI, sI, sJ, sK);
-    visc = visc + sa_visc_face<1>(L, F, C, t, sI, sJ, sK) - sa_visc_face<1>(L, F, C, t - sJ, sI, sJ, sK);
-    visc = visc + sa_visc_face<2>(L, F, C, t, sI, sJ, sK) - sa_visc_face<2>(L, F, C, t - sK, sI, sJ, sK);
-    // convection (rhs_sa_flux.F90:226-303)
-    double r = sa_conv_face<0>(L, F, t, sI) - sa_conv_face<0>(L, F, t - sI, sI);
-    r = r + sa_conv_face<1>(L, F, t, sJ) - sa_conv_face<1>(L, F, t - sJ, sJ);
-    r = r + sa_conv_face<2>(L, F, t, sK) - sa_conv_face<2>(L, F, t - sK, sK);
-    // BDF2, diffusion and source (rhs_sa_unst_visc.F90:188-200)
-    const double dtj = 2.0 * C.delti * F.aj[g];
-    const double dq5dt = C.e_source * (3.0 * q5[g] - 4.0 * F.qn[4][g] + F.qnm1[4][g]) / dtj;
-    r = r + dq5dt - visc - psa / F.aj[g];
-    // explicit source form (rhs_sa_implicit_source.F90:244-256, dk = 1)
-    F.rh5[t] = -F.dtev[g] * F.aj[g] * r;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        s_u[d][own] = own_ok ? F.ucn[d][col + sK * kb] : 0.0;
+        if (has_halo) s_u[d][halo] = halo_ok ? F.ucn[d][hc + sK * kb] : 0.0;
+    }
+    QMet mprev;
+    qload_met(F, o + col + sK * (kb - 1), own_ok, mprev);
+    double u2prev = own_ok ? F.ucn[2][col + sK * (kb - 1)] : 0.0;
+    double u2cur = own_ok ? F.ucn[2][col + sK * kb] : 0.0;
+    __syncthreads();
+    double fkv = 0.0, fkc = 0.0;  // fluxes of the face below the current plane
+    if (out) {
+        const int sl0 = (kb - 1) % 3, sl1 = kb % 3;
+        auto q = [&](int a, int b, int c) { return s_q[c <= 0 ? sl0 : sl1][own + b * kQLX + a]; };
+        fkv = sa_visc_face_t<2>(L, C, q, mprev, mcur);
+        double av;
+        sa_conv_av<2>(C, s_q[sl0][own], s_q[sl1][own], u2prev, u2cur, mprev, mcur, fkc, av);
+        if (kb == 1) F.av[2][col] = av;  // the face k = 1/2, position 0
+    }
+    for (int k = kb; k < ke; k++) {
+        // next planes into registers
+        QMet mnext, mhn;
+        qload_met(F, o + col + sK * (k + 1), own_ok, mnext);
+        qload_met(F, o + hc + sK * (k + 1), halo_ok, mhn);
+        double un[2], uh[2];
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            un[d] = own_ok ? F.ucn[d][col + sK * (k + 1)] : 0.0;
+            uh[d] = halo_ok ? F.ucn[d][hc + sK * (k + 1)] : 0.0;
+        }
+        const double u2next = own_ok ? F.ucn[2][col + sK * (k + 1)] : 0.0;
+        const bool more = k + 1 < ke;
+        double qn_o = 0.0, qn_h = 0.0;
+        if (more) {
+            qn_o = own_ok ? q5[o + col + sK * (k + 2)] : 0.0;
+            qn_h = halo_ok ? q5[o + hc + sK * (k + 2)] : 0.0;
+        }
+        const int slm = (k + 2) % 3, sl0 = k % 3, slp = (k + 1) % 3;
+        auto q = [&](int a, int b, int c) {
+            return s_q[c < 0 ? slm : c == 0 ? sl0 : slp][own + b * kQLX + a];
+        };
+        const long t = col + sK * k;  // level-local node
+        if (face_i) {
+            const QMet m1 = qmet_at(own + 1);
+            s_fv[0][ty][tx] = sa_visc_face_t<0>(L, C, q, mcur, m1);
+            double cv, av;
+            sa_conv_av<0>(C, q(0, 0, 0), q(1, 0, 0), s_u[0][own], s_u[0][own + 1], mcur, m1, cv, av);
+            s_fc[0][ty][tx] = cv;
+            F.av[0][t] = av;
+        }
+        if (face_j) {
+            const QMet m1 = qmet_at(own + kQLX);
+            s_fv[1][ty][tx] = sa_visc_face_t<1>(L, C, q, mcur, m1);
+            double cv, av;
+            sa_conv_av<1>(C, q(0, 0, 0), q(0, 1, 0), s_u[1][own], s_u[1][own + kQLX], mcur, m1, cv, av);
+            s_fc[1][ty][tx] = cv;
+            F.av[1][t] = av;
+        }
+        double fkv1 = 0.0, fkc1 = 0.0;
+        double dnu = 0.0;
+        if (out) {
+            fkv1 = sa_visc_face_t<2>(L, C, q, mcur, mnext);
+            double av;
+            sa_conv_av<2>(C, q(0, 0, 0), q(0, 0, 1), u2cur, u2next, mcur, mnext, fkc1, av);
+            F.av[2][t] = av;
+            // gradient of nu~ (rhs_sa_grdnu.F90:22-60)
+            const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
+            const double qip = q(1, 0, 0) / s_m[9][own + 1], qim = q(-1, 0, 0) / s_m[9][own - 1];
+            const double qjp = q(0, 1, 0) / s_m[9][own + kQLX], qjm = q(0, -1, 0) / s_m[9][own - kQLX];
+            const double qkp = q(0, 0, 1) / mnext.v[9], qkm = q(0, 0, -1) / mprev.v[9];
+            double d[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                d[c] = dc2 * (qip * s_m[c][own + 1] - qim * s_m[c][own - 1]);
+                d[c] = d[c] + de2 * (qjp * s_m[3 + c][own + kQLX] - qjm * s_m[3 + c][own - kQLX]);
+                d[c] = d[c] + dz2 * (qkp * mnext.v[6 + c] - qkm * mprev.v[6 + c]);
+            }
+            const double aj2 = mcur.v[9] * mcur.v[9];
+            dnu = aj2 * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        }
+        __syncthreads();
+        if (out) {
+            const long g = o + t;
+            const double aj = mcur.v[9];
+            // production, destruction, cb2 diffusion (rhs_sa_prod.F90:97-165)
+            double psa;
+            const double wdv = F.wd[g];
+            if (wdv == 0.0) {
+                psa = 0.0;
+            } else {
+                const double qv = q(0, 0, 0);
+                const double chi = qv / C.rei;
+                const double chic = chi * chi * chi;
+                const double fv1 = chic / (chic + cv1c);
+                const double xf = 1.0 + chi / cv2;
+                const double fv2 = 1.0 / (xf * xf * xf);
+                const double fv3 = ((1.0 + chi * fv1) * (1.0 - fv2)) / chi;
+                const double vtcm = sqrt(2.0 * F.wo[t]);
+                const double cawd = capa * capa * wdv * wdv;
+                const double vtc_org = fv3 * vtcm + qv * fv2 / cawd;
+                const double vtc = fmax(vtc_org, 0.00001);
+                psa = cb1 * vtc * qv;
+                const double rr = qv / (vtc * cawd);
+                const double rr6 = rr * rr * rr * rr * rr * rr;
+                const double grr = fmax(0.0, rr + cw2 * (rr6 - rr));
+                const double grr6 = grr * grr * grr * grr * grr * grr;
+                const double fw = grr * cbrt(sqrt((1.0 + cw3h) / (grr6 + cw3h)));
+                const double vwd = qv / wdv;
+                psa = psa - cw1 * fw * vwd * vwd;
+                psa = psa + cb2 * dnu / sig;
+            }
+            double visc = s_fv[0][ty][tx] - s_fv[0][ty][tx - 1];
+            visc = visc + s_fv[1][ty][tx] - s_fv[1][ty - 1][tx];
+            visc = visc + fkv1 - fkv;
+            double r = s_fc[0][ty][tx] - s_fc[0][ty][tx - 1];
+            r = r + s_fc[1][ty][tx] - s_fc[1][ty - 1][tx];
+            r = r + fkc1 - fkc;
+            const double dtjb = 2.0 * C.delti * aj;
+            const double dq5dt = C.e_source * (3.0 * q(0, 0, 0) - 4.0 * F.qn[4][g] + F.qnm1[4][g]) / dtjb;
+            r = r + dq5dt - visc - psa / aj;
+            const double dtev = F.dtev[g];
+            F.rh5[t] = -dtev * aj * r;
+            F.dtj[t] = 0.5 * dtev * aj;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kQM; c++) {
+            s_m[c][own] = mnext.v[c];
+            if (has_halo) s_m[c][halo] = mhn.v[c];
+        }
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            s_u[d][own] = un[d];
+            if (has_halo) s_u[d][halo] = uh[d];
+        }
+        if (more) {
+            s_q[slm][own] = qn_o;
+            if (has_halo) s_q[slm][halo] = qn_h;
+        }
+        mprev = mcur;
+        mcur = mnext;
+        u2cur = u2next;
+        fkv = fkv1;
+        fkc = fkc1;
+        __syncthreads();
+    }
 }
 
 // one ADI sweep (rhs_sa_adi_solver.F90:44-244), one thread per line:
@@ -542,8 +686,12 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
     const unsigned fj = (unsigned)(((long)L.ni * L.nk + b - 1) / b);
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
+    const long tiles = (long)((L.ni - 2 + kQX - 2) / (kQX - 1)) * ((L.nj - 2 + kQY - 2) / (kQY - 1));
+    const int kcq = (int)std::max(1L, std::min(16L, tiles * (L.nk - 2) / 2048));
+    const dim3 gq((unsigned)((L.ni - 2 + kQX - 2) / (kQX - 1)), (unsigned)((L.nj - 2 + kQY - 2) / (kQY - 1)),
+                  (unsigned)((L.nk - 2 + kcq - 1) / kcq));
     for (int it = 0; it < itk; it++) {
-        hipLaunchKernelGGL(k_sa_rhs, dim3(gn), dim3(b), 0, s, L, F, C);
+        hipLaunchKernelGGL(k_sa_rhs, gq, dim3(kQX * kQY), 0, s, L, F, C, kcq);
         hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(kSNT), 0, s, L, F);
         hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
