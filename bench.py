@@ -183,7 +183,10 @@ def dual_time_leg(args):
     t_build = time.perf_counter() - t_build
     sp = Solver(case.params, device=0).load_case(case)
     sm = Solver(case_mixed.params, device=0).load_case(case_mixed)
-    for sv in (sp, sm):
+    for sv, cc in ((sp, case), (sm, case_mixed)):
+        if args.autotune:
+            sv.autotune(reps=3)
+            sv.set_state(*cc.state())
         sv.ws_configure(hidden=32, samples_per_record=65536, capacity=1 << 22, seed=20260505)
     # warm-up history (3 plain steps at the cap's first 30 iterations), then
     # the training steps, each recorded after it converged
@@ -276,6 +279,8 @@ def main():
     ap.add_argument("--iters", type=int, default=10, help="pseudo-iterations per physical step")
     ap.add_argument("--n", type=int, default=129, help="nodes per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes of the LDS-tiled kernels")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--conv-itm", type=int, default=120,
@@ -303,11 +308,20 @@ def main():
 
     from desalm.params import PREC_FP64, PREC_MIXED
 
+    tiles_used = {}
+
     def timed_run(precision, profile):
         """warmup + timed physical steps; returns (Mcell-iters/s, wall s, iterations, profile stats,
         last residual history, state finite, pivot events)"""
         cs = cases.channel(args.n, args.n, args.n, ng=3, itm=args.iters, iters=(1, 2, 3), precision=precision)
         sv = Solver(cs.params, device=0).load_case(cs)
+        if args.autotune:
+            # launch shapes of the LDS-tiled kernels, timed on this state
+            # (scratch only), then the state is reloaded
+            tiles_used[precision] = sv.autotune(reps=3)
+            sv.set_state(*cs.state())
+        else:
+            tiles_used[precision] = sv.get_tiles()
         for _ in range(args.warmup):
             sv.mg_driver()
         if profile:
@@ -396,6 +410,9 @@ def main():
                       "fine_nodes": n1, "pseudo_iterations": iters_done, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "converge": conv,
            "kernels_ms": {f"{nm}@L{n}": [cnt, round(ms, 3)] for (nm, n), (cnt, ms) in sorted(stats.items())},
+           "tiles": {"k_planes_per_workgroup_visc_by_level": tiles_used[PREC_FP64][0],
+                     "k_planes_per_workgroup_sa_rhs": tiles_used[PREC_FP64][1],
+                     "chosen_by": "desalm_autotune" if args.autotune else "default"},
            "state_finite": finite, "pivot_events": pivots,
            "final_log10_update_norms": [round(float(v), 4) for v in hist[-1]]}
     print(json.dumps(out))

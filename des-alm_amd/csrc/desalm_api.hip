@@ -95,6 +95,10 @@ struct desalm_ctx {
     double cvin = 0.0;
     Alm alm;  // actuator-line rotors (desalm_set_turbines)
     WarmStart ws;
+    // k planes per workgroup of the LDS-tiled viscous kernel per level and
+    // of the SA right-hand side (0: default; desalm_autotune / desalm_set_tiles)
+    int kc_visc[kMaxLevels + 1] = {};
+    int kc_sa = 0;
 };
 
 namespace {
@@ -221,14 +225,15 @@ void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out) {
     const Level& L = c->lv[n];
     hipStream_t s = c->stream;
     timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C); });
-    timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk); });
+    timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk, c->kc_visc[n]); });
     // actuator-line forcing on every level (solver_daf.F90:378-381)
     if (!c->alm.t.empty()) timed(c, KID_ALM, n, [&] { launch_alm_rhs(s, L, c->F, c->alm, c->C.ren); });
     timed(c, KID_DAF_I, n, [&] { launch_daf_i(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_J, n, [&] { launch_daf_j(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_K, n, [&] { launch_daf_k(s, L, c->F, c->C, n, dq_out); });
     if (n == 1) timed(c, KID_BCOND, n, [&] { bcond(c, s); });
-    if (recalc == 1) timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0); });
+    if (recalc == 1)
+        timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0, c->kc_visc[n]); });
 }
 
 int copy_in_level(desalm_ctx* c, double* const* soa, int ncomp, const double* host, const Level& L) {
@@ -615,7 +620,7 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
                 if (n != ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
                 if (p.des && n == 1) {
                     for (desalm_ctx* c : zs)
-                        timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, c->lv[1], c->F, c->C, c->p.itk); });
+                        timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, c->lv[1], c->F, c->C, c->p.itk, c->kc_sa); });
                     if (have_table) timed(c0, KID_OVERSET, 1, [&] { exchange(1); });
                 }
                 for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr);
@@ -789,7 +794,7 @@ int desalm_des_eddy(desalm_ctx* c, double* q, const double* qn5, const double* q
     HIPCHK(c, hipMemcpyAsync(c->F.dtev, dtev, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemcpyAsync(c->F.xnut, xnut, sizeof(double) * L.n, hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
-    launch_des_eddy(s, L, c->F, c->C, c->p.itk);
+    launch_des_eddy(s, L, c->F, c->C, c->p.itk, c->kc_sa);
     HIPCHK(c, hipGetLastError());
     if (copy_out_level(c, c->F.q, 5, q, L)) return -1;
     HIPCHK(c, hipMemcpyAsync(xnut, c->F.xnut, sizeof(double) * L.n, hipMemcpyDeviceToHost, s));
@@ -1229,6 +1234,73 @@ int desalm_bintp(desalm_ctx* c, int which) {
     launch_bintp(c->stream, c->lv[1], c->F, c->C, c->ov, which != 0, c->cvin, c->p.des);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int desalm_set_tiles(desalm_ctx* c, const int* kc_visc, int kc_sa) {
+    if (!c) return fail(c, "null argument");
+    for (int n = 1; n <= c->ng; n++) {
+        const int v = kc_visc ? kc_visc[n - 1] : 0;
+        if (v < 0 || v > 64) return fail(c, "tiles: planes per workgroup must be in 0..64");
+    }
+    if (kc_sa < 0 || kc_sa > 64) return fail(c, "tiles: planes per workgroup must be in 0..64");
+    for (int n = 1; n <= c->ng; n++) c->kc_visc[n] = kc_visc ? kc_visc[n - 1] : 0;
+    c->kc_sa = kc_sa;
+    return 0;
+}
+
+int desalm_get_tiles(const desalm_ctx* c, int* kc_visc, int* kc_sa) {
+    if (!c) return fail(nullptr, "null argument");
+    for (int n = 1; n <= c->ng; n++)
+        if (kc_visc) kc_visc[n - 1] = c->kc_visc[n] > 0 ? c->kc_visc[n] : visc_kc_default(c->lv[n]);
+    if (kc_sa) *kc_sa = c->kc_sa > 0 ? c->kc_sa : sa_kc_default(c->lv[1]);
+    return 0;
+}
+
+int desalm_autotune(desalm_ctx* c, int reps) {
+    if (!c) return fail(c, "null argument");
+    if (reps < 1 || reps > 1000) return fail(c, "autotune: reps must be in 1..1000");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    static const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16};
+    // times one launch configuration: a warm-up launch, then reps launches
+    // between two events; the launches only rewrite scratch (rh, rh5, av, dtj)
+    auto time_it = [&](auto launch, float& ms) -> int {
+        launch();
+        HIPCHK(c, hipEventRecord(c->ev0, s));
+        for (int r = 0; r < reps; r++) launch();
+        HIPCHK(c, hipEventRecord(c->ev1, s));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        return 0;
+    };
+    for (int n = 1; n <= c->ng; n++) {
+        const Level& L = c->lv[n];
+        float best = 1e30f;
+        int pick = 0;
+        for (int kc : cand) {
+            if (kc > L.nk - 2) break;
+            float ms = 0.0f;
+            if (time_it([&] { launch_visc_pk(s, L, c->F, c->C, n, 0, kc); }, ms)) return -1;
+            if (ms < best) { best = ms; pick = kc; }
+        }
+        c->kc_visc[n] = pick;
+    }
+    if (c->p.des) {
+        const Level& L = c->lv[1];
+        launch_sa_prep(s, L, c->F);
+        float best = 1e30f;
+        int pick = 0;
+        for (int kc : cand) {
+            if (kc > L.nk - 2) break;
+            float ms = 0.0f;
+            if (time_it([&] { launch_sa_rhs(s, L, c->F, c->C, kc); }, ms)) return -1;
+            if (ms < best) { best = ms; pick = kc; }
+        }
+        c->kc_sa = pick;
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
     return 0;
 }
 

@@ -160,8 +160,11 @@ struct Ctx;  // host context, desalm_api.hip
 // ---- launchers (each enqueues on `s`) ----
 // residual assembly, k_residual.hip
 void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
+// kc: k planes per workgroup of the LDS-tiled kernels (0: default)
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n,
-                     int recalc, int calc_pk);
+                     int recalc, int calc_pk, int kc);
+void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int calc_pk, int kc);
+int visc_kc_default(const Level& L);
 // DAF implicit operator, k_daf.hip
 void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n);
 void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n);
@@ -171,7 +174,10 @@ void launch_daf_k(hipStream_t s, const Level& L, const Fields& F, const Consts& 
 void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix,
                      int jfix, int kfix);
 // SA / DES, k_sa.hip
-void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk);
+void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk, int kc);
+void launch_sa_prep(hipStream_t s, const Level& L, const Fields& F);
+void launch_sa_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int kc);
+int sa_kc_default(const Level& L);
 // multigrid, k_mg.hip
 void launch_inject(hipStream_t s, const Level* lv, int ng, int icrs, int jcrs, int kcrs,
                    double* const* comps, int ncomp);

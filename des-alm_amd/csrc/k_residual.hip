@@ -445,19 +445,28 @@ void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C
     hipLaunchKernelGGL(k_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C, 1);
 }
 
+int visc_kc_default(const Level& L) {
+    // planes per workgroup: enough workgroups (about 2048) to fill the chip
+    const long tiles = (long)((L.ni - 2 + kTX - 2) / (kTX - 1)) * ((L.nj - 2 + kTY - 2) / (kTY - 1));
+    return (int)std::max(1L, std::min(16L, tiles * (L.nk - 2) / 2048));
+}
+
+void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int calc_pk, int kc) {
+    if (kc <= 0) kc = visc_kc_default(L);
+    kc = std::min(kc, L.nk - 2);
+    const dim3 gv((unsigned)((L.ni - 2 + kTX - 2) / (kTX - 1)), (unsigned)((L.nj - 2 + kTY - 2) / (kTY - 1)),
+                  (unsigned)((L.nk - 2 + kc - 1) / kc));
+    hipLaunchKernelGGL(k_visc_pk, gv, dim3(kTX * kTY), 0, s, L, F, C, n, calc_pk, kc);
+}
+
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int recalc,
-                     int calc_pk) {
+                     int calc_pk, int kc) {
     const int b = 256;
     const dim3 grid((unsigned)((L.n + b - 1) / b));
     // the recalc branch needs U/J and p/J of the updated q (dtau is kept)
     if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, 0);
     hipLaunchKernelGGL(k_residual, grid, dim3(b), 0, s, L, F, C, n, recalc);
-    // planes per workgroup: enough workgroups (about 2048) to fill the chip
-    const long tiles = (long)((L.ni - 2 + kTX - 2) / (kTX - 1)) * ((L.nj - 2 + kTY - 2) / (kTY - 1));
-    const int kc = (int)std::max(1L, std::min(16L, tiles * (L.nk - 2) / 2048));
-    const dim3 gv((unsigned)((L.ni - 2 + kTX - 2) / (kTX - 1)), (unsigned)((L.nj - 2 + kTY - 2) / (kTY - 1)),
-                  (unsigned)((L.nk - 2 + kc - 1) / kc));
-    hipLaunchKernelGGL(k_visc_pk, gv, dim3(kTX * kTY), 0, s, L, F, C, n, recalc ? 0 : calc_pk, kc);
+    launch_visc_pk(s, L, F, C, n, recalc ? 0 : calc_pk, kc);
 }
 
 }  // namespace desalm

@@ -72,10 +72,14 @@ __global__ void k_sa_prep(Level L, Fields F) {
 // into the next plane.  The ADI coefficients are written on the faces the
 // sweeps read (positions 0 .. n_D - 2 along D, interior across), dtj on the
 // interior; rh5 on the interior (the sweeps never read its boundary).
-// Operand order follows the reference routines, except that the two powers
-// of rhs_sa_prod.F90:47,84 are taken as 1 / x^3 and cbrt(sqrt(y)) instead of
-// pow(x, -3) and pow(y, 1/6): a rounding-level difference that saves about
-// a quarter of the kernel's time (FP64 pow is long on the device).  The ADI
+// Operand order follows the reference routines, except for two
+// rounding-level substitutions, both for speed (the kernel is bound by its
+// FP64 arithmetic, not by memory):
+// - the two powers of rhs_sa_prod.F90:47,84 are 1 / x^3 and cbrt(sqrt(y))
+//   instead of pow(x, -3) and pow(y, 1/6), about a quarter of the kernel's
+//   time;
+// - every division is qdiv: a product with the Newton-refined hardware
+//   reciprocal, about a tenth.  The ADI
 // face coefficients are
 //   av_D = |U/J face average| + (1/Re + nu~ face average)/sig * |grad csi_D|^2_face / (J + J')
 // (rhs_sa_adi_solver.F90:53-75); nu~ does not change between the three
@@ -84,6 +88,16 @@ constexpr int kQX = 32, kQY = 8;
 constexpr int kQLX = kQX + 1, kQLY = kQY + 1, kQLN = kQLX * kQLY;
 constexpr int kQHalo = kQLN - kQX * kQY;
 constexpr int kQM = 10;  // csi 3, eta 3, zet 3, J
+
+// a / b as a times the reciprocal of b: the hardware reciprocal refined by
+// two Newton-Raphson steps r <- r (2 - b r) (quadratic convergence from the
+// instruction's ~2^-40 estimate), then one product
+__device__ __forceinline__ double qdiv(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return a * r;
+}
 
 struct QMet {
     double v[kQM];
@@ -131,8 +145,8 @@ __device__ __forceinline__ double sa_visc_face_t(const Level& L, const Consts& C
     const double cc3 = f1 * c1 + f2 * c2 + f3 * c3;
     const double q1 = D == 0 ? q(1, 0, 0) : D == 1 ? q(0, 1, 0) : q(0, 0, 1);
     const double q0 = q(0, 0, 0);
-    const double ret1 = (C.rei + q1) / sig, ret0 = (C.rei + q0) / sig;
-    const double vtj = 0.25 * (ret1 + ret0) / (m1.v[9] + m0.v[9]);
+    const double ret1 = qdiv(C.rei + q1, sig), ret0 = qdiv(C.rei + q0, sig);
+    const double vtj = qdiv(0.25 * (ret1 + ret0), m1.v[9] + m0.v[9]);
     return vtj * (cc1 * vcs + cc2 * vet + cc3 * vze);
 }
 
@@ -145,9 +159,9 @@ __device__ __forceinline__ void sa_conv_av(const Consts& C, double q0, double q1
     conv = 0.5 * (q1 * u1 + q0 * u0 - eigh * (q1 - q0));
     const double gx = m1.v[3 * D] + m0.v[3 * D], gy = m1.v[3 * D + 1] + m0.v[3 * D + 1];
     const double gz = m1.v[3 * D + 2] + m0.v[3 * D + 2];
-    const double ev = (C.rei + 0.5 * (q1 + q0)) / sig;
+    const double ev = qdiv(C.rei + 0.5 * (q1 + q0), sig);
     const double ucna = 0.5 * fabs(u1 + u0);
-    av = ucna + ev * (gx * gx + gy * gy + gz * gz) / (m1.v[9] + m0.v[9]);
+    av = ucna + qdiv(ev * (gx * gx + gy * gy + gz * gz), m1.v[9] + m0.v[9]);
 }
 
 __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts C, int kc) {
@@ -269,9 +283,9 @@ __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts 
             F.av[2][t] = av;
             // gradient of nu~ (rhs_sa_grdnu.F90:22-60)
             const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
-            const double qip = q(1, 0, 0) / s_m[9][own + 1], qim = q(-1, 0, 0) / s_m[9][own - 1];
-            const double qjp = q(0, 1, 0) / s_m[9][own + kQLX], qjm = q(0, -1, 0) / s_m[9][own - kQLX];
-            const double qkp = q(0, 0, 1) / mnext.v[9], qkm = q(0, 0, -1) / mprev.v[9];
+            const double qip = qdiv(q(1, 0, 0), s_m[9][own + 1]), qim = qdiv(q(-1, 0, 0), s_m[9][own - 1]);
+            const double qjp = qdiv(q(0, 1, 0), s_m[9][own + kQLX]), qjm = qdiv(q(0, -1, 0), s_m[9][own - kQLX]);
+            const double qkp = qdiv(q(0, 0, 1), mnext.v[9]), qkm = qdiv(q(0, 0, -1), mprev.v[9]);
             double d[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -293,25 +307,25 @@ __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts 
                 psa = 0.0;
             } else {
                 const double qv = q(0, 0, 0);
-                const double chi = qv / C.rei;
+                const double chi = qdiv(qv, C.rei);
                 const double chic = chi * chi * chi;
-                const double fv1 = chic / (chic + cv1c);
-                const double xf = 1.0 + chi / cv2;
-                const double fv2 = 1.0 / (xf * xf * xf);
-                const double fv3 = ((1.0 + chi * fv1) * (1.0 - fv2)) / chi;
+                const double fv1 = qdiv(chic, chic + cv1c);
+                const double xf = 1.0 + qdiv(chi, cv2);
+                const double fv2 = qdiv(1.0, xf * xf * xf);
+                const double fv3 = qdiv((1.0 + chi * fv1) * (1.0 - fv2), chi);
                 const double vtcm = sqrt(2.0 * F.wo[t]);
                 const double cawd = capa * capa * wdv * wdv;
-                const double vtc_org = fv3 * vtcm + qv * fv2 / cawd;
+                const double vtc_org = fv3 * vtcm + qdiv(qv * fv2, cawd);
                 const double vtc = fmax(vtc_org, 0.00001);
                 psa = cb1 * vtc * qv;
-                const double rr = qv / (vtc * cawd);
+                const double rr = qdiv(qv, vtc * cawd);
                 const double rr6 = rr * rr * rr * rr * rr * rr;
                 const double grr = fmax(0.0, rr + cw2 * (rr6 - rr));
                 const double grr6 = grr * grr * grr * grr * grr * grr;
-                const double fw = grr * cbrt(sqrt((1.0 + cw3h) / (grr6 + cw3h)));
-                const double vwd = qv / wdv;
+                const double fw = grr * cbrt(sqrt(qdiv(1.0 + cw3h, grr6 + cw3h)));
+                const double vwd = qdiv(qv, wdv);
                 psa = psa - cw1 * fw * vwd * vwd;
-                psa = psa + cb2 * dnu / sig;
+                psa = psa + qdiv(cb2 * dnu, sig);
             }
             double visc = s_fv[0][ty][tx] - s_fv[0][ty][tx - 1];
             visc = visc + s_fv[1][ty][tx] - s_fv[1][ty - 1][tx];
@@ -320,8 +334,8 @@ __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts 
             r = r + s_fc[1][ty][tx] - s_fc[1][ty - 1][tx];
             r = r + fkc1 - fkc;
             const double dtjb = 2.0 * C.delti * aj;
-            const double dq5dt = C.e_source * (3.0 * q(0, 0, 0) - 4.0 * F.qn[4][g] + F.qnm1[4][g]) / dtjb;
-            r = r + dq5dt - visc - psa / aj;
+            const double dq5dt = C.e_source * qdiv(3.0 * q(0, 0, 0) - 4.0 * F.qn[4][g] + F.qnm1[4][g], dtjb);
+            r = r + dq5dt - visc - qdiv(psa, aj);
             const double dtev = F.dtev[g];
             F.rh5[t] = -dtev * aj * r;
             F.dtj[t] = 0.5 * dtev * aj;
@@ -675,10 +689,28 @@ __global__ void k_sa_clamp(Level L, Fields F, Consts C) {
 
 }  // namespace
 
-void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk) {
+int sa_kc_default(const Level& L) {
+    const long tiles = (long)((L.ni - 2 + kQX - 2) / (kQX - 1)) * ((L.nj - 2 + kQY - 2) / (kQY - 1));
+    return (int)std::max(1L, std::min(16L, tiles * (L.nk - 2) / 2048));
+}
+
+void launch_sa_prep(hipStream_t s, const Level& L, const Fields& F) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_sa_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F);
+}
+
+void launch_sa_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int kc) {
+    if (kc <= 0) kc = sa_kc_default(L);
+    kc = std::min(kc, L.nk - 2);
+    const dim3 gq((unsigned)((L.ni - 2 + kQX - 2) / (kQX - 1)), (unsigned)((L.nj - 2 + kQY - 2) / (kQY - 1)),
+                  (unsigned)((L.nk - 2 + kc - 1) / kc));
+    hipLaunchKernelGGL(k_sa_rhs, gq, dim3(kQX * kQY), 0, s, L, F, C, kc);
+}
+
+void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk, int kc) {
     const int b = 256;
     const unsigned gn = (unsigned)((L.n + b - 1) / b);
-    hipLaunchKernelGGL(k_sa_prep, dim3(gn), dim3(b), 0, s, L, F);
+    launch_sa_prep(s, L, F);
     const int bl = 128;
     const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + kSL - 1) / kSL);
     const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + bl - 1) / bl);
@@ -686,12 +718,8 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
     const unsigned fj = (unsigned)(((long)L.ni * L.nk + b - 1) / b);
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
-    const long tiles = (long)((L.ni - 2 + kQX - 2) / (kQX - 1)) * ((L.nj - 2 + kQY - 2) / (kQY - 1));
-    const int kcq = (int)std::max(1L, std::min(16L, tiles * (L.nk - 2) / 2048));
-    const dim3 gq((unsigned)((L.ni - 2 + kQX - 2) / (kQX - 1)), (unsigned)((L.nj - 2 + kQY - 2) / (kQY - 1)),
-                  (unsigned)((L.nk - 2 + kcq - 1) / kcq));
     for (int it = 0; it < itk; it++) {
-        hipLaunchKernelGGL(k_sa_rhs, gq, dim3(kQX * kQY), 0, s, L, F, C, kcq);
+        launch_sa_rhs(s, L, F, C, kc);
         hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(kSNT), 0, s, L, F);
         hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);

@@ -87,11 +87,15 @@ def lib():
         L.desalm_mlp_forward.argtypes = [vp, fp, i64, fp]
         L.desalm_mlp_loss_grad.argtypes = [vp, fp, fp, i64, fp, dp]
         L.desalm_ws_history.argtypes = [vp, dp, dp]
+        L.desalm_set_tiles.argtypes = [vp, ip, ctypes.c_int]
+        L.desalm_get_tiles.argtypes = [vp, ip, ip]
+        L.desalm_autotune.argtypes = [vp, ctypes.c_int]
         L.desalm_turbine_angle.argtypes = [vp, ctypes.c_int, dp, dp]
         for f in ("desalm_set_convergence", "desalm_step", "desalm_ws_configure", "desalm_ws_mode",
                   "desalm_ws_record", "desalm_ws_train", "desalm_ws_param_count", "desalm_ws_params",
                   "desalm_ws_stats", "desalm_ws_features", "desalm_mlp_forward", "desalm_mlp_loss_grad",
-                  "desalm_ws_history", "desalm_turbine_angle"):
+                  "desalm_ws_history", "desalm_turbine_angle", "desalm_set_tiles", "desalm_get_tiles",
+                  "desalm_autotune"):
             getattr(L, f).restype = ctypes.c_int
         for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
@@ -218,6 +222,23 @@ class Solver:
         hist = np.zeros((max(self.params.itm, 1), 5))
         self._chk(lib().desalm_mg_driver(self.h, ctypes.byref(itc), _p(hist)), "desalm_mg_driver")
         return itc.value, hist[: itc.value]
+
+    def set_tiles(self, kc_visc=None, kc_sa=0):
+        """k planes per workgroup of the LDS-tiled kernels (0: default)."""
+        arr = (ctypes.c_int * self.params.ng)(*([0] * self.params.ng if kc_visc is None else kc_visc))
+        self._chk(lib().desalm_set_tiles(self.h, arr, kc_sa), "desalm_set_tiles")
+
+    def get_tiles(self):
+        arr = (ctypes.c_int * self.params.ng)()
+        sa = ctypes.c_int(0)
+        self._chk(lib().desalm_get_tiles(self.h, arr, ctypes.byref(sa)), "desalm_get_tiles")
+        return list(arr), sa.value
+
+    def autotune(self, reps=5):
+        """Time every launch shape of the LDS-tiled kernels on the current
+        state and keep the fastest; returns get_tiles()."""
+        self._chk(lib().desalm_autotune(self.h, reps), "desalm_autotune")
+        return self.get_tiles()
 
     def set_convergence(self, itm, it_min, er_min, eo_min):
         self._chk(lib().desalm_set_convergence(self.h, itm, it_min, er_min, eo_min), "desalm_set_convergence")

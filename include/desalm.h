@@ -290,6 +290,20 @@ int desalm_profile(desalm_ctx* ctx, int enable);
 int desalm_profile_get(const desalm_ctx* ctx, int kernel, int level, int64_t* count, double* total_ms);
 const char* desalm_profile_name(int kernel);
 
+/* Launch shape of the LDS-tiled stencil kernels: k planes per workgroup of
+ * the viscous kernel on each level (kc_visc[0 .. ng-1]) and of the SA
+ * right-hand side (fine level); 0 picks the default (about 2048 workgroups).
+ * The shape changes results only by rounding (where a workgroup's first k
+ * face is evaluated, multiply-adds may be contracted differently).
+ * desalm_autotune times every candidate
+ * (1, 2, 3, 4, 6, 8, 12, 16 planes) over reps launches on the context's
+ * current state and keeps the fastest; its launches only rewrite scratch.
+ * desalm_get_tiles reports the shapes in use.  No counterpart in the
+ * reference. */
+int desalm_set_tiles(desalm_ctx* ctx, const int* kc_visc, int kc_sa);
+int desalm_get_tiles(const desalm_ctx* ctx, int* kc_visc, int* kc_sa);
+int desalm_autotune(desalm_ctx* ctx, int reps);
+
 /* Convergence controls of the following desalm_mg_driver calls: at most
  * itm pseudo-iterations, stop after it_min when the largest u, v, w log10
  * norm has dropped below er_min relative to the first iteration or below

@@ -307,3 +307,25 @@ def test_mg_driver_alm(gpu, des):
     for m in range(5):
         assert rel(qg[:, m], qo[:, m]) < TOL, m
     assert rel(xg, xo) < TOL
+
+
+def test_tile_shapes_do_not_change_results(gpu):
+    # the LDS-tiled kernels' k planes per workgroup (set by hand or by the
+    # autotuner) only change the launch shape; where a workgroup's first k
+    # face is evaluated the compiler may contract multiply-adds differently,
+    # so results agree to rounding, not bit for bit
+    c = cases.channel(33, 33, 33, ng=2, itm=3, iters=(1, 2))
+    out = []
+    for kc_visc, kc_sa in (([1, 1], 1), ([16, 6], 16), (None, 0)):
+        s = Solver(c.params).load_case(c)
+        s.set_tiles(kc_visc, kc_sa)
+        if kc_visc is None:
+            tiles = s.autotune(reps=2)
+            assert all(t in (1, 2, 3, 4, 6, 8, 12, 16) for t in tiles[0] + [tiles[1]])
+            s.set_state(*c.state())
+        s.mg_driver()
+        out.append(s.get_state())
+        s.close()
+    for st in out[1:]:
+        for a, b in zip(out[0], st):
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-14 * np.abs(a).max())
