@@ -504,6 +504,167 @@ __global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
+// Two-way form of the j and k sweeps: each wave solves 32 lines, lanes
+// 0-31 eliminating the upper halves of the lines downwards (rows 0 .. m-1,
+// the Thomas recurrence of k_sa_adi), lanes 32-63 the lower halves upwards
+// (rows len-1 .. m+1, the mirrored recurrence e' = aw / den,
+// f' = (aq - ae f') / den).  The two meet at row m through a lane shuffle,
+// then substitute outwards.  Twice the waves and half the serial chain of
+// the one-way sweep, which runs one wave per CU at 129^3; rows below m are
+// eliminated in the other order, a rounding-level difference.  Rows where
+// LAPACK SGTSV would pivot are counted on the upper halves only.
+constexpr int kPF2 = 16;  // rows per prefetch group: 16 measured best among 4, 8, 16
+
+template <int D>
+__global__ void __launch_bounds__(128) k_sa_adi2(Level L, Fields F, Consts C) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const bool top = lane < 32;
+    const long nlines = D == 1 ? (long)(ni - 2) * (nk - 2) : (long)(ni - 2) * (nj - 2);
+    const long t = (long)blockIdx.x * 64 + wave * 32 + (lane & 31);
+    const bool act = t < nlines;
+    const long sD = D == 1 ? (long)ni : (long)ni * nj;
+    const int len = D == 1 ? nj - 2 : nk - 2;
+    long base = 0;
+    if (act) base = D == 1 ? (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2))
+                           : (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
+    const double* __restrict__ U = F.ucn[D];
+    const double* __restrict__ av = F.av[D];
+    const double* __restrict__ dtj = F.dtj;
+    double* __restrict__ rh = F.rh5;
+    double* __restrict__ cp = F.cp[0];
+    const int m = len / 2;
+    auto node = [&](int pos) { return base + (long)pos * sD; };  // line position 0 .. len + 1
+    // upper half: carries av(s), U(s), U(s+1); lower half: av(s+1), U(s+1), U(s+2)
+    double a_c = 0.0, u_m = 0.0, u_0 = 0.0;
+    if (act) {
+        if (top) { a_c = av[node(0)]; u_m = U[node(0)]; u_0 = U[node(1)]; }
+        else { a_c = av[node(len)]; u_m = U[node(len + 1)]; u_0 = U[node(len)]; }
+    }
+    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
+    unsigned piv = 0;
+    const int nrows = top ? m : len - 1 - m;
+    for (int g0 = 0; g0 < nrows; g0 += kPF2) {
+        double la[kPF2], lu[kPF2], ld[kPF2], lq[kPF2];
+#pragma unroll
+        for (int u = 0; u < kPF2; u++) {
+            const int r = g0 + u;
+            const int srow = top ? r : len - 1 - r;
+            const bool ok = act && r < nrows;
+            const long p = node(srow + 1);
+            la[u] = ok ? av[top ? node(srow + 1) : node(srow)] : 0.0;
+            lu[u] = ok ? U[top ? node(srow + 2) : node(srow)] : 0.0;
+            ld[u] = ok ? dtj[p] : 0.0;
+            lq[u] = ok ? rh[p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPF2; u++) {
+            const int r = g0 + u;
+            if (r < nrows) {
+                const int srow = top ? r : len - 1 - r;
+                double avm, avp, um, up;
+                if (top) { avm = a_c; avp = la[u]; um = u_m; up = lu[u]; }
+                else { avp = a_c; avm = la[u]; up = u_m; um = lu[u]; }
+                const double dt = ld[u];
+                const double aw = -dt * (um + avm);
+                const double ap = 1.0 + dt * (avm + avp);
+                const double ae = dt * (up - avp);
+                const double aq = lq[u];
+                double den;
+                if (r == 0) {
+                    den = ap;
+                } else {
+                    if (top && fabs(denprev) < fabs(aw)) piv++;
+                    den = top ? ap - aw * cprev : ap - ae * cprev;
+                }
+                const double inv = 1.0 / den;
+                if (top) {
+                    cprev = ae * inv;
+                    dprev = (r == 0) ? aq * inv : (aq - aw * dprev) * inv;
+                } else {
+                    cprev = aw * inv;
+                    dprev = (r == 0) ? aq * inv : (aq - ae * dprev) * inv;
+                }
+                denprev = den;
+                if (act) {
+                    const long p = node(srow + 1);
+                    cp[p] = cprev;
+                    rh[p] = dprev;
+                }
+                // carry the shared face and velocities to the next row
+                if (top) { a_c = avp; u_m = u_0; u_0 = up; }
+                else { a_c = avm; u_m = u_0; u_0 = um; }
+            }
+        }
+    }
+    // the middle row m, solved by the upper lane with the lower half's e', f'
+    const double e_lo = __shfl(cprev, (lane & 31) + 32, 64);
+    const double f_lo = __shfl(dprev, (lane & 31) + 32, 64);
+    double xm = 0.0;
+    if (top && act) {
+        const long p = node(m + 1);
+        const double avm = a_c, avp = av[node(m + 1)], um = u_m, up = U[node(m + 2)];
+        const double dt = dtj[p], aq = rh[p];
+        const double aw = -dt * (um + avm);
+        const double ap = 1.0 + dt * (avm + avp);
+        const double ae = dt * (up - avp);
+        if (m > 0 && fabs(denprev) < fabs(aw)) piv++;
+        const double den = (m > 0 ? ap - aw * cprev : ap) - (m < len - 1 ? ae * e_lo : 0.0);
+        const double num = (m > 0 ? aq - aw * dprev : aq) - (m < len - 1 ? ae * f_lo : 0.0);
+        xm = num / den;
+    }
+    xm = __shfl(xm, lane & 31, 64);
+    auto finish = [&](long p, double x) {
+        if (D < 2) {
+            rh[p] = x;
+        } else {
+            const long g = L.off + p;
+            const double qv = F.q[4][g];
+            const double qe5 = qv + x;  // qold == q5 at this point of the sub-iteration
+            const double dum = 1.0 + 1.5 * C.e_source * F.dtev[g] / C.delti;
+            F.q[4][g] = qv + (-qv + qe5) / dum;
+        }
+    };
+    if (top && act) finish(node(m + 1), xm);
+    // substitution outwards: upper rows m-1 .. 0, lower rows m+1 .. len-1
+    double x = xm;
+    const int nb = top ? m : len - 1 - m;
+    for (int g0 = 0; g0 < nb; g0 += kPF2) {
+        double dd[kPF2], cc[kPF2], qq[kPF2], dv[kPF2];
+#pragma unroll
+        for (int u = 0; u < kPF2; u++) {
+            const int r = g0 + u;
+            const int srow = top ? m - 1 - r : m + 1 + r;
+            const bool ok = act && r < nb;
+            const long p = node(srow + 1);
+            dd[u] = ok ? rh[p] : 0.0;
+            cc[u] = ok ? cp[p] : 0.0;
+            if (D == 2) {
+                qq[u] = ok ? F.q[4][L.off + p] : 0.0;
+                dv[u] = ok ? F.dtev[L.off + p] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPF2; u++) {
+            const int r = g0 + u;
+            if (r < nb && act) {
+                const int srow = top ? m - 1 - r : m + 1 + r;
+                x = dd[u] - cc[u] * x;
+                const long p = node(srow + 1);
+                if (D < 2) {
+                    rh[p] = x;
+                } else {
+                    const double qv = qq[u];
+                    const double qe5 = qv + x;
+                    const double dum = 1.0 + 1.5 * C.e_source * dv[u] / C.delti;
+                    F.q[4][L.off + p] = qv + (-qv + qe5) / dum;
+                }
+            }
+        }
+    }
+    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
+}
+
 // The i-direction sweep of k_sa_adi with its rows staged through LDS
 // (line_tile.h); one thread per line, kSL lines per workgroup, and a second
 // wave that only helps to move the tiles.  Chunks of av,
@@ -715,14 +876,18 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
     const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + kSL - 1) / kSL);
     const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + bl - 1) / bl);
     const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + bl - 1) / bl);
+    const unsigned gj2 = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + 63) / 64);
+    const unsigned gk2 = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + 63) / 64);
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
     const unsigned fj = (unsigned)(((long)L.ni * L.nk + b - 1) / b);
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
     for (int it = 0; it < itk; it++) {
         launch_sa_rhs(s, L, F, C, kc);
         hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(kSNT), 0, s, L, F);
-        hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
-        hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
+        if (L.nj - 2 >= 4) hipLaunchKernelGGL(k_sa_adi2<1>, dim3(gj2), dim3(128), 0, s, L, F, C);
+        else hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
+        if (L.nk - 2 >= 4) hipLaunchKernelGGL(k_sa_adi2<2>, dim3(gk2), dim3(128), 0, s, L, F, C);
+        else hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<0>, dim3(fi), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<1>, dim3(fj), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<2>, dim3(fk), dim3(b), 0, s, L, F, C);
