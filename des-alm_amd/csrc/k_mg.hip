@@ -5,7 +5,8 @@
 //   k_nlevel        mg_nlevel: injection of q, xnut, qold_mg and the 27-point
 //                   restriction of rh into pk (src/mg_nlevel.F90:2-150),
 //                   boundary pk zeroed (mg_zero_pk_boundary)
-//   k_prolong_fill  mg_prolong's correction and its k/j/i linear fills
+//   k_prolong_corr  mg_prolong's coarse correction q - qold_mg
+//   k_prolong_fill  its k/j/i linear fills onto the fine level
 //                   (src/mg_prolong.F90:57-170), evaluated per fine node
 //   k_rs<D>         rs_sweep constant-coefficient smoothing sweeps
 //                   (src/rs_sweep.F90:163-284), k sweep also adds dq to q
@@ -104,11 +105,20 @@ struct ProlongCtx {
     int icrs, jcrs, kcrs;
 };
 
-// coarse correction at a coincident fine node (mg_prolong.F90:57-75)
+// coarse correction q - qold_mg at every coarse node (mg_prolong.F90:57-75),
+// into the line-solve scratch F.cp (free during prolongation), level-local
+__global__ void __launch_bounds__(256) k_prolong_corr(Level Lc, Fields F) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Lc.n) return;
+    const long l = Lc.off + t;
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.cp[m][t] = F.q[m][l] - F.qold[m][l];
+}
+
+// coarse correction at a coincident fine node
 __device__ __forceinline__ double corr(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
     const int ci = P.icrs ? i / 2 : i, cj = P.jcrs ? j / 2 : j, ck = P.kcrs ? k / 2 : k;
-    const long l = P.Lc.off + lidx(P.Lc, ci, cj, ck);
-    return F.q[m][l] - F.qold[m][l];
+    return F.cp[m][lidx(P.Lc, ci, cj, ck)];
 }
 // after the k fill (i, j coincident)
 __device__ __forceinline__ double fill_k(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
@@ -464,6 +474,7 @@ void launch_zero_pk_boundary(hipStream_t s, const Level& L, const Fields& F) {
 void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F, const Consts& C, int icrs,
                     int jcrs, int kcrs, const double* rs_tab) {
     ProlongCtx P{Lf, Lc, icrs, jcrs, kcrs};
+    hipLaunchKernelGGL(k_prolong_corr, dim3(nblk(Lc.n, 256)), dim3(256), 0, s, Lc, F);
     hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(Lf.n, 256)), dim3(256), 0, s, P, F);
     // rs_tab: three direction tables of 2 * max(ni,nj,nk) doubles
     const long stride = 2L * (Lf.ni > Lf.nj ? (Lf.ni > Lf.nk ? Lf.ni : Lf.nk) : (Lf.nj > Lf.nk ? Lf.nj : Lf.nk));
