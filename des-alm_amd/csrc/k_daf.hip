@@ -360,6 +360,146 @@ __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, Da
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
 }
 
+// Two-way form of k_daf_tri for short lines: each wave solves 32 (line,
+// system) pairs, lanes 0-31 eliminating rows 0 .. m-1 downwards with the
+// Thomas recurrence above, lanes 32-63 rows len-1 .. m+1 upwards with the
+// mirrored one (e' = a / den, f' = (d - c f') / den).  They meet at row m
+// = len / 2 through a lane shuffle and substitute outwards.  On the coarse
+// levels, where the one-way sweep leaves most CUs idle, this doubles the
+// waves and halves the serial chain.  Rows below m are eliminated in the
+// other order (a rounding-level difference); SGTSV pivot events are counted
+// on the upper halves.  Needs len >= 4.
+template <int D, typename S>
+__global__ void __launch_bounds__(128) k_daf_tri2(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int m = blockIdx.y;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const bool top = lane < 32;
+    const long nlines = D == 0 ? (long)(nj - 2) * (nk - 2) : D == 1 ? (long)(ni - 2) * (nk - 2)
+                                                                    : (long)(ni - 2) * (nj - 2);
+    const long t = (long)blockIdx.x * 64 + wave * 32 + (lane & 31);
+    const bool act = t < nlines;
+    const long sD = D == 0 ? 1 : D == 1 ? (long)ni : (long)ni * nj;
+    const int len = D == 0 ? ni - 2 : D == 1 ? nj - 2 : nk - 2;
+    long base = 0;
+    if (act) {
+        if (D == 0) base = (long)ni * ((1 + t % (nj - 2)) + (long)nj * (1 + t / (nj - 2)));
+        else if (D == 1) base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
+        else base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
+    }
+    const S lam = m == 2 ? S(1) : m == 3 ? S(-1) : S(0);
+    const S h = (S)(D == 0 ? L.dc : D == 1 ? L.de : L.dz);
+    const S h2 = S(0.5) * h, hsq = h * h, epd = (S)C.ep[0][D];
+    const S* __restrict__ sd = B.sd;
+    S* __restrict__ dpm = B.d[m];
+    S* __restrict__ cpm = B.c[m];
+    S* __restrict__ rhm = B.x[m];
+    const S brh1 = (use_b1 && act) ? (S)F.brh[t * 8 + m] : S(0);
+    const S brh2 = (use_b2 && act) ? (S)F.brh[t * 8 + 4 + m] : S(0);
+    const int mid = len / 2;
+    auto node = [&](int pos) { return base + (long)pos * sD; };
+    // upper half carries sd(s), sd(s+1); lower half sd(s+1), sd(s+2)
+    S s_a = 0, s_b = 0;
+    if (act) {
+        if (top) { s_a = sd[node(0)]; s_b = sd[node(1)]; }
+        else { s_a = sd[node(len)]; s_b = sd[node(len + 1)]; }
+    }
+    auto coef = [&](S sdm, S sd0, S sdp, S& a, S& b, S& c) {
+        const S tmp = sd0 * epd * hsq;
+        const S dtc = (sd0 == S(0)) ? S(0) : S(1);
+        const S ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
+        a = -ddw * lam - tmp;
+        c = dde * lam - tmp;
+        b = S(1) + S(2) * tmp;
+    };
+    S cprev = 0, dprev = 0, denprev = 1;
+    unsigned piv = 0;
+    const int nrows = top ? mid : len - 1 - mid;
+    for (int g0 = 0; g0 < nrows; g0 += kPF) {
+        S ls[kPF], ld[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int r = g0 + u;
+            const int srow = top ? r : len - 1 - r;
+            const bool ok = act && r < nrows;
+            ls[u] = ok ? sd[top ? node(srow + 2) : node(srow)] : S(0);
+            ld[u] = ok ? dpm[node(srow + 1)] : S(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int r = g0 + u;
+            if (r < nrows) {
+                const int srow = top ? r : len - 1 - r;
+                S sdm, sd0, sdp;
+                if (top) { sdm = s_a; sd0 = s_b; sdp = ls[u]; }
+                else { sd0 = s_a; sdp = s_b; sdm = ls[u]; }
+                S a, b, c;
+                coef(sdm, sd0, sdp, a, b, c);
+                S d = ld[u];
+                if (srow == 0 && use_b1) d = d - a * brh1;
+                if (srow == len - 1 && use_b2) d = d - c * brh2;
+                S den;
+                if (r == 0) {
+                    den = b;
+                } else {
+                    if (top && fabs(denprev) < fabs(a)) piv++;  // SGTSV would interchange rows here
+                    den = top ? b - a * cprev : b - c * cprev;
+                }
+                const S inv = S(1) / den;
+                if (top) {
+                    cprev = c * inv;
+                    dprev = (r == 0) ? d * inv : (d - a * dprev) * inv;
+                } else {
+                    cprev = a * inv;
+                    dprev = (r == 0) ? d * inv : (d - c * dprev) * inv;
+                }
+                denprev = den;
+                if (act) {
+                    cpm[node(srow + 1)] = cprev;
+                    dpm[node(srow + 1)] = dprev;
+                }
+                if (top) { s_a = sd0; s_b = sdp; }
+                else { s_b = sd0; s_a = sdm; }
+            }
+        }
+    }
+    const S e_lo = __shfl(cprev, (lane & 31) + 32, 64);
+    const S f_lo = __shfl(dprev, (lane & 31) + 32, 64);
+    S xm = 0;
+    if (top && act) {
+        S a, b, c;
+        coef(s_a, s_b, sd[node(mid + 2)], a, b, c);
+        const S d = dpm[node(mid + 1)];
+        if (fabs(denprev) < fabs(a)) piv++;
+        xm = (d - a * dprev - c * f_lo) / (b - a * cprev - c * e_lo);
+        rhm[node(mid + 1)] = xm;
+    }
+    xm = __shfl(xm, lane & 31, 64);
+    S x = xm;
+    const int nb = top ? mid : len - 1 - mid;
+    for (int g0 = 0; g0 < nb; g0 += kPF) {
+        S dd[kPF], cc[kPF];
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int r = g0 + u;
+            const int srow = top ? mid - 1 - r : mid + 1 + r;
+            const bool ok = act && r < nb;
+            dd[u] = ok ? dpm[node(srow + 1)] : S(0);
+            cc[u] = ok ? cpm[node(srow + 1)] : S(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kPF; u++) {
+            const int r = g0 + u;
+            if (r < nb && act) {
+                const int srow = top ? mid - 1 - r : mid + 1 + r;
+                x = dd[u] - cc[u] * x;
+                rhm[node(srow + 1)] = x;
+            }
+        }
+    }
+    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
+}
+
 // The i-direction sweep of k_daf_tri with its rows staged through LDS
 // (line_tile.h): a workgroup owns kTL i-lines, thread = (system m, line l).
 // Chunks of kTR rows of sd and d are loaded one chunk ahead of the
@@ -583,6 +723,22 @@ static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Cons
     const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
                                                                     : (long)(L.ni - 2) * (L.nj - 2);
     const dim3 gt((unsigned)((nl + 127) / 128), 4);
+    const int len = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
+    if (n > 1 && len >= 4) {
+        // coarse levels: the two-way sweep
+        const dim3 g2((unsigned)((nl + 63) / 64), 4);
+        if (D == 0) {
+            hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+            hipLaunchKernelGGL((k_daf_tri2<0, S>), g2, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
+        } else if (D == 1) {
+            hipLaunchKernelGGL((k_daf_pre<1, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+            hipLaunchKernelGGL((k_daf_tri2<1, S>), g2, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
+        } else {
+            hipLaunchKernelGGL((k_daf_pre<2, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+            hipLaunchKernelGGL((k_daf_tri2<2, S>), g2, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
+        }
+        return;
+    }
     if (D == 0) {
         hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
         const unsigned gi = (unsigned)((nl + kTL - 1) / kTL);
