@@ -271,6 +271,49 @@ def dual_time_leg(args):
     return out
 
 
+def large_leg(args):
+    """The large-grid DES workload: 257x193x193 actuator-line box, 4-level
+    V-cycle, random-flow-generator inflow (5% intensity, integral length
+    0.1), FP32-DAF mixed precision, autotuned tiles, one HIP graph per
+    pseudo-iteration.  Reports Mcell-iters/s, the device memory the context
+    holds, and per kernel family on the fine level its algorithmic GB/s
+    (bench.py's words per node over the profiled launch time)."""
+    from desalm import cases, rfg
+    from desalm.lib import Solver
+    from desalm.params import PREC_MIXED
+
+    ni, nj, nk = 257, 193, 193
+    case = cases.alm_box(ni, nj, nk, ng=4, itm=args.iters, iters=(1, 2, 3, 3), precision=PREC_MIXED)
+    sv = Solver(case.params, device=0).load_case(case)
+    tiles = sv.autotune(reps=3)
+    sv.set_state(*case.state())
+    sv.set_rfg(rfg.modes(1000, 20260606), rfg.inflow_tables(nj, nk, intensity=0.05, length=0.1))
+    sv.step()
+    done = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        itc, _ = sv.step()
+        done += itc
+    t1 = time.perf_counter()
+    sv.profile(True)
+    sv.step()
+    st = sv.profile_stats()
+    sv.profile(False)
+    n1 = ni * nj * nk
+    per_family = {}
+    for (nm, lv), (cnt, ms) in st.items():
+        if lv == 1 and nm in WORDS:
+            b = level_words(nm, 1, case.params.itk) * 8 * n1
+            per_family[nm] = {"mean_launch_ms": round(ms / cnt, 4), "algorithmic_GBps": round(b / (ms / cnt * 1e-3) / 1e9, 1)}
+    out = {"workload": f"actuator-line box {ni}x{nj}x{nk}, SA-DES, 4 levels, RFG inflow, FP32-DAF mixed",
+           "value": round(n1 * done / (t1 - t0) / 1e6, 3), "unit": "Mcell-iters/s",
+           "ms_per_step": round((t1 - t0) / args.steps * 1e3, 3), "pseudo_iterations": done,
+           "device_bytes": sv.device_bytes, "fine_nodes": n1, "nodes_all_levels": case.params.nodes_all(),
+           "tiles": {"visc_by_level": tiles[0], "sa_rhs": tiles[1]}, "fine_families": per_family}
+    sv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +322,8 @@ def main():
     ap.add_argument("--iters", type=int, default=10, help="pseudo-iterations per physical step")
     ap.add_argument("--n", type=int, default=129, help="nodes per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 257x193x193 large-grid DES workload (several GB of device memory)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes of the LDS-tiled kernels")
     ap.add_argument("--cpu-iters", type=int, default=6)
@@ -409,6 +454,7 @@ def main():
                                   f"BDF2 dual time, FP64 plain, {args.iters} pseudo-iterations per step",
                       "fine_nodes": n1, "pseudo_iterations": iters_done, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "converge": conv,
+           "large": large_leg(args) if args.large else None,
            "kernels_ms": {f"{nm}@L{n}": [cnt, round(ms, 3)] for (nm, n), (cnt, ms) in sorted(stats.items())},
            "tiles": {"k_planes_per_workgroup_visc_by_level": tiles_used[PREC_FP64][0],
                      "k_planes_per_workgroup_sa_rhs": tiles_used[PREC_FP64][1],

@@ -63,6 +63,7 @@ struct WarmStart {
     double* lpart = nullptr;  // loss partial sums, kWsLossParts per training iteration
     long lcap = 0;
     std::vector<void*> dev;
+    long long bytes = 0;
     int np() const { return (kWsIn + 1) * nh + (nh + 1) * nh + (nh + 1) * kWsOut; }
     int o2() const { return (kWsIn + 1) * nh; }
     int o3() const { return (kWsIn + 1) * nh + (nh + 1) * nh; }
@@ -99,6 +100,13 @@ struct desalm_ctx {
     // of the SA right-hand side (0: default; desalm_autotune / desalm_set_tiles)
     int kc_visc[kMaxLevels + 1] = {};
     int kc_sa = 0;
+    long long dev_bytes = 0;  // fields and scratch (dalloc)
+    long long alm_bytes = 0;  // rotor tables
+    // random-flow-generator inflow (desalm_set_rfg); applied by desalm_step
+    Rfg rfg{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<void*> rfg_dev;
+    double rfg_time0 = 0.0;
+    long long steps = 0;  // desalm_step calls since desalm_set_rfg
 };
 
 namespace {
@@ -122,6 +130,7 @@ double* dalloc(desalm_ctx* c, long n) {
     void* p = nullptr;
     if (hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
     c->allocs.push_back(p);
+    c->dev_bytes += (long long)(n > 0 ? n : 1) * (long long)sizeof(double);
     if (hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
     return (double*)p;
 }
@@ -493,6 +502,7 @@ T* ws_alloc(desalm_ctx* c, long n) {
         return nullptr;
     }
     c->ws.dev.push_back(p);
+    c->ws.bytes += (long long)(n > 0 ? n : 1) * (long long)sizeof(T);
     if (hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(T)) != hipSuccess) {
         fail(c, "warm start: hipMemset failed");
         return nullptr;
@@ -892,6 +902,7 @@ const char* desalm_last_error(const desalm_ctx* c) { return c ? c->err.c_str() :
 static void free_alm(desalm_ctx* c) {
     for (void* p : c->alm.dev) (void)hipFree(p);
     c->alm = Alm();
+    c->alm_bytes = 0;
 }
 
 int desalm_set_turbines(desalm_ctx* c, int32_t n_tur, const desalm_turbine* tur, int32_t n_polar,
@@ -945,6 +956,7 @@ int desalm_set_turbines(desalm_ctx* c, int32_t n_tur, const desalm_turbine* tur,
         void* d = nullptr;
         if (hipMalloc(&d, bytes ? bytes : 8) != hipSuccess) { ok = false; return nullptr; }
         c->alm.dev.push_back(d);
+        c->alm_bytes += (long long)(bytes ? bytes : 8);
         if (bytes && src && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
         return d;
     };
@@ -1237,6 +1249,11 @@ int desalm_bintp(desalm_ctx* c, int which) {
     return 0;
 }
 
+int64_t desalm_device_bytes(const desalm_ctx* c) {
+    if (!c) return -1;
+    return c->dev_bytes + c->alm_bytes + c->ws.bytes;
+}
+
 int desalm_set_tiles(desalm_ctx* c, const int* kc_visc, int kc_sa) {
     if (!c) return fail(c, "null argument");
     for (int n = 1; n <= c->ng; n++) {
@@ -1314,8 +1331,67 @@ int desalm_set_convergence(desalm_ctx* c, int itm, int it_min, double er_min, do
     return 0;
 }
 
+static void free_rfg(desalm_ctx* c) {
+    for (void* p : c->rfg_dev) (void)hipFree(p);
+    c->rfg_dev.clear();
+    c->rfg = Rfg{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+}
+
+int desalm_set_rfg(desalm_ctx* c, int32_t n_modes, const double* k, const double* omega, const double* p,
+                   const double* q, const double* um, const double* m, const double* cc, const double* ts,
+                   const double* ls, double time0) {
+    if (!c) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_rfg(c);
+    if (n_modes == 0) return 0;
+    if (n_modes < 0 || !k || !omega || !p || !q || !um || !m || !cc || !ts || !ls)
+        return fail(c, "desalm_set_rfg: bad arguments");
+    const long nplane = (long)c->p.jmg * c->p.kmg;
+    for (long e = 0; e < nplane; e++) {
+        const long j = e % c->p.jmg, kk = e / c->p.jmg;
+        if (j == 0 || j == c->p.jmg - 1 || kk == 0 || kk == c->p.kmg - 1) continue;
+        if (!(ts[e] > 0.0) || !(ls[e] > 0.0)) return fail(c, "desalm_set_rfg: time and length scales must be > 0");
+        for (int d = 0; d < 3; d++)
+            if (!(cc[3 * e + d] > 0.0)) return fail(c, "desalm_set_rfg: Reynolds-stress eigenvalues must be > 0");
+    }
+    bool ok = true;
+    auto up = [&](const double* src, long n) -> const double* {
+        void* d = nullptr;
+        if (hipMalloc(&d, sizeof(double) * (size_t)n) != hipSuccess) { ok = false; return nullptr; }
+        c->rfg_dev.push_back(d);
+        if (hipMemcpy(d, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) ok = false;
+        return (const double*)d;
+    };
+    c->rfg.n = n_modes;
+    c->rfg.k = up(k, 3L * n_modes);
+    c->rfg.omega = up(omega, n_modes);
+    c->rfg.p = up(p, 3L * n_modes);
+    c->rfg.q = up(q, 3L * n_modes);
+    c->rfg.um = up(um, 3 * nplane);
+    c->rfg.m = up(m, 9 * nplane);
+    c->rfg.c = up(cc, 3 * nplane);
+    c->rfg.ts = up(ts, nplane);
+    c->rfg.ls = up(ls, nplane);
+    if (!ok) {
+        free_rfg(c);
+        return fail(c, "desalm_set_rfg: out of device memory");
+    }
+    c->rfg_time0 = time0;
+    c->steps = 0;
+    return 0;
+}
+
 int desalm_step(desalm_ctx* c, int* itc_out, double* resid_out) {
     if (!c) return fail(c, "null argument");
+    // inflow forcing at time = time0 + (steps so far) delti, before the
+    // V-cycles (main.F90:58, rfg_inlet.F90:107-173)
+    if (c->rfg.n > 0) {
+        HIPCHK(c, hipSetDevice(c->device));
+        launch_rfg_inflow(c->stream, c->lv[1], c->F, c->rfg, c->rfg_time0 + (double)c->steps * c->p.delti);
+        HIPCHK(c, hipGetLastError());
+    }
+    c->steps++;
     // blade angles (main.F90:42-49); dtheta = TSR delti length_scale / tur_rad (init.F90:1447)
     const double two_pi = 2.0 * 3.141592653589793;
     for (AlmTurbine& T : c->alm.t) {
@@ -1618,6 +1694,7 @@ int desalm_finalize(desalm_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_overset(c);
     free_alm(c);
+    free_rfg(c);
     ws_free(c->ws);
     prof_flush(c);
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);

@@ -155,6 +155,23 @@ struct WsQ {
     double* q[5];  // q^(n-2), fine level
 };
 
+// Random-flow-generator inflow (k_rfg.hip): N random Fourier modes and, per
+// (j, k) of the inflow plane (index j + nj k), the mean velocity, the
+// eigenvectors (column-major 3 x 3) and the square roots of the eigenvalues
+// of the Reynolds-stress tensor, and the time and length scales
+struct Rfg {
+    int n;
+    const double* k;      // [3 n]
+    const double* omega;  // [n]
+    const double* p;      // [3 n]
+    const double* q;      // [3 n]
+    const double* um;     // [3 nj nk]
+    const double* m;      // [9 nj nk]
+    const double* c;      // [3 nj nk]
+    const double* ts;     // [nj nk]
+    const double* ls;     // [nj nk]
+};
+
 struct Ctx;  // host context, desalm_api.hip
 
 // ---- launchers (each enqueues on `s`) ----
@@ -202,6 +219,7 @@ void launch_bintp_scatter(hipStream_t s, const Level& L, const Fields& F, const 
 // actuator-line model, k_alm.hip: forcing added to rh of level L, and
 // the hub zeroing of bcond_fm on the fine level
 void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A, double ren);
+void launch_rfg_inflow(hipStream_t s, const Level& L, const Fields& F, const Rfg& R, double t);
 void launch_alm_hub(hipStream_t s, const Level& L, const Fields& F, const Alm& A);
 // batched GEPP (LAPACK SGTSV semantics), k_daf.hip
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,

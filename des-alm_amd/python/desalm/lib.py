@@ -90,6 +90,10 @@ def lib():
         L.desalm_set_tiles.argtypes = [vp, ip, ctypes.c_int]
         L.desalm_get_tiles.argtypes = [vp, ip, ip]
         L.desalm_autotune.argtypes = [vp, ctypes.c_int]
+        L.desalm_set_rfg.argtypes = [vp, ctypes.c_int32] + [dp] * 9 + [ctypes.c_double]
+        L.desalm_set_rfg.restype = ctypes.c_int
+        L.desalm_device_bytes.argtypes = [vp]
+        L.desalm_device_bytes.restype = ctypes.c_int64
         L.desalm_turbine_angle.argtypes = [vp, ctypes.c_int, dp, dp]
         for f in ("desalm_set_convergence", "desalm_step", "desalm_ws_configure", "desalm_ws_mode",
                   "desalm_ws_record", "desalm_ws_train", "desalm_ws_param_count", "desalm_ws_params",
@@ -222,6 +226,23 @@ class Solver:
         hist = np.zeros((max(self.params.itm, 1), 5))
         self._chk(lib().desalm_mg_driver(self.h, ctypes.byref(itc), _p(hist)), "desalm_mg_driver")
         return itc.value, hist[: itc.value]
+
+    def set_rfg(self, modes, tables, time0=0.0):
+        """Random-flow-generator inflow: modes = (k, omega, p, q) and tables =
+        (um, m, c, ts, ls) as built by desalm.rfg; None removes it."""
+        if modes is None:
+            self._chk(lib().desalm_set_rfg(self.h, 0, *([None] * 9), 0.0), "desalm_set_rfg")
+            return
+        k, omega, p, q = (_c(a).ravel() for a in modes)
+        um, m, c, ts, ls = (_c(a).ravel() for a in tables)
+        self._keep_rfg = (k, omega, p, q, um, m, c, ts, ls)
+        self._chk(lib().desalm_set_rfg(self.h, len(omega), *[_p(a) for a in self._keep_rfg], time0),
+                  "desalm_set_rfg")
+
+    @property
+    def device_bytes(self):
+        """device memory held by the context (fields, scratch, rotors, warm start)"""
+        return lib().desalm_device_bytes(self.h)
 
     def set_tiles(self, kc_visc=None, kc_sa=0):
         """k planes per workgroup of the LDS-tiled kernels (0: default)."""

@@ -290,6 +290,10 @@ int desalm_profile(desalm_ctx* ctx, int enable);
 int desalm_profile_get(const desalm_ctx* ctx, int kernel, int level, int64_t* count, double* total_ms);
 const char* desalm_profile_name(int kernel);
 
+/* Device memory the context holds: fields, scratch, rotor tables and the
+ * warm start (bytes; overset tables of a group excluded). */
+int64_t desalm_device_bytes(const desalm_ctx* ctx);
+
 /* Launch shape of the LDS-tiled stencil kernels: k planes per workgroup of
  * the viscous kernel on each level (kc_visc[0 .. ng-1]) and of the SA
  * right-hand side (fine level); 0 picks the default (about 2048 workgroups).
@@ -312,12 +316,29 @@ int desalm_autotune(desalm_ctx* ctx, int reps);
  * call must hold 5 * itm doubles. */
 int desalm_set_convergence(desalm_ctx* ctx, int itm, int it_min, double er_min, double eo_min);
 
+/* Random-flow-generator inflow (rfg_inlet.F90:107-173 with rfg.F90:329-374,
+ * the method of Smirnov, Shi & Celik, J. Fluids Eng. 123 (2001) 359-371):
+ * before each desalm_step's V-cycles, at time = time0 + (steps since this
+ * call) * delti, every interior node (j, k) of the plane i = 1 gets
+ *   (u, v, w) = um + M (c o v1),
+ *   v1 = sqrt(2 / N) sum_n [p_n cos(theta_n) + q_n sin(theta_n)],
+ *   theta_n = sum_d k_nd (ls / ts) / c_d x_d / ls + omega_n time / ts,
+ * with M the eigenvectors (columns) and c the square roots of the eigenvalues
+ * of the node's Reynolds-stress tensor.  The caller supplies the N modes
+ * (k_n, omega_n normal; p_n = zeta_n x k_n, q_n = xi_n x k_n; the reference
+ * draws them with its own generator) and per (j, k), index j + jmg k: um
+ * (3), M (9, column-major), c (3), ts, ls.  n_modes = 0 removes it. */
+int desalm_set_rfg(desalm_ctx* ctx, int32_t n_modes, const double* k, const double* omega, const double* p,
+                   const double* q, const double* um, const double* m, const double* c, const double* ts,
+                   const double* ls, double time0);
+
 /* ---- Physical time step (main.F90:38-62) ----
  * One step of the physical-time loop around mg_driver: every rotor's blade
  * angle advances (angle > 2 pi wraps by -2 pi first, then angle += dtheta with
  * dtheta = TSR delti length_scale / tur_rad, main.F90:42-49 and
- * init.F90:1447), then desalm_mg_driver runs, from the warm start selected
- * by desalm_ws_mode. */
+ * init.F90:1447), the random-flow-generator inflow is applied when set
+ * (main.F90:58), then desalm_mg_driver runs, from the warm start selected by
+ * desalm_ws_mode. */
 int desalm_step(desalm_ctx* ctx, int* itc_out, double* log_resid_out);
 
 /* ---- Learned warm start ----

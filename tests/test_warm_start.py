@@ -165,3 +165,27 @@ def test_trained_network_beats_extrapolation(gpu):
     print("learned / extrapolation start error per unseen step:", np.round(ratio, 3))
     assert ratio.mean() < 0.95
     s.close()
+
+
+def test_rfg_inflow_matches_oracle(gpu):
+    """desalm_step applies the random-flow-generator inflow on the plane
+    i = 1 at time0 + steps * delti before its V-cycles (main.F90:58)."""
+    import rfg_oracle as ro
+    from desalm import rfg
+    c = cases.alm_box(25, 17, 17, ng=2, itm=1, rotor=False)
+    s = Solver(c.params).load_case(c)
+    dims = c.params.level_dims()[0]
+    md = rfg.modes(64, 9)
+    tb = rfg.inflow_tables(dims[1], dims[2], intensity=0.05, length=0.1)
+    s.set_rfg(md, tb, time0=0.35)
+    s.set_convergence(0, 0, -8.0, -30.0)  # the forcing only: no pseudo-iteration
+    for step in range(2):
+        q0 = s.get_state()[0]
+        s.step()
+        q1 = s.get_state()[0]
+        want = ro.inflow(q0, (c.x, c.y, c.z), dims, 0.35 + step * c.params.delti, md, tb)
+        g = np.arange(dims[1] * dims[2]) * dims[0]
+        assert np.allclose(q1[g], want[g], rtol=1e-12, atol=1e-13)
+        other = np.setdiff1d(np.arange(len(q0)), g)
+        assert np.array_equal(q1[other], q0[other])
+    s.close()
