@@ -268,6 +268,10 @@ int copy_out_level(desalm_ctx* c, double* const* soa, int ncomp, double* host, c
 
 }  // namespace
 
+namespace desalm {
+int host_fail(const std::string& msg) { return fail(nullptr, msg); }
+}  // namespace desalm
+
 extern "C" {
 
 int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/desalm.h"
@@ -16,6 +17,9 @@
 namespace desalm {
 
 constexpr int kMaxLevels = DESALM_MAX_LEVELS;
+
+// Records msg as the error desalm_last_error(NULL) returns; returns -1.
+int host_fail(const std::string& msg);
 
 // Workgroups are handed to the 8 XCDs round robin (workgroup b runs on XCD
 // b % 8), and every XCD has its own L2.  xcd_block(b, nb) renumbers the nb

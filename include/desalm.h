@@ -405,6 +405,76 @@ int desalm_ws_features(desalm_ctx* ctx, int64_t m, float* x, float* t);
 int desalm_mlp_forward(desalm_ctx* ctx, const float* x, int64_t m, float* y);
 int desalm_mlp_loss_grad(desalm_ctx* ctx, const float* x, const float* t, int64_t m, float* grad, double* loss);
 
+/*
+ * ---- The reference's files (host only: no device, no context) ----
+ *
+ * grid and solu are Fortran unformatted sequential files (every record framed
+ * by 4-byte byte counts; records over 2^31 - 9 bytes split into subrecords as
+ * gfortran writes them).  real_bytes: 4 (the reference's REAL*4,
+ * src/precision.F90:11), 8 (an FP64 build), or 0 on reading (taken from the
+ * first record).  dims[3 * nz + 0..2] = img, jmg, kmg of zone nz; the arrays of
+ * the zones follow each other, nodes i fastest, then j, then k.  Errors are
+ * reported by desalm_last_error(NULL).
+ */
+
+/* The grid file `grid` (src/init.F90:2226-2241): x, y, z of every zone, and
+ * the wall distance wd when turbulence != 0. */
+int desalm_read_grid_file(const char* path, int32_t nzone, const int32_t* dims, int32_t turbulence,
+                          int32_t real_bytes, double* x, double* y, double* z, double* wd);
+
+/* The solution file `solu` (src/init.F90:2384-2460) into q(1:5, n) and
+ * qn(1:5, n) (the layout of desalm_set_state) and xnut(n) when turbulence.
+ * Without turbulence the reference reads one record per field; the
+ * one-record-per-component files of the restart output are read as well. */
+int desalm_read_solu_file(const char* path, int32_t nzone, const int32_t* dims, int32_t turbulence,
+                          int32_t real_bytes, double* q, double* qn, double* xnut);
+
+/* The restart output of one rank (src/main.F90:343-372, solufiles/solu.NNN):
+ * five records of q, xnut when turbulence, five of qn, each of n1 reals,
+ * rounded to REAL*4 when real_bytes = 4.  Read back by
+ * desalm_read_solu_file with nzone = 1. */
+int desalm_write_solu_file(const char* path, int64_t n1, int32_t turbulence, int32_t real_bytes,
+                           const double* q, const double* qn, const double* xnut);
+
+/* `filestat` (src/init.F90:266-277): "time0 nt1" and one blade angle per
+ * rotor, list-directed; an angle >= 2 pi is reduced by 2 pi as the reference
+ * does on reading.  The writer follows src/main.F90:379-386. */
+int desalm_read_filestat(const char* path, int32_t n_tur, double* time0, int32_t* nt1, double* angle);
+int desalm_write_filestat(const char* path, double time, int32_t ntime, int32_t n_tur, const double* angle);
+
+/* The input deck ind3dmg.dat (src/init.F90:176-228), list-directed.  ng, the
+ * number of multigrid levels, is a compile-time constant of the reference
+ * (src/global.F90:17) and an argument here.  params receives every run
+ * parameter the deck holds (itm = itm(1)); unsteady and des take the
+ * reference's defaults (src/global_param.F90:82, :121) and precision FP64.
+ * The single zone of the reference (nzone = 1, src/global.F90:20) is read. */
+#define DESALM_DECK_MAX_TURBINES 32
+typedef struct desalm_deck_turbine {
+    int32_t itur1, itur2, jtur1, jtur2, ktur1, ktur2;  /* 1-based node box of the rotor */
+    double tur_rad, hub_rad, tsr;
+    int32_t nblades, nelems;
+} desalm_deck_turbine;
+
+typedef struct desalm_deck {
+    desalm_params params;
+    int32_t ns, icycle, nt2, icn, icnw, irk, checkpoint, nzfix;
+    int32_t itm[DESALM_MAX_LEVELS];      /* itm(1:ng) */
+    double alfa[4], eps[4];
+    double cdes, ckin, cein;
+    int32_t monitor_num_points;
+    int32_t monitor_point_ijk[10][4];    /* point j: i, j, k, zone */
+    double length_scale, vel_infty;
+    int32_t turbines, node_indexes_load; /* logicals: 0 or 1 */
+    int32_t n_tur;
+    desalm_deck_turbine turbine[DESALM_DECK_MAX_TURBINES];
+    int32_t nzblanking;                  /* 0 or 1 (nba_max = 1, src/global.F90:21) */
+    int32_t blktype[6];
+    int32_t blanking[3][2];              /* blanking(d, 1:2): index range in direction d */
+    int32_t reserved[8];
+} desalm_deck;
+
+int desalm_read_deck(const char* path, int32_t ng, desalm_deck* deck);
+
 const char* desalm_last_error(const desalm_ctx* ctx);
 int desalm_finalize(desalm_ctx* ctx);
 

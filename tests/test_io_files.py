@@ -1,0 +1,452 @@
+"""The reference's files (grid, solu, restart output, filestat, ind3dmg.dat)
+through libdesalm's host readers and writers (des-alm_amd/csrc/io_files.hip).
+
+Pinned three ways: the shipped sample deck (tests/golden/ind3dmg_sample.dat,
+the data lines of the reference's ind3dmg.dat) against values read off it
+by hand; files written by a Fortran compiler's own runtime (amdflang, from
+a small program in this file that writes and reads records in the order the
+reference's readers use: src/init.F90:2226-2241, :2392-2455,
+src/main.F90:352-372) against our readers, and our restart file read back by
+that runtime; list-directed edge cases (commas, repeat counts, null values,
+values over several lines, D exponents, logical forms) against the same
+runtime's list-directed READ.  No device is used."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from desalm import files
+from desalm.lib import DesalmError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FC = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+needs_fc = pytest.mark.skipif(FC is None, reason="no Fortran compiler (amdflang) in this image")
+
+DIMS = (5, 4, 3)
+N = 60
+
+# Writes grid and solu files with the record structure of the reference's
+# readers, REAL*4 (kind 4) and REAL*8 (kind 8), and reads a restart file back.
+FORTRAN_FILES = r"""
+program files_check
+  implicit none
+  integer, parameter :: ni = 5, nj = 4, nk = 3, n = ni * nj * nk
+  real(4) :: x4(ni, nj, nk), y4(ni, nj, nk), z4(ni, nj, nk), d4(ni, nj, nk)
+  real(4) :: q4(5, n), qn4(5, n), nu4(n)
+  real(8) :: q8(5, n), qn8(5, n), nu8(n), x8(ni, nj, nk)
+  integer :: i, j, k, l, m
+  character(len=16) :: mode
+  call get_command_argument(1, mode)
+  if (trim(mode) == 'write') then
+    do k = 1, nk
+      do j = 1, nj
+        do i = 1, ni
+          x4(i, j, k) = 0.1 * i + 1.0 * j + 10.0 * k
+          y4(i, j, k) = -0.5 * i + 2.0 * j * j
+          z4(i, j, k) = 3.0 * k - 0.25 * i * j
+          d4(i, j, k) = 1.0e-3 * (i + j + k)
+        end do
+      end do
+    end do
+    do l = 1, n
+      do m = 1, 5
+        q4(m, l) = 1.0 / (l + 0.5 * m)
+        qn4(m, l) = -2.0 * l + m
+        q8(m, l) = 1.0d0 / (l + 0.5d0 * m) + 1.0d-12
+        qn8(m, l) = -2.0d0 * l + m + 1.0d-13
+      end do
+      nu4(l) = 1.0e-5 * l
+      nu8(l) = 1.0d-5 * l + 1.0d-15
+    end do
+    x8 = x4 + 1.0d-9
+    open (41, file = 'grid4', form = 'unformatted')
+    write (41) (((x4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (41) (((y4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (41) (((z4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (41) (((d4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    close (41)
+    open (41, file = 'grid8_noturb', form = 'unformatted')
+    write (41) (((x8(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (41) (((x8(i, j, k) * 2, i = 1, ni), j = 1, nj), k = 1, nk)
+    write (41) (((x8(i, j, k) * 3, i = 1, ni), j = 1, nj), k = 1, nk)
+    close (41)
+    open (42, file = 'solu4', form = 'unformatted')
+    do m = 1, 5
+      write (42) (q4(m, l), l = 1, n)
+    end do
+    write (42) (nu4(l), l = 1, n)
+    do m = 1, 5
+      write (42) (qn4(m, l), l = 1, n)
+    end do
+    close (42)
+    open (42, file = 'solu8_noturb', form = 'unformatted')
+    write (42) ((q8(m, l), l = 1, n), m = 1, 5)
+    write (42) ((qn8(m, l), l = 1, n), m = 1, 5)
+    close (42)
+    open (43, file = 'fortran_values.txt')
+    write (43, '(es26.17)') (((x4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (43, '(es26.17)') (((y4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (43, '(es26.17)') (((z4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (43, '(es26.17)') (((d4(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (43, '(es26.17)') ((q4(m, l), m = 1, 5), l = 1, n)
+    write (43, '(es26.17)') ((qn4(m, l), m = 1, 5), l = 1, n)
+    write (43, '(es26.17)') (nu4(l), l = 1, n)
+    write (43, '(es26.17)') (((x8(i, j, k), i = 1, ni), j = 1, nj), k = 1, nk)
+    write (43, '(es26.17)') ((q8(m, l), m = 1, 5), l = 1, n)
+    write (43, '(es26.17)') ((qn8(m, l), m = 1, 5), l = 1, n)
+    close (43)
+  else
+    ! our restart file: five records of q, xnut, five of qn (REAL*4)
+    open (42, file = 'restart4', form = 'unformatted')
+    do m = 1, 5
+      read (42) (q4(m, l), l = 1, n)
+    end do
+    read (42) (nu4(l), l = 1, n)
+    do m = 1, 5
+      read (42) (qn4(m, l), l = 1, n)
+    end do
+    close (42)
+    open (43, file = 'restart_values.txt')
+    write (43, '(es26.17)') ((q4(m, l), m = 1, 5), l = 1, n)
+    write (43, '(es26.17)') ((qn4(m, l), m = 1, 5), l = 1, n)
+    write (43, '(es26.17)') (nu4(l), l = 1, n)
+    close (43)
+  end if
+end program files_check
+"""
+
+# An input deck with list-directed edge cases, and a Fortran program reading
+# it statement by statement in the deck's order (ng = 2, one turbine).
+EDGE_DECK = """\
+1,1   ! the rest of a line is skipped
+33, 2*17
+1 0
+1
+  1000 , 2.5D-3 , 4 , -3.0E0 , -9.5   trailing text
+2 3
+40
+   1
+1.5e5 1.0 10000 5
+1.0 / vnn1 keeps its value
+3 0.25, .5
+  1.0
+4*0.065
+2 0.65 1.0d-8 1.0E-06
+, 0.05
+1.0 2*2.5
+3*0.2
+10
+0.1
+2
+50 50 50 1
+9 8 7 1
+4,8,1,1,1,2
+1 1 2 2 3 3
+1 1 1 1
+0.450
+4.5e-1
+.T.
+F
+1
+59 63 181 181 65 65
+0.1350
+0.015 4.7
+3 30
+0
+"""
+
+FORTRAN_DECK = r"""
+program deck_check
+  implicit none
+  integer :: ns, icycle, img, jmg, kmg, icrs, jcrs, kcrs, nt2, it_min, iter(2), itm(2), icn, icnw, irk
+  integer :: itk, checkpoint, nmon, mon(10, 4), btype(6), bdir(6), ifix, jfix, kfix, nzfix, n_tur
+  integer :: it1, it2, jt1, jt2, kt1, kt2, nbl, nel, nzb, i, j, k
+  real(8) :: delti, er_min, eo_min, ren, beta, cfl1, vnn1, alfa(4), eps(4), cdes, ckin, cein
+  real(8) :: cfl2, vnn2, ep(2, 3), pdiss, ls, vinf, rad, hub, tsr
+  logical :: turbines, nil
+  alfa = 0
+  vnn1 = 0
+  cfl2 = 0
+  open (1, file = 'edge.dat')
+  read (1, *) ns, icycle
+  read (1, *) img, jmg, kmg
+  read (1, *) icrs, jcrs, kcrs
+  read (1, *) nt2, delti, it_min, er_min, eo_min
+  read (1, *) (iter(i), i = 1, 2)
+  read (1, *) (itm(i), i = 1, 2)
+  read (1, *) ren, beta, icn, icnw
+  read (1, *) cfl1, vnn1
+  read (1, *) irk, (alfa(i), i = 1, irk)
+  read (1, *) (eps(i), i = 1, 4)
+  read (1, *) itk, cdes, ckin, cein
+  read (1, *) cfl2, vnn2
+  read (1, *) ((ep(j, k), k = 1, 3), j = 1, 2)
+  read (1, *) checkpoint
+  read (1, *) pdiss
+  read (1, *) nmon
+  read (1, *) ((mon(j, k), k = 1, 4), j = 1, nmon)
+  read (1, *) (btype(i), i = 1, 6)
+  read (1, *) (bdir(i), i = 1, 6)
+  read (1, *) ifix, jfix, kfix, nzfix
+  read (1, *) ls
+  read (1, *) vinf
+  read (1, *) turbines
+  read (1, *) nil
+  read (1, *) n_tur
+  read (1, *) it1, it2, jt1, jt2, kt1, kt2, rad
+  read (1, *) hub, tsr
+  read (1, *) nbl, nel
+  read (1, *) nzb
+  close (1)
+  open (2, file = 'deck_values.txt')
+  write (2, *) ns, icycle, img, jmg, kmg, icrs, jcrs, kcrs, nt2, it_min, iter, itm, icn, icnw, irk
+  write (2, *) itk, checkpoint, nmon, ((mon(j, k), k = 1, 4), j = 1, nmon), btype, bdir, ifix, jfix, kfix, nzfix
+  write (2, *) n_tur, it1, it2, jt1, jt2, kt1, kt2, nbl, nel, nzb
+  write (2, '(es26.17)') delti, er_min, eo_min, ren, beta, cfl1, vnn1, alfa, eps, cdes, ckin, cein
+  write (2, '(es26.17)') cfl2, vnn2, ((ep(j, k), k = 1, 3), j = 1, 2), pdiss, ls, vinf, rad, hub, tsr
+  write (2, *) turbines, nil
+  close (2)
+end program deck_check
+"""
+
+
+def run_fortran(tmp, name, src, *args):
+    with open(os.path.join(tmp, name + ".f90"), "w") as f:
+        f.write(src)
+    subprocess.run([FC, "-O0", name + ".f90", "-o", name], cwd=tmp, check=True, capture_output=True)
+    subprocess.run(["./" + name, *args], cwd=tmp, check=True, capture_output=True)
+
+
+@pytest.fixture(scope="module")
+def fortran_files(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("ffiles"))
+    run_fortran(tmp, "files_check", FORTRAN_FILES, "write")
+    vals = np.loadtxt(os.path.join(tmp, "fortran_values.txt"))
+    parts = np.split(vals, np.cumsum([N, N, N, N, 5 * N, 5 * N, N, N, 5 * N]))
+    ref = dict(zip(["x", "y", "z", "d", "q4", "qn4", "nu4", "x8", "q8", "qn8"], parts))
+    for k in ("q4", "qn4", "q8", "qn8"):
+        ref[k] = ref[k].reshape(N, 5)
+    return tmp, ref
+
+
+def test_sample_deck():
+    """The reference's shipped deck, values read off its text."""
+    d = files.read_deck(os.path.join(HERE, "golden", "ind3dmg_sample.dat"), 3)
+    p = d.params
+    assert (d.ns, d.icycle) == (1, 1)
+    assert (p.img, p.jmg, p.kmg, p.icrs, p.jcrs, p.kcrs) == (282, 361, 101, 0, 0, 0)
+    assert (d.nt2, p.delti, p.it_min, p.er_min, p.eo_min) == (200000, 0.001, 7, -2.0, -7.5)
+    assert list(p.iter)[:3] == [1, 2, 3] and list(d.itm)[:3] == [40, 1, 1] and p.itm == 40
+    assert (p.ren, p.beta, d.icn, d.icnw) == (202500.0, 1.0, 10000, 5)
+    assert (p.cfl1, p.vnn1) == (1.0, 0.1)
+    assert d.irk == 4 and list(d.alfa) == [0.25, 0.33333333333333, 0.5, 1.0]
+    assert list(d.eps) == [0.065] * 4
+    assert (p.itk, d.cdes, d.ckin, d.cein) == (2, 0.65, 1.0e-8, 1.0e-6)
+    assert (p.cfl2, p.vnn2) == (0.5, 0.05)
+    assert [list(r) for r in p.ep] == [[1.0, 2.5, 2.5], [0.1, 0.2, 0.2]]
+    assert (d.checkpoint, p.pdiss_coef, d.monitor_num_points) == (10, 0.1, 1)
+    assert list(d.monitor_point_ijk[0]) == [50, 50, 50, 1]
+    assert list(p.btype) == [4, 8, 1, 1, 1, 2] and list(p.bdir) == [1, 1, 2, 2, 3, 3]
+    assert (p.ifix, p.jfix, p.kfix, d.nzfix) == (1, 1, 1, 1)
+    assert (d.length_scale, d.vel_infty, d.turbines, d.node_indexes_load, d.n_tur) == (0.45, 0.45, 1, 1, 1)
+    t = d.turbine[0]
+    assert (t.itur1, t.itur2, t.jtur1, t.jtur2, t.ktur1, t.ktur2) == (59, 63, 181, 181, 65, 65)
+    assert (t.tur_rad, t.hub_rad, t.tsr, t.nblades, t.nelems) == (0.135, 0.015, 4.7, 3, 30)
+    assert d.nzblanking == 0
+    assert (p.ng, p.unsteady, p.des) == (3, 1, 1)
+
+
+@needs_fc
+def test_deck_edge_cases_match_fortran_list_directed(tmp_path):
+    tmp = str(tmp_path)
+    with open(os.path.join(tmp, "edge.dat"), "w") as f:
+        f.write(EDGE_DECK)
+    run_fortran(tmp, "deck_check", FORTRAN_DECK)
+    with open(os.path.join(tmp, "deck_values.txt")) as f:
+        lines = f.read().split("\n")
+    ints = [int(v) for v in " ".join(lines[0:3]).split()]
+    reals = [float(v) for v in lines[3:3 + 32]]
+    logic = lines[3 + 32].split()
+    d = files.read_deck(os.path.join(tmp, "edge.dat"), 2)
+    p = d.params
+    t = d.turbine[0]
+    mine_i = [d.ns, d.icycle, p.img, p.jmg, p.kmg, p.icrs, p.jcrs, p.kcrs, d.nt2, p.it_min, *list(p.iter)[:2],
+              *list(d.itm)[:2], d.icn, d.icnw, d.irk, p.itk, d.checkpoint, d.monitor_num_points,
+              *[v for j in range(d.monitor_num_points) for v in d.monitor_point_ijk[j]], *p.btype, *p.bdir,
+              p.ifix, p.jfix, p.kfix, d.nzfix, d.n_tur, t.itur1, t.itur2, t.jtur1, t.jtur2, t.ktur1, t.ktur2,
+              t.nblades, t.nelems, d.nzblanking]
+    assert mine_i == ints
+    mine_r = [p.delti, p.er_min, p.eo_min, p.ren, p.beta, p.cfl1, p.vnn1, *d.alfa, *d.eps, d.cdes, d.ckin,
+              d.cein, p.cfl2, p.vnn2, *[v for r in p.ep for v in r], p.pdiss_coef, d.length_scale,
+              d.vel_infty, t.tur_rad, t.hub_rad, t.tsr]
+    assert np.array_equal(np.array(mine_r), np.array(reals))
+    assert [bool(d.turbines), bool(d.node_indexes_load)] == [v == "T" for v in logic]
+    # the deck's own quirks: "2*17" gives jmg = kmg = 17, irk = 3 reads alfa over two lines,
+    # a slash leaves vnn1 and a leading comma leaves cfl2 as they were (zero)
+    assert (p.jmg, p.kmg, d.irk, list(d.alfa)) == (17, 17, 3, [0.25, 0.5, 1.0, 0.0])
+    assert (p.icrs, p.jcrs, p.kcrs, p.vnn1, p.cfl2, p.vnn2) == (1, 0, 1, 0.0, 0.0, 0.05)
+
+
+def test_deck_errors(tmp_path):
+    bad = tmp_path / "bad.dat"
+    bad.write_text("1 1\n33 17 x\n")
+    with pytest.raises(DesalmError, match=r"read 2 \(kmg\): cannot convert 'x'"):
+        files.read_deck(bad, 2)
+    short = tmp_path / "short.dat"
+    short.write_text("1 1\n33 17 17\n1 1 1\n")
+    with pytest.raises(DesalmError, match="end of file"):
+        files.read_deck(short, 2)
+    with pytest.raises(DesalmError, match="ng must be"):
+        files.read_deck(os.path.join(HERE, "golden", "ind3dmg_sample.dat"), 0)
+
+
+@needs_fc
+def test_grid_file_from_fortran(fortran_files):
+    tmp, ref = fortran_files
+    for rb in (0, 4):
+        x, y, z, wd = files.read_grid(os.path.join(tmp, "grid4"), DIMS, True, rb)
+        for a, k in ((x, "x"), (y, "y"), (z, "z"), (wd, "d")):
+            assert np.array_equal(a, ref[k]), k
+    x, y, z, wd = files.read_grid(os.path.join(tmp, "grid8_noturb"), DIMS, False)
+    assert wd is None
+    assert np.array_equal(x, ref["x8"]) and np.array_equal(y, ref["x8"] * 2) and np.array_equal(z, ref["x8"] * 3)
+    with pytest.raises(DesalmError, match="record holds"):
+        files.read_grid(os.path.join(tmp, "grid4"), DIMS, True, 8)
+    with pytest.raises(DesalmError, match="record holds"):
+        files.read_grid(os.path.join(tmp, "grid4"), (5, 4, 4), True)
+    with pytest.raises(DesalmError, match="wall distance: unexpected end of file"):
+        files.read_grid(os.path.join(tmp, "grid8_noturb"), DIMS, True)
+
+
+@needs_fc
+def test_solu_file_from_fortran(fortran_files):
+    tmp, ref = fortran_files
+    q, qn, nu = files.read_solu(os.path.join(tmp, "solu4"), DIMS, True)
+    assert np.array_equal(q, ref["q4"]) and np.array_equal(qn, ref["qn4"]) and np.array_equal(nu, ref["nu4"])
+    q, qn, nu = files.read_solu(os.path.join(tmp, "solu8_noturb"), DIMS, False)
+    assert nu is None
+    assert np.array_equal(q, ref["q8"]) and np.array_equal(qn, ref["qn8"])
+
+
+@needs_fc
+def test_restart_file_read_by_fortran(fortran_files, tmp_path):
+    tmp, _ = fortran_files
+    rng = np.random.default_rng(3)
+    q = rng.normal(size=(N, 5))
+    qn = rng.normal(size=(N, 5))
+    nu = rng.uniform(0, 1e-3, N)
+    files.write_solu(os.path.join(tmp, "restart4"), q, qn, nu, real_bytes=4)
+    subprocess.run(["./files_check", "read"], cwd=tmp, check=True, capture_output=True)
+    vals = np.loadtxt(os.path.join(tmp, "restart_values.txt"))
+    assert np.array_equal(vals[:5 * N].reshape(N, 5), q.astype(np.float32).astype(np.float64))
+    assert np.array_equal(vals[5 * N:10 * N].reshape(N, 5), qn.astype(np.float32).astype(np.float64))
+    assert np.array_equal(vals[10 * N:], nu.astype(np.float32).astype(np.float64))
+    # and by our own reader, as a turbulence solu file of one zone
+    q1, qn1, nu1 = files.read_solu(os.path.join(tmp, "restart4"), DIMS, True)
+    assert np.array_equal(q1, q.astype(np.float32)) and np.array_equal(nu1, nu.astype(np.float32))
+
+
+def test_restart_round_trip_fp64_and_without_turbulence(tmp_path):
+    rng = np.random.default_rng(4)
+    q = rng.normal(size=(N, 5))
+    qn = rng.normal(size=(N, 5))
+    nu = rng.uniform(0, 1e-3, N)
+    files.write_solu(tmp_path / "r8", q, qn, nu, real_bytes=8)
+    q1, qn1, nu1 = files.read_solu(tmp_path / "r8", DIMS, True)
+    assert np.array_equal(q1, q) and np.array_equal(qn1, qn) and np.array_equal(nu1, nu)
+    # without turbulence the restart output has one record per component
+    files.write_solu(tmp_path / "r8n", q, qn, None, real_bytes=8)
+    q2, qn2, _ = files.read_solu(tmp_path / "r8n", DIMS, False)
+    assert np.array_equal(q2, q) and np.array_equal(qn2, qn)
+
+
+def record(payload, splits=None):
+    """Fortran sequential record bytes; splits = subrecord sizes (gfortran
+    convention: leading count negative while the record continues, trailing
+    count negative on continuation subrecords)."""
+    if not splits:
+        n = len(payload)
+        return struct.pack("<i", n) + payload + struct.pack("<i", n)
+    out = b""
+    at = 0
+    for s, size in enumerate(splits):
+        last = s == len(splits) - 1
+        out += struct.pack("<i", size if last else -size) + payload[at:at + size]
+        out += struct.pack("<i", -size if s else size)
+        at += size
+    assert at == len(payload)
+    return out
+
+
+def test_subrecords_and_damaged_files(tmp_path):
+    dims = (3, 2, 2)
+    n = 12
+    x = np.arange(n, dtype=np.float32) + 0.5
+    body = x.tobytes()
+    f = tmp_path / "grid_sub"
+    f.write_bytes(record(body, [8, 28, 12]) + record((2 * x).tobytes()) + record((3 * x).tobytes(), [4, 44]))
+    gx, gy, gz, _ = files.read_grid(f, dims, False)
+    assert np.array_equal(gx, x) and np.array_equal(gy, 2 * x) and np.array_equal(gz, 3 * x)
+    # a trailing count that disagrees with the leading one
+    bad = bytearray(record(body))
+    bad[-4:] = struct.pack("<i", 44)
+    (tmp_path / "bad1").write_bytes(bytes(bad) + record(body) + record(body))
+    with pytest.raises(DesalmError, match="markers disagree"):
+        files.read_grid(tmp_path / "bad1", dims, False)
+    # truncated payload
+    (tmp_path / "bad2").write_bytes(record(body) + record(body)[:20])
+    with pytest.raises(DesalmError, match="y: truncated record"):
+        files.read_grid(tmp_path / "bad2", dims, False)
+    # continuation subrecord whose trailing count is not marked
+    sub = bytearray(record(body, [8, 40]))
+    sub[-4:] = struct.pack("<i", 40)
+    (tmp_path / "bad3").write_bytes(bytes(sub))
+    with pytest.raises(DesalmError, match="subrecord markers"):
+        files.read_grid(tmp_path / "bad3", dims, False)
+    with pytest.raises(DesalmError, match="cannot open"):
+        files.read_grid(tmp_path / "missing", dims, False)
+
+
+def test_filestat(tmp_path):
+    f = tmp_path / "filestat"
+    files.write_filestat(f, 12.375, 4200, [0.5, 2 * np.pi + 0.25])
+    t, n, ang = files.read_filestat(f, 2)
+    assert (t, n) == (12.375, 4200)
+    # src/init.F90:272-274: an angle of 2 pi or more is reduced once on reading
+    assert ang[0] == 0.5 and ang[1] == (2 * np.pi + 0.25) - 2 * np.pi
+    # list-directed text as a Fortran runtime writes it, values over lines
+    f.write_text("   12.5000000     ,      7\n   3.14159274    \n  1.0E+1\n")
+    t, n, ang = files.read_filestat(f, 2)
+    assert (t, n) == (12.5, 7) and list(ang) == [3.14159274, 10.0 - 2 * np.pi]
+    with pytest.raises(DesalmError, match="end of file"):
+        files.read_filestat(f, 3)
+
+
+@pytest.mark.gpu
+def test_case_from_files_runs_like_the_case(gpu, tmp_path):
+    """A case written to the reference's files (REAL*8) and loaded back runs
+    the pseudo-time iteration exactly as the case loaded from memory."""
+    from desalm import cases
+    from desalm.lib import Solver
+    c = cases.wavy(13, 11, 9, ng=2, itm=2, iters=(1, 2), des=True)
+    dims = [c.params.level_dims()[0]]
+    q, qn, qnm1, xnut = c.state()
+    (tmp_path / "grid").write_bytes(b"".join(record(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+                                             for a in (c.x, c.y, c.z, c.wd)))
+    files.write_solu(tmp_path / "solu", q, qn, xnut, real_bytes=8)
+    x, y, z, wd = files.read_grid(tmp_path / "grid", dims)
+    q1, qn1, xnut1 = files.read_solu(tmp_path / "solu", dims)
+    a = Solver(c.params).load_case(c)
+    b = Solver(c.params)
+    b.set_grid(x, y, z)
+    b.set_wall_distance(wd)
+    b.set_state(q1, qn1, qnm1, xnut1)
+    a.mg_driver()
+    b.mg_driver()
+    for fa, fb in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(fa, fb)
+    a.close()
+    b.close()
