@@ -326,6 +326,9 @@ def main():
                     help="also run the 257x193x193 large-grid DES workload (several GB of device memory)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes of the LDS-tiled kernels")
+    ap.add_argument("--tiles", default=None, metavar="V1,V2,V3:S",
+                    help="fix the launch shapes instead of autotuning: k planes per workgroup of k_visc_pk "
+                         "per level and of k_sa_rhs (e.g. 8,4,1:8, as a previous run's 'tiles' reported)")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--conv-itm", type=int, default=120,
@@ -354,6 +357,11 @@ def main():
     from desalm.params import PREC_FP64, PREC_MIXED
 
     tiles_used = {}
+    fixed_tiles = None
+    if args.tiles:
+        v, _, sa = args.tiles.partition(":")
+        fixed_tiles = ([int(t) for t in v.split(",") if t], int(sa or 0))
+        args.autotune = False
 
     def timed_run(precision, profile):
         """warmup + timed physical steps; returns (Mcell-iters/s, wall s, iterations, profile stats,
@@ -366,6 +374,8 @@ def main():
             tiles_used[precision] = sv.autotune(reps=3)
             sv.set_state(*cs.state())
         else:
+            if fixed_tiles:
+                sv.set_tiles(*fixed_tiles)
             tiles_used[precision] = sv.get_tiles()
         for _ in range(args.warmup):
             sv.mg_driver()
@@ -458,7 +468,7 @@ def main():
            "kernels_ms": {f"{nm}@L{n}": [cnt, round(ms, 3)] for (nm, n), (cnt, ms) in sorted(stats.items())},
            "tiles": {"k_planes_per_workgroup_visc_by_level": tiles_used[PREC_FP64][0],
                      "k_planes_per_workgroup_sa_rhs": tiles_used[PREC_FP64][1],
-                     "chosen_by": "desalm_autotune" if args.autotune else "default"},
+                     "chosen_by": "desalm_autotune" if args.autotune else ("--tiles" if fixed_tiles else "default")},
            "state_finite": finite, "pivot_events": pivots,
            "final_log10_update_norms": [round(float(v), 4) for v in hist[-1]]}
     print(json.dumps(out))
