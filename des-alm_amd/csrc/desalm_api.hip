@@ -107,6 +107,10 @@ struct desalm_ctx {
     std::vector<void*> rfg_dev;
     double rfg_time0 = 0.0;
     long long steps = 0;  // desalm_step calls since desalm_set_rfg
+    // case hooks of mg_driver (DESALM_HOOK_*, desalm_set_hooks); the hooks
+    // read the grid coordinates, so they need desalm_set_grid
+    int hooks = 0;
+    bool have_grid = false;
 };
 
 namespace {
@@ -433,6 +437,20 @@ int desalm_set_grid(desalm_ctx* c, const double* x, const double* y, const doubl
     for (int n = 1; n <= c->ng; n++) launch_metrics(c->stream, c->lv[n], c->F, c->C, n == 1);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_grid = true;
+    return 0;
+}
+
+int desalm_set_coordinates(desalm_ctx* c, const double* x, const double* y, const double* z) {
+    if (!c || !x || !y || !z) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const double* h[3] = {x, y, z};
+    for (int m = 0; m < 3; m++)
+        HIPCHK(c, hipMemcpyAsync(c->F.x[m], h[m], sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
+    launch_inject(c->stream, c->lv, c->ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.x, 3);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_grid = true;
     return 0;
 }
 
@@ -612,6 +630,8 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
             if (c != c0) c0->err = c->err;
             return -1;
         }
+        // outlet mass-flux rescale of the starting iterate (mg_driver.F90:64-75)
+        if (c->hooks & DESALM_HOOK_CONSERVATION) launch_conservation(s, L1, c->F);
         launch_copy_qold(s, L1, c->F);
     }
     HIPCHK(c0, hipGetLastError());
@@ -638,6 +658,9 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
                     if (have_table) timed(c0, KID_OVERSET, 1, [&] { exchange(1); });
                 }
                 for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr);
+                // wall functions on the j walls of zone 1 (mg_driver.F90:218-224)
+                if (n == 1 && (c0->hooks & DESALM_HOOK_WALL_FUNCTIONS))
+                    timed(c0, KID_BCOND, 1, [&] { launch_wall_functions(s, c0->lv[1], c0->F, c0->p.ren); });
                 if (n == 1 && have_table && apply_mom) timed(c0, KID_OVERSET, 1, [&] { exchange(0); });
             }
         }
@@ -1144,6 +1167,10 @@ int desalm_group_set_overset(desalm_group* g, int64_t n, const int32_t* hosts, c
 int desalm_group_mg_driver(desalm_group* g, int* itc_out, double* resid_out) {
     if (!g) return fail(nullptr, "null argument");
     desalm_ctx* c0 = g->z[0];
+    for (desalm_ctx* c : g->z)
+        if (c->hooks)
+            return fail(c0, "desalm_set_hooks is for a single zone: the reference sums the outlet flux over "
+                            "every rank of every zone but zone 2 (conservation.F90:104-131)");
     HIPCHK(c0, hipSetDevice(c0->device));
     hipStream_t s = c0->stream;
     // every zone's work goes on zone 1's stream for the duration of the call
@@ -1322,6 +1349,16 @@ int desalm_autotune(desalm_ctx* c, int reps) {
         c->kc_sa = pick;
     }
     HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int desalm_set_hooks(desalm_ctx* c, int hooks) {
+    if (!c) return fail(c, "null argument");
+    if (hooks & ~(DESALM_HOOK_CONSERVATION | DESALM_HOOK_WALL_FUNCTIONS)) return fail(c, "unknown hook flags");
+    if (hooks && !c->have_grid)
+        return fail(c, "the hooks rebuild velocities from grid derivatives: call desalm_set_grid or "
+                       "desalm_set_coordinates first");
+    c->hooks = hooks;
     return 0;
 }
 

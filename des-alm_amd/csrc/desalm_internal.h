@@ -209,6 +209,10 @@ void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Field
                     const Consts& C, int icrs, int jcrs, int kcrs, const double* rs_tab);
 void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save_only);
 void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F);
+// case hooks of mg_driver (k_hooks.hip): the outlet mass-flux rescale and
+// the wall functions of both j walls, on the fine level
+void launch_conservation(hipStream_t s, const Level& L, const Fields& F);
+void launch_wall_functions(hipStream_t s, const Level& L, const Fields& F, double ren);
 // metrics, k_metrics.hip
 void launch_metrics(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int fine);
 // layout conversion, k_mg.hip

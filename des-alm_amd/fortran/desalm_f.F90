@@ -32,7 +32,8 @@ module desalm_f
      integer(c_int32_t) :: reserved(8)
   end type desalm_params_t
 
-  public :: desalm_init, desalm_set_metrics, desalm_set_wall_distance, desalm_set_state, &
+  public :: desalm_init, desalm_set_metrics, desalm_set_coordinates, desalm_set_hooks, &
+            desalm_set_wall_distance, desalm_set_state, &
             desalm_get_state, desalm_mg_driver, desalm_solver_daf, desalm_finalize
   public :: desalm_f_ctx, desalm_f_setup, desalm_f_check
 
@@ -49,6 +50,19 @@ module desalm_f
        type(c_ptr), value :: ctx
        real(c_double), intent(in) :: csi(*), eta(*), zet(*), aj(*)
      end function desalm_set_metrics
+
+     integer(c_int) function desalm_set_coordinates(ctx, x, y, z) bind(C, name = 'desalm_set_coordinates')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: x(*), y(*), z(*)
+     end function desalm_set_coordinates
+
+     ! conservation (1) and the j-wall wall functions (2) of mg_driver
+     integer(c_int) function desalm_set_hooks(ctx, hooks) bind(C, name = 'desalm_set_hooks')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: hooks
+     end function desalm_set_hooks
 
      integer(c_int) function desalm_set_wall_distance(ctx, wd, dtev) bind(C, name = 'desalm_set_wall_distance')
        import :: c_int, c_ptr, c_double

@@ -75,6 +75,8 @@ def lib():
         fp = ctypes.POINTER(ctypes.c_float)
         i64 = ctypes.c_int64
         L.desalm_set_convergence.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.desalm_set_hooks.argtypes = [vp, ctypes.c_int]
+        L.desalm_set_coordinates.argtypes = [vp, dp, dp, dp]
         L.desalm_step.argtypes = [vp, ip, dp]
         L.desalm_ws_configure.argtypes = [vp, ctypes.c_int, i64, i64, ctypes.c_uint64]
         L.desalm_ws_mode.argtypes = [vp, ctypes.c_int]
@@ -129,6 +131,10 @@ def _f(a):
 
 def _c(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+HOOK_CONSERVATION = 1
+HOOK_WALL_FUNCTIONS = 2
 
 
 class Solver:
@@ -260,6 +266,16 @@ class Solver:
         state and keep the fastest; returns get_tiles()."""
         self._chk(lib().desalm_autotune(self.h, reps), "desalm_autotune")
         return self.get_tiles()
+
+    def set_coordinates(self, x, y, z):
+        """Fine-level coordinates only (metrics from set_metrics stay)."""
+        self._chk(lib().desalm_set_coordinates(self.h, _p(x), _p(y), _p(z)), "desalm_set_coordinates")
+
+    def set_hooks(self, hooks):
+        """HOOK_CONSERVATION | HOOK_WALL_FUNCTIONS (the reference's mg_driver
+        calls both on every run; 0 turns them off)."""
+        self._chk(lib().desalm_set_hooks(self.h, int(hooks)), "desalm_set_hooks")
+        return self
 
     def set_convergence(self, itm, it_min, er_min, eo_min):
         self._chk(lib().desalm_set_convergence(self.h, itm, it_min, er_min, eo_min), "desalm_set_convergence")

@@ -101,6 +101,11 @@ int desalm_set_grid(desalm_ctx* ctx, const double* x, const double* y, const dou
 int desalm_set_metrics(desalm_ctx* ctx, const double* csi, const double* eta,
                        const double* zet, const double* aj);
 
+/* Fine-level coordinates only, for a caller that passes its own metrics with
+ * desalm_set_metrics (as the Fortran drop-in does): the case hooks of
+ * desalm_set_hooks read them.  Metrics are left as they are. */
+int desalm_set_coordinates(desalm_ctx* ctx, const double* x, const double* y, const double* z);
+
 /* Wall distance of the fine level (img*jmg*kmg), pre-scaled for DES as the
  * reference reads it from its grid file (src/init.F90:2313).  dtev may be
  * NULL (keep the value computed by desalm_set_grid) or a fine-level array. */
@@ -404,6 +409,29 @@ int desalm_ws_features(desalm_ctx* ctx, int64_t m, float* x, float* t);
  * scales) with its gradient with respect to the packed parameters. */
 int desalm_mlp_forward(desalm_ctx* ctx, const float* x, int64_t m, float* y);
 int desalm_mlp_loss_grad(desalm_ctx* ctx, const float* x, const float* t, int64_t m, float* grad, double* loss);
+
+/*
+ * The two case hooks the reference's mg_driver calls on every run of zone 1
+ * on one rank (bit flags, 0 = none, the default):
+ *  DESALM_HOOK_CONSERVATION: `conservation` (src/conservation.F90:1-174,
+ *    called at src/mg_driver.F90:64-75) after the time shift: U/J summed
+ *    over the i = 1 and i = img planes, U/J on the i = img plane scaled by
+ *    fluxin / fluxout and u, v, w rebuilt there.  Divides by the outlet flux:
+ *    a closed i = img face gives NaN, as in the reference.
+ *  DESALM_HOOK_WALL_FUNCTIONS: `wf_eta` on both j walls after every
+ *    fine-level DAF pass (src/wf_mpi.F90:246-413, src/mg_driver.F90:218-224):
+ *    log-law velocity and nu~ at the first node off the wall from the
+ *    friction velocity of the node two off.  u_tau is solved to rounding at
+ *    every node; the reference's Newton chain stops at |f| < 1e-4, so the two
+ *    differ by up to that tolerance (about 1e-4 of the velocity).
+ * Needs desalm_set_grid or desalm_set_coordinates (the hooks use the grid
+ * derivatives).  A group of
+ * zones rejects hooks (the reference's flux sums run over all ranks).
+ * A desalm_ws_mode warm start is applied before conservation.
+ */
+#define DESALM_HOOK_CONSERVATION 1
+#define DESALM_HOOK_WALL_FUNCTIONS 2
+int desalm_set_hooks(desalm_ctx* ctx, int hooks);
 
 /*
  * ---- The reference's files (host only: no device, no context) ----

@@ -51,6 +51,10 @@ struct or_state {
     double* ocoef;  /* 3 per donor */
     int* onodes;    /* 3 per donor, 1-based i,j,k */
     int apply_mom;  /* mg_bintp_mom runs (the reference's nzone > 1) */
+    /* case hooks of mg_driver (OR_HOOK_*): the outlet mass-flux rescale and
+     * the j-wall wall functions; wf_newton selects the reference's own
+     * Newton chain (1) or a root converged to rounding (0) */
+    int hooks, wf_newton;
     /* actuator-line model (module global_turb): deep copies of the tables */
     int n_tur, n_polar;
     desalm_turbine* tur;
@@ -2246,6 +2250,256 @@ int or_bintp(or_state* o, int which) {
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* conservation (src/conservation.F90:1-174) on one rank and one zone   */
+/* (igp = jgp = kgp = 0, init.F90:671): the contravariant flux U/J of   */
+/* rhs_contra_j (src/rhs_contra_j.F90:12-27) is summed over the whole   */
+/* i = 1 and i = img planes (j fastest, then k, as the send buffers are */
+/* packed), and U/J on the outlet plane is scaled by fluxin / fluxout   */
+/* before u, v, w are rebuilt there from the grid derivatives.          */
+/* ------------------------------------------------------------------ */
+static double contra(const or_state* o, const double* met, long l) {
+    const double tmp = 1.0 / o->aj[l];
+    return tmp * (met[3 * l] * o->q[5 * l + 1] + met[3 * l + 1] * o->q[5 * l + 2] + met[3 * l + 2] * o->q[5 * l + 3]);
+}
+
+static void conservation(or_state* o) {
+    LEVEL(1);
+    double fluxin = 0.0, fluxout = 0.0;
+    for (int k = 1; k <= nk; k++)
+        for (int j = 1; j <= nj; j++) fluxin = fluxin + contra(o, o->csi, IX(1, j, k));
+    for (int k = 1; k <= nk; k++)
+        for (int j = 1; j <= nj; j++) fluxout = fluxout + contra(o, o->csi, IX(ni, j, k));
+    const double delta_e = fluxin - fluxout;
+    for (int k = 1; k <= nk; k++)
+        for (int j = 1; j <= nj; j++) {
+            const long l = IX(ni, j, k);
+            double u1 = contra(o, o->csi, l);
+            const double u2 = contra(o, o->eta, l), u3 = contra(o, o->zet, l);
+            u1 = u1 + delta_e * u1 / fluxout;
+            const double a = o->aj[l];
+            o->q[5 * l + 1] = a * (u1 * o->xc[l] + u2 * o->xe[l] + u3 * o->xz[l]);
+            o->q[5 * l + 2] = a * (u1 * o->yc[l] + u2 * o->ye[l] + u3 * o->yz[l]);
+            o->q[5 * l + 3] = a * (u1 * o->zc[l] + u2 * o->ze[l] + u3 * o->zz[l]);
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Wall functions on the j walls: wf_eta (src/wf_mpi.F90:246-413) with  */
+/* its constants (:26-30; ks = 0 switches the roughness shift off).     */
+/* For every interior (i, k) of the wall j = 1 (jb = 1) or j = jmg      */
+/* (jb = 2) the friction velocity u_tau solves                          */
+/*   f(u) = (ln(Re y u) / kappa + B) u - u0   if Re y u > 11.3,          */
+/*   f(u) = Re y u^2 - u0                      otherwise,                 */
+/*   f = 0 where |u Re y - 11.3| < 1e-4 (funcv, :604-631),               */
+/* with u0 the wall-parallel speed and y the wall distance two nodes    */
+/* off the wall; the first node off the wall then gets the log-law (or  */
+/* linear) speed in the direction of the third node's velocity and      */
+/* nu~ = kappa y+ / Re.                                                 */
+/* ------------------------------------------------------------------ */
+static const double wf_kapa = 0.41, wf_blog = 5.45, wf_yplm = 11.30, wf_qmin = 1.0e-08;
+
+static double wf_f(double x, double u0, double yp0, double re0) {
+    double f;
+    if (re0 * yp0 * x > 11.3) f = (1.0 * log(re0 * yp0 * x) / wf_kapa + wf_blog - 0.0) * x - u0;
+    else f = re0 * (x * x) * yp0 - u0;
+    if (fabs(x * re0 * yp0 - 11.3) < 1.0e-4) f = 0.0;
+    return f;
+}
+
+static double wf_df(double x, double yp0, double re0) {
+    if (re0 * yp0 * x > 11.3) return log(re0 * yp0 * x) / wf_kapa + wf_blog + 1.0 / wf_kapa;
+    return re0 * 2.0 * x * yp0;
+}
+
+/* The reference's solve: newt (:634-720) for one unknown, i.e. ludcmp /
+ * lubksb reduce to p = -f / f', with the backtracking line search lnsrch
+ * (:765-850); the globally convergent Newton method of Press et al.,
+ * Numerical Recipes (2nd ed., section 9.7).  x is the start and the result.
+ * Returns -1 where the reference stops the program (a zero derivative in
+ * ludcmp, a non-descent direction in lnsrch). */
+static int wf_newt(double* xp, double u0, double yp0, double re0) {
+    const double TOLF = 1.e-4, TOLMIN = 1.e-6, TOLX = 1.e-7, STPMX = 100., ALF = 1.e-3;
+    double x = *xp;
+    double fvec = wf_f(x, u0, yp0, re0);
+    double f = 0.5 * (0.0 + fvec * fvec);
+    double test = fabs(fvec);
+    if (test < .01 * TOLF) return 0;
+    double sum = 0.0 + x * x;
+    const double stpmax = STPMX * dmax2(sqrt(sum), 1.0);
+    for (int its = 1; its <= 200; its++) {
+        fvec = wf_f(x, u0, yp0, re0);
+        const double fjac = wf_df(x, yp0, re0);
+        const double g = 0.0 + fjac * fvec;
+        const double xold = x, fold = f;
+        double p = -fvec;
+        if (fabs(fjac) == 0.0) return -1;             /* 'singular matrix in ludcmp' */
+        p = p / fjac;
+        /* lnsrch */
+        int check = 0;
+        sum = sqrt(0.0 + p * p);
+        if (sum > stpmax) p = p * stpmax / sum;
+        const double slope = 0.0 + g * p;
+        if (slope >= 0.) return -1;                   /* 'roundoff problem in lnsrch' */
+        const double alamin = TOLX / (fabs(p) / dmax2(fabs(xold), 1.));
+        double alam = 1., alam2 = 2., f2 = 2., tmplam = 0.0;
+        for (;;) {
+            x = xold + alam * p;
+            fvec = wf_f(x, u0, yp0, re0);
+            f = 0.5 * (0.0 + fvec * fvec);
+            if (alam < alamin) {
+                x = xold;
+                check = 1;
+                break;
+            } else if (f <= fold + ALF * alam * slope) {
+                break;
+            }
+            if (alam == 1.) {
+                tmplam = -slope / (2. * (f - fold - slope));
+            } else {
+                const double rhs1 = f - fold - alam * slope;
+                const double rhs2 = f2 - fold - alam2 * slope;
+                const double a = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
+                const double b = (-(alam2 * rhs1 / (alam * alam)) + alam * rhs2 / (alam2 * alam2)) / (alam - alam2);
+                if (a == 0.) {
+                    tmplam = -slope / (2. * b);
+                } else {
+                    const double disc = b * b - 3. * a * slope;
+                    if (disc < 0.) tmplam = .5 * alam;
+                    else if (b <= 0.) tmplam = (-b + sqrt(disc)) / (3. * a);
+                    else tmplam = -slope / (b + sqrt(disc));
+                }
+                if (tmplam > .5 * alam) tmplam = .5 * alam;
+            }
+            alam2 = alam;
+            f2 = f;
+            alam = dmax2(tmplam, .1 * alam);
+        }
+        /* back in newt */
+        if (fabs(fvec) < TOLF) break;
+        if (check) break;
+        if (fabs(x - xold) / dmax2(fabs(x), 1.) < TOLX) break;
+    }
+    *xp = x;
+    return 0;
+}
+
+/* The same root converged to rounding: f is increasing in u > 0 (both
+ * branches), convex on the log branch and jumps up at Re y u = 11.3, so
+ * the root is sqrt(u0 / (Re y)) when that lies on the quadratic branch,
+ * the jump point when f changes sign across it, and otherwise the limit
+ * of Newton's iteration started right of the root on the log branch
+ * (monotone for a convex increasing function).  Every node is solved on
+ * its own; the result does not depend on a start. */
+static double wf_root(double u0, double yp0, double re0) {
+    if (!(u0 > 0.0)) return u0 == 0.0 ? 0.0 : u0 - u0;
+    const double c = re0 * yp0;
+    const double us = sqrt(u0 / c);
+    if (c * us <= 11.3) return us;
+    const double lo = 11.3 / c;
+    if (log(c * lo) / wf_kapa * lo + wf_blog * lo - u0 >= 0.0) return lo;
+    double x = 2.0 * lo;
+    for (int t = 0; t < 200 && (log(c * x) / wf_kapa + wf_blog) * x - u0 < 0.0; t++) x = 2.0 * x;
+    for (int t = 0; t < 100; t++) {
+        const double lg = log(c * x);
+        const double fx = (lg / wf_kapa + wf_blog) * x - u0;
+        const double step = fx / (lg / wf_kapa + wf_blog + 1.0 / wf_kapa);
+        const double xn = x - step;
+        if (!(xn < x) || xn <= lo) break;
+        x = xn;
+    }
+    return x;
+}
+
+double or_wf_root(double u0, double yp0, double re0) { return wf_root(u0, yp0, re0); }
+
+int or_wf_newt(double* x, double u0, double yp0, double re0) { return wf_newt(x, u0, yp0, re0); }
+
+static int wf_eta(or_state* o, int jb) {
+    LEVEL(1);
+    const int j = jb == 1 ? 1 : nj, j1 = jb == 1 ? 2 : nj - 1, j2 = jb == 1 ? 3 : nj - 2;
+    const double re0 = o->p.ren;
+    double* ut0 = (double*)malloc(sizeof(double) * (size_t)ni * nk);
+    double ut = 0.03;  /* the reference carries the start from node to node */
+    const double* q = o->q;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int i = 2; i <= ni - 1; i++) {
+            const long l = IX(i, j, k), l2 = IX(i, j2, k);
+            const double* e = o->eta + 3 * l;
+            const double vmod = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            const double qnor = (q[5 * l2 + 1] * e[0] + q[5 * l2 + 2] * e[1] + q[5 * l2 + 3] * e[2]) / vmod;
+            double qpar = q[5 * l2 + 1] * q[5 * l2 + 1] + q[5 * l2 + 2] * q[5 * l2 + 2] + q[5 * l2 + 3] * q[5 * l2 + 3];
+            qpar = sqrt(qpar - qnor * qnor);
+            const double u0 = qpar, yp0 = o->wd[l2];
+            if (o->wf_newton) {
+                if (wf_newt(&ut, u0, yp0, re0)) {
+                    free(ut0);
+                    return -1;
+                }
+            } else {
+                ut = wf_root(u0, yp0, re0);
+            }
+            ut0[(i - 1) + (long)ni * (k - 1)] = ut;
+        }
+    const double* x = o->x;
+    const double* y = o->y;
+    const double* z = o->z;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int i = 2; i <= ni - 1; i++) {
+            const long l1 = IX(i, j1, k), l2 = IX(i, j2, k);
+            const double* q2v = q + 5 * l2;
+            double* q1v = o->q + 5 * l1;
+            const double qpar = q2v[1] * q2v[1] + q2v[2] * q2v[2] + q2v[3] * q2v[3];
+            const double u = ut0[(i - 1) + (long)ni * (k - 1)];
+            if (qpar > wf_qmin) {
+                const double yplus = o->wd[l1] * o->p.ren * u;
+                const double deltb = 0.0;
+                double qw;
+                if (yplus > wf_yplm) qw = (log(yplus) / wf_kapa + wf_blog - deltb) * u;
+                else qw = yplus * u;
+                long lu = IX(i + 1, j, k), ld = IX(i - 1, j, k);
+                const double xc = pt5 * (x[lu] - x[ld]), yc = pt5 * (y[lu] - y[ld]), zc = pt5 * (z[lu] - z[ld]);
+                lu = IX(i, j, k + 1);
+                ld = IX(i, j, k - 1);
+                const double xz = pt5 * (x[lu] - x[ld]), yz = pt5 * (y[lu] - y[ld]), zz = pt5 * (z[lu] - z[ld]);
+                const double sg11 = sqrt(xc * xc + yc * yc + zc * zc);
+                const double sg33 = sqrt(xz * xz + yz * yz + zz * zz);
+                const double* cs = o->csi + 3 * l2;
+                const double* zt = o->zet + 3 * l2;
+                double qc1 = (cs[0] * q2v[1] + cs[1] * q2v[2] + cs[2] * q2v[3]) * sg11;
+                double qc2 = (zt[0] * q2v[1] + zt[1] * q2v[2] + zt[2] * q2v[3]) * sg33;
+                const double cgama = (xc * xz + yc * yz + zc * zz) / (sg11 * sg33);
+                double fbeta = sqrt(qc1 * qc1 + qc2 * qc2 + 2. * qc1 * qc2 * cgama);
+                fbeta = ((qc1 + qc2 * cgama) / fbeta);
+                fbeta = dmin2(fbeta, 1.0);
+                fbeta = dmax2(fbeta, -1.0);
+                if (qc2 > 0.0) fbeta = acos(fbeta);
+                else fbeta = -acos(fbeta);
+                const double qstar = qw;
+                qc2 = qstar * sin(fbeta) / sin(acos(cgama));
+                qc1 = qstar * cos(fbeta) - qc2 * cgama;
+                q1v[1] = qc1 * xc / sg11 + qc2 * xz / sg33;
+                q1v[2] = qc1 * yc / sg11 + qc2 * yz / sg33;
+                q1v[3] = qc1 * zc / sg11 + qc2 * zz / sg33;
+                q1v[4] = wf_kapa * yplus / re0;
+            } else {
+                q1v[1] = 0.0;
+                q1v[2] = 0.0;
+                q1v[3] = 0.0;
+                q1v[4] = 0.1 / re0;
+            }
+        }
+    free(ut0);
+    return 0;
+}
+
+int or_set_hooks(or_state* o, int hooks, int wf_newton) {
+    if (hooks & ~(OR_HOOK_CONSERVATION | OR_HOOK_WALL_FUNCTIONS)) return -1;
+    o->hooks = hooks;
+    o->wf_newton = wf_newton;
+    return 0;
+}
+
 /* mg_driver (src/mg_driver.F90:2-323) with mg_ck_conver (:335-405) */
 int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
     const int ng = o->ng, ns = 1;
@@ -2262,6 +2516,7 @@ int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
         mg_inject(o, o->qn, 5, m);
         mg_inject(o, o->qnm1, 5, m);
     }
+    if (o->hooks & OR_HOOK_CONSERVATION) conservation(o); /* :64-75 */
     for (it = 1; it <= o->p.itm; it++) {
         double residual[5], eomax, ermax;
         int converged = 0;
@@ -2279,6 +2534,9 @@ int or_mg_driver(or_state* o, int* itc_out, double* resid_hist) {
                     bintp_tur(o);
                 }
                 solver_daf(o, n, decide_recalc_rh, itr);
+                /* wall functions of zone 1 on the j walls (:218-224) */
+                if (n == ns && (o->hooks & OR_HOOK_WALL_FUNCTIONS))
+                    if (wf_eta(o, 1) || wf_eta(o, 2)) return -2;
                 if (n == ns && o->apply_mom) bintp_mom(o);
             }
             mg_zero_pk_boundary(o, n);

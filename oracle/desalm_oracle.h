@@ -81,6 +81,21 @@ int or_set_turbines(or_state* o, int n_tur, const desalm_turbine* tur, int n_pol
 
 /* LAPACK SGTSV algorithm in FP64 (src/sgtsv.f:1-263); returns INFO */
 int or_gtsv(int n, int nrhs, double* dl, double* d, double* du, double* b, int ldb);
+/* The two case hooks the reference's mg_driver calls on every run:
+ * conservation (src/conservation.F90, src/mg_driver.F90:64-75) after the
+ * time shift, and wf_eta on both j walls after every fine-level solver_daf
+ * (src/wf_mpi.F90:246-413, src/mg_driver.F90:218-224).  wf_newton = 1
+ * solves for u_tau with the reference's own Newton chain (start 0.03 on the
+ * first node, then each node from the previous one's result, stopped at
+ * |f| < 1e-4); 0 converges every node's root to rounding, as the HIP
+ * library does.  Hooks start off. */
+#define OR_HOOK_CONSERVATION 1
+#define OR_HOOK_WALL_FUNCTIONS 2
+int or_set_hooks(or_state* o, int hooks, int wf_newton);
+/* u_tau of one node: the converged root, and the reference's Newton from *x */
+double or_wf_root(double u0, double yp0, double re0);
+int or_wf_newt(double* x, double u0, double yp0, double re0);
+
 /* row interchanges performed by or_gtsv so far (reset to 0 if reset != 0) */
 long or_gtsv_pivots(int reset);
 

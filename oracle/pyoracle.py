@@ -14,6 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_EXE = os.path.join(HERE, "_ref", "desalm_ref_harness")
+# the same harness with the reference's conservation and wf_mpi linked in
+HOOKS_EXE = os.path.join(HERE, "_ref", "desalm_ref_harness_hooks")
+HOOK_CONSERVATION = 1
+HOOK_WALL_FUNCTIONS = 2
 SHIM_MGD_EXE = os.path.join(HERE, "_ref", "desalm_shim_mgd")
 SHIM_DAF_EXE = os.path.join(HERE, "_ref", "desalm_shim_daf")
 
@@ -60,6 +64,10 @@ def lib():
         L.or_gtsv.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
         ip = ctypes.POINTER(ctypes.c_int)
         L.or_set_overset.argtypes = [vp, ctypes.c_long, ip, dp, ip, ctypes.c_int]
+        L.or_set_hooks.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_wf_root.argtypes = [ctypes.c_double] * 3
+        L.or_wf_root.restype = ctypes.c_double
+        L.or_wf_newt.argtypes = [dp] + [ctypes.c_double] * 3
         L.or_group_mg_driver.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_long, ip, dp, ip, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), dp]
         L.or_group_bintp.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_long, ip, dp, ip, ctypes.c_int]
@@ -126,6 +134,12 @@ class Oracle:
                                    ctypes.cast(ps, ctypes.c_void_p), alm.length_scale, alm.vel_infty)
         assert rc == 0
         del keep
+
+    def set_hooks(self, hooks, wf_newton=False):
+        """OR_HOOK_* flags; wf_newton: the reference's own u_tau Newton chain."""
+        if lib().or_set_hooks(self.h, int(hooks), int(bool(wf_newton))) != 0:
+            raise ValueError("bad hook flags")
+        return self
 
     def set_overset(self, ov):
         """Donor table (desalm.cases.Overset) of the fine level."""
@@ -218,6 +232,23 @@ def gtsv(dl, d, du, b):
 # ---------------------------------------------------------------------------
 def ref_available():
     return os.path.exists(REF_EXE)
+
+
+def hooks_available():
+    return os.path.exists(HOOKS_EXE)
+
+
+def wf_root(u0, yp0, re0):
+    """u_tau converged to rounding (the wall-function root)."""
+    return lib().or_wf_root(u0, yp0, re0)
+
+
+def wf_newt(x, u0, yp0, re0):
+    """The reference's Newton for u_tau from start x; returns the result."""
+    v = ctypes.c_double(x)
+    if lib().or_wf_newt(ctypes.byref(v), u0, yp0, re0) != 0:
+        raise RuntimeError("the reference's solve would stop the program here")
+    return v.value
 
 
 def group_mg_driver(oracles, table):

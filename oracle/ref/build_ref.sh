@@ -69,6 +69,19 @@ done
 "$FC" -o "$OUT/desalm_ref_harness" $REFOBJ -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB"
 echo "built $OUT/desalm_ref_harness"
 
+# The same harness with the reference's two case hooks linked in place of
+# the empty bodies: conservation (src/conservation.F90, which INCLUDEs
+# rhs_contra_j.F90 and rhs_exchng3_4d.F90 from its own directory) and the
+# wall-function module (src/wf_mpi.F90).  Its module file goes to a
+# directory of its own so the stub's wf_mpi.mod stays the one mg_driver saw.
+mkdir -p "$OUT/obj_hooks"
+"$FC" -O2 -fdefault-real-8 -ffp-contract=off -J"$OUT/obj_hooks" -I"$OUT/obj" -I"$MPI_INC" -c "$REF/wf_mpi.F90" -o "$OUT/obj_hooks/wf_mpi.o" \
+    2>"$OUT/obj_hooks/wf_mpi.log" || { cat "$OUT/obj_hooks/wf_mpi.log" >&2; exit 1; }
+cc "$REF/conservation.F90" conservation
+"$FC" -o "$OUT/desalm_ref_harness_hooks" ${REFOBJ/ $OUT\/obj\/hooks_off.o/} \
+    "$OUT/obj_hooks/wf_mpi.o" "$OUT/obj/conservation.o" -L"$MPI_LIB" -lmpifort -lmpi -Wl,-rpath,"$MPI_LIB"
+echo "built $OUT/desalm_ref_harness_hooks"
+
 # The same harness with the Fortran drop-in of des-alm_amd/fortran linked in
 # place of one reference routine: desalm_shim_mgd swaps mg_driver,
 # desalm_shim_daf swaps solver_daf (the reference mg_driver then calls the

@@ -44,6 +44,11 @@ FIXTURES = {
     "wavy_mg_driver_alm_sa": ("wavy", dict(ni=21, nj=17, nk=17, ng=2, itm=4, amp=0.02, des=True, iters=(1, 2, 3),
                                            alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)),
                               dict(mode=1, nsteps=2)),
+    # the case hooks of mg_driver (conservation, wall functions on the j walls),
+    # reference built with src/conservation.F90 and src/wf_mpi.F90 linked in
+    "wavy_mg_driver_hooks_sa": ("wavy", dict(ni=17, nj=17, nk=21, ng=2, itm=3, des=True, iters=(1, 2, 3),
+                                             btype=(3, 3, 1, 1, 2, 2), seed=99, amp=0.06),
+                                dict(mode=1, nsteps=2, hooks=3)),
     # early exit on the reference's convergence test (mg_ck_conver)
     "wavy_mg_driver_converge": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=12, it_min=2, er_min=-0.15,
                                              iters=(1, 2, 3)), dict(mode=1, nsteps=1)),
@@ -94,11 +99,18 @@ def conver_table(text):
 
 
 def main():
+    """Writes every fixture, or those named on the command line."""
     if not po.ref_available():
         sys.exit("oracle/_ref/desalm_ref_harness is not built (needs /root/reference)")
+    only = set(sys.argv[1:])
     for name, (kind, kw, opts) in FIXTURES.items():
+        if only and name not in only:
+            continue
         case = build(kind, kw)
-        out = po.run_reference(case, **opts)
+        run = {k: v for k, v in opts.items() if k != "hooks"}
+        if opts.get("hooks"):
+            run["exe"] = po.HOOKS_EXE
+        out = po.run_reference(case, **run)
         nall_c = case.params.nodes_all()
         keep = {"q": out["q"], "xnut": out["xnut"]}
         if opts["mode"] == 2:

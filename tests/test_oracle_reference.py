@@ -31,6 +31,8 @@ def _run_oracle(case, opts):
     """Oracle run of a harness mode; returns the oracle and, for mg_driver,
     the per-iteration rows (iteration, 5 log10 norms) of every step."""
     o = po.Oracle(case.params).load_case(case)
+    if opts.get("hooks"):
+        o.set_hooks(opts["hooks"], wf_newton=True)
     mode = opts["mode"]
     rows = []
     if mode == 1:
@@ -129,3 +131,23 @@ def test_oracle_overset_mg_driver_and_norms_live():
     assert np.array_equal(o.get_state()[0], r["q"])
     assert np.array_equal(o.get_state()[3], r["xnut"])
     check_conver(rows, conver_table(r["conver"]))
+
+
+@pytest.mark.skipif(not po.hooks_available(), reason="reference harness with hooks not built (needs /root/reference)")
+@pytest.mark.parametrize("ng,bt,hooks", [(2, (3, 3, 1, 1, 2, 2), 3), (3, (0, 0, 1, 1, 2, 2), 3),
+                                         (2, (3, 3, 1, 1, 2, 2), 1)])
+def test_oracle_hooks_match_reference_live(ng, bt, hooks):
+    """conservation and wf_eta (the reference's own Newton chain for u_tau)
+    against the reference built with src/conservation.F90 and src/wf_mpi.F90.
+    With only conservation on, the reference's wall functions are still
+    linked in, so that case checks that the oracle's flags select them."""
+    c = cases.wavy(17, 17, 21, ng=ng, itm=3, des=True, btype=bt, iters=(1, 2, 3), seed=5, amp=0.05)
+    o = po.Oracle(c.params).load_case(c).set_hooks(3, wf_newton=True)
+    o.mg_driver()
+    r = po.run_reference(c, mode=1, nsteps=1, exe=po.HOOKS_EXE)
+    assert np.array_equal(o.get_state()[0], r["q"])
+    assert np.array_equal(o.get_state()[3], r["xnut"])
+    if hooks != 3:
+        o2 = po.Oracle(c.params).load_case(c).set_hooks(hooks, wf_newton=True)
+        o2.mg_driver()
+        assert not np.array_equal(o2.get_state()[0], r["q"])
