@@ -160,6 +160,11 @@ int validate(const desalm_params* p, std::string& why) {
             why = "unsupported boundary type (characteristic outlet, btype 5, is out of scope)";
             return -1;
         }
+    for (int b = 0; b < 4; b++)
+        if (p->btype[b] == DESALM_BC_INFLOW && (p->bdir[b] < 1 || p->bdir[b] > 3)) {
+            why = "an inflow face (btype 4) needs bdir 1, 2 or 3: its pressure extrapolation uses that coordinate";
+            return -1;
+        }
     if (p->ifix < 1 || p->ifix > p->img || p->jfix < 1 || p->jfix > p->jmg || p->kfix < 1 || p->kfix > p->kmg) {
         why = "pinned-pressure node outside the fine level";
         return -1;
@@ -379,6 +384,10 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     F.resid_part = A(5L * kResidBlocks);
     F.resid = A(5);
     F.pfix = A(1);
+    for (int b = 0; b < 4; b++) {
+        const Level& L1 = c->lv[1];
+        F.rat[b] = p->btype[b] == DESALM_BC_INFLOW ? A(b < 2 ? (long)L1.nj * L1.nk : (long)L1.ni * L1.nk) : nullptr;
+    }
     {
         void* pv = nullptr;
         if (hipMalloc(&pv, sizeof(unsigned long long)) != hipSuccess) ok = false;
@@ -435,6 +444,7 @@ int desalm_set_grid(desalm_ctx* c, const double* x, const double* y, const doubl
         HIPCHK(c, hipMemcpyAsync(c->F.x[m], h[m], sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
     launch_inject(c->stream, c->lv, c->ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.x, 3);
     for (int n = 1; n <= c->ng; n++) launch_metrics(c->stream, c->lv[n], c->F, c->C, n == 1);
+    launch_outlet_ratios(c->stream, c->lv[1], c->F, c->p.btype, c->p.bdir);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_grid = true;
@@ -448,6 +458,7 @@ int desalm_set_coordinates(desalm_ctx* c, const double* x, const double* y, cons
     for (int m = 0; m < 3; m++)
         HIPCHK(c, hipMemcpyAsync(c->F.x[m], h[m], sizeof(double) * c->n1, hipMemcpyHostToDevice, c->stream));
     launch_inject(c->stream, c->lv, c->ng, c->p.icrs, c->p.jcrs, c->p.kcrs, c->F.x, 3);
+    launch_outlet_ratios(c->stream, c->lv[1], c->F, c->p.btype, c->p.bdir);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_grid = true;
@@ -605,6 +616,12 @@ int ws_start(desalm_ctx* c, hipStream_t s) {
 // Between two exchanges the zones' work is independent, so it runs zone by
 // zone.  The host synchronises once per pseudo-iteration for the norms.
 extern "C++" {
+static bool needs_coordinates(const desalm_ctx* c) {
+    for (int b = 0; b < 4; b++)
+        if (c->p.btype[b] == DESALM_BC_INFLOW) return true;
+    return false;
+}
+
 template <typename Exchange>
 static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom, Exchange exchange,
                    double* init_resd, int* itc_out, double* resid_out) {
@@ -612,6 +629,10 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     const desalm_params& p = c0->p;
     hipStream_t s = c0->stream;
     const int ng = c0->ng;
+    for (desalm_ctx* c : zs)
+        if (needs_coordinates(c) && !c->have_grid)
+            return fail(c0, "an inflow face (btype 4) extrapolates the pressure on the grid ratio: call "
+                            "desalm_set_grid or desalm_set_coordinates first");
     for (desalm_ctx* c : zs) {
         const Level& L1 = c->lv[1];
         HIPCHK(c0, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));

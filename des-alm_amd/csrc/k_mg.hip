@@ -306,7 +306,35 @@ __global__ void __launch_bounds__(4 * kTL) k_rs_i(Level L, Fields F, const doubl
     }
 }
 
+// ---- grid ratios of the inflow faces (init_outlet_bc, init.F90:2750-2891) ----
+// Face b (0..3: i = 1, i = img, j = 1, j = jmg), node t of the face plane
+// (i faces: j + nj k, j faces: i + ni k): the spacing of the first node pair
+// off the face over that of the second, along coordinate dir (bdir - 1).
+__global__ void k_outlet_ratio(Level L, Fields F, int b, int dir) {
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long np = b < 2 ? (long)nj * nk : (long)ni * nk;
+    if (t >= np) return;
+    const long sJ = ni, sK = (long)ni * nj;
+    long l0, st;
+    if (b < 2) {
+        const long base = sJ * (t % nj) + sK * (t / nj);
+        l0 = b == 0 ? base : base + ni - 1;
+        st = b == 0 ? 1 : -1;
+    } else {
+        const long base = (t % ni) + sK * (t / ni);
+        l0 = b == 2 ? base : base + sJ * (nj - 1);
+        st = b == 2 ? sJ : -sJ;
+    }
+    const double* c = F.x[dir] + L.off;
+    const double c0 = c[l0], c1 = c[l0 + st], c2 = c[l0 + 2 * st];
+    F.rat[b][t] = (b % 2 == 0) ? (c1 - c0) / (c2 - c1) : (c0 - c1) / (c1 - c2);
+}
+
 // ---- bcond_fm faces of direction D (both sides) ----
+// Inflow (btype 4) on a k face uses ratio 0: the reference fills its table
+// only where btype == 4 .and. btype == 5 (init.F90:2855, :2874), i.e. never,
+// and the table is zero as allocated.
 template <int D>
 __global__ void k_bc_fm(Level L, Fields F, Consts C) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
@@ -343,6 +371,11 @@ __global__ void k_bc_fm(Level L, Fields F, Consts C) {
                 for (int m = 0; m < 4; m++) F.q[m][lo] = F.q[m][hi];
             }
             break;
+        case 4: {  // inflow: pressure extrapolated on the grid ratio (bcond_fm.F90:95-102)
+            const double r = D < 2 ? F.rat[b0][t] : 0.0;
+            F.q[0][lo] = (1.0 + r) * F.q[0][lo + sD] - r * F.q[0][lo + 2 * sD];
+            break;
+        }
         default: break;
     }
     switch (C.btype[b1]) {
@@ -356,6 +389,11 @@ __global__ void k_bc_fm(Level L, Fields F, Consts C) {
                 for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][lo];
             }
             break;
+        case 4: {
+            const double r = D < 2 ? F.rat[b1][t] : 0.0;
+            F.q[0][hi] = (1.0 + r) * F.q[0][hi - sD] - r * F.q[0][hi - 2 * sD];
+            break;
+        }
         case 8:
             if (D == 0) {
 #pragma unroll
@@ -484,6 +522,14 @@ void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Field
                        rs_tab + stride, C.ep[1][1]);
     hipLaunchKernelGGL(k_rs<2>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nj - 2), 128), 4), dim3(128), 0, s, Lf, F,
                        rs_tab + 2 * stride, C.ep[1][2]);
+}
+
+void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const int* btype, const int* bdir) {
+    for (int b = 0; b < 4; b++) {
+        if (btype[b] != DESALM_BC_INFLOW) continue;
+        const long np = b < 2 ? (long)L.nj * L.nk : (long)L.ni * L.nk;
+        hipLaunchKernelGGL(k_outlet_ratio, dim3(nblk(np, 256)), dim3(256), 0, s, L, F, b, bdir[b] - 1);
+    }
 }
 
 void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix, int jfix, int kfix) {

@@ -189,6 +189,10 @@ contains
     rc = desalm_set_metrics(desalm_f_ctx, real(csi(:, 1:nall), c_double), real(eta(:, 1:nall), c_double), &
                             real(zet(:, 1:nall), c_double), real(aj(1:nall), c_double))
     call desalm_f_check(rc, 'desalm_set_metrics')
+    ! coordinates for the inflow faces' pressure extrapolation and the case hooks
+    rc = desalm_set_coordinates(desalm_f_ctx, real(x(1:n1), c_double), real(y(1:n1), c_double), &
+                                real(z(1:n1), c_double))
+    call desalm_f_check(rc, 'desalm_set_coordinates')
     rc = desalm_set_wall_distance(desalm_f_ctx, real(wd(1:n1), c_double), real(dtev(1:n1), c_double))
     call desalm_f_check(rc, 'desalm_set_wall_distance')
   end subroutine desalm_f_setup

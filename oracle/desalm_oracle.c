@@ -55,6 +55,10 @@ struct or_state {
      * the j-wall wall functions; wf_newton selects the reference's own
      * Newton chain (1) or a root converged to rounding (0) */
     int hooks, wf_newton;
+    /* grid ratio of the inflow pressure extrapolation per face (module
+     * global_app's rat, init_outlet_bc, src/init.F90:2750-2891); NULL where
+     * the face is not an inflow */
+    double* rat[6];
     /* actuator-line model (module global_turb): deep copies of the tables */
     int n_tur, n_polar;
     desalm_turbine* tur;
@@ -228,6 +232,7 @@ void or_destroy(or_state* o) {
     free(o->q); free(o->qn); free(o->qnm1); free(o->qold_mg); free(o->rh); free(o->pk);
     free(o->xnut); free(o->wd); free(o->dtev);
     free(o->ohosts); free(o->ocoef); free(o->onodes);
+    for (int b = 0; b < 6; b++) free(o->rat[b]);
     or_set_turbines(o, 0, NULL, 0, NULL, 0.0, 0.0);
     free(o);
 }
@@ -373,6 +378,35 @@ static void mg_metrics(or_state* o) {
     }
 }
 
+/* init_outlet_bc (src/init.F90:2750-2891): on every inflow face b (btype 4;
+ * 5 is not supported) the ratio of the spacings of the first two node pairs
+ * off the face in direction bdir(b), for bcond_fm's pressure extrapolation.
+ * The k faces test btype == 4 .and. btype == 5 (:2855, :2874), which never
+ * holds: their table keeps its allocation contents, zero here. */
+static void init_outlet_bc(or_state* o) {
+    LEVEL(1);
+    for (int b = 0; b < 6; b++) {
+        free(o->rat[b]);
+        o->rat[b] = NULL;
+        if (o->p.btype[b] != 4) continue;
+        const int na = b < 2 ? nj : ni, nb = nk;  /* in-plane extents (k faces: ni, nj) */
+        const int n2 = b < 4 ? nb : nj;
+        o->rat[b] = (double*)calloc((size_t)na * n2, sizeof(double));
+        if (b >= 4) continue;
+        const double* c = o->p.bdir[b] == 1 ? o->x : o->p.bdir[b] == 2 ? o->y : o->z;
+        for (int k = 1; k <= nk; k++)
+            for (int a = 1; a <= na; a++) {
+                long l0, l1, l2;
+                if (b == 0) { l0 = IX(1, a, k); l1 = IX(2, a, k); l2 = IX(3, a, k); }
+                else if (b == 1) { l0 = IX(ni, a, k); l1 = IX(ni - 1, a, k); l2 = IX(ni - 2, a, k); }
+                else if (b == 2) { l0 = IX(a, 1, k); l1 = IX(a, 2, k); l2 = IX(a, 3, k); }
+                else { l0 = IX(a, nj, k); l1 = IX(a, nj - 1, k); l2 = IX(a, nj - 2, k); }
+                o->rat[b][(a - 1) + (long)na * (k - 1)] =
+                    (b % 2 == 0) ? (c[l1] - c[l0]) / (c[l2] - c[l1]) : (c[l0] - c[l1]) / (c[l1] - c[l2]);
+            }
+    }
+}
+
 int or_set_grid(or_state* o, const double* x, const double* y, const double* z) {
     long n1 = o->nl[1];
     memcpy(o->x, x, n1 * sizeof(double));
@@ -382,6 +416,7 @@ int or_set_grid(or_state* o, const double* x, const double* y, const double* z) 
     mg_inject(o, o->y, 1, 0);
     mg_inject(o, o->z, 1, 0);
     mg_metrics(o);
+    init_outlet_bc(o);
     return 0;
 }
 
@@ -448,6 +483,7 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
     int i, j, k, m, b;
     double pfix;
 #define QB(m, i, j, k) qv[5 * IX(i, j, k) + (m)-1]
+#define RAT(b, a, c, na) o->rat[b][((a)-1) + (long)(na) * ((c)-1)]
     /* i = 1 */
     b = 1;
     switch (o->p.btype[0]) {
@@ -456,6 +492,13 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                 for (j = 1; j <= nj; j++)
                     for (m = 1; m <= 4; m++)
                         QB(m, 1, j, k) = o->sa[b - 1][m - 1] * QB(m, 2, j, k) + o->sb[b - 1][m - 1] * QB(m, 3, j, k);
+            break;
+        case 4: /* inflow: pressure extrapolated on the grid ratio (:95-102) */
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++) {
+                    const double r = RAT(0, j, k, nj);
+                    QB(1, 1, j, k) = (1.0 + r) * QB(1, 2, j, k) - r * QB(1, 3, j, k);
+                }
             break;
         default: break; /* 0 interface, 6 periodic (commented out, :105-110) */
     }
@@ -467,6 +510,13 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                 for (j = 1; j <= nj; j++)
                     for (m = 1; m <= 4; m++)
                         QB(m, ni, j, k) = o->sa[b - 1][m - 1] * QB(m, ni - 1, j, k) + o->sb[b - 1][m - 1] * QB(m, ni - 2, j, k);
+            break;
+        case 4:
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++) {
+                    const double r = RAT(1, j, k, nj);
+                    QB(1, ni, j, k) = (1.0 + r) * QB(1, ni - 1, j, k) - r * QB(1, ni - 2, j, k);
+                }
             break;
         case 8: /* free surface (:160-166) */
             for (k = 1; k <= nk; k++)
@@ -484,6 +534,13 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                     for (m = 1; m <= 4; m++)
                         QB(m, i, 1, k) = o->sa[b - 1][m - 1] * QB(m, i, 2, k) + o->sb[b - 1][m - 1] * QB(m, i, 3, k);
             break;
+        case 4:
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++) {
+                    const double r = RAT(2, i, k, ni);
+                    QB(1, i, 1, k) = (1.0 + r) * QB(1, i, 2, k) - r * QB(1, i, 3, k);
+                }
+            break;
         case 6: /* plane copy from j = jmax (:197-202) */
             for (k = 1; k <= nk; k++)
                 for (i = 1; i <= ni; i++)
@@ -499,6 +556,13 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                 for (i = 1; i <= ni; i++)
                     for (m = 1; m <= 4; m++)
                         QB(m, i, nj, k) = o->sa[b - 1][m - 1] * QB(m, i, nj - 1, k) + o->sb[b - 1][m - 1] * QB(m, i, nj - 2, k);
+            break;
+        case 4:
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++) {
+                    const double r = RAT(3, i, k, ni);
+                    QB(1, i, nj, k) = (1.0 + r) * QB(1, i, nj - 1, k) - r * QB(1, i, nj - 2, k);
+                }
             break;
         case 6:
             for (k = 1; k <= nk; k++)
@@ -516,6 +580,13 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                     for (m = 1; m <= 4; m++)
                         QB(m, i, j, 1) = o->sa[b - 1][m - 1] * QB(m, i, j, 2) + o->sb[b - 1][m - 1] * QB(m, i, j, 3);
             break;
+        case 4:
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    const double r = RAT(4, i, j, ni);
+                    QB(1, i, j, 1) = (1.0 + r) * QB(1, i, j, 2) - r * QB(1, i, j, 3);
+                }
+            break;
         case 6:
             for (j = 1; j <= nj; j++)
                 for (i = 1; i <= ni; i++)
@@ -531,6 +602,13 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                 for (i = 1; i <= ni; i++)
                     for (m = 1; m <= 4; m++)
                         QB(m, i, j, nk) = o->sa[b - 1][m - 1] * QB(m, i, j, nk - 1) + o->sb[b - 1][m - 1] * QB(m, i, j, nk - 2);
+            break;
+        case 4:
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++) {
+                    const double r = RAT(5, i, j, ni);
+                    QB(1, i, j, nk) = (1.0 + r) * QB(1, i, j, nk - 1) - r * QB(1, i, j, nk - 2);
+                }
             break;
         case 6:
             for (j = 1; j <= nj; j++)
@@ -556,6 +634,7 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
         }
     }
 #undef QB
+#undef RAT
 }
 
 /* ------------------------------------------------------------------ */

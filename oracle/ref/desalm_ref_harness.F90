@@ -183,6 +183,7 @@ program desalm_ref_harness
   call mg_inject(y)
   call mg_inject(z)
   call mg_metrics
+  call setup_rat()
 
   n = 1
   la = le_idx_a(1)
@@ -392,6 +393,49 @@ contains
   end subroutine setup_index_maps
 
   ! allocations of init_mg (init.F90:1025-1124)
+  ! The grid ratio of the pressure extrapolation on inflow faces (btype 4),
+  ! as init_outlet_bc fills module global_app's rat after init_grid
+  ! (src/init.F90:2750-2891): on face b the spacing of the first two node
+  ! pairs off the face in direction bdir(b).  Its k faces test btype == 4
+  ! .and. btype == 5 (:2855, :2874), which never holds, so they stay as
+  ! allocated; here they are zero.
+  subroutine setup_rat()
+    integer :: ijkmax, b, i, j, k, l0, l1, l2, d
+    real (kind = rdf) :: c0, c1, c2
+    ijkmax = max(img(1,1), jmg(1,1), kmg(1,1))
+    allocate (rat(ijkmax, ijkmax, 6))
+    rat = zero
+    do b = 1, 4
+       if (btype(b,1) /= 4 .and. btype(b,1) /= 5) cycle
+       d = bdir(b,1)
+       do k = 1, kmg(1,1)
+          do j = 1, merge(jmg(1,1), img(1,1), b <= 2)
+             if (b == 1) then
+                l0 = le_idx(1,j,k,1); l1 = le_idx(2,j,k,1); l2 = le_idx(3,j,k,1)
+             else if (b == 2) then
+                l0 = le_idx(img(1,1),j,k,1); l1 = le_idx(img(1,1)-1,j,k,1); l2 = le_idx(img(1,1)-2,j,k,1)
+             else if (b == 3) then
+                l0 = le_idx(j,1,k,1); l1 = le_idx(j,2,k,1); l2 = le_idx(j,3,k,1)
+             else
+                l0 = le_idx(j,jmg(1,1),k,1); l1 = le_idx(j,jmg(1,1)-1,k,1); l2 = le_idx(j,jmg(1,1)-2,k,1)
+             end if
+             if (d == 1) then
+                c0 = x(l0); c1 = x(l1); c2 = x(l2)
+             else if (d == 2) then
+                c0 = y(l0); c1 = y(l1); c2 = y(l2)
+             else
+                c0 = z(l0); c1 = z(l1); c2 = z(l2)
+             end if
+             if (mod(b, 2) == 1) then
+                rat(j,k,b) = (c1 - c0) / (c2 - c1)
+             else
+                rat(j,k,b) = (c0 - c1) / (c1 - c2)
+             end if
+          end do
+       end do
+    end do
+  end subroutine setup_rat
+
   subroutine setup_mg_arrays()
     integer :: la0, lb0, pad
     la0 = le_idx_a(1)

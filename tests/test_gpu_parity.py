@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from desalm import cases
-from desalm.lib import Solver, gtsv_batch
+from desalm.lib import DesalmError, Solver, gtsv_batch
 from desalm.params import PREC_MIXED
 import pyoracle as po
 
@@ -128,7 +128,9 @@ def check_des_eddy(c):
 
 
 @pytest.mark.parametrize("ng,des,bt,itm", [(2, False, None, 4), (3, True, None, 3),
-                                          (3, True, (6, 6, 1, 1, 6, 6), 3)])
+                                          (3, True, (6, 6, 1, 1, 6, 6), 3),
+                                          # inflow faces: pressure extrapolated on the grid ratio
+                                          (2, True, (4, 3, 4, 1, 4, 2), 3), (3, True, (3, 4, 1, 4, 2, 4), 3)])
 def test_mg_driver(gpu, ng, des, bt, itm):
     c = cases.wavy(17, 17, 17, ng=ng, itm=itm, des=des, btype=bt, iters=(1, 2, 3))
     o = po.Oracle(c.params).load_case(c)
@@ -329,3 +331,21 @@ def test_tile_shapes_do_not_change_results(gpu):
     for st in out[1:]:
         for a, b in zip(out[0], st):
             assert np.allclose(a, b, rtol=1e-12, atol=1e-14 * np.abs(a).max())
+
+
+def test_inflow_needs_coordinates(gpu):
+    c = cases.wavy(17, 17, 17, ng=2, itm=1, btype=(4, 3, 1, 1, 2, 2))
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params)
+    s.set_metrics(*[o.get(nm) for nm in ("csi", "eta", "zet", "aj")])
+    s.set_wall_distance(c.wd)
+    s.set_state(*c.state())
+    with pytest.raises(DesalmError, match="btype 4"):
+        s.mg_driver()
+    # coordinates alone (metrics kept) make it run as with desalm_set_grid
+    s.set_coordinates(c.x, c.y, c.z)
+    s.mg_driver()
+    o.mg_driver()
+    for m in range(5):
+        assert rel(s.get_state()[0][:, m], o.get_state()[0][:, m]) < TOL, m
+    s.close()
