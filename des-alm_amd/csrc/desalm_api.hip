@@ -156,13 +156,15 @@ int validate(const desalm_params* p, std::string& why) {
         }
     }
     for (int b = 0; b < 6; b++)
-        if (p->btype[b] == DESALM_BC_OUTLET_MOC || p->btype[b] < 0 || p->btype[b] > 8) {
-            why = "unsupported boundary type (characteristic outlet, btype 5, is out of scope)";
+        if (p->btype[b] < 0 || p->btype[b] > 8 || p->btype[b] == 7) {
+            why = "unsupported boundary type (btype 0-6 or 8)";
             return -1;
         }
     for (int b = 0; b < 4; b++)
-        if (p->btype[b] == DESALM_BC_INFLOW && (p->bdir[b] < 1 || p->bdir[b] > 3)) {
-            why = "an inflow face (btype 4) needs bdir 1, 2 or 3: its pressure extrapolation uses that coordinate";
+        if ((p->btype[b] == DESALM_BC_INFLOW || p->btype[b] == DESALM_BC_OUTLET_MOC) &&
+            (p->bdir[b] < 1 || p->bdir[b] > 3)) {
+            why = "an inflow or characteristic-outlet face (btype 4, 5) needs bdir 1, 2 or 3: its extrapolation "
+                  "uses that coordinate";
             return -1;
         }
     if (p->ifix < 1 || p->ifix > p->img || p->jfix < 1 || p->jfix > p->jmg || p->kfix < 1 || p->kfix > p->kmg) {
@@ -244,6 +246,9 @@ void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out) {
     hipStream_t s = c->stream;
     timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C); });
     timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk, c->kc_visc[n]); });
+    // characteristic-outlet planes of the fine level (solver_daf.F90:322-365)
+    if (n == 1 && (c->C.btype[0] == DESALM_BC_OUTLET_MOC || c->C.btype[1] == DESALM_BC_OUTLET_MOC))
+        timed(c, KID_RESIDUAL, n, [&] { launch_outlet_rhs(s, L, c->F, c->C); });
     // actuator-line forcing on every level (solver_daf.F90:378-381)
     if (!c->alm.t.empty()) timed(c, KID_ALM, n, [&] { launch_alm_rhs(s, L, c->F, c->alm, c->C.ren); });
     timed(c, KID_DAF_I, n, [&] { launch_daf_i(s, L, c->F, c->C, n); });
@@ -367,7 +372,9 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     F.sd = A(c->n1);
     {
         const Level& L1 = c->lv[1];
-        long ml = std::max((long)(L1.nj - 2) * (L1.nk - 2), std::max((long)(L1.ni - 2) * (L1.nk - 2), (long)(L1.ni - 2) * (L1.nj - 2)));
+        // lines of the fine sweeps; the j and k sweeps of a characteristic
+        // outlet also run on the plane i = ni - 1
+        long ml = std::max((long)(L1.nj - 2) * (L1.nk - 2), std::max((long)(L1.ni - 1) * (L1.nk - 2), (long)(L1.ni - 1) * (L1.nj - 2)));
         F.brh = A(8 * ml);
     }
     for (int m = 0; m < 3; m++) F.av[m] = A(c->n1);
@@ -386,7 +393,9 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     F.pfix = A(1);
     for (int b = 0; b < 4; b++) {
         const Level& L1 = c->lv[1];
-        F.rat[b] = p->btype[b] == DESALM_BC_INFLOW ? A(b < 2 ? (long)L1.nj * L1.nk : (long)L1.ni * L1.nk) : nullptr;
+        F.rat[b] = (p->btype[b] == DESALM_BC_INFLOW || p->btype[b] == DESALM_BC_OUTLET_MOC)
+                       ? A(b < 2 ? (long)L1.nj * L1.nk : (long)L1.ni * L1.nk)
+                       : nullptr;
     }
     {
         void* pv = nullptr;
@@ -618,7 +627,7 @@ int ws_start(desalm_ctx* c, hipStream_t s) {
 extern "C++" {
 static bool needs_coordinates(const desalm_ctx* c) {
     for (int b = 0; b < 4; b++)
-        if (c->p.btype[b] == DESALM_BC_INFLOW) return true;
+        if (c->p.btype[b] == DESALM_BC_INFLOW || c->p.btype[b] == DESALM_BC_OUTLET_MOC) return true;
     return false;
 }
 
@@ -631,8 +640,8 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     const int ng = c0->ng;
     for (desalm_ctx* c : zs)
         if (needs_coordinates(c) && !c->have_grid)
-            return fail(c0, "an inflow face (btype 4) extrapolates the pressure on the grid ratio: call "
-                            "desalm_set_grid or desalm_set_coordinates first");
+            return fail(c0, "an inflow or characteristic-outlet face (btype 4, 5) extrapolates on the grid ratio: "
+                            "call desalm_set_grid or desalm_set_coordinates first");
     for (desalm_ctx* c : zs) {
         const Level& L1 = c->lv[1];
         HIPCHK(c0, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));

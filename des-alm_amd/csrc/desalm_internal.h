@@ -91,9 +91,9 @@ struct Fields {
     double* resid_part;  // [kResidBlocks * 5] partial L1 sums
     double* resid;       // [5] L1 sums
     double* pfix;        // [1] pinned pressure value
-    // inflow faces (btype 4) of the i and j directions: grid ratio of the
-    // pressure extrapolation per face node (null where the face is not an
-    // inflow; the k faces use 0, see k_bc_fm)
+    // inflow (btype 4) and characteristic-outlet (btype 5) faces of the i and
+    // j directions: grid ratio of the extrapolation per face node (null where
+    // the face is neither; the k faces use 0, see k_bc_fm)
     double* rat[4];
     unsigned long long* pivots;  // [1] rows where LAPACK SGTSV would pivot
 };
@@ -197,6 +197,9 @@ void launch_daf_k(hipStream_t s, const Level& L, const Fields& F, const Consts& 
                   double* dq_out /* nullable: also store Mc x here (4 SoA, level-local) */);
 // fine-grid BCs, k_mg.hip
 void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const int* btype, const int* bdir);
+// right-hand side of the characteristic-outlet planes (btype 5 on i faces,
+// fine level): mg_brhs + nonreflect_ibc (k_outlet.hip)
+void launch_outlet_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
 void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix,
                      int jfix, int kfix);
 // SA / DES, k_sa.hip

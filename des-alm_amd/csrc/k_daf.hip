@@ -148,14 +148,15 @@ struct DafBuf {
 // from the untransformed rh, into F.brh[line * 8 + side * 4 + m].
 template <int D, typename S>
 __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2,
-                                                 int b2_full) {
+                                                 int b2_full, int ext) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
     const long g = L.off + t;
     B.sd[t] = (S)(F.dtau[g] * spr_at<D>(F, C, g));
-    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+    // ext: the characteristic-outlet plane i = ni - 1 is solved as well
+    if (i == 0 || i >= ni - 1 + ext || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
     const long sI = 1, sJ = ni, sK = (long)ni * nj;
     const double alpha_mult = C.e_source * 1.5 / C.delti;
     // S (-dtau J rh) at a node (:56-72)
@@ -188,8 +189,8 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, Da
     const int nd = D == 0 ? ni : D == 1 ? nj : nk;
     const long sD = D == 0 ? sI : D == 1 ? sJ : sK;
     const long line = D == 0 ? (long)(j - 1) + (long)(nj - 2) * (k - 1)
-                    : D == 1 ? (long)(i - 1) + (long)(ni - 2) * (k - 1)
-                             : (long)(i - 1) + (long)(ni - 2) * (j - 1);
+                    : D == 1 ? (long)(i - 1) + (long)(ni - 2 + ext) * (k - 1)
+                             : (long)(i - 1) + (long)(ni - 2 + ext) * (j - 1);
     double* brh = F.brh + line * 8;
     const int b0 = 2 * D, b1 = 2 * D + 1;
     if (p == 1 && use_b1) {
@@ -247,8 +248,10 @@ __global__ void __launch_bounds__(256) k_daf_pre(Level L, Fields F, Consts C, Da
 constexpr int kPF = 16;
 
 template <int D, typename S>
-__global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2) {
+__global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1, int use_b2,
+                                                 int ext) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const int nI = ni - 2 + ext;  // i positions of the j and k lines (ext: the outlet plane too)
     const int m = blockIdx.y;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long nlines, base, sD;
@@ -260,15 +263,15 @@ __global__ void __launch_bounds__(128) k_daf_tri(Level L, Fields F, Consts C, Da
         sD = 1;
         len = ni - 2;
     } else if (D == 1) {
-        nlines = (long)(ni - 2) * (nk - 2);
+        nlines = (long)nI * (nk - 2);
         if (t >= nlines) return;
-        base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
+        base = (1 + t % nI) + (long)ni * nj * (1 + t / nI);
         sD = ni;
         len = nj - 2;
     } else {
-        nlines = (long)(ni - 2) * (nj - 2);
+        nlines = (long)nI * (nj - 2);
         if (t >= nlines) return;
-        base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
+        base = (1 + t % nI) + (long)ni * (1 + t / nI);
         sD = (long)ni * nj;
         len = nk - 2;
     }
@@ -508,7 +511,7 @@ __global__ void __launch_bounds__(128) k_daf_tri2(Level L, Fields F, Consts C, D
 // k_daf_tri<0>.
 template <typename S>
 __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts C, DafBuf<S> B, int use_b1,
-                                                       int use_b2) {
+                                                       int use_b2, int ext) {
     constexpr int NT = 4 * kTL;
     constexpr int kSS = kTR + 3;  // LDS stride of the sd tile (kTR + 2 rows, odd)
     using IOr = TileIO<NT, kTL, kTR, kTS, S>;
@@ -527,7 +530,9 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
     if (threadIdx.x < kTL) s_base[threadIdx.x] = act ? iline_base(L, t) : 0;
     __syncthreads();
 
-    const int len = L.ni - 2;
+    // ext: the line closes on the characteristic-outlet plane i = ni - 1
+    const int len = L.ni - 2 + ext;
+    const int nsd = L.ni;  // sd rows loaded: nodes i = 0 .. ni - 1 of the line
     const int nch = (len + kTR - 1) / kTR;
     const S lam = m == 2 ? S(1) : m == 3 ? S(-1) : S(0);
     const S h = (S)L.dc;
@@ -542,7 +547,7 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
     // row s of a line is node base + 1 + s; sd of rows s-1..s+1 are nodes base + s .. base + s + 2
     S vd[4][IOr::kPer], vs[IOs::kPer];
     for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], B.d[mm] + 1, s_base, nl, 0, len);
-    IOs::load(vs, sd, s_base, nl, 0, len + 2);
+    IOs::load(vs, sd, s_base, nl, 0, nsd);
 
     S cprev = 0, dprev = 0, denprev = 1;
     unsigned piv = 0;
@@ -554,7 +559,7 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
         __syncthreads();
         if (ch + 1 < nch) {
             for (int mm = 0; mm < 4; mm++) IOr::load(vd[mm], B.d[mm] + 1, s_base, nl, s0 + kTR, len);
-            IOs::load(vs, sd, s_base, nl, s0 + kTR, len + 2);
+            IOs::load(vs, sd, s_base, nl, s0 + kTR, nsd);
         }
         if (act) {
             for (int u = 0; u < kTR; u++) {
@@ -564,9 +569,16 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
                 const S tmp = sd0 * epd * hsq;
                 const S dtc = (sd0 == S(0)) ? S(0) : S(1);
                 const S ddw = dtc * sdm * h2, dde = dtc * sdp * h2;
-                const S a = -ddw * lam - tmp;
-                const S c = dde * lam - tmp;
-                const S b = S(1) + S(2) * tmp;
+                S a = -ddw * lam - tmp;
+                S c = dde * lam - tmp;
+                S b = S(1) + S(2) * tmp;
+                if (ext && s == len - 1) {
+                    // outlet row (rhs_diag_solver.F90:207-221): upwind coupling
+                    // of the +1 eigenvalue only, no smoothing
+                    a = m == 2 ? -(sdm * h) : S(0);
+                    b = m == 2 ? S(1) + sd0 * h : S(1);
+                    c = S(0);
+                }
                 S d = myd[u];
                 if (s == 0 && use_b1) d = d - a * brh1;
                 if (s == len - 1 && use_b2) d = d - c * brh2;
@@ -629,12 +641,12 @@ __global__ void __launch_bounds__(4 * kTL) k_daf_tri_i(Level L, Fields F, Consts
 // dQ = Mc dQ*** and q = q + dQ on interior nodes (rhs_diag_solver.F90:624-633,
 // solver_daf.F90:409-418)
 template <typename S>
-__global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, DafBuf<S> B, double* dq_out) {
+__global__ void __launch_bounds__(256) k_daf_post(Level L, Fields F, Consts C, DafBuf<S> B, double* dq_out, int ext) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
-    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+    if (i == 0 || i >= ni - 1 + ext || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
     const long g = L.off + t;
     double v[4];
 #pragma unroll
@@ -719,36 +731,38 @@ static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Cons
     const int use_b1 = (n == 1 && C.btype[2 * D] != 5);
     const int use_b2 = (n == 1 && C.btype[2 * D + 1] != 5);
     const int b2_full = (n == 1 && C.btype[5] != 6);
+    // a characteristic outlet at i = ni - 1 joins the fine level's sweeps
+    const int ext = (n == 1 && C.btype[1] == DESALM_BC_OUTLET_MOC);
     const unsigned gp = (unsigned)((L.n + 255) / 256);
-    const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
-                                                                    : (long)(L.ni - 2) * (L.nj - 2);
+    const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2 + ext) * (L.nk - 2)
+                                                                    : (long)(L.ni - 2 + ext) * (L.nj - 2);
     const dim3 gt((unsigned)((nl + 127) / 128), 4);
     const int len = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
     if (n > 1 && len >= 4) {
         // coarse levels: the two-way sweep
         const dim3 g2((unsigned)((nl + 63) / 64), 4);
         if (D == 0) {
-            hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+            hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full, ext);
             hipLaunchKernelGGL((k_daf_tri2<0, S>), g2, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
         } else if (D == 1) {
-            hipLaunchKernelGGL((k_daf_pre<1, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+            hipLaunchKernelGGL((k_daf_pre<1, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full, ext);
             hipLaunchKernelGGL((k_daf_tri2<1, S>), g2, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
         } else {
-            hipLaunchKernelGGL((k_daf_pre<2, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+            hipLaunchKernelGGL((k_daf_pre<2, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full, ext);
             hipLaunchKernelGGL((k_daf_tri2<2, S>), g2, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
         }
         return;
     }
     if (D == 0) {
-        hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
+        hipLaunchKernelGGL((k_daf_pre<0, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full, ext);
         const unsigned gi = (unsigned)((nl + kTL - 1) / kTL);
-        hipLaunchKernelGGL(k_daf_tri_i<S>, dim3(gi), dim3(4 * kTL), 0, s, L, F, C, B, use_b1, use_b2);
+        hipLaunchKernelGGL(k_daf_tri_i<S>, dim3(gi), dim3(4 * kTL), 0, s, L, F, C, B, use_b1, use_b2, ext);
     } else if (D == 1) {
-        hipLaunchKernelGGL((k_daf_pre<1, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
-        hipLaunchKernelGGL((k_daf_tri<1, S>), gt, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
+        hipLaunchKernelGGL((k_daf_pre<1, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full, ext);
+        hipLaunchKernelGGL((k_daf_tri<1, S>), gt, dim3(128), 0, s, L, F, C, B, use_b1, use_b2, ext);
     } else {
-        hipLaunchKernelGGL((k_daf_pre<2, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full);
-        hipLaunchKernelGGL((k_daf_tri<2, S>), gt, dim3(128), 0, s, L, F, C, B, use_b1, use_b2);
+        hipLaunchKernelGGL((k_daf_pre<2, S>), dim3(gp), dim3(256), 0, s, L, F, C, B, use_b1, use_b2, b2_full, ext);
+        hipLaunchKernelGGL((k_daf_tri<2, S>), gt, dim3(128), 0, s, L, F, C, B, use_b1, use_b2, ext);
     }
 }
 
@@ -786,8 +800,9 @@ void launch_daf_j(hipStream_t s, const Level& L, const Fields& F, const Consts& 
 void launch_daf_k(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, double* dq_out) {
     daf_dir(s, L, F, C, n, 2);
     const dim3 g((unsigned)((L.n + 255) / 256));
-    if (C.mixed) hipLaunchKernelGGL(k_daf_post<float>, g, dim3(256), 0, s, L, F, C, buf32(F), dq_out);
-    else hipLaunchKernelGGL(k_daf_post<double>, g, dim3(256), 0, s, L, F, C, buf64(L, F), dq_out);
+    const int ext = (n == 1 && C.btype[1] == DESALM_BC_OUTLET_MOC);
+    if (C.mixed) hipLaunchKernelGGL(k_daf_post<float>, g, dim3(256), 0, s, L, F, C, buf32(F), dq_out, ext);
+    else hipLaunchKernelGGL(k_daf_post<double>, g, dim3(256), 0, s, L, F, C, buf64(L, F), dq_out, ext);
 }
 
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du, double* b, int* info) {

@@ -526,7 +526,7 @@ void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Field
 
 void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const int* btype, const int* bdir) {
     for (int b = 0; b < 4; b++) {
-        if (btype[b] != DESALM_BC_INFLOW) continue;
+        if (btype[b] != DESALM_BC_INFLOW && btype[b] != DESALM_BC_OUTLET_MOC) continue;
         const long np = b < 2 ? (long)L.nj * L.nk : (long)L.ni * L.nk;
         hipLaunchKernelGGL(k_outlet_ratio, dim3(nblk(np, 256)), dim3(256), 0, s, L, F, b, bdir[b] - 1);
     }

@@ -40,7 +40,7 @@ extern "C" {
 #define DESALM_BC_SYMMETRY 2
 #define DESALM_BC_FREESTREAM 3
 #define DESALM_BC_INFLOW 4
-#define DESALM_BC_OUTLET_MOC 5   /* not supported: rejected by desalm_init */
+#define DESALM_BC_OUTLET_MOC 5   /* characteristic outlet (mg_brhs + nonreflect_ibc on i faces) */
 #define DESALM_BC_PERIODIC 6
 #define DESALM_BC_FREE_SURFACE 8
 

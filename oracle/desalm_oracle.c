@@ -173,7 +173,7 @@ or_state* or_create(const desalm_params* p) {
     or_state* o;
     if (!p || p->ng < 1 || p->ng > DESALM_MAX_LEVELS) return NULL;
     for (b = 0; b < 6; b++)
-        if (p->btype[b] == DESALM_BC_OUTLET_MOC) return NULL;
+        if (p->btype[b] < 0 || p->btype[b] > 8) return NULL;
     o = (or_state*)calloc(1, sizeof(or_state));
     o->p = *p;
     o->ng = p->ng;
@@ -378,8 +378,8 @@ static void mg_metrics(or_state* o) {
     }
 }
 
-/* init_outlet_bc (src/init.F90:2750-2891): on every inflow face b (btype 4;
- * 5 is not supported) the ratio of the spacings of the first two node pairs
+/* init_outlet_bc (src/init.F90:2750-2891): on every inflow or
+ * characteristic-outlet face b (btype 4 or 5) the ratio of the spacings of the first two node pairs
  * off the face in direction bdir(b), for bcond_fm's pressure extrapolation.
  * The k faces test btype == 4 .and. btype == 5 (:2855, :2874), which never
  * holds: their table keeps its allocation contents, zero here. */
@@ -388,7 +388,7 @@ static void init_outlet_bc(or_state* o) {
     for (int b = 0; b < 6; b++) {
         free(o->rat[b]);
         o->rat[b] = NULL;
-        if (o->p.btype[b] != 4) continue;
+        if (o->p.btype[b] != 4 && o->p.btype[b] != 5) continue;
         const int na = b < 2 ? nj : ni, nb = nk;  /* in-plane extents (k faces: ni, nj) */
         const int n2 = b < 4 ? nb : nj;
         o->rat[b] = (double*)calloc((size_t)na * n2, sizeof(double));
@@ -1055,6 +1055,251 @@ static void zero_rh_boundaries(daf_t* d) {
             for (m = 1; m <= 4; m++) RH(m, i, j, d->ks - 1) = 0.0, RH(m, i, j, d->ke + 1) = 0.0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Characteristic outlet (btype 5) of an i face on the fine level:      */
+/* mg_brhs (src/mg_brhs.F90:2-108 with brhs_viscous, brhs_diss_p,        */
+/* brhs_flux_sans_convec, brhs_convec_quick(2), brhs_unst_visc_diss)     */
+/* builds rh on the boundary plane from its eta and zeta derivatives    */
+/* only; nonreflect_ibc (src/nonreflect_ibc.F90:1-244) then adds the     */
+/* i-direction terms as characteristic amplitudes, the incoming ones    */
+/* set to zero.  ip: the plane (1 or ni); inout: 1 or 2.                */
+/* ------------------------------------------------------------------ */
+static void mg_brhs_plane(daf_t* d, int ip) {
+    DLV;
+    const or_state* o = d->o;
+    const double de = d->de, dz = d->dz;
+    const int js = 2, je = nj - 1, ks = 2, ke = nk - 1;
+    const long np = (long)nj * nk;
+    double* visc = zalloc(3 * np);
+    double* diss = zalloc(4 * np);
+    double* fv = zalloc(3 * np);
+    double* ret = zalloc(np);
+    double* aux = zalloc(2 * np);
+    double* pp = zalloc(np);
+    double* up = zalloc(np);
+    double* um = zalloc(np);
+    double* rd = zalloc(3 * np);
+#define P2(j, k) ((long)((j)-1) + (long)nj * ((k)-1))
+#define PQ(m, j, k) Q(m, ip, j, k)
+    int j, k, c;
+    /* brhs_viscous */
+    const double rei = 1.0 / o->p.ren;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++) ret[P2(j, k)] = rei + XNUT(ip, j, k);
+    const double de4 = pt25 * de, dz4 = pt25 * dz;
+    for (k = ks; k <= ke; k++)
+        for (j = js - 1; j <= je; j++) {
+            const double uet = de * (PQ(2, j + 1, k) - PQ(2, j, k));
+            const double vet = de * (PQ(3, j + 1, k) - PQ(3, j, k));
+            const double wet = de * (PQ(4, j + 1, k) - PQ(4, j, k));
+            const double uze = dz4 * (PQ(2, j + 1, k + 1) - PQ(2, j + 1, k - 1) + PQ(2, j, k + 1) - PQ(2, j, k - 1));
+            const double vze = dz4 * (PQ(3, j + 1, k + 1) - PQ(3, j + 1, k - 1) + PQ(3, j, k + 1) - PQ(3, j, k - 1));
+            const double wze = dz4 * (PQ(4, j + 1, k + 1) - PQ(4, j + 1, k - 1) + PQ(4, j, k + 1) - PQ(4, j, k - 1));
+            const double b1 = ETA(1, ip, j + 1, k) + ETA(1, ip, j, k);
+            const double b2 = ETA(2, ip, j + 1, k) + ETA(2, ip, j, k);
+            const double b3 = ETA(3, ip, j + 1, k) + ETA(3, ip, j, k);
+            const double c1 = ZET(1, ip, j + 1, k) + ZET(1, ip, j, k);
+            const double c2 = ZET(2, ip, j + 1, k) + ZET(2, ip, j, k);
+            const double c3 = ZET(3, ip, j + 1, k) + ZET(3, ip, j, k);
+            const double uu1 = b1 * uet + c1 * uze, vv1 = b1 * vet + c1 * vze, ww1 = b1 * wet + c1 * wze;
+            const double uu2 = b2 * uet + c2 * uze, vv2 = b2 * vet + c2 * vze, ww2 = b2 * wet + c2 * wze;
+            const double uu3 = b3 * uet + c3 * uze, vv3 = b3 * vet + c3 * vze, ww3 = b3 * wet + c3 * wze;
+            const double c22 = b1 * b1 + b2 * b2 + b3 * b3;
+            const double c23 = b1 * c1 + b2 * c2 + b3 * c3;
+            const double vtj = one_fourth * (ret[P2(j + 1, k)] + ret[P2(j, k)]) / (AJ(ip, j + 1, k) + AJ(ip, j, k));
+            fv[3 * P2(j, k) + 0] = vtj * (b1 * uu1 + b2 * vv1 + b3 * ww1 + c22 * uet + c23 * uze);
+            fv[3 * P2(j, k) + 1] = vtj * (b1 * uu2 + b2 * vv2 + b3 * ww2 + c22 * vet + c23 * vze);
+            fv[3 * P2(j, k) + 2] = vtj * (b1 * uu3 + b2 * vv3 + b3 * ww3 + c22 * wet + c23 * wze);
+        }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (c = 0; c < 3; c++) visc[3 * P2(j, k) + c] = de * (fv[3 * P2(j, k) + c] - fv[3 * P2(j - 1, k) + c]);
+    for (k = ks - 1; k <= ke; k++)
+        for (j = js; j <= je; j++) {
+            const double uet = de4 * (PQ(2, j + 1, k + 1) - PQ(2, j - 1, k + 1) + PQ(2, j + 1, k) - PQ(2, j - 1, k));
+            const double vet = de4 * (PQ(3, j + 1, k + 1) - PQ(3, j - 1, k + 1) + PQ(3, j + 1, k) - PQ(3, j - 1, k));
+            const double wet = de4 * (PQ(4, j + 1, k + 1) - PQ(4, j - 1, k + 1) + PQ(4, j + 1, k) - PQ(4, j - 1, k));
+            const double uze = dz * (PQ(2, j, k + 1) - PQ(2, j, k));
+            const double vze = dz * (PQ(3, j, k + 1) - PQ(3, j, k));
+            const double wze = dz * (PQ(4, j, k + 1) - PQ(4, j, k));
+            const double b1 = ETA(1, ip, j, k + 1) + ETA(1, ip, j, k);
+            const double b2 = ETA(2, ip, j, k + 1) + ETA(2, ip, j, k);
+            const double b3 = ETA(3, ip, j, k + 1) + ETA(3, ip, j, k);
+            const double c1 = ZET(1, ip, j, k + 1) + ZET(1, ip, j, k);
+            const double c2 = ZET(2, ip, j, k + 1) + ZET(2, ip, j, k);
+            const double c3 = ZET(3, ip, j, k + 1) + ZET(3, ip, j, k);
+            const double uu1 = b1 * uet + c1 * uze, vv1 = b1 * vet + c1 * vze, ww1 = b1 * wet + c1 * wze;
+            const double uu2 = b2 * uet + c2 * uze, vv2 = b2 * vet + c2 * vze, ww2 = b2 * wet + c2 * wze;
+            const double uu3 = b3 * uet + c3 * uze, vv3 = b3 * vet + c3 * vze, ww3 = b3 * wet + c3 * wze;
+            const double c32 = c1 * b1 + c2 * b2 + c3 * b3;
+            const double c33 = c1 * c1 + c2 * c2 + c3 * c3;
+            const double vtj = one_fourth * (ret[P2(j, k + 1)] + ret[P2(j, k)]) / (AJ(ip, j, k + 1) + AJ(ip, j, k));
+            fv[3 * P2(j, k) + 0] = vtj * (c1 * uu1 + c2 * vv1 + c3 * ww1 + c32 * uet + c33 * uze);
+            fv[3 * P2(j, k) + 1] = vtj * (c1 * uu2 + c2 * vv2 + c3 * ww2 + c32 * vet + c33 * vze);
+            fv[3 * P2(j, k) + 2] = vtj * (c1 * uu3 + c2 * vv3 + c3 * ww3 + c32 * wet + c33 * wze);
+        }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (c = 0; c < 3; c++)
+                visc[3 * P2(j, k) + c] = visc[3 * P2(j, k) + c] + dz * (fv[3 * P2(j, k) + c] - fv[3 * P2(j, k - 1) + c]);
+    /* brhs_diss_p */
+    const double desq = de * de, dzsq = dz * dz;
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++) {
+            const double g2 = ETA(1, ip, j, k) * ETA(1, ip, j, k) + ETA(2, ip, j, k) * ETA(2, ip, j, k) +
+                              ETA(3, ip, j, k) * ETA(3, ip, j, k);
+            const double tmp1 = g2 * DTAU(ip, j, k) / AJ(ip, j, k);
+            const double tmp2 = desq * (PQ(1, j + 1, k) - 2.0 * PQ(1, j, k) + PQ(1, j - 1, k));
+            aux[2 * P2(j, k) + 0] = tmp1 * tmp2;
+        }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++) {
+            const double g3 = ZET(1, ip, j, k) * ZET(1, ip, j, k) + ZET(2, ip, j, k) * ZET(2, ip, j, k) +
+                              ZET(3, ip, j, k) * ZET(3, ip, j, k);
+            const double tmp1 = g3 * DTAU(ip, j, k) / AJ(ip, j, k);
+            const double tmp2 = dzsq * (PQ(1, j, k + 1) - 2.0 * PQ(1, j, k) + PQ(1, j, k - 1));
+            aux[2 * P2(j, k) + 1] = tmp1 * tmp2;
+        }
+    for (k = 1; k <= nk; k++) {
+        aux[2 * P2(js - 1, k)] = aux[2 * P2(js, k)];
+        aux[2 * P2(je + 1, k)] = aux[2 * P2(je, k)];
+    }
+    for (j = 1; j <= nj; j++) {
+        aux[2 * P2(j, ks - 1) + 1] = aux[2 * P2(j, ks) + 1];
+        aux[2 * P2(j, ke + 1) + 1] = aux[2 * P2(j, ke) + 1];
+    }
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            diss[4 * P2(j, k)] = o->p.pdiss_coef * ((aux[2 * P2(j + 1, k)] - 2.0 * aux[2 * P2(j, k)] + aux[2 * P2(j - 1, k)]) +
+                                                    (aux[2 * P2(j, k + 1) + 1] - 2.0 * aux[2 * P2(j, k) + 1] + aux[2 * P2(j, k - 1) + 1]));
+    /* brhs_flux_sans_convec */
+    const double de2 = pt5 * de, dz2 = pt5 * dz;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++) pp[P2(j, k)] = PQ(1, j, k) / AJ(ip, j, k);
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (c = 1; c <= 3; c++)
+                RH(c + 1, ip, j, k) = de2 * (pp[P2(j + 1, k)] * ETA(c, ip, j + 1, k) - pp[P2(j - 1, k)] * ETA(c, ip, j - 1, k));
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++)
+            for (c = 1; c <= 3; c++)
+                RH(c + 1, ip, j, k) = RH(c + 1, ip, j, k) +
+                                      dz2 * (pp[P2(j, k + 1)] * ZET(c, ip, j, k + 1) - pp[P2(j, k - 1)] * ZET(c, ip, j, k - 1));
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++) {
+            const double dudx = 0.0;
+            const double dvdy = de2 * (UCN(2, ip, j + 1, k) - UCN(2, ip, j - 1, k));
+            const double dwdz = dz2 * (UCN(3, ip, j, k + 1) - UCN(3, ip, j, k - 1));
+            RH(1, ip, j, k) = dudx + dvdy + dwdz;
+        }
+    /* brhs_convec_quick(2) */
+    const double coef = 0.125;
+    for (int dir = 2; dir <= 3; dir++) {
+        const double h = dir == 2 ? de : dz;
+        const int dj = dir == 2, dk = dir == 3;
+        for (k = 1; k <= nk; k++)
+            for (j = 1; j <= nj; j++) {
+                const double ucon = one_half * UCN(dir, ip, j, k);
+                up[P2(j, k)] = ucon + fabs(ucon);
+                um[P2(j, k)] = ucon - fabs(ucon);
+            }
+        const int sta = dir == 2 ? js + 1 : ks + 1, end = dir == 2 ? je - 1 : ke - 1;
+        for (k = ks; k <= ke; k++)
+            for (j = js; j <= je; j++) {
+                const int p = dir == 2 ? j : k;
+                double* r = rd + 3 * P2(j, k);
+                const double upv = up[P2(j, k)], umv = um[P2(j, k)];
+                for (c = 0; c < 3; c++) {
+                    const int m = c + 2;
+                    if (p >= sta && p <= end) {
+                        r[c] = coef * h *
+                               (upv * (3.0 * PQ(m, j + dj, k + dk) + 3.0 * PQ(m, j, k) - 7.0 * PQ(m, j - dj, k - dk) +
+                                       PQ(m, j - 2 * dj, k - 2 * dk)) +
+                                umv * (-3.0 * PQ(m, j - dj, k - dk) - 3.0 * PQ(m, j, k) + 7.0 * PQ(m, j + dj, k + dk) -
+                                       PQ(m, j + 2 * dj, k + 2 * dk)));
+                    } else {
+                        r[c] = h * (upv * (PQ(m, j, k) - PQ(m, j - dj, k - dk)) + umv * (PQ(m, j + dj, k + dk) - PQ(m, j, k)));
+                    }
+                }
+            }
+        for (k = ks; k <= ke; k++)
+            for (j = js; j <= je; j++)
+                for (c = 0; c < 3; c++) RH(c + 2, ip, j, k) = RH(c + 2, ip, j, k) + rd[3 * P2(j, k) + c];
+    }
+    /* brhs_unst_visc_diss (diss(2:4) = 0, set by brhs_convec_quick) */
+    for (k = ks; k <= ke; k++)
+        for (j = js; j <= je; j++) {
+            const double dtaj = 2.0 * o->p.delti * AJ(ip, j, k);
+            RH(1, ip, j, k) = RH(1, ip, j, k) + diss[4 * P2(j, k)];
+            for (c = 1; c <= 3; c++) {
+                const int m = c + 1;
+                const double dqdt = o->e_source * (3.0 * PQ(m, j, k) - 4.0 * QN(m, ip, j, k) + QNM1(m, ip, j, k)) / dtaj;
+                RH(m, ip, j, k) = RH(m, ip, j, k) + 0.0 + dqdt - visc[3 * P2(j, k) + c - 1];
+            }
+        }
+#undef PQ
+#undef P2
+    free(visc); free(diss); free(fv); free(ret); free(aux); free(pp); free(up); free(um); free(rd);
+}
+
+static void nonreflect_plane(daf_t* d, int ip, int inout) {
+    DLV;
+    const int nb = inout == 1 ? ip + 1 : ip - 1;  /* the interior neighbour plane */
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++) {
+            const double u = Q(2, ip, j, k), v = Q(3, ip, j, k), w = Q(4, ip, j, k);
+            const double ucon = UCN(1, ip, j, k) * AJ(ip, j, k);
+            const double c1 = CSI(1, ip, j, k), c2 = CSI(2, ip, j, k), c3 = CSI(3, ip, j, k);
+            const double gjj = c1 * c1 + c2 * c2 + c3 * c3;
+            const double sg = sqrt(gjj);
+            const double cj = sqrt(ucon * ucon + gjj);
+            double dq[4];
+            for (int m = 1; m <= 4; m++)
+                dq[m - 1] = inout == 1 ? (Q(m, nb, j, k) - Q(m, ip, j, k)) : (Q(m, ip, j, k) - Q(m, nb, j, k));
+            /* calc_L_general */
+            const double lam1 = ucon - cj, lam2 = ucon + cj, lam3 = ucon, lam4 = ucon;
+            const double kj = c1 + c2 + c3;
+            const double d1 = (c2 - c1 + lam1 * (v - u)) / kj;
+            const double d2 = (c2 - c1 + lam2 * (v - u)) / kj;
+            const double d3 = (c3 - c1 + lam1 * (w - u)) / kj;
+            const double d4 = (c3 - c1 + lam2 * (w - u)) / kj;
+            const double d5 = (c2 - c1 + lam3 * (v - u)) / kj * 2.0;
+            const double d6 = (c3 - c1 + lam4 * (w - u)) / kj * 2.0;
+            const double t2 = 2.0 * cj * cj / kj;
+            double si[4][4] = {{-lam2, c1, c2, c3},
+                               {-lam1, c1, c2, c3},
+                               {lam2 * d1 + lam1 * d2, -t2 - c1 * d5, t2 - c2 * d5, -c3 * d5},
+                               {lam2 * d3 + lam1 * d4, -t2 - c1 * d6, -c2 * d6, t2 - c3 * d6}};
+            const double den = 2.0 * cj * cj;
+            double lg[4];
+            for (int r = 0; r < 4; r++) {
+                double acc = 0.0;
+                for (int cc = 0; cc < 4; cc++) acc = acc + si[r][cc] * sg / den * dq[cc];
+                lg[r] = acc;
+            }
+            lg[0] = lam1 * lg[0];
+            lg[1] = lam2 * lg[1];
+            lg[2] = lam3 * lg[2];
+            lg[3] = lam4 * lg[3];
+            double L1, L2, L3, L4;
+            if (inout == 1) { /* calc_L_min */
+                L1 = lg[0]; L2 = 0.0;
+                if (ucon < 0.0) { L3 = lg[2]; L4 = lg[3]; } else { L3 = 0.0; L4 = 0.0; }
+            } else {          /* calc_L_max */
+                L1 = 0.0; L2 = lg[1];
+                if (ucon < 0.0) { L3 = 0.0; L4 = 0.0; } else { L3 = lg[2]; L4 = lg[3]; }
+            }
+            const double rh1 = (-cj * L1 + cj * L2) / sg;
+            const double rh2 = ((c1 + u * lam1) * L1 + (c1 + u * lam2) * L2 - c2 * L3 - c3 * L4) / sg;
+            const double rh3 = ((c2 + v * lam1) * L1 + (c2 + v * lam2) * L2 + (c1 + c3) * L3 - c3 * L4) / sg;
+            const double rh4 = ((c3 + w * lam1) * L1 + (c3 + w * lam2) * L2 - c2 * L3 + (c1 + c2) * L4) / sg;
+            RH(1, ip, j, k) = RH(1, ip, j, k) + rh1 / AJ(ip, j, k);
+            RH(2, ip, j, k) = RH(2, ip, j, k) + rh2 / AJ(ip, j, k);
+            RH(3, ip, j, k) = RH(3, ip, j, k) + rh3 / AJ(ip, j, k);
+            RH(4, ip, j, k) = RH(4, ip, j, k) + rh4 / AJ(ip, j, k);
+        }
+}
+
 /* rhs_modal_matrices (src/rhs_modal_matrices.F90:1-270) */
 static void matmul4(const double a[4][4], const double b[4][4], double c[4][4]) {
     int r, s, t;
@@ -1164,7 +1409,7 @@ static void matvec_node(const double* a /* 16, row-major */, double* v /* 4 */) 
 /* one tridiagonal line solve of the DAF: builds the 4 scalar systems of
  * rhs_diag_solver for a line in direction dir and solves them with SGTSV */
 static void daf_line(daf_t* d, int dir, int i0, int j0, int k0, int len, const double* brh1, const double* brh2,
-                     int use_b1, int use_b2, double* aw, double* ap, double* ae, double* aq) {
+                     int use_b1, int use_b2, double* aw, double* ap, double* ae, double* aq, int ext) {
     DLV;
     int s, m;
     const double h = dir == 1 ? d->dc : dir == 2 ? d->de : d->dz;
@@ -1173,7 +1418,7 @@ static void daf_line(daf_t* d, int dir, int i0, int j0, int k0, int len, const d
     const double lambda[4] = {0.0, 0.0, 1.0, -1.0};
     const int di = dir == 1, dj = dir == 2, dk = dir == 3;
     (void)nk;
-    for (s = 0; s < len; s++) {
+    for (s = 0; s < len - ext; s++) {
         int i = i0 + s * di, j = j0 + s * dj, k = k0 + s * dk;
         double dtc = 1.0, tmp, dds;
         if (DTAU(i, j, k) == 0.0) dtc = 0.0;
@@ -1190,6 +1435,25 @@ static void daf_line(daf_t* d, int dir, int i0, int j0, int k0, int len, const d
         ae[2 * len + s] = dds * lambda[2] - tmp;
         ae[3 * len + s] = dds * lambda[3] - tmp;
         for (m = 0; m < 4; m++) aq[m * len + s] = RH(m + 1, i, j, k);
+    }
+    if (ext) {
+        /* the characteristic-outlet plane i = idend closes the i line
+         * (rhs_diag_solver.F90:207-221): upwind coupling of the +1
+         * eigenvalue only, no smoothing */
+        const int i = i0 + (len - 1);
+        double dds = DTAU(i, j0, k0) * h * SPR(1, i, j0, k0);
+        s = len - 1;
+        ap[0 * len + s] = 1.0;
+        ap[1 * len + s] = 1.0;
+        ap[2 * len + s] = 1.0 + dds * lambda[2];
+        ap[3 * len + s] = 1.0;
+        dds = DTAU(i - 1, j0, k0) * h * SPR(1, i - 1, j0, k0);
+        aw[0 * len + s] = 0.0;
+        aw[1 * len + s] = 0.0;
+        aw[2 * len + s] = -dds * lambda[2];
+        aw[3 * len + s] = 0.0;
+        for (m = 0; m < 4; m++) ae[m * len + s] = 0.0;
+        for (m = 0; m < 4; m++) aq[m * len + s] = RH(m + 1, i, j0, k0);
     }
     if (use_b1)
         for (m = 0; m < 4; m++) aq[m * len + 0] = aq[m * len + 0] - aw[m * len + 0] * brh1[m];
@@ -1216,7 +1480,9 @@ static void rhs_diag_solver(daf_t* d) {
     const int *bt = o->p.btype;
     if (nk > maxlen) maxlen = nk;
     aw = zalloc(4 * maxlen); ap = zalloc(4 * maxlen); ae = zalloc(4 * maxlen); aq = zalloc(4 * maxlen);
-    brh = zalloc(8L * (long)maxlen * maxlen); /* brh(1:4, 1:2, line) */
+    /* brh(1:4, 1:2, line); the j and k sweeps index lines by (i, k) and
+     * (i, j) with i up to idend = ni on a characteristic outlet */
+    brh = zalloc(8L * (long)(maxlen + 1) * (maxlen + 1));
 #define BRH(m, side, a, b, na) brh[((long)((b)) * (na) + (a)) * 8 + ((side)-1) * 4 + (m)-1]
     /* rh = S (-dtau J rh) (:56-72) */
     for (k = ks; k <= ke; k++)
@@ -1229,7 +1495,7 @@ static void rhs_diag_solver(daf_t* d) {
                 for (m = 2; m <= 4; m++) RH(m, i, j, k) = 1.0 / alpha * r[m - 1];
             }
     /* ---- csi sweep ---- */
-    memset(brh, 0, 8L * maxlen * maxlen * sizeof(double));
+    memset(brh, 0, 8L * (maxlen + 1) * (maxlen + 1) * sizeof(double));
     if (n == 1 && bt[0] != 5)
         for (k = ks; k <= ke; k++)
             for (j = js; j <= je; j++) {
@@ -1250,11 +1516,11 @@ static void rhs_diag_solver(daf_t* d) {
         for (j = js; j <= je; j++) {
             /* front-side brh is built only under myfront /= MPI_PROC_NULL (:102),
              * so on the domain boundary it stays zero when consumed (:202-205) */
-            daf_line(d, 1, is, j, k, ie - is + 1, &BRH(1, 1, j, k, nj), &BRH(1, 2, j, k, nj),
-                     n == 1 && bt[0] != 5, n == 1 && bt[1] != 5, aw, ap, ae, aq);
+            daf_line(d, 1, is, j, k, idend - is + 1, &BRH(1, 1, j, k, nj), &BRH(1, 2, j, k, nj),
+                     n == 1 && bt[0] != 5, n == 1 && bt[1] != 5, aw, ap, ae, aq, idend > ie);
         }
     /* ---- eta sweep ---- */
-    memset(brh, 0, 8L * maxlen * maxlen * sizeof(double));
+    memset(brh, 0, 8L * (maxlen + 1) * (maxlen + 1) * sizeof(double));
     if (n == 1) {
         if (bt[2] != 5)
             for (k = ks; k <= ke; k++)
@@ -1284,9 +1550,9 @@ static void rhs_diag_solver(daf_t* d) {
     for (k = ks; k <= ke; k++)
         for (i = is; i <= idend; i++)
             daf_line(d, 2, i, js, k, je - js + 1, &BRH(1, 1, i, k, ni), &BRH(1, 2, i, k, ni),
-                     n == 1 && bt[2] != 5, n == 1 && bt[3] != 5, aw, ap, ae, aq);
+                     n == 1 && bt[2] != 5, n == 1 && bt[3] != 5, aw, ap, ae, aq, 0);
     /* ---- zet sweep ---- */
-    memset(brh, 0, 8L * maxlen * maxlen * sizeof(double));
+    memset(brh, 0, 8L * (maxlen + 1) * (maxlen + 1) * sizeof(double));
     if (n == 1) {
         if (bt[4] != 5)
             for (j = js; j <= je; j++)
@@ -1317,7 +1583,7 @@ static void rhs_diag_solver(daf_t* d) {
     for (j = js; j <= je; j++)
         for (i = is; i <= idend; i++)
             daf_line(d, 3, i, j, ks, ke - ks + 1, &BRH(1, 1, i, j, ni), &BRH(1, 2, i, j, ni),
-                     n == 1 && bt[4] != 5, n == 1 && bt[5] != 5, aw, ap, ae, aq);
+                     n == 1 && bt[4] != 5, n == 1 && bt[5] != 5, aw, ap, ae, aq, 0);
     /* dQ = Mc dQ*** (:624-633) */
     for (k = ks; k <= ke; k++)
         for (j = js; j <= je; j++)
@@ -1492,6 +1758,17 @@ static int solver_daf(or_state* o, int n, int recalc, int calc_pk) {
     rhs_unst_visc_diss(d);
     if (n > 1) rhs_pk(d, calc_pk == 1 ? 1 : 0);
     zero_rh_boundaries(d);
+    /* characteristic outlet on the i faces of the fine level (:322-365) */
+    if (n == 1 && o->p.btype[0] == 5) {
+        mg_brhs_plane(d, 1);
+        nonreflect_plane(d, 1, 1);
+    }
+    if (n == 1 && o->p.btype[1] == 5) {
+        mg_brhs_plane(d, ni);
+        nonreflect_plane(d, ni, 2);
+    }
+    /* the outlet plane joins the sweeps and the update (:370-373) */
+    if (n == 1 && o->p.btype[1] == 5) d->idend = d->ie + 1;
     rhs_lines(d);
     rhs_modal_matrices(d);
     rhs_diag_solver(d);
@@ -1890,6 +2167,7 @@ static void rhs_sa_adi_solver(sa_t* d) {
 }
 
 /* rhs_sa_bcond (src/rhs_sa_bcond.F90:2-294) */
+#define SRAT(b, a, c, na) o->rat[b][((a)-1) + (long)(na) * ((c)-1)]
 static void rhs_sa_bcond(sa_t* d) {
     SLV;
     const or_state* o = d->o;
@@ -1903,6 +2181,9 @@ static void rhs_sa_bcond(sa_t* d) {
             for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = o->sa[0][0] * Q5(2, j, k) + o->sb[0][0] * Q5(3, j, k);
             break;
         case 6: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = Q5(d->ie, j, k); break;
+        case 5: /* characteristic outlet: extrapolated on the grid ratio */
+            for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = (1.0 + SRAT(0, j, k, nj)) * Q5(2, j, k) - SRAT(0, j, k, nj) * Q5(3, j, k);
+            break;
         default: break;
     }
     switch (bt[1]) {
@@ -1912,6 +2193,9 @@ static void rhs_sa_bcond(sa_t* d) {
             break;
         case 6: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(d->is, j, k); break;
         case 8: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(ni - 1, j, k); break;
+        case 5:
+            for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = (1.0 + SRAT(1, j, k, nj)) * Q5(ni - 1, j, k) - SRAT(1, j, k, nj) * Q5(ni - 2, j, k);
+            break;
         default: break;
     }
     switch (bt[2]) {
@@ -1920,6 +2204,9 @@ static void rhs_sa_bcond(sa_t* d) {
             for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = o->sa[2][0] * Q5(i, 2, k) + o->sb[2][0] * Q5(i, 3, k);
             break;
         case 6: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = Q5(i, d->je, k); break;
+        case 5:
+            for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = (1.0 + SRAT(2, i, k, ni)) * Q5(i, 2, k) - SRAT(2, i, k, ni) * Q5(i, 3, k);
+            break;
         default: break;
     }
     switch (bt[3]) {
@@ -1928,6 +2215,9 @@ static void rhs_sa_bcond(sa_t* d) {
             for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = o->sa[3][0] * Q5(i, nj - 1, k) + o->sb[3][0] * Q5(i, nj - 2, k);
             break;
         case 6: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = Q5(i, d->js, k); break;
+        case 5:
+            for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = (1.0 + SRAT(3, i, k, ni)) * Q5(i, nj - 1, k) - SRAT(3, i, k, ni) * Q5(i, nj - 2, k);
+            break;
         default: break;
     }
     switch (bt[4]) {
@@ -1936,6 +2226,9 @@ static void rhs_sa_bcond(sa_t* d) {
             for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = o->sa[4][0] * Q5(i, j, 2) + o->sb[4][0] * Q5(i, j, 3);
             break;
         case 6: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = Q5(i, j, d->ke); break;
+        case 5:
+            for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = (1.0 + SRAT(4, i, j, ni)) * Q5(i, j, 2) - SRAT(4, i, j, ni) * Q5(i, j, 3);
+            break;
         default: break;
     }
     switch (bt[5]) {
@@ -1944,6 +2237,9 @@ static void rhs_sa_bcond(sa_t* d) {
             for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = o->sa[5][0] * Q5(i, j, nk - 1) + o->sb[5][0] * Q5(i, j, nk - 2);
             break;
         case 6: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = Q5(i, j, d->ks); break;
+        case 5:
+            for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = (1.0 + SRAT(5, i, j, ni)) * Q5(i, j, nk - 1) - SRAT(5, i, j, ni) * Q5(i, j, nk - 2);
+            break;
         default: break;
     }
     for (k = 1; k <= nk; k++)

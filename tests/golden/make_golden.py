@@ -54,6 +54,11 @@ FIXTURES = {
     "wavy_mg_driver_inflow_sa": ("wavy", dict(ni=17, nj=17, nk=21, ng=2, itm=3, des=True, iters=(1, 2, 3),
                                               btype=(4, 3, 4, 1, 4, 2), seed=99, amp=0.06),
                                  dict(mode=1, nsteps=2)),
+    # characteristic outlet (btype 5: mg_brhs + nonreflect_ibc on the i = max
+    # plane, which joins the fine sweeps) behind an inflow face
+    "wavy_mg_driver_outlet_sa": ("wavy", dict(ni=17, nj=17, nk=21, ng=2, itm=3, des=True, iters=(1, 2, 3),
+                                              btype=(4, 5, 1, 1, 2, 2), seed=99, amp=0.06),
+                                 dict(mode=1, nsteps=2)),
     # early exit on the reference's convergence test (mg_ck_conver)
     "wavy_mg_driver_converge": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=12, it_min=2, er_min=-0.15,
                                              iters=(1, 2, 3)), dict(mode=1, nsteps=1)),

@@ -130,7 +130,9 @@ def check_des_eddy(c):
 @pytest.mark.parametrize("ng,des,bt,itm", [(2, False, None, 4), (3, True, None, 3),
                                           (3, True, (6, 6, 1, 1, 6, 6), 3),
                                           # inflow faces: pressure extrapolated on the grid ratio
-                                          (2, True, (4, 3, 4, 1, 4, 2), 3), (3, True, (3, 4, 1, 4, 2, 4), 3)])
+                                          (2, True, (4, 3, 4, 1, 4, 2), 3), (3, True, (3, 4, 1, 4, 2, 4), 3),
+                                          # characteristic outlet: the i = max plane joins the sweeps
+                                          (2, True, (4, 5, 1, 1, 2, 2), 3), (3, True, (5, 5, 5, 1, 2, 5), 3)])
 def test_mg_driver(gpu, ng, des, bt, itm):
     c = cases.wavy(17, 17, 17, ng=ng, itm=itm, des=des, btype=bt, iters=(1, 2, 3))
     o = po.Oracle(c.params).load_case(c)

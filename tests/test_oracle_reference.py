@@ -87,7 +87,11 @@ def test_oracle_matches_reference_golden(path):
 @pytest.mark.skipif(not po.ref_available(), reason="reference harness not built (needs /root/reference)")
 @pytest.mark.parametrize("ng,des,bt", [(2, True, (1, 3, 1, 0, 2, 3)), (3, True, (6, 6, 1, 1, 6, 6)),
                                        (3, False, (0, 0, 0, 0, 0, 0)), (2, True, (4, 3, 1, 1, 2, 2)),
-                                       (3, True, (3, 4, 4, 4, 4, 4))])
+                                       (3, True, (3, 4, 4, 4, 4, 4)),
+                                       # characteristic outlet (btype 5) on i = max, on both i faces,
+                                       # and on j / k faces (nu~ extrapolation, no semi-explicit BC)
+                                       (2, True, (3, 5, 1, 1, 2, 2)), (3, False, (5, 5, 1, 1, 2, 2)),
+                                       (3, True, (4, 5, 5, 1, 2, 5))])
 def test_oracle_matches_reference_live(ng, des, bt):
     c = cases.wavy(17, 17, 21, ng=ng, itm=3, des=des, btype=bt, iters=(1, 2, 3), seed=99, amp=0.06)
     o = po.Oracle(c.params).load_case(c)
