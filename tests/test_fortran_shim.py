@@ -38,7 +38,9 @@ def check_state(a, b):
 
 
 @need_shims
-@pytest.mark.parametrize("des,bt", [(False, None), (True, (6, 6, 1, 1, 6, 6))])
+@pytest.mark.parametrize("des,bt", [(False, None), (True, (6, 6, 1, 1, 6, 6)),
+                                    # inflow and characteristic outlet: the shim passes the coordinates
+                                    (True, (4, 5, 1, 1, 2, 2))])
 def test_shim_mg_driver(gpu, des, bt):
     c = cases.wavy(17, 17, 17, ng=3, itm=3, des=des, btype=bt, iters=(1, 2, 3))
     ref = po.run_reference(c, mode=1, nsteps=2)

@@ -233,9 +233,9 @@ def test_mg_driver_overset(gpu, apply_mom):
 TOL_MIXED = 1e-6
 
 
-@pytest.mark.parametrize("des", [False, True])
-def test_mg_driver_mixed(gpu, des):
-    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=des, iters=(1, 2, 3), precision=PREC_MIXED)
+@pytest.mark.parametrize("des,bt", [(False, None), (True, None), (True, (4, 5, 1, 1, 2, 2))])
+def test_mg_driver_mixed(gpu, des, bt):
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=des, btype=bt, iters=(1, 2, 3), precision=PREC_MIXED)
     o = po.Oracle(c.params).load_case(c)
     s = Solver(c.params).load_case(c)
     for _ in range(2):
