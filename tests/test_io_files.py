@@ -450,3 +450,29 @@ def test_case_from_files_runs_like_the_case(gpu, tmp_path):
         assert np.array_equal(fa, fb)
     a.close()
     b.close()
+
+
+def test_two_zone_grid_and_solution(tmp_path):
+    """Zones follow each other in the files (src/init.F90:2227-2241,
+    :2392-2420): x, y, z, wd of zone 1, then of zone 2."""
+    dims = [(4, 3, 2), (3, 3, 3)]
+    rng = np.random.default_rng(8)
+    zones = [{nm: rng.normal(size=int(np.prod(d))).astype(np.float32) for nm in ("x", "y", "z", "d")} for d in dims]
+    (tmp_path / "grid").write_bytes(b"".join(record(z[nm].tobytes()) for z in zones for nm in ("x", "y", "z", "d")))
+    x, y, z, wd = files.read_grid(tmp_path / "grid", dims)
+    for a, nm in ((x, "x"), (y, "y"), (z, "z"), (wd, "d")):
+        assert np.array_equal(a, np.concatenate([zz[nm] for zz in zones]).astype(np.float64))
+    sol = []
+    for d in dims:
+        n = int(np.prod(d))
+        q, qn, nu = rng.normal(size=(n, 5)), rng.normal(size=(n, 5)), rng.uniform(size=n)
+        sol.append((q, qn, nu))
+    body = b""
+    for q, qn, nu in sol:
+        body += b"".join(record(q[:, m].tobytes()) for m in range(5)) + record(nu.tobytes())
+        body += b"".join(record(qn[:, m].tobytes()) for m in range(5))
+    (tmp_path / "solu").write_bytes(body)
+    q, qn, nu = files.read_solu(tmp_path / "solu", dims)
+    assert np.array_equal(q, np.concatenate([s[0] for s in sol]))
+    assert np.array_equal(qn, np.concatenate([s[1] for s in sol]))
+    assert np.array_equal(nu, np.concatenate([s[2] for s in sol]))
