@@ -51,7 +51,7 @@ __global__ void k_inject_p(Level Lc, Level Lf, Ptrs P, int ncomp, int icrs, int 
 }
 
 __global__ void __launch_bounds__(256) k_nlevel(Level Lc, Level Lf, Fields F, int icrs, int jcrs, int kcrs) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= Lc.n) return;
     const int ni = Lc.ni, nj = Lc.nj, nk = Lc.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
@@ -138,7 +138,7 @@ __device__ __forceinline__ double fill_i(const ProlongCtx& P, const Fields& F, i
 // dq at every fine interior node into F.dp (level-local); boundary dq is
 // zero after smoothing and never enters the interior sweeps
 __global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const Level& L = P.Lf;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;

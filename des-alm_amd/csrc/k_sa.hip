@@ -27,7 +27,7 @@ constexpr double cb1 = 0.1355, cb2 = 0.622, sig = 2.0 / 3.0, capa = 0.41, cw1 = 
                  cw3h = 64.0, cv1c = 357.911, cv2 = 5.0;
 
 __global__ void k_sa_prep(Level L, Fields F) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
