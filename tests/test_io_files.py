@@ -476,3 +476,52 @@ def test_two_zone_grid_and_solution(tmp_path):
     assert np.array_equal(q, np.concatenate([s[0] for s in sol]))
     assert np.array_equal(qn, np.concatenate([s[1] for s in sol]))
     assert np.array_equal(nu, np.concatenate([s[2] for s in sol]))
+
+
+@pytest.mark.gpu
+def test_sample_deck_parameters_drive_the_gpu_path(gpu):
+    """The reference's sample deck (ren 202500, cfl/vnn, ep, iter 1 2 3, an
+    inflow face, a free surface at i = max, walls elsewhere, pinned node
+    (1, 1, 1)) on a 17^3 curvilinear grid: the GPU V-cycles match the oracle
+    run with the same parameters."""
+    import pyoracle as po
+    from desalm import cases
+    from desalm.lib import Solver
+    d = files.read_deck(os.path.join(HERE, "golden", "ind3dmg_sample.dat"), 3)
+    p = d.params
+    p.img = p.jmg = p.kmg = 17
+    p.icrs = p.jcrs = p.kcrs = 1
+    p.itm = 3
+    p.it_min = 3
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=True, iters=(1, 2, 3), btype=tuple(p.btype), ren=p.ren)
+    c.params = p
+    o = po.Oracle(p).load_case(c)
+    s = Solver(p).load_case(c)
+    itc_o, _ = o.mg_driver()
+    itc_g, _ = s.mg_driver()
+    assert itc_o == itc_g == 3
+    qo, qg = o.get_state()[0], s.get_state()[0]
+    for m in range(5):
+        assert po.relerr(qg[:, m], qo[:, m]) < 1e-10, m
+    s.close()
+
+
+def test_sample_deck_parameters_oracle_matches_reference():
+    """The same deck-driven case on the CPU: the oracle against the reference
+    Fortran (skipped where the reference harness is not built)."""
+    import pyoracle as po
+    from desalm import cases
+    if not po.ref_available():
+        pytest.skip("reference harness not built (needs /root/reference)")
+    d = files.read_deck(os.path.join(HERE, "golden", "ind3dmg_sample.dat"), 3)
+    p = d.params
+    p.img = p.jmg = p.kmg = 17
+    p.icrs = p.jcrs = p.kcrs = 1
+    p.itm = 3
+    p.it_min = 3
+    c = cases.wavy(17, 17, 17, ng=3, itm=3, des=True, iters=(1, 2, 3), btype=tuple(p.btype), ren=p.ren)
+    c.params = p
+    o = po.Oracle(p).load_case(c)
+    o.mg_driver()
+    r = po.run_reference(c, mode=1, nsteps=1)
+    assert np.array_equal(o.get_state()[0], r["q"])
