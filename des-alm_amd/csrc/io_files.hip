@@ -54,6 +54,21 @@ bool read_record(FILE* f, std::vector<unsigned char>& buf, std::string& err) {
             return false;
         }
         const uint32_t len = head < 0 ? (uint32_t)(-(int64_t)head) : (uint32_t)head;
+        // refuse a length the rest of the file cannot hold before allocating it
+        const long here = std::ftell(f);
+        if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) {
+            err = "cannot seek in the file";
+            return false;
+        }
+        const long end = std::ftell(f);
+        if (std::fseek(f, here, SEEK_SET) != 0 || end < 0) {
+            err = "cannot seek in the file";
+            return false;
+        }
+        if ((int64_t)len + 4 > (int64_t)end - (int64_t)here) {
+            err = "truncated record";
+            return false;
+        }
         const size_t at = buf.size();
         buf.resize(at + len);
         if (len && std::fread(buf.data() + at, 1, len, f) != len) {

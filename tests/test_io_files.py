@@ -525,3 +525,31 @@ def test_sample_deck_parameters_oracle_matches_reference():
     o.mg_driver()
     r = po.run_reference(c, mode=1, nsteps=1)
     assert np.array_equal(o.get_state()[0], r["q"])
+
+
+def test_damaged_files_raise_instead_of_crashing(tmp_path):
+    """Every truncation and a run of random byte flips of a valid grid file
+    either read or raise DesalmError (the reader checks every length)."""
+    dims = (4, 3, 3)
+    n = 36
+    x = np.arange(n, dtype=np.float32)
+    good = b"".join(record((x + c).tobytes()) for c in range(4))
+    f = tmp_path / "g"
+    for cut in range(0, len(good), 7):
+        f.write_bytes(good[:cut])
+        with pytest.raises(DesalmError):
+            files.read_grid(f, dims)
+    rng = np.random.default_rng(12)
+    for _ in range(200):
+        b = bytearray(good)
+        for p in rng.integers(0, len(b), size=3):
+            b[p] = int(rng.integers(0, 256))
+        f.write_bytes(bytes(b))
+        try:
+            files.read_grid(f, dims)
+        except DesalmError:
+            pass
+    # a huge declared record length is refused without reading past the file
+    f.write_bytes(struct.pack("<i", 2 ** 31 - 9) + b"\0" * 16)
+    with pytest.raises(DesalmError):
+        files.read_grid(f, dims)
