@@ -330,6 +330,8 @@ def main():
                     help="fix the launch shapes instead of autotuning: k planes per workgroup of k_visc_pk "
                          "per level and of k_sa_rhs (e.g. 8,4,1:8, as a previous run's 'tiles' reported)")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--ref-iters", type=int, default=4,
+                    help="pseudo-iterations of the reference Fortran timed for the CPU baseline (0: port only)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--conv-itm", type=int, default=120,
                     help="pseudo-iteration cap per physical step of the dual-time leg (0 skips it)")
@@ -455,6 +457,25 @@ def main():
                "kind": "port",
                "sample": f"{itc_c} pseudo-iterations of the same {args.n}^3 channel V-cycle, C oracle, 1 thread "
                          f"({tc1 - tc0:.1f} s)"}
+        # the reference's own Fortran (built from its sources in FP64,
+        # oracle/_ref), when present: timed inside the program around its
+        # mg_driver calls, one rank
+        if po.ref_available() and args.ref_iters > 0:
+            cr = cases.channel(args.n, args.n, args.n, ng=3, itm=args.ref_iters, it_min=args.ref_iters,
+                               iters=(1, 2, 3))
+            try:
+                r = po.run_reference(cr, mode=1, nsteps=1, timeout=900)
+            except Exception as e:  # keep the port figure if the reference cannot run here
+                r = {"seconds": None, "error": str(e)[:200]}
+            if r.get("seconds"):
+                port = dict(cpu)
+                cpu = {"value": round(n1 * args.ref_iters / r["seconds"] / 1e6, 4), "unit": "Mcell-iters/s",
+                       "cores": 1, "kind": "reference",
+                       "sample": f"{args.ref_iters} pseudo-iterations of the same {args.n}^3 channel V-cycle, the "
+                                 f"reference's mg_driver (its Fortran sources built FP64 with amdflang -O2), one "
+                                 f"rank ({r['seconds']:.1f} s inside the program)",
+                       "port": port, "reference_over_port_time": round(
+                           (r["seconds"] / args.ref_iters) / ((tc1 - tc0) / itc_c), 3)}
 
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Mcell-iters/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),

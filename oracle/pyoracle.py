@@ -314,6 +314,19 @@ def _write_alm(f, td, alm):
             pf.write("-999 0 0\n")
 
 
+def _big_stack():
+    """The reference's procedures keep their level-sized work arrays as
+    automatic arrays on the stack (e.g. solver_daf's), which overflow the
+    default 8 MiB stack beyond about 65^3 nodes on one rank: the child gets
+    the largest stack the hard limit allows."""
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    try:
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+    except (ValueError, OSError):
+        pass
+
+
 def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600, exe=None):
     """Run the reference's own Fortran (compiled from its sources by
     oracle/ref/build_ref.sh) on `case`.  The reference is built with three
@@ -322,7 +335,8 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
     exe: another build of the same harness (SHIM_MGD_EXE, SHIM_DAF_EXE: the
     reference with the Fortran drop-in linked in place of one routine).
     Returns a dict of fine-level arrays and all-level metrics, plus the text
-    of the 'conver' file under "conver" when the run wrote one."""
+    of the 'conver' file under "conver" when the run wrote one and, for
+    mode 1, the wall time of the mg_driver calls under "seconds"."""
     p = case.params
     if p.ng > 3:
         raise ValueError("the reference is compiled for 3 levels")
@@ -369,9 +383,11 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
                 _write_alm(f, td, alm)
         env = dict(os.environ)
         subprocess.run([exe or REF_EXE, fin, fout], check=True, cwd=td, timeout=timeout, env=env,
-                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, preexec_fn=_big_stack)
         conver = os.path.join(td, "conver")
         text = open(conver).read() if os.path.exists(conver) else None
+        timing = os.path.join(td, "timing")
+        seconds = float(open(timing).read()) if os.path.exists(timing) else None
         with open(fout, "rb") as f:
             nall, n1r = np.fromfile(f, dtype=np.int32, count=2)
             assert n1r == n1
@@ -387,6 +403,7 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
             out["aj"] = np.fromfile(f, dtype=np.float64, count=nall)
             out["dtev"] = np.fromfile(f, dtype=np.float64, count=nall)
     out["conver"] = text
+    out["seconds"] = seconds  # wall time of the mg_driver calls (mode 1)
     return out
 
 

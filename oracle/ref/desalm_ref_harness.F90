@@ -54,6 +54,7 @@ program desalm_ref_harness
   real (kind = rdf), allocatable :: r2(:,:), r3(:,:,:)
   real (kind = rdf) :: rt(2)
   integer :: la, lb, u
+  integer (kind = 8) :: c0, c1, crate
   real (kind = rdf), allocatable :: buf(:)
 
   call get_command_argument(1, fin)
@@ -190,9 +191,15 @@ program desalm_ref_harness
   lb = le_idx_b(1)
   select case (mode)
   case (1)
+     ! wall time of the mg_driver calls alone, for the CPU baseline
+     call system_clock(count = c0, count_rate = crate)
      do istep = 1, nsteps
         call mg_driver
      end do
+     call system_clock(count = c1)
+     open (newunit = u, file = 'timing', form = 'formatted')
+     write (u, '(es24.16)') real(c1 - c0, kind = 8) / real(crate, kind = 8)
+     close (u)
   case (2)
      call solver_daf (me, 1, hdr(34), hdr(35), &
           le_ia(n),le_ib(n),le_ja(n),le_jb(n),le_ka(n),le_kb(n), &
