@@ -93,3 +93,20 @@ def test_hooks_need_the_grid_and_one_zone(gpu):
     g.close()
     a.close()
     b.close()
+
+
+def combined_case():
+    """Actuator line, an inflow face, a characteristic outlet, j walls and
+    both hooks on one curvilinear block: the features together."""
+    from golden.make_golden import build
+    return build("wavy", dict(ni=21, nj=17, nk=17, ng=2, itm=3, amp=0.02, des=True, iters=(1, 2, 3),
+                              btype=(4, 5, 1, 1, 2, 2), alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)))
+
+
+def test_everything_together_matches_oracle(gpu):
+    c = combined_case()
+    q, _, _, xnut = run_gpu(c, BOTH)
+    qo, _, _, xnuto = run_oracle(c, BOTH, wf_newton=False)
+    ok, err = close(q, qo, 1e-8)
+    assert ok, err
+    assert np.all(np.isfinite(q))

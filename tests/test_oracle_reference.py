@@ -156,3 +156,17 @@ def test_oracle_hooks_match_reference_live(ng, bt, hooks):
         o2 = po.Oracle(c.params).load_case(c).set_hooks(hooks, wf_newton=True)
         o2.mg_driver()
         assert not np.array_equal(o2.get_state()[0], r["q"])
+
+
+@pytest.mark.skipif(not po.hooks_available(), reason="reference harness with hooks not built (needs /root/reference)")
+def test_oracle_everything_together_matches_reference_live():
+    """Actuator line, inflow, characteristic outlet, j walls and both hooks
+    (the reference's Newton chain) in one run, against the reference built
+    with conservation.F90 and wf_mpi.F90."""
+    c = build("wavy", dict(ni=21, nj=17, nk=17, ng=2, itm=3, amp=0.02, des=True, iters=(1, 2, 3),
+                           btype=(4, 5, 1, 1, 2, 2), alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)))
+    o = po.Oracle(c.params).load_case(c).set_hooks(3, wf_newton=True)
+    o.mg_driver()
+    r = po.run_reference(c, mode=1, nsteps=1, exe=po.HOOKS_EXE)
+    assert np.array_equal(o.get_state()[0], r["q"])
+    assert np.array_equal(o.get_state()[3], r["xnut"])
