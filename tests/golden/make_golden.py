@@ -59,6 +59,12 @@ FIXTURES = {
     "wavy_mg_driver_outlet_sa": ("wavy", dict(ni=17, nj=17, nk=21, ng=2, itm=3, des=True, iters=(1, 2, 3),
                                               btype=(4, 5, 1, 1, 2, 2), seed=99, amp=0.06),
                                  dict(mode=1, nsteps=2)),
+    # everything at once: actuator line, inflow, characteristic outlet, j
+    # walls with wall functions, outlet flux rescale
+    "wavy_mg_driver_all_sa": ("wavy", dict(ni=21, nj=17, nk=17, ng=2, itm=3, amp=0.02, des=True, iters=(1, 2, 3),
+                                           btype=(4, 5, 1, 1, 2, 2),
+                                           alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)),
+                              dict(mode=1, nsteps=2, hooks=3)),
     # early exit on the reference's convergence test (mg_ck_conver)
     "wavy_mg_driver_converge": ("wavy", dict(ni=17, nj=17, nk=17, ng=3, itm=12, it_min=2, er_min=-0.15,
                                              iters=(1, 2, 3)), dict(mode=1, nsteps=1)),
