@@ -734,7 +734,14 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     for (desalm_ctx* c : zs) profiling = profiling || c->prof.on;
     const int carried = recalc_out(0);
     const bool use_graph = !profiling && p.itm > 1 && schedule(0) == schedule(carried);
-    hipGraphExec_t exec = nullptr;
+    // the executable graph is destroyed on every way out of this function
+    struct ExecGuard {
+        hipGraphExec_t e = nullptr;
+        ~ExecGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+        }
+    } guard;
+    hipGraphExec_t& exec = guard.e;
     int decide_recalc_rh = 0;
     for (int it = 1; it <= p.itm; it++) {
         itc++;
@@ -750,10 +757,7 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
                 if (ie != hipSuccess) return fail(c0, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
             }
             const hipError_t le = hipGraphLaunch(exec, s);
-            if (le != hipSuccess) {
-                (void)hipGraphExecDestroy(exec);
-                return fail(c0, std::string("hipGraphLaunch failed: ") + hipGetErrorString(le));
-            }
+            if (le != hipSuccess) return fail(c0, std::string("hipGraphLaunch failed: ") + hipGetErrorString(le));
             decide_recalc_rh = carried;
         } else {
             decide_recalc_rh = body(decide_recalc_rh);
@@ -781,7 +785,6 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
         if (itc > p.it_min && eomax < p.eo_min) converged = true;
         if (converged) break;
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     for (desalm_ctx* c : zs)
         HIPCHK(c0, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     if (itc_out) *itc_out = itc;

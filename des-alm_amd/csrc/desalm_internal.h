@@ -13,13 +13,11 @@
 #include <vector>
 
 #include "../../include/desalm.h"
+#include "host_error.h"
 
 namespace desalm {
 
 constexpr int kMaxLevels = DESALM_MAX_LEVELS;
-
-// Records msg as the error desalm_last_error(NULL) returns; returns -1.
-int host_fail(const std::string& msg);
 
 // Workgroups are handed to the 8 XCDs round robin (workgroup b runs on XCD
 // b % 8), and every XCD has its own L2.  xcd_block(b, nb) renumbers the nb

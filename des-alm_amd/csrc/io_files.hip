@@ -27,7 +27,8 @@
 #include <string>
 #include <vector>
 
-#include "desalm_internal.h"
+#include "../../include/desalm.h"
+#include "host_error.h"
 
 namespace desalm {
 namespace {
@@ -120,7 +121,9 @@ bool read_reals(FILE* f, std::vector<unsigned char>& buf, long count, int& real_
     }
     if (real_bytes == 0 && count > 0 && (buf.size() == 4u * (size_t)count || buf.size() == 8u * (size_t)count))
         real_bytes = (int)(buf.size() / (size_t)count);
-    if (buf.size() != (size_t)real_bytes * (size_t)count) {
+    // a record whose size fits neither REAL*4 nor REAL*8 (an empty one
+    // included) leaves real_bytes undetected: never accept it as 0 * count
+    if ((real_bytes != 4 && real_bytes != 8) || buf.size() != (size_t)real_bytes * (size_t)count) {
         err = std::string(what) + ": record holds " + std::to_string(buf.size()) + " bytes, expected " +
               std::to_string(count) + " reals";
         return false;

@@ -4,8 +4,9 @@
 ! V-cycles of DAF smoothing, SA/DES update, fine-grid BCs, convergence test
 ! of mg_ck_conver :335-405) runs on the GPU through libdesalm.so.
 !
-! In:  q, qn, qnm1, xnut of the fine level (module global), everything
-!      desalm_f_setup reads on the first call.
+! In:  q, qn, qnm1, xnut of the fine level (module global), the blade
+!      angles of module global_turb, everything desalm_f_setup reads on the
+!      first call.
 ! Out: q, qn, qnm1, xnut of the fine level; itc, itmax, nstop; one line per
 !      pseudo-iteration appended to 'conver' on the root rank, in the format
 !      of mg_ck_conver (:363).
@@ -37,6 +38,7 @@ subroutine mg_driver
   xb = real(xnut(1:n1), c_double)
   rc = desalm_set_state(desalm_f_ctx, qb, qnb, qnm1b, xb)
   call desalm_f_check(rc, 'desalm_set_state')
+  call desalm_f_angles()
 
   rc = desalm_mg_driver(desalm_f_ctx, itc_c, resid)
   call desalm_f_check(rc, 'desalm_mg_driver')

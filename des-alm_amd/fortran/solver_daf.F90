@@ -49,6 +49,7 @@ subroutine solver_daf (me, decide_grid_level, decide_recalc_rh, &
   qb = real(q, c_double)
   pkb = real(pk, c_double)
   rhb = 0.0_c_double
+  call desalm_f_angles()
   rc = desalm_solver_daf(desalm_f_ctx, int(n, c_int), int(decide_recalc_rh, c_int), &
                          int(decide_calc_pk, c_int), qb, real(qn, c_double), real(qnm1, c_double), &
                          real(csi, c_double), real(eta, c_double), real(zet, c_double), &

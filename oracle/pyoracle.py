@@ -12,7 +12,9 @@ import tempfile
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# DESALM_ORACLE_LIB: another build of the same sources (the sanitizer build
+# of the Makefile's `sanitized` target)
+LIB_PATH = os.environ.get("DESALM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_EXE = os.path.join(HERE, "_ref", "desalm_ref_harness")
 # the same harness with the reference's conservation and wf_mpi linked in
 HOOKS_EXE = os.path.join(HERE, "_ref", "desalm_ref_harness_hooks")
@@ -20,6 +22,8 @@ HOOK_CONSERVATION = 1
 HOOK_WALL_FUNCTIONS = 2
 SHIM_MGD_EXE = os.path.join(HERE, "_ref", "desalm_shim_mgd")
 SHIM_DAF_EXE = os.path.join(HERE, "_ref", "desalm_shim_daf")
+# solver_daf drop-in inside the reference built with its case hooks
+SHIM_DAF_HOOKS_EXE = os.path.join(HERE, "_ref", "desalm_shim_daf_hooks")
 
 _lib = None
 
@@ -327,13 +331,16 @@ def _big_stack():
         pass
 
 
-def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600, exe=None):
+def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iters=None, timeout=600, exe=None,
+                  rotate=False):
     """Run the reference's own Fortran (compiled from its sources by
     oracle/ref/build_ref.sh) on `case`.  The reference is built with three
     multigrid levels (src/global.F90:17); a two-level case runs with iter(3)=0,
     which leaves level 2 untouched by the third level's zero correction.
     exe: another build of the same harness (SHIM_MGD_EXE, SHIM_DAF_EXE: the
     reference with the Fortran drop-in linked in place of one routine).
+    rotate: before every mg_driver call the blade angles advance as the
+    reference's physical time loop does (src/main.F90:42-49).
     Returns a dict of fine-level arrays and all-level metrics, plus the text
     of the 'conver' file under "conver" when the run wrote one and, for
     mode 1, the wall time of the mg_driver calls under "seconds"."""
@@ -367,6 +374,7 @@ def run_reference(case, mode=1, nsteps=1, daf_level=1, recalc=0, calc_pk=0, iter
     hdr[35] = 0 if ov is None else len(ov.coef)
     alm = getattr(case, "alm", None)
     hdr[36] = 0 if alm is None else len(alm.turbines)
+    hdr[37] = 1 if rotate else 0
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")

@@ -97,5 +97,11 @@ if [ -f "$LIBDIR/libdesalm.so" ]; then
         "$OUT/obj/shim_mg_driver.o" "${LIBS[@]}"
     "$FC" -o "$OUT/desalm_shim_daf" ${REFOBJ/ $OUT\/obj\/solver_daf.o/} "$OUT/obj/shim_desalm_f.o" \
         "$OUT/obj/shim_solver_daf.o" "${LIBS[@]}"
-    echo "built $OUT/desalm_shim_mgd $OUT/desalm_shim_daf"
+    # solver_daf swapped inside the reference built with its own case hooks
+    # (conservation, wf_mpi): the reference's mg_driver runs the hooks, the
+    # drop-in the smoother passes
+    HOOKOBJ="${REFOBJ/ $OUT\/obj\/hooks_off.o/}"
+    "$FC" -o "$OUT/desalm_shim_daf_hooks" ${HOOKOBJ/ $OUT\/obj\/solver_daf.o/} "$OUT/obj_hooks/wf_mpi.o" \
+        "$OUT/obj/conservation.o" "$OUT/obj/shim_desalm_f.o" "$OUT/obj/shim_solver_daf.o" "${LIBS[@]}"
+    echo "built $OUT/desalm_shim_mgd $OUT/desalm_shim_daf $OUT/desalm_shim_daf_hooks"
 fi

@@ -13,7 +13,10 @@
 ! input  (stream, native endian):
 !   int32 hdr(40): mode, unused, img, jmg, kmg, icrs, jcrs, kcrs, iter(1:3),
 !                  itm, it_min, unsteady, des, itk, btype(1:6), bdir(1:6),
-!                  ifix, jfix, kfix, nsteps, daf_level, recalc, calc_pk, (pad)
+!                  ifix, jfix, kfix, nsteps, daf_level, recalc, calc_pk, nd, nt,
+!                  rotate (1: before every mg_driver call of mode 1 the blade
+!                  angles advance as the physical time loop does,
+!                  src/main.F90:42-49, dtheta of src/init.F90:1447), (pad)
 !   real64 rhd(20): er_min, eo_min, ren, beta, delti, cfl1, vnn1, cfl2, vnn2,
 !                   pdiss, ep(1,1:3), ep(2,1:3), (pad)
 !   real64 x(n1), y(n1), z(n1), wd(n1), q(5,n1), qn(5,n1), qnm1(5,n1), xnut(n1)
@@ -194,6 +197,12 @@ program desalm_ref_harness
      ! wall time of the mg_driver calls alone, for the CPU baseline
      call system_clock(count = c0, count_rate = crate)
      do istep = 1, nsteps
+        if (hdr(38) /= 0 .and. turbines) then
+           do t = 1, n_tur
+              if (angle(t) > 2 * acos(-1.0_rdf)) angle(t) = angle(t) - 2 * acos(-1.0_rdf)
+              angle(t) = angle(t) + TSR(t) * delti * length_scale / tur_rad(t)
+           end do
+        end if
         call mg_driver
      end do
      call system_clock(count = c1)

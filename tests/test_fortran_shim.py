@@ -62,3 +62,65 @@ def test_shim_solver_daf(gpu, des):
     ref = po.run_reference(c, mode=1, nsteps=1)
     shim = po.run_reference(c, mode=1, nsteps=1, exe=po.SHIM_DAF_EXE)
     check_state(shim, ref)
+
+
+# The reference's own case runs the actuator line (turbines = .true. in its
+# ind3dmg.dat) and mg_driver's two case hooks on every call: the drop-in must
+# carry all three.  The case below has an actuator line, an inflow face, a
+# characteristic outlet and j walls; the harness advances the blade angles
+# before every mg_driver call as the physical time loop does.
+need_hook_shims = pytest.mark.skipif(
+    not (po.hooks_available() and os.path.exists(po.SHIM_MGD_EXE) and os.path.exists(po.SHIM_DAF_HOOKS_EXE)),
+    reason="reference harness with hooks and the Fortran drop-ins not built (oracle/ref/build_ref.sh)")
+
+
+def reference_case():
+    from golden.make_golden import build
+    return build("wavy", dict(ni=21, nj=17, nk=17, ng=3, itm=3, amp=0.02, des=True, iters=(1, 2, 3),
+                              btype=(4, 5, 1, 1, 2, 2), alm=dict(i_rot=9, center=[0.5, 0.5], radius=0.3, hub=0.06)))
+
+
+def scaled_err(a, b):
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    return (np.abs(a - b).max(axis=0) / scale).max()
+
+
+@need_hook_shims
+def test_shim_solver_daf_with_turbines_and_hooks(gpu):
+    """solver_daf swapped inside the reference with its own hooks: the
+    reference's mg_driver runs conservation and wf_eta, the drop-in every
+    smoother pass with the rotating actuator line.  1e-10 as the other shim
+    tests."""
+    c = reference_case()
+    ref = po.run_reference(c, mode=1, nsteps=2, exe=po.HOOKS_EXE, rotate=True)
+    shim = po.run_reference(c, mode=1, nsteps=2, exe=po.SHIM_DAF_HOOKS_EXE, rotate=True)
+    check_state(shim, ref)
+
+
+@need_hook_shims
+def test_shim_mg_driver_with_turbines_and_hooks(gpu):
+    """mg_driver swapped: the library runs the actuator line, conservation
+    and the wall functions.  Against the library driven directly with the
+    same rotors, hooks and blade rotation: 1e-8 of each component's scale
+    (the tolerance of the wall-function tests, acos near +-1); against the
+    reference: 2e-3, the bound of its wall-function Newton chain
+    (|f| < 1e-4, src/wf_mpi.F90:634-850), which the library replaces by an
+    exact root solve.  Without the turbines the state is far outside that
+    bound, so the forcing is what the drop-in carried."""
+    from desalm.lib import HOOK_CONSERVATION, HOOK_WALL_FUNCTIONS, Solver
+    c = reference_case()
+    shim = po.run_reference(c, mode=1, nsteps=2, exe=po.SHIM_MGD_EXE, rotate=True)
+    s = Solver(c.params).load_case(c).set_hooks(HOOK_CONSERVATION | HOOK_WALL_FUNCTIONS)
+    for _ in range(2):
+        s.step()
+    q, qn, qnm1, xnut = s.get_state()
+    s.close()
+    assert scaled_err(shim["q"], q) < 1e-8
+    assert scaled_err(shim["qn"], qn) < 1e-8
+    assert scaled_err(shim["xnut"][:, None], xnut[:, None]) < 1e-8
+    ref = po.run_reference(c, mode=1, nsteps=2, exe=po.HOOKS_EXE, rotate=True)
+    assert scaled_err(shim["q"], ref["q"]) < 2e-3
+    c.alm = None
+    bare = po.run_reference(c, mode=1, nsteps=2, exe=po.HOOKS_EXE)
+    assert scaled_err(bare["q"], ref["q"]) > 1e-2
+    assert np.all(np.isfinite(shim["q"]))

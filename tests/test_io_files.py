@@ -553,3 +553,12 @@ def test_damaged_files_raise_instead_of_crashing(tmp_path):
     f.write_bytes(struct.pack("<i", 2 ** 31 - 9) + b"\0" * 16)
     with pytest.raises(DesalmError):
         files.read_grid(f, dims)
+    # an empty first record fits neither REAL*4 nor REAL*8: refused, in the
+    # grid and in the solution reader (both detect the real size from it)
+    f.write_bytes(record(b"") + good)
+    with pytest.raises(DesalmError, match="expected 36 reals"):
+        files.read_grid(f, dims)
+    with pytest.raises(DesalmError, match="expected 36 reals"):
+        files.read_grid(f, dims, turbulence=False)
+    with pytest.raises(DesalmError, match="expected 36 reals"):
+        files.read_solu(f, dims, turbulence=True)
