@@ -1,19 +1,23 @@
 """Throughput of the pseudo-time inner iteration on one MI355X.
 
-Workload (default): turbulent channel 129x129x129, SA/DES eddy viscosity
-(itk = 2), 3-level FAS V-cycle (129 -> 65 -> 33, iter = 1, 2, 3 smoother
-passes), BDF2 dual time stepping, FP64 plain.  A timed step is one physical
-time step: one desalm_mg_driver call of `--iters` pseudo-iterations (no early
-exit, each pseudo-iteration one HIP graph replay), inputs resident in HBM.
+Workload (default, the one the metric is quoted on): SA-DES dual time
+stepping of a uniform inflow U = 1 past one actuator line (3 blades, radius
+0.5, tip-speed ratio 6) on a 193x129x129 stretched box, dt = 0.01, 3-level
+FAS V-cycle (iter = 1, 2, 3 smoother passes), SA itk = 2, FP64 plain.  A
+timed step is one physical time step (desalm_step: the blades advance, then
+`--iters` pseudo-iterations, no early exit, each pseudo-iteration one HIP
+graph replay), inputs resident in HBM.
 value = fine nodes x pseudo-iterations / wall second (Mcell-iters/s).
 
 The roofline object is computed live for the dominant kernel family from
 HIP events recorded by the library on its stream around every launch during
 a second, profiled pass of the same timed steps (launches then run
-eagerly): achieved GB/s = algorithmic bytes per launch / mean launch time.  Algorithmic
-bytes count each compulsory FP64 array read or written once per node of the
-level (scratch excluded).  The CPU baseline times the C oracle (a
-single-core restatement of the reference algorithm) on a bounded sample.
+eagerly): achieved GB/s = algorithmic bytes per launch / mean launch time.
+Algorithmic bytes count each compulsory FP64 array read or written once per
+node of the level (scratch excluded).  The CPU baseline times the
+reference's own Fortran (built from its sources, when oracle/_ref is
+present) or else the C restatement, on a bounded sample of the same
+workload.
 """
 import argparse
 import json
@@ -59,17 +63,22 @@ def level_words(name, n, itk=2):
 
 # kernel-name prefix -> family, for the fine-level dispatches of a counter run
 # (k_prep, k_residual and k_visc_pk are told apart by their place in the pass)
-FAMILY_OF = [("k_sa_", "des_eddy"), ("k_daf_pre<0", "daf_i"), ("k_daf_tri_i", "daf_i"), ("k_daf_pre<1", "daf_j"),
-             ("k_daf_tri<1", "daf_j"), ("k_daf_pre<2", "daf_k"), ("k_daf_tri<2", "daf_k"), ("k_daf_post", "daf_k"),
-             ("k_prolong_fill", "prolong"), ("k_rs", "prolong"), ("k_resid_", "resid_norm"),
-             ("k_bc_fm", "bcond_fm"), ("k_pfix_", "bcond_fm")]
+FAMILY_OF = [("k_sa_", "des_eddy"), ("k_daf_sweep<0", "daf_i"), ("k_daf_sweep<1", "daf_j"),
+             ("k_daf_sweep<2", "daf_k"), ("k_prolong_fill", "prolong"), ("k_rs", "prolong"),
+             ("k_resid_", "resid_norm"), ("k_bc_fm", "bcond_fm"), ("k_pfix_", "bcond_fm")]
 # one dispatch per family call
-FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_tri_i", "daf_j": "k_daf_tri<1", "daf_k": "k_daf_post",
+FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_sweep<0", "daf_j": "k_daf_sweep<1", "daf_k": "k_daf_sweep<2",
                "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_pfix_sub",
                "dtau": "k_prep", "residual": "k_residual", "residual_recalc": "k_residual"}
 
+HEADLINE_DIMS = (193, 129, 129)
 
-def pmc_summary(fetch_dir, write_dir, n):
+
+def workload_name(dims):
+    return "actuator-line box %dx%dx%d" % tuple(dims)
+
+
+def pmc_summary(fetch_dir, write_dir, dims):
     """HBM bytes per fine-level launch of every kernel family from two
     rocprofv3 counter runs of the same command (FETCH_SIZE, WRITE_SIZE, KiB
     per dispatch).  Each counter is scaled by what it reads for k_copy_qold,
@@ -96,7 +105,7 @@ def pmc_summary(fetch_dir, write_dir, n):
     fe, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
     if [(a[1], a[2]) for a in fe] != [(b[1], b[2]) for b in wr]:
         raise SystemExit("the two counter runs launched different kernel sequences")
-    n1 = n ** 3
+    n1 = dims[0] * dims[1] * dims[2]
     copy_kib = 5 * n1 * 8 / 1024.0
     cf = [v for _, nm, _, v in fe if nm == "k_copy_qold"]
     cw = [v for _, nm, _, v in wr if nm == "k_copy_qold"]
@@ -117,7 +126,7 @@ def pmc_summary(fetch_dir, write_dir, n):
             f = ("residual_recalc" if after_daf else ("dtau" if nm == "k_prep" else "residual")) if fine else None
         elif f not in ("bcond_fm",) and not nm.startswith("__amd"):
             after_daf = False
-        if nm.startswith("k_daf_post") and fine:
+        if nm.startswith("k_daf_sweep<2") and fine:
             after_daf = True
         if f is None or not (fine or nm == "k_resid_final"):
             continue
@@ -136,7 +145,7 @@ def pmc_summary(fetch_dir, write_dir, n):
         fams[f"{f}@1"] = {"hbm_bytes_per_launch": int(round((e["read"] + e["write"]) / c)),
                           "read_bytes": int(round(e["read"] / c)), "write_bytes": int(round(e["write"] / c)),
                           "calls": e["calls"]}
-    out = {"workload_n": n, "calibration": {"kernel": "k_copy_qold", "bytes_each_way": 5 * n1 * 8,
+    out = {"workload": workload_name(dims), "calibration": {"kernel": "k_copy_qold", "bytes_each_way": 5 * n1 * 8,
                                             "fetch_scale": round(sf, 4), "write_scale": round(sw, 4)},
            "families": fams,
            "kernels": {k: {"dispatches": c, "read_bytes": int(r / c), "write_bytes": int(w / c)}
@@ -164,19 +173,20 @@ def dual_time_leg(args):
     """Physical time steps of SA-DES dual time stepping on the actuator-line
     box: the plain FP64 method (every step starts from q^n, as the
     reference) against extrapolation and the learned warm start, FP64 and
-    mixed precision.  Every start iterates to the same absolute tolerance:
-    `dt_drop` decades below the first update norm of the plain start on the
-    first compared step (mg_ck_conver's absolute test, eo_min), at most
-    `conv_itm` pseudo-iterations.  The 1e-8 drop of the workload is reported
-    as reached or not, with the plain method's tail rate projecting the
-    pseudo-iterations it would take."""
+    mixed precision.  For each of a ladder of tolerances -- d decades below
+    the plain start's first update norm (max over u, v, w of log10 mean
+    |dq|, mg_ck_conver's absolute test eo_min), d in --dt-drops -- every form
+    iterates from the same saved state to that tolerance, at most
+    `conv_itm` pseudo-iterations, and its wall time per physical step is
+    recorded.  The 1e-8 drop of the workload is the ladder's 8-decade rung."""
     import numpy as np
     from desalm import cases
     from desalm.lib import Solver
-    from desalm.params import PREC_FP64, PREC_MIXED
+    from desalm.params import PREC_MIXED
 
     ni, nj, nk = (int(v) for v in args.dt_grid.split(","))
     cap = args.conv_itm
+    drops = [float(d) for d in args.dt_drops.split(",")]
     t_build = time.perf_counter()
     case = cases.alm_box(ni, nj, nk, ng=3, itm=cap, it_min=1, iters=(1, 2, 3))
     case_mixed = cases.alm_box(ni, nj, nk, ng=3, itm=cap, it_min=1, iters=(1, 2, 3), precision=PREC_MIXED)
@@ -188,15 +198,15 @@ def dual_time_leg(args):
             sv.autotune(reps=3)
             sv.set_state(*cc.state())
         sv.ws_configure(hidden=32, samples_per_record=65536, capacity=1 << 22, seed=20260505)
-    # warm-up history (3 plain steps at the cap's first 30 iterations), then
-    # the training steps, each recorded after it converged
+    # warm-up history (3 plain steps of 30 pseudo-iterations), then the
+    # training steps, each recorded after it converged to the first rung
     sp.set_convergence(30, 30, -99.0, -99.0)
     for _ in range(3):
         sp.step()
     itc0, h0 = sp.step()
     r0 = float(h0[0, 1:4].max())
-    tol = r0 - args.dt_drop
-    sp.set_convergence(cap, 1, -99.0, tol)
+    tol_train = r0 - args.dt_drop
+    sp.set_convergence(cap, 1, -99.0, tol_train)
     t_gen = time.perf_counter()
     gen_itc = []
     for _ in range(args.dt_train_steps):
@@ -212,29 +222,37 @@ def dual_time_leg(args):
 
     forms = [("fp64_plain", sp, Solver.WS_NONE), ("fp64_extrap", sp, Solver.WS_EXTRAP),
              ("fp64_learned", sp, Solver.WS_LEARNED), ("mixed_learned", sm, Solver.WS_LEARNED)]
-    res = {f: {"itc": [], "wall_s": [], "first": [], "reached": []} for f, _, _ in forms}
+    res = {d: {f: {"itc": [], "wall_s": [], "first": [], "reached": [], "last": []} for f, _, _ in forms}
+           for d in drops}
+    fallbacks = {f: 0 for f, _, _ in forms}
     tail = []
     for _ in range(args.dt_eval_steps):
         snap = sp.snapshot()
         after = None
-        for name, sv, mode in forms:
-            sv.restore(snap)
-            sv.ws_mode(mode)
-            sv.set_convergence(cap, 1, -99.0, tol)
-            t0 = time.perf_counter()
-            itc, h = sv.step()
-            wall = time.perf_counter() - t0
-            d = h[:itc, 1:4].max(axis=1)
-            r = res[name]
-            r["itc"].append(int(itc))
-            r["wall_s"].append(wall)
-            r["first"].append(float(d[0]))
-            r["reached"].append(bool(d[-1] < tol))
-            if name == "fp64_plain":
-                after = sp.snapshot()
-                if itc >= 20:
+        for d in drops:
+            tol = r0 - d
+            for name, sv, mode in forms:
+                sv.restore(snap)
+                sv.ws_mode(mode)
+                sv.set_convergence(cap, 1, -99.0, tol)
+                fb0 = sv.last_status()["fallbacks"]
+                t0 = time.perf_counter()
+                itc, h = sv.step()
+                wall = time.perf_counter() - t0
+                fallbacks[name] += sv.last_status()["fallbacks"] - fb0
+                dd = h[:itc, 1:4].max(axis=1)
+                r = res[d][name]
+                r["itc"].append(int(itc))
+                r["wall_s"].append(wall)
+                r["first"].append(float(dd[0]))
+                r["last"].append(float(dd[-1]))
+                r["reached"].append(bool(dd[-1] < tol))
+                if name == "fp64_plain" and d == drops[0]:
+                    after = sp.snapshot()
+                if name == "fp64_plain" and d == drops[-1] and itc >= 20:
                     half = itc // 2
-                    tail.append(float((d[-1] - d[half]) / (itc - 1 - half)))
+                    tail.append(float((dd[-1] - dd[half]) / (itc - 1 - half)))
+        # the trajectory continues from the plain step at the first rung
         sp.restore(after)
         sp.ws_mode(Solver.WS_NONE)
 
@@ -242,25 +260,33 @@ def dual_time_leg(args):
         return {"pseudo_iterations_mean": round(float(np.mean(r["itc"])), 2),
                 "wall_s_per_step": round(float(np.mean(r["wall_s"])), 4),
                 "first_log10_norm_mean": round(float(np.mean(r["first"])), 3),
+                "last_log10_norm_mean": round(float(np.mean(r["last"])), 3),
                 "all_reached": all(r["reached"]), "pseudo_iterations": r["itc"]}
 
+    ladder = []
+    for d in drops:
+        fs = {f: summary(r) for f, r in res[d].items()}
+        wp, wm = fs["fp64_plain"]["wall_s_per_step"], fs["mixed_learned"]["wall_s_per_step"]
+        ladder.append({"decades_below_plain_start": d, "log10_norm": round(r0 - d, 3), "forms": fs,
+                       "speedup_mixed_learned_vs_fp64_plain": round(wp / wm, 3) if wm > 0 else None,
+                       "speedup_valid": fs["fp64_plain"]["all_reached"] and fs["mixed_learned"]["all_reached"]})
+    reached_all = [e["decades_below_plain_start"] for e in ladder
+                   if all(f["all_reached"] for f in e["forms"].values())]
+    rate = float(np.mean(tail)) if tail else None
+    rung8 = next((e for e in ladder if e["decades_below_plain_start"] >= 8.0), None)
     out = {"workload": f"actuator-line box {ni}x{nj}x{nk}, SA-DES, dt {case.params.delti}, 3 levels, "
                        f"rotor 3 blades R 0.5 TSR 6",
-           "tolerance": {"log10_norm": round(tol, 3), "decades_below_plain_start": args.dt_drop,
-                         "cap": cap, "test": "max over u, v, w of log10 mean |dq| < tolerance (eo_min)"},
-           "forms": {f: summary(r) for f, r in res.items()}}
-    wp = out["forms"]["fp64_plain"]["wall_s_per_step"]
-    wm = out["forms"]["mixed_learned"]["wall_s_per_step"]
-    out["speedup_mixed_learned_vs_fp64_plain"] = round(wp / wm, 3) if wm > 0 else None
-    rate = float(np.mean(tail)) if tail else None
-    out["target_1e-8_drop"] = {
-        "reached": False if rate is None or rate >= 0 else None,
-        "plain_tail_rate_decades_per_iter": None if rate is None else round(rate, 5),
-        "projected_pseudo_iterations": (None if rate is None or rate >= 0 else
-                                        int(round(8.0 / -rate)))}
-    if out["target_1e-8_drop"]["reached"] is None:
-        out["target_1e-8_drop"]["reached"] = bool(args.dt_drop >= 8.0 and
-                                                   out["forms"]["fp64_plain"]["all_reached"])
+           "plain_start_log10_norm": round(r0, 3), "cap": cap,
+           "test": "max over u, v, w of log10 mean |dq| < tolerance (eo_min of mg_ck_conver)",
+           "ladder": ladder,
+           "deepest_drop_all_forms_reached": max(reached_all) if reached_all else None,
+           "fallbacks_to_plain": fallbacks,
+           "target_1e-8_drop": {
+               "reached_by": None if rung8 is None else
+               [f for f, v in rung8["forms"].items() if v["all_reached"]],
+               "plain_tail_rate_decades_per_iter": None if rate is None else round(rate, 5),
+               "plain_last_log10_norm_at_cap": None if rung8 is None else
+               rung8["forms"]["fp64_plain"]["last_log10_norm_mean"]}}
     out["training"] = {"steps": args.dt_train_steps, "mean_pseudo_iterations": round(float(np.mean(gen_itc)), 2),
                        "samples": int(min(args.dt_train_steps * 65536, 1 << 22)), "adam_steps": args.dt_train_iters,
                        "loss_first": round(float(loss[:20].mean()), 5), "loss_last": round(float(loss[-20:].mean()), 5),
@@ -274,7 +300,8 @@ def dual_time_leg(args):
 def large_leg(args):
     """The large-grid DES workload: 257x193x193 actuator-line box, 4-level
     V-cycle, random-flow-generator inflow (5% intensity, integral length
-    0.1), FP32-DAF mixed precision, autotuned tiles, one HIP graph per
+    0.1), FP32-DAF mixed precision with the learned warm start (trained on
+    the GPU on plain steps of this run), autotuned tiles, one HIP graph per
     pseudo-iteration.  Reports Mcell-iters/s, the device memory the context
     holds, and per kernel family on the fine level its algorithmic GB/s
     (bench.py's words per node over the profiled launch time)."""
@@ -288,6 +315,17 @@ def large_leg(args):
     tiles = sv.autotune(reps=3)
     sv.set_state(*case.state())
     sv.set_rfg(rfg.modes(1000, 20260606), rfg.inflow_tables(nj, nk, intensity=0.05, length=0.1))
+    sv.ws_configure(hidden=32, samples_per_record=65536, capacity=1 << 21, seed=20260707)
+    # history, then recorded plain steps of 20 pseudo-iterations, training
+    sv.set_convergence(20, 20, -99.0, -99.0)
+    for _ in range(3):
+        sv.step()
+    for _ in range(args.large_train_steps):
+        sv.step()
+        sv.ws_record()
+    loss = sv.ws_train(400, batch=16384, lr=2e-3)
+    sv.ws_mode(Solver.WS_LEARNED)
+    sv.set_convergence(args.iters, args.iters, -99.0, -99.0)
     sv.step()
     done = 0
     t0 = time.perf_counter()
@@ -304,14 +342,66 @@ def large_leg(args):
     for (nm, lv), (cnt, ms) in st.items():
         if lv == 1 and nm in WORDS:
             b = level_words(nm, 1, case.params.itk) * 8 * n1
-            per_family[nm] = {"mean_launch_ms": round(ms / cnt, 4), "algorithmic_GBps": round(b / (ms / cnt * 1e-3) / 1e9, 1)}
-    out = {"workload": f"actuator-line box {ni}x{nj}x{nk}, SA-DES, 4 levels, RFG inflow, FP32-DAF mixed",
+            per_family[nm] = {"mean_launch_ms": round(ms / cnt, 4),
+                              "algorithmic_GBps": round(b / (ms / cnt * 1e-3) / 1e9, 1)}
+    out = {"workload": f"actuator-line box {ni}x{nj}x{nk}, SA-DES, 4 levels, RFG inflow, FP32-DAF mixed, "
+                       f"learned warm start",
            "value": round(n1 * done / (t1 - t0) / 1e6, 3), "unit": "Mcell-iters/s",
            "ms_per_step": round((t1 - t0) / args.steps * 1e3, 3), "pseudo_iterations": done,
            "device_bytes": sv.device_bytes, "fine_nodes": n1, "nodes_all_levels": case.params.nodes_all(),
+           "warm_start": {"train_steps": args.large_train_steps, "adam_steps": 400,
+                          "loss_first": round(float(loss[:20].mean()), 5),
+                          "loss_last": round(float(loss[-20:].mean()), 5),
+                          "fallbacks_to_plain": sv.last_status()["fallbacks"]},
            "tiles": {"visc_by_level": tiles[0], "sa_rhs": tiles[1]}, "fine_families": per_family}
     sv.close()
     return out
+
+
+def cpu_baseline(args, dims):
+    """The reference's own mg_driver (oracle/_ref, its Fortran built FP64 from
+    its sources, one rank, timed inside the program) and the C restatement
+    (oracle, one thread) on a bounded sample of the headline workload.  The
+    reference is the baseline when it is present; a missing build is said
+    on stderr and in the record."""
+    import pyoracle as po
+    from desalm import cases
+
+    ni, nj, nk = dims
+    n1 = ni * nj * nk
+    cc = cases.alm_box(ni, nj, nk, ng=3, itm=args.cpu_iters, it_min=args.cpu_iters, iters=(1, 2, 3))
+    o = po.Oracle(cc.params).load_case(cc)
+    tc0 = time.perf_counter()
+    itc_c, _ = o.mg_driver()
+    tc1 = time.perf_counter()
+    del o
+    port = {"value": round(n1 * itc_c / (tc1 - tc0) / 1e6, 4), "unit": "Mcell-iters/s", "cores": 1, "kind": "port",
+            "sample": f"{itc_c} pseudo-iterations of the same {workload_name(dims)} V-cycle, C restatement, "
+                      f"1 thread ({tc1 - tc0:.1f} s)"}
+    why = None
+    if args.ref_iters <= 0:
+        why = "--ref-iters 0"
+    elif not po.ref_available():
+        why = "oracle/_ref/desalm_ref_harness is not built (oracle/ref/build_ref.sh needs the reference sources)"
+    else:
+        cr = cases.alm_box(ni, nj, nk, ng=3, itm=args.ref_iters, it_min=args.ref_iters, iters=(1, 2, 3))
+        try:
+            r = po.run_reference(cr, mode=1, nsteps=1, timeout=900)
+        except Exception as e:
+            r = {"seconds": None}
+            why = "the reference run failed: " + str(e)[:200]
+        if r.get("seconds"):
+            return {"value": round(n1 * args.ref_iters / r["seconds"] / 1e6, 4), "unit": "Mcell-iters/s",
+                    "cores": 1, "kind": "reference",
+                    "sample": f"{args.ref_iters} pseudo-iterations of the same {workload_name(dims)} V-cycle, the "
+                              f"reference's mg_driver (its Fortran sources built FP64 with amdflang -O2), one rank "
+                              f"({r['seconds']:.1f} s inside the program)",
+                    "port": port, "reference_over_port_time": round(
+                        (r["seconds"] / args.ref_iters) / ((tc1 - tc0) / itc_c), 3)}
+        why = why or "the reference run reported no time"
+    print(f"bench.py: CPU baseline is the C restatement, not the reference: {why}", file=sys.stderr)
+    port["why_not_reference"] = why
+    return port
 
 
 def main():
@@ -320,33 +410,38 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters", type=int, default=10, help="pseudo-iterations per physical step")
-    ap.add_argument("--n", type=int, default=129, help="nodes per direction")
+    ap.add_argument("--channel-n", type=int, default=129, help="nodes per direction of the channel extra")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--large", action="store_true",
-                    help="also run the 257x193x193 large-grid DES workload (several GB of device memory)")
+    ap.add_argument("--no-large", dest="large", action="store_false",
+                    help="skip the 257x193x193 large-grid DES workload")
+    ap.add_argument("--large-train-steps", type=int, default=12,
+                    help="plain steps recorded for the large leg's warm start")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes of the LDS-tiled kernels")
     ap.add_argument("--tiles", default=None, metavar="V1,V2,V3:S",
                     help="fix the launch shapes instead of autotuning: k planes per workgroup of k_visc_pk "
                          "per level and of k_sa_rhs (e.g. 8,4,1:8, as a previous run's 'tiles' reported)")
-    ap.add_argument("--cpu-iters", type=int, default=6)
-    ap.add_argument("--ref-iters", type=int, default=4,
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--ref-iters", type=int, default=2,
                     help="pseudo-iterations of the reference Fortran timed for the CPU baseline (0: port only)")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
-    ap.add_argument("--conv-itm", type=int, default=120,
+    ap.add_argument("--no-extras", action="store_true", help="skip the mixed-precision and channel extras")
+    ap.add_argument("--conv-itm", type=int, default=300,
                     help="pseudo-iteration cap per physical step of the dual-time leg (0 skips it)")
     ap.add_argument("--dt-grid", default="193,129,129", help="dual-time leg grid")
+    ap.add_argument("--dt-drops", default="2,3,4,8",
+                    help="tolerance ladder of the dual-time leg: decades below the plain start's first norm")
     ap.add_argument("--dt-train-steps", type=int, default=40,
                     help="plain physical steps recorded as training samples for the warm start")
-    ap.add_argument("--dt-eval-steps", type=int, default=4, help="physical steps compared after training")
+    ap.add_argument("--dt-eval-steps", type=int, default=3, help="physical steps compared after training")
     ap.add_argument("--dt-drop", type=float, default=2.0,
-                    help="decades below the plain start's first update norm that every start iterates to")
+                    help="decades below the plain start's first norm the training steps converge to")
     ap.add_argument("--dt-train-iters", type=int, default=1500, help="Adam steps of the warm-start network")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="no run: summarise two rocprofv3 counter runs into profiles/traffic.json")
+                    help="no run: summarise two rocprofv3 counter runs of the headline workload into "
+                         "profiles/traffic.json")
     args = ap.parse_args()
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.n)
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], HEADLINE_DIMS)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -358,57 +453,78 @@ def main():
 
     from desalm.params import PREC_FP64, PREC_MIXED
 
-    tiles_used = {}
     fixed_tiles = None
     if args.tiles:
         v, _, sa = args.tiles.partition(":")
         fixed_tiles = ([int(t) for t in v.split(",") if t], int(sa or 0))
         args.autotune = False
 
-    def timed_run(precision, profile):
-        """warmup + timed physical steps; returns (Mcell-iters/s, wall s, iterations, profile stats,
-        last residual history, state finite, pivot events)"""
-        cs = cases.channel(args.n, args.n, args.n, ng=3, itm=args.iters, iters=(1, 2, 3), precision=precision)
+    def timed_run(make_case, precision, profile, physical):
+        """warmup + timed physical steps of --iters pseudo-iterations each;
+        returns a dict (Mcell-iters/s, wall s, iterations, profile stats, last
+        norm history, state finite, pivot events, params, tiles)"""
+        cs = make_case(precision)
         sv = Solver(cs.params, device=0).load_case(cs)
         if args.autotune:
             # launch shapes of the LDS-tiled kernels, timed on this state
             # (scratch only), then the state is reloaded
-            tiles_used[precision] = sv.autotune(reps=3)
+            tiles = sv.autotune(reps=3)
             sv.set_state(*cs.state())
         else:
             if fixed_tiles:
                 sv.set_tiles(*fixed_tiles)
-            tiles_used[precision] = sv.get_tiles()
+            tiles = sv.get_tiles()
+        run = sv.step if physical else sv.mg_driver
         for _ in range(args.warmup):
-            sv.mg_driver()
+            run()
         if profile:
             sv.profile(True)
         done = 0
         ta = time.perf_counter()
         for _ in range(args.steps):
-            itc, hist = sv.mg_driver()
+            itc, hist = run()
             done += itc
         tb = time.perf_counter()
         st = sv.profile_stats() if profile else None
         if profile:
             sv.profile(False)
-        fin = bool(np.all(np.isfinite(sv.get_state()[0])))
-        piv = sv.pivot_events
+        r = {"value": cs.n1 * done / (tb - ta) / 1e6, "wall": tb - ta, "done": done, "stats": st, "hist": hist,
+             "finite": bool(np.all(np.isfinite(sv.get_state()[0]))), "pivots": sv.pivot_events,
+             "params": cs.params, "tiles": tiles}
         sv.close()
-        return cs.n1 * done / (tb - ta) / 1e6, tb - ta, done, st, hist, fin, piv, cs.params
+        return r
+
+    ni, nj, nk = HEADLINE_DIMS
+
+    def alm(precision):
+        return cases.alm_box(ni, nj, nk, ng=3, itm=args.iters, it_min=args.iters, iters=(1, 2, 3),
+                             precision=precision)
+
+    def channel(precision):
+        n = args.channel_n
+        return cases.channel(n, n, n, ng=3, itm=args.iters, it_min=args.iters, iters=(1, 2, 3), precision=precision)
 
     # production path: every pseudo-iteration is one HIP graph replay
-    value, wall, iters_done, _, hist, finite, pivots, p = timed_run(PREC_FP64, False)
+    head = timed_run(alm, PREC_FP64, False, True)
+    p = head["params"]
     n1 = p.img * p.jmg * p.kmg
     # the same workload with the library's per-launch HIP-event profiler
     # (launches then run eagerly): per-kernel-family device times
-    value_prof, _, _, stats, _, _, _, _ = timed_run(PREC_FP64, True)
-    mixed = None
-    if not args.no_mixed:
-        v_mixed, wall_mixed, _, _, _, fin_mixed, _, _ = timed_run(PREC_MIXED, False)
-        mixed = {"value": round(v_mixed, 3), "unit": "Mcell-iters/s",
-                 "ms_per_step": round(wall_mixed / args.steps * 1e3, 3), "state_finite": fin_mixed,
-                 "dtype": "f64 residual, f32 DAF sweeps"}
+    prof = timed_run(alm, PREC_FP64, True, True)
+    stats = prof["stats"]
+    extras = None
+    if not args.no_extras:
+        mx = timed_run(alm, PREC_MIXED, False, True)
+        ch = timed_run(channel, PREC_FP64, False, False)
+        chm = timed_run(channel, PREC_MIXED, False, False)
+        extras = {"mixed": {"value": round(mx["value"], 3), "unit": "Mcell-iters/s",
+                            "ms_per_step": round(mx["wall"] / args.steps * 1e3, 3), "state_finite": mx["finite"],
+                            "dtype": "f64 residual, f32 DAF sweeps"},
+                  "channel": {"workload": f"channel {args.channel_n}^3 SA, 3-level V-cycle, periodic-copy faces "
+                                          f"(the reference's btype 6), {args.iters} pseudo-iterations per step",
+                              "value_fp64": round(ch["value"], 3), "value_mixed": round(chm["value"], 3),
+                              "unit": "Mcell-iters/s", "ms_per_step_fp64": round(ch["wall"] / args.steps * 1e3, 3),
+                              "state_finite": ch["finite"] and chm["finite"]}}
 
     # dominant kernel family by total device time
     fam = {}
@@ -417,9 +533,8 @@ def main():
         fam[nm][0] += cnt
         fam[nm][1] += ms
     dominant = max((k for k in fam if k in WORDS and (k, 1) in stats), key=lambda k: fam[k][1])
-    dims = p.level_dims()
     cnt1, ms1 = stats[(dominant, 1)]
-    bytes_per_launch = level_words(dominant, 1, p.itk) * 8 * dims[0][0] * dims[0][1] * dims[0][2]
+    bytes_per_launch = level_words(dominant, 1, p.itk) * 8 * n1
     achieved = bytes_per_launch / (ms1 / cnt1 * 1e-3) / 1e9
     # HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE,
     # WRITE_SIZE, scaled by a plain copy of known size), if recorded for
@@ -430,68 +545,43 @@ def main():
         with open(tpath) as f:
             tr = json.load(f)
         ent = tr.get("families", {}).get(f"{dominant}@1")
-        if ent and tr.get("workload_n") == args.n:
+        if ent and tr.get("workload") == workload_name(HEADLINE_DIMS):
             traffic = ent["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "kernel": f"{dominant} (fine level)", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "bytes_per_launch": bytes_per_launch,
                 "mean_launch_ms": round(ms1 / cnt1, 4),
                 "timing": "HIP events around each launch of the family on the library stream, profiled pass "
-                          f"of the same workload ({round(value_prof, 3)} Mcell-iters/s with eager launches)"}
+                          f"of the same workload ({round(prof['value'], 3)} Mcell-iters/s with eager launches)"}
+    # every fine-level family beside it (algorithmic GB/s and roofline fraction)
+    families = {}
+    for (nm, lv), (cnt, ms) in sorted(stats.items()):
+        if lv == 1 and nm in WORDS:
+            b = level_words(nm, 1, p.itk) * 8 * n1
+            gbs = b / (ms / cnt * 1e-3) / 1e9
+            families[nm] = {"mean_launch_ms": round(ms / cnt, 4), "algorithmic_GBps": round(gbs, 1),
+                            "frac": round(gbs / HBM_PEAK_GBS, 3)}
 
-    # dual-time leg: the actuator-line box (193 x 129 x 129, dt = 0.01),
-    # physical steps to one absolute tolerance for every start
-    conv = None
-    if args.conv_itm > 0:
-        conv = dual_time_leg(args)
+    conv = dual_time_leg(args) if args.conv_itm > 0 else None
+    large = large_leg(args) if args.large else None
+    cpu = None if args.no_cpu else cpu_baseline(args, HEADLINE_DIMS)
 
-    cpu = None
-    if not args.no_cpu:
-        import pyoracle as po
-        cc = cases.channel(args.n, args.n, args.n, ng=3, itm=args.cpu_iters, iters=(1, 2, 3))
-        o = po.Oracle(cc.params).load_case(cc)
-        tc0 = time.perf_counter()
-        itc_c, _ = o.mg_driver()
-        tc1 = time.perf_counter()
-        cpu = {"value": round(n1 * itc_c / (tc1 - tc0) / 1e6, 4), "unit": "Mcell-iters/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{itc_c} pseudo-iterations of the same {args.n}^3 channel V-cycle, C oracle, 1 thread "
-                         f"({tc1 - tc0:.1f} s)"}
-        # the reference's own Fortran (built from its sources in FP64,
-        # oracle/_ref), when present: timed inside the program around its
-        # mg_driver calls, one rank
-        if po.ref_available() and args.ref_iters > 0:
-            cr = cases.channel(args.n, args.n, args.n, ng=3, itm=args.ref_iters, it_min=args.ref_iters,
-                               iters=(1, 2, 3))
-            try:
-                r = po.run_reference(cr, mode=1, nsteps=1, timeout=900)
-            except Exception as e:  # keep the port figure if the reference cannot run here
-                r = {"seconds": None, "error": str(e)[:200]}
-            if r.get("seconds"):
-                port = dict(cpu)
-                cpu = {"value": round(n1 * args.ref_iters / r["seconds"] / 1e6, 4), "unit": "Mcell-iters/s",
-                       "cores": 1, "kind": "reference",
-                       "sample": f"{args.ref_iters} pseudo-iterations of the same {args.n}^3 channel V-cycle, the "
-                                 f"reference's mg_driver (its Fortran sources built FP64 with amdflang -O2), one "
-                                 f"rank ({r['seconds']:.1f} s inside the program)",
-                       "port": port, "reference_over_port_time": round(
-                           (r["seconds"] / args.ref_iters) / ((tc1 - tc0) / itc_c), 3)}
-
-    out = {"metric": METRIC, "value": round(value, 3), "unit": "Mcell-iters/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
+    out = {"metric": METRIC, "value": round(head["value"], 3), "unit": "Mcell-iters/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["wall"] / args.steps * 1e3, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic",
-           "config": {"workload": f"channel {args.n}^3 SA/DES, 3-level FAS V-cycle (iter 1,2,3), "
-                                  f"BDF2 dual time, FP64 plain, {args.iters} pseudo-iterations per step",
-                      "fine_nodes": n1, "pseudo_iterations": iters_done, "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "converge": conv,
-           "large": large_leg(args) if args.large else None,
+           "config": {"workload": f"{workload_name(HEADLINE_DIMS)} SA-DES, actuator line 3 blades R 0.5 TSR 6, "
+                                  f"dt 0.01, 3-level FAS V-cycle (iter 1,2,3), FP64 plain, {args.iters} "
+                                  f"pseudo-iterations per physical step",
+                      "fine_nodes": n1, "pseudo_iterations": head["done"], "parallelism": "single GPU"},
+           "roofline": roofline, "fine_families": families, "cpu_baseline": cpu, "extras": extras,
+           "converge": conv, "large": large,
            "kernels_ms": {f"{nm}@L{n}": [cnt, round(ms, 3)] for (nm, n), (cnt, ms) in sorted(stats.items())},
-           "tiles": {"k_planes_per_workgroup_visc_by_level": tiles_used[PREC_FP64][0],
-                     "k_planes_per_workgroup_sa_rhs": tiles_used[PREC_FP64][1],
+           "tiles": {"k_planes_per_workgroup_visc_by_level": head["tiles"][0],
+                     "k_planes_per_workgroup_sa_rhs": head["tiles"][1],
                      "chosen_by": "desalm_autotune" if args.autotune else ("--tiles" if fixed_tiles else "default")},
-           "state_finite": finite, "pivot_events": pivots,
-           "final_log10_update_norms": [round(float(v), 4) for v in hist[-1]]}
+           "state_finite": head["finite"], "pivot_events": head["pivots"],
+           "final_log10_update_norms": [round(float(v), 4) for v in head["hist"][-1]]}
     print(json.dumps(out))
 
 

@@ -111,9 +111,18 @@ struct desalm_ctx {
     // read the grid coordinates, so they need desalm_set_grid
     int hooks = 0;
     bool have_grid = false;
+    // failure detection of the last driver call (desalm_last_status) and the
+    // entry state a non-plain form falls back from (qnm1 5, q^(n-2) 5, xnut)
+    int st_zero_pivot = 0, st_nonfinite = 0, st_diverged = 0;
+    long long fallbacks = 0;
+    double* fb_save = nullptr;
 };
 
 namespace {
+
+// growth of the largest u, v, w update norm over the first iteration's, in
+// decades, beyond which a step counts as diverged
+constexpr double kDivergeDecades = 3.0;
 
 thread_local std::string g_err;  // errors without a context (init, gtsv)
 
@@ -154,6 +163,10 @@ int validate(const desalm_params* p, std::string& why) {
             j = (j + p->jcrs) / (1 << p->jcrs);
             k = (k + p->kcrs) / (1 << p->kcrs);
         }
+    }
+    if (p->img > 4098 || p->jmg > 4098 || p->kmg > 4098) {
+        why = "at most 4098 nodes per direction (lines of up to 4096 rows)";
+        return -1;
     }
     for (int b = 0; b < 6; b++)
         if (p->btype[b] < 0 || p->btype[b] > 8 || p->btype[b] == 7) {
@@ -389,7 +402,10 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
         for (int m = 0; m < 4; m++) { F.dpf[m] = Af(c->n1); F.cpf[m] = Af(c->n1); F.xf[m] = Af(c->n1); }
     }
     F.resid_part = A(5L * kResidBlocks);
-    F.resid = A(5);
+    // the five L1 sums, then the first zero or non-finite pivot of the line
+    // solves (one word, read back with the sums)
+    F.resid = A(6);
+    F.zero_pivot = (unsigned*)(F.resid + 5);
     F.pfix = A(1);
     for (int b = 0; b < 4; b++) {
         const Level& L1 = c->lv[1];
@@ -645,6 +661,8 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     for (desalm_ctx* c : zs) {
         const Level& L1 = c->lv[1];
         HIPCHK(c0, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
+        HIPCHK(c0, hipMemsetAsync(c->F.zero_pivot, 0xff, sizeof(unsigned), s));
+        c->st_zero_pivot = c->st_nonfinite = c->st_diverged = 0;
         // qnm1 <- qn, qn <- q on the fine level (mg_driver.F90:48-51), and
         // q^(n-2) <- qnm1 first when a warm start keeps it
         for (int m = 0; m < 5; m++) {
@@ -667,8 +685,10 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     HIPCHK(c0, hipGetLastError());
     const Level& L0 = c0->lv[1];
     const double tnu = (double)((long)(L0.ni - 2) * (L0.nj - 2) * (L0.nk - 2));
-    std::vector<double> parts(5 * zs.size());
+    // per zone: the five L1 sums and the zero-pivot word (F.resid[0..5])
+    std::vector<double> parts(6 * zs.size());
     int itc = 0;
+    int rc = 0;
     // decide_recalc_rh is carried from level to level and from one
     // pseudo-iteration into the next (mg_driver.F90:144-150); the body of an
     // iteration is fixed once its starting value is
@@ -764,22 +784,48 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
         }
         HIPCHK(c0, hipGetLastError());
         for (size_t z = 0; z < zs.size(); z++)
-            HIPCHK(c0, hipMemcpyAsync(&parts[5 * z], zs[z]->F.resid, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(c0, hipMemcpyAsync(&parts[6 * z], zs[z]->F.resid, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(c0, hipStreamSynchronize(s));
         for (desalm_ctx* c : zs) prof_flush(c);
         double resd[5] = {0, 0, 0, 0, 0};
         for (size_t z = 0; z < zs.size(); z++)
-            for (int m = 0; m < 5; m++) resd[m] = resd[m] + parts[5 * z + m];
+            for (int m = 0; m < 5; m++) resd[m] = resd[m] + parts[6 * z + m];
         double residual[5];
         for (int m = 0; m < 5; m++) residual[m] = log10(resd[m] / tnu);
         if (resid_out)
             for (int m = 0; m < 5; m++) resid_out[5 * (it - 1) + m] = residual[m];
+        // failure detection: the first row of a line solve whose pivot is
+        // zero or not finite (SGTSV's INFO), an update norm that is not
+        // finite, and growth of the largest u, v, w norm by more than
+        // kDivergeDecades over the first iteration's
+        unsigned zp = 0xffffffffu;
+        for (size_t z = 0; z < zs.size(); z++) {
+            unsigned w = 0;
+            memcpy(&w, &parts[6 * z + 5], sizeof(unsigned));
+            zp = std::min(zp, w);
+        }
+        bool nonfinite = false;
+        for (int m = 0; m < 5; m++) nonfinite = nonfinite || !std::isfinite(resd[m]);
         // mg_ck_conver (mg_driver.F90:372-398)
         if (itc == 1)
             for (int m = 0; m < 5; m++) init_resd[m] = residual[m];
         const double eomax = fmax(fmax(residual[1], residual[2]), residual[3]);
         const double ermax = fmax(fmax(residual[1] - init_resd[1], residual[2] - init_resd[2]),
                                   residual[3] - init_resd[3]);
+        if (zp != 0xffffffffu) {
+            c0->st_zero_pivot = (int)zp;
+            rc = (int)zp;
+            c0->err = "a line solve met a zero or non-finite pivot at row " + std::to_string(zp) +
+                      " in pseudo-iteration " + std::to_string(itc);
+            break;
+        }
+        if (nonfinite) {
+            c0->st_nonfinite = itc;
+            rc = DESALM_ERR_NONFINITE;
+            c0->err = "the update norms are not finite after pseudo-iteration " + std::to_string(itc);
+            break;
+        }
+        if (itc > 1 && ermax > kDivergeDecades && !c0->st_diverged) c0->st_diverged = itc;
         bool converged = false;
         if (itc > p.it_min && ermax < p.er_min) converged = true;
         if (itc > p.it_min && eomax < p.eo_min) converged = true;
@@ -788,6 +834,41 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     for (desalm_ctx* c : zs)
         HIPCHK(c0, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     if (itc_out) *itc_out = itc;
+    return rc;
+}
+
+// Entry state of a physical step that the fallback restores: qnm1, q^(n-2)
+// and xnut of the fine level (q and qn come back from the shifted qn and
+// qnm1, which the pseudo-iterations leave alone).
+static int fb_save(desalm_ctx* c) {
+    const long n1 = c->n1;
+    if (!c->fb_save) {
+        c->fb_save = dalloc(c, 11 * n1);
+        if (!c->fb_save) return fail(c, "fallback: out of device memory");
+    }
+    hipStream_t s = c->stream;
+    for (int m = 0; m < 5; m++) {
+        HIPCHK(c, hipMemcpyAsync(c->fb_save + m * n1, c->F.qnm1[m], sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
+        if (c->ws.q2.q[m])
+            HIPCHK(c, hipMemcpyAsync(c->fb_save + (5 + m) * n1, c->ws.q2.q[m], sizeof(double) * n1,
+                                     hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->fb_save + 10 * n1, c->F.xnut, sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
+static int fb_restore(desalm_ctx* c) {
+    const long n1 = c->n1;
+    hipStream_t s = c->stream;
+    for (int m = 0; m < 5; m++) {
+        HIPCHK(c, hipMemcpyAsync(c->F.q[m], c->F.qn[m], sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->F.qn[m], c->F.qnm1[m], sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->F.qnm1[m], c->fb_save + m * n1, sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
+        if (c->ws.q2.q[m])
+            HIPCHK(c, hipMemcpyAsync(c->ws.q2.q[m], c->fb_save + (5 + m) * n1, sizeof(double) * n1,
+                                     hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->F.xnut, c->fb_save + 10 * n1, sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
     return 0;
 }
 }  // extern "C++"
@@ -800,13 +881,30 @@ int desalm_mg_driver(desalm_ctx* c, int* itc_out, double* resid_out) {
     std::vector<desalm_ctx*> zs{c};
     const Level& L1 = c->lv[1];
     auto exchange = [&](int tur) { launch_bintp(s, L1, c->F, c->C, c->ov, tur, c->cvin, c->p.des); };
-    if (vcycles(zs, c->ov.n > 0, c->ov_mom, exchange, c->init_resd, itc_out, resid_out)) return -1;
+    // the FP32-DAF form and the warm starts fall back to the plain FP64
+    // method for a step they do not carry: a zero or non-finite pivot, norms
+    // that are not finite, or growth past kDivergeDecades
+    const bool plain = !c->C.mixed && c->ws.mode == DESALM_WS_NONE;
+    if (!plain && fb_save(c)) return -1;
+    int rc = vcycles(zs, c->ov.n > 0, c->ov_mom, exchange, c->init_resd, itc_out, resid_out);
+    if (rc == -1) return -1;
+    if (!plain && (rc != 0 || c->st_diverged)) {
+        if (fb_restore(c)) return -1;
+        const int mixed = c->C.mixed, mode = c->ws.mode;
+        c->C.mixed = 0;
+        c->ws.mode = DESALM_WS_NONE;
+        rc = vcycles(zs, c->ov.n > 0, c->ov_mom, exchange, c->init_resd, itc_out, resid_out);
+        c->C.mixed = mixed;
+        c->ws.mode = mode;
+        c->fallbacks++;
+        if (rc == -1) return -1;
+    }
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipStreamSynchronize(s));
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    return 0;
+    return rc;
 }
 
 int desalm_solver_daf(desalm_ctx* c, int n, int decide_recalc_rh, int decide_calc_pk, double* q, const double* qn,
@@ -830,6 +928,7 @@ int desalm_solver_daf(desalm_ctx* c, int n, int decide_recalc_rh, int decide_cal
     double* dq = c->dq_buf;
     HIPCHK(c, hipMemsetAsync(dq, 0, sizeof(double) * 4 * L.n, s));
     HIPCHK(c, hipMemsetAsync(c->F.pivots, 0, sizeof(unsigned long long), s));
+    HIPCHK(c, hipMemsetAsync(c->F.zero_pivot, 0xff, sizeof(unsigned), s));
     solver_daf(c, n, decide_recalc_rh, decide_calc_pk, decide_recalc_rh == 1 ? nullptr : dq);
     HIPCHK(c, hipGetLastError());
     if (decide_recalc_rh != 1) {
@@ -841,7 +940,14 @@ int desalm_solver_daf(desalm_ctx* c, int n, int decide_recalc_rh, int decide_cal
     if (copy_out_level(c, c->F.pk, 4, pk, L)) return -1;
     if (copy_out_level(c, c->F.rh, 4, rh, L)) return -1;
     HIPCHK(c, hipMemcpyAsync(&c->last_pivots, c->F.pivots, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    unsigned zp = 0xffffffffu;
+    HIPCHK(c, hipMemcpyAsync(&zp, c->F.zero_pivot, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
+    c->st_zero_pivot = zp == 0xffffffffu ? 0 : (int)zp;
+    if (c->st_zero_pivot) {
+        c->err = "a line solve met a zero or non-finite pivot at row " + std::to_string(zp);
+        return c->st_zero_pivot;
+    }
     return 0;
 }
 
@@ -934,6 +1040,16 @@ int desalm_get_level_field(desalm_ctx* c, int which, int n, double* out) {
 double desalm_last_driver_ms(const desalm_ctx* c) { return c ? c->last_ms : -1.0; }
 
 int64_t desalm_last_pivot_events(const desalm_ctx* c) { return c ? (int64_t)c->last_pivots : -1; }
+
+int desalm_last_status(const desalm_ctx* c, int32_t* zero_pivot_row, int32_t* nonfinite_iteration,
+                       int32_t* diverged_iteration, int64_t* fallbacks) {
+    if (!c) return fail(nullptr, "null argument");
+    if (zero_pivot_row) *zero_pivot_row = c->st_zero_pivot;
+    if (nonfinite_iteration) *nonfinite_iteration = c->st_nonfinite;
+    if (diverged_iteration) *diverged_iteration = c->st_diverged;
+    if (fallbacks) *fallbacks = c->fallbacks;
+    return 0;
+}
 
 int desalm_profile(desalm_ctx* c, int enable) {
     if (!c) return fail(c, "null argument");

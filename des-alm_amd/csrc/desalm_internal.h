@@ -94,6 +94,7 @@ struct Fields {
     // the face is neither; the k faces use 0, see k_bc_fm)
     double* rat[4];
     unsigned long long* pivots;  // [1] rows where LAPACK SGTSV would pivot
+    unsigned* zero_pivot;        // [1] first line row with a zero or non-finite pivot (0xffffffff: none)
 };
 
 constexpr int kResidBlocks = 1024;

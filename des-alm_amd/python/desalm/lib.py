@@ -21,6 +21,9 @@ FIELD = {"q": (0, 5), "qn": (1, 5), "qnm1": (2, 5), "qold_mg": (3, 5), "rh": (4,
 _lib = None
 
 
+ERR_NONFINITE = -2  # DESALM_ERR_NONFINITE
+
+
 class DesalmError(RuntimeError):
     pass
 
@@ -53,6 +56,9 @@ def lib():
         L.desalm_last_driver_ms.restype = ctypes.c_double
         L.desalm_last_pivot_events.argtypes = [vp]
         L.desalm_last_pivot_events.restype = ctypes.c_int64
+        L.desalm_last_status.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
+        L.desalm_last_status.restype = ctypes.c_int
         L.desalm_profile.argtypes = [vp, ctypes.c_int]
         L.desalm_profile.restype = ctypes.c_int
         L.desalm_profile_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
@@ -415,6 +421,17 @@ class Solver:
     @property
     def pivot_events(self):
         return lib().desalm_last_pivot_events(self.h)
+
+    def last_status(self):
+        """Failure detection of the last driver call: dict of zero_pivot_row,
+        nonfinite_iteration, diverged_iteration (0 = none) and the number of
+        fallbacks to the plain FP64 method so far."""
+        z, nf, dv = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        fb = ctypes.c_int64(0)
+        self._chk(lib().desalm_last_status(self.h, ctypes.byref(z), ctypes.byref(nf), ctypes.byref(dv),
+                                           ctypes.byref(fb)), "desalm_last_status")
+        return dict(zero_pivot_row=z.value, nonfinite_iteration=nf.value, diverged_iteration=dv.value,
+                    fallbacks=fb.value)
 
 
 class Group:

@@ -12,8 +12,9 @@
  * The library converts to its device structure-of-arrays layout internally.
  *
  * Return codes: 0 = success, < 0 = argument or HIP error (text available from
- * desalm_last_error), > 0 = LAPACK-INFO-like zero pivot reported by a line
- * solve (1-based row of the first singular row).
+ * desalm_last_error; DESALM_ERR_NONFINITE for update norms that are NaN or
+ * Inf), > 0 = LAPACK-INFO-like zero pivot reported by a line solve (1-based
+ * row, in line order, of the first row whose pivot is zero or not finite).
  *
  * One context drives one HIP device on one stream; a context is not
  * re-entrant (the reference runs one MPI rank per context).
@@ -28,6 +29,11 @@ extern "C" {
 #endif
 
 #define DESALM_MAX_LEVELS 6
+
+/* desalm_mg_driver / desalm_group_mg_driver / desalm_step: the L1 update
+ * norms of a pseudo-iteration are NaN or Inf.  The call stops after that
+ * iteration; itc_out and the norm history hold it. */
+#define DESALM_ERR_NONFINITE (-2)
 
 /* Arithmetic used by the DAF factor and its line solves.  The residual,
  * metrics and multigrid restriction are always FP64. */
@@ -286,6 +292,21 @@ double desalm_last_driver_ms(const desalm_ctx* ctx);
  * interchanges; a non-zero count says their row order departed from the
  * reference's SGTSV on that call. */
 int64_t desalm_last_pivot_events(const desalm_ctx* ctx);
+
+/* Failure detection of the last desalm_mg_driver / desalm_step /
+ * desalm_solver_daf call (no counterpart in the reference, which prints and
+ * carries on): zero_pivot_row = the INFO of the line solves (first row,
+ * 1-based in line order, whose pivot was zero or not finite; 0 = none),
+ * nonfinite_iteration = the pseudo-iteration whose update norms were NaN or
+ * Inf (0 = none), diverged_iteration = the first pseudo-iteration whose
+ * largest u, v, w update norm exceeded the first iteration's by more than
+ * three decades (0 = none).  fallbacks: physical steps so far that the
+ * FP32-DAF form or a warm start did not carry (a zero pivot, non-finite or
+ * diverging norms) and that were redone from the step's entry state with
+ * the plain FP64 method and the reference's start; the status then describes
+ * the plain run.  Any pointer may be NULL. */
+int desalm_last_status(const desalm_ctx* ctx, int32_t* zero_pivot_row, int32_t* nonfinite_iteration,
+                       int32_t* diverged_iteration, int64_t* fallbacks);
 
 /* Device time per kernel family and level, from HIP events recorded on the
  * context's stream around each launch inside desalm_mg_driver.
