@@ -51,6 +51,7 @@ def lib():
         L.desalm_solver_daf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [dp] * 10
         L.desalm_des_eddy.argtypes = [vp] + [dp] * 10
         L.desalm_gtsv_batch.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ip]
+        L.desalm_gtsv_batch_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ip]
         L.desalm_get_level_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.desalm_last_driver_ms.argtypes = [vp]
         L.desalm_last_driver_ms.restype = ctypes.c_double
@@ -111,7 +112,7 @@ def lib():
             getattr(L, f).restype = ctypes.c_int
         for f in ("desalm_init", "desalm_set_grid", "desalm_set_metrics", "desalm_set_wall_distance",
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
-                  "desalm_des_eddy", "desalm_gtsv_batch", "desalm_get_level_field", "desalm_finalize",
+                  "desalm_des_eddy", "desalm_gtsv_batch", "desalm_gtsv_batch_noalloc", "desalm_get_level_field", "desalm_finalize",
                   "desalm_set_overset", "desalm_bintp", "desalm_group_create", "desalm_group_set_overset",
                   "desalm_group_mg_driver", "desalm_group_destroy", "desalm_set_turbines"):
             getattr(L, f).restype = ctypes.c_int
@@ -480,16 +481,18 @@ class Group:
         return itc.value, hist[: itc.value]
 
 
-def gtsv_batch(dl, d, du, b):
-    """Batched SGTSV on the device.  Arrays (nsys, n); returns (x, info)."""
+def gtsv_batch(dl, d, du, b, noalloc=False):
+    """Batched SGTSV on the device.  Arrays (nsys, n); returns (x, info).
+    noalloc: through desalm_gtsv_batch_noalloc (fixed device staging, at most
+    DESALM_GTSV_NOALLOC_MAX = n * nsys entries)."""
     d = np.array(d, dtype=np.float64, copy=True, order="C")
     nsys, n = d.shape
     dl = np.array(dl, dtype=np.float64, copy=True, order="C").reshape(nsys, n)
     du = np.array(du, dtype=np.float64, copy=True, order="C").reshape(nsys, n)
     b = np.array(b, dtype=np.float64, copy=True, order="C").reshape(nsys, n)
     info = np.zeros(nsys, dtype=np.int32)
-    rc = lib().desalm_gtsv_batch(n, nsys, _p(dl), _p(d), _p(du), _p(b),
-                                 info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    f = lib().desalm_gtsv_batch_noalloc if noalloc else lib().desalm_gtsv_batch
+    rc = f(n, nsys, _p(dl), _p(d), _p(du), _p(b), info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     if rc != 0:
         raise DesalmError("desalm_gtsv_batch: " + lib().desalm_last_error(None).decode())
     return b, info

@@ -174,6 +174,20 @@ int desalm_gtsv_batch(int n, int nsys, double* dl, double* d, double* du, double
                       int* info_out);
 
 /*
+ * desalm_gtsv_batch without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernel and results as
+ * desalm_gtsv_batch on the current HIP device; the device copies of the four
+ * arrays and INFO live in one fixed device array of the library (9 MiB
+ * per GPU, held until the process ends), so n * nsys may not exceed
+ * DESALM_GTSV_NOALLOC_MAX (larger batches return < 0 with a message).  All
+ * calls on one GPU share that array: calls that overlap on one GPU race.
+ */
+#define DESALM_GTSV_NOALLOC_MAX 262144
+int desalm_gtsv_batch_noalloc(int n, int nsys, double* dl, double* d, double* du, double* b,
+                              int* info_out);
+
+/*
  * Overset donor table of the fine level, in place of the tables that
  * init_osg_interface reads (src/global_osg.F90:18-28, src/init.F90:1979-2000),
  * for donors and fringe nodes of this one zone: donor d interpolates
