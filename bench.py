@@ -72,6 +72,12 @@ FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_sweep<0", "daf_j": "k_da
                "dtau": "k_prep", "residual": "k_residual", "residual_recalc": "k_residual"}
 
 HEADLINE_DIMS = (193, 129, 129)
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """progress line on stderr (stdout carries only the JSON result)"""
+    print(f"bench.py [{time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def workload_name(dims):
@@ -214,6 +220,7 @@ def dual_time_leg(args):
         sp.ws_record()
         gen_itc.append(itc)
     t_gen = time.perf_counter() - t_gen
+    log(f"dual-time leg: {args.dt_train_steps} training steps generated")
     t_train = time.perf_counter()
     loss = sp.ws_train(args.dt_train_iters, batch=16384, lr=2e-3)
     t_train = time.perf_counter() - t_train
@@ -252,6 +259,7 @@ def dual_time_leg(args):
                 if name == "fp64_plain" and d == drops[-1] and itc >= 20:
                     half = itc // 2
                     tail.append(float((dd[-1] - dd[half]) / (itc - 1 - half)))
+        log(f"dual-time leg: evaluation step {len(res[drops[0]]['fp64_plain']['itc'])} done")
         # the trajectory continues from the plain step at the first rung
         sp.restore(after)
         sp.ws_mode(Solver.WS_NONE)
@@ -324,6 +332,7 @@ def large_leg(args):
         sv.step()
         sv.ws_record()
     loss = sv.ws_train(400, batch=16384, lr=2e-3)
+    log("large leg: warm start trained")
     sv.ws_mode(Solver.WS_LEARNED)
     sv.set_convergence(args.iters, args.iters, -99.0, -99.0)
     sv.step()
@@ -370,6 +379,7 @@ def cpu_baseline(args, dims):
     ni, nj, nk = dims
     n1 = ni * nj * nk
     cc = cases.alm_box(ni, nj, nk, ng=3, itm=args.cpu_iters, it_min=args.cpu_iters, iters=(1, 2, 3))
+    log("CPU baseline: C restatement")
     o = po.Oracle(cc.params).load_case(cc)
     tc0 = time.perf_counter()
     itc_c, _ = o.mg_driver()
@@ -385,6 +395,7 @@ def cpu_baseline(args, dims):
         why = "oracle/_ref/desalm_ref_harness is not built (oracle/ref/build_ref.sh needs the reference sources)"
     else:
         cr = cases.alm_box(ni, nj, nk, ng=3, itm=args.ref_iters, it_min=args.ref_iters, iters=(1, 2, 3))
+        log("CPU baseline: the reference's Fortran")
         try:
             r = po.run_reference(cr, mode=1, nsteps=1, timeout=900)
         except Exception as e:
@@ -505,16 +516,20 @@ def main():
         return cases.channel(n, n, n, ng=3, itm=args.iters, it_min=args.iters, iters=(1, 2, 3), precision=precision)
 
     # production path: every pseudo-iteration is one HIP graph replay
+    log("headline: actuator-line box, FP64 plain, graph replays")
     head = timed_run(alm, PREC_FP64, False, True)
+    log(f"headline: {head['value']:.1f} Mcell-iters/s")
     p = head["params"]
     n1 = p.img * p.jmg * p.kmg
     # the same workload with the library's per-launch HIP-event profiler
     # (launches then run eagerly): per-kernel-family device times
     prof = timed_run(alm, PREC_FP64, True, True)
+    log("profiled pass done")
     stats = prof["stats"]
     extras = None
     if not args.no_extras:
         mx = timed_run(alm, PREC_MIXED, False, True)
+        log(f"mixed: {mx['value']:.1f} Mcell-iters/s")
         ch = timed_run(channel, PREC_FP64, False, False)
         chm = timed_run(channel, PREC_MIXED, False, False)
         extras = {"mixed": {"value": round(mx["value"], 3), "unit": "Mcell-iters/s",
@@ -564,7 +579,11 @@ def main():
 
     conv = dual_time_leg(args) if args.conv_itm > 0 else None
     large = large_leg(args) if args.large else None
+    if large:
+        log(f"large leg: {large['value']:.1f} Mcell-iters/s")
     cpu = None if args.no_cpu else cpu_baseline(args, HEADLINE_DIMS)
+    if cpu:
+        log(f"CPU baseline ({cpu['kind']}): {cpu['value']} Mcell-iters/s")
 
     out = {"metric": METRIC, "value": round(head["value"], 3), "unit": "Mcell-iters/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["wall"] / args.steps * 1e3, 3),

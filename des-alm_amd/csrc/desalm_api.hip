@@ -116,6 +116,9 @@ struct desalm_ctx {
     int st_zero_pivot = 0, st_nonfinite = 0, st_diverged = 0;
     long long fallbacks = 0;
     double* fb_save = nullptr;
+    // desalm_probe: node indices and values of up to probe_cap points
+    void* probe_dev = nullptr;
+    int probe_cap = 0;
 };
 
 namespace {
@@ -1041,6 +1044,52 @@ double desalm_last_driver_ms(const desalm_ctx* c) { return c ? c->last_ms : -1.0
 
 int64_t desalm_last_pivot_events(const desalm_ctx* c) { return c ? (int64_t)c->last_pivots : -1; }
 
+int desalm_real_residual(desalm_ctx* c, double* log10_out) {
+    if (!c || !log10_out) return fail(c, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->lv[1];
+    hipStream_t s = c->stream;
+    launch_real_residual(s, L, c->F);
+    HIPCHK(c, hipGetLastError());
+    double sums[5];
+    HIPCHK(c, hipMemcpyAsync(sums, c->F.resid, sizeof(sums), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const double tnu = (double)((long)(L.ni - 2) * (L.nj - 2) * (L.nk - 2));
+    for (int m = 0; m < 5; m++) log10_out[m] = log10(sqrt(sums[m] / tnu));
+    return 0;
+}
+
+int desalm_probe(desalm_ctx* c, int32_t n, const int32_t* ijk, double* out) {
+    if (!c || n < 0 || (n > 0 && (!ijk || !out))) return fail(c, "null argument");
+    if (n == 0) return 0;
+    const Level& L = c->lv[1];
+    std::vector<long> node(n);
+    for (int p = 0; p < n; p++) {
+        const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
+        if (i < 1 || i > L.ni || j < 1 || j > L.nj || k < 1 || k > L.nk)
+            return fail(c, "desalm_probe: point " + std::to_string(p) + " outside the fine level");
+        node[p] = (i - 1) + (long)L.ni * ((j - 1) + (long)L.nj * (k - 1));
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (n > c->probe_cap) {
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (c->probe_dev) (void)hipFree(c->probe_dev);
+        c->probe_dev = nullptr;
+        c->probe_cap = 0;
+        HIPCHK(c, hipMalloc(&c->probe_dev, (sizeof(long) + 6 * sizeof(double)) * (size_t)n));
+        c->probe_cap = n;
+    }
+    long* dnode = (long*)c->probe_dev;
+    double* dout = (double*)(dnode + c->probe_cap);
+    HIPCHK(c, hipMemcpyAsync(dnode, node.data(), sizeof(long) * n, hipMemcpyHostToDevice, s));
+    launch_probe(s, c->F, n, dnode, dout);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
 int desalm_last_status(const desalm_ctx* c, int32_t* zero_pivot_row, int32_t* nonfinite_iteration,
                        int32_t* diverged_iteration, int64_t* fallbacks) {
     if (!c) return fail(nullptr, "null argument");
@@ -1889,6 +1938,7 @@ int desalm_finalize(desalm_ctx* c) {
     prof_flush(c);
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->probe_dev) (void)hipFree(c->probe_dev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

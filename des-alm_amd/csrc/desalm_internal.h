@@ -216,6 +216,10 @@ void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Field
                     const Consts& C, int icrs, int jcrs, int kcrs, const double* rs_tab);
 void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save_only);
 void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F);
+// physical-step outputs: the L2 sums of q - qn over the fine interior into
+// F.resid (src/main.F90:214-300), and q(1:5), xnut at fine-level nodes
+void launch_real_residual(hipStream_t s, const Level& L, const Fields& F);
+void launch_probe(hipStream_t s, const Fields& F, int n, const long* node, double* out);
 // case hooks of mg_driver (k_hooks.hip): the outlet mass-flux rescale and
 // the wall functions of both j walls, on the fine level
 void launch_conservation(hipStream_t s, const Level& L, const Fields& F);

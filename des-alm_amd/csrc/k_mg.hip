@@ -1,7 +1,7 @@
 // k_mg.hip -- FAS multigrid transfers, fine-grid boundary conditions,
 // update norms and layout conversion.
 //
-//   k_inject        mg_inject (src/mg_inject.F90:207-269)
+//   k_inject        mg_inject (src/mg_inject.F90:2-64)
 //   k_nlevel        mg_nlevel: injection of q, xnut, qold_mg and the 27-point
 //                   restriction of rh into pk (src/mg_nlevel.F90:2-150),
 //                   boundary pk zeroed (mg_zero_pk_boundary)
@@ -9,7 +9,7 @@
 //   k_prolong_fill  its k/j/i linear fills onto the fine level
 //                   (src/mg_prolong.F90:57-170), evaluated per fine node
 //   k_rs<D>         rs_sweep constant-coefficient smoothing sweeps
-//                   (src/rs_sweep.F90:163-284), k sweep also adds dq to q
+//                   (src/rs_sweep.F90:59-178), k sweep also adds dq to q
 //   k_bc_fm<D>      bcond_fm faces (src/bcond_fm.F90:79-306), k_pfix_*
 //                   the pinned-pressure shift (:310-322)
 //   k_resid_*       mg_residual's L1 update norms (src/mg_residual.F90:58-67)
@@ -539,6 +539,59 @@ void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Const
     const long g = L.off + (ifix - 1) + (long)L.ni * ((jfix - 1) + (long)L.nj * (kfix - 1));
     hipLaunchKernelGGL(k_pfix_read, dim3(1), dim3(1), 0, s, F, g);
     hipLaunchKernelGGL(k_pfix_sub, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
+}
+
+// Partial sums of (q - qn)^2 over the fine interior (real_residual,
+// src/main.F90:214-300), deterministic like k_resid_part; k_resid_final adds
+// the parts into F.resid
+__global__ void __launch_bounds__(256) k_real_part(Level L, Fields F) {
+    __shared__ double sh[5][256];
+    double acc[5] = {0, 0, 0, 0, 0};
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < L.n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+        if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) continue;
+        const long g = L.off + t;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            const double d = F.q[m][g] - F.qn[m][g];
+            acc[m] += d * d;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) sh[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) sh[m][threadIdx.x] += sh[m][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) F.resid_part[blockIdx.x * 5 + m] = sh[m][0];
+    }
+}
+
+// q(1:5) and xnut at n fine-level nodes (level-local index node[p]) into
+// out[6 p .. 6 p + 5]
+__global__ void k_probe(Fields F, int n, const long* node, double* out) {
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= n) return;
+    const long g = node[p];
+#pragma unroll
+    for (int m = 0; m < 5; m++) out[6 * p + m] = F.q[m][g];
+    out[6 * p + 5] = F.xnut[g];
+}
+
+void launch_real_residual(hipStream_t s, const Level& L, const Fields& F) {
+    hipLaunchKernelGGL(k_real_part, dim3(kResidBlocks), dim3(256), 0, s, L, F);
+    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, s, F, kResidBlocks);
+}
+
+void launch_probe(hipStream_t s, const Fields& F, int n, const long* node, double* out) {
+    if (n > 0) hipLaunchKernelGGL(k_probe, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, F, n, node, out);
 }
 
 void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save) {

@@ -27,9 +27,9 @@ struct Nb {
 __device__ __forceinline__ double sqr(double a) { return a * a; }
 
 // Per-node inputs of the residual, every node of the level:
-// rhs_daf_dtau (src/rhs_daf_dtau.F90:85-117) when with_dtau, and always
+// rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83) when with_dtau, and always
 // U/J (rhs_contra_j.F90:14-29) into F.ucn and p/J
-// (rhs_flux_sans_convec.F90:265-271) into F.pj (level-local).  With dtau
+// (rhs_flux_sans_convec.F90:60) into F.pj (level-local).  With dtau
 // also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
 // nodes interior in that direction, into F.fv (level-local): the residual
 // then reads it at a node and its two neighbours.
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
     const Nb nb{1, (long)ni, (long)ni * nj};
     const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
 
-    // central pressure gradient (rhs_flux_sans_convec.F90:280-326)
+    // central pressure gradient (rhs_flux_sans_convec.F90:75-116)
     double r[4];
     {
         const double* pj = F.pj;
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
             v = v + dz2 * (pkp * F.zet[c][g + nb.sK] - pkm * F.zet[c][g - nb.sK]);
             r[c + 1] = v;
         }
-        // divergence of U/J (:333-344)
+        // divergence of U/J (:120-135)
         const double dudx = dc2 * (ucn<0>(F, t + nb.sI) - ucn<0>(F, t - nb.sI));
         const double dvdy = de2 * (ucn<1>(F, t + nb.sJ) - ucn<1>(F, t - nb.sJ));
         const double dwdz = dz2 * (ucn<2>(F, t + nb.sK) - ucn<2>(F, t - nb.sK));
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
     } else {
         diss1 = F.diss1[g];
     }
-    // unsteady and dissipation terms (rhs_unst_visc_diss.F90:171-186);
+    // unsteady and dissipation terms (rhs_unst_visc_diss.F90:29-48);
     // diss(2:4) is zero (rhs_convec_quick.F90:433)
     {
         const double dtaj = 2.0 * C.delti * F.aj[g];
@@ -202,8 +202,8 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
 }
 
 // Viscous term subtracted from rh(2:4) (rhs_viscous.F90:112-329,
-// rhs_unst_visc_diss.F90:181-183) and the FAS forcing of coarse levels
-// (rhs_pk.F90:232-264): pk = pk - rh when calc_pk, then rh = rh + pk.
+// rhs_unst_visc_diss.F90:45-47) and the FAS forcing of coarse levels
+// (rhs_pk.F90:35-65): pk = pk - rh when calc_pk, then rh = rh + pk.
 //
 // A workgroup of kTX x kTY threads owns an (i, j) tile of (kTX-1) x (kTY-1)
 // interior nodes plus one halo column and row on the low side, and marches

@@ -150,7 +150,7 @@ __device__ __forceinline__ double sa_visc_face_t(const Level& L, const Consts& C
     return vtj * (cc1 * vcs + cc2 * vet + cc3 * vze);
 }
 
-// convective flux (rhs_sa_flux.F90:226-294) and ADI coefficient
+// convective flux (rhs_sa_flux.F90:32-110) and ADI coefficient
 // (rhs_sa_adi_solver.F90:53-75) of the face node 0 -> node 1 along D
 template <int D>
 __device__ __forceinline__ void sa_conv_av(const Consts& C, double q0, double q1, double u0, double u1,
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts 
         if (out) {
             const long g = o + t;
             const double aj = mcur.v[9];
-            // production, destruction, cb2 diffusion (rhs_sa_prod.F90:97-165)
+            // production, destruction, cb2 diffusion (rhs_sa_prod.F90:44-100)
             double psa;
             const double wdv = F.wd[g];
             if (wdv == 0.0) {

@@ -147,3 +147,74 @@ def read_deck(path, ng):
     d = Deck()
     _chk(_lib().desalm_read_deck(str(path).encode(), ng, ctypes.byref(d)), "desalm_read_deck")
     return d
+
+
+# ---- text outputs of a physical step (src/main.F90:71-75) ----
+
+def fortran_g(x, w, d):
+    """x written with the Fortran edit descriptor Gw.d (nearest rounding):
+    F editing with d significant digits followed by four blanks when the
+    rounded magnitude lies in [0.1, 10**d), E editing otherwise; zero as
+    F(w-4).(d-1) followed by four blanks."""
+    x = float(x)
+    if x != x:
+        return "NaN".rjust(w)
+    if x in (float("inf"), float("-inf")):
+        return ("-Infinity" if x < 0 else "Infinity").rjust(w)
+    neg = x < 0 or (x == 0.0 and str(x).startswith("-"))
+
+    def fit(body, width):
+        s = ("-" if neg else "") + body
+        if len(s) > width and body.startswith("0."):
+            s = ("-" if neg else "") + body[1:]
+        return "*" * width if len(s) > width else s.rjust(width)
+
+    if x == 0.0:
+        return fit("%.*f" % (d - 1, 0.0), w - 4) + "    "
+    mant, ex = ("%.*e" % (d - 1, abs(x))).split("e")
+    e10 = int(ex)          # exponent of the value rounded to d digits
+    s = e10 + 1            # 10**(s-1) <= rounded value < 10**s
+    if 0 <= s <= d:
+        body = "%.*f" % (d - s, abs(x))
+        return fit(body if d > s else body + ".", w - 4) + "    "
+    digits = mant.replace(".", "")
+    e = e10 + 1
+    exp = ("E%+03d" % e) if abs(e) <= 99 else ("%+04d" % e)
+    return fit("0." + digits + exp, w)
+
+
+def conver_real_line(ntime, residual):
+    """'conver_real' line, format (i5.5,1x,5(g13.6,1x)) (src/main.F90:199)."""
+    return "%05d " % ntime + "".join(fortran_g(v, 13, 6) + " " for v in residual)
+
+
+def history_line(ntime, values):
+    """'history' line of one monitor point: q(1:5) and xnut, format
+    (i5.5,1x,6(g12.6,1x)) (src/main.F90:552)."""
+    return "%05d " % ntime + "".join(fortran_g(v, 12, 6) + " " for v in values)
+
+
+def ctcp_line(ntime, ct, cp):
+    """'resultsCTCPnn.dat' line, format (i5.5,2(1x,G20.7)) (src/main.F90:103)."""
+    return "%05d" % ntime + " " + fortran_g(ct, 20, 7) + " " + fortran_g(cp, 20, 7)
+
+
+def append_step_outputs(directory, ntime, solver, monitor_points=(), n_turbines=0):
+    """Append the reference's per-step text outputs after a physical step
+    (src/main.F90:71-75): the real residual to 'conver_real'; q(1:5) and
+    xnut at every monitor point (1-based fine-level i, j, k) to 'history';
+    and per rotor CT and CP to 'resultsCTCPnn.dat'.  The reference's
+    actuator-line path sets C_T = C_P = 1 (src/rhs_lines.F90:458-459, the
+    thrust and power expressions are commented out there), so those are
+    the values written."""
+    import os
+    with open(os.path.join(directory, "conver_real"), "a") as f:
+        f.write(conver_real_line(ntime, solver.real_residual()) + "\n")
+    if len(monitor_points):
+        vals = solver.probe(monitor_points)
+        with open(os.path.join(directory, "history"), "a") as f:
+            for v in vals:
+                f.write(history_line(ntime, v) + "\n")
+    for t in range(n_turbines):
+        with open(os.path.join(directory, "resultsCTCP%02d.dat" % (t + 1)), "a") as f:
+            f.write(ctcp_line(ntime, 1.0, 1.0) + "\n")

@@ -60,6 +60,10 @@ def lib():
         L.desalm_last_status.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
         L.desalm_last_status.restype = ctypes.c_int
+        L.desalm_real_residual.argtypes = [vp, dp]
+        L.desalm_real_residual.restype = ctypes.c_int
+        L.desalm_probe.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), dp]
+        L.desalm_probe.restype = ctypes.c_int
         L.desalm_profile.argtypes = [vp, ctypes.c_int]
         L.desalm_profile.restype = ctypes.c_int
         L.desalm_profile_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
@@ -422,6 +426,22 @@ class Solver:
     @property
     def pivot_events(self):
         return lib().desalm_last_pivot_events(self.h)
+
+    def real_residual(self):
+        """log10 sqrt(mean over the fine interior of (q - qn)^2) per component:
+        the 'conver_real' line of the reference (src/main.F90:191-300)."""
+        out = np.zeros(5)
+        self._chk(lib().desalm_real_residual(self.h, _p(out)), "desalm_real_residual")
+        return out
+
+    def probe(self, ijk):
+        """q(1:5) and xnut (rows of 6) at 1-based fine-level points (n, 3): the
+        'history' values of the reference's monitor points."""
+        pts = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros((len(pts), 6))
+        self._chk(lib().desalm_probe(self.h, len(pts), pts.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                     _p(out)), "desalm_probe")
+        return out
 
     def last_status(self):
         """Failure detection of the last driver call: dict of zero_pivot_row,

@@ -381,6 +381,18 @@ int desalm_set_rfg(desalm_ctx* ctx, int32_t n_modes, const double* k, const doub
  * desalm_ws_mode. */
 int desalm_step(desalm_ctx* ctx, int* itc_out, double* log_resid_out);
 
+/* ---- Outputs of a physical step (main.F90:71-75) ----
+ * log10_out[0..4]: the real residual of output_real_residual
+ * (src/main.F90:191-300), log10 sqrt(sum over the fine interior of
+ * (q - qn)^2 / interior count) per component of the current state: after a
+ * desalm_step, the change over the step (the 'conver_real' line).  Uses the
+ * norm buffer of the driver; not during a driver call. */
+int desalm_real_residual(desalm_ctx* ctx, double* log10_out);
+/* Monitor points of output_probe (src/main.F90:536-555): for n points
+ * ijk[3p..3p+2] (1-based fine-level i, j, k), out[6p..6p+5] = q(1:5) and
+ * xnut there (the 'history' line of each point). */
+int desalm_probe(desalm_ctx* ctx, int32_t n, const int32_t* ijk, double* out);
+
 /* ---- Learned warm start ----
  * No counterpart in the reference, which starts every physical step's
  * pseudo-iterations from the previous solution (mg_driver.F90:48-51).  With

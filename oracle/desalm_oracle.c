@@ -7,8 +7,11 @@
  * Loops, operand order and quirks follow the reference routine cited above
  * each function; single zone, single rank (every MPI neighbour is
  * MPI_PROC_NULL, no ghost layers, src/init.F90:575,671-680), no blanking
- * boxes (nblk = 0), case hooks (wf_eta, conservation, ALM forcing, hub
- * zeroing) off, characteristic outlet (btype 5) unsupported.
+ * boxes (nblk = 0).  The case hooks are options of a run (or_set_hooks:
+ * conservation, and the wall functions with the reference's own Newton
+ * chain or an exact u_tau), as are the actuator-line forcing with the hub
+ * zeroing (or_set_turbines), inflow faces (btype 4) and the characteristic
+ * outlet (btype 5); several zones run through the group driver.
  *
  * Pinning of each part (what checks it beyond this restatement) is listed in
  * the tests that import it: the reference Fortran compiled from its own
@@ -241,7 +244,7 @@ long or_nall(const or_state* o) { return o->nall; }
 long or_level_offset(const or_state* o, int n) { return (n >= 1 && n <= o->ng) ? o->off[n] : -1; }
 
 /* ------------------------------------------------------------------ */
-/* mg_inject (src/mg_inject.F90:207-269): scalar field fine -> coarse  */
+/* mg_inject (src/mg_inject.F90:2-64): scalar field fine -> coarse  */
 /* levels, all nodes, injection at ii = 2^icrs*i - icrs.               */
 /* ------------------------------------------------------------------ */
 static void mg_inject(const or_state* o, double* dum, int stride, int comp) {
@@ -2045,7 +2048,7 @@ static void rhs_sa_viscous(sa_t* d) {
     free(ret);
 }
 
-/* rhs_sa_flux (src/rhs_sa_flux.F90:196-306): first-order upwind FDS */
+/* rhs_sa_flux (src/rhs_sa_flux.F90:2-112): first-order upwind FDS */
 static void rhs_sa_flux(sa_t* d) {
     SLV;
     int i, j, k, dir;
@@ -2071,7 +2074,7 @@ static void rhs_sa_flux(sa_t* d) {
     }
 }
 
-/* rhs_sa_unst_visc (src/rhs_sa_unst_visc.F90:171-202) */
+/* rhs_sa_unst_visc (src/rhs_sa_unst_visc.F90:1-32) */
 static void rhs_sa_unst_visc(sa_t* d) {
     SLV;
     int i, j, k;
@@ -2086,7 +2089,7 @@ static void rhs_sa_unst_visc(sa_t* d) {
             }
 }
 
-/* rhs_sa_implicit_source (src/rhs_sa_implicit_source.F90:205-258): all nodes */
+/* rhs_sa_implicit_source (src/rhs_sa_implicit_source.F90:2-55): all nodes */
 static void rhs_sa_implicit_source(sa_t* d) {
     SLV;
     int i, j, k;
@@ -2399,7 +2402,7 @@ int or_mg_nlevel(or_state* o, int n) {
     return 0;
 }
 
-/* rs_sweep (src/rs_sweep.F90:111-291): constant-coefficient implicit
+/* rs_sweep (src/rs_sweep.F90:2-182): constant-coefficient implicit
  * smoothing of dq on interior nodes, with SGTSV and NRHS = 4 */
 static void rs_sweep(or_state* o, int n, double* dq /* level n, stride 4 */) {
     LEVEL(n);

@@ -562,3 +562,40 @@ def test_damaged_files_raise_instead_of_crashing(tmp_path):
         files.read_grid(f, dims, turbulence=False)
     with pytest.raises(DesalmError, match="expected 36 reals"):
         files.read_solu(f, dims, turbulence=True)
+
+
+STEP_VALUES = [-0.73614778, -1.17446624, 0.0, 1.23e-5, -3.2e8, 123456.7, 999999.6, 0.09999996, 1.0,
+               -2.5e-12, 7.5, 0.1, 99.99995, -1.0e-300, 3.14159265358979, 4.2e17]
+
+
+@needs_fc
+def test_step_output_lines_match_fortran_formatting(tmp_path):
+    """The per-step text outputs (conver_real, history, resultsCTCPnn.dat,
+    src/main.F90:71-75) against the same formats written by a Fortran
+    runtime (amdflang), value by value."""
+    vals = ", ".join("%.17e_8" % v for v in STEP_VALUES)
+    src = f"""
+program step_lines
+  implicit none
+  real(kind=8) :: v({len(STEP_VALUES)}) = (/ {vals} /)
+  integer :: i
+  open (unit = 9, file = 'lines.txt')
+  do i = 1, size(v) - 4
+     write (9, '(i5.5,1x,5(g13.6,1x))') i, v(i:i+4)
+     write (9, '(i5.5,1x,6(g12.6,1x))') i, v(i:i+3), v(i), v(i+1)
+     write (9, '(i5.5,2(1x,G20.7))') i, v(i), v(i+2)
+  end do
+  close (9)
+end program step_lines
+"""
+    run_fortran(str(tmp_path), "step_lines", src)
+    got = open(tmp_path / "lines.txt").read().splitlines()
+    v = STEP_VALUES
+    want = []
+    for i in range(len(v) - 4):
+        want.append(files.conver_real_line(i + 1, v[i:i + 5]))
+        want.append(files.history_line(i + 1, v[i:i + 4] + [v[i], v[i + 1]]))
+        want.append(files.ctcp_line(i + 1, v[i], v[i + 2]))
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert g.rstrip() == w.rstrip(), (g, w)
