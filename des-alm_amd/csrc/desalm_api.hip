@@ -80,7 +80,6 @@ struct desalm_ctx {
     Consts C;
     Fields F;
     std::vector<void*> allocs;
-    double* rs_tab = nullptr;      // per prolongation level: 3 directions x 2*maxdim
     double* staging = nullptr;     // 5 * nall doubles: AoS transfers
     double* dq_buf = nullptr;      // 4 * n1 doubles: dq of the solver_daf parity entry
     std::string err;
@@ -193,18 +192,6 @@ int validate(const desalm_params* p, std::string& why) {
         return -1;
     }
     return 0;
-}
-
-// SGTSV factorisation of the constant smoothing matrix (-ep, 1+2ep, -ep)
-// of length len: fact[0..len-2] then d[0..len-1] (src/sgtsv.f:166-182)
-void rs_factor(double ep, int len, double* out) {
-    double* fact = out;
-    double* d = out + (len - 1);
-    for (int s = 0; s < len; s++) d[s] = 1.0 + 2.0 * ep;
-    for (int s = 0; s < len - 1; s++) {
-        fact[s] = (-ep) / d[s];
-        d[s + 1] = d[s + 1] - fact[s] * (-ep);
-    }
 }
 
 hipEvent_t prof_event(desalm_ctx* c) {
@@ -427,27 +414,9 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     }
     c->staging = A(5 * c->nall);
     c->dq_buf = A(4 * c->n1);
-    // prolongation smoothing factorisations per target level
-    const long per_level = 3 * 2 * c->maxdim;
-    c->rs_tab = A(per_level * (c->ng > 1 ? c->ng - 1 : 1));
     if (!ok) {
         desalm_finalize(c);
         return fail(nullptr, "device allocation failed");
-    }
-    {
-        std::vector<double> tab(per_level * (c->ng > 1 ? c->ng - 1 : 1), 0.0);
-        for (int n = 1; n < c->ng; n++) {
-            const Level& L = c->lv[n];
-            const int dims[3] = {L.ni, L.nj, L.nk};
-            for (int d = 0; d < 3; d++) {
-                const long stride = 2L * std::max(L.ni, std::max(L.nj, L.nk));
-                rs_factor(p->ep[1][d], dims[d] - 2, tab.data() + (n - 1) * per_level + d * stride);
-            }
-        }
-        if (hipMemcpy(c->rs_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-            desalm_finalize(c);
-            return fail(nullptr, "rs table upload failed");
-        }
     }
     if (hipDeviceSynchronize() != hipSuccess) {
         desalm_finalize(c);
@@ -719,10 +688,8 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
         }
         for (desalm_ctx* c : zs) {
             for (int n = ng - 1; n >= 1; n--) {
-                const long per_level = 3 * 2 * c->maxdim;
                 timed(c, KID_PROLONG, n, [&] {
-                    launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, c->p.icrs, c->p.jcrs, c->p.kcrs,
-                                   c->rs_tab + (n - 1) * per_level);
+                    launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, c->p.icrs, c->p.jcrs, c->p.kcrs);
                 });
             }
             timed(c, KID_BCOND, 1, [&] { bcond(c, s); });

@@ -213,7 +213,7 @@ void launch_nlevel(hipStream_t s, const Level& Lc, const Level& Lf, const Fields
                    int jcrs, int kcrs);
 void launch_zero_pk_boundary(hipStream_t s, const Level& L, const Fields& F);
 void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F,
-                    const Consts& C, int icrs, int jcrs, int kcrs, const double* rs_tab);
+                    const Consts& C, int icrs, int jcrs, int kcrs);
 void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save_only);
 void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F);
 // physical-step outputs: the L2 sums of q - qn over the fine interior into

@@ -8,13 +8,14 @@
 //   k_prolong_corr  mg_prolong's coarse correction q - qold_mg
 //   k_prolong_fill  its k/j/i linear fills onto the fine level
 //                   (src/mg_prolong.F90:57-170), evaluated per fine node
-//   k_rs<D>         rs_sweep constant-coefficient smoothing sweeps
-//                   (src/rs_sweep.F90:59-178), k sweep also adds dq to q
+//   k_rs_sweep<D>   rs_sweep constant-coefficient smoothing sweeps
+//                   (src/rs_sweep.F90:59-178) with every row on chip,
+//                   k sweep also adds dq to q
 //   k_bc_fm<D>      bcond_fm faces (src/bcond_fm.F90:79-306), k_pfix_*
 //                   the pinned-pressure shift (:310-322)
 //   k_resid_*       mg_residual's L1 update norms (src/mg_residual.F90:58-67)
 #include "desalm_internal.h"
-#include "line_tile.h"
+#include "seg_sweep.h"
 
 namespace desalm {
 
@@ -148,162 +149,60 @@ __global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
     for (int m = 0; m < 4; m++) F.dp[m][t] = fill_i(P, F, m, i, j, k);
 }
 
-// rs_sweep in direction D on the interior of the fine level, one thread per
-// (line, component m = blockIdx.y); tab holds the SGTSV factorization of the
-// constant matrix (-ep, 1+2ep, -ep): fact[len-1] then d[len].  D = 2 also
-// adds dq to q.
-// rows loaded per group, one group ahead of the elimination (16 measured
-// best among 4, 8 and 16 at 129^3)
-constexpr int kPF = 16;
-
-template <int D>
-__global__ void __launch_bounds__(128) k_rs(Level L, Fields F, const double* tab, double epn) {
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long sI = 1, sJ = ni, sK = (long)ni * nj;
-    const int m = blockIdx.y;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long base, sD;
-    int len;
+// rs_sweep in direction D on the interior of a level (src/rs_sweep.F90:59-178):
+// the constant system (-ep, 1+2ep, -ep) applied to the four components of dq
+// (F.dp, level-local) with every row on chip (seg_sweep.h), solved in place;
+// D = 2 adds the result to q instead.  The reference calls SGTSV with four
+// right-hand sides; this is its non-interchanging elimination.
+template <int D, int R>
+__global__ void __launch_bounds__(512) k_rs_sweep(Level L, Fields F, SegShape sh, double epn) {
+    extern __shared__ double smem_rs[];
+    const int P = sh.P, TI = sh.TI, n = sh.n;
+    const int tid = (int)threadIdx.x;
+    const int li = D == 0 ? tid / P : tid % TI;
+    const int ps = D == 0 ? tid % P : tid / TI;
+    const long line0 = (long)blockIdx.x * TI;
+    const long t = line0 + li;
+    const bool act = t < sh.nlines;
+    const int ni = L.ni, nj = L.nj;
+    long base = 0, sD;
     if (D == 0) {
-        if (t >= (long)(nj - 2) * (nk - 2)) return;
-        base = sJ * (1 + t % (nj - 2)) + sK * (1 + t / (nj - 2));
-        sD = sI;
-        len = ni - 2;
+        sD = 1;
+        if (act) base = (long)ni * ((1 + t % (nj - 2)) + (long)nj * (1 + t / (nj - 2)));
     } else if (D == 1) {
-        if (t >= (long)(ni - 2) * (nk - 2)) return;
-        base = (1 + t % (ni - 2)) + sK * (1 + t / (ni - 2));
-        sD = sJ;
-        len = nj - 2;
+        sD = ni;
+        if (act) base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
     } else {
-        if (t >= (long)(ni - 2) * (nj - 2)) return;
-        base = (1 + t % (ni - 2)) + sJ * (1 + t / (ni - 2));
-        sD = sK;
-        len = nk - 2;
+        sD = (long)ni * nj;
+        if (act) base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
     }
-    const double* fact = tab;
-    const double* dd = tab + (len - 1);
-    const double du = -epn;
-    double* __restrict__ v = F.dp[m];
-    double prev = v[base + sD];
-    double bcur[kPF];
+    const int s0 = ps * R;
+    const int nr = act ? max(0, min(R, n - s0)) : 0;
+    double a[R], b[R], c[R], d[R][4];
 #pragma unroll
-    for (int u = 0; u < kPF; u++) {
-        const int s = 1 + u;
-        bcur[u] = s < len ? v[base + (1 + s) * sD] : 0.0;
+    for (int r = 0; r < R; r++) {
+        a[r] = -epn;
+        b[r] = 1.0 + 2.0 * epn;
+        c[r] = -epn;
+#pragma unroll
+        for (int m = 0; m < 4; m++) d[r][m] = r < nr ? F.dp[m][base + (long)(s0 + r + 1) * sD] : 0.0;
     }
-    for (int s0 = 1; s0 < len; s0 += kPF) {
-        double bnx[kPF];
+    unsigned piv = 0;
+    int zrow = 0;
+    seg_solve<R, 4, double>(a, b, c, d, -epn, nr, n, s0, smem_rs, P, TI, sh.nlines, line0, li, ps, piv, zrow);
 #pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 + kPF + u;
-            bnx[u] = s < len ? v[base + (1 + s) * sD] : 0.0;
-        }
+    for (int r = 0; r < R; r++) {
+        if (r < nr) {
+            const long tl = base + (long)(s0 + r + 1) * sD;
 #pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 + u;
-            if (s < len) {
-                prev = bcur[u] - fact[s - 1] * prev;
-                v[base + (1 + s) * sD] = prev;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPF; u++) bcur[u] = bnx[u];
-    }
-    double* qm = F.q[m] + L.off;
-    double x = prev / dd[len - 1];
-    {
-        const long p = base + (long)len * sD;
-        if (D < 2) v[p] = x;
-        else qm[p] = qm[p] + x;
-    }
-    for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
-        double b[kPF], qb[kPF];
-#pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 - u;
-            b[u] = s >= 0 ? v[base + (1 + s) * sD] : 0.0;
-            if (D == 2) qb[u] = s >= 0 ? qm[base + (1 + s) * sD] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 - u;
-            if (s >= 0) {
-                x = (b[u] - du * x) / dd[s];
-                const long p = base + (1 + s) * sD;
-                if (D < 2) v[p] = x;
-                else qm[p] = qb[u] + x;
+            for (int m = 0; m < 4; m++) {
+                if (D < 2) F.dp[m][tl] = d[r][m];
+                else F.q[m][L.off + tl] = F.q[m][L.off + tl] + d[r][m];
             }
         }
     }
-}
-
-// The i-direction rs_sweep of k_rs with its rows staged through LDS
-// (line_tile.h): thread = (component m, line l), kTL lines per workgroup.
-// Same arithmetic as k_rs<0>.
-__global__ void __launch_bounds__(4 * kTL) k_rs_i(Level L, Fields F, const double* tab, double epn) {
-    constexpr int NT = 4 * kTL;
-    using IO = TileIO<NT, kTL, kTR, kTS>;
-    __shared__ long s_base[kTL];
-    __shared__ double s_v[4][kTL * kTS];
-
-    const long nlines = (long)(L.nj - 2) * (L.nk - 2);
-    const long t0 = (long)blockIdx.x * kTL;
-    const int nl = (int)min((long)kTL, nlines - t0);
-    const int m = (int)threadIdx.x / kTL, l = (int)threadIdx.x % kTL;
-    const bool act = l < nl;
-    if (threadIdx.x < kTL) s_base[threadIdx.x] = act ? iline_base(L, t0 + l) : 0;
-    __syncthreads();
-
-    const int len = L.ni - 2;
-    const int nch = (len + kTR - 1) / kTR;
-    const double* fact = tab;
-    const double* dd = tab + (len - 1);
-    const double du = -epn;
-    double* myv = s_v[m] + l * kTS;
-
-    double v[4][IO::kPer];
-    for (int mm = 0; mm < 4; mm++) IO::load(v[mm], F.dp[mm] + 1, s_base, nl, 0, len);
-    double prev = 0.0;
-    for (int ch = 0; ch < nch; ch++) {
-        const int s0 = ch * kTR;
-        __syncthreads();
-        for (int mm = 0; mm < 4; mm++) IO::to_lds(v[mm], s_v[mm]);
-        __syncthreads();
-        if (ch + 1 < nch)
-            for (int mm = 0; mm < 4; mm++) IO::load(v[mm], F.dp[mm] + 1, s_base, nl, s0 + kTR, len);
-        if (act) {
-            for (int u = 0; u < kTR; u++) {
-                const int s = s0 + u;
-                if (s >= len) break;
-                prev = (s == 0) ? myv[u] : myv[u] - fact[s - 1] * prev;
-                myv[u] = prev;
-            }
-        }
-        __syncthreads();
-        if (ch + 1 < nch)
-            for (int mm = 0; mm < 4; mm++) IO::store(s_v[mm], F.dp[mm] + 1, s_base, nl, s0, len);
-    }
-    double x = 0.0;
-    for (int ch = nch - 1; ch >= 0; ch--) {
-        const int s0 = ch * kTR;
-        if (ch < nch - 1) {
-            __syncthreads();
-            for (int mm = 0; mm < 4; mm++) IO::to_lds(v[mm], s_v[mm]);
-            __syncthreads();
-        }
-        if (ch > 0)
-            for (int mm = 0; mm < 4; mm++) IO::load(v[mm], F.dp[mm] + 1, s_base, nl, s0 - kTR, len);
-        if (act) {
-            for (int u = kTR - 1; u >= 0; u--) {
-                const int s = s0 + u;
-                if (s >= len) continue;
-                x = (s == len - 1) ? myv[u] / dd[len - 1] : (myv[u] - du * x) / dd[s];
-                myv[u] = x;
-            }
-        }
-        __syncthreads();
-        for (int mm = 0; mm < 4; mm++) IO::store(s_v[mm], F.dp[mm] + 1, s_base, nl, s0, len);
-    }
+    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
+    if (zrow) atomicMin(F.zero_pivot, (unsigned)zrow);
 }
 
 // ---- grid ratios of the inflow faces (init_outlet_bc, init.F90:2750-2891) ----
@@ -509,19 +408,30 @@ void launch_zero_pk_boundary(hipStream_t s, const Level& L, const Fields& F) {
     hipLaunchKernelGGL(k_zero_pk_boundary, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
 }
 
+// the smoothing sweep of direction D over the interior lines of level L
+template <int D>
+static void rs_sweep(hipStream_t s, const Level& L, const Fields& F, double ep) {
+    const int n = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
+    const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
+                                                                    : (long)(L.ni - 2) * (L.nj - 2);
+    const int R = n <= 64 ? 4 : 8;
+    const SegShape sh = seg_shape(n, nl, R, D == 0);
+    const unsigned nb = (unsigned)((sh.nlines + sh.TI - 1) / sh.TI);
+    const size_t lds = sizeof(double) * seg_slot(4) * (size_t)sh.TI * sh.P;
+    if (R == 4)
+        hipLaunchKernelGGL((k_rs_sweep<D, 4>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, sh, ep);
+    else
+        hipLaunchKernelGGL((k_rs_sweep<D, 8>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, sh, ep);
+}
+
 void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F, const Consts& C, int icrs,
-                    int jcrs, int kcrs, const double* rs_tab) {
+                    int jcrs, int kcrs) {
     ProlongCtx P{Lf, Lc, icrs, jcrs, kcrs};
     hipLaunchKernelGGL(k_prolong_corr, dim3(nblk(Lc.n, 256)), dim3(256), 0, s, Lc, F);
     hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(Lf.n, 256)), dim3(256), 0, s, P, F);
-    // rs_tab: three direction tables of 2 * max(ni,nj,nk) doubles
-    const long stride = 2L * (Lf.ni > Lf.nj ? (Lf.ni > Lf.nk ? Lf.ni : Lf.nk) : (Lf.nj > Lf.nk ? Lf.nj : Lf.nk));
-    hipLaunchKernelGGL(k_rs_i, dim3(nblk((long)(Lf.nj - 2) * (Lf.nk - 2), kTL)), dim3(4 * kTL), 0, s, Lf, F, rs_tab,
-                       C.ep[1][0]);
-    hipLaunchKernelGGL(k_rs<1>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nk - 2), 128), 4), dim3(128), 0, s, Lf, F,
-                       rs_tab + stride, C.ep[1][1]);
-    hipLaunchKernelGGL(k_rs<2>, dim3(nblk((long)(Lf.ni - 2) * (Lf.nj - 2), 128), 4), dim3(128), 0, s, Lf, F,
-                       rs_tab + 2 * stride, C.ep[1][2]);
+    rs_sweep<0>(s, Lf, F, C.ep[1][0]);
+    rs_sweep<1>(s, Lf, F, C.ep[1][1]);
+    rs_sweep<2>(s, Lf, F, C.ep[1][2]);
 }
 
 void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const int* btype, const int* bdir) {

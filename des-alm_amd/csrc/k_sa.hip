@@ -9,14 +9,14 @@
 //               rhs_sa_unst_visc, rhs_sa_implicit_source on the interior,
 //               and the ADI operators' face coefficients and dtev J / 2
 //               (LDS-staged tiles marching in k)
-//   k_sa_adi_i  rhs_sa_adi_solver sweep in i (rows staged through LDS)
-//   k_sa_adi<D> the sweeps in j and k (D = 2 also applies the update of
-//               des_eddy.F90:260-272)
+//   k_sa_sweep<D> the rhs_sa_adi_solver sweeps in i, j and k with every
+//               row on chip (seg_sweep.h); D = 2 also applies the update of
+//               des_eddy.F90:260-272
 //   k_sa_bc<D>  rhs_sa_bcond faces, k_sa_clamp its nu~ floor and xnut
 #include <algorithm>
 
 #include "desalm_internal.h"
-#include "line_tile.h"
+#include "seg_sweep.h"
 
 namespace desalm {
 
@@ -364,430 +364,6 @@ __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts 
     }
 }
 
-// one ADI sweep (rhs_sa_adi_solver.F90:44-244), one thread per line:
-//   aw = -dtj (U_(s-1) + av_(s-1/2)), ap = 1 + dtj (av_(s-1/2) + av_(s+1/2)),
-//   ae = dtj (U_(s+1) - av_(s+1/2)).
-// D = 2 also does the relaxed update of des_eddy.F90:260-272:
-//   q5 += (-q5 + qold + rh) / (1 + 1.5 e_source dtev / delti)
-// rows loaded per group, one group ahead of the elimination (16 measured
-// best among 4, 8 and 16 at 129^3)
-constexpr int kPF = 16;
-
-template <int D>
-__global__ void __launch_bounds__(128) k_sa_adi(Level L, Fields F, Consts C) {
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long base, sD;
-    int len;
-    if (D == 0) {
-        if (t >= (long)(nj - 2) * (nk - 2)) return;
-        base = (long)ni * ((1 + t % (nj - 2)) + (long)nj * (1 + t / (nj - 2)));
-        sD = 1;
-        len = ni - 2;
-    } else if (D == 1) {
-        if (t >= (long)(ni - 2) * (nk - 2)) return;
-        base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
-        sD = ni;
-        len = nj - 2;
-    } else {
-        if (t >= (long)(ni - 2) * (nj - 2)) return;
-        base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
-        sD = (long)ni * nj;
-        len = nk - 2;
-    }
-    const double* __restrict__ U = F.ucn[D];
-    const double* __restrict__ av = F.av[D];
-    const double* __restrict__ dtj = F.dtj;
-    double* __restrict__ rh = F.rh5;
-    double* __restrict__ cp = F.cp[0];
-    double avm = av[base];
-    double um = U[base], u0 = U[base + sD];
-    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
-    unsigned piv = 0;
-    double b_av[kPF], b_dt[kPF], b_up[kPF], b_rh[kPF];
-#pragma unroll
-    for (int u = 0; u < kPF; u++) {
-        const long p = base + (1 + u) * sD;
-        const bool ok = u < len;
-        b_av[u] = ok ? av[p] : 0.0;
-        b_dt[u] = ok ? dtj[p] : 0.0;
-        b_up[u] = ok ? U[p + sD] : 0.0;
-        b_rh[u] = ok ? rh[p] : 0.0;
-    }
-    for (int s0 = 0; s0 < len; s0 += kPF) {
-        double n_av[kPF], n_dt[kPF], n_up[kPF], n_rh[kPF];
-#pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 + kPF + u;
-            const long p = base + (1 + s) * sD;
-            const bool ok = s < len;
-            n_av[u] = ok ? av[p] : 0.0;
-            n_dt[u] = ok ? dtj[p] : 0.0;
-            n_up[u] = ok ? U[p + sD] : 0.0;
-            n_rh[u] = ok ? rh[p] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 + u;
-            if (s < len) {
-                const long p = base + (1 + s) * sD;
-                const double avp = b_av[u], dt = b_dt[u], up = b_up[u];
-                const double aw = -dt * (um + avm);
-                const double ap = 1.0 + dt * (avm + avp);
-                const double ae = dt * (up - avp);
-                const double aq = b_rh[u];
-                double den;
-                if (s == 0) {
-                    den = ap;
-                } else {
-                    if (fabs(denprev) < fabs(aw)) piv++;
-                    den = ap - aw * cprev;
-                }
-                const double inv = 1.0 / den;
-                cprev = ae * inv;
-                dprev = (s == 0) ? aq * inv : (aq - aw * dprev) * inv;
-                denprev = den;
-                cp[p] = cprev;
-                rh[p] = dprev;
-                avm = avp;
-                um = u0;
-                u0 = up;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPF; u++) { b_av[u] = n_av[u]; b_dt[u] = n_dt[u]; b_up[u] = n_up[u]; b_rh[u] = n_rh[u]; }
-    }
-    auto finish = [&](long p, double x) {
-        if (D < 2) {
-            rh[p] = x;
-        } else {
-            const long g = L.off + p;
-            const double qv = F.q[4][g];
-            const double qe5 = qv + x;  // qold == q5 at this point of the sub-iteration
-            const double dum = 1.0 + 1.5 * C.e_source * F.dtev[g] / C.delti;
-            F.q[4][g] = qv + (-qv + qe5) / dum;
-        }
-    };
-    double x = dprev;
-    finish(base + (long)len * sD, x);
-    for (int s0 = len - 2; s0 >= 0; s0 -= kPF) {
-        double dd[kPF], cc[kPF], qq[kPF], dv[kPF];
-#pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 - u;
-            const long p = base + (1 + s) * sD;
-            dd[u] = s >= 0 ? rh[p] : 0.0;
-            cc[u] = s >= 0 ? cp[p] : 0.0;
-            if (D == 2) {
-                qq[u] = s >= 0 ? F.q[4][L.off + p] : 0.0;
-                dv[u] = s >= 0 ? F.dtev[L.off + p] : 0.0;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPF; u++) {
-            const int s = s0 - u;
-            if (s >= 0) {
-                x = dd[u] - cc[u] * x;
-                const long p = base + (1 + s) * sD;
-                if (D < 2) {
-                    rh[p] = x;
-                } else {
-                    // the update of finish() with q5 and dtev already loaded
-                    const double qv = qq[u];
-                    const double qe5 = qv + x;
-                    const double dum = 1.0 + 1.5 * C.e_source * dv[u] / C.delti;
-                    F.q[4][L.off + p] = qv + (-qv + qe5) / dum;
-                }
-            }
-        }
-    }
-    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
-}
-
-// Two-way form of the j and k sweeps: each wave solves 32 lines, lanes
-// 0-31 eliminating the upper halves of the lines downwards (rows 0 .. m-1,
-// the Thomas recurrence of k_sa_adi), lanes 32-63 the lower halves upwards
-// (rows len-1 .. m+1, the mirrored recurrence e' = aw / den,
-// f' = (aq - ae f') / den).  The two meet at row m through a lane shuffle,
-// then substitute outwards.  Twice the waves and half the serial chain of
-// the one-way sweep, which runs one wave per CU at 129^3; rows below m are
-// eliminated in the other order, a rounding-level difference.  Rows where
-// LAPACK SGTSV would pivot are counted on the upper halves only.
-constexpr int kPF2 = 16;  // rows per prefetch group: 16 measured best among 4, 8, 16
-
-template <int D>
-__global__ void __launch_bounds__(128) k_sa_adi2(Level L, Fields F, Consts C) {
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    const bool top = lane < 32;
-    const long nlines = D == 1 ? (long)(ni - 2) * (nk - 2) : (long)(ni - 2) * (nj - 2);
-    const long t = (long)blockIdx.x * 64 + wave * 32 + (lane & 31);
-    const bool act = t < nlines;
-    const long sD = D == 1 ? (long)ni : (long)ni * nj;
-    const int len = D == 1 ? nj - 2 : nk - 2;
-    long base = 0;
-    if (act) base = D == 1 ? (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2))
-                           : (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
-    const double* __restrict__ U = F.ucn[D];
-    const double* __restrict__ av = F.av[D];
-    const double* __restrict__ dtj = F.dtj;
-    double* __restrict__ rh = F.rh5;
-    double* __restrict__ cp = F.cp[0];
-    const int m = len / 2;
-    auto node = [&](int pos) { return base + (long)pos * sD; };  // line position 0 .. len + 1
-    // upper half: carries av(s), U(s), U(s+1); lower half: av(s+1), U(s+1), U(s+2)
-    double a_c = 0.0, u_m = 0.0, u_0 = 0.0;
-    if (act) {
-        if (top) { a_c = av[node(0)]; u_m = U[node(0)]; u_0 = U[node(1)]; }
-        else { a_c = av[node(len)]; u_m = U[node(len + 1)]; u_0 = U[node(len)]; }
-    }
-    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
-    unsigned piv = 0;
-    const int nrows = top ? m : len - 1 - m;
-    for (int g0 = 0; g0 < nrows; g0 += kPF2) {
-        double la[kPF2], lu[kPF2], ld[kPF2], lq[kPF2];
-#pragma unroll
-        for (int u = 0; u < kPF2; u++) {
-            const int r = g0 + u;
-            const int srow = top ? r : len - 1 - r;
-            const bool ok = act && r < nrows;
-            const long p = node(srow + 1);
-            la[u] = ok ? av[top ? node(srow + 1) : node(srow)] : 0.0;
-            lu[u] = ok ? U[top ? node(srow + 2) : node(srow)] : 0.0;
-            ld[u] = ok ? dtj[p] : 0.0;
-            lq[u] = ok ? rh[p] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kPF2; u++) {
-            const int r = g0 + u;
-            if (r < nrows) {
-                const int srow = top ? r : len - 1 - r;
-                double avm, avp, um, up;
-                if (top) { avm = a_c; avp = la[u]; um = u_m; up = lu[u]; }
-                else { avp = a_c; avm = la[u]; up = u_m; um = lu[u]; }
-                const double dt = ld[u];
-                const double aw = -dt * (um + avm);
-                const double ap = 1.0 + dt * (avm + avp);
-                const double ae = dt * (up - avp);
-                const double aq = lq[u];
-                double den;
-                if (r == 0) {
-                    den = ap;
-                } else {
-                    if (top && fabs(denprev) < fabs(aw)) piv++;
-                    den = top ? ap - aw * cprev : ap - ae * cprev;
-                }
-                const double inv = 1.0 / den;
-                if (top) {
-                    cprev = ae * inv;
-                    dprev = (r == 0) ? aq * inv : (aq - aw * dprev) * inv;
-                } else {
-                    cprev = aw * inv;
-                    dprev = (r == 0) ? aq * inv : (aq - ae * dprev) * inv;
-                }
-                denprev = den;
-                if (act) {
-                    const long p = node(srow + 1);
-                    cp[p] = cprev;
-                    rh[p] = dprev;
-                }
-                // carry the shared face and velocities to the next row
-                if (top) { a_c = avp; u_m = u_0; u_0 = up; }
-                else { a_c = avm; u_m = u_0; u_0 = um; }
-            }
-        }
-    }
-    // the middle row m, solved by the upper lane with the lower half's e', f'
-    const double e_lo = __shfl(cprev, (lane & 31) + 32, 64);
-    const double f_lo = __shfl(dprev, (lane & 31) + 32, 64);
-    double xm = 0.0;
-    if (top && act) {
-        const long p = node(m + 1);
-        const double avm = a_c, avp = av[node(m + 1)], um = u_m, up = U[node(m + 2)];
-        const double dt = dtj[p], aq = rh[p];
-        const double aw = -dt * (um + avm);
-        const double ap = 1.0 + dt * (avm + avp);
-        const double ae = dt * (up - avp);
-        if (m > 0 && fabs(denprev) < fabs(aw)) piv++;
-        const double den = (m > 0 ? ap - aw * cprev : ap) - (m < len - 1 ? ae * e_lo : 0.0);
-        const double num = (m > 0 ? aq - aw * dprev : aq) - (m < len - 1 ? ae * f_lo : 0.0);
-        xm = num / den;
-    }
-    xm = __shfl(xm, lane & 31, 64);
-    auto finish = [&](long p, double x) {
-        if (D < 2) {
-            rh[p] = x;
-        } else {
-            const long g = L.off + p;
-            const double qv = F.q[4][g];
-            const double qe5 = qv + x;  // qold == q5 at this point of the sub-iteration
-            const double dum = 1.0 + 1.5 * C.e_source * F.dtev[g] / C.delti;
-            F.q[4][g] = qv + (-qv + qe5) / dum;
-        }
-    };
-    if (top && act) finish(node(m + 1), xm);
-    // substitution outwards: upper rows m-1 .. 0, lower rows m+1 .. len-1
-    double x = xm;
-    const int nb = top ? m : len - 1 - m;
-    for (int g0 = 0; g0 < nb; g0 += kPF2) {
-        double dd[kPF2], cc[kPF2], qq[kPF2], dv[kPF2];
-#pragma unroll
-        for (int u = 0; u < kPF2; u++) {
-            const int r = g0 + u;
-            const int srow = top ? m - 1 - r : m + 1 + r;
-            const bool ok = act && r < nb;
-            const long p = node(srow + 1);
-            dd[u] = ok ? rh[p] : 0.0;
-            cc[u] = ok ? cp[p] : 0.0;
-            if (D == 2) {
-                qq[u] = ok ? F.q[4][L.off + p] : 0.0;
-                dv[u] = ok ? F.dtev[L.off + p] : 0.0;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPF2; u++) {
-            const int r = g0 + u;
-            if (r < nb && act) {
-                const int srow = top ? m - 1 - r : m + 1 + r;
-                x = dd[u] - cc[u] * x;
-                const long p = node(srow + 1);
-                if (D < 2) {
-                    rh[p] = x;
-                } else {
-                    const double qv = qq[u];
-                    const double qe5 = qv + x;
-                    const double dum = 1.0 + 1.5 * C.e_source * dv[u] / C.delti;
-                    F.q[4][L.off + p] = qv + (-qv + qe5) / dum;
-                }
-            }
-        }
-    }
-    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
-}
-
-// The i-direction sweep of k_sa_adi with its rows staged through LDS
-// (line_tile.h); one thread per line, kSL lines per workgroup, and a second
-// wave that only helps to move the tiles.  Chunks of av,
-// U/J, dtj and rh are loaded one chunk ahead of the elimination; c' and d'
-// go back to HBM per chunk except for the last one.  Same arithmetic as
-// k_sa_adi<0>.
-constexpr int kSL = 64;
-constexpr int kSNT = 2 * kSL;  // threads per workgroup
-
-__global__ void __launch_bounds__(kSNT) k_sa_adi_i(Level L, Fields F) {
-    constexpr int NT = kSNT;
-    using IOr = TileIO<NT, kSL, kTR, kTS>;          // rows s0.. at node base + 1 + s
-    using IOa = TileIO<NT, kSL, kTR + 1, kTR + 1>;  // av at nodes base + s0 .. base + s0 + kTR
-    using IOu = TileIO<NT, kSL, kTR + 2, kTR + 3>;  // U/J at nodes base + s0 .. base + s0 + kTR + 1
-    __shared__ long s_base[kSL];
-    __shared__ double s_av[kSL * (kTR + 1)];
-    __shared__ double s_u[kSL * (kTR + 3)];
-    __shared__ double s_dt[kSL * kTS];
-    __shared__ double s_r[kSL * kTS];
-    __shared__ double s_c[kSL * kTS];
-
-    const long nlines = (long)(L.nj - 2) * (L.nk - 2);
-    const long t0 = (long)blockIdx.x * kSL;
-    const int nl = (int)min((long)kSL, nlines - t0);
-    const int l = (int)threadIdx.x;
-    const bool act = l < nl;  // threads kSL.. only move tiles
-    if (l < kSL) s_base[l] = act ? iline_base(L, t0 + l) : 0;
-    __syncthreads();
-
-    const int len = L.ni - 2;
-    const int nch = (len + kTR - 1) / kTR;
-    const double* __restrict__ U = F.ucn[0];
-    const double* __restrict__ av = F.av[0];
-    double* __restrict__ rh = F.rh5;
-    double* __restrict__ cp = F.cp[0];
-    const int lc = l < kSL ? l : 0;
-    double* myr = s_r + lc * kTS;
-    double* myc = s_c + lc * kTS;
-    const double* mydt = s_dt + lc * kTS;
-    const double* mya = s_av + lc * (kTR + 1);
-    const double* myu = s_u + lc * (kTR + 3);
-
-    double va[IOa::kPer], vu[IOu::kPer], vt[IOr::kPer], vr[IOr::kPer];
-    IOa::load(va, av, s_base, nl, 0, len + 1);
-    IOu::load(vu, U, s_base, nl, 0, len + 2);
-    IOr::load(vt, F.dtj + 1, s_base, nl, 0, len);
-    IOr::load(vr, rh + 1, s_base, nl, 0, len);
-
-    double cprev = 0.0, dprev = 0.0, denprev = 1.0;
-    unsigned piv = 0;
-    for (int ch = 0; ch < nch; ch++) {
-        const int s0 = ch * kTR;
-        __syncthreads();
-        IOa::to_lds(va, s_av);
-        IOu::to_lds(vu, s_u);
-        IOr::to_lds(vt, s_dt);
-        IOr::to_lds(vr, s_r);
-        __syncthreads();
-        if (ch + 1 < nch) {
-            const int s1 = s0 + kTR;
-            IOa::load(va, av, s_base, nl, s1, len + 1);
-            IOu::load(vu, U, s_base, nl, s1, len + 2);
-            IOr::load(vt, F.dtj + 1, s_base, nl, s1, len);
-            IOr::load(vr, rh + 1, s_base, nl, s1, len);
-        }
-        if (act) {
-            for (int u = 0; u < kTR; u++) {
-                const int s = s0 + u;
-                if (s >= len) break;
-                const double um = myu[u], up = myu[u + 2];
-                const double avm = mya[u], avp = mya[u + 1], dt = mydt[u];
-                const double aw = -dt * (um + avm);
-                const double ap = 1.0 + dt * (avm + avp);
-                const double ae = dt * (up - avp);
-                const double aq = myr[u];
-                double den;
-                if (s == 0) {
-                    den = ap;
-                } else {
-                    if (fabs(denprev) < fabs(aw)) piv++;
-                    den = ap - aw * cprev;
-                }
-                const double inv = 1.0 / den;
-                cprev = ae * inv;
-                dprev = (s == 0) ? aq * inv : (aq - aw * dprev) * inv;
-                denprev = den;
-                myc[u] = cprev;
-                myr[u] = dprev;
-            }
-        }
-        __syncthreads();
-        if (ch + 1 < nch) {
-            IOr::store(s_c, cp + 1, s_base, nl, s0, len);
-            IOr::store(s_r, rh + 1, s_base, nl, s0, len);
-        }
-    }
-    double x = 0.0;
-    for (int ch = nch - 1; ch >= 0; ch--) {
-        const int s0 = ch * kTR;
-        if (ch < nch - 1) {
-            __syncthreads();
-            IOr::to_lds(vt, s_c);
-            IOr::to_lds(vr, s_r);
-            __syncthreads();
-        }
-        if (ch > 0) {
-            IOr::load(vt, cp + 1, s_base, nl, s0 - kTR, len);
-            IOr::load(vr, rh + 1, s_base, nl, s0 - kTR, len);
-        }
-        if (act) {
-            for (int u = kTR - 1; u >= 0; u--) {
-                const int s = s0 + u;
-                if (s >= len) continue;
-                x = (s == len - 1) ? myr[u] : myr[u] - myc[u] * x;
-                myr[u] = x;
-            }
-        }
-        __syncthreads();
-        IOr::store(s_r, rh + 1, s_base, nl, s0, len);
-    }
-    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
-}
-
 // rhs_sa_bcond faces (rhs_sa_bcond.F90:40-269); faces of direction D, both sides
 template <int D>
 __global__ void k_sa_bc(Level L, Fields F, Consts C) {
@@ -858,6 +434,81 @@ __global__ void k_sa_clamp(Level L, Fields F, Consts C) {
     F.xnut[g] = q * fv1;
 }
 
+// One ADI sweep of rhs_sa_adi_solver (src/rhs_sa_adi_solver.F90:44-244) on
+// every interior line of direction D with the rows on chip (seg_sweep.h):
+//   a = -dtj (U_(s-1) + av_(s-1/2)),  b = 1 + dtj (av_(s-1/2) + av_(s+1/2)),
+//   c = dtj (U_(s+1) - av_(s+1/2)),   right-hand side rh5, solved in place
+// (a line's rows are read before and written after the workgroup's barriers).
+// D = 2 also applies the relaxed update of des_eddy.F90:260-272:
+//   q5 += (-q5 + qold + x) / (1 + 1.5 e_source dtev / delti), qold = q5 here.
+template <int D, int R>
+__global__ void __launch_bounds__(512) k_sa_sweep(Level L, Fields F, Consts C, SegShape sh) {
+    extern __shared__ double smem_sa[];
+    const int P = sh.P, TI = sh.TI, n = sh.n;
+    const int tid = (int)threadIdx.x;
+    const int li = D == 0 ? tid / P : tid % TI;
+    const int ps = D == 0 ? tid % P : tid / TI;
+    const long line0 = (long)blockIdx.x * TI;
+    const long t = line0 + li;
+    const bool act = t < sh.nlines;
+    const int ni = L.ni, nj = L.nj;
+    long base = 0, sD;
+    if (D == 0) {
+        sD = 1;
+        if (act) base = (long)ni * ((1 + t % (nj - 2)) + (long)nj * (1 + t / (nj - 2)));
+    } else if (D == 1) {
+        sD = ni;
+        if (act) base = (1 + t % (ni - 2)) + (long)ni * nj * (1 + t / (ni - 2));
+    } else {
+        sD = (long)ni * nj;
+        if (act) base = (1 + t % (ni - 2)) + (long)ni * (1 + t / (ni - 2));
+    }
+    const int s0 = ps * R;
+    const int nr = act ? max(0, min(R, n - s0)) : 0;
+    const double* __restrict__ U = F.ucn[D];
+    const double* __restrict__ av = F.av[D];
+    const double* __restrict__ dtj = F.dtj;
+    double* __restrict__ rh = F.rh5;
+    double a[R], b[R], c[R], d[R][1];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        a[r] = 0.0; b[r] = 1.0; c[r] = 0.0; d[r][0] = 0.0;
+        if (r < nr) {
+            const long tl = base + (long)(s0 + r + 1) * sD;
+            const double avm = av[tl - sD], avp = av[tl], dt = dtj[tl];
+            a[r] = -dt * (U[tl - sD] + avm);
+            b[r] = 1.0 + dt * (avm + avp);
+            c[r] = dt * (U[tl + sD] - avp);
+            d[r][0] = rh[tl];
+        }
+    }
+    double a_next = 0.0;
+    if (nr > 0 && s0 + nr < n) {
+        const long tl = base + (long)(s0 + nr + 1) * sD;
+        a_next = -dtj[tl] * (U[tl - sD] + av[tl - sD]);
+    }
+    unsigned piv = 0;
+    int zrow = 0;
+    seg_solve<R, 1, double>(a, b, c, d, a_next, nr, n, s0, smem_sa, P, TI, sh.nlines, line0, li, ps, piv, zrow);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (r < nr) {
+            const long tl = base + (long)(s0 + r + 1) * sD;
+            if (D < 2) {
+                rh[tl] = d[r][0];
+            } else {
+                const long g = L.off + tl;
+                const double qv = F.q[4][g];
+                const double qe5 = qv + d[r][0];
+                const double dum = 1.0 + 1.5 * C.e_source * F.dtev[g] / C.delti;
+                F.q[4][g] = qv + (-qv + qe5) / dum;
+            }
+        }
+    }
+    if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
+    if (zrow) atomicMin(F.zero_pivot, (unsigned)zrow);
+}
+
 }  // namespace
 
 int sa_kc_default(const Level& L) {
@@ -878,26 +529,34 @@ void launch_sa_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts&
     hipLaunchKernelGGL(k_sa_rhs, gq, dim3(kQX * kQY), 0, s, L, F, C, kc);
 }
 
+// the ADI sweep of direction D over the interior lines of the fine level
+template <int D>
+static void sa_sweep(hipStream_t s, const Level& L, const Fields& F, const Consts& C) {
+    const int n = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
+    const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
+                                                                    : (long)(L.ni - 2) * (L.nj - 2);
+    const int R = n <= 64 ? 4 : 8;
+    const SegShape sh = seg_shape(n, nl, R, D == 0);
+    const unsigned nb = (unsigned)((sh.nlines + sh.TI - 1) / sh.TI);
+    const size_t lds = sizeof(double) * seg_slot(1) * (size_t)sh.TI * sh.P;
+    if (R == 4)
+        hipLaunchKernelGGL((k_sa_sweep<D, 4>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, sh);
+    else
+        hipLaunchKernelGGL((k_sa_sweep<D, 8>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, sh);
+}
+
 void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk, int kc) {
     const int b = 256;
     const unsigned gn = (unsigned)((L.n + b - 1) / b);
     launch_sa_prep(s, L, F);
-    const int bl = 128;
-    const unsigned gi = (unsigned)(((long)(L.nj - 2) * (L.nk - 2) + kSL - 1) / kSL);
-    const unsigned gj = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + bl - 1) / bl);
-    const unsigned gk = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + bl - 1) / bl);
-    const unsigned gj2 = (unsigned)(((long)(L.ni - 2) * (L.nk - 2) + 63) / 64);
-    const unsigned gk2 = (unsigned)(((long)(L.ni - 2) * (L.nj - 2) + 63) / 64);
     const unsigned fi = (unsigned)(((long)L.nj * L.nk + b - 1) / b);
     const unsigned fj = (unsigned)(((long)L.ni * L.nk + b - 1) / b);
     const unsigned fk = (unsigned)(((long)L.ni * L.nj + b - 1) / b);
     for (int it = 0; it < itk; it++) {
         launch_sa_rhs(s, L, F, C, kc);
-        hipLaunchKernelGGL(k_sa_adi_i, dim3(gi), dim3(kSNT), 0, s, L, F);
-        if (L.nj - 2 >= 4) hipLaunchKernelGGL(k_sa_adi2<1>, dim3(gj2), dim3(128), 0, s, L, F, C);
-        else hipLaunchKernelGGL(k_sa_adi<1>, dim3(gj), dim3(bl), 0, s, L, F, C);
-        if (L.nk - 2 >= 4) hipLaunchKernelGGL(k_sa_adi2<2>, dim3(gk2), dim3(128), 0, s, L, F, C);
-        else hipLaunchKernelGGL(k_sa_adi<2>, dim3(gk), dim3(bl), 0, s, L, F, C);
+        sa_sweep<0>(s, L, F, C);
+        sa_sweep<1>(s, L, F, C);
+        sa_sweep<2>(s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<0>, dim3(fi), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<1>, dim3(fj), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<2>, dim3(fk), dim3(b), 0, s, L, F, C);

@@ -3,8 +3,12 @@ the zero-pivot INFO) and the fallback of the FP32-DAF form and the warm
 starts to the plain FP64 method.  Every condition is forced from the host
 with NaN inputs, so every kernel still runs to its end.
 
-- NaN in the velocity field: the pseudo-iteration's update norms are NaN,
-  the driver stops after that iteration with DESALM_ERR_NONFINITE.
+- NaN in the velocity field of a laminar block: the pseudo-iteration's
+  update norms are NaN, the driver stops after that iteration with
+  DESALM_ERR_NONFINITE.
+- The same NaN with the SA model on: the convection coefficients of the
+  SA line solves through that node are NaN, so their pivots are, and the
+  driver returns the INFO-like row (> 0) before any norm is read.
 - NaN in one grid coordinate: the metrics around it, dtau and the pivots of
   the line solves through it are NaN: the driver returns the INFO-like row
   (> 0).  (A NaN eddy viscosity alone does not reach dtau: the minimum of
@@ -23,12 +27,12 @@ from desalm.lib import ERR_NONFINITE, DesalmError, Solver
 pytestmark = pytest.mark.gpu
 
 
-def small_case(**kw):
-    return cases.wavy(17, 17, 17, ng=2, itm=4, des=True, iters=(1, 2), **kw)
+def small_case(des=True, **kw):
+    return cases.wavy(17, 17, 17, ng=2, itm=4, des=des, iters=(1, 2), **kw)
 
 
 def test_nan_velocity_stops_with_nonfinite_norms(gpu):
-    c = small_case()
+    c = small_case(des=False)
     s = Solver(c.params).load_case(c)
     q, qn, qnm1, xnut = c.state()
     q = q.copy()
@@ -38,6 +42,20 @@ def test_nan_velocity_stops_with_nonfinite_norms(gpu):
         s.mg_driver()
     st = s.last_status()
     assert st["nonfinite_iteration"] == 1 and st["zero_pivot_row"] == 0 and st["fallbacks"] == 0
+    s.close()
+
+
+def test_nan_velocity_with_sa_reports_the_sa_pivot(gpu):
+    c = small_case()
+    s = Solver(c.params).load_case(c)
+    q, qn, qnm1, xnut = c.state()
+    q = q.copy()
+    q[(c.n1 // 2), 1] = np.nan
+    s.set_state(q, qn, qnm1, xnut)
+    with pytest.raises(DesalmError, match=r"rc=[1-9][0-9]*: a line solve met a zero or non-finite pivot"):
+        s.mg_driver()
+    st = s.last_status()
+    assert st["zero_pivot_row"] > 0 and st["nonfinite_iteration"] == 0
     s.close()
 
 
