@@ -39,105 +39,127 @@ namespace desalm {
 
 namespace {
 
-// ---- modal matrices (rhs_modal_matrices.F90:112-264) ----
-// Ma-type matrix for metric row (m1, m2, m3)
-__device__ __forceinline__ void build_m(double m1, double m2, double m3, double sigma, double M[4][4]) {
+// ---- modal matrices (rhs_modal_matrices.F90:112-264), applied in registers ----
+// dtau-dependent factors of a node: sigma = sqrt(beta alpha) with
+// alpha = 1 + e_source 1.5 dtau / delti, and theta = sqrt(beta / alpha) =
+// beta / sigma (rhs_modal_matrices.F90:82-84)
+struct NodeDt {
+    double dt, alpha, sigma, isigma, theta;
+};
+__device__ __forceinline__ NodeDt node_dt(const Consts& C, double dt) {
+    NodeDt n;
+    n.dt = dt;
+    n.alpha = 1.0 + C.e_source * 1.5 * dt / C.delti;
+    n.sigma = sqrt(C.beta * n.alpha);
+    n.isigma = 1.0 / n.sigma;
+    n.theta = C.beta * n.isigma;
+    return n;
+}
+
+// v <- M v for the Ma-type matrix of metric row (m1, m2, m3):
+//   row 0 = (0, 0, sigma, sigma); rows 1-3 = (t1/s, t2/(s sg), m/sg, -m/sg)
+//   with sg = |m|, s = sqrt(2 |m|^2 + (m2 - m3)^2)
+__device__ __forceinline__ void mul_m(double m1, double m2, double m3, double sigma, double v[4]) {
     const double g = m1 * m1 + m2 * m2 + m3 * m3;
-    const double sg = sqrt(g);
-    const double s = sqrt(2.0 * g + (m2 - m3) * (m2 - m3));
-    M[0][0] = 0.0; M[0][1] = 0.0; M[0][2] = sigma; M[0][3] = sigma;
-    M[1][0] = -(m2 + m3) / s; M[1][1] = m1 * (m3 - m2) / (s * sg); M[1][2] = m1 / sg; M[1][3] = -m1 / sg;
-    M[2][0] = m1 / s; M[2][1] = (g + m2 * (m3 - m2)) / (s * sg); M[2][2] = m2 / sg; M[2][3] = -m2 / sg;
-    M[3][0] = m1 / s; M[3][1] = -(g - m3 * (m3 - m2)) / (s * sg); M[3][2] = m3 / sg; M[3][3] = -m3 / sg;
+    const double isg = 1.0 / sqrt(g);
+    const double is = 1.0 / sqrt(2.0 * g + (m2 - m3) * (m2 - m3));
+    const double w0 = v[0] * is, w1 = v[1] * (is * isg), u = (v[2] - v[3]) * isg;
+    const double dm = m3 - m2;
+    const double o0 = sigma * (v[2] + v[3]);
+    const double o1 = -(m2 + m3) * w0 + m1 * dm * w1 + m1 * u;
+    const double o2 = m1 * w0 + (g + m2 * dm) * w1 + m2 * u;
+    const double o3 = m1 * w0 - (g - m3 * dm) * w1 + m3 * u;
+    v[0] = o0; v[1] = o1; v[2] = o2; v[3] = o3;
 }
 
-// Ma^-1-type matrix for metric row (m1, m2, m3)
-__device__ __forceinline__ void build_minv(double m1, double m2, double m3, double sigma, double Mi[4][4]) {
+// v <- M^-1 v for the same matrix, given g = |m|^2 and sg = |m| (the inverse
+// in closed form, d = |m|^2 + 2 m2 m3 + m1^2)
+__device__ __forceinline__ void mul_minv(double m1, double m2, double m3, double g, double sg, double isigma,
+                                         double v[4]) {
     const double m1s = m1 * m1, m2s = m2 * m2, m3s = m3 * m3;
-    const double g = m1s + m2s + m3s;
-    const double sg = sqrt(g);
     const double s = sqrt(2.0 * g + (m2 - m3) * (m2 - m3));
-    const double d = g + 2.0 * m2 * m3 + m1 * m1;
+    const double id = 1.0 / (g + 2.0 * m2 * m3 + m1s);
+    const double isg = 1.0 / sg;
     const double r1 = m2 + m3, r2 = m2 - m3, r3 = m2s - m3s;
-    Mi[0][0] = 0.0; Mi[0][1] = -s * (m2 + m3) / d; Mi[0][2] = s * m1 / d; Mi[0][3] = Mi[0][2];
-    const double ssgd = s / sg / d;
-    const double sgd2 = 1.0 / sg / d / 2.0;
-    Mi[1][0] = 0.0; Mi[1][1] = m1 * (m3 - m2) * ssgd;
-    Mi[1][2] = (m2 * m3 + m1s + m3s) * ssgd; Mi[1][3] = -(m2 * m3 + m1s + m2s) * ssgd;
-    Mi[2][0] = 0.5 / sigma; Mi[2][1] = m1 * (2.0 * g + 2.0 * m2 * m3 - m2s - m3s) * sgd2;
-    Mi[2][2] = (g * r1 + m1s * r2 + m3 * r3) * sgd2; Mi[2][3] = (g * r1 - m1s * r2 - m2 * r3) * sgd2;
-    Mi[3][0] = 0.5 / sigma; Mi[3][1] = -Mi[2][1]; Mi[3][2] = -Mi[2][2]; Mi[3][3] = -Mi[2][3];
+    const double o0 = s * id * (m1 * (v[2] + v[3]) - r1 * v[1]);
+    const double o1 = s * isg * id *
+                      (m1 * (m3 - m2) * v[1] + (m2 * m3 + m1s + m3s) * v[2] - (m2 * m3 + m1s + m2s) * v[3]);
+    const double t = 0.5 * isigma * v[0];
+    const double p = 0.5 * isg * id *
+                     (m1 * (2.0 * g + 2.0 * m2 * m3 - m2s - m3s) * v[1] + (g * r1 + m1s * r2 + m3 * r3) * v[2] +
+                      (g * r1 - m1s * r2 - m2 * r3) * v[3]);
+    v[0] = o0; v[1] = o1; v[2] = t + p; v[3] = t - p;
 }
 
-__device__ __forceinline__ void matmul4(const double a[4][4], const double b[4][4], double c[4][4]) {
+// S (-dtau J rh) at global node g (rhs_diag_solver.F90:56-72): beta on the
+// pressure row, 1 / alpha on the velocity rows
+__device__ __forceinline__ void scaled_rh(const Fields& F, const Consts& C, long g, const NodeDt& nd, double v[4]) {
+    const double f = -nd.dt * F.aj[g];
+    const double ia = 1.0 / nd.alpha;
+    v[0] = C.beta * (f * F.rh[0][g]);
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc = acc + a[r][t] * b[t][s];
-            c[r][s] = acc;
-        }
+    for (int m = 1; m < 4; m++) v[m] = ia * (f * F.rh[m][g]);
 }
 
-__device__ __forceinline__ void matvec4(const double a[4][4], double v[4]) {
-    double rp[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc = acc + a[r][c] * v[c];
-        rp[r] = acc;
+// the metric row of direction D at node g
+template <int D>
+__device__ __forceinline__ void metric_row(const Fields& F, long g, double& m1, double& m2, double& m3) {
+    const double* const* mt = D == 0 ? F.csi : D == 1 ? F.eta : F.zet;
+    m1 = mt[0][g]; m2 = mt[1][g]; m3 = mt[2][g];
+}
+
+// The modal transform that precedes sweep D at node g, applied to v
+// (Ma^-1; N1^-1 = Mb^-1 Ma; N2^-1 = Mc^-1 Mb, applied factor by factor),
+// and dtau times the spectral radius of direction D at g,
+// dtau theta |grad xi_D| (rhs_diag_solver.F90:149-160).
+// D = 0 builds its input S (-dtau J rh) here; D = 1, 2 take v as given.
+template <int D>
+__device__ __forceinline__ double transform_row(const Fields& F, const Consts& C, long g, double v[4]) {
+    const NodeDt nd = node_dt(C, F.dtau[g]);
+    if (D == 0) scaled_rh(F, C, g, nd, v);
+    if (D > 0) {
+        double a1, a2, a3;
+        metric_row<D - 1>(F, g, a1, a2, a3);
+        mul_m(a1, a2, a3, nd.sigma, v);
     }
-#pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = rp[r];
+    double m1, m2, m3;
+    metric_row<D>(F, g, m1, m2, m3);
+    const double gg = m1 * m1 + m2 * m2 + m3 * m3;
+    const double sg = sqrt(gg);
+    mul_minv(m1, m2, m3, gg, sg, nd.isigma, v);
+    return nd.dt * (nd.theta * sg);
 }
 
-// sigma of the modal matrices at a node (rhs_modal_matrices.F90:82-84)
-__device__ __forceinline__ double modal_sigma(const Consts& C, double dtau) {
-    const double alpha = 1.0 + C.e_source * 1.5 * dtau / C.delti;
-    return sqrt(C.beta * alpha);
-}
-__device__ __forceinline__ double modal_theta(const Consts& C, double dtau) {
-    const double alpha = 1.0 + C.e_source * 1.5 * dtau / C.delti;
-    return sqrt(C.beta / alpha);
+// dtau times the spectral radius of direction D alone
+template <int D>
+__device__ __forceinline__ double row_sd(const Fields& F, const Consts& C, long g) {
+    const NodeDt nd = node_dt(C, F.dtau[g]);
+    double m1, m2, m3;
+    metric_row<D>(F, g, m1, m2, m3);
+    return nd.dt * (nd.theta * sqrt(m1 * m1 + m2 * m2 + m3 * m3));
 }
 
-// v <- Ma^-1 v at global node g
-__device__ __forceinline__ void apply_mai(const Fields& F, const Consts& C, long g, double v[4]) {
-    double M[4][4];
-    build_minv(F.csi[0][g], F.csi[1][g], F.csi[2][g], modal_sigma(C, F.dtau[g]), M);
-    matvec4(M, v);
+// the transform of sweep D alone (boundary vectors)
+template <int D>
+__device__ __forceinline__ void pre_transform(const Fields& F, const Consts& C, long g, double v[4]) {
+    const NodeDt nd = node_dt(C, F.dtau[g]);
+    if (D > 0) {
+        double a1, a2, a3;
+        metric_row<D - 1>(F, g, a1, a2, a3);
+        mul_m(a1, a2, a3, nd.sigma, v);
+    }
+    double m1, m2, m3;
+    metric_row<D>(F, g, m1, m2, m3);
+    const double gg = m1 * m1 + m2 * m2 + m3 * m3;
+    mul_minv(m1, m2, m3, gg, sqrt(gg), nd.isigma, v);
 }
-// v <- (Mb^-1 Ma) v
-__device__ __forceinline__ void apply_n1i(const Fields& F, const Consts& C, long g, double v[4]) {
-    const double sigma = modal_sigma(C, F.dtau[g]);
-    double A[4][4], B[4][4], P[4][4];
-    build_m(F.csi[0][g], F.csi[1][g], F.csi[2][g], sigma, A);
-    build_minv(F.eta[0][g], F.eta[1][g], F.eta[2][g], sigma, B);
-    matmul4(B, A, P);
-    matvec4(P, v);
-}
-// v <- (Mc^-1 Mb) v
-__device__ __forceinline__ void apply_n2i(const Fields& F, const Consts& C, long g, double v[4]) {
-    const double sigma = modal_sigma(C, F.dtau[g]);
-    double A[4][4], B[4][4], P[4][4];
-    build_m(F.eta[0][g], F.eta[1][g], F.eta[2][g], sigma, A);
-    build_minv(F.zet[0][g], F.zet[1][g], F.zet[2][g], sigma, B);
-    matmul4(B, A, P);
-    matvec4(P, v);
-}
-// v <- Mc v
+
+// v <- Mc v at global node g (rhs_diag_solver.F90:624-633)
 __device__ __forceinline__ void apply_mc(const Fields& F, const Consts& C, long g, double v[4]) {
-    double M[4][4];
-    build_m(F.zet[0][g], F.zet[1][g], F.zet[2][g], modal_sigma(C, F.dtau[g]), M);
-    matvec4(M, v);
+    const NodeDt nd = node_dt(C, F.dtau[g]);
+    mul_m(F.zet[0][g], F.zet[1][g], F.zet[2][g], nd.sigma, v);
 }
 
-// spectral radius component D at a node times dtau: returns (dtau, spr)
-
-// rows per segment: registers per thread hold R rows of the four systems
 template <typename S>
 struct SweepIO {
     const S* xin[4];  // j and k sweeps: the previous sweep's solution (level-local)
@@ -152,14 +174,11 @@ struct SweepShape {
     long nlines;
 };
 
-// values of one segment slot in LDS (the largest phase: 3 Moebius maps)
+// values of one segment slot in LDS (the largest phase: 3 Moebius maps);
+// a line's slots are followed by one pad value so that the scanning threads
+// of adjacent lines fall on different LDS banks
 constexpr int kSlot = 12;
-
-// sigma, theta of the modal matrices at a node from its dtau
-// (rhs_modal_matrices.F90:82-84)
-__device__ __forceinline__ double node_alpha(const Consts& C, double dtau) {
-    return 1.0 + C.e_source * 1.5 * dtau / C.delti;
-}
+__host__ __device__ constexpr int line_stride(int P) { return P * kSlot + 1; }
 
 // tridiagonal row of matrix mi (eigenvalue 0, +1, -1) from sd = dtau spr of
 // the row's node and its two neighbours (rhs_diag_solver.F90:149-189); the
@@ -184,26 +203,15 @@ __device__ __forceinline__ void row_coef(int mi, S sdm, S sd0, S sdp, S h, S h2,
 }
 
 // matrix of system m: systems 0 and 1 share eigenvalue 0
-__device__ __forceinline__ int mat_of(int m) { return m < 2 ? 0 : m - 1; }
+__host__ __device__ constexpr int mat_of(int m) { return m < 2 ? 0 : m - 1; }
 
-// S (-dtau J rh) at global node g (rhs_diag_solver.F90:56-72)
-__device__ __forceinline__ void scaled_rh(const Fields& F, const Consts& C, long g, double v[4]) {
-    const double dt = F.dtau[g];
-    const double alpha = 1.0 + C.e_source * 1.5 / C.delti * dt;
-    double r[4];
-#pragma unroll
-    for (int m = 0; m < 4; m++) r[m] = -dt * F.rh[m][g] * F.aj[g];
-    v[0] = C.beta * r[0];
-#pragma unroll
-    for (int m = 1; m < 4; m++) v[m] = 1.0 / alpha * r[m];
-}
-
-// the modal transform of sweep D applied to v at global node g
-template <int D>
-__device__ __forceinline__ void pre_transform(const Fields& F, const Consts& C, long g, double v[4]) {
-    if (D == 0) apply_mai(F, C, g, v);
-    else if (D == 1) apply_n1i(F, C, g, v);
-    else apply_n2i(F, C, g, v);
+// scale a projective pair or map by a power of two (exact) so that its
+// largest entry lies in [0.5, 1)
+template <typename S>
+__device__ __forceinline__ int pow2_exponent(S mx) {
+    int e = 0;
+    (void)frexp(mx, &e);
+    return e;
 }
 
 // One direction's DAF line solves on level L.
@@ -222,6 +230,7 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     extern __shared__ double smem_d[];
     S* lds = reinterpret_cast<S*>(smem_d);
     const int P = sh.P, TI = sh.TI, n = sh.n;
+    const int LS = line_stride(P);
     const int tid = (int)threadIdx.x;
     // i lines are contiguous: segments of one line on adjacent lanes; j and k
     // lines: adjacent lines (adjacent i) on adjacent lanes
@@ -251,42 +260,35 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     auto gnode = [&](int p) { return L.off + base + (long)p * sD; };  // global node at position p
 
     // ---- load: right-hand sides (transformed) and dtau spr of rows s0-1 .. s0+nr ----
+    // window entry w is node position s0 + w (row s0 + w - 1)
     S dv[R][4];
     S sdv[R + 2];
 #pragma unroll
-    for (int w = 0; w < R + 2; w++) {
-        const int p = s0 + w;  // node position of window entry w (row s0 + w - 1)
-        sdv[w] = S(0);
-        if (w <= nr + 1 && p <= nd - 1 && nr > 0) {
-            const long g = gnode(p);
-            const double* m0 = D == 0 ? F.csi[0] : D == 1 ? F.eta[0] : F.zet[0];
-            const double* m1 = D == 0 ? F.csi[1] : D == 1 ? F.eta[1] : F.zet[1];
-            const double* m2 = D == 0 ? F.csi[2] : D == 1 ? F.eta[2] : F.zet[2];
-            const double dt = F.dtau[g];
-            const double gg = m0[g] * m0[g] + m1[g] * m1[g] + m2[g] * m2[g];
-            sdv[w] = (S)(dt * (sqrt(C.beta / node_alpha(C, dt)) * sqrt(gg)));
-        }
-    }
-#pragma unroll
     for (int r = 0; r < R; r++) {
+        sdv[r + 1] = S(0);
+#pragma unroll
+        for (int m = 0; m < 4; m++) dv[r][m] = S(0);
         if (r < nr) {
             const int p = s0 + r + 1;
-            const long g = gnode(p);
-            double v[4];
-            if (D == 0) {
-                scaled_rh(F, C, g, v);
-            } else {
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+            if (D > 0) {
                 const long tl = base + (long)p * sD;
 #pragma unroll
                 for (int m = 0; m < 4; m++) v[m] = (double)io.xin[m][tl];
             }
-            pre_transform<D>(F, C, g, v);
+            sdv[r + 1] = (S)transform_row<D>(F, C, gnode(p), v);
 #pragma unroll
             for (int m = 0; m < 4; m++) dv[r][m] = (S)v[m];
-        } else {
-#pragma unroll
-            for (int m = 0; m < 4; m++) dv[r][m] = S(0);
         }
+    }
+    sdv[0] = nr > 0 ? (S)row_sd<D>(F, C, gnode(s0)) : S(0);
+    {
+        const int pe = s0 + nr + 1;
+        const S se = (nr > 0 && pe <= nd - 1) ? (S)row_sd<D>(F, C, gnode(pe)) : S(0);
+        sdv[R + 1] = S(0);
+#pragma unroll
+        for (int w = 1; w <= R + 1; w++)
+            if (w == nr + 1) sdv[w] = se;
     }
     // semi-explicit boundary vectors of the first and last rows
     const bool first = act && s0 == 0 && nr > 0;
@@ -296,8 +298,9 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
         const int b0 = 2 * D;
         if (D == 0) {
             double r1[4], r2[4];
-            scaled_rh(F, C, gnode(1), r1);
-            scaled_rh(F, C, gnode(2), r2);
+            const long g1 = gnode(1), g2 = gnode(2);
+            scaled_rh(F, C, g1, node_dt(C, F.dtau[g1]), r1);
+            scaled_rh(F, C, g2, node_dt(C, F.dtau[g2]), r2);
 #pragma unroll
             for (int m = 0; m < 4; m++) w[m] = C.sa[b0][m] * r1[m] + C.sb[b0][m] * r2[m];
         } else {
@@ -322,10 +325,10 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
             // writes brh(:,2) on a single rank (:102, :135), so it is used
             // as it is
             if (use_b1) {
-                double r2[4], r3[4];
-                scaled_rh(F, C, gnode(2), r2);
-                scaled_rh(F, C, gnode(3), r3);
-                w[0] = C.sa[0][4] * r2[0] + C.sb[0][4] * r3[0];
+                const long g2 = gnode(2), g3 = gnode(3);
+                const double x2 = -F.dtau[g2] * F.rh[0][g2] * F.aj[g2];
+                const double x3 = -F.dtau[g3] * F.rh[0][g3] * F.aj[g3];
+                w[0] = C.sa[0][4] * (C.beta * x2) + C.sb[0][4] * (C.beta * x3);
             }
         } else if (D == 1 || b2_full) {
 #pragma unroll
@@ -346,20 +349,27 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
         }
         const int r = nr - 1;
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            S am, bm, cm;
-            row_coef<S>(mat_of(m), sdv[r], sdv[r + 1], sdv[r + 2], h, h2, hsq, epd, outlet_lines, am, bm, cm);
-            dv[r][m] = dv[r][m] - cm * (S)w[m];
+        for (int rr = 0; rr < R; rr++) {
+            if (rr == r) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    S am, bm, cm;
+                    row_coef<S>(mat_of(m), sdv[rr], sdv[rr + 1], sdv[rr + 2], h, h2, hsq, epd, outlet_lines, am, bm,
+                                cm);
+                    dv[rr][m] = dv[rr][m] - cm * (S)w[m];
+                }
+            }
         }
     }
     const bool outlet_row_here = outlet_lines && last;  // row n - 1 is the outlet row
     auto coef = [&](int r, int mi, S& a, S& b, S& c) {
         row_coef<S>(mi, sdv[r], sdv[r + 1], sdv[r + 2], h, h2, hsq, epd, outlet_row_here && r == nr - 1, a, b, c);
     };
-    S* slot = lds + (long)(li * P + ps) * kSlot;
+    S* slot = lds + li * LS + ps * kSlot;
     const bool scanner = tid < 4 * TI;
     const int sl = tid / 4, sm = tid % 4;
     const bool scan_act = scanner && (long)blockIdx.x * TI + sl < sh.nlines;
+    S* sline = lds + sl * LS;  // the scanned line's slots
 
     // ---- phase A: Moebius maps of c' over the segment, scanned ----
     {
@@ -376,9 +386,8 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
                     // [c'; 1] -> [c; b - a c']
                     const S n0 = c * M[mi][2], n1 = c * M[mi][3];
                     const S n2 = b * M[mi][2] - a * M[mi][0], n3 = b * M[mi][3] - a * M[mi][1];
-                    const S sc = fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3)));
-                    const S is = sc > S(0) ? S(1) / sc : S(1);
-                    M[mi][0] = n0 * is; M[mi][1] = n1 * is; M[mi][2] = n2 * is; M[mi][3] = n3 * is;
+                    const int e = -pow2_exponent(fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3))));
+                    M[mi][0] = ldexp(n0, e); M[mi][1] = ldexp(n1, e); M[mi][2] = ldexp(n2, e); M[mi][3] = ldexp(n3, e);
                 }
             }
         }
@@ -389,12 +398,25 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     }
     __syncthreads();
     if (scan_act && sm < 3) {
-        S cin = S(0);
-        for (int sg = 0; sg < P; sg++) {
-            S* q = lds + (long)(sl * P + sg) * kSlot + 4 * sm;
-            const S m0 = q[0], m1 = q[1], m2 = q[2], m3 = q[3];
-            q[0] = cin;
-            cin = (m0 * cin + m1) / (m2 * cin + m3);
+        // projective pair (cn, cd): c' entering the segment is cn / cd
+        S cn = S(0), cd = S(1);
+        for (int sg0 = 0; sg0 < P; sg0 += 4) {
+            S q[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (sg0 + u < P)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) q[u][e] = sline[(sg0 + u) * kSlot + 4 * sm + e];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (sg0 + u < P) {
+                    sline[(sg0 + u) * kSlot + 4 * sm] = cn / cd;
+                    const S nn = q[u][0] * cn + q[u][1] * cd, dd = q[u][2] * cn + q[u][3] * cd;
+                    const int e = -pow2_exponent(fmax(fabs(nn), fabs(dd)));
+                    cn = ldexp(nn, e);
+                    cd = ldexp(dd, e);
+                }
+            }
         }
     }
     __syncthreads();
@@ -402,8 +424,8 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
 #pragma unroll
     for (int mi = 0; mi < 3; mi++) cst[mi] = slot[4 * mi];
 
-    // ---- phase B: exact c' over the rows; affine maps of d', scanned ----
-    S cp[R][3];
+    // ---- phase B: exact c' over the rows (their 1 / den kept); affine maps of d', scanned ----
+    S inv[R][3];
     unsigned piv = 0;
     int zrow = 0;  // first row (1-based in the line) with a zero or non-finite pivot
     {
@@ -415,26 +437,28 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
             S cprev = cst[mi];
 #pragma unroll
             for (int r = 0; r < R; r++) {
+                inv[r][mi] = S(0);
                 if (r < nr) {
                     S a, b, c;
                     coef(r, mi, a, b, c);
                     const S den = b - a * cprev;
-                    const S inv = S(1) / den;
-                    cprev = c * inv;
-                    cp[r][mi] = cprev;
+                    const S iv = S(1) / den;
+                    inv[r][mi] = iv;
+                    cprev = c * iv;
                     if (!(den != S(0) && isfinite(den)) && zrow == 0) zrow = s0 + r + 1;
                     // SGTSV's interchange test against the next row's subdiagonal
+                    // (a depends on the row's sd and its lower neighbour's only)
                     if (s0 + r + 1 < n) {
                         S an, bn, cn;
-                        row_coef<S>(mi, sdv[r + 1], sdv[r + 2], r + 3 < R + 2 ? sdv[min(r + 3, R + 1)] : S(0), h,
-                                    h2, hsq, epd, outlet_lines && s0 + r + 2 == n, an, bn, cn);
+                        row_coef<S>(mi, sdv[r + 1], sdv[r + 2], S(0), h, h2, hsq, epd, outlet_lines && s0 + r + 2 == n,
+                                    an, bn, cn);
                         if (fabs(den) < fabs(an)) piv += mi == 0 ? 2u : 1u;
                     }
 #pragma unroll
                     for (int m = 0; m < 4; m++) {
                         if (mat_of(m) == mi) {
-                            const S v = -a * inv;
-                            U[m] = dv[r][m] * inv + v * U[m];
+                            const S v = -a * iv;
+                            U[m] = dv[r][m] * iv + v * U[m];
                             V[m] = v * V[m];
                         }
                     }
@@ -447,11 +471,18 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     __syncthreads();
     if (scan_act) {
         S din = S(0);
-        for (int sg = 0; sg < P; sg++) {
-            S* q = lds + (long)(sl * P + sg) * kSlot + 2 * sm;
-            const S u = q[0], v = q[1];
-            q[0] = din;
-            din = u + v * din;
+        for (int sg0 = 0; sg0 < P; sg0 += 8) {
+            S u[8], v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (sg0 + k < P) { u[k] = sline[(sg0 + k) * kSlot + 2 * sm]; v[k] = sline[(sg0 + k) * kSlot + 2 * sm + 1]; }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (sg0 + k < P) {
+                    sline[(sg0 + k) * kSlot + 2 * sm] = din;
+                    din = u[k] + v[k] * din;
+                }
+            }
         }
     }
     __syncthreads();
@@ -468,12 +499,10 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
                 for (int mi = 0; mi < 3; mi++) {
                     S a, b, c;
                     coef(r, mi, a, b, c);
-                    const S den = b - a * (r == 0 ? cst[mi] : cp[r - 1][mi]);
-                    const S inv = S(1) / den;
 #pragma unroll
                     for (int m = 0; m < 4; m++) {
                         if (mat_of(m) == mi) {
-                            dprev[m] = (dv[r][m] - a * dprev[m]) * inv;
+                            dprev[m] = (dv[r][m] - a * dprev[m]) * inv[r][mi];
                             dv[r][m] = dprev[m];
                         }
                     }
@@ -487,10 +516,17 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
         for (int r = R - 1; r >= 0; r--) {
             if (r < nr) {
 #pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    const S cc = cp[r][mat_of(m)];
-                    A[m] = dv[r][m] - cc * A[m];
-                    B[m] = -cc * B[m];
+                for (int mi = 0; mi < 3; mi++) {
+                    S a, b, c;
+                    coef(r, mi, a, b, c);
+                    const S cc = c * inv[r][mi];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        if (mat_of(m) == mi) {
+                            A[m] = dv[r][m] - cc * A[m];
+                            B[m] = -cc * B[m];
+                        }
+                    }
                 }
             }
         }
@@ -500,11 +536,18 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     __syncthreads();
     if (scan_act) {
         S xin = S(0);
-        for (int sg = P - 1; sg >= 0; sg--) {
-            S* q = lds + (long)(sl * P + sg) * kSlot + 2 * sm;
-            const S a = q[0], b = q[1];
-            q[0] = xin;
-            xin = a + b * xin;
+        for (int sg1 = P - 1; sg1 >= 0; sg1 -= 8) {
+            S u[8], v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (sg1 - k >= 0) { u[k] = sline[(sg1 - k) * kSlot + 2 * sm]; v[k] = sline[(sg1 - k) * kSlot + 2 * sm + 1]; }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (sg1 - k >= 0) {
+                    sline[(sg1 - k) * kSlot + 2 * sm] = xin;
+                    xin = u[k] + v[k] * xin;
+                }
+            }
         }
     }
     __syncthreads();
@@ -516,12 +559,20 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
 #pragma unroll
     for (int r = R - 1; r >= 0; r--) {
         if (r < nr) {
-            const int p = s0 + r + 1;
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                x[m] = dv[r][m] - cp[r][mat_of(m)] * x[m];
-                dv[r][m] = x[m];
+            for (int mi = 0; mi < 3; mi++) {
+                S a, b, c;
+                coef(r, mi, a, b, c);
+                const S cc = c * inv[r][mi];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    if (mat_of(m) == mi) {
+                        x[m] = dv[r][m] - cc * x[m];
+                        dv[r][m] = x[m];
+                    }
+                }
             }
+            const int p = s0 + r + 1;
             const long tl = base + (long)p * sD;
             if (D < 2) {
 #pragma unroll
@@ -633,10 +684,12 @@ static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Cons
     // a characteristic outlet at i = ni - 1 joins the fine level's sweeps
     const int ext = (n == 1 && C.btype[1] == DESALM_BC_OUTLET_MOC);
     const int len = D == 0 ? L.ni - 2 + ext : D == 1 ? L.nj - 2 : L.nk - 2;
-    const int R = len <= 64 ? 4 : 8;
+    // i lines: short segments (their rows are adjacent in memory, so lanes
+    // reading row r of adjacent segments are R rows apart)
+    const int R = (D == 0 || len <= 64) ? 4 : 8;
     const SweepShape sh = sweep_shape(L, D, ext, R);
     const unsigned nb = (unsigned)((sh.nlines + sh.TI - 1) / sh.TI);
-    const size_t lds = sizeof(S) * kSlot * (size_t)sh.TI * sh.P;
+    const size_t lds = sizeof(S) * (size_t)sh.TI * line_stride(sh.P);
     if (R == 4)
         hipLaunchKernelGGL((k_daf_sweep<D, 4, S>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, io, sh, use_b1,
                            use_b2, b2_full, ext, dq_out);

@@ -414,10 +414,12 @@ static void rs_sweep(hipStream_t s, const Level& L, const Fields& F, double ep) 
     const int n = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
     const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
                                                                     : (long)(L.ni - 2) * (L.nj - 2);
-    const int R = n <= 64 ? 4 : 8;
+    // i lines: short segments (their rows are adjacent in memory, so lanes
+    // reading row r of adjacent segments are R rows apart)
+    const int R = (D == 0 || n <= 64) ? 4 : 8;
     const SegShape sh = seg_shape(n, nl, R, D == 0);
     const unsigned nb = (unsigned)((sh.nlines + sh.TI - 1) / sh.TI);
-    const size_t lds = sizeof(double) * seg_slot(4) * (size_t)sh.TI * sh.P;
+    const size_t lds = seg_lds_bytes(sh, 4);
     if (R == 4)
         hipLaunchKernelGGL((k_rs_sweep<D, 4>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, sh, ep);
     else

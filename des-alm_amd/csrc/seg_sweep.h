@@ -34,6 +34,12 @@ struct SegShape {
 
 // LDS values per segment slot: the larger of 4 (a Moebius map) and 2 NS
 __host__ __device__ constexpr int seg_slot(int ns) { return 2 * ns > 4 ? 2 * ns : 4; }
+// LDS values per line: its P slots and one pad value, so that the scanning
+// threads of adjacent lines fall on different LDS banks
+__host__ __device__ constexpr int seg_line_stride(int P, int ns) { return P * seg_slot(ns) + 1; }
+inline size_t seg_lds_bytes(const SegShape& sh, int ns) {
+    return sizeof(double) * (size_t)sh.TI * seg_line_stride(sh.P, ns);
+}
 
 // Segments of R rows; j and k lines (seg_fast = false) put 16 adjacent lines
 // (adjacent i) on adjacent lanes, i lines (seg_fast = true) put the segments
@@ -49,21 +55,30 @@ inline SegShape seg_shape(int n, long nlines, int R, bool seg_fast) {
     return sh;
 }
 
+// exponent e of mx = f 2^e with f in [0.5, 1): scaling by 2^-e is exact
+__device__ __forceinline__ int seg_exponent(double mx) {
+    int e = 0;
+    (void)frexp(mx, &e);
+    return e;
+}
+
 // The solve.  On entry a, b, c, d hold the thread's nr rows (row r is line
 // row s0 + r), a_next the subdiagonal of row s0 + nr when that row exists
 // (for the interchange test).  On exit d holds the solution x.  Every thread
 // of the workgroup must call it (it contains barriers); lds holds
-// TI * P * seg_slot(NS) values of S; slot = this thread's slot index
-// li * P + ps.
+// TI * seg_line_stride(P, NS) values of S; this thread's slot is segment ps
+// of line li.
 template <int R, int NS, typename S>
 __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_next, int nr, int n, int s0,
                           S* lds, int P, int TI, long nlines, long line0, int li, int ps, unsigned& piv,
                           int& zrow) {
     constexpr int K = seg_slot(NS);
+    const int LS = seg_line_stride(P, NS);
     const int tid = (int)threadIdx.x;
-    S* slot = lds + (long)(li * P + ps) * K;
+    S* slot = lds + li * LS + ps * K;
     const int sl = tid / NS, sm = tid % NS;  // scanner: line, right-hand side
     const bool scanner = tid < NS * TI && line0 + sl < nlines;
+    S* sline = lds + sl * LS;
 
     // phase A: Moebius maps of c' over the segment
     {
@@ -74,29 +89,41 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
                 // [c'; 1] -> [c; b - a c']
                 const S n0 = c[r] * m2, n1 = c[r] * m3;
                 const S n2 = b[r] * m2 - a[r] * m0, n3 = b[r] * m3 - a[r] * m1;
-                const S sc = fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3)));
-                const S is = sc > S(0) ? S(1) / sc : S(1);
-                m0 = n0 * is; m1 = n1 * is; m2 = n2 * is; m3 = n3 * is;
+                const int e = -seg_exponent(fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3))));
+                m0 = ldexp(n0, e); m1 = ldexp(n1, e); m2 = ldexp(n2, e); m3 = ldexp(n3, e);
             }
         }
         slot[0] = m0; slot[1] = m1; slot[2] = m2; slot[3] = m3;
     }
     __syncthreads();
     if (scanner && sm == 0) {
-        S cin = S(0);
-        for (int sg = 0; sg < P; sg++) {
-            S* q = lds + (long)(sl * P + sg) * K;
-            const S q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-            q[0] = cin;
-            cin = (q0 * cin + q1) / (q2 * cin + q3);
+        // projective pair (cn, cd): c' entering the segment is cn / cd
+        S cn = S(0), cd = S(1);
+        for (int sg0 = 0; sg0 < P; sg0 += 4) {
+            S q[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (sg0 + u < P)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) q[u][e] = sline[(sg0 + u) * K + e];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (sg0 + u < P) {
+                    sline[(sg0 + u) * K] = cn / cd;
+                    const S nn = q[u][0] * cn + q[u][1] * cd, dd = q[u][2] * cn + q[u][3] * cd;
+                    const int e = -seg_exponent(fmax(fabs(nn), fabs(dd)));
+                    cn = ldexp(nn, e);
+                    cd = ldexp(dd, e);
+                }
+            }
         }
     }
     __syncthreads();
     const S cst = slot[0];
 
-    // phase B: exact c' over the rows, the interchange test, and affine maps
-    // of d' over the segment
-    S cp[R];
+    // phase B: exact c' over the rows (their 1 / den kept), the interchange
+    // test, and affine maps of d' over the segment
+    S inv[R];
     {
         S U[NS], V[NS];
 #pragma unroll
@@ -104,17 +131,18 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
         S cprev = cst;
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            inv[r] = S(0);
             if (r < nr) {
                 const S den = b[r] - a[r] * cprev;
-                const S inv = S(1) / den;
-                cprev = c[r] * inv;
-                cp[r] = cprev;
+                const S iv = S(1) / den;
+                inv[r] = iv;
+                cprev = c[r] * iv;
                 if (!(den != S(0) && isfinite(den)) && zrow == 0) zrow = s0 + r + 1;
                 if (s0 + r + 1 < n && fabs(den) < fabs(r + 1 < nr ? a[r + 1 < R ? r + 1 : R - 1] : a_next)) piv++;
-                const S v = -a[r] * inv;
+                const S v = -a[r] * iv;
 #pragma unroll
                 for (int m = 0; m < NS; m++) {
-                    U[m] = d[r][m] * inv + v * U[m];
+                    U[m] = d[r][m] * iv + v * U[m];
                     V[m] = v * V[m];
                 }
             }
@@ -126,11 +154,18 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
     __syncthreads();
     if (scanner) {
         S din = S(0);
-        for (int sg = 0; sg < P; sg++) {
-            S* q = lds + (long)(sl * P + sg) * K + 2 * sm;
-            const S u = q[0], v = q[1];
-            q[0] = din;
-            din = u + v * din;
+        for (int sg0 = 0; sg0 < P; sg0 += 8) {
+            S u[8], v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (sg0 + k < P) { u[k] = sline[(sg0 + k) * K + 2 * sm]; v[k] = sline[(sg0 + k) * K + 2 * sm + 1]; }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (sg0 + k < P) {
+                    sline[(sg0 + k) * K + 2 * sm] = din;
+                    din = u[k] + v[k] * din;
+                }
+            }
         }
     }
     __syncthreads();
@@ -143,11 +178,9 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
 #pragma unroll
         for (int r = 0; r < R; r++) {
             if (r < nr) {
-                const S den = b[r] - a[r] * (r == 0 ? cst : cp[r > 0 ? r - 1 : 0]);
-                const S inv = S(1) / den;
 #pragma unroll
                 for (int m = 0; m < NS; m++) {
-                    dprev[m] = (d[r][m] - a[r] * dprev[m]) * inv;
+                    dprev[m] = (d[r][m] - a[r] * dprev[m]) * inv[r];
                     d[r][m] = dprev[m];
                 }
             }
@@ -158,10 +191,11 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
 #pragma unroll
         for (int r = R - 1; r >= 0; r--) {
             if (r < nr) {
+                const S cp = c[r] * inv[r];
 #pragma unroll
                 for (int m = 0; m < NS; m++) {
-                    A[m] = d[r][m] - cp[r] * A[m];
-                    B[m] = -cp[r] * B[m];
+                    A[m] = d[r][m] - cp * A[m];
+                    B[m] = -cp * B[m];
                 }
             }
         }
@@ -171,11 +205,18 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
     __syncthreads();
     if (scanner) {
         S xin = S(0);
-        for (int sg = P - 1; sg >= 0; sg--) {
-            S* q = lds + (long)(sl * P + sg) * K + 2 * sm;
-            const S aa = q[0], bb = q[1];
-            q[0] = xin;
-            xin = aa + bb * xin;
+        for (int sg1 = P - 1; sg1 >= 0; sg1 -= 8) {
+            S u[8], v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (sg1 - k >= 0) { u[k] = sline[(sg1 - k) * K + 2 * sm]; v[k] = sline[(sg1 - k) * K + 2 * sm + 1]; }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (sg1 - k >= 0) {
+                    sline[(sg1 - k) * K + 2 * sm] = xin;
+                    xin = u[k] + v[k] * xin;
+                }
+            }
         }
     }
     __syncthreads();
@@ -187,9 +228,10 @@ __device__ void seg_solve(S (&a)[R], S (&b)[R], S (&c)[R], S (&d)[R][NS], S a_ne
 #pragma unroll
     for (int r = R - 1; r >= 0; r--) {
         if (r < nr) {
+            const S cp = c[r] * inv[r];
 #pragma unroll
             for (int m = 0; m < NS; m++) {
-                x[m] = d[r][m] - cp[r] * x[m];
+                x[m] = d[r][m] - cp * x[m];
                 d[r][m] = x[m];
             }
         }
