@@ -71,6 +71,10 @@ FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_sweep<0", "daf_j": "k_da
                "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_pfix_sub",
                "dtau": "k_prep", "residual": "k_residual", "residual_recalc": "k_residual"}
 
+# warm-start samples per recorded training step and the sample buffer: every
+# one of the 200 training steps stays in the buffer
+DT_SAMPLES = 32768
+DT_CAPACITY = 1 << 23
 HEADLINE_DIMS = (193, 129, 129)
 T_START = time.perf_counter()
 
@@ -203,7 +207,7 @@ def dual_time_leg(args):
         if args.autotune:
             sv.autotune(reps=3)
             sv.set_state(*cc.state())
-        sv.ws_configure(hidden=32, samples_per_record=65536, capacity=1 << 22, seed=20260505)
+        sv.ws_configure(hidden=32, samples_per_record=DT_SAMPLES, capacity=DT_CAPACITY, seed=20260505)
     # warm-up history (3 plain steps of 30 pseudo-iterations), then the
     # training steps, each recorded after it converged to the first rung
     sp.set_convergence(30, 30, -99.0, -99.0)
@@ -296,7 +300,9 @@ def dual_time_leg(args):
                "plain_last_log10_norm_at_cap": None if rung8 is None else
                rung8["forms"]["fp64_plain"]["last_log10_norm_mean"]}}
     out["training"] = {"steps": args.dt_train_steps, "mean_pseudo_iterations": round(float(np.mean(gen_itc)), 2),
-                       "samples": int(min(args.dt_train_steps * 65536, 1 << 22)), "adam_steps": args.dt_train_iters,
+                       "samples": int(min(args.dt_train_steps * DT_SAMPLES, DT_CAPACITY)),
+                       "stencil": "node, 6 face neighbours, 2nd neighbours along i; last 3 steps",
+                       "adam_steps": args.dt_train_iters,
                        "loss_first": round(float(loss[:20].mean()), 5), "loss_last": round(float(loss[-20:].mean()), 5),
                        "wall_s": {"case_build": round(t_build, 2), "generate": round(t_gen, 2),
                                   "train": round(t_train, 3)}}
@@ -441,7 +447,7 @@ def main():
     ap.add_argument("--dt-grid", default="193,129,129", help="dual-time leg grid")
     ap.add_argument("--dt-drops", default="2,3,4,8",
                     help="tolerance ladder of the dual-time leg: decades below the plain start's first norm")
-    ap.add_argument("--dt-train-steps", type=int, default=40,
+    ap.add_argument("--dt-train-steps", type=int, default=200,
                     help="plain physical steps recorded as training samples for the warm start")
     ap.add_argument("--dt-eval-steps", type=int, default=3, help="physical steps compared after training")
     ap.add_argument("--dt-drop", type=float, default=2.0,

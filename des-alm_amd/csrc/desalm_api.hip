@@ -341,6 +341,7 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     C.e_source = p->unsteady ? 1.0 : 0.0;
     C.unsteady = p->unsteady ? 1 : 0;
     C.mixed = p->precision == DESALM_PREC_MIXED ? 1 : 0;
+    C.dtau_met = p->dtau_formula == DESALM_DTAU_MET ? 1 : 0;
     C.cfl1 = p->cfl1; C.vnn1 = p->vnn1; C.cfl2 = p->cfl2; C.vnn2 = p->vnn2;
     C.pdiss = p->pdiss_coef;
     for (int a = 0; a < 2; a++)

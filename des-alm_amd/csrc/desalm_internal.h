@@ -46,7 +46,8 @@ struct Consts {
     double sa[6][5], sb[6][5];
     int btype[6];
     int unsteady;
-    int mixed;  // DESALM_PREC_MIXED: DAF sweeps in FP32 on the *f scratch
+    int mixed;     // DESALM_PREC_MIXED: DAF sweeps in FP32 on the *f scratch
+    int dtau_met;  // DESALM_DTAU_MET: viscous dtau limits without nu_t
 };
 
 // Device pointers.  Fields marked "all" span every level; "fine" span the
@@ -152,8 +153,8 @@ struct Alm {
 // coefficients' departures (a - 1, b) of the start
 // q^n + a (q^n - q^(n-1)) + b (q^(n-1) - q^(n-2)).
 constexpr int kWsComp = 5;
-constexpr int kWsPts = 7;
-constexpr int kWsIn = kWsComp * 2 * kWsPts + kWsComp;  // 75
+constexpr int kWsPts = 9;                              // the node and 8 neighbours
+constexpr int kWsIn = kWsComp * 2 * kWsPts + kWsComp;  // 95
 constexpr int kWsOut = 2 * kWsComp;                     // network outputs
 constexpr int kWsTgt = 3 * kWsComp;                     // per sample: r, d1, d2
 constexpr int kWsStats = 2 * kWsIn + 2 * kWsComp;       // mu, 1/sd, scale, unused

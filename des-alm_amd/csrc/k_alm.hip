@@ -1,7 +1,7 @@
 // k_alm.hip -- actuator-line model: rhs_lines (src/rhs_lines.F90:47-483)
 // and the hub zeroing of bcond_fm (src/bcond_fm.F90:328-355).
 //
-//   k_alm_elements  one thread per (blade, element): velocity at the
+//   k_alm_elements  one wave per (blade, element): velocity at the
 //                   element, angle of attack, polar lookup, lift and drag ->
 //                   the element's position and force (forces_lines)
 //   k_alm_spread    one thread per spreading node: Gaussian sum over all
@@ -21,41 +21,55 @@ namespace {
 constexpr double kPi = 3.141592653589793;
 constexpr double kRho = 1178.62;  // rhs_lines.F90:86
 
-// Cl, Cd from a polar: first consecutive pair with a1 < aoa <= a2, linear
-// in between (rhs_lines.F90:232-265); outside the polar the end values
+// Cl, Cd from a polar (rhs_lines.F90:232-265).  The reference walks the
+// pairs (a1, a2) = (angle(k), angle(k+1)) from the first and stops at the
+// first pair with a1 < aoa <= a2, or at the first a1 >= aoa.  Either way the
+// walk ends at b, the first index with aoa <= angle(b): b = 0 gives the first
+// values, 0 < b < n the linear interpolation on (b-1, b), no such b the last
+// values.  The 64 lanes of the wave find b together, 64 angles per step
+// (every lane calls this with the same polar and aoa).
 __device__ void polar_lookup(const AlmPolars& P, int p, double aoa, double& cl, double& cd) {
     const int o = P.off[p], n = P.cnt[p];
     const double* a = P.angle + o;
     const double* l = P.cl + o;
     const double* d = P.cd + o;
-    if (aoa <= a[0]) {
-        cl = l[0];
-        cd = d[0];
-        return;
-    }
-    for (int k = 0; k + 1 < n; k++) {
-        const double a1 = a[k], a2 = a[k + 1];
-        if (aoa <= a1) break;
-        if (aoa <= a2 && aoa >= a1) {
-            double m = (l[k + 1] - l[k]) / (a2 - a1);
-            cl = m * (aoa - a1) + l[k];
-            m = (d[k + 1] - d[k]) / (a2 - a1);
-            cd = m * (aoa - a1) + d[k];
-            return;
+    const int lane = (int)(threadIdx.x % 64);
+    int b = n;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + lane;
+        const unsigned long long hit = __ballot(k < n && aoa <= a[k]);
+        if (hit) {
+            b = k0 + __ffsll((long long)hit) - 1;
+            break;
         }
     }
-    cl = l[n - 1];
-    cd = d[n - 1];
+    if (b == 0) {
+        cl = l[0];
+        cd = d[0];
+    } else if (b < n) {
+        const double a1 = a[b - 1], a2 = a[b];
+        double m = (l[b] - l[b - 1]) / (a2 - a1);
+        cl = m * (aoa - a1) + l[b - 1];
+        m = (d[b] - d[b - 1]) / (a2 - a1);
+        cd = m * (aoa - a1) + d[b - 1];
+    } else {
+        cl = l[n - 1];
+        cd = d[n - 1];
+    }
 }
 
 __global__ void k_alm_elements(Level L, Fields F, AlmTurbine T, AlmPolars P, double length_scale, double vel_infty,
                                double ren, int jmg, int kmg) {
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    // one wave per (blade, element); every lane computes the element, lane 0
+    // writes it
+    const int e = (int)blockIdx.x;
     if (e >= T.nblades * T.nelems) return;
+    const bool writer = threadIdx.x == 0;
     const int bld = 1 + e / T.nelems, elem = 1 + e % T.nelems;
     double* f = T.forces + 6L * e;
+    if (writer)
 #pragma unroll
-    for (int c = 0; c < 6; c++) f[c] = 0.0;  // forces_lines = 0 (:95)
+        for (int c = 0; c < 6; c++) f[c] = 0.0;  // forces_lines = 0 (:95)
     const double vel_scale = ren * 10E-7 / length_scale;
     const double omega = (vel_infty * T.tsr) / T.tur_rad;
     double angle_aux = T.angle + (bld - 1) * 2 * kPi / T.nblades;
@@ -101,6 +115,7 @@ __global__ void k_alm_elements(Level L, Fields F, AlmTurbine T, AlmPolars P, dou
     const double urel = sqrt(u_theta * u_theta + u_x * u_x);
     const double fL = 0.5 * cl * kRho * (urel * urel) * chord;
     const double fD = 0.5 * cd * kRho * (urel * urel) * chord;
+    if (!writer) return;
     f[0] = xf;
     f[1] = yf;
     f[2] = zf;
@@ -157,8 +172,8 @@ __global__ void k_alm_hub(Level L, Fields F, AlmTurbine T, double length_scale, 
 void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A, double ren) {
     for (const AlmTurbine& T : A.t) {
         const int ne = T.nblades * T.nelems;
-        hipLaunchKernelGGL(k_alm_elements, dim3((ne + 63) / 64), dim3(64), 0, s, L, F, T, A.pol, A.length_scale,
-                           A.vel_infty, ren, A.jmg, A.kmg);
+        hipLaunchKernelGGL(k_alm_elements, dim3(ne), dim3(64), 0, s, L, F, T, A.pol, A.length_scale, A.vel_infty,
+                           ren, A.jmg, A.kmg);
         if (T.n_ker > 0)
             hipLaunchKernelGGL(k_alm_spread, dim3((T.n_ker + 127) / 128), dim3(128), 0, s, L, F, T, A.length_scale,
                                ren);

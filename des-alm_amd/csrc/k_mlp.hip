@@ -8,8 +8,10 @@
 //   q(0) = q^n + a d1 + b d2,  d1 = q^n - q^(n-1),  d2 = q^(n-1) - q^(n-2),
 // with (a - 1, b) the outputs of a multilayer perceptron (zero outputs give
 // linear extrapolation).  Its inputs are scale-free: for each of the five
-// variables, the 14 time differences d1, d2 at the node and its six face
-// neighbours divided by their root mean square, plus q^n at the node.  The
+// variables, the 18 time differences d1, d2 at the node and its eight
+// neighbours (the six face neighbours and the second neighbours along i, the
+// inflow direction; clamped to the block) divided by their root mean square,
+// plus q^n at the node.  The
 // pseudo-iterations then run to the same tolerance as from any other start,
 // so the warm start only changes how many of them are needed.
 //
@@ -28,7 +30,7 @@ namespace {
 
 constexpr int kBM = 64;     // rows (samples) per output tile
 constexpr int kBN = 64;     // columns per output tile
-constexpr int kKMax = 80;   // largest K + 1 a layer may have
+constexpr int kKMax = 96;   // largest K + 1 a layer may have
 constexpr int kWgRows = 512;  // rows per weight-gradient partial
 
 // 64-bit mix of (seed, a, b) -> uniform index source (splitmix64 finaliser)
@@ -49,14 +51,17 @@ __device__ __forceinline__ long interior_node(const Level& L, long r) {
 
 // Features and (optionally) training target of level-local node t.
 // Feature order: for component c, for stencil point s (node, -i, +i, -j,
-// +j, -k, +k): d1, d2, each divided by the root mean square of the
-// component's 14 (0 when they are all zero); then q^n(c) at the node.
+// +j, -k, +k, -2i, +2i; the second neighbours clamped to i = 0 and ni - 1):
+// d1, d2, each divided by the root mean square of the component's 18 (0
+// when they are all zero); then q^n(c) at the node.
 // mu / isd standardise the features when non-null.  Target: per component
 // r = q - 2 q^n + q^(n-1), then d1 and d2 at the node (raw).
 __device__ void ws_row(const Level& L, const Fields& F, double* const* qnm2, long t, float* x, float* y,
                        const float* mu, const float* isd) {
     const long g = L.off + t;
-    const long st[kWsPts] = {0, -1, 1, -(long)L.ni, (long)L.ni, -(long)L.ni * L.nj, (long)L.ni * L.nj};
+    const int i = (int)(t % L.ni);
+    const long st[kWsPts] = {0, -1, 1, -(long)L.ni, (long)L.ni, -(long)L.ni * L.nj, (long)L.ni * L.nj,
+                             i >= 2 ? -2 : -(long)i, i + 2 <= L.ni - 1 ? 2 : (long)(L.ni - 1 - i)};
     int f = 0;
     for (int c = 0; c < kWsComp; c++) {
         const double* a = F.qn[c];

@@ -27,7 +27,8 @@ struct Nb {
 __device__ __forceinline__ double sqr(double a) { return a * a; }
 
 // Per-node inputs of the residual, every node of the level:
-// rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83) when with_dtau, and always
+// rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83), or met_daf_dtau
+// (src/met_daf_dtau.F90:53-93) when C.dtau_met, when with_dtau, and always
 // U/J (rhs_contra_j.F90:14-29) into F.ucn and p/J
 // (rhs_flux_sans_convec.F90:60) into F.pj (level-local).  With dtau
 // also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
@@ -54,10 +55,18 @@ __global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
     const double dti1 = C.cfl1 / sqrt(g1) / L.dc;
     const double dti2 = C.cfl1 / sqrt(g2) / L.de;
     const double dti3 = C.cfl1 / sqrt(g3) / L.dz;
-    const double ret = C.rei + F.xnut[g];
-    const double dtv1 = C.vnn1 / ret / (L.dc * L.dc) / g1;
-    const double dtv2 = C.vnn1 / ret / (L.de * L.de) / g2;
-    const double dtv3 = C.vnn1 / ret / (L.dz * L.dz) / g3;
+    double dtv1, dtv2, dtv3;
+    if (C.dtau_met) {
+        const double vr = C.vnn1 * C.ren;
+        dtv1 = vr / (L.dc * L.dc) / g1;
+        dtv2 = vr / (L.de * L.de) / g2;
+        dtv3 = vr / (L.dz * L.dz) / g3;
+    } else {
+        const double ret = C.rei + F.xnut[g];
+        dtv1 = C.vnn1 / ret / (L.dc * L.dc) / g1;
+        dtv2 = C.vnn1 / ret / (L.de * L.de) / g2;
+        dtv3 = C.vnn1 / ret / (L.dz * L.dz) / g3;
+    }
     double d = fmin(dti1, dti2);
     d = fmin(d, dti3);
     d = fmin(d, dtv1);

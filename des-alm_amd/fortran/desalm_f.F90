@@ -29,7 +29,8 @@ module desalm_f
      real(c_double) :: ep(3, 2)          ! C ep[a][d] is ep(d+1, a+1) here
      integer(c_int32_t) :: btype(6), bdir(6)
      integer(c_int32_t) :: ifix, jfix, kfix
-     integer(c_int32_t) :: reserved(8)
+     integer(c_int32_t) :: dtau_formula  ! 0: rhs_daf_dtau, the reference's live formula
+     integer(c_int32_t) :: reserved(7)
   end type desalm_params_t
 
   ! struct desalm_turbine and struct desalm_polar of include/desalm.h
@@ -228,6 +229,7 @@ contains
     p%btype = btype(:, 1)
     p%bdir = bdir(:, 1)
     p%ifix = ifix; p%jfix = jfix; p%kfix = kfix
+    p%dtau_formula = 0
     p%reserved = 0
     rc = desalm_init(p, 0_c_int, desalm_f_ctx)
     call desalm_f_check(rc, 'desalm_init')

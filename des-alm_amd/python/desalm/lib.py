@@ -328,7 +328,7 @@ class Solver:
         return out
 
     def ws_stats(self, set_to=None):
-        out = np.zeros(2 * 75 + 2 * 5, dtype=np.float32)
+        out = np.zeros(2 * 95 + 2 * 5, dtype=np.float32)
         src = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.float32)
         self._chk(lib().desalm_ws_stats(self.h, _f(out), _f(src)), "desalm_ws_stats")
         return out
@@ -362,7 +362,7 @@ class Solver:
             self.turbine_angle(t, a)
 
     def ws_features(self, m):
-        x = np.zeros((m, 75), dtype=np.float32)
+        x = np.zeros((m, 95), dtype=np.float32)
         t = np.zeros((m, 15), dtype=np.float32)
         self._chk(lib().desalm_ws_features(self.h, m, _f(x), _f(t)), "desalm_ws_features")
         return x, t

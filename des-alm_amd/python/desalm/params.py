@@ -11,6 +11,9 @@ MAX_LEVELS = 6
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+# local pseudo time step formula (DESALM_DTAU_*)
+DTAU_LIVE = 0
+DTAU_MET = 1
 
 BC_INTERFACE = 0
 BC_WALL = 1
@@ -46,7 +49,8 @@ class Params(ctypes.Structure):
         ("btype", ctypes.c_int32 * 6),
         ("bdir", ctypes.c_int32 * 6),
         ("ifix", ctypes.c_int32), ("jfix", ctypes.c_int32), ("kfix", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("dtau_formula", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
     def level_dims(self):

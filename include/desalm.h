@@ -58,6 +58,15 @@ extern "C" {
  * Indices in this struct (iter[], btype[], ep[][]) are 0-based versions of
  * the reference's 1-based arrays.
  */
+/* Local pseudo time step of the DAF smoother (desalm_params.dtau_formula).
+ * LIVE: rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83), the routine solver_daf
+ *       calls: the viscous limits use 1/Re + nu_t.
+ * MET:  met_daf_dtau (src/met_daf_dtau.F90:53-93), the reference's other
+ *       formula, which no reference path calls: the viscous limits use
+ *       vnn1 Re (laminar viscosity only). */
+#define DESALM_DTAU_LIVE 0
+#define DESALM_DTAU_MET 1
+
 typedef struct desalm_params {
     int32_t ng;                          /* multigrid levels (global.F90:17) */
     int32_t img, jmg, kmg;               /* fine-level node counts */
@@ -81,7 +90,8 @@ typedef struct desalm_params {
     int32_t btype[6];                    /* faces: i=1, i=max, j=1, j=max, k=1, k=max */
     int32_t bdir[6];                     /* symmetry-plane normal direction (1,2,3) */
     int32_t ifix, jfix, kfix;            /* 1-based node whose pressure is pinned to 0 */
-    int32_t reserved[8];
+    int32_t dtau_formula;                /* DESALM_DTAU_*: local pseudo time step formula */
+    int32_t reserved[7];
 } desalm_params;
 
 typedef struct desalm_ctx desalm_ctx;
@@ -402,11 +412,13 @@ int desalm_probe(desalm_ctx* ctx, int32_t n, const int32_t* ijk, double* out);
  *   DESALM_WS_LEARNED  q^n + a d1 + b d2, d1 = q^n - q^(n-1),
  *                      d2 = q^(n-1) - q^(n-2), with (a - 1, b) per node and
  *                      per variable the 10 outputs of a multilayer
- *                      perceptron (75 inputs, two tanh layers of `hidden`
+ *                      perceptron (95 inputs, two tanh layers of `hidden`
  *                      units, FP32).  Inputs: for each of p, u, v, w, nu~ the
- *                      14 differences d1, d2 at the node and its six face
- *                      neighbours divided by their root mean square, then
- *                      q^n at the node; standardised by the statistics.
+ *                      18 differences d1, d2 at the node and its eight
+ *                      neighbours (the six face neighbours and the second
+ *                      neighbours along i, clamped to the block) divided by
+ *                      their root mean square, then q^n at the node;
+ *                      standardised by the statistics.
  * nu~ keeps its floor of 1e-7.  The pseudo-iterations then run to the same
  * tolerance as from the reference's start. */
 #define DESALM_WS_NONE 0
@@ -433,9 +445,9 @@ int desalm_ws_record(desalm_ctx* ctx);
  * the first call.  loss_out, if not NULL, receives the loss of every step
  * (iters doubles). */
 int desalm_ws_train(desalm_ctx* ctx, int iters, int batch, double lr, double* loss_out);
-/* Packed parameters (count desalm_ws_param_count: W1 76 x hidden, W2
+/* Packed parameters (count desalm_ws_param_count: W1 96 x hidden, W2
  * (hidden + 1) x hidden, W3 (hidden + 1) x 10, row-major, last row of each
- * the bias) and statistics (160 floats: 75 feature means, 75 feature 1/sd,
+ * the bias) and statistics (200 floats: 95 feature means, 95 feature 1/sd,
  * 5 target scales, 5 unused); get when `get` is not NULL, then set when
  * `set` is not NULL. */
 int desalm_ws_param_count(const desalm_ctx* ctx);
@@ -447,12 +459,12 @@ int desalm_ws_stats(desalm_ctx* ctx, float* get, const float* set);
 int desalm_ws_history(desalm_ctx* ctx, double* get, const double* set);
 /* Blade angle (rad) of rotor t, get and/or set (global_turb's angle(n_t)). */
 int desalm_turbine_angle(desalm_ctx* ctx, int t, double* get, const double* set);
-/* Unstandardised features (m x 75) and targets (m x 15: r, d1, d2 per
+/* Unstandardised features (m x 95) and targets (m x 15: r, d1, d2 per
  * variable) of the current state at interior ordinals 0 .. m-1 (i fastest),
  * as desalm_ws_record builds them. */
 int desalm_ws_features(desalm_ctx* ctx, int64_t m, float* x, float* t);
 /* The network on host rows: y (m x 10) from standardised features x
- * (m x 75); and the loss against targets t (m x 15, already divided by the
+ * (m x 95); and the loss against targets t (m x 15, already divided by the
  * scales) with its gradient with respect to the packed parameters. */
 int desalm_mlp_forward(desalm_ctx* ctx, const float* x, int64_t m, float* y);
 int desalm_mlp_loss_grad(desalm_ctx* ctx, const float* x, const float* t, int64_t m, float* grad, double* loss);

@@ -688,11 +688,14 @@ static void rhs_contra_j(daf_t* d) {
             }
 }
 
-/* rhs_daf_dtau (src/rhs_daf_dtau.F90:2-100), the live local time step */
+/* rhs_daf_dtau (src/rhs_daf_dtau.F90:2-100), the live local time step, or
+ * with dtau_formula = DESALM_DTAU_MET the viscous limits of met_daf_dtau
+ * (src/met_daf_dtau.F90:53-93): vnn * ren / h^2 / g, no nu_t */
 static void rhs_daf_dtau(daf_t* d) {
     DLV;
     int i, j, k;
-    const double cfl = d->o->p.cfl1, vnn = d->o->p.vnn1, rei = 1.0 / d->o->p.ren;
+    const double cfl = d->o->p.cfl1, vnn = d->o->p.vnn1, rei = 1.0 / d->o->p.ren, ren = d->o->p.ren;
+    const int met = d->o->p.dtau_formula == DESALM_DTAU_MET;
     const double dcsq = d->dc * d->dc, desq = d->de * d->de, dzsq = d->dz * d->dz;
     for (k = 1; k <= nk; k++)
         for (j = 1; j <= nj; j++)
@@ -702,8 +705,17 @@ static void rhs_daf_dtau(daf_t* d) {
                 double g3 = ZET(1, i, j, k) * ZET(1, i, j, k) + ZET(2, i, j, k) * ZET(2, i, j, k) + ZET(3, i, j, k) * ZET(3, i, j, k);
                 double e1 = sqrt(g1), e2 = sqrt(g2), e3 = sqrt(g3);
                 double dti1 = cfl / e1 / d->dc, dti2 = cfl / e2 / d->de, dti3 = cfl / e3 / d->dz;
-                double ret = rei + XNUT(i, j, k);
-                double dtv1 = vnn / ret / dcsq / g1, dtv2 = vnn / ret / desq / g2, dtv3 = vnn / ret / dzsq / g3;
+                double dtv1, dtv2, dtv3;
+                if (met) {
+                    dtv1 = vnn * ren / dcsq / g1;
+                    dtv2 = vnn * ren / desq / g2;
+                    dtv3 = vnn * ren / dzsq / g3;
+                } else {
+                    double ret = rei + XNUT(i, j, k);
+                    dtv1 = vnn / ret / dcsq / g1;
+                    dtv2 = vnn / ret / desq / g2;
+                    dtv3 = vnn / ret / dzsq / g3;
+                }
                 double t = dmin2(dti1, dti2);
                 t = dmin2(t, dti3);
                 t = dmin2(t, dtv1);
@@ -1797,6 +1809,24 @@ static int solver_daf(or_state* o, int n, int recalc, int calc_pk) {
 
 /* residual assembly only (solver_daf.F90:195-315): rh of level n from the
  * stored state, nothing else changed (used by the closed-form checks) */
+/* the local pseudo time step of level n (rhs_daf_dtau, or met_daf_dtau's
+ * viscous limits per p.dtau_formula) into out (the level's node count) */
+int or_dtau(or_state* o, int n, double* out) {
+    daf_t dd, *d = &dd;
+    if (n < 1 || n > o->ng) return -1;
+    {
+        LEVEL(n);
+        memset(d, 0, sizeof(*d));
+        d->o = o; d->n = n; d->ni = ni; d->nj = nj; d->nk = nk;
+        d->dc = o->dc[n]; d->de = o->de[n]; d->dz = o->dz[n];
+        d->csi = o->csi + 3 * off; d->eta = o->eta + 3 * off; d->zet = o->zet + 3 * off;
+        d->aj = o->aj + off; d->xnut = o->xnut + off;
+        d->dtau = out;
+        rhs_daf_dtau(d);
+    }
+    return 0;
+}
+
 int or_residual(or_state* o, int n, int calc_pk) {
     daf_t dd, *d = &dd;
     if (n < 1 || n > o->ng) return -1;

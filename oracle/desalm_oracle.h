@@ -46,6 +46,7 @@ int or_mg_driver(or_state* o, int* itc, double* resid);
 int or_solver_daf(or_state* o, int n, int decide_recalc_rh, int decide_calc_pk);
 /* residual assembly only on level n (rh of solver_daf before the DAF) */
 int or_residual(or_state* o, int n, int calc_pk);
+int or_dtau(or_state* o, int n, double* out);
 /* one des_eddy call on the fine level of the stored state (src/des_eddy.F90) */
 int or_des_eddy(or_state* o);
 /* mg_nlevel(n) + mg_zero_pk_boundary (src/mg_nlevel.F90:2-150) */

@@ -6,12 +6,13 @@ file, as the checker of the HIP path; the library never calls it.
 The reference has no learned part: it starts every physical step from the
 previous solution (src/mg_driver.F90:48-51).  What is restated here is the
 library's own definition (include/desalm.h, desalm_ws_*):
-  features  per interior node, for c = p, u, v, w, nu~: the 14 differences
+  features  per interior node, for c = p, u, v, w, nu~: the 18 differences
             d1 = q^n - q^(n-1), d2 = q^(n-1) - q^(n-2) at the stencil (node,
-            -i, +i, -j, +j, -k, +k), divided by their root mean square (0
-            when all are 0); then q^n at the node (75 values)
+            -i, +i, -j, +j, -k, +k, -2i, +2i; the second neighbours clamped
+            to i = 0 and ni - 1), divided by their root mean square (0 when
+            all are 0); then q^n at the node (95 values)
   targets   per c: r = q^(n+1) - 2 q^n + q^(n-1), d1, d2 at the node
-  network   75 -> tanh(nh) -> tanh(nh) -> 10, each layer one (K+1) x N matrix
+  network   95 -> tanh(nh) -> tanh(nh) -> 10, each layer one (K+1) x N matrix
             whose last row is the bias; outputs (a - 1, b) per variable
   loss      mean over rows and variables of ((a - 1) d1 + b d2 - r)^2, all
             three in units of the variable's scale
@@ -23,8 +24,8 @@ network against plain linear extrapolation.
 """
 import numpy as np
 
-N_IN, N_OUT, N_COMP = 75, 10, 5
-OFFS = ((0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1))
+N_IN, N_OUT, N_COMP = 95, 10, 5
+OFFS = ((0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1), (-2, 0, 0), (2, 0, 0))
 
 
 def param_count(nh):
@@ -49,14 +50,17 @@ def interior_index(dims):
 
 
 def features(qn, qnm1, qnm2, dims):
-    """(m, 75) float32 features of every interior node, computed as the
+    """(m, 95) float32 features of every interior node, computed as the
     library does (float64 differences and root mean square, one rounding)"""
     ni, nj, _ = dims
     g = interior_index(dims)
+    gi = g % ni
     cols = []
     for c in range(N_COMP):
         v = []
         for di, dj, dk in OFFS:
+            if abs(di) == 2:  # second neighbours along i, clamped to the block
+                di = np.clip(gi + di, 0, ni - 1) - gi
             p = g + di + ni * (dj + nj * dk)
             v.append(qn[p, c] - qnm1[p, c])
             v.append(qnm1[p, c] - qnm2[p, c])
@@ -64,7 +68,7 @@ def features(qn, qnm1, qnm2, dims):
         ss = np.zeros(len(g))
         for s in range(len(OFFS)):
             ss = ss + (v[:, 2 * s] * v[:, 2 * s] + v[:, 2 * s + 1] * v[:, 2 * s + 1])
-        rms = np.sqrt(ss / 14.0)
+        rms = np.sqrt(ss / (2.0 * len(OFFS)))
         safe = np.where(rms > 0.0, rms, 1.0)
         cols.append(np.where(rms[:, None] > 0.0, v / safe[:, None], 0.0).astype(np.float32))
     cols.append(qn[g].astype(np.float32))

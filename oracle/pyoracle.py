@@ -60,6 +60,8 @@ def lib():
         L.or_des_eddy.argtypes = [vp]
         L.or_residual.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.or_residual.restype = ctypes.c_int
+        L.or_dtau.argtypes = [vp, ctypes.c_int, ctypes.c_void_p]
+        L.or_dtau.restype = ctypes.c_int
         L.or_gtsv_pivots.argtypes = [ctypes.c_int]
         L.or_gtsv_pivots.restype = ctypes.c_long
         L.or_mg_nlevel.argtypes = [vp, ctypes.c_int]
@@ -195,6 +197,14 @@ class Oracle:
     def residual_only(self, n=1, calc_pk=0):
         assert lib().or_residual(self.h, n, calc_pk) == 0
         return self.get("rh")
+
+    def dtau(self, n=1):
+        """The local pseudo time step of level n (rhs_daf_dtau, or
+        met_daf_dtau's formula per params.dtau_formula)."""
+        ni, nj, nk = self.params.level_dims()[n - 1]
+        out = np.zeros(ni * nj * nk)
+        assert lib().or_dtau(self.h, n, _p(out)) == 0
+        return out
 
     def des_eddy(self):
         assert lib().or_des_eddy(self.h) == 0

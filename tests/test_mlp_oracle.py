@@ -68,21 +68,28 @@ def test_features_hand_indexed():
     qn, qm1, qm2 = (rng.normal(size=(n1, 5)) for _ in range(3))
     x = mo.features(qn, qm1, qm2, dims)
     assert x.shape == (4 * 3 * 2, mo.N_IN)
-    # interior ordinal 5 -> (i, j, k) = (1 + 5 % 4, 1 + (5 // 4) % 3, 1 + 5 // 12) = (2, 2, 1)
-    i, j, k = 2, 2, 1
     node = lambda a, b, c: a + 6 * (b + 5 * c)  # noqa: E731
-    pts = [node(i, j, k), node(i - 1, j, k), node(i + 1, j, k), node(i, j - 1, k), node(i, j + 1, k),
-           node(i, j, k - 1), node(i, j, k + 1)]
-    c = 1  # u
-    d = np.array([[qn[p, c] - qm1[p, c], qm1[p, c] - qm2[p, c]] for p in pts]).ravel()
-    rms = np.sqrt(np.mean(d ** 2))
-    assert np.allclose(x[5, 14 * c:14 * c + 14], d / rms, rtol=1e-6, atol=0)
-    # +j neighbour's d2 sits at column 14 c + 2 * 4 + 1
-    assert np.isclose(x[5, 14 * c + 9], (qm1[pts[4], c] - qm2[pts[4], c]) / rms, rtol=1e-6)
-    assert np.array_equal(x[5, 70:], qn[node(i, j, k)].astype(np.float32))
+
+    def stencil(i, j, k):
+        # node, -i, +i, -j, +j, -k, +k, then the second neighbours along i,
+        # clamped to the block (i = 0 .. 5)
+        return [node(i, j, k), node(i - 1, j, k), node(i + 1, j, k), node(i, j - 1, k), node(i, j + 1, k),
+                node(i, j, k - 1), node(i, j, k + 1), node(max(i - 2, 0), j, k), node(min(i + 2, 5), j, k)]
+
+    # interior ordinal 5 -> (i, j, k) = (1 + 5 % 4, 1 + (5 // 4) % 3, 1 + 5 // 12) = (2, 2, 1)
+    # ordinal 4 -> (1, 2, 1): -2i clamps to i = 0; ordinal 7 -> (4, 2, 1): +2i clamps to i = 5
+    for r, (i, j, k) in ((5, (2, 2, 1)), (4, (1, 2, 1)), (7, (4, 2, 1))):
+        pts = stencil(i, j, k)
+        c = 1  # u
+        d = np.array([[qn[p, c] - qm1[p, c], qm1[p, c] - qm2[p, c]] for p in pts]).ravel()
+        rms = np.sqrt(np.mean(d ** 2))
+        assert np.allclose(x[r, 18 * c:18 * c + 18], d / rms, rtol=1e-6, atol=0), r
+        # +j neighbour's d2 sits at column 18 c + 2 * 4 + 1
+        assert np.isclose(x[r, 18 * c + 9], (qm1[pts[4], c] - qm2[pts[4], c]) / rms, rtol=1e-6)
+        assert np.array_equal(x[r, 90:], qn[node(i, j, k)].astype(np.float32))
     # an unchanged history gives zero features, not a division by zero
     z = mo.features(qn, qn, qn, dims)
-    assert np.array_equal(z[:, :70], np.zeros((24, 70), dtype=np.float32))
+    assert np.array_equal(z[:, :90], np.zeros((24, 90), dtype=np.float32))
 
 
 def test_zero_network_is_linear_extrapolation():
@@ -92,7 +99,7 @@ def test_zero_network_is_linear_extrapolation():
     q, qn, qm1, qm2 = (rng.normal(size=(n1, 5)) for _ in range(4))
     nh = 4
     w = np.zeros(mo.param_count(nh))
-    stats = np.concatenate([np.zeros(75), np.ones(75), np.ones(10)])
+    stats = np.concatenate([np.zeros(95), np.ones(95), np.ones(10)])
     lin = mo.warm_start(q, qn, qm1, qm2, dims)
     net = mo.warm_start(q, qn, qm1, qm2, dims, w, nh, stats)
     g = mo.interior_index(dims)
@@ -104,7 +111,7 @@ def test_zero_network_is_linear_extrapolation():
     W3 = np.zeros((nh + 1, 10))
     W3[-1, :5] = 1.0
     W3[-1, 5:] = -1.0
-    w2 = np.concatenate([np.zeros((76 * nh) + (nh + 1) * nh), W3.ravel()])
+    w2 = np.concatenate([np.zeros(((mo.N_IN + 1) * nh) + (nh + 1) * nh), W3.ravel()])
     quad = mo.warm_start(q, qn, qm1, qm2, dims, w2, nh, stats)
     want = 3 * qn[g] - 3 * qm1[g] + qm2[g]
     assert np.allclose(quad[g, :4], want[:, :4], rtol=0, atol=1e-12)

@@ -64,7 +64,7 @@ def test_features_and_targets_match_oracle(gpu):
     to = mo.targets(q, qn, qnm1, q2, dims).astype(np.float32)
     # the root mean square may be contracted into FMAs on the device: a few ulps
     assert np.allclose(x, xo, rtol=1e-6, atol=1e-7)
-    assert np.array_equal(x[:, 70:], xo[:, 70:])
+    assert np.array_equal(x[:, 90:], xo[:, 90:])
     assert np.array_equal(t, to)
     s.close()
 
@@ -91,7 +91,7 @@ def test_learned_start_matches_oracle(gpu):
     q, qn, qnm1, _ = s.get_state()
     rng = np.random.default_rng(21)
     w = rng.normal(0.0, 0.3, mo.param_count(nh)).astype(np.float32)
-    stats = np.concatenate([rng.normal(0.0, 0.3, 75), rng.uniform(0.5, 2.0, 75),
+    stats = np.concatenate([rng.normal(0.0, 0.3, 95), rng.uniform(0.5, 2.0, 95),
                             rng.uniform(0.01, 0.1, 5), np.zeros(5)]).astype(np.float32)
     s.ws_params(w)
     s.ws_stats(stats)
