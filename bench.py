@@ -541,8 +541,9 @@ def main():
         extras = {"mixed": {"value": round(mx["value"], 3), "unit": "Mcell-iters/s",
                             "ms_per_step": round(mx["wall"] / args.steps * 1e3, 3), "state_finite": mx["finite"],
                             "dtype": "f64 residual, f32 DAF sweeps"},
-                  "channel": {"workload": f"channel {args.channel_n}^3 SA, 3-level V-cycle, periodic-copy faces "
-                                          f"(the reference's btype 6), {args.iters} pseudo-iterations per step",
+                  "channel": {"workload": f"channel {args.channel_n}^3 SA at Re_tau 395, 3-level V-cycle, "
+                                          f"streamwise/spanwise periodic (wrap) with the mean pressure gradient "
+                                          f"as a body force, {args.iters} pseudo-iterations per step",
                               "value_fp64": round(ch["value"], 3), "value_mixed": round(chm["value"], 3),
                               "unit": "Mcell-iters/s", "ms_per_step_fp64": round(ch["wall"] / args.steps * 1e3, 3),
                               "state_finite": ch["finite"] and chm["finite"]}}

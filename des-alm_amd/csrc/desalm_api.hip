@@ -171,8 +171,18 @@ int validate(const desalm_params* p, std::string& why) {
         return -1;
     }
     for (int b = 0; b < 6; b++)
-        if (p->btype[b] < 0 || p->btype[b] > 8 || p->btype[b] == 7) {
-            why = "unsupported boundary type (btype 0-6 or 8)";
+        if (p->btype[b] < 0 || p->btype[b] > 8) {
+            why = "unsupported boundary type (btype 0-8)";
+            return -1;
+        }
+    for (int d = 0; d < 3; d++)
+        if ((p->btype[2 * d] == DESALM_BC_PERIODIC_WRAP) != (p->btype[2 * d + 1] == DESALM_BC_PERIODIC_WRAP)) {
+            why = "a periodic-wrap direction (btype 7) needs btype 7 on both of its faces";
+            return -1;
+        }
+    for (int c = 0; c < 3; c++)
+        if (!std::isfinite(p->body_force[c])) {
+            why = "body force must be finite";
             return -1;
         }
     for (int b = 0; b < 4; b++)
@@ -342,6 +352,7 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     C.unsteady = p->unsteady ? 1 : 0;
     C.mixed = p->precision == DESALM_PREC_MIXED ? 1 : 0;
     C.dtau_met = p->dtau_formula == DESALM_DTAU_MET ? 1 : 0;
+    for (int m = 0; m < 3; m++) C.fb[m] = p->body_force[m];
     C.cfl1 = p->cfl1; C.vnn1 = p->vnn1; C.cfl2 = p->cfl2; C.vnn2 = p->vnn2;
     C.pdiss = p->pdiss_coef;
     for (int a = 0; a < 2; a++)

@@ -48,6 +48,7 @@ struct Consts {
     int unsteady;
     int mixed;     // DESALM_PREC_MIXED: DAF sweeps in FP32 on the *f scratch
     int dtau_met;  // DESALM_DTAU_MET: viscous dtau limits without nu_t
+    double fb[3];  // body force per unit mass (x, y, z), -fb / J in rh(2:4)
 };
 
 // Device pointers.  Fields marked "all" span every level; "fine" span the

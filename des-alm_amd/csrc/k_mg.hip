@@ -270,6 +270,10 @@ __global__ void k_bc_fm(Level L, Fields F, Consts C) {
                 for (int m = 0; m < 4; m++) F.q[m][lo] = F.q[m][hi];
             }
             break;
+        case 7:  // periodic wrap: the image of the opposite interior plane
+#pragma unroll
+            for (int m = 0; m < 4; m++) F.q[m][lo] = F.q[m][hi - sD];
+            break;
         case 4: {  // inflow: pressure extrapolated on the grid ratio (bcond_fm.F90:95-102)
             const double r = D < 2 ? F.rat[b0][t] : 0.0;
             F.q[0][lo] = (1.0 + r) * F.q[0][lo + sD] - r * F.q[0][lo + 2 * sD];
@@ -287,6 +291,10 @@ __global__ void k_bc_fm(Level L, Fields F, Consts C) {
 #pragma unroll
                 for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][lo];
             }
+            break;
+        case 7:
+#pragma unroll
+            for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][lo + sD];
             break;
         case 4: {
             const double r = D < 2 ? F.rat[b1][t] : 0.0;

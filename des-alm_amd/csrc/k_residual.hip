@@ -196,15 +196,20 @@ __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, i
         diss1 = F.diss1[g];
     }
     // unsteady and dissipation terms (rhs_unst_visc_diss.F90:29-48);
-    // diss(2:4) is zero (rhs_convec_quick.F90:433)
+    // diss(2:4) is zero (rhs_convec_quick.F90:433); then the body force
+    // (no counterpart in the reference), -f / J
     {
-        const double dtaj = 2.0 * C.delti * F.aj[g];
+        const double aj = F.aj[g];
+        const double dtaj = 2.0 * C.delti * aj;
         r[0] = r[0] + diss1;
 #pragma unroll
         for (int m = 1; m < 4; m++) {
             const double dqdt = C.e_source * (3.0 * F.q[m][g] - 4.0 * F.qn[m][g] + F.qnm1[m][g]) / dtaj;
             r[m] = r[m] + 0.0 + dqdt;
         }
+        if (C.fb[0] != 0.0 || C.fb[1] != 0.0 || C.fb[2] != 0.0)
+#pragma unroll
+            for (int m = 1; m < 4; m++) r[m] = r[m] - C.fb[m - 1] / aj;
     }
 #pragma unroll
     for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];

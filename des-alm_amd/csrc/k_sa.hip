@@ -397,7 +397,7 @@ __global__ void k_sa_bc(Level L, Fields F, Consts C) {
     switch (C.btype[b0]) {
         case 1: q5[lo] = 0.0; break;
         case 2: case 3: q5[lo] = C.sa[b0][0] * q5[lo + sD] + C.sb[b0][0] * q5[lo + 2 * sD]; break;
-        case 6: q5[lo] = q5[base + (long)(nd - 2) * sD]; break;
+        case 6: case 7: q5[lo] = q5[base + (long)(nd - 2) * sD]; break;  // wraps in both
         case 5: {  // characteristic outlet: extrapolated on the grid ratio (k faces: ratio 0)
             const double r = D < 2 ? F.rat[b0][t] : 0.0;
             q5[lo] = (1.0 + r) * q5[lo + sD] - r * q5[lo + 2 * sD];
@@ -408,7 +408,7 @@ __global__ void k_sa_bc(Level L, Fields F, Consts C) {
     switch (C.btype[b1]) {
         case 1: q5[hi] = 0.0; break;
         case 2: case 3: q5[hi] = C.sa[b1][0] * q5[hi - sD] + C.sb[b1][0] * q5[hi - 2 * sD]; break;
-        case 6: q5[hi] = q5[base + sD]; break;
+        case 6: case 7: q5[hi] = q5[base + sD]; break;
         case 8:
             if (D == 0) q5[hi] = q5[hi - sD];
             break;

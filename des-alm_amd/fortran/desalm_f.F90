@@ -30,7 +30,8 @@ module desalm_f
      integer(c_int32_t) :: btype(6), bdir(6)
      integer(c_int32_t) :: ifix, jfix, kfix
      integer(c_int32_t) :: dtau_formula  ! 0: rhs_daf_dtau, the reference's live formula
-     integer(c_int32_t) :: reserved(7)
+     integer(c_int32_t) :: reserved(6)
+     real(c_double) :: body_force(3)     ! 0: the reference has no body force
   end type desalm_params_t
 
   ! struct desalm_turbine and struct desalm_polar of include/desalm.h
@@ -231,6 +232,7 @@ contains
     p%ifix = ifix; p%jfix = jfix; p%kfix = kfix
     p%dtau_formula = 0
     p%reserved = 0
+    p%body_force = 0.0_c_double
     rc = desalm_init(p, 0_c_int, desalm_f_ctx)
     call desalm_f_check(rc, 'desalm_init')
     nall = le_idx_b(ng)

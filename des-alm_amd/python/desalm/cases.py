@@ -16,7 +16,7 @@ import math
 
 import numpy as np
 
-from .params import make_params, BC_WALL, BC_INTERFACE, BC_PERIODIC, BC_SYMMETRY, BC_FREESTREAM, BC_INFLOW
+from .params import make_params, BC_WALL, BC_INTERFACE, BC_PERIODIC, BC_PERIODIC_WRAP, BC_SYMMETRY, BC_FREESTREAM, BC_INFLOW
 
 
 class Overset:
@@ -131,13 +131,29 @@ def loglaw(yplus):
     return np.where(yplus < 11.0, yplus, np.log(np.maximum(yplus, 1e-30)) / 0.41 + 5.2)
 
 
+def periodic_axis(n, length):
+    """Uniform nodes of a periodic direction: nodes 1 .. n-2 (0-based) cover
+    one period [0, length) and the end planes are the images of the opposite
+    interior planes, x(0) = x(n-2) - length, x(n-1) = x(1) + length."""
+    h = length / (n - 2)
+    return (np.arange(n) - 1.0) * h
+
+
 def channel(ni=129, nj=129, nk=129, ng=3, itm=30, ren=395.0, delti=0.005, seed=20260101,
-            noise=0.05, des=True, **kw):
-    """Turbulent channel, SA (workload 2).  Streamwise/spanwise faces use the
-    reference's btype 6; the walls are at j = 1 and j = jmax."""
-    xi = np.linspace(0.0, 2.0 * math.pi, ni)
+            noise=0.05, des=True, periodic="wrap", **kw):
+    """Turbulent channel, SA (workload 2): Re_tau = ren with u_tau = 1 and
+    half-height 1, walls at j = 1 and j = jmax, domain 2 pi x 2 x pi.
+
+    periodic = "wrap": streamwise and spanwise periodic (btype 7, whose end
+    planes are images of the opposite interior planes) and the mean pressure
+    gradient as a body force f_x = u_tau^2 / h = 1.  periodic = "copy": the
+    reference's btype 6 (the end planes copy each other and freeze) and no
+    body force, on the same grid spacing."""
+    if periodic not in ("wrap", "copy"):
+        raise ValueError("periodic must be 'wrap' or 'copy'")
+    xi = periodic_axis(ni, 2.0 * math.pi)
     yj = channel_y(nj, 2.0 / ren if nj < 65 else 1.0 / ren)
-    zk = np.linspace(0.0, math.pi, nk)
+    zk = periodic_axis(nk, math.pi)
     x, y, z = mesh(xi, yj, zk)
     n1 = ni * nj * nk
     wd = 1.0 - np.abs(y)
@@ -155,10 +171,43 @@ def channel(ni=129, nj=129, nk=129, ng=3, itm=30, ren=395.0, delti=0.005, seed=2
     nut = np.maximum(0.41 * wd * np.minimum(yplus, 200.0) / ren * (1.0 - np.exp(-yplus / 26.0)), 1.0e-7)
     q[:, 4] = nut
     xnut = nut * fv1(nut, ren)
-    bt = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+    per = BC_PERIODIC_WRAP if periodic == "wrap" else BC_PERIODIC
+    bt = [per, per, BC_WALL, BC_WALL, per, per]
+    fb = (1.0, 0.0, 0.0) if periodic == "wrap" else (0.0, 0.0, 0.0)
+    if periodic == "wrap":
+        # the end planes start as the images of the opposite interior planes
+        for a in (q,):
+            a3 = a.reshape(nk, nj, ni, 5)
+            a3[:, :, 0] = a3[:, :, ni - 2]
+            a3[:, :, ni - 1] = a3[:, :, 1]
+            a3[0] = a3[nk - 2]
+            a3[nk - 1] = a3[1]
+        xnut = q[:, 4] * fv1(q[:, 4], ren)
     p = make_params(ni, nj, nk, ng=ng, itm=itm, ren=ren, beta=1.0, delti=delti, unsteady=True,
-                    des=des, btype=bt, fix=(2, nj // 2, 2), **kw)
+                    des=des, btype=bt, fix=(2, nj // 2, 2), body_force=fb, **kw)
     return Case("channel", p, x, y, z, wd, q, q.copy(), q.copy(), xnut)
+
+
+def laminar_channel(ni=9, nj=17, nk=9, ng=2, itm=20, ren=10.0, force=1.0, **kw):
+    """Steady laminar plane Poiseuille flow: walls at y = -1 and 1 (uniform
+    spacing), periodic-wrap i and k, body force (force, 0, 0).  Its exact
+    solution u = force ren (1 - y^2) / 2, v = w = p = 0 is also the exact
+    discrete steady state (central differences are exact for a quadratic on
+    a uniform grid), and the returned case starts from it."""
+    xi = periodic_axis(ni, 1.0)
+    yj = np.linspace(-1.0, 1.0, nj)
+    zk = periodic_axis(nk, 0.5)
+    x, y, z = mesh(xi, yj, zk)
+    n1 = ni * nj * nk
+    q = np.zeros((n1, 5))
+    q[:, 1] = 0.5 * force * ren * (1.0 - y * y)
+    q[np.abs(y) == 1.0, 1] = 0.0
+    wd = 1.0 - np.abs(y)
+    xnut = np.zeros(n1)
+    bt = [BC_PERIODIC_WRAP, BC_PERIODIC_WRAP, BC_WALL, BC_WALL, BC_PERIODIC_WRAP, BC_PERIODIC_WRAP]
+    p = make_params(ni, nj, nk, ng=ng, itm=itm, ren=ren, beta=1.0, unsteady=False, des=False, btype=bt,
+                    fix=(2, nj // 2, 2), body_force=(force, 0.0, 0.0), **kw)
+    return Case("laminar_channel", p, x, y, z, wd, q, q.copy(), q.copy(), xnut)
 
 
 def wavy(ni, nj, nk, amp=0.04, ng=2, seed=7, des=False, unsteady=True, btype=None, ren=500.0,

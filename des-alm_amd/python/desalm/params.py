@@ -22,6 +22,7 @@ BC_FREESTREAM = 3
 BC_INFLOW = 4
 BC_OUTLET_MOC = 5
 BC_PERIODIC = 6
+BC_PERIODIC_WRAP = 7
 BC_FREE_SURFACE = 8
 
 
@@ -50,7 +51,8 @@ class Params(ctypes.Structure):
         ("bdir", ctypes.c_int32 * 6),
         ("ifix", ctypes.c_int32), ("jfix", ctypes.c_int32), ("kfix", ctypes.c_int32),
         ("dtau_formula", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("reserved", ctypes.c_int32 * 6),
+        ("body_force", ctypes.c_double * 3),
     ]
 
     def level_dims(self):
@@ -77,7 +79,8 @@ def make_params(img, jmg, kmg, ng=2, crs=(1, 1, 1), iters=(1, 1, 1, 1, 1, 1), it
                 unsteady=False, des=False, itk=2, precision=PREC_FP64,
                 cfl1=1.0, vnn1=0.1, cfl2=0.5, vnn2=0.05, pdiss=0.1,
                 ep=((1.0, 2.5, 2.5), (0.1, 0.2, 0.2)),
-                btype=(1, 1, 1, 1, 1, 1), bdir=(1, 1, 2, 2, 3, 3), fix=(2, 2, 2)):
+                btype=(1, 1, 1, 1, 1, 1), bdir=(1, 1, 2, 2, 3, 3), fix=(2, 2, 2), body_force=(0.0, 0.0, 0.0),
+                dtau_formula=0):
     """Build a Params.  Defaults follow the shipped deck (ind3dmg.dat:8-16):
     cfl1=1.0, vnn1=0.1, pdiss=0.1, ep=(1.0,2.5,2.5; 0.1,0.2,0.2), itk=2,
     cfl2=0.5, vnn2=0.05.  it_min defaults to itm (no early exit)."""
@@ -104,4 +107,7 @@ def make_params(img, jmg, kmg, ng=2, crs=(1, 1, 1), iters=(1, 1, 1, 1, 1, 1), it
         p.btype[b] = btype[b]
         p.bdir[b] = bdir[b]
     p.ifix, p.jfix, p.kfix = fix
+    p.dtau_formula = dtau_formula
+    for c in range(3):
+        p.body_force[c] = body_force[c]
     return p

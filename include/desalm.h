@@ -47,7 +47,11 @@ extern "C" {
 #define DESALM_BC_FREESTREAM 3
 #define DESALM_BC_INFLOW 4
 #define DESALM_BC_OUTLET_MOC 5   /* characteristic outlet (mg_brhs + nonreflect_ibc on i faces) */
-#define DESALM_BC_PERIODIC 6
+#define DESALM_BC_PERIODIC 6       /* the reference's plane copy (the planes freeze) */
+#define DESALM_BC_PERIODIC_WRAP 7  /* no counterpart in the reference: both planes of the
+                                     * direction are images of the opposite interior planes
+                                     * (q(1) = q(n-1), q(n) = q(2)) on the fine level, for q
+                                     * and nu~; the faces of a direction are 7 together */
 #define DESALM_BC_FREE_SURFACE 8
 
 /*
@@ -91,7 +95,11 @@ typedef struct desalm_params {
     int32_t bdir[6];                     /* symmetry-plane normal direction (1,2,3) */
     int32_t ifix, jfix, kfix;            /* 1-based node whose pressure is pinned to 0 */
     int32_t dtau_formula;                /* DESALM_DTAU_*: local pseudo time step formula */
-    int32_t reserved[7];
+    int32_t reserved[6];
+    double body_force[3];                /* constant body force per unit mass in x, y, z, added to
+                                          * the momentum equations of every level (-f / J in rh);
+                                          * no counterpart in the reference (a channel's mean
+                                          * pressure gradient); zero in the reference's cases */
 } desalm_params;
 
 typedef struct desalm_ctx desalm_ctx;

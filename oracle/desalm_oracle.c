@@ -503,6 +503,12 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                     QB(1, 1, j, k) = (1.0 + r) * QB(1, 2, j, k) - r * QB(1, 3, j, k);
                 }
             break;
+        case 7: /* periodic wrap (no counterpart in the reference): the image of the
+                 * opposite interior plane */
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++)
+                    for (m = 1; m <= 4; m++) QB(m, 1, j, k) = QB(m, ni - 1, j, k);
+            break;
         default: break; /* 0 interface, 6 periodic (commented out, :105-110) */
     }
     /* i = imax */
@@ -525,6 +531,12 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
             for (k = 1; k <= nk; k++)
                 for (j = 1; j <= nj; j++)
                     for (m = 1; m <= 4; m++) QB(m, ni, j, k) = QB(m, ni - 1, j, k);
+            break;
+        case 7: /* periodic wrap (no counterpart in the reference): the image of the
+                 * opposite interior plane */
+            for (k = 1; k <= nk; k++)
+                for (j = 1; j <= nj; j++)
+                    for (m = 1; m <= 4; m++) QB(m, ni, j, k) = QB(m, 2, j, k);
             break;
         default: break;
     }
@@ -549,6 +561,12 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                 for (i = 1; i <= ni; i++)
                     for (m = 1; m <= 4; m++) QB(m, i, 1, k) = QB(m, i, nj, k);
             break;
+        case 7: /* periodic wrap (no counterpart in the reference): the image of the
+                 * opposite interior plane */
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, 1, k) = QB(m, i, nj - 1, k);
+            break;
         default: break;
     }
     /* j = jmax */
@@ -571,6 +589,12 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
             for (k = 1; k <= nk; k++)
                 for (i = 1; i <= ni; i++)
                     for (m = 1; m <= 4; m++) QB(m, i, nj, k) = QB(m, i, 1, k);
+            break;
+        case 7: /* periodic wrap (no counterpart in the reference): the image of the
+                 * opposite interior plane */
+            for (k = 1; k <= nk; k++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, nj, k) = QB(m, i, 2, k);
             break;
         default: break;
     }
@@ -595,6 +619,12 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
                 for (i = 1; i <= ni; i++)
                     for (m = 1; m <= 4; m++) QB(m, i, j, 1) = QB(m, i, j, nk);
             break;
+        case 7: /* periodic wrap (no counterpart in the reference): the image of the
+                 * opposite interior plane */
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, j, 1) = QB(m, i, j, nk - 1);
+            break;
         default: break;
     }
     /* k = kmax */
@@ -617,6 +647,12 @@ static void bcond_fm(const or_state* o, double* qv /* level 1, stride 5 */) {
             for (j = 1; j <= nj; j++)
                 for (i = 1; i <= ni; i++)
                     for (m = 1; m <= 4; m++) QB(m, i, j, nk) = QB(m, i, j, 1);
+            break;
+        case 7: /* periodic wrap (no counterpart in the reference): the image of the
+                 * opposite interior plane */
+            for (j = 1; j <= nj; j++)
+                for (i = 1; i <= ni; i++)
+                    for (m = 1; m <= 4; m++) QB(m, i, j, nk) = QB(m, i, j, 2);
             break;
         default: break;
     }
@@ -1026,6 +1062,7 @@ static void rhs_unst_visc_diss(daf_t* d) {
     DLV;
     int i, j, k, m;
     const double es = d->o->e_source, delti = d->o->p.delti;
+    const double* fb = d->o->p.body_force;
     (void)nk;
     for (k = d->ks; k <= d->ke; k++)
         for (j = d->js; j <= d->je; j++)
@@ -1037,6 +1074,9 @@ static void rhs_unst_visc_diss(daf_t* d) {
                 RH(1, i, j, k) = RH(1, i, j, k) + DISS(1, i, j, k);
                 for (m = 2; m <= 4; m++)
                     RH(m, i, j, k) = RH(m, i, j, k) + DISS(m, i, j, k) + dqdt[m - 2] - VISC(m - 1, i, j, k);
+                /* the body force (no counterpart in the reference): -f / J */
+                if (fb[0] != 0.0 || fb[1] != 0.0 || fb[2] != 0.0)
+                    for (m = 2; m <= 4; m++) RH(m, i, j, k) = RH(m, i, j, k) - fb[m - 2] / AJ(i, j, k);
             }
 }
 
@@ -2213,7 +2253,7 @@ static void rhs_sa_bcond(sa_t* d) {
         case 2: case 3:
             for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = o->sa[0][0] * Q5(2, j, k) + o->sb[0][0] * Q5(3, j, k);
             break;
-        case 6: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = Q5(d->ie, j, k); break;
+        case 6: case 7: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = Q5(d->ie, j, k); break;
         case 5: /* characteristic outlet: extrapolated on the grid ratio */
             for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(1, j, k) = (1.0 + SRAT(0, j, k, nj)) * Q5(2, j, k) - SRAT(0, j, k, nj) * Q5(3, j, k);
             break;
@@ -2224,7 +2264,7 @@ static void rhs_sa_bcond(sa_t* d) {
         case 2: case 3:
             for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = o->sa[1][0] * Q5(ni - 1, j, k) + o->sb[1][0] * Q5(ni - 2, j, k);
             break;
-        case 6: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(d->is, j, k); break;
+        case 6: case 7: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(d->is, j, k); break;
         case 8: for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = Q5(ni - 1, j, k); break;
         case 5:
             for (k = 1; k <= nk; k++) for (j = 1; j <= nj; j++) Q5(ni, j, k) = (1.0 + SRAT(1, j, k, nj)) * Q5(ni - 1, j, k) - SRAT(1, j, k, nj) * Q5(ni - 2, j, k);
@@ -2236,7 +2276,7 @@ static void rhs_sa_bcond(sa_t* d) {
         case 2: case 3:
             for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = o->sa[2][0] * Q5(i, 2, k) + o->sb[2][0] * Q5(i, 3, k);
             break;
-        case 6: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = Q5(i, d->je, k); break;
+        case 6: case 7: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = Q5(i, d->je, k); break;
         case 5:
             for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, 1, k) = (1.0 + SRAT(2, i, k, ni)) * Q5(i, 2, k) - SRAT(2, i, k, ni) * Q5(i, 3, k);
             break;
@@ -2247,7 +2287,7 @@ static void rhs_sa_bcond(sa_t* d) {
         case 2: case 3:
             for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = o->sa[3][0] * Q5(i, nj - 1, k) + o->sb[3][0] * Q5(i, nj - 2, k);
             break;
-        case 6: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = Q5(i, d->js, k); break;
+        case 6: case 7: for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = Q5(i, d->js, k); break;
         case 5:
             for (k = 1; k <= nk; k++) for (i = 1; i <= ni; i++) Q5(i, nj, k) = (1.0 + SRAT(3, i, k, ni)) * Q5(i, nj - 1, k) - SRAT(3, i, k, ni) * Q5(i, nj - 2, k);
             break;
@@ -2258,7 +2298,7 @@ static void rhs_sa_bcond(sa_t* d) {
         case 2: case 3:
             for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = o->sa[4][0] * Q5(i, j, 2) + o->sb[4][0] * Q5(i, j, 3);
             break;
-        case 6: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = Q5(i, j, d->ke); break;
+        case 6: case 7: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = Q5(i, j, d->ke); break;
         case 5:
             for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, 1) = (1.0 + SRAT(4, i, j, ni)) * Q5(i, j, 2) - SRAT(4, i, j, ni) * Q5(i, j, 3);
             break;
@@ -2269,7 +2309,7 @@ static void rhs_sa_bcond(sa_t* d) {
         case 2: case 3:
             for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = o->sa[5][0] * Q5(i, j, nk - 1) + o->sb[5][0] * Q5(i, j, nk - 2);
             break;
-        case 6: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = Q5(i, j, d->ks); break;
+        case 6: case 7: for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = Q5(i, j, d->ks); break;
         case 5:
             for (j = 1; j <= nj; j++) for (i = 1; i <= ni; i++) Q5(i, j, nk) = (1.0 + SRAT(5, i, j, ni)) * Q5(i, j, nk - 1) - SRAT(5, i, j, ni) * Q5(i, j, nk - 2);
             break;
