@@ -88,6 +88,40 @@ def workload_name(dims):
     return "actuator-line box %dx%dx%d" % tuple(dims)
 
 
+def kernel_short(name):
+    return name.replace("desalm::(anonymous namespace)::", "").replace("void ", "").split("(")[0].strip()
+
+
+def classify_dispatches(seq):
+    """Fine-level kernel family (or None) of every dispatch of a profiled
+    run, given (short kernel name, grid size) in launch order.  Dispatches
+    before the first k_copy_qold (the tile autotuner's trial launches) are
+    skipped.  A dispatch is fine-level when its grid is the largest of its
+    kernel and first template argument (so the i, j and k sweeps and their
+    instantiations for short lines are told apart); k_prep, k_residual and
+    k_visc_pk belong to the recalc branch when they follow a fine k sweep."""
+    first = next((i for i, (nm, _) in enumerate(seq) if nm == "k_copy_qold"), 0)
+    key = lambda nm: nm.split(",")[0]  # noqa: E731
+    fine_grid = {}
+    for nm, g in seq[first:]:
+        fine_grid[key(nm)] = max(fine_grid.get(key(nm), 0), g)
+    out = [None] * len(seq)
+    after_daf = False
+    for i in range(first, len(seq)):
+        nm, g = seq[i]
+        fine = g == fine_grid[key(nm)]
+        f = next((fm for pre, fm in FAMILY_OF if nm.startswith(pre)), None)
+        if nm in ("k_prep", "k_residual", "k_visc_pk"):
+            f = ("residual_recalc" if after_daf else ("dtau" if nm == "k_prep" else "residual")) if fine else None
+        elif f not in ("bcond_fm",) and not nm.startswith("__amd") and nm != "k_alm_hub":
+            # (the hub zeroing closes bcond_fm on the actuator-line box)
+            after_daf = False
+        if nm.startswith("k_daf_sweep<2") and fine:
+            after_daf = True
+        out[i] = f if (f is not None and (fine or nm == "k_resid_final")) else None
+    return out
+
+
 def pmc_summary(fetch_dir, write_dir, dims):
     """HBM bytes per fine-level launch of every kernel family from two
     rocprofv3 counter runs of the same command (FETCH_SIZE, WRITE_SIZE, KiB
@@ -122,23 +156,12 @@ def pmc_summary(fetch_dir, write_dir, dims):
     if not cf:
         raise SystemExit("no k_copy_qold dispatch to calibrate on")
     sf, sw = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
-    fine_grid = {}
-    for _, nm, g, _ in fe:
-        fine_grid[nm] = max(fine_grid.get(nm, 0), g)
     fam = {}
     kern = {}
-    after_daf = False
-    for (_, nm, g, vf), (_, _, _, vw) in zip(fe, wr):
+    fams_of = classify_dispatches([(nm, g) for _, nm, g, _ in fe])
+    for (_, nm, g, vf), (_, _, _, vw), f in zip(fe, wr, fams_of):
         rd, wt = vf * sf * 1024.0, vw * sw * 1024.0
-        fine = g == fine_grid[nm]
-        f = next((fm for pre, fm in FAMILY_OF if nm.startswith(pre)), None)
-        if nm in ("k_prep", "k_residual", "k_visc_pk"):
-            f = ("residual_recalc" if after_daf else ("dtau" if nm == "k_prep" else "residual")) if fine else None
-        elif f not in ("bcond_fm",) and not nm.startswith("__amd"):
-            after_daf = False
-        if nm.startswith("k_daf_sweep<2") and fine:
-            after_daf = True
-        if f is None or not (fine or nm == "k_resid_final"):
+        if f is None:
             continue
         e = fam.setdefault(f, {"calls": 0, "read": 0.0, "write": 0.0})
         if nm.startswith(FAMILY_MARK[f]):
@@ -177,6 +200,49 @@ def pmc_summary(fetch_dir, write_dir, dims):
     print("|---|---:|---:|---:|")
     for k, v in sorted(out["kernels"].items(), key=lambda kv: -(kv[1]["read_bytes"] + kv[1]["write_bytes"])):
         print(f"| `{k}` | {v['dispatches']} | {v['read_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} |")
+
+
+def trace_summary(trace_dir):
+    """Per-kernel and per-family device times from a rocprofv3
+    --kernel-trace run (its results database) of the headline workload:
+    dispatches, mean and total duration per kernel and grid size, and the
+    mean time per call of every fine-level kernel family (the figure the
+    bench's roofline divides by).  Prints markdown.  Runs on the CPU."""
+    import glob
+    import sqlite3
+
+    def short(name):
+        return name.replace("desalm::(anonymous namespace)::", "").replace("void ", "").split("(")[0].strip()
+
+    dbs = glob.glob(os.path.join(trace_dir, "**", "*results.db"), recursive=True)
+    if not dbs:
+        raise SystemExit(f"no rocprofv3 results database under {trace_dir}")
+    con = sqlite3.connect(dbs[0])
+    rows = con.execute("select name, grid_x * grid_y * grid_z, start, duration from kernels order by start").fetchall()
+    kern = {}
+    for nm, g, _, d in rows:
+        e = kern.setdefault((short(nm), g), [0, 0.0])
+        e[0] += 1
+        e[1] += d
+    total = sum(e[1] for e in kern.values())
+    fam = {}
+    seq = [(short(nm), g) for nm, g, _, _ in rows]
+    for (nm, _), (_, _, _, d), f in zip(seq, rows, classify_dispatches(seq)):
+        if f is None:
+            continue
+        e = fam.setdefault(f, [0, 0.0])
+        if nm.startswith(FAMILY_MARK[f]):
+            e[0] += 1
+        e[1] += d
+    print(f"{len(rows)} dispatches, {total / 1e6:.3f} ms of kernel time\n")
+    print("| fine-level family | calls | mean ms per call | total ms |")
+    print("|---|---:|---:|---:|")
+    for f, (c, d) in sorted(fam.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {f} | {c} | {d / max(c, 1) / 1e6:.4f} | {d / 1e6:.3f} |")
+    print("\n| kernel | grid (work-items) | dispatches | mean us | total ms | share |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for (nm, g), (c, d) in sorted(kern.items(), key=lambda kv: -kv[1][1])[:40]:
+        print(f"| `{nm}` | {g} | {c} | {d / c / 1e3:.2f} | {d / 1e6:.3f} | {100.0 * d / total:.1f}% |")
 
 
 def dual_time_leg(args):
@@ -456,7 +522,12 @@ def main():
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="no run: summarise two rocprofv3 counter runs of the headline workload into "
                          "profiles/traffic.json")
+    ap.add_argument("--trace-summary", metavar="TRACE_DIR",
+                    help="no run: summarise a rocprofv3 --kernel-trace run of the headline workload")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], HEADLINE_DIMS)
         return
@@ -476,13 +547,17 @@ def main():
         fixed_tiles = ([int(t) for t in v.split(",") if t], int(sa or 0))
         args.autotune = False
 
-    def timed_run(make_case, precision, profile, physical):
+    def timed_run(make_case, precision, profile, physical, tiles_from=None):
         """warmup + timed physical steps of --iters pseudo-iterations each;
         returns a dict (Mcell-iters/s, wall s, iterations, profile stats, last
-        norm history, state finite, pivot events, params, tiles)"""
+        norm history, state finite, pivot events, params, tiles).  tiles_from:
+        the launch shapes of an earlier run of the same workload, reused."""
         cs = make_case(precision)
         sv = Solver(cs.params, device=0).load_case(cs)
-        if args.autotune:
+        if tiles_from is not None:
+            sv.set_tiles(*tiles_from)
+            tiles = sv.get_tiles()
+        elif args.autotune:
             # launch shapes of the LDS-tiled kernels, timed on this state
             # (scratch only), then the state is reloaded
             tiles = sv.autotune(reps=3)
@@ -529,7 +604,7 @@ def main():
     n1 = p.img * p.jmg * p.kmg
     # the same workload with the library's per-launch HIP-event profiler
     # (launches then run eagerly): per-kernel-family device times
-    prof = timed_run(alm, PREC_FP64, True, True)
+    prof = timed_run(alm, PREC_FP64, True, True, tiles_from=head["tiles"])
     log("profiled pass done")
     stats = prof["stats"]
     extras = None
