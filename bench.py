@@ -245,6 +245,45 @@ def trace_summary(trace_dir):
         print(f"| `{nm}` | {g} | {c} | {d / c / 1e3:.2f} | {d / 1e6:.3f} | {100.0 * d / total:.1f}% |")
 
 
+# the overset workload: six curvilinear blocks of uneven sizes over a
+# sinusoidal hill wall (amplitude 0.2), SA at Re 10595
+OVERSET_BLOCKS = [(97, 65, 49), (113, 81, 57), (129, 65, 65), (145, 97, 73), (161, 97, 81), (81, 49, 41)]
+
+
+def overset_run(args):
+    """The six-block overset group, FP64, graph path: Mcell-iters/s with one
+    launch per kernel over all blocks and zone by zone, from the same state."""
+    import numpy as np
+    from desalm import overset
+    from desalm.lib import Group, Solver
+
+    cs, tab = overset.hill_blocks(OVERSET_BLOCKS, [0.15, 0.32, 0.5, 0.68, 0.86], itm=args.iters, amp=0.2,
+                                  ren=10595.0, ng=3)
+    n = sum(a * b * c for a, b, c in OVERSET_BLOCKS)
+    out = {"workload": "overset: 6 blocks " + ", ".join(f"{a}x{b}x{c}" for a, b, c in OVERSET_BLOCKS) +
+                       f" ({n} nodes) over a sinusoidal hill wall, SA Re 10595, 3-level V-cycle, "
+                       f"{args.iters} pseudo-iterations per call", "unit": "Mcell-iters/s"}
+    for key, batched in (("value_one_launch", True), ("value_zone_by_zone", False)):
+        svs = [Solver(c.params, device=0).load_case(c) for c in cs]
+        g = Group(svs)
+        g.set_batched(batched)
+        g.set_overset(tab)
+        g.mg_driver()
+        done = 0
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            itc, _ = g.mg_driver()
+            done += itc
+        tb = time.perf_counter()
+        out[key] = round(n * done / (tb - ta) / 1e6, 3)
+        out["state_finite"] = out.get("state_finite", True) and all(
+            bool(np.all(np.isfinite(sv.get_state()[0]))) for sv in svs)
+        g.close()
+        for sv in svs:
+            sv.close()
+    return out
+
+
 def dual_time_leg(args):
     """Physical time steps of SA-DES dual time stepping on the actuator-line
     box: the plain FP64 method (every step starts from q^n, as the
@@ -621,7 +660,10 @@ def main():
                                           f"as a body force, {args.iters} pseudo-iterations per step",
                               "value_fp64": round(ch["value"], 3), "value_mixed": round(chm["value"], 3),
                               "unit": "Mcell-iters/s", "ms_per_step_fp64": round(ch["wall"] / args.steps * 1e3, 3),
-                              "state_finite": ch["finite"] and chm["finite"]}}
+                              "state_finite": ch["finite"] and chm["finite"]},
+                  "overset": overset_run(args)}
+        log(f"overset: {extras['overset']['value_one_launch']:.1f} Mcell-iters/s in one launch per kernel, "
+            f"{extras['overset']['value_zone_by_zone']:.1f} zone by zone")
 
     # dominant kernel family by total device time
     fam = {}

@@ -631,9 +631,35 @@ static bool needs_coordinates(const desalm_ctx* c) {
     return false;
 }
 
+// solver_daf of level n for every zone of a group, one launch per kernel
+// (the zones' own outlet rows, actuator-line forcing and hub zeroing per
+// zone); the launch order within each zone is solver_daf's
+static void solver_daf_zones(std::vector<desalm_ctx*>& zs, const ZoneList& Z, int n, int recalc, int calc_pk) {
+    desalm_ctx* c0 = zs[0];
+    hipStream_t s = c0->stream;
+    const int kc = c0->kc_visc[n];
+    timed(c0, KID_DTAU, n, [&] { launch_dtau_z(s, Z, n); });
+    timed(c0, KID_RESIDUAL, n, [&] { launch_residual_z(s, Z, n, 0, calc_pk, kc); });
+    for (desalm_ctx* c : zs) {
+        if (n == 1 && (c->C.btype[0] == DESALM_BC_OUTLET_MOC || c->C.btype[1] == DESALM_BC_OUTLET_MOC))
+            timed(c0, KID_RESIDUAL, n, [&] { launch_outlet_rhs(s, c->lv[n], c->F, c->C); });
+        if (!c->alm.t.empty()) timed(c0, KID_ALM, n, [&] { launch_alm_rhs(s, c->lv[n], c->F, c->alm, c->C.ren); });
+    }
+    timed(c0, KID_DAF_I, n, [&] { launch_daf_z(s, Z, n, 0); });
+    timed(c0, KID_DAF_J, n, [&] { launch_daf_z(s, Z, n, 1); });
+    timed(c0, KID_DAF_K, n, [&] { launch_daf_z(s, Z, n, 2); });
+    if (n == 1)
+        timed(c0, KID_BCOND, n, [&] {
+            launch_bcond_fm_z(s, Z);
+            for (desalm_ctx* c : zs)
+                if (!c->alm.t.empty()) launch_alm_hub(s, c->lv[1], c->F, c->alm);
+        });
+    if (recalc == 1) timed(c0, KID_RESIDUAL_RECALC, n, [&] { launch_residual_z(s, Z, n, 1, 0, kc); });
+}
+
 template <typename Exchange>
 static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom, Exchange exchange,
-                   double* init_resd, int* itc_out, double* resid_out) {
+                   double* init_resd, int* itc_out, double* resid_out, const ZoneList* zl = nullptr) {
     desalm_ctx* c0 = zs[0];
     const desalm_params& p = c0->p;
     hipStream_t s = c0->stream;
@@ -679,34 +705,55 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
     auto body = [&](int recalc_in) {
         int decide_recalc_rh = recalc_in;
         for (int n = 1; n <= ng; n++) {
-            if (n > 1)
-                for (desalm_ctx* c : zs)
-                    timed(c, KID_NLEVEL, n, [&] {
-                        launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, c->p.icrs, c->p.jcrs, c->p.kcrs);
-                    });
+            if (n > 1) {
+                if (zl)
+                    timed(c0, KID_NLEVEL, n, [&] { launch_nlevel_z(s, *zl, n); });
+                else
+                    for (desalm_ctx* c : zs)
+                        timed(c, KID_NLEVEL, n, [&] {
+                            launch_nlevel(s, c->lv[n], c->lv[n - 1], c->F, c->p.icrs, c->p.jcrs, c->p.kcrs);
+                        });
+            }
             for (int itr = 1; itr <= p.iter[n - 1]; itr++) {
                 if (n != ng) decide_recalc_rh = (itr != p.iter[n - 1]) ? 0 : 1;
                 if (p.des && n == 1) {
-                    for (desalm_ctx* c : zs)
-                        timed(c, KID_DES_EDDY, 1, [&] { launch_des_eddy(s, c->lv[1], c->F, c->C, c->p.itk, c->kc_sa); });
+                    if (zl)
+                        timed(c0, KID_DES_EDDY, 1, [&] { launch_des_eddy_z(s, *zl, c0->p.itk, c0->kc_sa); });
+                    else
+                        for (desalm_ctx* c : zs)
+                            timed(c, KID_DES_EDDY, 1,
+                                  [&] { launch_des_eddy(s, c->lv[1], c->F, c->C, c->p.itk, c->kc_sa); });
                     if (have_table) timed(c0, KID_OVERSET, 1, [&] { exchange(1); });
                 }
-                for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr);
+                if (zl)
+                    solver_daf_zones(zs, *zl, n, decide_recalc_rh, itr);
+                else
+                    for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr);
                 // wall functions on the j walls of zone 1 (mg_driver.F90:218-224)
                 if (n == 1 && (c0->hooks & DESALM_HOOK_WALL_FUNCTIONS))
                     timed(c0, KID_BCOND, 1, [&] { launch_wall_functions(s, c0->lv[1], c0->F, c0->p.ren); });
                 if (n == 1 && have_table && apply_mom) timed(c0, KID_OVERSET, 1, [&] { exchange(0); });
             }
         }
-        for (desalm_ctx* c : zs) {
-            for (int n = ng - 1; n >= 1; n--) {
-                timed(c, KID_PROLONG, n, [&] {
-                    launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, c->p.icrs, c->p.jcrs, c->p.kcrs);
-                });
+        if (zl) {
+            for (int n = ng - 1; n >= 1; n--) timed(c0, KID_PROLONG, n, [&] { launch_prolong_z(s, *zl, n); });
+            timed(c0, KID_BCOND, 1, [&] {
+                launch_bcond_fm_z(s, *zl);
+                for (desalm_ctx* c : zs)
+                    if (!c->alm.t.empty()) launch_alm_hub(s, c->lv[1], c->F, c->alm);
+            });
+            timed(c0, KID_RESID, 1, [&] { launch_resid_and_save_z(s, *zl, 1); });
+        } else {
+            for (desalm_ctx* c : zs) {
+                for (int n = ng - 1; n >= 1; n--) {
+                    timed(c, KID_PROLONG, n, [&] {
+                        launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, c->p.icrs, c->p.jcrs, c->p.kcrs);
+                    });
+                }
+                timed(c, KID_BCOND, 1, [&] { bcond(c, s); });
+                // update norms, then qold_mg <- q for the next pseudo-iteration
+                timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1); });
             }
-            timed(c, KID_BCOND, 1, [&] { bcond(c, s); });
-            // update norms, then qold_mg <- q for the next pseudo-iteration
-            timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1); });
         }
         return decide_recalc_rh;
     };
@@ -1222,7 +1269,38 @@ struct desalm_group {
     double init_resd[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // one launch per kernel over all zones (desalm_group_set_batched)
+    int batched = 1;
+    ZoneList zl;
 };
+
+// the block-descriptor table of the group's zones on the device; false when
+// the zones cannot share launches (more than kMaxBatchZones zones, or FP64
+// and mixed-precision zones together)
+static bool group_zone_table(desalm_group* g) {
+    if (!g->batched || g->z.size() < 2 || g->z.size() > (size_t)kMaxBatchZones) return false;
+    for (desalm_ctx* c : g->z)
+        if (c->C.mixed != g->z[0]->C.mixed) return false;
+    g->zl.host.clear();
+    for (desalm_ctx* c : g->z) {
+        ZoneDesc d{};
+        for (int n = 0; n <= kMaxLevels; n++) d.lv[n] = c->lv[n];
+        d.F = c->F;
+        d.C = c->C;
+        const Level& L1 = c->lv[1];
+        d.pfix = L1.off + (c->p.ifix - 1) + (long)L1.ni * ((c->p.jfix - 1) + (long)L1.nj * (c->p.kfix - 1));
+        d.icrs = c->p.icrs;
+        d.jcrs = c->p.jcrs;
+        d.kcrs = c->p.kcrs;
+        g->zl.host.push_back(d);
+    }
+    if (!g->zl.dev && hipMalloc((void**)&g->zl.dev, sizeof(ZoneDesc) * kMaxBatchZones) != hipSuccess) {
+        g->zl.dev = nullptr;
+        return false;
+    }
+    return hipMemcpy(g->zl.dev, g->zl.host.data(), sizeof(ZoneDesc) * g->zl.host.size(), hipMemcpyHostToDevice) ==
+           hipSuccess;
+}
 
 static void group_free_table(desalm_group* g) {
     for (void* p : g->dev) (void)hipFree(p);
@@ -1374,8 +1452,9 @@ int desalm_group_mg_driver(desalm_group* g, int* itc_out, double* resid_out) {
             launch_bintp_scatter(s, c->lv[1], c->F, c->C, o, tur, c->cvin, c->p.des);
         }
     };
+    const bool batch = group_zone_table(g);
     int rc = hipEventRecord(g->ev0, s) == hipSuccess ? 0 : fail(c0, "hipEventRecord failed");
-    if (!rc) rc = vcycles(g->z, g->nd > 0, g->mom, exchange, g->init_resd, itc_out, resid_out);
+    if (!rc) rc = vcycles(g->z, g->nd > 0, g->mom, exchange, g->init_resd, itc_out, resid_out, batch ? &g->zl : nullptr);
     if (!rc && hipEventRecord(g->ev1, s) != hipSuccess) rc = fail(c0, "hipEventRecord failed");
     if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(c0, "hipStreamSynchronize failed");
     for (size_t z = 0; z < g->z.size(); z++) g->z[z]->stream = own[z];
@@ -1392,9 +1471,16 @@ int desalm_group_destroy(desalm_group* g) {
     (void)hipSetDevice(g->z[0]->device);
     for (desalm_ctx* c : g->z) (void)hipStreamSynchronize(c->stream);
     group_free_table(g);
+    if (g->zl.dev) (void)hipFree(g->zl.dev);
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
     delete g;
+    return 0;
+}
+
+int desalm_group_set_batched(desalm_group* g, int on) {
+    if (!g) return fail(nullptr, "null argument");
+    g->batched = on ? 1 : 0;
     return 0;
 }
 

@@ -183,6 +183,59 @@ struct Rfg {
 
 struct Ctx;  // host context, desalm_api.hip
 
+// ---- one launch over the zones of a group ----
+// The block-descriptor table: every zone's levels, fields and constants in
+// device memory.  A batched launch covers the workgroups of all zones, zone
+// z owning workgroups start[z] .. start[z+1]-1; a workgroup finds its zone by
+// a scan of the (at most kMaxBatchZones + 1) starts, all in its kernel
+// arguments, and runs the single-zone body with that zone's level, fields
+// and constants and its zone-local workgroup index.
+constexpr int kMaxBatchZones = 16;
+struct ZoneDesc {
+    Level lv[kMaxLevels + 1];
+    Fields F;
+    Consts C;
+    long pfix;  // global node of the pinned pressure (bcond_fm.F90:310-322)
+    int icrs, jcrs, kcrs;
+};
+struct ZoneGrid {
+    const ZoneDesc* __restrict__ z;
+    int nz;
+    unsigned start[kMaxBatchZones + 1];
+};
+struct ZoneBlock {
+    int z;
+    unsigned bid, nblk;
+};
+__device__ __forceinline__ ZoneBlock zone_block(const ZoneGrid& G) {
+    const unsigned b = blockIdx.x;
+    int z = 0;
+    while (z + 1 < G.nz && b >= G.start[z + 1]) z++;
+    return {z, b - G.start[z], G.start[z + 1] - G.start[z]};
+}
+// host side of a table: the host copies (block counts are computed from
+// them) and the device copy
+struct ZoneList {
+    std::vector<ZoneDesc> host;
+    ZoneDesc* dev = nullptr;
+    int nz() const { return (int)host.size(); }
+    // grid of one launch; nb(zone) = that zone's workgroup count
+    template <class NB>
+    ZoneGrid grid(NB nb, unsigned& total) const {
+        ZoneGrid G{};
+        G.z = dev;
+        G.nz = nz();
+        unsigned acc = 0;
+        for (int z = 0; z < G.nz; z++) {
+            G.start[z] = acc;
+            acc += (unsigned)nb(host[z]);
+        }
+        G.start[G.nz] = acc;
+        total = acc;
+        return G;
+    }
+};
+
 // ---- launchers (each enqueues on `s`) ----
 // residual assembly, k_residual.hip
 void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
@@ -242,6 +295,15 @@ void launch_bintp_scatter(hipStream_t s, const Level& L, const Fields& F, const 
 void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A, double ren);
 void launch_rfg_inflow(hipStream_t s, const Level& L, const Fields& F, const Rfg& R, double t);
 void launch_alm_hub(hipStream_t s, const Level& L, const Fields& F, const Alm& A);
+// every zone of a group in one launch per kernel (the table's level n)
+void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n);
+void launch_residual_z(hipStream_t s, const ZoneList& Z, int n, int recalc, int calc_pk, int kc);
+void launch_daf_z(hipStream_t s, const ZoneList& Z, int n, int D);
+void launch_des_eddy_z(hipStream_t s, const ZoneList& Z, int itk, int kc);
+void launch_nlevel_z(hipStream_t s, const ZoneList& Z, int n);
+void launch_prolong_z(hipStream_t s, const ZoneList& Z, int n);
+void launch_bcond_fm_z(hipStream_t s, const ZoneList& Z);
+void launch_resid_and_save_z(hipStream_t s, const ZoneList& Z, int save);
 // batched GEPP (LAPACK SGTSV semantics), k_daf.hip
 void launch_gtsv_batch(hipStream_t s, int n, int nsys, double* dl, double* d, double* du,
                        double* b, int* info);

@@ -225,8 +225,9 @@ __device__ __forceinline__ int pow2_exponent(S mx) {
 // last rows (fine level, rhs_diag_solver.F90:87-135 and its j/k copies),
 // built here by the threads of those rows.
 template <int D, int R, typename S>
-__global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, SweepIO<S> io, SweepShape sh,
-                                                  int use_b1, int use_b2, int b2_full, int ext, double* dq_out) {
+__device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const Consts& C, const SweepIO<S>& io,
+                                         const SweepShape& sh, int use_b1, int use_b2, int b2_full, int ext,
+                                         double* dq_out, unsigned bid) {
     extern __shared__ double smem_d[];
     S* lds = reinterpret_cast<S*>(smem_d);
     const int P = sh.P, TI = sh.TI, n = sh.n;
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     // lines: adjacent lines (adjacent i) on adjacent lanes
     const int li = D == 0 ? tid / P : tid % TI;
     const int ps = D == 0 ? tid % P : tid / TI;
-    const long t = (long)blockIdx.x * TI + li;
+    const long t = (long)bid * TI + li;
     const bool act = t < sh.nlines;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     long base = 0, sD;
@@ -368,7 +369,7 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     S* slot = lds + li * LS + ps * kSlot;
     const bool scanner = tid < 4 * TI;
     const int sl = tid / 4, sm = tid % 4;
-    const bool scan_act = scanner && (long)blockIdx.x * TI + sl < sh.nlines;
+    const bool scan_act = scanner && (long)bid * TI + sl < sh.nlines;
     S* sline = lds + sl * LS;  // the scanned line's slots
 
     // ---- phase A: Moebius maps of c' over the segment, scanned ----
@@ -596,6 +597,84 @@ __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, 
     if (zrow) atomicMin(F.zero_pivot, (unsigned)zrow);
 }
 
+// the boundary-vector flags and the line shape of direction D on level n
+// (the i lines take the characteristic-outlet row when ext); P and TI are
+// the launch's
+struct SweepFlags {
+    int use_b1, use_b2, b2_full, ext;
+};
+__host__ __device__ inline SweepFlags sweep_flags(const Consts& C, int D, int n) {
+    SweepFlags f;
+    f.use_b1 = (n == 1 && C.btype[2 * D] != 5);
+    f.use_b2 = (n == 1 && C.btype[2 * D + 1] != 5);
+    f.b2_full = (n == 1 && C.btype[5] != 6);
+    // a characteristic outlet at i = ni - 1 joins the fine level's sweeps
+    f.ext = (n == 1 && C.btype[1] == DESALM_BC_OUTLET_MOC);
+    return f;
+}
+__host__ __device__ inline int sweep_len(const Level& L, int D, int ext) {
+    return D == 0 ? L.ni - 2 + ext : D == 1 ? L.nj - 2 : L.nk - 2;
+}
+__host__ __device__ inline SweepShape sweep_lines(const Level& L, int D, int ext, int P, int TI) {
+    SweepShape sh;
+    sh.nI = L.ni - 2 + ext;
+    sh.n = sweep_len(L, D, ext);
+    sh.nlines = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)sh.nI * (L.nk - 2) : (long)sh.nI * (L.nj - 2);
+    sh.P = P;
+    sh.TI = TI;
+    return sh;
+}
+// segments per line and lines per workgroup for lines of up to len rows
+__host__ __device__ inline void sweep_tiles(int D, int len, int R, int& P, int& TI) {
+    P = (len + R - 1) / R;
+    if (P < 4) P = 4;
+    // j and k lines: 16 adjacent i lines per workgroup (128-byte rows)
+    TI = D == 0 ? (256 / P > 1 ? 256 / P : 1) : 16;
+    while (TI * P > 512 && TI > 1) TI /= 2;
+}
+
+// the solution of the i sweep goes to F.dp, of the j sweep to F.cp
+// (level-local); the mixed form uses their FP32 counterparts
+template <typename S>
+__host__ __device__ inline SweepIO<S> sweep_io(const Fields& F, int D);
+template <>
+__host__ __device__ inline SweepIO<double> sweep_io<double>(const Fields& F, int D) {
+    SweepIO<double> io;
+    for (int m = 0; m < 4; m++) {
+        io.xin[m] = D == 1 ? F.dp[m] : F.cp[m];
+        io.xout[m] = D == 0 ? F.dp[m] : F.cp[m];
+    }
+    return io;
+}
+template <>
+__host__ __device__ inline SweepIO<float> sweep_io<float>(const Fields& F, int D) {
+    SweepIO<float> io;
+    for (int m = 0; m < 4; m++) {
+        io.xin[m] = D == 1 ? F.dpf[m] : F.cpf[m];
+        io.xout[m] = D == 0 ? F.dpf[m] : F.cpf[m];
+    }
+    return io;
+}
+
+template <int D, int R, typename S>
+__global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, SweepIO<S> io, SweepShape sh,
+                                                  int use_b1, int use_b2, int b2_full, int ext, double* dq_out) {
+    daf_body<D, R, S>(L, F, C, io, sh, use_b1, use_b2, b2_full, ext, dq_out, blockIdx.x);
+}
+
+// every zone of a group in one launch; P and TI are the launch's (from the
+// longest line of any zone): a zone's shorter lines leave trailing segments
+// empty, which pass the scanned maps through unchanged
+template <int D, int R, typename S>
+__global__ void __launch_bounds__(512) k_daf_sweep_z(ZoneGrid G, int n, int P, int TI) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    const Level& L = Z.lv[n];
+    const SweepFlags f = sweep_flags(Z.C, D, n);
+    daf_body<D, R, S>(L, Z.F, Z.C, sweep_io<S>(Z.F, D), sweep_lines(L, D, f.ext, P, TI), f.use_b1, f.use_b2,
+                      f.b2_full, f.ext, nullptr, b.bid);
+}
+
 // ---- batched LAPACK SGTSV (src/sgtsv.f:89-259) in FP64, one system per thread ----
 __global__ void k_gtsv_batch(int n, int nsys, double* dl, double* d, double* du, double* b, int* info) {
     const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -662,69 +741,61 @@ __global__ void k_gtsv_batch(int n, int nsys, double* dl, double* d, double* du,
 
 }  // namespace
 
-// launch shape of one direction's sweep on a level
-static SweepShape sweep_shape(const Level& L, int D, int ext, int R) {
-    SweepShape sh;
-    sh.nI = L.ni - 2 + ext;
-    sh.n = D == 0 ? L.ni - 2 + ext : D == 1 ? L.nj - 2 : L.nk - 2;
-    sh.nlines = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)sh.nI * (L.nk - 2) : (long)sh.nI * (L.nj - 2);
-    sh.P = std::max(4, (sh.n + R - 1) / R);
-    // j and k lines: 16 adjacent i lines per workgroup (128-byte rows)
-    sh.TI = D == 0 ? std::max(1, 256 / sh.P) : 16;
-    while (sh.TI * sh.P > 512 && sh.TI > 1) sh.TI /= 2;
-    return sh;
-}
-
 template <int D, typename S>
-static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Consts& C, const SweepIO<S>& io, int n,
-                      double* dq_out) {
-    const int use_b1 = (n == 1 && C.btype[2 * D] != 5);
-    const int use_b2 = (n == 1 && C.btype[2 * D + 1] != 5);
-    const int b2_full = (n == 1 && C.btype[5] != 6);
-    // a characteristic outlet at i = ni - 1 joins the fine level's sweeps
-    const int ext = (n == 1 && C.btype[1] == DESALM_BC_OUTLET_MOC);
-    const int len = D == 0 ? L.ni - 2 + ext : D == 1 ? L.nj - 2 : L.nk - 2;
+static void daf_sweep(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, double* dq_out) {
+    const SweepFlags f = sweep_flags(C, D, n);
+    const int len = sweep_len(L, D, f.ext);
     // i lines: short segments (their rows are adjacent in memory, so lanes
     // reading row r of adjacent segments are R rows apart)
     const int R = (D == 0 || len <= 64) ? 4 : 8;
-    const SweepShape sh = sweep_shape(L, D, ext, R);
+    int P = 0, TI = 0;
+    sweep_tiles(D, len, R, P, TI);
+    const SweepShape sh = sweep_lines(L, D, f.ext, P, TI);
     const unsigned nb = (unsigned)((sh.nlines + sh.TI - 1) / sh.TI);
     const size_t lds = sizeof(S) * (size_t)sh.TI * line_stride(sh.P);
+    const SweepIO<S> io = sweep_io<S>(F, D);
     if (R == 4)
-        hipLaunchKernelGGL((k_daf_sweep<D, 4, S>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, io, sh, use_b1,
-                           use_b2, b2_full, ext, dq_out);
+        hipLaunchKernelGGL((k_daf_sweep<D, 4, S>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, io, sh, f.use_b1,
+                           f.use_b2, f.b2_full, f.ext, dq_out);
     else
-        hipLaunchKernelGGL((k_daf_sweep<D, 8, S>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, io, sh, use_b1,
-                           use_b2, b2_full, ext, dq_out);
+        hipLaunchKernelGGL((k_daf_sweep<D, 8, S>), dim3(nb), dim3(sh.TI * sh.P), lds, s, L, F, C, io, sh, f.use_b1,
+                           f.use_b2, f.b2_full, f.ext, dq_out);
 }
 
-// the solution of the i sweep goes to F.dp, of the j sweep to F.cp
-// (level-local); the mixed form uses their FP32 counterparts
-template <typename S>
-static SweepIO<S> sweep_io(const Fields& F, int D);
-template <>
-SweepIO<double> sweep_io<double>(const Fields& F, int D) {
-    SweepIO<double> io;
-    for (int m = 0; m < 4; m++) {
-        io.xin[m] = D == 1 ? F.dp[m] : F.cp[m];
-        io.xout[m] = D == 0 ? F.dp[m] : F.cp[m];
-    }
-    return io;
-}
-template <>
-SweepIO<float> sweep_io<float>(const Fields& F, int D) {
-    SweepIO<float> io;
-    for (int m = 0; m < 4; m++) {
-        io.xin[m] = D == 1 ? F.dpf[m] : F.cpf[m];
-        io.xout[m] = D == 0 ? F.dpf[m] : F.cpf[m];
-    }
-    return io;
+template <int D, typename S>
+static void daf_sweep_z(hipStream_t s, const ZoneList& Z, int n) {
+    int len = 0;
+    for (const ZoneDesc& z : Z.host) len = std::max(len, sweep_len(z.lv[n], D, sweep_flags(z.C, D, n).ext));
+    const int R = (D == 0 || len <= 64) ? 4 : 8;
+    int P = 0, TI = 0;
+    sweep_tiles(D, len, R, P, TI);
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid(
+        [&](const ZoneDesc& z) {
+            const SweepShape sh = sweep_lines(z.lv[n], D, sweep_flags(z.C, D, n).ext, P, TI);
+            return (unsigned)((sh.nlines + TI - 1) / TI);
+        },
+        nb);
+    const size_t lds = sizeof(S) * (size_t)TI * line_stride(P);
+    if (R == 4)
+        hipLaunchKernelGGL((k_daf_sweep_z<D, 4, S>), dim3(nb), dim3(TI * P), lds, s, G, n, P, TI);
+    else
+        hipLaunchKernelGGL((k_daf_sweep_z<D, 8, S>), dim3(nb), dim3(TI * P), lds, s, G, n, P, TI);
 }
 
 template <int D>
 static void daf_dir(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, double* dq_out) {
-    if (C.mixed) daf_sweep<D, float>(s, L, F, C, sweep_io<float>(F, D), n, dq_out);
-    else daf_sweep<D, double>(s, L, F, C, sweep_io<double>(F, D), n, dq_out);
+    if (C.mixed) daf_sweep<D, float>(s, L, F, C, n, dq_out);
+    else daf_sweep<D, double>(s, L, F, C, n, dq_out);
+}
+
+// the sweep of direction D on level n over every zone of a group in one
+// launch (the zones share the precision form)
+void launch_daf_z(hipStream_t s, const ZoneList& Z, int n, int D) {
+    const bool f = Z.host[0].C.mixed;
+    if (D == 0) f ? daf_sweep_z<0, float>(s, Z, n) : daf_sweep_z<0, double>(s, Z, n);
+    else if (D == 1) f ? daf_sweep_z<1, float>(s, Z, n) : daf_sweep_z<1, double>(s, Z, n);
+    else f ? daf_sweep_z<2, float>(s, Z, n) : daf_sweep_z<2, double>(s, Z, n);
 }
 
 void launch_daf_i(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n) {

@@ -51,8 +51,9 @@ __global__ void k_inject_p(Level Lc, Level Lf, Ptrs P, int ncomp, int icrs, int 
     for (int m = 0; m < ncomp; m++) P.p[m][gc] = P.p[m][gf];
 }
 
-__global__ void __launch_bounds__(256) k_nlevel(Level Lc, Level Lf, Fields F, int icrs, int jcrs, int kcrs) {
-    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nlevel_body(const Level& Lc, const Level& Lf, const Fields& F, int icrs, int jcrs,
+                                            int kcrs, unsigned bid, unsigned nb) {
+    const long t = (long)xcd_block(bid, nb) * blockDim.x + threadIdx.x;
     if (t >= Lc.n) return;
     const int ni = Lc.ni, nj = Lc.nj, nk = Lc.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
@@ -108,8 +109,8 @@ struct ProlongCtx {
 
 // coarse correction q - qold_mg at every coarse node (mg_prolong.F90:57-75),
 // into the line-solve scratch F.cp (free during prolongation), level-local
-__global__ void __launch_bounds__(256) k_prolong_corr(Level Lc, Fields F) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prolong_corr_body(const Level& Lc, const Fields& F, unsigned bid) {
+    const long t = (long)bid * blockDim.x + threadIdx.x;
     if (t >= Lc.n) return;
     const long l = Lc.off + t;
 #pragma unroll
@@ -138,8 +139,8 @@ __device__ __forceinline__ double fill_i(const ProlongCtx& P, const Fields& F, i
 
 // dq at every fine interior node into F.dp (level-local); boundary dq is
 // zero after smoothing and never enters the interior sweeps
-__global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
-    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prolong_fill_body(const ProlongCtx& P, const Fields& F, unsigned bid, unsigned nb) {
+    const long t = (long)xcd_block(bid, nb) * blockDim.x + threadIdx.x;
     const Level& L = P.Lf;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
@@ -155,13 +156,14 @@ __global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
 // D = 2 adds the result to q instead.  The reference calls SGTSV with four
 // right-hand sides; this is its non-interchanging elimination.
 template <int D, int R>
-__global__ void __launch_bounds__(512) k_rs_sweep(Level L, Fields F, SegShape sh, double epn) {
+__device__ __forceinline__ void rs_sweep_body(const Level& L, const Fields& F, const SegShape& sh, double epn,
+                                              unsigned bid) {
     extern __shared__ double smem_rs[];
     const int P = sh.P, TI = sh.TI, n = sh.n;
     const int tid = (int)threadIdx.x;
     const int li = D == 0 ? tid / P : tid % TI;
     const int ps = D == 0 ? tid % P : tid / TI;
-    const long line0 = (long)blockIdx.x * TI;
+    const long line0 = (long)bid * TI;
     const long t = line0 + li;
     const bool act = t < sh.nlines;
     const int ni = L.ni, nj = L.nj;
@@ -235,10 +237,10 @@ __global__ void k_outlet_ratio(Level L, Fields F, int b, int dir) {
 // only where btype == 4 .and. btype == 5 (init.F90:2855, :2874), i.e. never,
 // and the table is zero as allocated.
 template <int D>
-__global__ void k_bc_fm(Level L, Fields F, Consts C) {
+__device__ __forceinline__ void bc_fm_body(const Level& L, const Fields& F, const Consts& C, unsigned bid) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const long sI = 1, sJ = ni, sK = (long)ni * nj;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)bid * blockDim.x + threadIdx.x;
     long base, sD;
     int nd;
     if (D == 0) {
@@ -313,19 +315,20 @@ __global__ void k_bc_fm(Level L, Fields F, Consts C) {
 
 __global__ void k_pfix_read(Fields F, long g) { *F.pfix = F.q[0][g]; }
 
-__global__ void k_pfix_sub(Level L, Fields F) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pfix_sub_body(const Level& L, const Fields& F, unsigned bid) {
+    const long t = (long)bid * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const double pf = *F.pfix;
     F.q[0][L.off + t] = F.q[0][L.off + t] - pf;
 }
 
 // ---- mg_residual: deterministic two-stage L1 sums ----
-__global__ void __launch_bounds__(256) k_resid_part(Level L, Fields F, int save) {
+__device__ __forceinline__ void resid_part_body(const Level& L, const Fields& F, int save, unsigned bid,
+                                                unsigned nb) {
     __shared__ double sh[5][256];
     double acc[5] = {0, 0, 0, 0, 0};
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < L.n; t += (long)gridDim.x * blockDim.x) {
+    for (long t = (long)bid * blockDim.x + threadIdx.x; t < L.n; t += (long)nb * blockDim.x) {
         const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
         const long g = L.off + t;
         const bool in = !(i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1);
@@ -348,11 +351,11 @@ __global__ void __launch_bounds__(256) k_resid_part(Level L, Fields F, int save)
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) F.resid_part[blockIdx.x * 5 + m] = sh[m][0];
+        for (int m = 0; m < 5; m++) F.resid_part[bid * 5 + m] = sh[m][0];
     }
 }
 
-__global__ void __launch_bounds__(256) k_resid_final(Fields F, int nblocks) {
+__device__ __forceinline__ void resid_final_body(const Fields& F, int nblocks) {
     __shared__ double sh[5][256];
     double acc[5] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
@@ -374,6 +377,8 @@ __global__ void __launch_bounds__(256) k_resid_final(Fields F, int nblocks) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_resid_final(Fields F, int nblocks) { resid_final_body(F, nblocks); }
+
 __global__ void k_copy_qold(Level L, Fields F) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
@@ -391,6 +396,70 @@ __global__ void k_soa_to_aos(Ptrs P, int ncomp, long n, double* aos) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     for (int m = 0; m < ncomp; m++) aos[t * ncomp + m] = P.p[m][t];
+}
+
+__global__ void __launch_bounds__(256) k_nlevel(Level Lc, Level Lf, Fields F, int icrs, int jcrs, int kcrs) {
+    nlevel_body(Lc, Lf, F, icrs, jcrs, kcrs, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) k_nlevel_z(ZoneGrid G, int n) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    nlevel_body(Z.lv[n], Z.lv[n - 1], Z.F, Z.icrs, Z.jcrs, Z.kcrs, b.bid, b.nblk);
+}
+__global__ void __launch_bounds__(256) k_prolong_corr(Level Lc, Fields F) { prolong_corr_body(Lc, F, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_prolong_corr_z(ZoneGrid G, int nc) {
+    const ZoneBlock b = zone_block(G);
+    prolong_corr_body(G.z[b.z].lv[nc], G.z[b.z].F, b.bid);
+}
+__global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
+    prolong_fill_body(P, F, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) k_prolong_fill_z(ZoneGrid G, int n) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    const ProlongCtx P{Z.lv[n], Z.lv[n + 1], Z.icrs, Z.jcrs, Z.kcrs};
+    prolong_fill_body(P, Z.F, b.bid, b.nblk);
+}
+template <int D, int R>
+__global__ void __launch_bounds__(512) k_rs_sweep(Level L, Fields F, SegShape sh, double epn) {
+    rs_sweep_body<D, R>(L, F, sh, epn, blockIdx.x);
+}
+template <int D, int R>
+__global__ void __launch_bounds__(512) k_rs_sweep_z(ZoneGrid G, int n, int P, int TI) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    rs_sweep_body<D, R>(Z.lv[n], Z.F, seg_lines(Z.lv[n], D, P, TI), Z.C.ep[1][D], b.bid);
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_bc_fm(Level L, Fields F, Consts C) {
+    bc_fm_body<D>(L, F, C, blockIdx.x);
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_bc_fm_z(ZoneGrid G) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    bc_fm_body<D>(Z.lv[1], Z.F, Z.C, b.bid);
+}
+// one thread per zone
+__global__ void k_pfix_read_z(const ZoneDesc* __restrict__ z, int nz) {
+    const int t = (int)threadIdx.x;
+    if (t < nz) *z[t].F.pfix = z[t].F.q[0][z[t].pfix];
+}
+__global__ void __launch_bounds__(256) k_pfix_sub(Level L, Fields F) { pfix_sub_body(L, F, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_pfix_sub_z(ZoneGrid G) {
+    const ZoneBlock b = zone_block(G);
+    pfix_sub_body(G.z[b.z].lv[1], G.z[b.z].F, b.bid);
+}
+__global__ void __launch_bounds__(256) k_resid_part(Level L, Fields F, int save) {
+    resid_part_body(L, F, save, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) k_resid_part_z(ZoneGrid G, int save) {
+    const ZoneBlock b = zone_block(G);
+    resid_part_body(G.z[b.z].lv[1], G.z[b.z].F, save, b.bid, b.nblk);
+}
+// one workgroup per zone
+__global__ void __launch_bounds__(256) k_resid_final_z(const ZoneDesc* __restrict__ z, int nblocks) {
+    resid_final_body(z[blockIdx.x].F, nblocks);
 }
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -517,6 +586,64 @@ void launch_probe(hipStream_t s, const Fields& F, int n, const long* node, doubl
 void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save) {
     hipLaunchKernelGGL(k_resid_part, dim3(kResidBlocks), dim3(256), 0, s, L, F, save);
     hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, s, F, kResidBlocks);
+}
+
+// ---- every zone of a group in one launch per kernel ----
+void launch_nlevel_z(hipStream_t s, const ZoneList& Z, int n) {
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid([&](const ZoneDesc& z) { return nblk(z.lv[n].n, 256); }, nb);
+    hipLaunchKernelGGL(k_nlevel_z, dim3(nb), dim3(256), 0, s, G, n);
+}
+
+template <int D>
+static void rs_sweep_z(hipStream_t s, const ZoneList& Z, int n) {
+    int len = 0;
+    for (const ZoneDesc& z : Z.host) len = std::max(len, seg_lines(z.lv[n], D, 1, 1).n);
+    const int R = (D == 0 || len <= 64) ? 4 : 8;
+    const SegShape sh = seg_shape(len, 1, R, D == 0);
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid(
+        [&](const ZoneDesc& z) { return (unsigned)((seg_lines(z.lv[n], D, sh.P, sh.TI).nlines + sh.TI - 1) / sh.TI); },
+        nb);
+    const size_t lds = seg_lds_bytes(sh, 4);
+    if (R == 4)
+        hipLaunchKernelGGL((k_rs_sweep_z<D, 4>), dim3(nb), dim3(sh.TI * sh.P), lds, s, G, n, sh.P, sh.TI);
+    else
+        hipLaunchKernelGGL((k_rs_sweep_z<D, 8>), dim3(nb), dim3(sh.TI * sh.P), lds, s, G, n, sh.P, sh.TI);
+}
+
+// prolongation from level n + 1 onto level n
+void launch_prolong_z(hipStream_t s, const ZoneList& Z, int n) {
+    unsigned nc = 0, nf = 0;
+    const ZoneGrid Gc = Z.grid([&](const ZoneDesc& z) { return nblk(z.lv[n + 1].n, 256); }, nc);
+    const ZoneGrid Gf = Z.grid([&](const ZoneDesc& z) { return nblk(z.lv[n].n, 256); }, nf);
+    hipLaunchKernelGGL(k_prolong_corr_z, dim3(nc), dim3(256), 0, s, Gc, n + 1);
+    hipLaunchKernelGGL(k_prolong_fill_z, dim3(nf), dim3(256), 0, s, Gf, n);
+    rs_sweep_z<0>(s, Z, n);
+    rs_sweep_z<1>(s, Z, n);
+    rs_sweep_z<2>(s, Z, n);
+}
+
+// bcond_fm faces and the pinned-pressure shift of every zone's fine level
+void launch_bcond_fm_z(hipStream_t s, const ZoneList& Z) {
+    unsigned fi = 0, fj = 0, fk = 0, gn = 0;
+    const ZoneGrid Gi = Z.grid([&](const ZoneDesc& z) { return nblk((long)z.lv[1].nj * z.lv[1].nk, 256); }, fi);
+    const ZoneGrid Gj = Z.grid([&](const ZoneDesc& z) { return nblk((long)z.lv[1].ni * z.lv[1].nk, 256); }, fj);
+    const ZoneGrid Gk = Z.grid([&](const ZoneDesc& z) { return nblk((long)z.lv[1].ni * z.lv[1].nj, 256); }, fk);
+    const ZoneGrid Gn = Z.grid([&](const ZoneDesc& z) { return nblk(z.lv[1].n, 256); }, gn);
+    hipLaunchKernelGGL(k_bc_fm_z<0>, dim3(fi), dim3(256), 0, s, Gi);
+    hipLaunchKernelGGL(k_bc_fm_z<1>, dim3(fj), dim3(256), 0, s, Gj);
+    hipLaunchKernelGGL(k_bc_fm_z<2>, dim3(fk), dim3(256), 0, s, Gk);
+    hipLaunchKernelGGL(k_pfix_read_z, dim3(1), dim3(64), 0, s, (const ZoneDesc*)Z.dev, Z.nz());
+    hipLaunchKernelGGL(k_pfix_sub_z, dim3(gn), dim3(256), 0, s, Gn);
+}
+
+// update norms of every zone's fine level into its F.resid, then qold <- q
+void launch_resid_and_save_z(hipStream_t s, const ZoneList& Z, int save) {
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid([&](const ZoneDesc&) { return (unsigned)kResidBlocks; }, nb);
+    hipLaunchKernelGGL(k_resid_part_z, dim3(nb), dim3(256), 0, s, G, save);
+    hipLaunchKernelGGL(k_resid_final_z, dim3(Z.nz()), dim3(256), 0, s, (const ZoneDesc*)Z.dev, kResidBlocks);
 }
 
 void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F) {

@@ -34,8 +34,9 @@ __device__ __forceinline__ double sqr(double a) { return a * a; }
 // also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
 // nodes interior in that direction, into F.fv (level-local): the residual
 // then reads it at a node and its two neighbours.
-__global__ void k_prep(Level L, Fields F, Consts C, int with_dtau) {
-    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const Consts& C, int with_dtau,
+                                          unsigned bid, unsigned nblk) {
+    const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
     const double c1 = F.csi[0][g], c2 = F.csi[1][g], c3 = F.csi[2][g];
@@ -125,8 +126,9 @@ __device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, 
 // (diss(1) fresh and stored); recalc = 1: the decide_recalc_rh branch
 // (src/solver_daf.F90:432-484), which reuses the stored diss(1) of the
 // pre-update q and dtau.
-__global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc) {
-    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void residual_body(const Level& L, const Fields& F, const Consts& C, int n, int recalc,
+                                              unsigned bid, unsigned nblk) {
+    const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni);
@@ -303,14 +305,20 @@ __device__ __forceinline__ void visc_face_t(const Level& L, const Consts& C, U u
         fv[r] = vtj * (f1 * uu[r][0] + f2 * uu[r][1] + f3 * uu[r][2] + cca * dcs[r] + ccb * det[r] + ccc * dze[r]);
 }
 
-__global__ void __launch_bounds__(kTX * kTY) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk, int kc) {
+// tiles of the viscous kernel along i and j
+__host__ __device__ inline unsigned visc_gx(const Level& L) { return (unsigned)((L.ni - 2 + kTX - 2) / (kTX - 1)); }
+__host__ __device__ inline unsigned visc_gy(const Level& L) { return (unsigned)((L.nj - 2 + kTY - 2) / (kTY - 1)); }
+
+// bid: linear workgroup index over (i tile, j tile, k chunk), i fastest
+__device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, const Consts& C, int n, int calc_pk,
+                                             int kc, unsigned bid, unsigned nblk) {
     __shared__ double s_u[3][3][kLN];  // velocity component, plane slot (k mod 3), tile node
     __shared__ double s_m[kMet][kLN];  // metrics, J, nu_t of the current plane
     __shared__ double s_fi[3][kTY][kTX];
     __shared__ double s_fj[3][kTY][kTX];
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+    const unsigned gx = visc_gx(L), gy = visc_gy(L);
+    const unsigned lb = xcd_block(bid, nblk);
     const int bx = (int)(lb % gx), by = (int)((lb / gx) % gy), bz = (int)(lb / (gx * gy));
     const int tid = (int)threadIdx.x;
     const int tx = tid % kTX, ty = tid / kTX;
@@ -452,11 +460,63 @@ __global__ void __launch_bounds__(kTX * kTY) k_visc_pk(Level L, Fields F, Consts
     }
 }
 
+__global__ void __launch_bounds__(256) k_prep(Level L, Fields F, Consts C, int with_dtau) {
+    prep_body(L, F, C, with_dtau, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) k_prep_z(ZoneGrid G, int n, int with_dtau) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    prep_body(Z.lv[n], Z.F, Z.C, with_dtau, b.bid, b.nblk);
+}
+__global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc) {
+    residual_body(L, F, C, n, recalc, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) k_residual_z(ZoneGrid G, int n, int recalc) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    residual_body(Z.lv[n], Z.F, Z.C, n, recalc, b.bid, b.nblk);
+}
+__global__ void __launch_bounds__(kTX * kTY) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk, int kc) {
+    visc_pk_body(L, F, C, n, calc_pk, kc, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                 gridDim.x * gridDim.y * gridDim.z);
+}
+__global__ void __launch_bounds__(kTX * kTY) k_visc_pk_z(ZoneGrid G, int n, int calc_pk, int kc) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    visc_pk_body(Z.lv[n], Z.F, Z.C, n, calc_pk, kc, b.bid, b.nblk);
+}
+
 }  // namespace
 
 void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C) {
     const int b = 256;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C, 1);
+}
+
+static unsigned node_blocks(const Level& L) { return (unsigned)((L.n + 255) / 256); }
+
+void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n) {
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid([&](const ZoneDesc& z) { return node_blocks(z.lv[n]); }, nb);
+    hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, 1);
+}
+
+void launch_residual_z(hipStream_t s, const ZoneList& Z, int n, int recalc, int calc_pk, int kc) {
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid([&](const ZoneDesc& z) { return node_blocks(z.lv[n]); }, nb);
+    if (recalc) hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, 0);
+    hipLaunchKernelGGL(k_residual_z, dim3(nb), dim3(256), 0, s, G, n, recalc);
+    // one plane count for every zone: the smallest zone's default
+    if (kc <= 0) {
+        kc = 16;
+        for (const ZoneDesc& z : Z.host) kc = std::min(kc, visc_kc_default(z.lv[n]));
+    }
+    for (const ZoneDesc& z : Z.host) kc = std::min(kc, z.lv[n].nk - 2);
+    unsigned nv = 0;
+    const ZoneGrid V = Z.grid(
+        [&](const ZoneDesc& z) { return visc_gx(z.lv[n]) * visc_gy(z.lv[n]) * (unsigned)((z.lv[n].nk - 2 + kc - 1) / kc); },
+        nv);
+    hipLaunchKernelGGL(k_visc_pk_z, dim3(nv), dim3(kTX * kTY), 0, s, V, n, recalc ? 0 : calc_pk, kc);
 }
 
 int visc_kc_default(const Level& L) {
@@ -468,8 +528,7 @@ int visc_kc_default(const Level& L) {
 void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int calc_pk, int kc) {
     if (kc <= 0) kc = visc_kc_default(L);
     kc = std::min(kc, L.nk - 2);
-    const dim3 gv((unsigned)((L.ni - 2 + kTX - 2) / (kTX - 1)), (unsigned)((L.nj - 2 + kTY - 2) / (kTY - 1)),
-                  (unsigned)((L.nk - 2 + kc - 1) / kc));
+    const dim3 gv(visc_gx(L), visc_gy(L), (unsigned)((L.nk - 2 + kc - 1) / kc));
     hipLaunchKernelGGL(k_visc_pk, gv, dim3(kTX * kTY), 0, s, L, F, C, n, calc_pk, kc);
 }
 

@@ -26,8 +26,8 @@ namespace {
 constexpr double cb1 = 0.1355, cb2 = 0.622, sig = 2.0 / 3.0, capa = 0.41, cw1 = 3.23906782, cw2 = 0.3,
                  cw3h = 64.0, cv1c = 357.911, cv2 = 5.0;
 
-__global__ void k_sa_prep(Level L, Fields F) {
-    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sa_prep_body(const Level& L, const Fields& F, unsigned bid, unsigned nblk) {
+    const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
@@ -164,15 +164,21 @@ __device__ __forceinline__ void sa_conv_av(const Consts& C, double q0, double q1
     av = ucna + qdiv(ev * (gx * gx + gy * gy + gz * gz), m1.v[9] + m0.v[9]);
 }
 
-__global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts C, int kc) {
+// tiles of the SA right-hand side along i and j
+__host__ __device__ inline unsigned sa_gx(const Level& L) { return (unsigned)((L.ni - 2 + kQX - 2) / (kQX - 1)); }
+__host__ __device__ inline unsigned sa_gy(const Level& L) { return (unsigned)((L.nj - 2 + kQY - 2) / (kQY - 1)); }
+
+// bid: linear workgroup index over (i tile, j tile, k chunk), i fastest
+__device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, const Consts& C, int kc, unsigned bid,
+                                            unsigned nblk) {
     __shared__ double s_q[3][kQLN];      // nu~ of planes k-1, k, k+1 (slot k mod 3)
     __shared__ double s_m[kQM][kQLN];    // metrics, J of plane k
     __shared__ double s_u[2][kQLN];      // U/J, V/J of plane k
     __shared__ double s_fv[2][kQY][kQX];  // viscous flux of the +i, +j faces
     __shared__ double s_fc[2][kQY][kQX];  // convective flux of the +i, +j faces
     const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+    const unsigned gx = sa_gx(L), gy = sa_gy(L);
+    const unsigned lb = xcd_block(bid, nblk);
     const int bx = (int)(lb % gx), by = (int)((lb / gx) % gy), bz = (int)(lb / (gx * gy));
     const int tid = (int)threadIdx.x;
     const int tx = tid % kQX, ty = tid / kQX;
@@ -366,10 +372,10 @@ __global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts 
 
 // rhs_sa_bcond faces (rhs_sa_bcond.F90:40-269); faces of direction D, both sides
 template <int D>
-__global__ void k_sa_bc(Level L, Fields F, Consts C) {
+__device__ __forceinline__ void sa_bc_body(const Level& L, const Fields& F, const Consts& C, unsigned bid) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const long sI = 1, sJ = ni, sK = (long)ni * nj;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)bid * blockDim.x + threadIdx.x;
     long nface, base, sD;
     int nd;
     if (D == 0) {
@@ -422,8 +428,8 @@ __global__ void k_sa_bc(Level L, Fields F, Consts C) {
 }
 
 // nu~ floor and xnut = nu~ fv1 on every node (rhs_sa_bcond.F90:279-291)
-__global__ void k_sa_clamp(Level L, Fields F, Consts C) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sa_clamp_body(const Level& L, const Fields& F, const Consts& C, unsigned bid) {
+    const long t = (long)bid * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
     const double q = fmax(1.0e-07, F.q[4][g]);
@@ -442,13 +448,14 @@ __global__ void k_sa_clamp(Level L, Fields F, Consts C) {
 // D = 2 also applies the relaxed update of des_eddy.F90:260-272:
 //   q5 += (-q5 + qold + x) / (1 + 1.5 e_source dtev / delti), qold = q5 here.
 template <int D, int R>
-__global__ void __launch_bounds__(512) k_sa_sweep(Level L, Fields F, Consts C, SegShape sh) {
+__device__ __forceinline__ void sa_sweep_body(const Level& L, const Fields& F, const Consts& C, const SegShape& sh,
+                                              unsigned bid) {
     extern __shared__ double smem_sa[];
     const int P = sh.P, TI = sh.TI, n = sh.n;
     const int tid = (int)threadIdx.x;
     const int li = D == 0 ? tid / P : tid % TI;
     const int ps = D == 0 ? tid % P : tid / TI;
-    const long line0 = (long)blockIdx.x * TI;
+    const long line0 = (long)bid * TI;
     const long t = line0 + li;
     const bool act = t < sh.nlines;
     const int ni = L.ni, nj = L.nj;
@@ -509,6 +516,47 @@ __global__ void __launch_bounds__(512) k_sa_sweep(Level L, Fields F, Consts C, S
     if (zrow) atomicMin(F.zero_pivot, (unsigned)zrow);
 }
 
+__global__ void __launch_bounds__(256) k_sa_prep(Level L, Fields F) { sa_prep_body(L, F, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) k_sa_prep_z(ZoneGrid G) {
+    const ZoneBlock b = zone_block(G);
+    sa_prep_body(G.z[b.z].lv[1], G.z[b.z].F, b.bid, b.nblk);
+}
+__global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts C, int kc) {
+    sa_rhs_body(L, F, C, kc, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                gridDim.x * gridDim.y * gridDim.z);
+}
+__global__ void __launch_bounds__(kQX * kQY) k_sa_rhs_z(ZoneGrid G, int kc) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    sa_rhs_body(Z.lv[1], Z.F, Z.C, kc, b.bid, b.nblk);
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_sa_bc(Level L, Fields F, Consts C) {
+    sa_bc_body<D>(L, F, C, blockIdx.x);
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_sa_bc_z(ZoneGrid G) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    sa_bc_body<D>(Z.lv[1], Z.F, Z.C, b.bid);
+}
+__global__ void __launch_bounds__(256) k_sa_clamp(Level L, Fields F, Consts C) { sa_clamp_body(L, F, C, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_sa_clamp_z(ZoneGrid G) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    sa_clamp_body(Z.lv[1], Z.F, Z.C, b.bid);
+}
+template <int D, int R>
+__global__ void __launch_bounds__(512) k_sa_sweep(Level L, Fields F, Consts C, SegShape sh) {
+    sa_sweep_body<D, R>(L, F, C, sh, blockIdx.x);
+}
+template <int D, int R>
+__global__ void __launch_bounds__(512) k_sa_sweep_z(ZoneGrid G, int P, int TI) {
+    const ZoneBlock b = zone_block(G);
+    const ZoneDesc& Z = G.z[b.z];
+    sa_sweep_body<D, R>(Z.lv[1], Z.F, Z.C, seg_lines(Z.lv[1], D, P, TI), b.bid);
+}
+
 }  // namespace
 
 int sa_kc_default(const Level& L) {
@@ -524,8 +572,7 @@ void launch_sa_prep(hipStream_t s, const Level& L, const Fields& F) {
 void launch_sa_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int kc) {
     if (kc <= 0) kc = sa_kc_default(L);
     kc = std::min(kc, L.nk - 2);
-    const dim3 gq((unsigned)((L.ni - 2 + kQX - 2) / (kQX - 1)), (unsigned)((L.nj - 2 + kQY - 2) / (kQY - 1)),
-                  (unsigned)((L.nk - 2 + kc - 1) / kc));
+    const dim3 gq(sa_gx(L), sa_gy(L), (unsigned)((L.nk - 2 + kc - 1) / kc));
     hipLaunchKernelGGL(k_sa_rhs, gq, dim3(kQX * kQY), 0, s, L, F, C, kc);
 }
 
@@ -563,6 +610,55 @@ void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Const
         hipLaunchKernelGGL(k_sa_bc<1>, dim3(fj), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_bc<2>, dim3(fk), dim3(b), 0, s, L, F, C);
         hipLaunchKernelGGL(k_sa_clamp, dim3(gn), dim3(b), 0, s, L, F, C);
+    }
+}
+
+// the ADI sweep of direction D over every zone of a group in one launch,
+// with the segment count of the longest line
+template <int D>
+static void sa_sweep_z(hipStream_t s, const ZoneList& Z) {
+    int len = 0;
+    for (const ZoneDesc& z : Z.host) len = std::max(len, seg_lines(z.lv[1], D, 1, 1).n);
+    const int R = (D == 0 || len <= 64) ? 4 : 8;
+    const SegShape sh = seg_shape(len, 1, R, D == 0);
+    unsigned nb = 0;
+    const ZoneGrid G = Z.grid(
+        [&](const ZoneDesc& z) { return (unsigned)((seg_lines(z.lv[1], D, sh.P, sh.TI).nlines + sh.TI - 1) / sh.TI); },
+        nb);
+    const size_t lds = seg_lds_bytes(sh, 1);
+    if (R == 4)
+        hipLaunchKernelGGL((k_sa_sweep_z<D, 4>), dim3(nb), dim3(sh.TI * sh.P), lds, s, G, sh.P, sh.TI);
+    else
+        hipLaunchKernelGGL((k_sa_sweep_z<D, 8>), dim3(nb), dim3(sh.TI * sh.P), lds, s, G, sh.P, sh.TI);
+}
+
+// des_eddy on the fine level of every zone of a group, one launch per kernel
+void launch_des_eddy_z(hipStream_t s, const ZoneList& Z, int itk, int kc) {
+    const int b = 256;
+    auto nodes = [&](const ZoneDesc& z) { return (unsigned)((z.lv[1].n + b - 1) / b); };
+    unsigned gn = 0, fi = 0, fj = 0, fk = 0, gq = 0;
+    const ZoneGrid Gn = Z.grid(nodes, gn);
+    const ZoneGrid Gi = Z.grid([&](const ZoneDesc& z) { return (unsigned)(((long)z.lv[1].nj * z.lv[1].nk + b - 1) / b); }, fi);
+    const ZoneGrid Gj = Z.grid([&](const ZoneDesc& z) { return (unsigned)(((long)z.lv[1].ni * z.lv[1].nk + b - 1) / b); }, fj);
+    const ZoneGrid Gk = Z.grid([&](const ZoneDesc& z) { return (unsigned)(((long)z.lv[1].ni * z.lv[1].nj + b - 1) / b); }, fk);
+    if (kc <= 0) {
+        kc = 16;
+        for (const ZoneDesc& z : Z.host) kc = std::min(kc, sa_kc_default(z.lv[1]));
+    }
+    for (const ZoneDesc& z : Z.host) kc = std::min(kc, z.lv[1].nk - 2);
+    const ZoneGrid Gq = Z.grid(
+        [&](const ZoneDesc& z) { return sa_gx(z.lv[1]) * sa_gy(z.lv[1]) * (unsigned)((z.lv[1].nk - 2 + kc - 1) / kc); },
+        gq);
+    hipLaunchKernelGGL(k_sa_prep_z, dim3(gn), dim3(b), 0, s, Gn);
+    for (int it = 0; it < itk; it++) {
+        hipLaunchKernelGGL(k_sa_rhs_z, dim3(gq), dim3(kQX * kQY), 0, s, Gq, kc);
+        sa_sweep_z<0>(s, Z);
+        sa_sweep_z<1>(s, Z);
+        sa_sweep_z<2>(s, Z);
+        hipLaunchKernelGGL(k_sa_bc_z<0>, dim3(fi), dim3(b), 0, s, Gi);
+        hipLaunchKernelGGL(k_sa_bc_z<1>, dim3(fj), dim3(b), 0, s, Gj);
+        hipLaunchKernelGGL(k_sa_bc_z<2>, dim3(fk), dim3(b), 0, s, Gk);
+        hipLaunchKernelGGL(k_sa_clamp_z, dim3(gn), dim3(b), 0, s, Gn);
     }
 }
 

@@ -55,6 +55,17 @@ inline SegShape seg_shape(int n, long nlines, int R, bool seg_fast) {
     return sh;
 }
 
+// lines of direction D over the interior of level L with the launch's P, TI
+inline __host__ __device__ SegShape seg_lines(const Level& L, int D, int P, int TI) {
+    SegShape sh;
+    sh.n = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
+    sh.nlines = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
+                                                                : (long)(L.ni - 2) * (L.nj - 2);
+    sh.P = P;
+    sh.TI = TI;
+    return sh;
+}
+
 // exponent e of mx = f 2^e with f in [0.5, 1): scaling by 2^-e is exact
 __device__ __forceinline__ int seg_exponent(double mx) {
     int e = 0;

@@ -81,6 +81,7 @@ def lib():
         L.desalm_group_set_overset.argtypes = [vp, ctypes.c_int64, i32p, dp, i32p, ctypes.c_int]
         L.desalm_group_mg_driver.argtypes = [vp, ip, dp]
         L.desalm_group_destroy.argtypes = [vp]
+        L.desalm_group_set_batched.argtypes = [vp, ctypes.c_int]
         L.desalm_set_turbines.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, ctypes.c_double,
                                           ctypes.c_double]
         fp = ctypes.POINTER(ctypes.c_float)
@@ -118,7 +119,8 @@ def lib():
                   "desalm_set_state", "desalm_get_state", "desalm_mg_driver", "desalm_solver_daf",
                   "desalm_des_eddy", "desalm_gtsv_batch", "desalm_gtsv_batch_noalloc", "desalm_get_level_field", "desalm_finalize",
                   "desalm_set_overset", "desalm_bintp", "desalm_group_create", "desalm_group_set_overset",
-                  "desalm_group_mg_driver", "desalm_group_destroy", "desalm_set_turbines"):
+                  "desalm_group_mg_driver", "desalm_group_destroy", "desalm_group_set_batched",
+                  "desalm_set_turbines"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -489,6 +491,12 @@ class Group:
         if rc != 0:
             raise DesalmError(f"desalm_group_set_overset: rc={rc}: " +
                               lib().desalm_last_error(self.solvers[0].h).decode())
+
+    def set_batched(self, on=True):
+        """One launch per kernel over all zones (the default), or zone by zone."""
+        rc = lib().desalm_group_set_batched(self.h, 1 if on else 0)
+        if rc != 0:
+            raise DesalmError(f"desalm_group_set_batched: rc={rc}")
 
     def mg_driver(self):
         itm = self.solvers[0].params.itm

@@ -256,6 +256,16 @@ int desalm_group_set_overset(desalm_group* group, int64_t n, const int32_t* host
  * root rank does (src/mg_residual.F90:36,71-82). */
 int desalm_group_mg_driver(desalm_group* group, int* itc_out, double* resid_out);
 int desalm_group_destroy(desalm_group* group);
+/* One launch per kernel over all zones of the group (on = 1, the default):
+ * every kernel of a pseudo-iteration runs once over the workgroups of all
+ * zones, each workgroup finding its zone in a block-descriptor table of the
+ * zones' levels, fields and constants.  The zones' characteristic-outlet
+ * rows, actuator-line forcing and hub zeroing, the wall functions of zone 1
+ * and the donor exchanges keep their own launches.  on = 0 launches zone by
+ * zone.  Zones of mixed precision forms, or more than 16 zones, always run
+ * zone by zone.  No counterpart in the reference, whose zones run on their
+ * own MPI ranks (mg_driver.F90:241-247). */
+int desalm_group_set_batched(desalm_group* group, int on);
 
 /*
  * Actuator-line model (rhs_lines, src/rhs_lines.F90:47-483, called from

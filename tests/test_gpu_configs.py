@@ -15,7 +15,8 @@ it.
 - Lid-driven cavity (laminar, steady, two levels): the full 200
   pseudo-iterations: 1e-10 and identical norm histories to 1e-9.
 - Overset: six hill blocks of uneven sizes coupled by the project's donor
-  search, zone group against the oracle's group driver: 1e-10.
+  search, zone group against the oracle's group driver: 1e-10, with one
+  launch per kernel over all blocks and zone by zone.
 - Full sizes (129^3 channel, 193x129x129 and 257x193x193 actuator-line
   boxes): finite state, no row where SGTSV would interchange, and the HIP
   graph replay of the pseudo-iteration equal to eager launches.
@@ -150,11 +151,13 @@ def test_cavity_full_200_iterations_match_oracle(gpu):
 HILL_BLOCKS = [(21, 15, 11), (23, 17, 13), (27, 15, 15), (29, 21, 15), (33, 21, 17), (17, 11, 9)]
 
 
-def test_six_hill_blocks_match_oracle_group(gpu):
+@pytest.mark.parametrize("batched", [True, False], ids=["one_launch", "zone_by_zone"])
+def test_six_hill_blocks_match_oracle_group(gpu, batched):
     cs, tab = overset.hill_blocks(HILL_BLOCKS, [0.15, 0.32, 0.5, 0.68, 0.86], itm=3, amp=0.2, ren=10595.0)
     ors = [po.Oracle(c.params).load_case(c) for c in cs]
     svs = [Solver(c.params).load_case(c) for c in cs]
     g = Group(svs)
+    g.set_batched(batched)
     g.set_overset(tab)
     for _ in range(2):
         itc_o, ho = po.group_mg_driver(ors, tab)
