@@ -170,6 +170,20 @@ int validate(const desalm_params* p, std::string& why) {
         why = "at most 4098 nodes per direction (lines of up to 4096 rows)";
         return -1;
     }
+    {
+        // kernels address every array with 32-bit byte offsets
+        long nall = 0, a = p->img, b = p->jmg, d = p->kmg;
+        for (int n = 1; n <= p->ng; n++) {
+            nall += a * b * d;
+            a = (a + p->icrs) / (1 << p->icrs);
+            b = (b + p->jcrs) / (1 << p->jcrs);
+            d = (d + p->kcrs) / (1 << p->kcrs);
+        }
+        if (nall > (1L << 29)) {
+            why = "at most 2^29 nodes over all levels (arrays below 4 GiB)";
+            return -1;
+        }
+    }
     for (int b = 0; b < 6; b++)
         if (p->btype[b] < 0 || p->btype[b] > 8) {
             why = "unsupported boundary type (btype 0-8)";
@@ -384,14 +398,6 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     F.dtau = A(c->nall); F.diss1 = A(c->nall);
     for (int m = 0; m < 4; m++) F.cp[m] = A(c->n1);
     for (int m = 0; m < 4; m++) F.dp[m] = A(c->n1);
-    F.sd = A(c->n1);
-    {
-        const Level& L1 = c->lv[1];
-        // lines of the fine sweeps; the j and k sweeps of a characteristic
-        // outlet also run on the plane i = ni - 1
-        long ml = std::max((long)(L1.nj - 2) * (L1.nk - 2), std::max((long)(L1.ni - 1) * (L1.nk - 2), (long)(L1.ni - 1) * (L1.nj - 2)));
-        F.brh = A(8 * ml);
-    }
     for (int m = 0; m < 3; m++) F.av[m] = A(c->n1);
     F.dtj = A(c->n1);
     for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
@@ -400,8 +406,7 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     if (c->C.mixed) {
         // FP32 DAF scratch: 13 floats per fine node, carved from doubles
         auto Af = [&](long n) { return (float*)A((n + 1) / 2); };
-        F.sdf = Af(c->n1);
-        for (int m = 0; m < 4; m++) { F.dpf[m] = Af(c->n1); F.cpf[m] = Af(c->n1); F.xf[m] = Af(c->n1); }
+        for (int m = 0; m < 4; m++) { F.dpf[m] = Af(c->n1); F.cpf[m] = Af(c->n1); }
     }
     F.resid_part = A(5L * kResidBlocks);
     // the five L1 sums, then the first zero or non-finite pivot of the line

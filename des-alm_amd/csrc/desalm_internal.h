@@ -30,6 +30,20 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
     return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + l;
 }
 
+// Element idx of an array addressed as its base plus a 32-bit byte offset:
+// the form of the hardware's scalar-base + vector-offset loads and stores,
+// which lets the loads of several arrays at one node share one offset
+// register instead of holding a 64-bit address each.  Every device array is
+// below 4 GiB (desalm_init refuses more than 2^29 nodes over all levels).
+template <typename T>
+__device__ __forceinline__ T ld32(const T* a, unsigned idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a) + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* a, unsigned idx, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(a) + idx * (unsigned)sizeof(T)) = v;
+}
+
 struct Level {
     int ni, nj, nk;
     long off;   // node offset of the level in all-level arrays
@@ -74,18 +88,14 @@ struct Fields {
     double* diss1;    // all: pressure dissipation kept for the recalc branch
     double* cp[4];    // fine: line-solve c' per scalar system
     double* dp[4];    // fine: sweep right-hand sides / d', prolongation dq
-    double* sd;       // fine: dtau * spectral radius of the sweep direction
-    double* brh;      // 8 per line: semi-explicit boundary vectors of a sweep
     double* av[3];    // fine: SA ADI face coefficients per direction
     double* dtj;      // fine: SA ADI 0.5 dtev J
     double* ucn[3];   // fine: contravariant velocity / J (residual, SA update)
     double* pj;       // fine: p / J
     double* fv[3];    // fine: pressure-dissipation fv per direction
     // FP32 DAF scratch of the mixed-precision form (fine size; null in FP64)
-    float* sdf;
     float* dpf[4];
     float* cpf[4];
-    float* xf[4];
     double* wo;       // fine: SA vorticity measure
     double* rh5;      // fine: SA right-hand side
     double* resid_part;  // [kResidBlocks * 5] partial L1 sums
@@ -99,7 +109,7 @@ struct Fields {
     unsigned* zero_pivot;        // [1] first line row with a zero or non-finite pivot (0xffffffff: none)
 };
 
-constexpr int kResidBlocks = 1024;
+constexpr int kResidBlocks = 4096;
 
 // Overset donor table of the fine level (device pointers, level-local node
 // indices): donor d reads the cell whose lower corner is host[d] with

@@ -174,11 +174,12 @@ struct SweepShape {
     long nlines;
 };
 
-// values of one segment slot in LDS (the largest phase: 3 Moebius maps);
-// a line's slots are followed by one pad value so that the scanning threads
-// of adjacent lines fall on different LDS banks
-constexpr int kSlot = 12;
-__host__ __device__ constexpr int line_stride(int P) { return P * kSlot + 1; }
+// values of one segment slot in LDS (the largest phase: 3 Moebius maps, 12
+// values) rounded up to an odd count, and a line's slots padded to an odd
+// count: the threads of adjacent segments, and the scanning threads of
+// adjacent lines, then fall on different LDS banks
+constexpr int kSlot = 13;
+__host__ __device__ constexpr int line_stride(int P) { return (P * kSlot) | 1; }
 
 // tridiagonal row of matrix mi (eigenvalue 0, +1, -1) from sd = dtau spr of
 // the row's node and its two neighbours (rhs_diag_solver.F90:149-189); the

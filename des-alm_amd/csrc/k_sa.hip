@@ -106,11 +106,11 @@ struct QMet {
 __device__ __forceinline__ void qload_met(const Fields& F, long g, bool ok, QMet& m) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        m.v[c] = ok ? F.csi[c][g] : 0.0;
-        m.v[3 + c] = ok ? F.eta[c][g] : 0.0;
-        m.v[6 + c] = ok ? F.zet[c][g] : 0.0;
+        m.v[c] = ok ? ld32(F.csi[c], (unsigned)(g)) : 0.0;
+        m.v[3 + c] = ok ? ld32(F.eta[c], (unsigned)(g)) : 0.0;
+        m.v[6 + c] = ok ? ld32(F.zet[c], (unsigned)(g)) : 0.0;
     }
-    m.v[9] = ok ? F.aj[g] : 0.0;
+    m.v[9] = ok ? ld32(F.aj, (unsigned)(g)) : 0.0;
 }
 
 // viscous flux of nu~ across the face node 0 -> node 0 + e_D
@@ -211,8 +211,8 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
     // prologue: nu~ of planes kb-1 .. kb+1; metrics, J, U/J, V/J of plane kb
     for (int k = kb - 1; k <= kb + 1; k++) {
         const int sl = k % 3;
-        s_q[sl][own] = own_ok ? q5[o + col + sK * k] : 0.0;
-        if (has_halo) s_q[sl][halo] = halo_ok ? q5[o + hc + sK * k] : 0.0;
+        s_q[sl][own] = own_ok ? ld32(q5, (unsigned)(o + col + sK * k)) : 0.0;
+        if (has_halo) s_q[sl][halo] = halo_ok ? ld32(q5, (unsigned)(o + hc + sK * k)) : 0.0;
     }
     QMet mcur, mh;
     qload_met(F, o + col + sK * kb, own_ok, mcur);
@@ -224,13 +224,13 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
     }
 #pragma unroll
     for (int d = 0; d < 2; d++) {
-        s_u[d][own] = own_ok ? F.ucn[d][col + sK * kb] : 0.0;
-        if (has_halo) s_u[d][halo] = halo_ok ? F.ucn[d][hc + sK * kb] : 0.0;
+        s_u[d][own] = own_ok ? ld32(F.ucn[d], (unsigned)(col + sK * kb)) : 0.0;
+        if (has_halo) s_u[d][halo] = halo_ok ? ld32(F.ucn[d], (unsigned)(hc + sK * kb)) : 0.0;
     }
     QMet mprev;
     qload_met(F, o + col + sK * (kb - 1), own_ok, mprev);
-    double u2prev = own_ok ? F.ucn[2][col + sK * (kb - 1)] : 0.0;
-    double u2cur = own_ok ? F.ucn[2][col + sK * kb] : 0.0;
+    double u2prev = own_ok ? ld32(F.ucn[2], (unsigned)(col + sK * (kb - 1))) : 0.0;
+    double u2cur = own_ok ? ld32(F.ucn[2], (unsigned)(col + sK * kb)) : 0.0;
     __syncthreads();
     double fkv = 0.0, fkc = 0.0;  // fluxes of the face below the current plane
     if (out) {
@@ -239,7 +239,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
         fkv = sa_visc_face_t<2>(L, C, q, mprev, mcur);
         double av;
         sa_conv_av<2>(C, s_q[sl0][own], s_q[sl1][own], u2prev, u2cur, mprev, mcur, fkc, av);
-        if (kb == 1) F.av[2][col] = av;  // the face k = 1/2, position 0
+        if (kb == 1) st32(F.av[2], (unsigned)(col), av);  // the face k = 1/2, position 0
     }
     for (int k = kb; k < ke; k++) {
         // next planes into registers
@@ -249,15 +249,15 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
         double un[2], uh[2];
 #pragma unroll
         for (int d = 0; d < 2; d++) {
-            un[d] = own_ok ? F.ucn[d][col + sK * (k + 1)] : 0.0;
-            uh[d] = halo_ok ? F.ucn[d][hc + sK * (k + 1)] : 0.0;
+            un[d] = own_ok ? ld32(F.ucn[d], (unsigned)(col + sK * (k + 1))) : 0.0;
+            uh[d] = halo_ok ? ld32(F.ucn[d], (unsigned)(hc + sK * (k + 1))) : 0.0;
         }
-        const double u2next = own_ok ? F.ucn[2][col + sK * (k + 1)] : 0.0;
+        const double u2next = own_ok ? ld32(F.ucn[2], (unsigned)(col + sK * (k + 1))) : 0.0;
         const bool more = k + 1 < ke;
         double qn_o = 0.0, qn_h = 0.0;
         if (more) {
-            qn_o = own_ok ? q5[o + col + sK * (k + 2)] : 0.0;
-            qn_h = halo_ok ? q5[o + hc + sK * (k + 2)] : 0.0;
+            qn_o = own_ok ? ld32(q5, (unsigned)(o + col + sK * (k + 2))) : 0.0;
+            qn_h = halo_ok ? ld32(q5, (unsigned)(o + hc + sK * (k + 2))) : 0.0;
         }
         const int slm = (k + 2) % 3, sl0 = k % 3, slp = (k + 1) % 3;
         auto q = [&](int a, int b, int c) {
@@ -270,7 +270,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             double cv, av;
             sa_conv_av<0>(C, q(0, 0, 0), q(1, 0, 0), s_u[0][own], s_u[0][own + 1], mcur, m1, cv, av);
             s_fc[0][ty][tx] = cv;
-            F.av[0][t] = av;
+            st32(F.av[0], (unsigned)(t), av);
         }
         if (face_j) {
             const QMet m1 = qmet_at(own + kQLX);
@@ -278,7 +278,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             double cv, av;
             sa_conv_av<1>(C, q(0, 0, 0), q(0, 1, 0), s_u[1][own], s_u[1][own + kQLX], mcur, m1, cv, av);
             s_fc[1][ty][tx] = cv;
-            F.av[1][t] = av;
+            st32(F.av[1], (unsigned)(t), av);
         }
         double fkv1 = 0.0, fkc1 = 0.0;
         double dnu = 0.0;
@@ -286,7 +286,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             fkv1 = sa_visc_face_t<2>(L, C, q, mcur, mnext);
             double av;
             sa_conv_av<2>(C, q(0, 0, 0), q(0, 0, 1), u2cur, u2next, mcur, mnext, fkc1, av);
-            F.av[2][t] = av;
+            st32(F.av[2], (unsigned)(t), av);
             // gradient of nu~ (rhs_sa_grdnu.F90:22-60)
             const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
             const double qip = qdiv(q(1, 0, 0), s_m[9][own + 1]), qim = qdiv(q(-1, 0, 0), s_m[9][own - 1]);
@@ -308,7 +308,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             const double aj = mcur.v[9];
             // production, destruction, cb2 diffusion (rhs_sa_prod.F90:44-100)
             double psa;
-            const double wdv = F.wd[g];
+            const double wdv = ld32(F.wd, (unsigned)(g));
             if (wdv == 0.0) {
                 psa = 0.0;
             } else {
@@ -319,7 +319,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
                 const double xf = 1.0 + qdiv(chi, cv2);
                 const double fv2 = qdiv(1.0, xf * xf * xf);
                 const double fv3 = qdiv((1.0 + chi * fv1) * (1.0 - fv2), chi);
-                const double vtcm = sqrt(2.0 * F.wo[t]);
+                const double vtcm = sqrt(2.0 * ld32(F.wo, (unsigned)(t)));
                 const double cawd = capa * capa * wdv * wdv;
                 const double vtc_org = fv3 * vtcm + qdiv(qv * fv2, cawd);
                 const double vtc = fmax(vtc_org, 0.00001);
@@ -340,11 +340,11 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             r = r + s_fc[1][ty][tx] - s_fc[1][ty - 1][tx];
             r = r + fkc1 - fkc;
             const double dtjb = 2.0 * C.delti * aj;
-            const double dq5dt = C.e_source * qdiv(3.0 * q(0, 0, 0) - 4.0 * F.qn[4][g] + F.qnm1[4][g], dtjb);
+            const double dq5dt = C.e_source * qdiv(3.0 * q(0, 0, 0) - 4.0 * ld32(F.qn[4], (unsigned)(g)) + ld32(F.qnm1[4], (unsigned)(g)), dtjb);
             r = r + dq5dt - visc - qdiv(psa, aj);
-            const double dtev = F.dtev[g];
-            F.rh5[t] = -dtev * aj * r;
-            F.dtj[t] = 0.5 * dtev * aj;
+            const double dtev = ld32(F.dtev, (unsigned)(g));
+            st32(F.rh5, (unsigned)(t), -dtev * aj * r);
+            st32(F.dtj, (unsigned)(t), 0.5 * dtev * aj);
         }
         __syncthreads();
 #pragma unroll
@@ -521,11 +521,12 @@ __global__ void __launch_bounds__(256) k_sa_prep_z(ZoneGrid G) {
     const ZoneBlock b = zone_block(G);
     sa_prep_body(G.z[b.z].lv[1], G.z[b.z].F, b.bid, b.nblk);
 }
-__global__ void __launch_bounds__(kQX * kQY) k_sa_rhs(Level L, Fields F, Consts C, int kc) {
+// three waves per SIMD: fits in 168 VGPRs without spilling
+__global__ void __launch_bounds__(kQX * kQY, 3) k_sa_rhs(Level L, Fields F, Consts C, int kc) {
     sa_rhs_body(L, F, C, kc, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
                 gridDim.x * gridDim.y * gridDim.z);
 }
-__global__ void __launch_bounds__(kQX * kQY) k_sa_rhs_z(ZoneGrid G, int kc) {
+__global__ void __launch_bounds__(kQX * kQY, 3) k_sa_rhs_z(ZoneGrid G, int kc) {
     const ZoneBlock b = zone_block(G);
     const ZoneDesc& Z = G.z[b.z];
     sa_rhs_body(Z.lv[1], Z.F, Z.C, kc, b.bid, b.nblk);

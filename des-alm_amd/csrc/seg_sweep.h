@@ -32,11 +32,12 @@ struct SegShape {
     long nlines;
 };
 
-// LDS values per segment slot: the larger of 4 (a Moebius map) and 2 NS
-__host__ __device__ constexpr int seg_slot(int ns) { return 2 * ns > 4 ? 2 * ns : 4; }
-// LDS values per line: its P slots and one pad value, so that the scanning
-// threads of adjacent lines fall on different LDS banks
-__host__ __device__ constexpr int seg_line_stride(int P, int ns) { return P * seg_slot(ns) + 1; }
+// LDS values per segment slot: the larger of 4 (a Moebius map) and 2 NS,
+// rounded up to an odd count, and per line its P slots rounded up to an odd
+// count: the threads of adjacent segments, and the scanning threads of
+// adjacent lines, then fall on different LDS banks
+__host__ __device__ constexpr int seg_slot(int ns) { return (2 * ns > 4 ? 2 * ns : 4) | 1; }
+__host__ __device__ constexpr int seg_line_stride(int P, int ns) { return (P * seg_slot(ns)) | 1; }
 inline size_t seg_lds_bytes(const SegShape& sh, int ns) {
     return sizeof(double) * (size_t)sh.TI * seg_line_stride(sh.P, ns);
 }
