@@ -124,7 +124,17 @@ __global__ void k_alm_elements(Level L, Fields F, AlmTurbine T, AlmPolars P, dou
     f[5] = fL * cos(angle_aux) * sin(phi) - fD * cos(angle_aux) * cos(phi);
 }
 
-__global__ void k_alm_spread(Level L, Fields F, AlmTurbine T, double length_scale, double ren) {
+// element forces staged in LDS per workgroup (up to kAlmStage elements; the
+// rest are read from global memory)
+constexpr int kAlmStage = 256;
+
+__global__ void __launch_bounds__(128) k_alm_spread(Level L, Fields F, AlmTurbine T, double length_scale,
+                                                    double ren) {
+    __shared__ double s_f[kAlmStage * 6];
+    const int ne = T.nblades * T.nelems;
+    const int ns = ne < kAlmStage ? ne : kAlmStage;
+    for (int e = (int)threadIdx.x; e < 6 * ns; e += (int)blockDim.x) s_f[e] = T.forces[e];
+    __syncthreads();
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c >= T.n_ker) return;
     const int ik = T.ker_nodes[3 * c], jk = T.ker_nodes[3 * c + 1], kk = T.ker_nodes[3 * c + 2];
@@ -138,9 +148,8 @@ __global__ void k_alm_spread(Level L, Fields F, AlmTurbine T, double length_scal
     const double eps = 2 * pow(dV, 1.0 / 3.0);
     const double kscale = 1 / ((eps * eps * eps) * pow(kPi, 3.0 / 2.0));
     double fxa = 0.0, fya = 0.0, fza = 0.0;
-    const int ne = T.nblades * T.nelems;
     for (int e = 0; e < ne; e++) {
-        const double* f = T.forces + 6L * e;
+        const double* f = e < ns ? s_f + 6 * e : T.forces + 6L * e;
         const double rdis = sqrt((xk - f[0]) * (xk - f[0]) + (yk - f[1]) * (yk - f[1]) + (zk - f[2]) * (zk - f[2])) * Ls;
         const double ker = kscale * exp(-(rdis * rdis) / (eps * eps));
         fxa = fxa + ker * f[3] * deltar;

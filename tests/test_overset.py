@@ -115,3 +115,33 @@ def test_gpu_group_matches_oracle_group(gpu, blocks, splits, mom):
             assert po.relerr(qg[:, m], qo[:, m]) < TOL, m
         assert po.relerr(xg, xo) < TOL
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batched", [True, False], ids=["one_launch", "zone_by_zone"])
+def test_gpu_group_with_inflow_and_outlet_zones(gpu, batched):
+    """Zones that differ in their boundary types: an inflow face (btype 4) on
+    zone 1 and a characteristic outlet (btype 5) on zone 3, whose rows and
+    grid ratios are per zone also in one launch over all zones."""
+    cs, tab = overset.hill_blocks([(17, 13, 9), (21, 17, 9), (15, 13, 9)], [0.35, 0.7], itm=3)
+    cs[0].params.btype[0] = 4
+    cs[0].params.bdir[0] = 1
+    cs[2].params.btype[1] = 5
+    cs[2].params.bdir[1] = 1
+    ors = [po.Oracle(c.params).load_case(c) for c in cs]
+    svs = [Solver(c.params).load_case(c) for c in cs]
+    g = Group(svs)
+    g.set_batched(batched)
+    g.set_overset(tab)
+    for _ in range(2):
+        itc_o, ho = po.group_mg_driver(ors, tab)
+        itc_g, hg = g.mg_driver()
+        assert itc_o == itc_g
+        assert np.max(np.abs(hg - ho)) < 1e-9
+    for o, s in zip(ors, svs):
+        qo, _, _, xo = o.get_state()
+        qg, _, _, xg = s.get_state()
+        for m in range(5):
+            assert po.relerr(qg[:, m], qo[:, m]) < TOL, m
+        assert po.relerr(xg, xo) < TOL
+    g.close()
