@@ -155,9 +155,9 @@ __device__ __forceinline__ void pre_transform(const Fields& F, const Consts& C, 
 }
 
 // v <- Mc v at global node g (rhs_diag_solver.F90:624-633)
-__device__ __forceinline__ void apply_mc(const Fields& F, const Consts& C, long g, double v[4]) {
-    const NodeDt nd = node_dt(C, F.dtau[g]);
-    mul_m(F.zet[0][g], F.zet[1][g], F.zet[2][g], nd.sigma, v);
+__device__ __forceinline__ void apply_mc(const Consts& C, double dtau, double z1, double z2, double z3, double v[4]) {
+    const NodeDt nd = node_dt(C, dtau);
+    mul_m(z1, z2, z3, nd.sigma, v);
 }
 
 template <typename S>
@@ -262,7 +262,12 @@ __device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const 
     auto gnode = [&](int p) { return L.off + base + (long)p * sD; };  // global node at position p
 
     // ---- load: right-hand sides (transformed) and dtau spr of rows s0-1 .. s0+nr ----
-    // window entry w is node position s0 + w (row s0 + w - 1)
+    // window entry w is node position s0 + w (row s0 + w - 1).  The i sweep
+    // (R = 4) loads and transforms every row without a branch, so that the
+    // loads of all its rows are in flight together: rows past the line's end
+    // (and every row of an idle thread) read the line's last row and are
+    // discarded.  The j and k sweeps (R = 8) keep one branch per row: all
+    // their rows' loads in flight would not fit the registers.
     S dv[R][4];
     S sdv[R + 2];
 #pragma unroll
@@ -270,23 +275,34 @@ __device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const 
         sdv[r + 1] = S(0);
 #pragma unroll
         for (int m = 0; m < 4; m++) dv[r][m] = S(0);
-        if (r < nr) {
-            const int p = s0 + r + 1;
+        if (D == 0 || r < nr) {
+            const int p = min(s0 + r, n - 1) + 1;
             double v[4] = {0.0, 0.0, 0.0, 0.0};
             if (D > 0) {
                 const long tl = base + (long)p * sD;
 #pragma unroll
                 for (int m = 0; m < 4; m++) v[m] = (double)io.xin[m][tl];
             }
-            sdv[r + 1] = (S)transform_row<D>(F, C, gnode(p), v);
+            const double sd = transform_row<D>(F, C, gnode(p), v);
+            const bool in = r < nr;
+            sdv[r + 1] = in ? (S)sd : S(0);
 #pragma unroll
-            for (int m = 0; m < 4; m++) dv[r][m] = (S)v[m];
+            for (int m = 0; m < 4; m++) dv[r][m] = in ? (S)v[m] : S(0);
         }
     }
-    sdv[0] = nr > 0 ? (S)row_sd<D>(F, C, gnode(s0)) : S(0);
     {
         const int pe = s0 + nr + 1;
-        const S se = (nr > 0 && pe <= nd - 1) ? (S)row_sd<D>(F, C, gnode(pe)) : S(0);
+        S slo = S(0), se = S(0);
+        if (D == 0) {
+            const double a = row_sd<D>(F, C, gnode(min(s0, n - 1)));
+            const double b = row_sd<D>(F, C, gnode(min(pe, nd - 1)));
+            slo = nr > 0 ? (S)a : S(0);
+            se = (nr > 0 && pe <= nd - 1) ? (S)b : S(0);
+        } else {
+            slo = nr > 0 ? (S)row_sd<D>(F, C, gnode(s0)) : S(0);
+            se = (nr > 0 && pe <= nd - 1) ? (S)row_sd<D>(F, C, gnode(pe)) : S(0);
+        }
+        sdv[0] = slo;
         sdv[R + 1] = S(0);
 #pragma unroll
         for (int w = 1; w <= R + 1; w++)
@@ -574,22 +590,43 @@ __device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const 
                     }
                 }
             }
-            const int p = s0 + r + 1;
-            const long tl = base + (long)p * sD;
             if (D < 2) {
+                const long tl = base + (long)(s0 + r + 1) * sD;
 #pragma unroll
                 for (int m = 0; m < 4; m++) io.xout[m][tl] = dv[r][m];
-            } else {
-                const long g = L.off + tl;
-                double v[4];
+            }
+        }
+    }
+    if (D == 2) {
+        // Mc and the q update, four rows at a time: the loads of a group are
+        // all in flight, and every value formed, before its first store (rows
+        // past the segment's end read its last row and are not stored)
 #pragma unroll
-                for (int m = 0; m < 4; m++) v[m] = (double)dv[r][m];
-                apply_mc(F, C, g, v);
+        for (int r0 = 0; r0 < R; r0 += 4) {
+            double v[4][4], qv[4][4];
 #pragma unroll
-                for (int m = 0; m < 4; m++) F.q[m][g] = F.q[m][g] + v[m];
-                if (dq_out) {
+            for (int u = 0; u < 4; u++) {
+                const long g = L.off + base + (long)(min(s0 + r0 + u, n - 1) + 1) * sD;
+                const double dt = F.dtau[g], z1 = F.zet[0][g], z2 = F.zet[1][g], z3 = F.zet[2][g];
 #pragma unroll
-                    for (int m = 0; m < 4; m++) dq_out[m * L.n + tl] = v[m];
+                for (int m = 0; m < 4; m++) qv[u][m] = F.q[m][g];
+#pragma unroll
+                for (int m = 0; m < 4; m++) v[u][m] = (double)dv[r0 + u][m];
+                apply_mc(C, dt, z1, z2, z3, v[u]);
+#pragma unroll
+                for (int m = 0; m < 4; m++) qv[u][m] = qv[u][m] + v[u][m];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = r0 + u;
+                if (r < nr) {
+                    const long tl = base + (long)(s0 + r + 1) * sD;
+#pragma unroll
+                    for (int m = 0; m < 4; m++) F.q[m][L.off + tl] = qv[u][m];
+                    if (dq_out) {
+#pragma unroll
+                        for (int m = 0; m < 4; m++) dq_out[m * L.n + tl] = v[u][m];
+                    }
                 }
             }
         }

@@ -59,13 +59,17 @@ __device__ __forceinline__ void nlevel_body(const Level& Lc, const Level& Lf, co
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
     const long ll = Lf.off + lidx(Lf, c2f(i, icrs), c2f(j, jcrs), c2f(k, kcrs));
     const long l = Lc.off + t;
+    // loads before stores throughout, so that they are in flight together
+    double v[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) v[m] = F.q[m][ll];
+    const double xn = F.xnut[ll];
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-        const double v = F.q[m][ll];
-        F.q[m][l] = v;
-        F.qold[m][l] = v;
+        F.q[m][l] = v[m];
+        F.qold[m][l] = v[m];
     }
-    F.xnut[l] = F.xnut[ll];
+    F.xnut[l] = xn;
     if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
         // the reference evaluates these from out-of-level neighbours and then
         // zeroes them (src/mg_driver.F90:134)
@@ -74,20 +78,23 @@ __device__ __forceinline__ void nlevel_body(const Level& Lc, const Level& Lf, co
         return;
     }
     const long ic = 1, ie = Lf.ni, iz = (long)Lf.ni * Lf.nj;
+    double pkv[4];
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         const double* r = F.rh[m] + ll;
-        F.pk[m][l] = (0.25 * r[ic + ie + iz] + 0.5 * r[ic + ie] + 0.25 * r[ic + ie - iz] +
-                      0.5 * r[ie + iz] + r[ie] + 0.5 * r[ie - iz] +
-                      0.25 * r[-ic + ie + iz] + 0.5 * r[-ic + ie] + 0.25 * r[-ic + ie - iz] +
-                      0.5 * r[ic + iz] + r[ic] + 0.5 * r[ic - iz] +
-                      r[iz] + 2.0 * r[0] + r[-iz] +
-                      0.5 * r[-ic + iz] + r[-ic] + 0.5 * r[-ic - iz] +
-                      0.25 * r[ic - ie + iz] + 0.5 * r[ic - ie] + 0.25 * r[ic - ie - iz] +
-                      0.5 * r[-ie + iz] + r[-ie] + 0.5 * r[-ie - iz] +
-                      0.25 * r[-ic - ie + iz] + 0.5 * r[-ic - ie] + 0.25 * r[-ic - ie - iz]) *
-                     (1.0 / 16.0);
+        pkv[m] = (0.25 * r[ic + ie + iz] + 0.5 * r[ic + ie] + 0.25 * r[ic + ie - iz] +
+                   0.5 * r[ie + iz] + r[ie] + 0.5 * r[ie - iz] +
+                   0.25 * r[-ic + ie + iz] + 0.5 * r[-ic + ie] + 0.25 * r[-ic + ie - iz] +
+                   0.5 * r[ic + iz] + r[ic] + 0.5 * r[ic - iz] +
+                   r[iz] + 2.0 * r[0] + r[-iz] +
+                   0.5 * r[-ic + iz] + r[-ic] + 0.5 * r[-ic - iz] +
+                   0.25 * r[ic - ie + iz] + 0.5 * r[ic - ie] + 0.25 * r[ic - ie - iz] +
+                   0.5 * r[-ie + iz] + r[-ie] + 0.5 * r[-ie - iz] +
+                   0.25 * r[-ic - ie + iz] + 0.5 * r[-ic - ie] + 0.25 * r[-ic - ie - iz]) *
+                  (1.0 / 16.0);
     }
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.pk[m][l] = pkv[m];
 }
 
 __global__ void k_zero_pk_boundary(Level L, Fields F) {
@@ -117,28 +124,11 @@ __device__ __forceinline__ void prolong_corr_body(const Level& Lc, const Fields&
     for (int m = 0; m < 4; m++) F.cp[m][t] = F.q[m][l] - F.qold[m][l];
 }
 
-// coarse correction at a coincident fine node
-__device__ __forceinline__ double corr(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
-    const int ci = P.icrs ? i / 2 : i, cj = P.jcrs ? j / 2 : j, ck = P.kcrs ? k / 2 : k;
-    return F.cp[m][lidx(P.Lc, ci, cj, ck)];
-}
-// after the k fill (i, j coincident)
-__device__ __forceinline__ double fill_k(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
-    if (P.kcrs && (k & 1)) return 0.5 * corr(P, F, m, i, j, k + 1) + 0.5 * corr(P, F, m, i, j, k - 1);
-    return corr(P, F, m, i, j, k);
-}
-// after the j fill (i coincident)
-__device__ __forceinline__ double fill_j(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
-    if (P.jcrs && (j & 1)) return 0.5 * fill_k(P, F, m, i, j + 1, k) + 0.5 * fill_k(P, F, m, i, j - 1, k);
-    return fill_k(P, F, m, i, j, k);
-}
-__device__ __forceinline__ double fill_i(const ProlongCtx& P, const Fields& F, int m, int i, int j, int k) {
-    if (P.icrs && (i & 1)) return 0.5 * fill_j(P, F, m, i + 1, j, k) + 0.5 * fill_j(P, F, m, i - 1, j, k);
-    return fill_j(P, F, m, i, j, k);
-}
-
 // dq at every fine interior node into F.dp (level-local); boundary dq is
-// zero after smoothing and never enters the interior sweeps
+// zero after smoothing and never enters the interior sweeps.  The fill of
+// mg_prolong.F90:82-161 in k, then j, then i: a fine node midway between two
+// coarse nodes of a coarsened direction takes 0.5 * high + 0.5 * low of the
+// values filled so far, a coincident one the coarse node's value
 __device__ __forceinline__ void prolong_fill_body(const ProlongCtx& P, const Fields& F, unsigned bid, unsigned nb) {
     const long t = (long)xcd_block(bid, nb) * blockDim.x + threadIdx.x;
     const Level& L = P.Lf;
@@ -146,8 +136,41 @@ __device__ __forceinline__ void prolong_fill_body(const ProlongCtx& P, const Fie
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
     if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
+    // without branches: the coarse nodes on the high and low side of (i, j, k)
+    // in every direction (the same node where the fine node is coincident in
+    // that direction), all eight loaded up front, then the weighted sums
+    // selected per direction
+    const bool oi = P.icrs && (i & 1), oj = P.jcrs && (j & 1), ok = P.kcrs && (k & 1);
+    const int ih = P.icrs ? (i + oi) / 2 : i, il = P.icrs ? (i - oi) / 2 : i;
+    const int jh = P.jcrs ? (j + oj) / 2 : j, jl = P.jcrs ? (j - oj) / 2 : j;
+    const int kh = P.kcrs ? (k + ok) / 2 : k, kl = P.kcrs ? (k - ok) / 2 : k;
+    long c[2][2][2];  // [i side][j side][k side], side 1 = high
 #pragma unroll
-    for (int m = 0; m < 4; m++) F.dp[m][t] = fill_i(P, F, m, i, j, k);
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) c[a][b][e] = lidx(P.Lc, a ? ih : il, b ? jh : jl, e ? kh : kl);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double* cp = F.cp[m];
+        double v[2][2][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int e = 0; e < 2; e++) v[a][b][e] = cp[c[a][b][e]];
+        double fj[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            double fk[2];
+#pragma unroll
+            for (int b = 0; b < 2; b++) fk[b] = ok ? 0.5 * v[a][b][1] + 0.5 * v[a][b][0] : v[a][b][1];
+            fj[a] = oj ? 0.5 * fk[1] + 0.5 * fk[0] : fk[1];
+        }
+        F.dp[m][t] = oi ? 0.5 * fj[1] + 0.5 * fj[0] : fj[1];
+    }
 }
 
 // rs_sweep in direction D on the interior of a level (src/rs_sweep.F90:59-178):
@@ -192,14 +215,36 @@ __device__ __forceinline__ void rs_sweep_body(const Level& L, const Fields& F, c
     unsigned piv = 0;
     int zrow = 0;
     seg_solve<R, 4, double>(a, b, c, d, -epn, nr, n, s0, smem_rs, P, TI, sh.nlines, line0, li, ps, piv, zrow);
+    if (D == 2) {
+        // every q of the segment is loaded, and every sum formed, before
+        // the first store: the loads are in flight together and the stores
+        // wait for nothing
+        double qv[R][4];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        if (r < nr) {
-            const long tl = base + (long)(s0 + r + 1) * sD;
+        for (int r = 0; r < R; r++) {
+            const long g = L.off + base + (long)(min(s0 + r, n - 1) + 1) * sD;
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                if (D < 2) F.dp[m][tl] = d[r][m];
-                else F.q[m][L.off + tl] = F.q[m][L.off + tl] + d[r][m];
+            for (int m = 0; m < 4; m++) qv[r][m] = F.q[m][g];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) qv[r][m] = qv[r][m] + d[r][m];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (r < nr) {
+                const long g = L.off + base + (long)(s0 + r + 1) * sD;
+#pragma unroll
+                for (int m = 0; m < 4; m++) F.q[m][g] = qv[r][m];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (r < nr) {
+                const long tl = base + (long)(s0 + r + 1) * sD;
+#pragma unroll
+                for (int m = 0; m < 4; m++) F.dp[m][tl] = d[r][m];
             }
         }
     }
@@ -236,6 +281,17 @@ __global__ void k_outlet_ratio(Level L, Fields F, int b, int dir) {
 // Inflow (btype 4) on a k face uses ratio 0: the reference fills its table
 // only where btype == 4 .and. btype == 5 (init.F90:2855, :2874), i.e. never,
 // and the table is zero as allocated.
+// the four components of q at node g; each face below loads all it reads
+// before its first store, so that the loads are in flight together
+__device__ __forceinline__ void load4(const Fields& F, long g, double v[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) v[m] = F.q[m][g];
+}
+__device__ __forceinline__ void store4(const Fields& F, long g, const double v[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) F.q[m][g] = v[m];
+}
+
 template <int D>
 __device__ __forceinline__ void bc_fm_body(const Level& L, const Fields& F, const Consts& C, unsigned bid) {
     const int ni = L.ni, nj = L.nj, nk = L.nk;
@@ -261,20 +317,24 @@ __device__ __forceinline__ void bc_fm_body(const Level& L, const Fields& F, cons
     }
     const long lo = L.off + base, hi = L.off + base + (long)(nd - 1) * sD;
     const int b0 = 2 * D, b1 = 2 * D + 1;
+    double a[4], b[4];
     switch (C.btype[b0]) {
         case 1: case 2: case 3:
+            load4(F, lo + sD, a);
+            load4(F, lo + 2 * sD, b);
 #pragma unroll
-            for (int m = 0; m < 4; m++) F.q[m][lo] = C.sa[b0][m] * F.q[m][lo + sD] + C.sb[b0][m] * F.q[m][lo + 2 * sD];
+            for (int m = 0; m < 4; m++) a[m] = C.sa[b0][m] * a[m] + C.sb[b0][m] * b[m];
+            store4(F, lo, a);
             break;
         case 6:  // plane copy from the opposite boundary plane (not done for i)
             if (D > 0) {
-#pragma unroll
-                for (int m = 0; m < 4; m++) F.q[m][lo] = F.q[m][hi];
+                load4(F, hi, a);
+                store4(F, lo, a);
             }
             break;
         case 7:  // periodic wrap: the image of the opposite interior plane
-#pragma unroll
-            for (int m = 0; m < 4; m++) F.q[m][lo] = F.q[m][hi - sD];
+            load4(F, hi - sD, a);
+            store4(F, lo, a);
             break;
         case 4: {  // inflow: pressure extrapolated on the grid ratio (bcond_fm.F90:95-102)
             const double r = D < 2 ? F.rat[b0][t] : 0.0;
@@ -285,18 +345,21 @@ __device__ __forceinline__ void bc_fm_body(const Level& L, const Fields& F, cons
     }
     switch (C.btype[b1]) {
         case 1: case 2: case 3:
+            load4(F, hi - sD, a);
+            load4(F, hi - 2 * sD, b);
 #pragma unroll
-            for (int m = 0; m < 4; m++) F.q[m][hi] = C.sa[b1][m] * F.q[m][hi - sD] + C.sb[b1][m] * F.q[m][hi - 2 * sD];
+            for (int m = 0; m < 4; m++) a[m] = C.sa[b1][m] * a[m] + C.sb[b1][m] * b[m];
+            store4(F, hi, a);
             break;
         case 6:
             if (D > 0) {
-#pragma unroll
-                for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][lo];
+                load4(F, lo, a);
+                store4(F, hi, a);
             }
             break;
         case 7:
-#pragma unroll
-            for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][lo + sD];
+            load4(F, lo + sD, a);
+            store4(F, hi, a);
             break;
         case 4: {
             const double r = D < 2 ? F.rat[b1][t] : 0.0;
@@ -305,8 +368,8 @@ __device__ __forceinline__ void bc_fm_body(const Level& L, const Fields& F, cons
         }
         case 8:
             if (D == 0) {
-#pragma unroll
-                for (int m = 0; m < 4; m++) F.q[m][hi] = F.q[m][hi - sD];
+                load4(F, hi - sD, a);
+                store4(F, hi, a);
             }
             break;
         default: break;
@@ -332,11 +395,17 @@ __device__ __forceinline__ void resid_part_body(const Level& L, const Fields& F,
         const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
         const long g = L.off + t;
         const bool in = !(i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1);
+        // all ten loads before the first store
+        double qv[5], qo[5];
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            const double qv = F.q[m][g];
-            if (in) acc[m] += fabs(qv - F.qold[m][g]);
-            if (save) F.qold[m][g] = qv;
+            qv[m] = F.q[m][g];
+            qo[m] = F.qold[m][g];
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            if (in) acc[m] += fabs(qv[m] - qo[m]);
+            if (save) F.qold[m][g] = qv[m];
         }
     }
 #pragma unroll

@@ -39,16 +39,34 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
     const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
+    const int ni = L.ni, nj = L.nj;
+    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+    const int pos[3] = {i, j, k}, nd[3] = {ni, nj, L.nk};
+    const long st[3] = {1, (long)ni, (long)ni * nj};
+    // every load is issued before the first store; the pressure neighbours
+    // without a branch (a node on a boundary plane of direction D reads
+    // itself there and stores no fv of D)
     const double c1 = F.csi[0][g], c2 = F.csi[1][g], c3 = F.csi[2][g];
     const double e1 = F.eta[0][g], e2 = F.eta[1][g], e3 = F.eta[2][g];
     const double z1 = F.zet[0][g], z2 = F.zet[1][g], z3 = F.zet[2][g];
     const double aj = F.aj[g];
     const double u = F.q[1][g], v = F.q[2][g], w = F.q[3][g];
+    const double* p = F.q[0];
+    const double p0 = p[g];
+    double pm[3] = {0.0, 0.0, 0.0}, pp[3] = {0.0, 0.0, 0.0}, xnut = 0.0;
+    if (with_dtau) {
+#pragma unroll
+        for (int D = 0; D < 3; D++) {
+            pm[D] = p[g - (pos[D] >= 1 ? st[D] : 0)];
+            pp[D] = p[g + (pos[D] <= nd[D] - 2 ? st[D] : 0)];
+        }
+        if (!C.dtau_met) xnut = F.xnut[g];
+    }
     const double tmp = 1.0 / aj;
     F.ucn[0][t] = tmp * (c1 * u + c2 * v + c3 * w);
     F.ucn[1][t] = tmp * (e1 * u + e2 * v + e3 * w);
     F.ucn[2][t] = tmp * (z1 * u + z2 * v + z3 * w);
-    F.pj[t] = F.q[0][g] / aj;
+    F.pj[t] = p0 / aj;
     if (!with_dtau) return;
     const double g1 = c1 * c1 + c2 * c2 + c3 * c3;
     const double g2 = e1 * e1 + e2 * e2 + e3 * e3;
@@ -63,7 +81,7 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
         dtv2 = vr / (L.de * L.de) / g2;
         dtv3 = vr / (L.dz * L.dz) / g3;
     } else {
-        const double ret = C.rei + F.xnut[g];
+        const double ret = C.rei + xnut;
         dtv1 = C.vnn1 / ret / (L.dc * L.dc) / g1;
         dtv2 = C.vnn1 / ret / (L.de * L.de) / g2;
         dtv3 = C.vnn1 / ret / (L.dz * L.dz) / g3;
@@ -75,18 +93,13 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
     d = fmin(d, dtv3);
     F.dtau[g] = d;
     // fv = |grad xi_D|^2 dtau / J * h^2 (p(+1) - 2 p + p(-1))
-    const int ni = L.ni, nj = L.nj;
-    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
-    const double* p = F.q[0];
     const double gg[3] = {g1, g2, g3};
     const double hh[3] = {L.dc, L.de, L.dz};
-    const int pos[3] = {i, j, k}, nd[3] = {ni, nj, L.nk};
-    const long st[3] = {1, (long)ni, (long)ni * nj};
 #pragma unroll
     for (int D = 0; D < 3; D++) {
         if (pos[D] < 1 || pos[D] > nd[D] - 2) continue;
         const double tmp1 = gg[D] * d / aj;
-        const double tmp2 = (hh[D] * hh[D]) * (p[g + st[D]] - 2.0 * p[g] + p[g - st[D]]);
+        const double tmp2 = (hh[D] * hh[D]) * (pp[D] - 2.0 * p0 + pm[D]);
         F.fv[D][t] = tmp1 * tmp2;
     }
 }
@@ -106,18 +119,21 @@ __device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, 
     const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
     const double ucon = 0.5 * ucn<D>(F, t);
     const double up = ucon + fabs(ucon), um = ucon - fabs(ucon);
-    // first interior node is pos 1, last is len - 2: first-order there
+    // first interior node is pos 1, last is len - 2: first-order there.
+    // Both forms are evaluated and one is selected, so that every load is
+    // issued up front; where the first-order form applies the +-2 loads read
+    // the +-1 neighbours (the +-2 ones may lie outside the level)
     const bool first_order = !quick || pos == 1 || pos == len - 2;
+    const long s2 = first_order ? sD : 2 * sD;
 #pragma unroll
     for (int m = 0; m < 3; m++) {
         const double* u = F.q[m + 1];
-        if (first_order) {
-            out[m] = h * (up * (u[g] - u[g - sD]) + um * (u[g + sD] - u[g]));
-        } else {
-            out[m] = 0.125 * h *
-                     (up * (3.0 * u[g + sD] + 3.0 * u[g] - 7.0 * u[g - sD] + u[g - 2 * sD]) +
-                      um * (-3.0 * u[g - sD] - 3.0 * u[g] + 7.0 * u[g + sD] - u[g + 2 * sD]));
-        }
+        const double um2 = u[g - s2], um1 = u[g - sD], u0 = u[g], up1 = u[g + sD], up2 = u[g + s2];
+        const double fo = h * (up * (u0 - um1) + um * (up1 - u0));
+        const double qk = 0.125 * h *
+                          (up * (3.0 * up1 + 3.0 * u0 - 7.0 * um1 + um2) +
+                           um * (-3.0 * um1 - 3.0 * u0 + 7.0 * up1 - up2));
+        out[m] = first_order ? fo : qk;
     }
 }
 
@@ -183,15 +199,16 @@ __device__ __forceinline__ void residual_body(const Level& L, const Fields& F, c
         const double* fv0 = F.fv[0];
         const double* fv1 = F.fv[1];
         const double* fv2 = F.fv[2];
+        // (computed offsets rather than branches, so the loads issue together)
         const double f0i = fv0[t];
-        const double fmi = (i == 1) ? f0i : fv0[t - nb.sI];
-        const double fpi = (i == ni - 2) ? f0i : fv0[t + nb.sI];
+        const double fmi = fv0[t - (i == 1 ? 0 : nb.sI)];
+        const double fpi = fv0[t + (i == ni - 2 ? 0 : nb.sI)];
         const double f0j = fv1[t];
-        const double fmj = (j == 1) ? f0j : fv1[t - nb.sJ];
-        const double fpj = (j == nj - 2) ? f0j : fv1[t + nb.sJ];
+        const double fmj = fv1[t - (j == 1 ? 0 : nb.sJ)];
+        const double fpj = fv1[t + (j == nj - 2 ? 0 : nb.sJ)];
         const double f0k = fv2[t];
-        const double fmk = (k == 1) ? f0k : fv2[t - nb.sK];
-        const double fpk = (k == nk - 2) ? f0k : fv2[t + nb.sK];
+        const double fmk = fv2[t - (k == 1 ? 0 : nb.sK)];
+        const double fpk = fv2[t + (k == nk - 2 ? 0 : nb.sK)];
         diss1 = C.pdiss * ((fpi - 2.0 * f0i + fmi) + (fpj - 2.0 * f0j + fmj) + (fpk - 2.0 * f0k + fmk));
         F.diss1[g] = diss1;
     } else {
@@ -402,6 +419,12 @@ __device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, co
         double ro[3] = {0.0, 0.0, 0.0}, rh_[3] = {0.0, 0.0, 0.0};
         const bool more = k + 1 < ke;
         if (more) load_uvw(k + 2, ro, rh_);
+        // this plane's rh as well, so that its loads overlap the face fluxes
+        const long g = col + sK * k;
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        if (out)
+#pragma unroll
+            for (int m = 0; m < 4; m++) r[m] = F.rh[m][g];
         const int slm = (k + 2) % 3, sl0 = k % 3, slp = (k + 1) % 3;
         auto uk = [&](int m, int a, int b, int c) {
             return s_u[m][c < 0 ? slm : c == 0 ? sl0 : slp][own + b * kLX + a];
@@ -422,7 +445,6 @@ __device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, co
         if (out) visc_face_t<2>(L, C, uk, mcur, mnext, fkp);
         __syncthreads();
         if (out) {
-            const long g = col + sK * k;
             double visc[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) visc[c] = L.dc * (s_fi[c][ty][tx] - s_fi[c][ty][tx - 1]);
@@ -430,19 +452,19 @@ __device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, co
             for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.de * (s_fj[c][ty][tx] - s_fj[c][ty - 1][tx]);
 #pragma unroll
             for (int c = 0; c < 3; c++) visc[c] = visc[c] + L.dz * (fkp[c] - fk[c]);
-            double r[4];
-            r[0] = F.rh[0][g];
 #pragma unroll
-            for (int m = 1; m < 4; m++) r[m] = F.rh[m][g] - visc[m - 1];
+            for (int m = 1; m < 4; m++) r[m] = r[m] - visc[m - 1];
             if (n > 1) {
+                double pk[4];  // all four loaded before the first store
+#pragma unroll
+                for (int m = 0; m < 4; m++) pk[m] = F.pk[m][g];
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
-                    double pk = F.pk[m][g];
                     if (calc_pk == 1) {
-                        pk = pk - r[m];
-                        F.pk[m][g] = pk;
+                        pk[m] = pk[m] - r[m];
+                        F.pk[m][g] = pk[m];
                     }
-                    r[m] = r[m] + pk;
+                    r[m] = r[m] + pk[m];
                 }
             }
 #pragma unroll
@@ -476,11 +498,11 @@ __global__ void __launch_bounds__(256) k_residual_z(ZoneGrid G, int n, int recal
     const ZoneDesc& Z = G.z[b.z];
     residual_body(Z.lv[n], Z.F, Z.C, n, recalc, b.bid, b.nblk);
 }
-__global__ void __launch_bounds__(kTX * kTY) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk, int kc) {
+__global__ void __launch_bounds__(kTX * kTY, 2) k_visc_pk(Level L, Fields F, Consts C, int n, int calc_pk, int kc) {
     visc_pk_body(L, F, C, n, calc_pk, kc, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
                  gridDim.x * gridDim.y * gridDim.z);
 }
-__global__ void __launch_bounds__(kTX * kTY) k_visc_pk_z(ZoneGrid G, int n, int calc_pk, int kc) {
+__global__ void __launch_bounds__(kTX * kTY, 2) k_visc_pk_z(ZoneGrid G, int n, int calc_pk, int kc) {
     const ZoneBlock b = zone_block(G);
     const ZoneDesc& Z = G.z[b.z];
     visc_pk_body(Z.lv[n], Z.F, Z.C, n, calc_pk, kc, b.bid, b.nblk);

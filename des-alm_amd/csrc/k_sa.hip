@@ -32,30 +32,39 @@ __device__ __forceinline__ void sa_prep_body(const Level& L, const Fields& F, un
     const int ni = L.ni, nj = L.nj, nk = L.nk;
     const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
     const long g = L.off + t;
-    const double u = F.q[1][g], v = F.q[2][g], w = F.q[3][g];
+    const bool bnd = i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1;
+    // every load is issued before the first store: the neighbours of a
+    // boundary node are replaced by the node itself (its wo is zero anyway)
+    const long sI = bnd ? 0 : 1, sJ = bnd ? 0 : ni, sK = bnd ? 0 : (long)ni * nj;
+    const double* U = F.q[1];
+    const double* V = F.q[2];
+    const double* W = F.q[3];
+    const double u = U[g], v = V[g], w = W[g];
+    const double c1 = F.csi[0][g], c2 = F.csi[1][g], c3 = F.csi[2][g];
+    const double e1 = F.eta[0][g], e2 = F.eta[1][g], e3 = F.eta[2][g];
+    const double z1 = F.zet[0][g], z2 = F.zet[1][g], z3 = F.zet[2][g];
     const double tmp = 1.0 / F.aj[g];
-    F.ucn[0][t] = tmp * (F.csi[0][g] * u + F.csi[1][g] * v + F.csi[2][g] * w);
-    F.ucn[1][t] = tmp * (F.eta[0][g] * u + F.eta[1][g] * v + F.eta[2][g] * w);
-    F.ucn[2][t] = tmp * (F.zet[0][g] * u + F.zet[1][g] * v + F.zet[2][g] * w);
-    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) {
+    const double uip = U[g + sI], uim = U[g - sI], vip = V[g + sI], vim = V[g - sI], wip = W[g + sI], wim = W[g - sI];
+    const double ujp = U[g + sJ], ujm = U[g - sJ], vjp = V[g + sJ], vjm = V[g - sJ], wjp = W[g + sJ], wjm = W[g - sJ];
+    const double ukp = U[g + sK], ukm = U[g - sK], vkp = V[g + sK], vkm = V[g - sK], wkp = W[g + sK], wkm = W[g - sK];
+    F.ucn[0][t] = tmp * (c1 * u + c2 * v + c3 * w);
+    F.ucn[1][t] = tmp * (e1 * u + e2 * v + e3 * w);
+    F.ucn[2][t] = tmp * (z1 * u + z2 * v + z3 * w);
+    if (bnd) {
         F.wo[t] = 0.0;
         return;
     }
     // rhs_sa_source.F90:44-93 (interior only)
-    const long sI = 1, sJ = ni, sK = (long)ni * nj;
     const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
-    const double* U = F.q[1];
-    const double* V = F.q[2];
-    const double* W = F.q[3];
-    const double uc = dc2 * (U[g + sI] - U[g - sI]), vc = dc2 * (V[g + sI] - V[g - sI]), wc = dc2 * (W[g + sI] - W[g - sI]);
-    const double ue = de2 * (U[g + sJ] - U[g - sJ]), ve = de2 * (V[g + sJ] - V[g - sJ]), we = de2 * (W[g + sJ] - W[g - sJ]);
-    const double uz = dz2 * (U[g + sK] - U[g - sK]), vz = dz2 * (V[g + sK] - V[g - sK]), wz = dz2 * (W[g + sK] - W[g - sK]);
-    const double u2 = uc * F.csi[1][g] + ue * F.eta[1][g] + uz * F.zet[1][g];
-    const double u3 = uc * F.csi[2][g] + ue * F.eta[2][g] + uz * F.zet[2][g];
-    const double v1 = vc * F.csi[0][g] + ve * F.eta[0][g] + vz * F.zet[0][g];
-    const double v3 = vc * F.csi[2][g] + ve * F.eta[2][g] + vz * F.zet[2][g];
-    const double w1 = wc * F.csi[0][g] + we * F.eta[0][g] + wz * F.zet[0][g];
-    const double w2 = wc * F.csi[1][g] + we * F.eta[1][g] + wz * F.zet[1][g];
+    const double uc = dc2 * (uip - uim), vc = dc2 * (vip - vim), wc = dc2 * (wip - wim);
+    const double ue = de2 * (ujp - ujm), ve = de2 * (vjp - vjm), we = de2 * (wjp - wjm);
+    const double uz = dz2 * (ukp - ukm), vz = dz2 * (vkp - vkm), wz = dz2 * (wkp - wkm);
+    const double u2 = uc * c2 + ue * e2 + uz * z2;
+    const double u3 = uc * c3 + ue * e3 + uz * z3;
+    const double v1 = vc * c1 + ve * e1 + vz * z1;
+    const double v3 = vc * c3 + ve * e3 + vz * z3;
+    const double w1 = wc * c1 + we * e1 + wz * z1;
+    const double w2 = wc * c2 + we * e2 + wz * z2;
     const double w12 = 0.5 * (u2 - v1), w13 = 0.5 * (u3 - w1), w23 = 0.5 * (v3 - w2);
     F.wo[t] = 2.0 * (w12 * w12 + w13 * w13 + w23 * w23);
 }
@@ -264,6 +273,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             return s_q[c < 0 ? slm : c == 0 ? sl0 : slp][own + b * kQLX + a];
         };
         const long t = col + sK * k;  // level-local node
+
         if (face_i) {
             const QMet m1 = qmet_at(own + 1);
             s_fv[0][ty][tx] = sa_visc_face_t<0>(L, C, q, mcur, m1);
@@ -304,11 +314,14 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
         }
         __syncthreads();
         if (out) {
-            const long g = o + t;
+            // every point input loaded before the first use, so that the
+            // loads are in flight together
+            const unsigned g = (unsigned)(o + t);
+            const double wdv = ld32(F.wd, g), wov = ld32(F.wo, (unsigned)t);
+            const double qn5 = ld32(F.qn[4], g), qnm15 = ld32(F.qnm1[4], g), dtev = ld32(F.dtev, g);
             const double aj = mcur.v[9];
             // production, destruction, cb2 diffusion (rhs_sa_prod.F90:44-100)
             double psa;
-            const double wdv = ld32(F.wd, (unsigned)(g));
             if (wdv == 0.0) {
                 psa = 0.0;
             } else {
@@ -319,7 +332,7 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
                 const double xf = 1.0 + qdiv(chi, cv2);
                 const double fv2 = qdiv(1.0, xf * xf * xf);
                 const double fv3 = qdiv((1.0 + chi * fv1) * (1.0 - fv2), chi);
-                const double vtcm = sqrt(2.0 * ld32(F.wo, (unsigned)(t)));
+                const double vtcm = sqrt(2.0 * wov);
                 const double cawd = capa * capa * wdv * wdv;
                 const double vtc_org = fv3 * vtcm + qdiv(qv * fv2, cawd);
                 const double vtc = fmax(vtc_org, 0.00001);
@@ -340,9 +353,8 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             r = r + s_fc[1][ty][tx] - s_fc[1][ty - 1][tx];
             r = r + fkc1 - fkc;
             const double dtjb = 2.0 * C.delti * aj;
-            const double dq5dt = C.e_source * qdiv(3.0 * q(0, 0, 0) - 4.0 * ld32(F.qn[4], (unsigned)(g)) + ld32(F.qnm1[4], (unsigned)(g)), dtjb);
+            const double dq5dt = C.e_source * qdiv(3.0 * q(0, 0, 0) - 4.0 * qn5 + qnm15, dtjb);
             r = r + dq5dt - visc - qdiv(psa, aj);
-            const double dtev = ld32(F.dtev, (unsigned)(g));
             st32(F.rh5, (unsigned)(t), -dtev * aj * r);
             st32(F.dtj, (unsigned)(t), 0.5 * dtev * aj);
         }
@@ -477,40 +489,56 @@ __device__ __forceinline__ void sa_sweep_body(const Level& L, const Fields& F, c
     const double* __restrict__ dtj = F.dtj;
     double* __restrict__ rh = F.rh5;
     double a[R], b[R], c[R], d[R][1];
+    // every row is loaded without a branch, so that the loads of all rows
+    // can be in flight together: rows past the line's end (and every row of
+    // an idle thread) read the line's last row and are discarded
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        a[r] = 0.0; b[r] = 1.0; c[r] = 0.0; d[r][0] = 0.0;
-        if (r < nr) {
-            const long tl = base + (long)(s0 + r + 1) * sD;
-            const double avm = av[tl - sD], avp = av[tl], dt = dtj[tl];
-            a[r] = -dt * (U[tl - sD] + avm);
-            b[r] = 1.0 + dt * (avm + avp);
-            c[r] = dt * (U[tl + sD] - avp);
-            d[r][0] = rh[tl];
-        }
+        const long tl = base + (long)(min(s0 + r, n - 1) + 1) * sD;
+        const double avm = av[tl - sD], avp = av[tl], dt = dtj[tl];
+        const double ar = -dt * (U[tl - sD] + avm);
+        const double br = 1.0 + dt * (avm + avp);
+        const double cr = dt * (U[tl + sD] - avp);
+        const double dr = rh[tl];
+        const bool in = r < nr;
+        a[r] = in ? ar : 0.0;
+        b[r] = in ? br : 1.0;
+        c[r] = in ? cr : 0.0;
+        d[r][0] = in ? dr : 0.0;
     }
-    double a_next = 0.0;
-    if (nr > 0 && s0 + nr < n) {
-        const long tl = base + (long)(s0 + nr + 1) * sD;
-        a_next = -dtj[tl] * (U[tl - sD] + av[tl - sD]);
+    double a_next;
+    {
+        const long tl = base + (long)(min(s0 + nr, n - 1) + 1) * sD;
+        const double an = -dtj[tl] * (U[tl - sD] + av[tl - sD]);
+        a_next = (nr > 0 && s0 + nr < n) ? an : 0.0;
     }
     unsigned piv = 0;
     int zrow = 0;
     seg_solve<R, 1, double>(a, b, c, d, a_next, nr, n, s0, smem_sa, P, TI, sh.nlines, line0, li, ps, piv, zrow);
+    if (D < 2) {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        if (r < nr) {
-            const long tl = base + (long)(s0 + r + 1) * sD;
-            if (D < 2) {
-                rh[tl] = d[r][0];
-            } else {
-                const long g = L.off + tl;
-                const double qv = F.q[4][g];
-                const double qe5 = qv + d[r][0];
-                const double dum = 1.0 + 1.5 * C.e_source * F.dtev[g] / C.delti;
-                F.q[4][g] = qv + (-qv + qe5) / dum;
-            }
+        for (int r = 0; r < R; r++)
+            if (r < nr) rh[base + (long)(s0 + r + 1) * sD] = d[r][0];
+    } else {
+        // nu~ and dtev of every row are loaded, and every new nu~ formed,
+        // before the first store: the loads are in flight together and the
+        // stores wait for nothing
+        double qv[R], dv[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const long g = L.off + base + (long)(min(s0 + r, n - 1) + 1) * sD;
+            qv[r] = F.q[4][g];
+            dv[r] = F.dtev[g];
         }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double qe5 = qv[r] + d[r][0];
+            const double dum = 1.0 + 1.5 * C.e_source * dv[r] / C.delti;
+            qv[r] = qv[r] + (-qv[r] + qe5) / dum;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (r < nr) F.q[4][L.off + base + (long)(s0 + r + 1) * sD] = qv[r];
     }
     if (piv) atomicAdd(F.pivots, (unsigned long long)piv);
     if (zrow) atomicMin(F.zero_pivot, (unsigned)zrow);
