@@ -80,6 +80,11 @@ struct desalm_ctx {
     Consts C;
     Fields F;
     std::vector<void*> allocs;
+    // the fields' allocation (dalloc)
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    long arena_n = 0;
+    bool sizing = false;
     double* staging = nullptr;     // 5 * nall doubles: AoS transfers
     double* dq_buf = nullptr;      // 4 * n1 doubles: dq of the solver_daf parity entry
     std::string err;
@@ -141,13 +146,34 @@ int fail(desalm_ctx* c, const std::string& msg) {
             return fail((c), std::string(#x " failed: ") + hipGetErrorString(e_));        \
     } while (0)
 
+// Device fields are carved from one allocation: a k-line sweep touches a
+// dozen arrays at rows about 200 KB apart, and separate allocations cost it
+// an address translation per array and row (a sixth of its accesses missed
+// the L1 translation cache).  Consecutive arrays start a further 4 KiB +
+// 256 B into the allocation (16 steps, then again from zero), so that the
+// same node of different arrays falls on different L2 channels.  With
+// c->sizing set, dalloc only adds up the bytes the arena needs.
+constexpr size_t kStaggerWords = 544;  // 4352 B
+
 double* dalloc(desalm_ctx* c, long n) {
+    const size_t words = (size_t)(n > 0 ? n : 1);
+    const size_t pad = (size_t)(c->arena_n++ % 16) * kStaggerWords;
+    const size_t bytes = ((pad + words) * sizeof(double) + 255) & ~(size_t)255;
+    if (c->sizing) {
+        c->arena_used += bytes;
+        return nullptr;
+    }
+    c->dev_bytes += (long long)bytes;
+    if (c->arena && c->arena_used + bytes <= c->arena_cap) {
+        double* q = reinterpret_cast<double*>(c->arena + c->arena_used) + pad;
+        c->arena_used += bytes;
+        return q;
+    }
     void* p = nullptr;
-    if (hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
     c->allocs.push_back(p);
-    c->dev_bytes += (long long)(n > 0 ? n : 1) * (long long)sizeof(double);
-    if (hipMemset(p, 0, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
-    return (double*)p;
+    if (hipMemset(p, 0, bytes) != hipSuccess) return nullptr;
+    return (double*)p + pad;
 }
 
 int validate(const desalm_params* p, std::string& why) {
@@ -387,39 +413,61 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
     bool ok = true;
     auto A = [&](long n) {
         double* q = dalloc(c, n);
-        if (!q) ok = false;
+        if (!q && !c->sizing) ok = false;
         return q;
     };
-    for (int m = 0; m < 3; m++) { F.csi[m] = A(c->nall); F.eta[m] = A(c->nall); F.zet[m] = A(c->nall); F.x[m] = A(c->nall); }
-    F.aj = A(c->nall);
-    for (int m = 0; m < 5; m++) { F.q[m] = A(c->nall); F.qn[m] = A(c->nall); F.qnm1[m] = A(c->nall); F.qold[m] = A(c->nall); }
-    for (int m = 0; m < 4; m++) { F.rh[m] = A(c->nall); F.pk[m] = A(c->nall); }
-    F.xnut = A(c->nall); F.wd = A(c->nall); F.dtev = A(c->nall);
-    F.dtau = A(c->nall); F.diss1 = A(c->nall);
-    for (int m = 0; m < 4; m++) F.cp[m] = A(c->n1);
-    for (int m = 0; m < 4; m++) F.dp[m] = A(c->n1);
-    for (int m = 0; m < 3; m++) F.av[m] = A(c->n1);
-    F.dtj = A(c->n1);
-    for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
-    F.wo = A(c->n1); F.rh5 = A(c->n1); F.pj = A(c->n1);
-    for (int m = 0; m < 3; m++) F.fv[m] = A(c->n1);
-    if (c->C.mixed) {
-        // FP32 DAF scratch: 13 floats per fine node, carved from doubles
-        auto Af = [&](long n) { return (float*)A((n + 1) / 2); };
-        for (int m = 0; m < 4; m++) { F.dpf[m] = Af(c->n1); F.cpf[m] = Af(c->n1); }
+    auto fields = [&] {
+        for (int m = 0; m < 3; m++) { F.csi[m] = A(c->nall); F.eta[m] = A(c->nall); F.zet[m] = A(c->nall); F.x[m] = A(c->nall); }
+        F.aj = A(c->nall);
+        for (int m = 0; m < 5; m++) { F.q[m] = A(c->nall); F.qn[m] = A(c->nall); F.qnm1[m] = A(c->nall); F.qold[m] = A(c->nall); }
+        for (int m = 0; m < 4; m++) { F.rh[m] = A(c->nall); F.pk[m] = A(c->nall); }
+        F.xnut = A(c->nall); F.wd = A(c->nall); F.dtev = A(c->nall);
+        F.dtau = A(c->nall); F.diss1 = A(c->nall);
+        for (int m = 0; m < 4; m++) F.cp[m] = A(c->n1);
+        for (int m = 0; m < 4; m++) F.dp[m] = A(c->n1);
+        for (int m = 0; m < 3; m++) F.av[m] = A(c->n1);
+        F.dtj = A(c->n1);
+        for (int m = 0; m < 3; m++) F.ucn[m] = A(c->n1);
+        F.wo = A(c->n1); F.rh5 = A(c->n1); F.pj = A(c->n1);
+        for (int m = 0; m < 3; m++) F.fv[m] = A(c->n1);
+        if (c->C.mixed) {
+            // FP32 DAF scratch: 13 floats per fine node, carved from doubles
+            auto Af = [&](long n) { return (float*)A((n + 1) / 2); };
+            for (int m = 0; m < 4; m++) { F.dpf[m] = Af(c->n1); F.cpf[m] = Af(c->n1); }
+        }
+        F.resid_part = A(5L * kResidBlocks);
+        // the five L1 sums, then the first zero or non-finite pivot of the line
+        // solves (one word, read back with the sums)
+        F.resid = A(6);
+        F.pfix = A(1);
+        for (int b = 0; b < 4; b++) {
+            const Level& L1 = c->lv[1];
+            F.rat[b] = (p->btype[b] == DESALM_BC_INFLOW || p->btype[b] == DESALM_BC_OUTLET_MOC)
+                           ? A(b < 2 ? (long)L1.nj * L1.nk : (long)L1.ni * L1.nk)
+                           : nullptr;
+        }
+        c->staging = A(5 * c->nall);
+        c->dq_buf = A(4 * c->n1);
+    };
+    // first the arena's size, then the arrays carved from it
+    c->sizing = true;
+    fields();
+    c->sizing = false;
+    {
+        void* pa = nullptr;
+        if (hipMalloc(&pa, c->arena_used) != hipSuccess || hipMemset(pa, 0, c->arena_used) != hipSuccess) {
+            if (pa) (void)hipFree(pa);
+            desalm_finalize(c);
+            return fail(nullptr, "device allocation failed");
+        }
+        c->allocs.push_back(pa);
+        c->arena = (char*)pa;
+        c->arena_cap = c->arena_used;
+        c->arena_used = 0;
+        c->arena_n = 0;
     }
-    F.resid_part = A(5L * kResidBlocks);
-    // the five L1 sums, then the first zero or non-finite pivot of the line
-    // solves (one word, read back with the sums)
-    F.resid = A(6);
-    F.zero_pivot = (unsigned*)(F.resid + 5);
-    F.pfix = A(1);
-    for (int b = 0; b < 4; b++) {
-        const Level& L1 = c->lv[1];
-        F.rat[b] = (p->btype[b] == DESALM_BC_INFLOW || p->btype[b] == DESALM_BC_OUTLET_MOC)
-                       ? A(b < 2 ? (long)L1.nj * L1.nk : (long)L1.ni * L1.nk)
-                       : nullptr;
-    }
+    fields();
+    if (ok) F.zero_pivot = (unsigned*)(F.resid + 5);
     {
         void* pv = nullptr;
         if (hipMalloc(&pv, sizeof(unsigned long long)) != hipSuccess) ok = false;
@@ -429,8 +477,6 @@ int desalm_init(const desalm_params* p, int device, desalm_ctx** out) {
             if (hipMemset(pv, 0, sizeof(unsigned long long)) != hipSuccess) ok = false;
         }
     }
-    c->staging = A(5 * c->nall);
-    c->dq_buf = A(4 * c->n1);
     if (!ok) {
         desalm_finalize(c);
         return fail(nullptr, "device allocation failed");

@@ -666,8 +666,9 @@ __host__ __device__ inline SweepShape sweep_lines(const Level& L, int D, int ext
 __host__ __device__ inline void sweep_tiles(int D, int len, int R, int& P, int& TI) {
     P = (len + R - 1) / R;
     if (P < 4) P = 4;
-    // j and k lines: 16 adjacent i lines per workgroup (128-byte rows)
-    TI = D == 0 ? (256 / P > 1 ? 256 / P : 1) : 16;
+    // j and k lines: 32 adjacent i lines per workgroup (256-byte rows, so
+    // that a wave's row loads span two segments instead of four)
+    TI = D == 0 ? (256 / P > 1 ? 256 / P : 1) : 32;
     while (TI * P > 512 && TI > 1) TI /= 2;
 }
 
