@@ -470,9 +470,9 @@ __device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, co
 #pragma unroll
             for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
         }
-        __syncthreads();
         // plane k+1 becomes current: its metrics into LDS, plane k+2's
-        // velocities into the slot of plane k-1
+        // velocities into the slot of plane k-1 (no barrier needed before:
+        // the part above reads only the face fluxes)
         store_met(mnext, mhn);
         if (more) store_uvw(k + 2, ro, rh_);
         mcur = mnext;

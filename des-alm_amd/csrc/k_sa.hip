@@ -358,7 +358,9 @@ __device__ __forceinline__ void sa_rhs_body(const Level& L, const Fields& F, con
             st32(F.rh5, (unsigned)(t), -dtev * aj * r);
             st32(F.dtj, (unsigned)(t), 0.5 * dtev * aj);
         }
-        __syncthreads();
+        // plane k+1 into LDS: no barrier needed before it, as the part above
+        // reads only the face fluxes and nu~ of plane k (slot k mod 3), and
+        // the next plane's nu~ goes to the slot of plane k-1
 #pragma unroll
         for (int c = 0; c < kQM; c++) {
             s_m[c][own] = mnext.v[c];
