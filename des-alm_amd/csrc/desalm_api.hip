@@ -294,10 +294,12 @@ void bcond(desalm_ctx* c, hipStream_t s) {
     if (!c->alm.t.empty()) launch_alm_hub(s, c->lv[1], c->F, c->alm);
 }
 
-void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out) {
+// ucn_kept: des_eddy has just stored U/J of the current velocities (fine
+// level, SA/DES on; it changes only nu~ and xnut)
+void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out, bool ucn_kept = false) {
     const Level& L = c->lv[n];
     hipStream_t s = c->stream;
-    timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C); });
+    timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C, ucn_kept); });
     timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk, c->kc_visc[n]); });
     // characteristic-outlet planes of the fine level (solver_daf.F90:322-365)
     if (n == 1 && (c->C.btype[0] == DESALM_BC_OUTLET_MOC || c->C.btype[1] == DESALM_BC_OUTLET_MOC))
@@ -685,11 +687,12 @@ static bool needs_coordinates(const desalm_ctx* c) {
 // solver_daf of level n for every zone of a group, one launch per kernel
 // (the zones' own outlet rows, actuator-line forcing and hub zeroing per
 // zone); the launch order within each zone is solver_daf's
-static void solver_daf_zones(std::vector<desalm_ctx*>& zs, const ZoneList& Z, int n, int recalc, int calc_pk) {
+static void solver_daf_zones(std::vector<desalm_ctx*>& zs, const ZoneList& Z, int n, int recalc, int calc_pk,
+                             bool ucn_kept) {
     desalm_ctx* c0 = zs[0];
     hipStream_t s = c0->stream;
     const int kc = c0->kc_visc[n];
-    timed(c0, KID_DTAU, n, [&] { launch_dtau_z(s, Z, n); });
+    timed(c0, KID_DTAU, n, [&] { launch_dtau_z(s, Z, n, ucn_kept); });
     timed(c0, KID_RESIDUAL, n, [&] { launch_residual_z(s, Z, n, 0, calc_pk, kc); });
     for (desalm_ctx* c : zs) {
         if (n == 1 && (c->C.btype[0] == DESALM_BC_OUTLET_MOC || c->C.btype[1] == DESALM_BC_OUTLET_MOC))
@@ -776,10 +779,11 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
                                   [&] { launch_des_eddy(s, c->lv[1], c->F, c->C, c->p.itk, c->kc_sa); });
                     if (have_table) timed(c0, KID_OVERSET, 1, [&] { exchange(1); });
                 }
+                const bool ucn_kept = p.des && n == 1;
                 if (zl)
-                    solver_daf_zones(zs, *zl, n, decide_recalc_rh, itr);
+                    solver_daf_zones(zs, *zl, n, decide_recalc_rh, itr, ucn_kept);
                 else
-                    for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr);
+                    for (desalm_ctx* c : zs) solver_daf(c, n, decide_recalc_rh, itr, nullptr, ucn_kept);
                 // wall functions on the j walls of zone 1 (mg_driver.F90:218-224)
                 if (n == 1 && (c0->hooks & DESALM_HOOK_WALL_FUNCTIONS))
                     timed(c0, KID_BCOND, 1, [&] { launch_wall_functions(s, c0->lv[1], c0->F, c0->p.ren); });

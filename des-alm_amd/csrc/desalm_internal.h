@@ -248,7 +248,9 @@ struct ZoneList {
 
 // ---- launchers (each enqueues on `s`) ----
 // residual assembly, k_residual.hip
-void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
+// ucn_kept: U/J of the current velocities is already in F.ucn (stored by the
+// SA update of the same pseudo-iteration), so only dtau, p/J and fv are formed
+void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, bool ucn_kept);
 // kc: k planes per workgroup of the LDS-tiled kernels (0: default)
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n,
                      int recalc, int calc_pk, int kc);
@@ -306,7 +308,7 @@ void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A
 void launch_rfg_inflow(hipStream_t s, const Level& L, const Fields& F, const Rfg& R, double t);
 void launch_alm_hub(hipStream_t s, const Level& L, const Fields& F, const Alm& A);
 // every zone of a group in one launch per kernel (the table's level n)
-void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n);
+void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n, bool ucn_kept);
 void launch_residual_z(hipStream_t s, const ZoneList& Z, int n, int recalc, int calc_pk, int kc);
 void launch_daf_z(hipStream_t s, const ZoneList& Z, int n, int D);
 void launch_des_eddy_z(hipStream_t s, const ZoneList& Z, int itk, int kc);

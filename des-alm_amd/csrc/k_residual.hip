@@ -28,14 +28,17 @@ __device__ __forceinline__ double sqr(double a) { return a * a; }
 
 // Per-node inputs of the residual, every node of the level:
 // rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83), or met_daf_dtau
-// (src/met_daf_dtau.F90:53-93) when C.dtau_met, when with_dtau, and always
-// U/J (rhs_contra_j.F90:14-29) into F.ucn and p/J
+// (src/met_daf_dtau.F90:53-93) when C.dtau_met, when mode & kPrepDtau, and
+// U/J (rhs_contra_j.F90:14-29) into F.ucn unless mode & kPrepUcnKept (the SA
+// update has just stored U/J of the same velocities) and always p/J
 // (rhs_flux_sans_convec.F90:60) into F.pj (level-local).  With dtau
 // also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
 // nodes interior in that direction, into F.fv (level-local): the residual
 // then reads it at a node and its two neighbours.
-__device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const Consts& C, int with_dtau,
-                                          unsigned bid, unsigned nblk) {
+constexpr int kPrepDtau = 1, kPrepUcnKept = 2;
+__device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const Consts& C, int mode, unsigned bid,
+                                          unsigned nblk) {
+    const bool with_dtau = mode & kPrepDtau, with_ucn = !(mode & kPrepUcnKept);
     const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
@@ -50,7 +53,12 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
     const double e1 = F.eta[0][g], e2 = F.eta[1][g], e3 = F.eta[2][g];
     const double z1 = F.zet[0][g], z2 = F.zet[1][g], z3 = F.zet[2][g];
     const double aj = F.aj[g];
-    const double u = F.q[1][g], v = F.q[2][g], w = F.q[3][g];
+    double u = 0.0, v = 0.0, w = 0.0;
+    if (with_ucn) {
+        u = F.q[1][g];
+        v = F.q[2][g];
+        w = F.q[3][g];
+    }
     const double* p = F.q[0];
     const double p0 = p[g];
     double pm[3] = {0.0, 0.0, 0.0}, pp[3] = {0.0, 0.0, 0.0}, xnut = 0.0;
@@ -62,10 +70,12 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
         }
         if (!C.dtau_met) xnut = F.xnut[g];
     }
-    const double tmp = 1.0 / aj;
-    F.ucn[0][t] = tmp * (c1 * u + c2 * v + c3 * w);
-    F.ucn[1][t] = tmp * (e1 * u + e2 * v + e3 * w);
-    F.ucn[2][t] = tmp * (z1 * u + z2 * v + z3 * w);
+    if (with_ucn) {
+        const double tmp = 1.0 / aj;
+        F.ucn[0][t] = tmp * (c1 * u + c2 * v + c3 * w);
+        F.ucn[1][t] = tmp * (e1 * u + e2 * v + e3 * w);
+        F.ucn[2][t] = tmp * (z1 * u + z2 * v + z3 * w);
+    }
     F.pj[t] = p0 / aj;
     if (!with_dtau) return;
     const double g1 = c1 * c1 + c2 * c2 + c3 * c3;
@@ -482,13 +492,13 @@ __device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, co
     }
 }
 
-__global__ void __launch_bounds__(256) k_prep(Level L, Fields F, Consts C, int with_dtau) {
-    prep_body(L, F, C, with_dtau, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) k_prep(Level L, Fields F, Consts C, int mode) {
+    prep_body(L, F, C, mode, blockIdx.x, gridDim.x);
 }
-__global__ void __launch_bounds__(256) k_prep_z(ZoneGrid G, int n, int with_dtau) {
+__global__ void __launch_bounds__(256) k_prep_z(ZoneGrid G, int n, int mode) {
     const ZoneBlock b = zone_block(G);
     const ZoneDesc& Z = G.z[b.z];
-    prep_body(Z.lv[n], Z.F, Z.C, with_dtau, b.bid, b.nblk);
+    prep_body(Z.lv[n], Z.F, Z.C, mode, b.bid, b.nblk);
 }
 __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc) {
     residual_body(L, F, C, n, recalc, blockIdx.x, gridDim.x);
@@ -510,17 +520,18 @@ __global__ void __launch_bounds__(kTX * kTY, 2) k_visc_pk_z(ZoneGrid G, int n, i
 
 }  // namespace
 
-void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C) {
+void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, bool ucn_kept) {
     const int b = 256;
-    hipLaunchKernelGGL(k_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C, 1);
+    hipLaunchKernelGGL(k_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C,
+                       kPrepDtau | (ucn_kept ? kPrepUcnKept : 0));
 }
 
 static unsigned node_blocks(const Level& L) { return (unsigned)((L.n + 255) / 256); }
 
-void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n) {
+void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n, bool ucn_kept) {
     unsigned nb = 0;
     const ZoneGrid G = Z.grid([&](const ZoneDesc& z) { return node_blocks(z.lv[n]); }, nb);
-    hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, 1);
+    hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, kPrepDtau | (ucn_kept ? kPrepUcnKept : 0));
 }
 
 void launch_residual_z(hipStream_t s, const ZoneList& Z, int n, int recalc, int calc_pk, int kc) {
