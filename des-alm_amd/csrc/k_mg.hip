@@ -209,8 +209,33 @@ __device__ __forceinline__ void rs_sweep_body(const Level& L, const Fields& F, c
         a[r] = -epn;
         b[r] = 1.0 + 2.0 * epn;
         c[r] = -epn;
+    }
+    if (D == 0) {
+        // i lines: the R rows are one run of consecutive nodes, loaded as
+        // such (16-byte loads; lanes are R rows apart).  A partial last
+        // segment loads the line's last R rows and shifts them into place.
+        const int pb = max(0, min(s0, n - R));
+        const int shift = s0 - pb;
+        double w[R][4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) d[r][m] = r < nr ? F.dp[m][base + (long)(s0 + r + 1) * sD] : 0.0;
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) w[r][m] = F.dp[m][base + pb + r + 1];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                double v = w[r][m];
+#pragma unroll
+                for (int t = 1; t < R; t++)
+                    if (shift == t) v = r + t < R ? w[r + t < R ? r + t : R - 1][m] : 0.0;
+                d[r][m] = r < nr ? v : 0.0;
+            }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) d[r][m] = r < nr ? F.dp[m][base + (long)(s0 + r + 1) * sD] : 0.0;
     }
     unsigned piv = 0;
     int zrow = 0;
@@ -238,6 +263,13 @@ __device__ __forceinline__ void rs_sweep_body(const Level& L, const Fields& F, c
                 for (int m = 0; m < 4; m++) F.q[m][g] = qv[r][m];
             }
         }
+    } else if (D == 0 && nr == R) {
+        // a whole segment of an i line: one run of consecutive nodes per
+        // component
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int r = 0; r < R; r++) F.dp[m][base + s0 + r + 1] = d[r][m];
     } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
