@@ -93,12 +93,17 @@ __device__ __forceinline__ void mul_minv(double m1, double m2, double m3, double
 
 // S (-dtau J rh) at global node g (rhs_diag_solver.F90:56-72): beta on the
 // pressure row, 1 / alpha on the velocity rows
-__device__ __forceinline__ void scaled_rh(const Fields& F, const Consts& C, long g, const NodeDt& nd, double v[4]) {
-    const double f = -nd.dt * F.aj[g];
+__device__ __forceinline__ void scaled_rh_vals(const Consts& C, double aj, const double rh[4], const NodeDt& nd,
+                                               double v[4]) {
+    const double f = -nd.dt * aj;
     const double ia = 1.0 / nd.alpha;
-    v[0] = C.beta * (f * F.rh[0][g]);
+    v[0] = C.beta * (f * rh[0]);
 #pragma unroll
-    for (int m = 1; m < 4; m++) v[m] = ia * (f * F.rh[m][g]);
+    for (int m = 1; m < 4; m++) v[m] = ia * (f * rh[m]);
+}
+__device__ __forceinline__ void scaled_rh(const Fields& F, const Consts& C, long g, const NodeDt& nd, double v[4]) {
+    const double rh[4] = {F.rh[0][g], F.rh[1][g], F.rh[2][g], F.rh[3][g]};
+    scaled_rh_vals(C, F.aj[g], rh, nd, v);
 }
 
 // the metric row of direction D at node g
@@ -128,6 +133,22 @@ __device__ __forceinline__ double transform_row(const Fields& F, const Consts& C
     const double sg = sqrt(gg);
     mul_minv(m1, m2, m3, gg, sg, nd.isigma, v);
     return nd.dt * (nd.theta * sg);
+}
+
+// the i sweep's transform (S then Ma^-1) and dtau spr from values already
+// loaded: dtau, J, rh and the csi row of one node
+__device__ __forceinline__ double transform_vals0(const Consts& C, double dt, double aj, const double rh[4], double m1,
+                                                  double m2, double m3, double v[4]) {
+    const NodeDt nd = node_dt(C, dt);
+    scaled_rh_vals(C, aj, rh, nd, v);
+    const double gg = m1 * m1 + m2 * m2 + m3 * m3;
+    const double sg = sqrt(gg);
+    mul_minv(m1, m2, m3, gg, sg, nd.isigma, v);
+    return nd.dt * (nd.theta * sg);
+}
+__device__ __forceinline__ double sd_vals(const Consts& C, double dt, double m1, double m2, double m3) {
+    const NodeDt nd = node_dt(C, dt);
+    return nd.dt * (nd.theta * sqrt(m1 * m1 + m2 * m2 + m3 * m3));
 }
 
 // dtau times the spectral radius of direction D alone
@@ -262,47 +283,77 @@ __device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const 
     auto gnode = [&](int p) { return L.off + base + (long)p * sD; };  // global node at position p
 
     // ---- load: right-hand sides (transformed) and dtau spr of rows s0-1 .. s0+nr ----
-    // window entry w is node position s0 + w (row s0 + w - 1).  The i sweep
-    // (R = 4) loads and transforms every row without a branch, so that the
-    // loads of all its rows are in flight together: rows past the line's end
-    // (and every row of an idle thread) read the line's last row and are
-    // discarded.  The j and k sweeps (R = 8) keep one branch per row: all
-    // their rows' loads in flight would not fit the registers.
+    // window entry w is node position s0 + w (row s0 + w - 1)
     S dv[R][4];
     S sdv[R + 2];
+    if (D == 0) {
+        // i lines: a segment's rows and its two window nodes are one run of
+        // consecutive nodes, loaded as such (16-byte loads; lanes are R rows
+        // apart).  The run of a partial last segment (or an idle thread)
+        // reaches past the line's end into the next line or plane of the
+        // level; those values are never used.
+        double dt[R + 2], m1[R + 2], m2[R + 2], m3[R + 2], aj[R], rh[4][R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        sdv[r + 1] = S(0);
+        for (int w = 0; w < R + 2; w++) {
+            const long g = gnode(s0 + w);
+            dt[w] = F.dtau[g];
+            m1[w] = F.csi[0][g];
+            m2[w] = F.csi[1][g];
+            m3[w] = F.csi[2][g];
+        }
 #pragma unroll
-        for (int m = 0; m < 4; m++) dv[r][m] = S(0);
-        if (D == 0 || r < nr) {
-            const int p = min(s0 + r, n - 1) + 1;
-            double v[4] = {0.0, 0.0, 0.0, 0.0};
-            if (D > 0) {
+        for (int r = 0; r < R; r++) {
+            const long g = gnode(s0 + r + 1);
+            aj[r] = F.aj[g];
+#pragma unroll
+            for (int m = 0; m < 4; m++) rh[m][r] = F.rh[m][g];
+        }
+        // sdw[w]: dtau spr at window entry w; rows r < nr transformed
+        double sdw[R + 2];
+#pragma unroll
+        for (int w = 0; w < R + 2; w++) {
+            if (w >= 1 && w <= R) {
+                const double rv[4] = {rh[0][w - 1], rh[1][w - 1], rh[2][w - 1], rh[3][w - 1]};
+                double v[4];
+                sdw[w] = transform_vals0(C, dt[w], aj[w - 1], rv, m1[w], m2[w], m3[w], v);
+                const bool in = w - 1 < nr;
+#pragma unroll
+                for (int m = 0; m < 4; m++) dv[w - 1][m] = in ? (S)v[m] : S(0);
+            } else {
+                sdw[w] = sd_vals(C, dt[w], m1[w], m2[w], m3[w]);
+            }
+        }
+        const int pe = s0 + nr + 1;
+        sdv[0] = nr > 0 ? (S)sdw[0] : S(0);
+#pragma unroll
+        for (int w = 1; w <= R + 1; w++) {
+            // rows, then the node after the segment at entry nr + 1
+            const bool row = w - 1 < nr;
+            const bool after = w == nr + 1 && nr > 0 && pe <= nd - 1;
+            sdv[w] = (row || after) ? (S)sdw[w] : S(0);
+        }
+    } else {
+        // j and k lines: one branch per row (all their rows' loads in flight
+        // would not fit the registers)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            sdv[r + 1] = S(0);
+#pragma unroll
+            for (int m = 0; m < 4; m++) dv[r][m] = S(0);
+            if (r < nr) {
+                const int p = s0 + r + 1;
+                double v[4];
                 const long tl = base + (long)p * sD;
 #pragma unroll
                 for (int m = 0; m < 4; m++) v[m] = (double)io.xin[m][tl];
-            }
-            const double sd = transform_row<D>(F, C, gnode(p), v);
-            const bool in = r < nr;
-            sdv[r + 1] = in ? (S)sd : S(0);
+                sdv[r + 1] = (S)transform_row<D>(F, C, gnode(p), v);
 #pragma unroll
-            for (int m = 0; m < 4; m++) dv[r][m] = in ? (S)v[m] : S(0);
+                for (int m = 0; m < 4; m++) dv[r][m] = (S)v[m];
+            }
         }
-    }
-    {
         const int pe = s0 + nr + 1;
-        S slo = S(0), se = S(0);
-        if (D == 0) {
-            const double a = row_sd<D>(F, C, gnode(min(s0, n - 1)));
-            const double b = row_sd<D>(F, C, gnode(min(pe, nd - 1)));
-            slo = nr > 0 ? (S)a : S(0);
-            se = (nr > 0 && pe <= nd - 1) ? (S)b : S(0);
-        } else {
-            slo = nr > 0 ? (S)row_sd<D>(F, C, gnode(s0)) : S(0);
-            se = (nr > 0 && pe <= nd - 1) ? (S)row_sd<D>(F, C, gnode(pe)) : S(0);
-        }
-        sdv[0] = slo;
+        sdv[0] = nr > 0 ? (S)row_sd<D>(F, C, gnode(s0)) : S(0);
+        const S se = (nr > 0 && pe <= nd - 1) ? (S)row_sd<D>(F, C, gnode(pe)) : S(0);
         sdv[R + 1] = S(0);
 #pragma unroll
         for (int w = 1; w <= R + 1; w++)
@@ -590,12 +641,20 @@ __device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const 
                     }
                 }
             }
-            if (D < 2) {
+            if (D == 1 || (D == 0 && nr < R)) {
                 const long tl = base + (long)(s0 + r + 1) * sD;
 #pragma unroll
                 for (int m = 0; m < 4; m++) io.xout[m][tl] = dv[r][m];
             }
         }
+    }
+    if (D == 0 && nr == R) {
+        // a whole segment of an i line: one run of consecutive nodes per
+        // component
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int r = 0; r < R; r++) io.xout[m][base + s0 + r + 1] = dv[r][m];
     }
     if (D == 2) {
         // Mc and the q update, four rows at a time: the loads of a group are

@@ -491,33 +491,69 @@ __device__ __forceinline__ void sa_sweep_body(const Level& L, const Fields& F, c
     const double* __restrict__ dtj = F.dtj;
     double* __restrict__ rh = F.rh5;
     double a[R], b[R], c[R], d[R][1];
-    // every row is loaded without a branch, so that the loads of all rows
-    // can be in flight together: rows past the line's end (and every row of
-    // an idle thread) read the line's last row and are discarded
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const long tl = base + (long)(min(s0 + r, n - 1) + 1) * sD;
-        const double avm = av[tl - sD], avp = av[tl], dt = dtj[tl];
-        const double ar = -dt * (U[tl - sD] + avm);
-        const double br = 1.0 + dt * (avm + avp);
-        const double cr = dt * (U[tl + sD] - avp);
-        const double dr = rh[tl];
-        const bool in = r < nr;
-        a[r] = in ? ar : 0.0;
-        b[r] = in ? br : 1.0;
-        c[r] = in ? cr : 0.0;
-        d[r][0] = in ? dr : 0.0;
-    }
     double a_next;
-    {
-        const long tl = base + (long)(min(s0 + nr, n - 1) + 1) * sD;
-        const double an = -dtj[tl] * (U[tl - sD] + av[tl - sD]);
-        a_next = (nr > 0 && s0 + nr < n) ? an : 0.0;
+    if (D == 0) {
+        // i lines: the segment's rows and their neighbours are one run of
+        // consecutive nodes per array, loaded as such (16-byte loads; lanes
+        // are R rows apart).  The run of a partial last segment (or an idle
+        // thread) reaches past the line's end into the next line or plane of
+        // the level; those values are never used.  Run entry e is node
+        // position s0 + e (av, U) or s0 + 1 + e (dtj, rh).
+        double avw[R + 1], uw[R + 2], dtw[R + 1], rhw[R];
+#pragma unroll
+        for (int e = 0; e < R + 2; e++) {
+            const long tl = base + s0 + e;
+            uw[e] = U[tl];
+            if (e <= R) avw[e] = av[tl];
+            if (e <= R) dtw[e] = dtj[tl + 1];
+            if (e < R) rhw[e] = rh[tl + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double avm = avw[r], avp = avw[r + 1], dt = dtw[r];
+            const double ar = -dt * (uw[r] + avm);
+            const double br = 1.0 + dt * (avm + avp);
+            const double cr = dt * (uw[r + 2] - avp);
+            const bool in = r < nr;
+            a[r] = in ? ar : 0.0;
+            b[r] = in ? br : 1.0;
+            c[r] = in ? cr : 0.0;
+            d[r][0] = in ? rhw[r] : 0.0;
+        }
+        const double an = -dtw[R] * (uw[R] + avw[R]);
+        a_next = (nr == R && s0 + R < n) ? an : 0.0;
+    } else {
+        // every row is loaded without a branch, so that the loads of all rows
+        // can be in flight together: rows past the line's end (and every row of
+        // an idle thread) read the line's last row and are discarded
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const long tl = base + (long)(min(s0 + r, n - 1) + 1) * sD;
+            const double avm = av[tl - sD], avp = av[tl], dt = dtj[tl];
+            const double ar = -dt * (U[tl - sD] + avm);
+            const double br = 1.0 + dt * (avm + avp);
+            const double cr = dt * (U[tl + sD] - avp);
+            const double dr = rh[tl];
+            const bool in = r < nr;
+            a[r] = in ? ar : 0.0;
+            b[r] = in ? br : 1.0;
+            c[r] = in ? cr : 0.0;
+            d[r][0] = in ? dr : 0.0;
+        }
+        {
+            const long tl = base + (long)(min(s0 + nr, n - 1) + 1) * sD;
+            const double an = -dtj[tl] * (U[tl - sD] + av[tl - sD]);
+            a_next = (nr > 0 && s0 + nr < n) ? an : 0.0;
+        }
     }
     unsigned piv = 0;
     int zrow = 0;
     seg_solve<R, 1, double>(a, b, c, d, a_next, nr, n, s0, smem_sa, P, TI, sh.nlines, line0, li, ps, piv, zrow);
-    if (D < 2) {
+    if (D == 0 && nr == R) {
+        // a whole segment of an i line: one run of consecutive nodes
+#pragma unroll
+        for (int r = 0; r < R; r++) rh[base + s0 + r + 1] = d[r][0];
+    } else if (D < 2) {
 #pragma unroll
         for (int r = 0; r < R; r++)
             if (r < nr) rh[base + (long)(s0 + r + 1) * sD] = d[r][0];
