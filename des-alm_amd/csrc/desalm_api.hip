@@ -288,9 +288,10 @@ void prof_flush(desalm_ctx* c) {
 }
 
 // bcond_fm on the fine level (src/bcond_fm.F90:2-362), with the
-// actuator-line hub zeroing at its end (:328-355)
-void bcond(desalm_ctx* c, hipStream_t s) {
-    launch_bcond_fm(s, c->lv[1], c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix);
+// actuator-line hub zeroing at its end (:328-355); pfix_deferred leaves the
+// pinned-pressure shift to the caller's next pass over the level
+void bcond(desalm_ctx* c, hipStream_t s, bool pfix_deferred = false) {
+    launch_bcond_fm(s, c->lv[1], c->F, c->C, c->p.ifix, c->p.jfix, c->p.kfix, pfix_deferred);
     if (!c->alm.t.empty()) launch_alm_hub(s, c->lv[1], c->F, c->alm);
 }
 
@@ -309,9 +310,11 @@ void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out, b
     timed(c, KID_DAF_I, n, [&] { launch_daf_i(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_J, n, [&] { launch_daf_j(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_K, n, [&] { launch_daf_k(s, L, c->F, c->C, n, dq_out); });
-    if (n == 1) timed(c, KID_BCOND, n, [&] { bcond(c, s); });
+    // on the fine level the recalc's first pass applies the pressure shift
+    const bool defer = n == 1 && recalc == 1;
+    if (n == 1) timed(c, KID_BCOND, n, [&] { bcond(c, s, defer); });
     if (recalc == 1)
-        timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0, c->kc_visc[n]); });
+        timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0, c->kc_visc[n], defer); });
 }
 
 int copy_in_level(desalm_ctx* c, double* const* soa, int ncomp, const double* host, const Level& L) {
@@ -805,9 +808,10 @@ static int vcycles(std::vector<desalm_ctx*>& zs, bool have_table, int apply_mom,
                         launch_prolong(s, c->lv[n], c->lv[n + 1], c->F, c->C, c->p.icrs, c->p.jcrs, c->p.kcrs);
                     });
                 }
-                timed(c, KID_BCOND, 1, [&] { bcond(c, s); });
-                // update norms, then qold_mg <- q for the next pseudo-iteration
-                timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1); });
+                timed(c, KID_BCOND, 1, [&] { bcond(c, s, true); });
+                // the pressure shift, the update norms, then qold_mg <- q for
+                // the next pseudo-iteration
+                timed(c, KID_RESID, 1, [&] { launch_resid_and_save(s, c->lv[1], c->F, 1, true); });
             }
         }
         return decide_recalc_rh;

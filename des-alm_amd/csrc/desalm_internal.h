@@ -252,8 +252,10 @@ struct ZoneList {
 // SA update of the same pseudo-iteration), so only dtau, p/J and fv are formed
 void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, bool ucn_kept);
 // kc: k planes per workgroup of the LDS-tiled kernels (0: default)
+// pfix (recalc only): first subtract the pinned pressure F.pfix from p at
+// every node, the last step of a bcond_fm launched with pfix_deferred
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n,
-                     int recalc, int calc_pk, int kc);
+                     int recalc, int calc_pk, int kc, bool pfix = false);
 void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int calc_pk, int kc);
 int visc_kc_default(const Level& L);
 // DAF implicit operator, k_daf.hip
@@ -266,8 +268,11 @@ void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const 
 // right-hand side of the characteristic-outlet planes (btype 5 on i faces,
 // fine level): mg_brhs + nonreflect_ibc (k_outlet.hip)
 void launch_outlet_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
+// pfix_deferred: read the pinned pressure into F.pfix but leave its
+// subtraction to the next pass over the level (launch_residual or
+// launch_resid_and_save with pfix), which then reads p first
 void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix,
-                     int jfix, int kfix);
+                     int jfix, int kfix, bool pfix_deferred = false);
 // SA / DES, k_sa.hip
 void launch_des_eddy(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int itk, int kc);
 void launch_sa_prep(hipStream_t s, const Level& L, const Fields& F);
@@ -281,7 +286,7 @@ void launch_nlevel(hipStream_t s, const Level& Lc, const Level& Lf, const Fields
 void launch_zero_pk_boundary(hipStream_t s, const Level& L, const Fields& F);
 void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F,
                     const Consts& C, int icrs, int jcrs, int kcrs);
-void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save_only);
+void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save_only, bool pfix = false);
 void launch_copy_qold(hipStream_t s, const Level& L, const Fields& F);
 // physical-step outputs: the L2 sums of q - qn over the fine interior into
 // F.resid (src/main.F90:214-300), and q(1:5), xnut at fine-level nodes

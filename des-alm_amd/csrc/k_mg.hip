@@ -418,11 +418,14 @@ __device__ __forceinline__ void pfix_sub_body(const Level& L, const Fields& F, u
 }
 
 // ---- mg_residual: deterministic two-stage L1 sums ----
-__device__ __forceinline__ void resid_part_body(const Level& L, const Fields& F, int save, unsigned bid,
+// pfix: first subtract the pinned pressure F.pfix from p at every node (the
+// last step of bcond_fm, deferred to this pass when it follows directly)
+__device__ __forceinline__ void resid_part_body(const Level& L, const Fields& F, int save, int pfix, unsigned bid,
                                                 unsigned nb) {
     __shared__ double sh[5][256];
     double acc[5] = {0, 0, 0, 0, 0};
     const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const double pf = pfix ? *F.pfix : 0.0;
     for (long t = (long)bid * blockDim.x + threadIdx.x; t < L.n; t += (long)nb * blockDim.x) {
         const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
         const long g = L.off + t;
@@ -433,6 +436,10 @@ __device__ __forceinline__ void resid_part_body(const Level& L, const Fields& F,
         for (int m = 0; m < 5; m++) {
             qv[m] = F.q[m][g];
             qo[m] = F.qold[m][g];
+        }
+        if (pfix) {
+            qv[0] = qv[0] - pf;
+            F.q[0][g] = qv[0];
         }
 #pragma unroll
         for (int m = 0; m < 5; m++) {
@@ -551,12 +558,12 @@ __global__ void __launch_bounds__(256) k_pfix_sub_z(ZoneGrid G) {
     const ZoneBlock b = zone_block(G);
     pfix_sub_body(G.z[b.z].lv[1], G.z[b.z].F, b.bid);
 }
-__global__ void __launch_bounds__(256) k_resid_part(Level L, Fields F, int save) {
-    resid_part_body(L, F, save, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) k_resid_part(Level L, Fields F, int save, int pfix) {
+    resid_part_body(L, F, save, pfix, blockIdx.x, gridDim.x);
 }
 __global__ void __launch_bounds__(256) k_resid_part_z(ZoneGrid G, int save) {
     const ZoneBlock b = zone_block(G);
-    resid_part_body(G.z[b.z].lv[1], G.z[b.z].F, save, b.bid, b.nblk);
+    resid_part_body(G.z[b.z].lv[1], G.z[b.z].F, save, 0, b.bid, b.nblk);
 }
 // one workgroup per zone
 __global__ void __launch_bounds__(256) k_resid_final_z(const ZoneDesc* __restrict__ z, int nblocks) {
@@ -622,13 +629,14 @@ void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const 
     }
 }
 
-void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix, int jfix, int kfix) {
+void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix, int jfix, int kfix,
+                     bool pfix_deferred) {
     hipLaunchKernelGGL(k_bc_fm<0>, dim3(nblk((long)L.nj * L.nk, 256)), dim3(256), 0, s, L, F, C);
     hipLaunchKernelGGL(k_bc_fm<1>, dim3(nblk((long)L.ni * L.nk, 256)), dim3(256), 0, s, L, F, C);
     hipLaunchKernelGGL(k_bc_fm<2>, dim3(nblk((long)L.ni * L.nj, 256)), dim3(256), 0, s, L, F, C);
     const long g = L.off + (ifix - 1) + (long)L.ni * ((jfix - 1) + (long)L.nj * (kfix - 1));
     hipLaunchKernelGGL(k_pfix_read, dim3(1), dim3(1), 0, s, F, g);
-    hipLaunchKernelGGL(k_pfix_sub, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
+    if (!pfix_deferred) hipLaunchKernelGGL(k_pfix_sub, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
 }
 
 // Partial sums of (q - qn)^2 over the fine interior (real_residual,
@@ -684,8 +692,8 @@ void launch_probe(hipStream_t s, const Fields& F, int n, const long* node, doubl
     if (n > 0) hipLaunchKernelGGL(k_probe, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, F, n, node, out);
 }
 
-void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save) {
-    hipLaunchKernelGGL(k_resid_part, dim3(kResidBlocks), dim3(256), 0, s, L, F, save);
+void launch_resid_and_save(hipStream_t s, const Level& L, const Fields& F, int save, bool pfix) {
+    hipLaunchKernelGGL(k_resid_part, dim3(kResidBlocks), dim3(256), 0, s, L, F, save, pfix ? 1 : 0);
     hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, s, F, kResidBlocks);
 }
 

@@ -35,7 +35,7 @@ __device__ __forceinline__ double sqr(double a) { return a * a; }
 // also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
 // nodes interior in that direction, into F.fv (level-local): the residual
 // then reads it at a node and its two neighbours.
-constexpr int kPrepDtau = 1, kPrepUcnKept = 2;
+constexpr int kPrepDtau = 1, kPrepUcnKept = 2, kPrepPfix = 4;
 __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const Consts& C, int mode, unsigned bid,
                                           unsigned nblk) {
     const bool with_dtau = mode & kPrepDtau, with_ucn = !(mode & kPrepUcnKept);
@@ -60,7 +60,12 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
         w = F.q[3][g];
     }
     const double* p = F.q[0];
-    const double p0 = p[g];
+    double p0 = p[g];
+    if (mode & kPrepPfix) {
+        // the pinned-pressure shift of bcond_fm, deferred to this pass
+        p0 = p0 - *F.pfix;
+        F.q[0][g] = p0;
+    }
     double pm[3] = {0.0, 0.0, 0.0}, pp[3] = {0.0, 0.0, 0.0}, xnut = 0.0;
     if (with_dtau) {
 #pragma unroll
@@ -566,11 +571,11 @@ void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts
 }
 
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int recalc,
-                     int calc_pk, int kc) {
+                     int calc_pk, int kc, bool pfix) {
     const int b = 256;
     const dim3 grid((unsigned)((L.n + b - 1) / b));
     // the recalc branch needs U/J and p/J of the updated q (dtau is kept)
-    if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, 0);
+    if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, pfix ? kPrepPfix : 0);
     hipLaunchKernelGGL(k_residual, grid, dim3(b), 0, s, L, F, C, n, recalc);
     launch_visc_pk(s, L, F, C, n, recalc ? 0 : calc_pk, kc);
 }
