@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 WORDS = {
     "dtau": 9 + 1 + 1,                        # metrics, xnut -> dtau
     "residual": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4 + 1,   # q,metrics,aj,xnut,qn,qnm1,dtau -> rh,diss1 (fine)
-    "residual_recalc": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4,  # ... diss1 -> rh
+    "residual_recalc": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4 + 1,  # ... diss1 -> rh, p (pinned-pressure shift)
     "daf_i": 4 + 1 + 1 + 3 + 4,               # rh,dtau,aj,csi -> rh
     "daf_j": 4 + 1 + 3 + 3 + 4,               # rh,dtau,csi,eta -> rh
     "daf_k": 4 + 1 + 3 + 3 + 4 + 4,           # rh,dtau,eta,zet,q -> q
@@ -46,7 +46,7 @@ WORDS = {
     # sweeps 11 words each) + vorticity source once (13 words); itk applied below
     "des_eddy": None,
     "prolong": 33,                            # coarse q, qold, q, dq of the three smoothing sweeps
-    "resid_norm": 15,                         # q, qold -> qold (L1 norms)
+    "resid_norm": 16,                         # q, qold -> qold (L1 norms), p (pinned-pressure shift)
 }
 
 
@@ -68,7 +68,7 @@ FAMILY_OF = [("k_sa_", "des_eddy"), ("k_daf_sweep<0", "daf_i"), ("k_daf_sweep<1"
              ("k_resid_", "resid_norm"), ("k_bc_fm", "bcond_fm"), ("k_pfix_", "bcond_fm")]
 # one dispatch per family call
 FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_sweep<0", "daf_j": "k_daf_sweep<1", "daf_k": "k_daf_sweep<2",
-               "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_pfix_sub",
+               "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_pfix_read",
                "dtau": "k_prep", "residual": "k_residual", "residual_recalc": "k_residual"}
 
 # warm-start samples per recorded training step and the sample buffer: every
