@@ -659,7 +659,7 @@ __device__ __forceinline__ void daf_body(const Level& L, const Fields& F, const 
     if (D == 2) {
         // Mc and the q update, four rows at a time: the loads of a group are
         // all in flight, and every value formed, before its first store (rows
-        // past the segment's end read its last row and are not stored)
+        // past the line's end read its last row and are not stored)
 #pragma unroll
         for (int r0 = 0; r0 < R; r0 += 4) {
             double v[4][4], qv[4][4];
