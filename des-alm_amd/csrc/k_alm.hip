@@ -4,8 +4,8 @@
 //   k_alm_elements  one wave per (blade, element): velocity at the
 //                   element, angle of attack, polar lookup, lift and drag ->
 //                   the element's position and force (forces_lines)
-//   k_alm_spread    one thread per spreading node: Gaussian sum over all
-//                   elements of its rotor, added to rh(2:4) / J
+//   k_alm_spread    per spreading node: Gaussian sum over all elements of
+//                   its rotor, added to rh(2:4) / J
 //   k_alm_hub       one thread per disk node: u = 0 on hub nodes,
 //                   i = itur1..itur2
 // The tables hold fine-level node indices.  rhs_lines runs on every level
@@ -21,40 +21,71 @@ namespace {
 constexpr double kPi = 3.141592653589793;
 constexpr double kRho = 1178.62;  // rhs_lines.F90:86
 
-// Cl, Cd from a polar (rhs_lines.F90:232-265).  The reference walks the
+// Cl, Cd from polars (rhs_lines.F90:232-265).  The reference walks the
 // pairs (a1, a2) = (angle(k), angle(k+1)) from the first and stops at the
 // first pair with a1 < aoa <= a2, or at the first a1 >= aoa.  Either way the
 // walk ends at b, the first index with aoa <= angle(b): b = 0 gives the first
 // values, 0 < b < n the linear interpolation on (b-1, b), no such b the last
-// values.  The 64 lanes of the wave find b together, 64 angles per step
-// (every lane calls this with the same polar and aoa).
-__device__ void polar_lookup(const AlmPolars& P, int p, double aoa, double& cl, double& cd) {
-    const int o = P.off[p], n = P.cnt[p];
-    const double* a = P.angle + o;
-    const double* l = P.cl + o;
-    const double* d = P.cd + o;
+// values.  The 64 lanes of the wave find b together, 256 angles of each polar
+// per step with all their loads issued first (every lane calls this with the
+// same polars and aoa); the two polars of an element are searched at once.
+constexpr int kPolarStep = 256;
+
+__device__ void polar_lookup2(const AlmPolars& P, const int (&p)[2], double aoa, double (&cl)[2], double (&cd)[2]) {
     const int lane = (int)(threadIdx.x % 64);
-    int b = n;
-    for (int k0 = 0; k0 < n; k0 += 64) {
-        const int k = k0 + lane;
-        const unsigned long long hit = __ballot(k < n && aoa <= a[k]);
-        if (hit) {
-            b = k0 + __ffsll((long long)hit) - 1;
-            break;
-        }
+    int o[2], n[2], b[2];
+    bool found[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        o[s] = P.off[p[s]];
+        n[s] = P.cnt[p[s]];
+        b[s] = n[s];
+        found[s] = false;
     }
-    if (b == 0) {
-        cl = l[0];
-        cd = d[0];
-    } else if (b < n) {
-        const double a1 = a[b - 1], a2 = a[b];
-        double m = (l[b] - l[b - 1]) / (a2 - a1);
-        cl = m * (aoa - a1) + l[b - 1];
-        m = (d[b] - d[b - 1]) / (a2 - a1);
-        cd = m * (aoa - a1) + d[b - 1];
-    } else {
-        cl = l[n - 1];
-        cd = d[n - 1];
+    for (int k0 = 0; k0 < max(n[0], n[1]) && !(found[0] && found[1]); k0 += kPolarStep) {
+        double v[2][kPolarStep / 64];
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int u = 0; u < kPolarStep / 64; u++) {
+                const int k = k0 + 64 * u + lane;
+                v[s][u] = !found[s] && k < n[s] ? P.angle[o[s] + k] : 0.0;
+            }
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int u = 0; u < kPolarStep / 64; u++) {
+                const int k = k0 + 64 * u + lane;
+                const unsigned long long hit = __ballot(!found[s] && k < n[s] && aoa <= v[s][u]);
+                if (hit) {
+                    b[s] = k0 + 64 * u + __ffsll((long long)hit) - 1;
+                    found[s] = true;
+                }
+            }
+    }
+    // the operands of all three cases, then the case of b
+    double a1[2], a2[2], l1[2], l2[2], d1[2], d2[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int hi = min(max(b[s], 0), n[s] - 1), lo = max(b[s] - 1, 0);
+        a1[s] = P.angle[o[s] + lo]; a2[s] = P.angle[o[s] + hi];
+        l1[s] = P.cl[o[s] + lo]; l2[s] = P.cl[o[s] + hi];
+        d1[s] = P.cd[o[s] + lo]; d2[s] = P.cd[o[s] + hi];
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (b[s] == 0) {
+            cl[s] = l2[s];
+            cd[s] = d2[s];
+        } else if (b[s] < n[s]) {
+            double m = (l2[s] - l1[s]) / (a2[s] - a1[s]);
+            cl[s] = m * (aoa - a1[s]) + l1[s];
+            m = (d2[s] - d1[s]) / (a2[s] - a1[s]);
+            cd[s] = m * (aoa - a1[s]) + d1[s];
+        } else {
+            cl[s] = l2[s];
+            cd[s] = d2[s];
+        }
     }
 }
 
@@ -107,9 +138,9 @@ __global__ void k_alm_elements(Level L, Fields F, AlmTurbine T, AlmPolars P, dou
     const double u_theta = omega * radius + u_y * sin(angle_aux) - u_z * cos(angle_aux);
     const double phi = atan2(u_x, u_theta);
     const double aoa = phi * 180.0 / kPi - twist;
-    double cl_b, cd_b, cl_a, cd_a;
-    polar_lookup(P, T.pol_b[jk], aoa, cl_b, cd_b);
-    polar_lookup(P, T.pol_a[jk], aoa, cl_a, cd_a);
+    double clv[2], cdv[2];
+    polar_lookup2(P, {T.pol_b[jk], T.pol_a[jk]}, aoa, clv, cdv);
+    const double cl_b = clv[0], cd_b = cdv[0], cl_a = clv[1], cd_a = cdv[1];
     const double cl = (cl_a - cl_b) * inter + cl_b;
     const double cd = (cd_a - cd_b) * inter + cd_b;
     const double urel = sqrt(u_theta * u_theta + u_x * u_x);
@@ -124,38 +155,67 @@ __global__ void k_alm_elements(Level L, Fields F, AlmTurbine T, AlmPolars P, dou
     f[5] = fL * cos(angle_aux) * sin(phi) - fD * cos(angle_aux) * cos(phi);
 }
 
-// element forces staged in LDS per workgroup (up to kAlmStage elements; the
-// rest are read from global memory)
-constexpr int kAlmStage = 256;
+// Spreading: a workgroup of kSpreadWaves waves owns 64 spreading nodes (one
+// per lane) and walks the elements in chunks of 64.  Per chunk the chunk's
+// element positions and forces go to LDS, wave w evaluates the Gaussian
+// kernel of every element e = w (mod kSpreadWaves) of the chunk at its lane's
+// node into LDS, and after a barrier wave 0 adds the chunk's terms to its
+// node's sums in element order, as the reference's element loop does.
+constexpr int kSpreadWaves = 4;
+constexpr int kSpreadChunk = 64;
 
-__global__ void __launch_bounds__(128) k_alm_spread(Level L, Fields F, AlmTurbine T, double length_scale,
-                                                    double ren) {
-    __shared__ double s_f[kAlmStage * 6];
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_alm_spread(Level L, Fields F, AlmTurbine T,
+                                                                  double length_scale, double ren) {
+    __shared__ double s_f[kSpreadChunk * 6];
+    __shared__ double s_k[kSpreadChunk][65];  // [element of the chunk][node]; odd stride: no bank conflicts
     const int ne = T.nblades * T.nelems;
-    const int ns = ne < kAlmStage ? ne : kAlmStage;
-    for (int e = (int)threadIdx.x; e < 6 * ns; e += (int)blockDim.x) s_f[e] = T.forces[e];
-    __syncthreads();
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (c >= T.n_ker) return;
-    const int ik = T.ker_nodes[3 * c], jk = T.ker_nodes[3 * c + 1], kk = T.ker_nodes[3 * c + 2];
-    if (!(ik >= 2 && ik <= L.ni - 1 && jk >= 2 && jk <= L.nj - 1 && kk >= 2 && kk <= L.nk - 1)) return;
+    const int lane = (int)(threadIdx.x % 64), w = (int)(threadIdx.x / 64);
+    const int c = (int)blockIdx.x * 64 + lane;
+    bool ok = c < T.n_ker;
+    int ik = 0, jk = 0, kk = 0;
+    if (ok) {
+        ik = T.ker_nodes[3 * c];
+        jk = T.ker_nodes[3 * c + 1];
+        kk = T.ker_nodes[3 * c + 2];
+        ok = ik >= 2 && ik <= L.ni - 1 && jk >= 2 && jk <= L.nj - 1 && kk >= 2 && kk <= L.nk - 1;
+    }
     const double Ls = length_scale;
-    const double radius = T.ker_dat[3 * c] * Ls;
-    if (!(radius > T.hub_rad)) return;
-    const double dV = T.ker_dat[3 * c + 2] * (Ls * Ls * Ls);
-    const double xk = T.ker_pos[3 * c], yk = T.ker_pos[3 * c + 1], zk = T.ker_pos[3 * c + 2];
+    double dV = 0.0, xk = 0.0, yk = 0.0, zk = 0.0;
+    if (ok) {
+        const double radius = T.ker_dat[3 * c] * Ls;
+        ok = radius > T.hub_rad;
+        dV = T.ker_dat[3 * c + 2] * (Ls * Ls * Ls);
+        xk = T.ker_pos[3 * c];
+        yk = T.ker_pos[3 * c + 1];
+        zk = T.ker_pos[3 * c + 2];
+    }
     const double deltar = fabs(T.tur_rad - T.hub_rad) / T.nelems;
     const double eps = 2 * pow(dV, 1.0 / 3.0);
     const double kscale = 1 / ((eps * eps * eps) * pow(kPi, 3.0 / 2.0));
     double fxa = 0.0, fya = 0.0, fza = 0.0;
-    for (int e = 0; e < ne; e++) {
-        const double* f = e < ns ? s_f + 6 * e : T.forces + 6L * e;
-        const double rdis = sqrt((xk - f[0]) * (xk - f[0]) + (yk - f[1]) * (yk - f[1]) + (zk - f[2]) * (zk - f[2])) * Ls;
-        const double ker = kscale * exp(-(rdis * rdis) / (eps * eps));
-        fxa = fxa + ker * f[3] * deltar;
-        fya = fya + ker * f[4] * deltar;
-        fza = fza + ker * f[5] * deltar;
+    for (int e0 = 0; e0 < ne; e0 += kSpreadChunk) {
+        const int n = min(kSpreadChunk, ne - e0);
+        for (int v = (int)threadIdx.x; v < 6 * n; v += (int)blockDim.x) s_f[v] = T.forces[6L * e0 + v];
+        __syncthreads();
+        if (ok)
+            for (int e = w; e < n; e += kSpreadWaves) {
+                const double* f = s_f + 6 * e;
+                const double rdis =
+                    sqrt((xk - f[0]) * (xk - f[0]) + (yk - f[1]) * (yk - f[1]) + (zk - f[2]) * (zk - f[2])) * Ls;
+                s_k[e][lane] = kscale * exp(-(rdis * rdis) / (eps * eps));
+            }
+        __syncthreads();
+        if (ok && w == 0)
+            for (int e = 0; e < n; e++) {
+                const double* f = s_f + 6 * e;
+                const double ker = s_k[e][lane];
+                fxa = fxa + ker * f[3] * deltar;
+                fya = fya + ker * f[4] * deltar;
+                fza = fza + ker * f[5] * deltar;
+            }
+        __syncthreads();
     }
+    if (!ok || w != 0) return;
     const double vel_scale = ren * 10E-7 / Ls;
     const double s = Ls / (kRho * (vel_scale * vel_scale));
     const long g = L.off + (ik - 1) + (long)L.ni * ((jk - 1) + (long)L.nj * (kk - 1));
@@ -184,8 +244,8 @@ void launch_alm_rhs(hipStream_t s, const Level& L, const Fields& F, const Alm& A
         hipLaunchKernelGGL(k_alm_elements, dim3(ne), dim3(64), 0, s, L, F, T, A.pol, A.length_scale, A.vel_infty,
                            ren, A.jmg, A.kmg);
         if (T.n_ker > 0)
-            hipLaunchKernelGGL(k_alm_spread, dim3((T.n_ker + 127) / 128), dim3(128), 0, s, L, F, T, A.length_scale,
-                               ren);
+            hipLaunchKernelGGL(k_alm_spread, dim3((T.n_ker + 63) / 64), dim3(64 * kSpreadWaves), 0, s, L, F, T,
+                               A.length_scale, ren);
     }
 }
 

@@ -5,9 +5,9 @@
 //   k_nlevel        mg_nlevel: injection of q, xnut, qold_mg and the 27-point
 //                   restriction of rh into pk (src/mg_nlevel.F90:2-150),
 //                   boundary pk zeroed (mg_zero_pk_boundary)
-//   k_prolong_corr  mg_prolong's coarse correction q - qold_mg
-//   k_prolong_fill  its k/j/i linear fills onto the fine level
-//                   (src/mg_prolong.F90:57-170), evaluated per fine node
+//   k_prolong_fill  mg_prolong's coarse correction q - qold_mg and its
+//                   k/j/i linear fills onto the fine level
+//                   (src/mg_prolong.F90:57-170), one coarse cell per thread
 //   k_rs_sweep<D>   rs_sweep constant-coefficient smoothing sweeps
 //                   (src/rs_sweep.F90:59-178) with every row on chip,
 //                   k sweep also adds dq to q
@@ -114,62 +114,93 @@ struct ProlongCtx {
     int icrs, jcrs, kcrs;
 };
 
-// coarse correction q - qold_mg at every coarse node (mg_prolong.F90:57-75),
-// into the line-solve scratch F.cp (free during prolongation), level-local
-__device__ __forceinline__ void prolong_corr_body(const Level& Lc, const Fields& F, unsigned bid) {
-    const long t = (long)bid * blockDim.x + threadIdx.x;
-    if (t >= Lc.n) return;
-    const long l = Lc.off + t;
-#pragma unroll
-    for (int m = 0; m < 4; m++) F.cp[m][t] = F.q[m][l] - F.qold[m][l];
+// cells of the fill per direction: a coarsened direction has one cell per
+// pair of adjacent coarse nodes (a, a + 1), with the fine nodes 2a (on coarse
+// node a; interior for a >= 1) and 2a + 1 (between a and a + 1); a direction
+// that is not coarsened has one cell per interior fine node f = a + 1, on
+// coarse node a + 1.  Coarsened directions have odd fine node counts, so
+// 2a + 1 <= nf - 2 for every cell.
+__host__ __device__ inline int fill_cells(int nf, int nc, int crs) { return crs ? nc - 1 : nf - 2; }
+__host__ __device__ inline long fill_cells(const ProlongCtx& P) {
+    return (long)fill_cells(P.Lf.ni, P.Lc.ni, P.icrs) * fill_cells(P.Lf.nj, P.Lc.nj, P.jcrs) *
+           fill_cells(P.Lf.nk, P.Lc.nk, P.kcrs);
 }
 
-// dq at every fine interior node into F.dp (level-local); boundary dq is
-// zero after smoothing and never enters the interior sweeps.  The fill of
-// mg_prolong.F90:82-161 in k, then j, then i: a fine node midway between two
-// coarse nodes of a coarsened direction takes 0.5 * high + 0.5 * low of the
-// values filled so far, a coincident one the coarse node's value
+// The coarse correction q - qold_mg (mg_prolong.F90:57-75) filled linearly
+// onto the fine level's interior nodes into F.dp, level-local: k, then j,
+// then i (mg_prolong.F90:82-161).  One thread per cell: it loads q and
+// qold_mg at the cell's eight coarse corners once and forms each of its (up
+// to eight) fine nodes from them.  A fine node coincident with a coarse node
+// in a direction takes the low corner there (the reference's high and low
+// nodes are then the same node), one between two coarse nodes takes half of
+// each: the same operands in the same order as the reference's fills.
 __device__ __forceinline__ void prolong_fill_body(const ProlongCtx& P, const Fields& F, unsigned bid, unsigned nb) {
+    const Level& Lf = P.Lf;
+    const Level& Lc = P.Lc;
+    const int ca = fill_cells(Lf.ni, Lc.ni, P.icrs), cb = fill_cells(Lf.nj, Lc.nj, P.jcrs);
     const long t = (long)xcd_block(bid, nb) * blockDim.x + threadIdx.x;
-    const Level& L = P.Lf;
-    if (t >= L.n) return;
-    const int ni = L.ni, nj = L.nj, nk = L.nk;
-    const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
-    if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1 || k == 0 || k == nk - 1) return;
-    // without branches: the coarse nodes on the high and low side of (i, j, k)
-    // in every direction (the same node where the fine node is coincident in
-    // that direction), all eight loaded up front, then the weighted sums
-    // selected per direction
-    const bool oi = P.icrs && (i & 1), oj = P.jcrs && (j & 1), ok = P.kcrs && (k & 1);
-    const int ih = P.icrs ? (i + oi) / 2 : i, il = P.icrs ? (i - oi) / 2 : i;
-    const int jh = P.jcrs ? (j + oj) / 2 : j, jl = P.jcrs ? (j - oj) / 2 : j;
-    const int kh = P.kcrs ? (k + ok) / 2 : k, kl = P.kcrs ? (k - ok) / 2 : k;
-    long c[2][2][2];  // [i side][j side][k side], side 1 = high
+    if (t >= fill_cells(P)) return;
+    const int a = (int)(t % ca), b = (int)((t / ca) % cb), c = (int)(t / ((long)ca * cb));
+    // per direction: low and high coarse corner, fine index of node 0, and
+    // whether nodes 0 (coincident) and 1 (between) exist
+    const int cl[3] = {P.icrs ? a : a + 1, P.jcrs ? b : b + 1, P.kcrs ? c : c + 1};
+    const int crs[3] = {P.icrs, P.jcrs, P.kcrs};
+    int ch[3], f0[3];
+    bool has0[3], has1[3];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int d = 0; d < 3; d++) {
+        ch[d] = crs[d] ? cl[d] + 1 : cl[d];
+        f0[d] = crs[d] ? 2 * cl[d] : cl[d];
+        has0[d] = !crs[d] || cl[d] >= 1;
+        has1[d] = crs[d];
+    }
+    long corner[2][2][2];  // [i side][j side][k side], side 1 = high
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-            for (int e = 0; e < 2; e++) c[a][b][e] = lidx(P.Lc, a ? ih : il, b ? jh : jl, e ? kh : kl);
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int z = 0; z < 2; z++)
+                corner[x][y][z] = Lc.off + lidx(Lc, x ? ch[0] : cl[0], y ? ch[1] : cl[1], z ? ch[2] : cl[2]);
+    const long base = lidx(Lf, f0[0], f0[1], f0[2]);
+    const long sI = 1, sJ = Lf.ni, sK = (long)Lf.ni * Lf.nj;
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-        const double* cp = F.cp[m];
         double v[2][2][2];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int x = 0; x < 2; x++)
 #pragma unroll
-            for (int b = 0; b < 2; b++)
+            for (int y = 0; y < 2; y++)
 #pragma unroll
-                for (int e = 0; e < 2; e++) v[a][b][e] = cp[c[a][b][e]];
-        double fj[2];
+                for (int z = 0; z < 2; z++) {
+                    const long l = corner[x][y][z];
+                    v[x][y][z] = F.q[m][l] - F.qold[m][l];
+                }
+        // fk[x][y][pk]: the k fill of node parity pk on the (x, y) corner column
+        double fk[2][2][2];
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-            double fk[2];
+        for (int x = 0; x < 2; x++)
 #pragma unroll
-            for (int b = 0; b < 2; b++) fk[b] = ok ? 0.5 * v[a][b][1] + 0.5 * v[a][b][0] : v[a][b][1];
-            fj[a] = oj ? 0.5 * fk[1] + 0.5 * fk[0] : fk[1];
+            for (int y = 0; y < 2; y++) {
+                fk[x][y][0] = v[x][y][0];
+                fk[x][y][1] = 0.5 * v[x][y][1] + 0.5 * v[x][y][0];
+            }
+#pragma unroll
+        for (int pk = 0; pk < 2; pk++) {
+            if (!(pk ? has1[2] : has0[2])) continue;
+#pragma unroll
+            for (int pj = 0; pj < 2; pj++) {
+                if (!(pj ? has1[1] : has0[1])) continue;
+                double fj[2];
+#pragma unroll
+                for (int x = 0; x < 2; x++) fj[x] = pj ? 0.5 * fk[x][1][pk] + 0.5 * fk[x][0][pk] : fk[x][0][pk];
+#pragma unroll
+                for (int pi = 0; pi < 2; pi++) {
+                    if (!(pi ? has1[0] : has0[0])) continue;
+                    F.dp[m][base + pi * sI + pj * sJ + pk * sK] = pi ? 0.5 * fj[1] + 0.5 * fj[0] : fj[0];
+                }
+            }
         }
-        F.dp[m][t] = oi ? 0.5 * fj[1] + 0.5 * fj[0] : fj[1];
     }
 }
 
@@ -514,11 +545,6 @@ __global__ void __launch_bounds__(256) k_nlevel_z(ZoneGrid G, int n) {
     const ZoneDesc& Z = G.z[b.z];
     nlevel_body(Z.lv[n], Z.lv[n - 1], Z.F, Z.icrs, Z.jcrs, Z.kcrs, b.bid, b.nblk);
 }
-__global__ void __launch_bounds__(256) k_prolong_corr(Level Lc, Fields F) { prolong_corr_body(Lc, F, blockIdx.x); }
-__global__ void __launch_bounds__(256) k_prolong_corr_z(ZoneGrid G, int nc) {
-    const ZoneBlock b = zone_block(G);
-    prolong_corr_body(G.z[b.z].lv[nc], G.z[b.z].F, b.bid);
-}
 __global__ void __launch_bounds__(256) k_prolong_fill(ProlongCtx P, Fields F) {
     prolong_fill_body(P, F, blockIdx.x, gridDim.x);
 }
@@ -614,8 +640,7 @@ static void rs_sweep(hipStream_t s, const Level& L, const Fields& F, double ep) 
 void launch_prolong(hipStream_t s, const Level& Lf, const Level& Lc, const Fields& F, const Consts& C, int icrs,
                     int jcrs, int kcrs) {
     ProlongCtx P{Lf, Lc, icrs, jcrs, kcrs};
-    hipLaunchKernelGGL(k_prolong_corr, dim3(nblk(Lc.n, 256)), dim3(256), 0, s, Lc, F);
-    hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(Lf.n, 256)), dim3(256), 0, s, P, F);
+    hipLaunchKernelGGL(k_prolong_fill, dim3(nblk(fill_cells(P), 256)), dim3(256), 0, s, P, F);
     rs_sweep<0>(s, Lf, F, C.ep[1][0]);
     rs_sweep<1>(s, Lf, F, C.ep[1][1]);
     rs_sweep<2>(s, Lf, F, C.ep[1][2]);
@@ -723,10 +748,10 @@ static void rs_sweep_z(hipStream_t s, const ZoneList& Z, int n) {
 
 // prolongation from level n + 1 onto level n
 void launch_prolong_z(hipStream_t s, const ZoneList& Z, int n) {
-    unsigned nc = 0, nf = 0;
-    const ZoneGrid Gc = Z.grid([&](const ZoneDesc& z) { return nblk(z.lv[n + 1].n, 256); }, nc);
-    const ZoneGrid Gf = Z.grid([&](const ZoneDesc& z) { return nblk(z.lv[n].n, 256); }, nf);
-    hipLaunchKernelGGL(k_prolong_corr_z, dim3(nc), dim3(256), 0, s, Gc, n + 1);
+    unsigned nf = 0;
+    const ZoneGrid Gf = Z.grid(
+        [&](const ZoneDesc& z) { return nblk(fill_cells(ProlongCtx{z.lv[n], z.lv[n + 1], z.icrs, z.jcrs, z.kcrs}), 256); },
+        nf);
     hipLaunchKernelGGL(k_prolong_fill_z, dim3(nf), dim3(256), 0, s, Gf, n);
     rs_sweep_z<0>(s, Z, n);
     rs_sweep_z<1>(s, Z, n);
