@@ -757,7 +757,7 @@ __host__ __device__ inline SweepIO<float> sweep_io<float>(const Fields& F, int D
 template <int D, int R, typename S>
 __global__ void __launch_bounds__(512) k_daf_sweep(Level L, Fields F, Consts C, SweepIO<S> io, SweepShape sh,
                                                   int use_b1, int use_b2, int b2_full, int ext, double* dq_out) {
-    daf_body<D, R, S>(L, F, C, io, sh, use_b1, use_b2, b2_full, ext, dq_out, blockIdx.x);
+    daf_body<D, R, S>(L, F, C, io, sh, use_b1, use_b2, b2_full, ext, dq_out, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // every zone of a group in one launch; P and TI are the launch's (from the
@@ -770,7 +770,7 @@ __global__ void __launch_bounds__(512) k_daf_sweep_z(ZoneGrid G, int n, int P, i
     const Level& L = Z.lv[n];
     const SweepFlags f = sweep_flags(Z.C, D, n);
     daf_body<D, R, S>(L, Z.F, Z.C, sweep_io<S>(Z.F, D), sweep_lines(L, D, f.ext, P, TI), f.use_b1, f.use_b2,
-                      f.b2_full, f.ext, nullptr, b.bid);
+                      f.b2_full, f.ext, nullptr, xcd_block(b.bid, b.nblk));
 }
 
 // ---- batched LAPACK SGTSV (src/sgtsv.f:89-259) in FP64, one system per thread ----

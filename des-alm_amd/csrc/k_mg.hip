@@ -556,13 +556,13 @@ __global__ void __launch_bounds__(256) k_prolong_fill_z(ZoneGrid G, int n) {
 }
 template <int D, int R>
 __global__ void __launch_bounds__(512) k_rs_sweep(Level L, Fields F, SegShape sh, double epn) {
-    rs_sweep_body<D, R>(L, F, sh, epn, blockIdx.x);
+    rs_sweep_body<D, R>(L, F, sh, epn, xcd_block(blockIdx.x, gridDim.x));
 }
 template <int D, int R>
 __global__ void __launch_bounds__(512) k_rs_sweep_z(ZoneGrid G, int n, int P, int TI) {
     const ZoneBlock b = zone_block(G);
     const ZoneDesc& Z = G.z[b.z];
-    rs_sweep_body<D, R>(Z.lv[n], Z.F, seg_lines(Z.lv[n], D, P, TI), Z.C.ep[1][D], b.bid);
+    rs_sweep_body<D, R>(Z.lv[n], Z.F, seg_lines(Z.lv[n], D, P, TI), Z.C.ep[1][D], xcd_block(b.bid, b.nblk));
 }
 template <int D>
 __global__ void __launch_bounds__(256) k_bc_fm(Level L, Fields F, Consts C) {

@@ -615,13 +615,13 @@ __global__ void __launch_bounds__(256) k_sa_clamp_z(ZoneGrid G) {
 }
 template <int D, int R>
 __global__ void __launch_bounds__(512) k_sa_sweep(Level L, Fields F, Consts C, SegShape sh) {
-    sa_sweep_body<D, R>(L, F, C, sh, blockIdx.x);
+    sa_sweep_body<D, R>(L, F, C, sh, xcd_block(blockIdx.x, gridDim.x));
 }
 template <int D, int R>
 __global__ void __launch_bounds__(512) k_sa_sweep_z(ZoneGrid G, int P, int TI) {
     const ZoneBlock b = zone_block(G);
     const ZoneDesc& Z = G.z[b.z];
-    sa_sweep_body<D, R>(Z.lv[1], Z.F, Z.C, seg_lines(Z.lv[1], D, P, TI), b.bid);
+    sa_sweep_body<D, R>(Z.lv[1], Z.F, Z.C, seg_lines(Z.lv[1], D, P, TI), xcd_block(b.bid, b.nblk));
 }
 
 }  // namespace
