@@ -300,7 +300,7 @@ void bcond(desalm_ctx* c, hipStream_t s, bool pfix_deferred = false) {
 void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out, bool ucn_kept = false) {
     const Level& L = c->lv[n];
     hipStream_t s = c->stream;
-    timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C, ucn_kept); });
+    timed(c, KID_DTAU, n, [&] { launch_dtau(s, L, c->F, c->C, n, ucn_kept); });
     timed(c, KID_RESIDUAL, n, [&] { launch_residual(s, L, c->F, c->C, n, 0, calc_pk, c->kc_visc[n]); });
     // characteristic-outlet planes of the fine level (solver_daf.F90:322-365)
     if (n == 1 && (c->C.btype[0] == DESALM_BC_OUTLET_MOC || c->C.btype[1] == DESALM_BC_OUTLET_MOC))
@@ -310,11 +310,9 @@ void solver_daf(desalm_ctx* c, int n, int recalc, int calc_pk, double* dq_out, b
     timed(c, KID_DAF_I, n, [&] { launch_daf_i(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_J, n, [&] { launch_daf_j(s, L, c->F, c->C, n); });
     timed(c, KID_DAF_K, n, [&] { launch_daf_k(s, L, c->F, c->C, n, dq_out); });
-    // on the fine level the recalc's first pass applies the pressure shift
-    const bool defer = n == 1 && recalc == 1;
-    if (n == 1) timed(c, KID_BCOND, n, [&] { bcond(c, s, defer); });
+    if (n == 1) timed(c, KID_BCOND, n, [&] { bcond(c, s); });
     if (recalc == 1)
-        timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0, c->kc_visc[n], defer); });
+        timed(c, KID_RESIDUAL_RECALC, n, [&] { launch_residual(s, L, c->F, c->C, n, 1, 0, c->kc_visc[n]); });
 }
 
 int copy_in_level(desalm_ctx* c, double* const* soa, int ncomp, const double* host, const Level& L) {

@@ -248,14 +248,15 @@ struct ZoneList {
 
 // ---- launchers (each enqueues on `s`) ----
 // residual assembly, k_residual.hip
-// ucn_kept: U/J of the current velocities is already in F.ucn (stored by the
-// SA update of the same pseudo-iteration), so only dtau, p/J and fv are formed
-void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, bool ucn_kept);
-// kc: k planes per workgroup of the LDS-tiled kernels (0: default)
-// pfix (recalc only): first subtract the pinned pressure F.pfix from p at
-// every node, the last step of a bcond_fm launched with pfix_deferred
+// dtau and fv of level n; on a fine level with a characteristic outlet also
+// p/J and U/J for it, U/J unless ucn_kept (already in F.ucn, stored by the SA
+// update of the same pseudo-iteration)
+void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, bool ucn_kept);
+// kc: k planes per workgroup of the LDS-tiled kernels (0: default).  The
+// recalc branch reads p as it is: a bcond_fm before it applies the
+// pinned-pressure shift itself (pfix_deferred false).
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n,
-                     int recalc, int calc_pk, int kc, bool pfix = false);
+                     int recalc, int calc_pk, int kc);
 void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int calc_pk, int kc);
 int visc_kc_default(const Level& L);
 // DAF implicit operator, k_daf.hip
@@ -269,8 +270,8 @@ void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const 
 // fine level): mg_brhs + nonreflect_ibc (k_outlet.hip)
 void launch_outlet_rhs(hipStream_t s, const Level& L, const Fields& F, const Consts& C);
 // pfix_deferred: read the pinned pressure into F.pfix but leave its
-// subtraction to the next pass over the level (launch_residual or
-// launch_resid_and_save with pfix), which then reads p first
+// subtraction to the next pass over the level (launch_resid_and_save with
+// pfix), which then reads p first
 void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix,
                      int jfix, int kfix, bool pfix_deferred = false);
 // SA / DES, k_sa.hip

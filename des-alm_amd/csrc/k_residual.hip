@@ -2,7 +2,7 @@
 //
 // Three kernels produce the right-hand side of solver_daf steps 2-10
 // (src/solver_daf.F90:199-315):
-//   k_prep     dtau, U/J and p/J per node
+//   k_prep     dtau and the pressure dissipation's fv per node
 //   k_residual central pressure gradient and divergence, QUICK or first-order
 //              convection, 4th-difference pressure dissipation, BDF2 term,
 //              zeroing of the boundary planes
@@ -20,25 +20,24 @@ namespace desalm {
 
 namespace {
 
-struct Nb {
-    long sI, sJ, sK;
-};
-
-__device__ __forceinline__ double sqr(double a) { return a * a; }
-
-// Per-node inputs of the residual, every node of the level:
-// rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83), or met_daf_dtau
-// (src/met_daf_dtau.F90:53-93) when C.dtau_met, when mode & kPrepDtau, and
-// U/J (rhs_contra_j.F90:14-29) into F.ucn unless mode & kPrepUcnKept (the SA
-// update has just stored U/J of the same velocities) and always p/J
-// (rhs_flux_sans_convec.F90:60) into F.pj (level-local).  With dtau
-// also the fv of rhs_diss_p per direction (rhs_diss_p.F90:58-117), at the
-// nodes interior in that direction, into F.fv (level-local): the residual
-// then reads it at a node and its two neighbours.
-constexpr int kPrepDtau = 1, kPrepUcnKept = 2, kPrepPfix = 4;
+// Per-node inputs of the residual, every node of the level: dtau of
+// rhs_daf_dtau (src/rhs_daf_dtau.F90:51-83), or of met_daf_dtau
+// (src/met_daf_dtau.F90:53-93) when C.dtau_met, and the fv of rhs_diss_p per
+// direction (rhs_diss_p.F90:58-117) at the nodes interior in that direction
+// into F.fv (level-local): the residual reads it at a node and its two
+// neighbours.  For the characteristic outlet (k_outlet.hip), which reads
+// them, also U/J (rhs_contra_j.F90:14-29) into F.ucn (mode & kPrepUcn; not
+// where the SA update has just stored U/J of the same velocities) and p/J
+// (rhs_flux_sans_convec.F90:60) into F.pj (mode & kPrepPj), level-local; the
+// residual forms both where it uses them.
+constexpr int kPrepUcn = 1, kPrepPj = 2;
+__host__ __device__ inline int prep_mode(const Consts& C, int n, bool ucn_kept) {
+    const bool outlet = n == 1 && (C.btype[0] == DESALM_BC_OUTLET_MOC || C.btype[1] == DESALM_BC_OUTLET_MOC);
+    return outlet ? kPrepPj | (ucn_kept ? 0 : kPrepUcn) : 0;
+}
 __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const Consts& C, int mode, unsigned bid,
                                           unsigned nblk) {
-    const bool with_dtau = mode & kPrepDtau, with_ucn = !(mode & kPrepUcnKept);
+    const bool with_ucn = mode & kPrepUcn, with_pj = mode & kPrepPj;
     const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
     const long g = L.off + t;
@@ -60,29 +59,21 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
         w = F.q[3][g];
     }
     const double* p = F.q[0];
-    double p0 = p[g];
-    if (mode & kPrepPfix) {
-        // the pinned-pressure shift of bcond_fm, deferred to this pass
-        p0 = p0 - *F.pfix;
-        F.q[0][g] = p0;
-    }
-    double pm[3] = {0.0, 0.0, 0.0}, pp[3] = {0.0, 0.0, 0.0}, xnut = 0.0;
-    if (with_dtau) {
+    const double p0 = p[g];
+    double pm[3], pp[3];
 #pragma unroll
-        for (int D = 0; D < 3; D++) {
-            pm[D] = p[g - (pos[D] >= 1 ? st[D] : 0)];
-            pp[D] = p[g + (pos[D] <= nd[D] - 2 ? st[D] : 0)];
-        }
-        if (!C.dtau_met) xnut = F.xnut[g];
+    for (int D = 0; D < 3; D++) {
+        pm[D] = p[g - (pos[D] >= 1 ? st[D] : 0)];
+        pp[D] = p[g + (pos[D] <= nd[D] - 2 ? st[D] : 0)];
     }
+    const double xnut = C.dtau_met ? 0.0 : F.xnut[g];
     if (with_ucn) {
         const double tmp = 1.0 / aj;
         F.ucn[0][t] = tmp * (c1 * u + c2 * v + c3 * w);
         F.ucn[1][t] = tmp * (e1 * u + e2 * v + e3 * w);
         F.ucn[2][t] = tmp * (z1 * u + z2 * v + z3 * w);
     }
-    F.pj[t] = p0 / aj;
-    if (!with_dtau) return;
+    if (with_pj) F.pj[t] = p0 / aj;
     const double g1 = c1 * c1 + c2 * c2 + c3 * c3;
     const double g2 = e1 * e1 + e2 * e2 + e3 * e3;
     const double g3 = z1 * z1 + z2 * z2 + z3 * z3;
@@ -119,44 +110,36 @@ __device__ __forceinline__ void prep_body(const Level& L, const Fields& F, const
     }
 }
 
-// contravariant velocity / J, component D, at level-local node t (k_prep)
-template <int D>
-__device__ __forceinline__ double ucn(const Fields& F, long t) {
-    return F.ucn[D][t];
+// QUICK (rhs_convec_quick.F90:89-201 and its eta/zet copies) or first-order
+// upwind term of one velocity component along one direction, for rh(2:4):
+// ucon is half of U/J of that direction at the node, h the grid spacing,
+// um2 .. up2 the component at offsets -2 .. +2 along the line (where
+// first_order, um2 and up2 hold the -1 and +1 values and are unused).  Both
+// forms are evaluated and one is selected, so that every load is issued up
+// front.
+__device__ __forceinline__ double convec_term(double ucon, double h, bool first_order, double um2, double um1,
+                                              double u0, double up1, double up2) {
+    const double up = ucon + fabs(ucon), um = ucon - fabs(ucon);
+    const double fo = h * (up * (u0 - um1) + um * (up1 - u0));
+    const double qk = 0.125 * h *
+                      (up * (3.0 * up1 + 3.0 * u0 - 7.0 * um1 + um2) +
+                       um * (-3.0 * um1 - 3.0 * u0 + 7.0 * up1 - up2));
+    return first_order ? fo : qk;
 }
 
-// QUICK / first-order upwind contribution of direction D to rh(2:4)
-// (rhs_convec_quick.F90:89-201 and its eta/zet copies)
-template <int D>
-__device__ __forceinline__ void convec(const Level& L, const Fields& F, long g, long t, const Nb& nb, int pos,
-                                       int len, int quick, double out[3]) {
-    const long sD = D == 0 ? nb.sI : D == 1 ? nb.sJ : nb.sK;
-    const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
-    const double ucon = 0.5 * ucn<D>(F, t);
-    const double up = ucon + fabs(ucon), um = ucon - fabs(ucon);
-    // first interior node is pos 1, last is len - 2: first-order there.
-    // Both forms are evaluated and one is selected, so that every load is
-    // issued up front; where the first-order form applies the +-2 loads read
-    // the +-1 neighbours (the +-2 ones may lie outside the level)
-    const bool first_order = !quick || pos == 1 || pos == len - 2;
-    const long s2 = first_order ? sD : 2 * sD;
-#pragma unroll
-    for (int m = 0; m < 3; m++) {
-        const double* u = F.q[m + 1];
-        const double um2 = u[g - s2], um1 = u[g - sD], u0 = u[g], up1 = u[g + sD], up2 = u[g + s2];
-        const double fo = h * (up * (u0 - um1) + um * (up1 - u0));
-        const double qk = 0.125 * h *
-                          (up * (3.0 * up1 + 3.0 * u0 - 7.0 * um1 + um2) +
-                           um * (-3.0 * um1 - 3.0 * u0 + 7.0 * up1 - up2));
-        out[m] = first_order ? fo : qk;
-    }
+// U/J of the direction whose metric row is m (rhs_contra_j.F90:14-29)
+__device__ __forceinline__ double contra_j(double aj, const double (&m)[3], const double (&u)[3]) {
+    const double tmp = 1.0 / aj;
+    return tmp * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]);
 }
 
 // Residual of solver_daf on one level without its viscous term and FAS
 // forcing (k_visc_pk adds those).  recalc = 0: first evaluation of the pass
 // (diss(1) fresh and stored); recalc = 1: the decide_recalc_rh branch
 // (src/solver_daf.F90:432-484), which reuses the stored diss(1) of the
-// pre-update q and dtau.
+// pre-update q and dtau.  p/J (rhs_flux_sans_convec.F90:60) and U/J are
+// formed at the node and its six neighbours from p, J, the metrics and the
+// velocities, which the convection and the pressure gradient gather anyway.
 __device__ __forceinline__ void residual_body(const Level& L, const Fields& F, const Consts& C, int n, int recalc,
                                               unsigned bid, unsigned nblk) {
     const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
@@ -173,77 +156,132 @@ __device__ __forceinline__ void residual_body(const Level& L, const Fields& F, c
         for (int m = 0; m < 4; m++) F.rh[m][g] = 0.0;
         return;
     }
-    const Nb nb{1, (long)ni, (long)ni * nj};
+    const long sD[3] = {1, (long)ni, (long)ni * nj};
+    const int pos[3] = {i, j, k}, len[3] = {ni, nj, nk};
     const double dc2 = 0.5 * L.dc, de2 = 0.5 * L.de, dz2 = 0.5 * L.dz;
+    const double* const* met[3] = {F.csi, F.eta, F.zet};
 
+    // every load first.  Neighbour q = 2 D + s (s = 0: -1, s = 1: +1 along
+    // D): p, J, the metric row of D and the velocities
+    double pn[6], ajn[6], mn[6][3], un[6][3];
+#pragma unroll
+    for (int D = 0; D < 3; D++)
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int q = 2 * D + s;
+            const long gn = g + (s ? sD[D] : -sD[D]);
+            pn[q] = F.q[0][gn];
+            ajn[q] = F.aj[gn];
+#pragma unroll
+            for (int c = 0; c < 3; c++) mn[q][c] = met[D][c][gn];
+#pragma unroll
+            for (int m = 0; m < 3; m++) un[q][m] = F.q[m + 1][gn];
+        }
+    // the node
+    const double aj0 = F.aj[g];
+    double m0[3][3], u0[3];
+#pragma unroll
+    for (int D = 0; D < 3; D++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) m0[D][c] = met[D][c][g];
+#pragma unroll
+    for (int m = 0; m < 3; m++) u0[m] = F.q[m + 1][g];
+    // the velocities two nodes away along each line for QUICK: on the fine
+    // level but at the first and last interior node (where the +-2 loads
+    // read the +-1 neighbours; the +-2 ones may lie outside the level)
+    const bool quick = n == 1;
+    bool fo[3];
+    double u2[3][2][3];
+#pragma unroll
+    for (int D = 0; D < 3; D++) {
+        fo[D] = !quick || pos[D] == 1 || pos[D] == len[D] - 2;
+        const long s2 = fo[D] ? sD[D] : 2 * sD[D];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            u2[D][0][m] = F.q[m + 1][g - s2];
+            u2[D][1][m] = F.q[m + 1][g + s2];
+        }
+    }
+    double fvn[9], d1 = 0.0;
+    if (!recalc) {
+        // the boundary copies of fv (rhs_diss_p.F90:124-129) replace the
+        // out-of-domain neighbour by the node itself
+#pragma unroll
+        for (int D = 0; D < 3; D++) {
+            const double* fv = F.fv[D];
+            fvn[3 * D] = fv[t - (pos[D] == 1 ? 0 : sD[D])];
+            fvn[3 * D + 1] = fv[t];
+            fvn[3 * D + 2] = fv[t + (pos[D] == len[D] - 2 ? 0 : sD[D])];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) fvn[c] = 0.0;
+        d1 = F.diss1[g];
+    }
+    double qnv[3], qmv[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        qnv[m] = F.qn[m + 1][g];
+        qmv[m] = F.qnm1[m + 1][g];
+    }
+
+    // p/J and U/J of the neighbours
+    double pjn[6], ucn[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        pjn[q] = pn[q] / ajn[q];
+        ucn[q] = contra_j(ajn[q], mn[q], un[q]);
+    }
     // central pressure gradient (rhs_flux_sans_convec.F90:75-116)
     double r[4];
-    {
-        const double* pj = F.pj;
-        const double pim = pj[t - nb.sI], pip = pj[t + nb.sI];
-        const double pjm = pj[t - nb.sJ], pjp = pj[t + nb.sJ];
-        const double pkm = pj[t - nb.sK], pkp = pj[t + nb.sK];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double v = dc2 * (pip * F.csi[c][g + nb.sI] - pim * F.csi[c][g - nb.sI]);
-            v = v + de2 * (pjp * F.eta[c][g + nb.sJ] - pjm * F.eta[c][g - nb.sJ]);
-            v = v + dz2 * (pkp * F.zet[c][g + nb.sK] - pkm * F.zet[c][g - nb.sK]);
-            r[c + 1] = v;
-        }
-        // divergence of U/J (:120-135)
-        const double dudx = dc2 * (ucn<0>(F, t + nb.sI) - ucn<0>(F, t - nb.sI));
-        const double dvdy = de2 * (ucn<1>(F, t + nb.sJ) - ucn<1>(F, t - nb.sJ));
-        const double dwdz = dz2 * (ucn<2>(F, t + nb.sK) - ucn<2>(F, t - nb.sK));
+    for (int c = 0; c < 3; c++) {
+        double v = dc2 * (pjn[1] * mn[1][c] - pjn[0] * mn[0][c]);
+        v = v + de2 * (pjn[3] * mn[3][c] - pjn[2] * mn[2][c]);
+        v = v + dz2 * (pjn[5] * mn[5][c] - pjn[4] * mn[4][c]);
+        r[c + 1] = v;
+    }
+    // divergence of U/J (:120-135)
+    {
+        const double dudx = dc2 * (ucn[1] - ucn[0]);
+        const double dvdy = de2 * (ucn[3] - ucn[2]);
+        const double dwdz = dz2 * (ucn[5] - ucn[4]);
         r[0] = dudx + dvdy + dwdz;
     }
     // convection: QUICK on the fine level, first-order upwind on coarse levels
-    {
-        const int quick = (n == 1);
-        double fv[3];
-        convec<0>(L, F, g, t, nb, i, ni, quick, fv);
-        r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
-        convec<1>(L, F, g, t, nb, j, nj, quick, fv);
-        r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
-        convec<2>(L, F, g, t, nb, k, nk, quick, fv);
-        r[1] = r[1] + fv[0]; r[2] = r[2] + fv[1]; r[3] = r[3] + fv[2];
+#pragma unroll
+    for (int D = 0; D < 3; D++) {
+        const double h = D == 0 ? L.dc : D == 1 ? L.de : L.dz;
+        const double ucon = 0.5 * contra_j(aj0, m0[D], u0);
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            r[m + 1] = r[m + 1] + convec_term(ucon, h, fo[D], u2[D][0][m], un[2 * D][m], u0[m], un[2 * D + 1][m],
+                                              u2[D][1][m]);
     }
     // pressure dissipation (rhs_diss_p.F90:58-203); the boundary copies of
     // fv (:124-129) replace the out-of-domain neighbour by the node itself
     double diss1;
     if (!recalc) {
-        const double* fv0 = F.fv[0];
-        const double* fv1 = F.fv[1];
-        const double* fv2 = F.fv[2];
-        // (computed offsets rather than branches, so the loads issue together)
-        const double f0i = fv0[t];
-        const double fmi = fv0[t - (i == 1 ? 0 : nb.sI)];
-        const double fpi = fv0[t + (i == ni - 2 ? 0 : nb.sI)];
-        const double f0j = fv1[t];
-        const double fmj = fv1[t - (j == 1 ? 0 : nb.sJ)];
-        const double fpj = fv1[t + (j == nj - 2 ? 0 : nb.sJ)];
-        const double f0k = fv2[t];
-        const double fmk = fv2[t - (k == 1 ? 0 : nb.sK)];
-        const double fpk = fv2[t + (k == nk - 2 ? 0 : nb.sK)];
-        diss1 = C.pdiss * ((fpi - 2.0 * f0i + fmi) + (fpj - 2.0 * f0j + fmj) + (fpk - 2.0 * f0k + fmk));
+        diss1 = C.pdiss * ((fvn[2] - 2.0 * fvn[1] + fvn[0]) + (fvn[5] - 2.0 * fvn[4] + fvn[3]) +
+                           (fvn[8] - 2.0 * fvn[7] + fvn[6]));
         F.diss1[g] = diss1;
     } else {
-        diss1 = F.diss1[g];
+        diss1 = d1;
     }
     // unsteady and dissipation terms (rhs_unst_visc_diss.F90:29-48);
     // diss(2:4) is zero (rhs_convec_quick.F90:433); then the body force
     // (no counterpart in the reference), -f / J
     {
-        const double aj = F.aj[g];
-        const double dtaj = 2.0 * C.delti * aj;
+        const double dtaj = 2.0 * C.delti * aj0;
         r[0] = r[0] + diss1;
 #pragma unroll
         for (int m = 1; m < 4; m++) {
-            const double dqdt = C.e_source * (3.0 * F.q[m][g] - 4.0 * F.qn[m][g] + F.qnm1[m][g]) / dtaj;
+            const double dqdt = C.e_source * (3.0 * u0[m - 1] - 4.0 * qnv[m - 1] + qmv[m - 1]) / dtaj;
             r[m] = r[m] + 0.0 + dqdt;
         }
         if (C.fb[0] != 0.0 || C.fb[1] != 0.0 || C.fb[2] != 0.0)
 #pragma unroll
-            for (int m = 1; m < 4; m++) r[m] = r[m] - C.fb[m - 1] / aj;
+            for (int m = 1; m < 4; m++) r[m] = r[m] - C.fb[m - 1] / aj0;
     }
 #pragma unroll
     for (int m = 0; m < 4; m++) F.rh[m][g] = r[m];
@@ -500,10 +538,10 @@ __device__ __forceinline__ void visc_pk_body(const Level& L, const Fields& F, co
 __global__ void __launch_bounds__(256) k_prep(Level L, Fields F, Consts C, int mode) {
     prep_body(L, F, C, mode, blockIdx.x, gridDim.x);
 }
-__global__ void __launch_bounds__(256) k_prep_z(ZoneGrid G, int n, int mode) {
+__global__ void __launch_bounds__(256) k_prep_z(ZoneGrid G, int n, int ucn_kept) {
     const ZoneBlock b = zone_block(G);
     const ZoneDesc& Z = G.z[b.z];
-    prep_body(Z.lv[n], Z.F, Z.C, mode, b.bid, b.nblk);
+    prep_body(Z.lv[n], Z.F, Z.C, prep_mode(Z.C, n, ucn_kept), b.bid, b.nblk);
 }
 __global__ void __launch_bounds__(256) k_residual(Level L, Fields F, Consts C, int n, int recalc) {
     residual_body(L, F, C, n, recalc, blockIdx.x, gridDim.x);
@@ -525,10 +563,10 @@ __global__ void __launch_bounds__(kTX * kTY, 2) k_visc_pk_z(ZoneGrid G, int n, i
 
 }  // namespace
 
-void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, bool ucn_kept) {
+void launch_dtau(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, bool ucn_kept) {
     const int b = 256;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)((L.n + b - 1) / b)), dim3(b), 0, s, L, F, C,
-                       kPrepDtau | (ucn_kept ? kPrepUcnKept : 0));
+                       prep_mode(C, n, ucn_kept));
 }
 
 static unsigned node_blocks(const Level& L) { return (unsigned)((L.n + 255) / 256); }
@@ -536,13 +574,12 @@ static unsigned node_blocks(const Level& L) { return (unsigned)((L.n + 255) / 25
 void launch_dtau_z(hipStream_t s, const ZoneList& Z, int n, bool ucn_kept) {
     unsigned nb = 0;
     const ZoneGrid G = Z.grid([&](const ZoneDesc& z) { return node_blocks(z.lv[n]); }, nb);
-    hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, kPrepDtau | (ucn_kept ? kPrepUcnKept : 0));
+    hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, (int)ucn_kept);
 }
 
 void launch_residual_z(hipStream_t s, const ZoneList& Z, int n, int recalc, int calc_pk, int kc) {
     unsigned nb = 0;
     const ZoneGrid G = Z.grid([&](const ZoneDesc& z) { return node_blocks(z.lv[n]); }, nb);
-    if (recalc) hipLaunchKernelGGL(k_prep_z, dim3(nb), dim3(256), 0, s, G, n, 0);
     hipLaunchKernelGGL(k_residual_z, dim3(nb), dim3(256), 0, s, G, n, recalc);
     // one plane count for every zone: the smallest zone's default
     if (kc <= 0) {
@@ -571,11 +608,9 @@ void launch_visc_pk(hipStream_t s, const Level& L, const Fields& F, const Consts
 }
 
 void launch_residual(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int n, int recalc,
-                     int calc_pk, int kc, bool pfix) {
+                     int calc_pk, int kc) {
     const int b = 256;
     const dim3 grid((unsigned)((L.n + b - 1) / b));
-    // the recalc branch needs U/J and p/J of the updated q (dtau is kept)
-    if (recalc) hipLaunchKernelGGL(k_prep, grid, dim3(b), 0, s, L, F, C, pfix ? kPrepPfix : 0);
     hipLaunchKernelGGL(k_residual, grid, dim3(b), 0, s, L, F, C, n, recalc);
     launch_visc_pk(s, L, F, C, n, recalc ? 0 : calc_pk, kc);
 }
