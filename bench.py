@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 WORDS = {
     "dtau": 9 + 1 + 1,                        # metrics, xnut -> dtau
     "residual": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4 + 1,   # q,metrics,aj,xnut,qn,qnm1,dtau -> rh,diss1 (fine)
-    "residual_recalc": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4 + 1,  # ... diss1 -> rh, p (pinned-pressure shift)
+    "residual_recalc": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4,  # q,metrics,aj,xnut,qn,qnm1,diss1 -> rh
     "daf_i": 4 + 1 + 1 + 3 + 4,               # rh,dtau,aj,csi -> rh
     "daf_j": 4 + 1 + 3 + 3 + 4,               # rh,dtau,csi,eta -> rh
     "daf_k": 4 + 1 + 3 + 3 + 4 + 4,           # rh,dtau,eta,zet,q -> q

@@ -725,9 +725,11 @@ __host__ __device__ inline SweepShape sweep_lines(const Level& L, int D, int ext
 __host__ __device__ inline void sweep_tiles(int D, int len, int R, int& P, int& TI) {
     P = (len + R - 1) / R;
     if (P < 4) P = 4;
-    // j and k lines: 32 adjacent i lines per workgroup (256-byte rows, so
-    // that a wave's row loads span two segments instead of four)
-    TI = D == 0 ? (256 / P > 1 ? 256 / P : 1) : 32;
+    // j and k lines: 16 adjacent i lines per workgroup (128-byte rows; the
+    // partial cache lines at a run's ends are shared with the neighbouring
+    // workgroups, which run on the same XCD).  A 256-thread workgroup leaves
+    // room for a second one on the CU at up to 256 VGPRs.
+    TI = D == 0 ? (256 / P > 1 ? 256 / P : 1) : 16;
     while (TI * P > 512 && TI > 1) TI /= 2;
 }
 

@@ -42,7 +42,7 @@ inline size_t seg_lds_bytes(const SegShape& sh, int ns) {
     return sizeof(double) * (size_t)sh.TI * seg_line_stride(sh.P, ns);
 }
 
-// Segments of R rows; j and k lines (seg_fast = false) put 32 adjacent lines
+// Segments of R rows; j and k lines (seg_fast = false) put 16 adjacent lines
 // (adjacent i) on adjacent lanes, i lines (seg_fast = true) put the segments
 // of one line on adjacent lanes.
 inline SegShape seg_shape(int n, long nlines, int R, bool seg_fast) {
@@ -51,8 +51,10 @@ inline SegShape seg_shape(int n, long nlines, int R, bool seg_fast) {
     sh.nlines = nlines;
     sh.P = (n + R - 1) / R;
     if (sh.P < 4) sh.P = 4;
-    // j and k lines: 32 adjacent i lines per workgroup (256-byte rows)
-    sh.TI = seg_fast ? (256 / sh.P > 1 ? 256 / sh.P : 1) : 32;
+    // j and k lines: 16 adjacent i lines per workgroup (128-byte rows; the
+    // launches order workgroups XCD-contiguously, so neighbouring runs share
+    // their edge cache lines in one L2)
+    sh.TI = seg_fast ? (256 / sh.P > 1 ? 256 / sh.P : 1) : 16;
     while (sh.TI * sh.P > 512 && sh.TI > 1) sh.TI /= 2;
     return sh;
 }
