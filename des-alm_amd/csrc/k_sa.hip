@@ -649,9 +649,11 @@ static void sa_sweep(hipStream_t s, const Level& L, const Fields& F, const Const
     const int n = D == 0 ? L.ni - 2 : D == 1 ? L.nj - 2 : L.nk - 2;
     const long nl = D == 0 ? (long)(L.nj - 2) * (L.nk - 2) : D == 1 ? (long)(L.ni - 2) * (L.nk - 2)
                                                                     : (long)(L.ni - 2) * (L.nj - 2);
-    // i lines: short segments (their rows are adjacent in memory, so lanes
-    // reading row r of adjacent segments are R rows apart)
-    const int R = (D == 0 || n <= 64) ? 4 : 8;
+    // eight rows per segment, four on lines of at most 64 rows (the i lines
+    // load their rows as runs, so their segment length sets no access
+    // stride; the DAF and rs i sweeps, with more values per row, run faster
+    // at four)
+    const int R = n <= 64 ? 4 : 8;
     const SegShape sh = seg_shape(n, nl, R, D == 0);
     const unsigned nb = (unsigned)((sh.nlines + sh.TI - 1) / sh.TI);
     const size_t lds = seg_lds_bytes(sh, 1);
@@ -686,7 +688,7 @@ template <int D>
 static void sa_sweep_z(hipStream_t s, const ZoneList& Z) {
     int len = 0;
     for (const ZoneDesc& z : Z.host) len = std::max(len, seg_lines(z.lv[1], D, 1, 1).n);
-    const int R = (D == 0 || len <= 64) ? 4 : 8;
+    const int R = len <= 64 ? 4 : 8;
     const SegShape sh = seg_shape(len, 1, R, D == 0);
     unsigned nb = 0;
     const ZoneGrid G = Z.grid(
