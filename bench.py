@@ -68,7 +68,7 @@ FAMILY_OF = [("k_sa_", "des_eddy"), ("k_daf_sweep<0", "daf_i"), ("k_daf_sweep<1"
              ("k_resid_", "resid_norm"), ("k_bc_fm", "bcond_fm"), ("k_pfix_", "bcond_fm")]
 # one dispatch per family call
 FAMILY_MARK = {"des_eddy": "k_sa_prep", "daf_i": "k_daf_sweep<0", "daf_j": "k_daf_sweep<1", "daf_k": "k_daf_sweep<2",
-               "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_pfix_read",
+               "prolong": "k_prolong_fill", "resid_norm": "k_resid_part", "bcond_fm": "k_bc_fm<2>",
                "dtau": "k_prep", "residual": "k_residual", "residual_recalc": "k_residual"}
 
 # warm-start samples per recorded training step and the sample buffer: every

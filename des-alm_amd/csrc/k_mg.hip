@@ -439,7 +439,6 @@ __device__ __forceinline__ void bc_fm_body(const Level& L, const Fields& F, cons
     }
 }
 
-__global__ void k_pfix_read(Fields F, long g) { *F.pfix = F.q[0][g]; }
 
 __device__ __forceinline__ void pfix_sub_body(const Level& L, const Fields& F, unsigned bid) {
     const long t = (long)bid * blockDim.x + threadIdx.x;
@@ -564,9 +563,14 @@ __global__ void __launch_bounds__(512) k_rs_sweep_z(ZoneGrid G, int n, int P, in
     const ZoneDesc& Z = G.z[b.z];
     rs_sweep_body<D, R>(Z.lv[n], Z.F, seg_lines(Z.lv[n], D, P, TI), Z.C.ep[1][D], xcd_block(b.bid, b.nblk));
 }
+// D = 2 also reads the pinned pressure into F.pfix (bcond_fm.F90:310-322),
+// in the thread of the pinned node's k column (face node tfix) after that
+// thread's own face writes: no other thread of the launch writes the
+// column, and the i and j faces are complete
 template <int D>
-__global__ void __launch_bounds__(256) k_bc_fm(Level L, Fields F, Consts C) {
+__global__ void __launch_bounds__(256) k_bc_fm(Level L, Fields F, Consts C, long tfix, long gfix) {
     bc_fm_body<D>(L, F, C, blockIdx.x);
+    if (D == 2 && (long)blockIdx.x * blockDim.x + threadIdx.x == tfix) *F.pfix = F.q[0][gfix];
 }
 template <int D>
 __global__ void __launch_bounds__(256) k_bc_fm_z(ZoneGrid G) {
@@ -656,11 +660,11 @@ void launch_outlet_ratios(hipStream_t s, const Level& L, const Fields& F, const 
 
 void launch_bcond_fm(hipStream_t s, const Level& L, const Fields& F, const Consts& C, int ifix, int jfix, int kfix,
                      bool pfix_deferred) {
-    hipLaunchKernelGGL(k_bc_fm<0>, dim3(nblk((long)L.nj * L.nk, 256)), dim3(256), 0, s, L, F, C);
-    hipLaunchKernelGGL(k_bc_fm<1>, dim3(nblk((long)L.ni * L.nk, 256)), dim3(256), 0, s, L, F, C);
-    hipLaunchKernelGGL(k_bc_fm<2>, dim3(nblk((long)L.ni * L.nj, 256)), dim3(256), 0, s, L, F, C);
-    const long g = L.off + (ifix - 1) + (long)L.ni * ((jfix - 1) + (long)L.nj * (kfix - 1));
-    hipLaunchKernelGGL(k_pfix_read, dim3(1), dim3(1), 0, s, F, g);
+    const long tfix = (ifix - 1) + (long)L.ni * (jfix - 1);
+    const long g = L.off + tfix + (long)L.ni * L.nj * (kfix - 1);
+    hipLaunchKernelGGL(k_bc_fm<0>, dim3(nblk((long)L.nj * L.nk, 256)), dim3(256), 0, s, L, F, C, -1L, g);
+    hipLaunchKernelGGL(k_bc_fm<1>, dim3(nblk((long)L.ni * L.nk, 256)), dim3(256), 0, s, L, F, C, -1L, g);
+    hipLaunchKernelGGL(k_bc_fm<2>, dim3(nblk((long)L.ni * L.nj, 256)), dim3(256), 0, s, L, F, C, tfix, g);
     if (!pfix_deferred) hipLaunchKernelGGL(k_pfix_sub, dim3(nblk(L.n, 256)), dim3(256), 0, s, L, F);
 }
 
