@@ -137,9 +137,10 @@ __device__ __forceinline__ double contra_j(double aj, const double (&m)[3], cons
 // forcing (k_visc_pk adds those).  recalc = 0: first evaluation of the pass
 // (diss(1) fresh and stored); recalc = 1: the decide_recalc_rh branch
 // (src/solver_daf.F90:432-484), which reuses the stored diss(1) of the
-// pre-update q and dtau.  p/J (rhs_flux_sans_convec.F90:60) and U/J are
-// formed at the node and its six neighbours from p, J, the metrics and the
-// velocities, which the convection and the pressure gradient gather anyway.
+// pre-update q and dtau.  p/J (rhs_flux_sans_convec.F90:60) at the six
+// neighbours and U/J (rhs_contra_j.F90:14-29) at the node and its six
+// neighbours are formed from p, J, the metrics and the velocities, which the
+// convection and the pressure gradient gather anyway.
 __device__ __forceinline__ void residual_body(const Level& L, const Fields& F, const Consts& C, int n, int recalc,
                                               unsigned bid, unsigned nblk) {
     const long t = (long)xcd_block(bid, nblk) * blockDim.x + threadIdx.x;
