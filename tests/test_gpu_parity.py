@@ -313,6 +313,39 @@ def test_mg_driver_alm(gpu, des):
     assert rel(xg, xo) < TOL
 
 
+def edge_polars():
+    """Polars on which the lookup ends where its branches differ, whatever the
+    elements' angles of attack (|aoa| < 180 degrees): at index 256 (after a
+    whole step of 256 entries, so b - 1 lies in the step before), at 64 (b - 1
+    on the last lane of the first 64 entries), at 0 (every angle above aoa)
+    and past the end (every angle below it)."""
+    out = []
+    for n, first_hi in ((300, 256), (200, 64), (5, 0), (7, None)):
+        k = np.arange(n, dtype=float)
+        a = -2000.0 + k if first_hi is None else np.where(k < first_hi, -1000.0 + k, 1000.0 + k)
+        out.append((a, 0.3 + 0.001 * k, 0.01 + 0.0001 * k))
+    return out
+
+
+def test_mg_driver_alm_polar_edges(gpu):
+    from desalm import alm
+    c = cases.wavy(25, 17, 21, ng=2, amp=0.03, seed=13, iters=(1, 2, 3), itm=2)
+    c.alm = alm.rotor(c, i_rot=11, center=(0.45, 0.55), radius=0.28, hub=0.05, nblades=2, nelems=6,
+                      nangles=37, angle=1.1, tsr=5.0, n_polar=4)
+    c.alm.polars = edge_polars()
+    T = c.alm.turbines[0]
+    assert set(np.unique(np.concatenate([T.polar_before, T.polar_after]))) == {0, 1, 2, 3}
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    for _ in range(2):
+        assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL
+
+
 def test_tile_shapes_do_not_change_results(gpu):
     # the LDS-tiled kernels' k planes per workgroup (set by hand or by the
     # autotuner) only change the launch shape; where a workgroup's first k
