@@ -152,6 +152,24 @@ def test_mg_driver(gpu, ng, des, bt, itm):
     assert s.pivot_events == 0
 
 
+@pytest.mark.parametrize("crs", [(1, 0, 1), (0, 1, 1), (1, 1, 0)])
+def test_mg_driver_semicoarsening(gpu, crs):
+    # one direction kept at full resolution on every level (the reference's
+    # own case coarsens i and k only): restriction, injection and the
+    # prolongation fill take the direction as it is
+    dims = [17 if c else 12 for c in crs]
+    c = cases.wavy(*dims, ng=3, itm=3, des=True, iters=(1, 2, 3), crs=crs)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    for _ in range(2):
+        assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL
+
+
 @pytest.mark.parametrize("dims", LONG_I)
 def test_mg_driver_long_i(gpu, dims):
     # the prolongation's i-direction smoothing runs on the long lines too
