@@ -170,6 +170,25 @@ def test_mg_driver_semicoarsening(gpu, crs):
     assert rel(xg, xo) < TOL
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_mg_driver_lines_longer_than_64(gpu, mixed):
+    # lines of more than 64 rows in every direction, as in the benchmark
+    # workloads: the line sweeps take eight rows per segment there (the DAF
+    # and rs i sweeps four), FP64 and FP32 DAF sweeps alike
+    kw = {"precision": PREC_MIXED} if mixed else {}
+    c = cases.wavy(81, 73, 69, ng=3, itm=2, des=True, iters=(1, 2, 3), **kw)
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    tol = TOL_MIXED if mixed else TOL
+    errs = [rel(qg[:, m], qo[:, m]) for m in range(5)]
+    assert max(errs) < tol, errs
+    assert rel(xg, xo) < tol
+    assert s.pivot_events == 0
+
+
 @pytest.mark.parametrize("dims", LONG_I)
 def test_mg_driver_long_i(gpu, dims):
     # the prolongation's i-direction smoothing runs on the long lines too
