@@ -350,6 +350,24 @@ def test_mg_driver_alm(gpu, des):
     assert rel(xg, xo) < TOL
 
 
+def test_mg_driver_alm_many_elements(gpu):
+    # 3 blades x 24 elements: the spreading walks the elements in two chunks
+    from desalm import alm
+    c = cases.wavy(25, 17, 21, ng=2, amp=0.03, seed=13, iters=(1, 2, 3), itm=2)
+    c.alm = alm.rotor(c, i_rot=11, center=(0.45, 0.55), radius=0.28, hub=0.05, nblades=3, nelems=24,
+                      nangles=37, angle=0.4, tsr=5.0)
+    assert 3 * 24 > 64
+    o = po.Oracle(c.params).load_case(c)
+    s = Solver(c.params).load_case(c)
+    for _ in range(2):
+        assert o.mg_driver()[0] == s.mg_driver()[0]
+    qo, _, _, xo = o.get_state()
+    qg, _, _, xg = s.get_state()
+    for m in range(5):
+        assert rel(qg[:, m], qo[:, m]) < TOL, m
+    assert rel(xg, xo) < TOL
+
+
 def edge_polars():
     """Polars on which the lookup ends where its branches differ, whatever the
     elements' angles of attack (|aoa| < 180 degrees): at index 256 (after a
