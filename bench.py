@@ -33,8 +33,9 @@ for p in (os.path.join(ROOT, "des-alm_amd", "python"), os.path.join(ROOT, "oracl
 METRIC = "Mcell-iters/s + wall s/physical step to 1e-8 resid, FP64 plain vs mixed+learned"
 HBM_PEAK_GBS = 8000.0
 
-# compulsory FP64 words per node of the level, per launch (reads + writes),
-# after the per-kernel byte table of the hot-path survey
+# compulsory FP64 words per node of the level, per launch (reads + writes):
+# each live input array read once and each output written once, the
+# library's scratch arrays excluded
 WORDS = {
     "dtau": 9 + 1 + 1,                        # metrics, xnut -> dtau
     "residual": 4 + 9 + 1 + 1 + 3 + 3 + 1 + 4 + 1,   # q,metrics,aj,xnut,qn,qnm1,dtau -> rh,diss1 (fine)
